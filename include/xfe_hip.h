@@ -1,0 +1,151 @@
+/* xfe_hip.h -- C ABI of the MI355X (gfx950) fixed-effects absorber.
+ *
+ * Plain C: POD structs, pointers and sizes, int status codes, no exceptions,
+ * no Eigen/torch types.  Each entry point names the reference interface it
+ * replaces (xhdfe-xfe, file:line).  A C++ caller that used
+ * hdfe::detail::absorb_fixed_effects_cuda binds these through the thin adapter
+ * shown in INTEGRATION.md.
+ *
+ * Threading: one host thread per call; the library keeps one device
+ * workspace per calling thread (thread_local), like the reference's
+ * persistent CUDA workspace (src/fe_absorption_cuda.cu:1536).
+ */
+#ifndef XFE_HIP_H
+#define XFE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XFE_MAX_DIMS 8
+#define XFE_MAX_COLS 256
+
+/* Per-dimension view, mirrors hdfe::detail::GpuFeInput
+ * (src/fe_absorption_cuda.hpp:12-21).  group_ids are compact 0-based ids
+ * (length n); weight_sums has num_groups entries (sum of weights, or counts). */
+typedef struct {
+    const int* group_ids;
+    int num_groups;
+    int num_levels_present;
+    const double* weight_sums;
+    int is_slope;              /* heterogeneous slope: rejected (returns 0) */
+    int slope_has_intercept;
+    const double* slope_values;
+    const double* slope_sum_z;
+    const double* slope_sum_zz;
+} xfe_fe_input;
+
+/* Options: the subset of hdfe::HdfeOptions (include/hdfe/hdfe_regressor.hpp:44-97)
+ * that absorb_fixed_effects_cuda reads, plus the AbsorptionMethod argument. */
+typedef struct {
+    double tol;                 /* HdfeOptions::tol (default 1e-8) */
+    int tolerance_mode;         /* ToleranceMode: 0 xhdfe-fast, 1 reghdfe-comparable, 2 strict-residual */
+    int max_iter;               /* HdfeOptions::max_iter */
+    int method;                 /* AbsorptionMethod: 0 Auto, 1 GaussSeidel, 2 SymmetricGaussSeidel */
+    int force_cg;               /* explicit reghdfe convergence criterion -> CG accelerator */
+    int precision;              /* 0: FP64 throughout (the plain method); 1: FP32 sweeps under FP64 refinement */
+} xfe_options;
+
+/* Outputs of one absorption (the fields of hdfe::detail::AbsorptionResult,
+ * include/fe_absorption.hpp:24-47, that the GPU backend fills). */
+typedef struct {
+    int iterations;
+    int converged;
+    int used_accel;
+    int used_cg;
+    int polish_sweeps;
+    double polish_final_max;
+    int sweep_order_used[XFE_MAX_DIMS];
+    int fe_levels[XFE_MAX_DIMS];
+    int sweeps;                 /* projection sweeps executed (diagnostic) */
+    int refinements;            /* FP64 refinement rounds (precision 1) */
+    double solve_ms;            /* device wall time of the solve */
+} xfe_absorb_result;
+
+/* cuda_backend_available (src/fe_absorption_cuda.hpp:125,
+ * impl src/fe_absorption_cuda.cu:2281).  1 if a gfx950 device is usable. */
+int xfe_backend_available(void);
+
+/* absorb_fixed_effects_cuda (src/fe_absorption_cuda.hpp:127-134,
+ * impl src/fe_absorption_cuda.cu:2294-4738).  Host buffers: y (n), X (n x k,
+ * column-major, ld = n), optional weights (n).  sweep_order: num_fes entries
+ * (0-based dims) or NULL for the default.  Writes y_out (n) and X_out (n x k).
+ * Returns 1 on success (check res->converged, as the reference does), 0 on a
+ * device/allocation/launch failure (the caller maps it to gpu status 4). */
+int xfe_absorb(const double* y, const double* X, int n, int k,
+               const xfe_fe_input* fes, int num_fes, const double* weights,
+               const int* sweep_order, const xfe_options* opt,
+               xfe_absorb_result* res, double* y_out, double* X_out);
+
+/* Same computation with device-resident buffers (inputs already in HBM):
+ * d_V holds [y | X] column-major with leading dimension ld >= n and is
+ * overwritten with the within transform; d_group_ids is num_fes x n
+ * (dimension-major, ld n) compact ids; d_weight_sums may be NULL (computed on
+ * the device).  No counterpart in the reference (its backend always stages
+ * host memory); it serves callers that keep their panel on the device. */
+int xfe_absorb_device(double* d_V, int n, int C, int ld,
+                      const int* d_group_ids, const int* num_groups, int num_fes,
+                      const double* d_weights, const int* sweep_order,
+                      const xfe_options* opt, xfe_absorb_result* res);
+
+/* compute_keep_mask_drop_singletons (src/hdfe_regressor_v11.cpp:4056-4302):
+ * iterative singleton drop on the device, bit-exact.  fes: num_fes x n raw
+ * int32 ids (host).  keep (n bytes) out.  Returns dropped rows, -1 on error. */
+int xfe_drop_singletons(const int* fes, int n, int num_fes, int max_rounds,
+                        unsigned char* keep, int* rounds_used);
+
+/* count_bipartite_components (src/hdfe_regressor_v11.cpp:3579-3685) with the
+ * node numbering of build_lookup_compact (:1354-1417): mobility groups of two
+ * FE dimensions.  a, b: raw ids (host, n).  expected_a/expected_b: level
+ * counts as the reference's caller passes them (-1 if unknown).  labels (n,
+ * optional) receives 1-based component ids ranked by the smallest node.
+ * Returns the component count, -1 on error. */
+int xfe_bipartite_components(const int* a, const int* b, int n, int expected_a,
+                             int expected_b, int* labels);
+
+/* compute_fe_dof_reghdfe (src/hdfe_regressor_v11.cpp:3743-3986), pairwise
+ * ("all") method: redundant levels per dimension from mobility groups, with the
+ * nested-FE coarsening.  fes: num_fes x n raw ids; levels: levels per dim.
+ * Writes redundant / num_coefs (num_fes each).  Returns df_a, -1 on error. */
+int xfe_fe_dof(const int* fes, int n, int num_fes, const int* levels,
+               int* redundant, int* num_coefs);
+
+/* X~'X~ Gram and cluster meat for the OLS step after absorption
+ * (src/ols.cpp:1338 Gram, :889-1046 one-way cluster meat).  V: n x C
+ * column-major [y~ | X~] (host).  Outputs: gram (C x C, column-major, over all
+ * columns incl. y), and when cluster != NULL the k x k meat
+ * sum_g (X_g' e_g)(X_g' e_g)' for residuals e = y~ - X~ beta (beta: k).
+ * Returns 0 on success. */
+int xfe_gram(const double* V, int n, int C, double* gram);
+int xfe_cluster_meat(const double* V, int n, int C, const double* beta,
+                     const int* cluster, int num_clusters, double* meat);
+
+/* fe_recovery_max_delta_cuda (src/fe_absorption_cuda.hpp:152-156, impl
+ * src/fe_absorption_cuda.cu:5227): one Gauss-Seidel pass of `residual` over the
+ * dims in sweep_order, returns max_d max_g |weighted mean_g| in *out_max_delta.
+ * residual (host, n) is not modified.  Returns 1 on success, 0 on failure. */
+int xfe_recovery_max_delta(const double* residual, int n, const xfe_fe_input* fes,
+                           int num_fes, const double* weights, const int* sweep_order,
+                           double* out_max_delta);
+
+/* Frees the calling thread's device workspace (it otherwise lives until the
+ * process ends).  No counterpart in the reference. */
+void xfe_release_workspace(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char* xfe_last_error(void);
+
+/* Kernel timing for the roofline figure: when enabled, the library brackets
+ * every launch of the fused sweep pass with HIP events on its own stream and
+ * accumulates the total milliseconds, launch count and algorithmic bytes. */
+void xfe_profile_enable(int on);
+void xfe_profile_reset(void);
+void xfe_profile_get(double* pass_ms, long long* pass_launches, double* pass_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XFE_HIP_H */

@@ -1,0 +1,285 @@
+"""CPU oracle for the fixed-effects absorber -- TEST INFRASTRUCTURE ONLY.
+
+Loaded only by tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg, always as the checker.  The product (the HIP library
+under ``xhdfe-xfe_amd/``) never imports it and never falls back to it.
+
+* ``liboracle.so`` (oracle.c): plain-C restatement of the reference CPU code
+  (citations per function in oracle.c).
+* ``_ref/libxhdfe_ref.so``: the reference's own CPU core compiled from
+  /root/reference by ``oracle/Makefile`` (present only where the reference
+  is; used to pin the restatement).
+* ``ols_cluster``: numpy restatement of the OLS + one-way cluster VCV
+  (src/ols.cpp:1237-1556, scale G/(G-1) (N-1)/df_resid at :682-687).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+_ref = None
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_p = ctypes.c_void_p
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("tol", _d), ("tolerance_mode", _i), ("max_iter", _i), ("symmetric", _i),
+                ("force_cg", _i), ("threads", _i), ("sweep_order", _p)]
+
+
+class _Res(ctypes.Structure):
+    _fields_ = [("iterations", _i), ("converged", _i), ("used_accel", _i), ("used_cg", _i),
+                ("polish_sweeps", _i), ("polish_final_max", _d), ("sweep_order", _i * 8)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            import subprocess
+            subprocess.run(["make", "-C", _HERE, "oracle"], check=True, capture_output=True)
+        _lib = ctypes.CDLL(path)
+    return _lib
+
+
+def ref():
+    """The compiled reference CPU core, or None where it is not built."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(_HERE, "_ref", "libxhdfe_ref.so")
+        if not os.path.exists(path):
+            return None
+        _ref = ctypes.CDLL(path)
+    return _ref
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_p) if a is not None else None
+
+
+def build_indexer(raw):
+    raw = np.ascontiguousarray(raw, dtype=np.int32)
+    gid = np.empty_like(raw)
+    G, lp, cont = _i(), _i(), _i()
+    lib().or_build_indexer(_i(raw.size), _ptr(raw), _ptr(gid), ctypes.byref(G), ctypes.byref(lp),
+                           ctypes.byref(cont))
+    return gid, G.value, lp.value, bool(cont.value)
+
+
+def absorb(V, gids, G, weights=None, tol=1e-8, tolerance_mode=1, max_iter=100000,
+           symmetric=None, force_cg=False, threads=None, sweep_order=None):
+    """Within transform of the n x C block V (column-major [y | X]).
+
+    gids: list of compact id arrays; G: level counts.  ``symmetric`` defaults to
+    the reference's method choice (SGS for two FEs, GS otherwise;
+    src/fe_absorption.cpp:5568-5589)."""
+    V = np.array(V, dtype=np.float64, order="F", copy=True)
+    n, C = V.shape
+    D = len(gids)
+    ids = np.ascontiguousarray(np.stack([np.asarray(g, dtype=np.int32) for g in gids])) if D else \
+        np.zeros((0, n), dtype=np.int32)
+    Garr = np.asarray(G, dtype=np.int32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    if symmetric is None:
+        symmetric = D == 2
+    order = None if sweep_order is None else np.asarray(sweep_order, dtype=np.int32)
+    o = _Opts(tol, tolerance_mode, max_iter, int(bool(symmetric)), int(bool(force_cg)),
+              threads or min(8, os.cpu_count() or 1), _ptr(order) if order is not None else None)
+    r = _Res()
+    rc = lib().or_absorb(_i(n), _i(C), _ptr(V), _i(D), _ptr(ids), _ptr(Garr), _ptr(w),
+                         ctypes.byref(o), ctypes.byref(r))
+    if rc != 0:
+        raise RuntimeError("or_absorb failed")
+    info = dict(iterations=r.iterations, converged=bool(r.converged), used_accel=bool(r.used_accel),
+                used_cg=bool(r.used_cg), polish_sweeps=r.polish_sweeps,
+                polish_final_max=r.polish_final_max, sweep_order=list(r.sweep_order)[:D])
+    return V, info
+
+
+def project(V, gid, G, weights=None):
+    V = np.array(V, dtype=np.float64, order="F", copy=True)
+    n, C = V.shape
+    gid = np.ascontiguousarray(gid, dtype=np.int32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    lib().or_project(_i(n), _i(C), _ptr(V), _ptr(gid), _i(G), _ptr(w))
+    return V
+
+
+def drop_singletons(fes, max_rounds=100):
+    fes = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    D, n = fes.shape
+    keep = np.empty(n, dtype=np.uint8)
+    rounds = _i()
+    dropped = lib().or_drop_singletons(_i(n), _i(D), _ptr(fes), _i(max_rounds), _ptr(keep),
+                                       ctypes.byref(rounds))
+    return keep.astype(bool), dropped, rounds.value
+
+
+def components(a, b, expected_a=-1, expected_b=-1, labels=True):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    lab = np.empty(a.size, dtype=np.int32) if labels else None
+    c = lib().or_components(_i(a.size), _ptr(a), _ptr(b), _i(expected_a), _i(expected_b), _ptr(lab))
+    return c, lab
+
+
+def dof_reghdfe(fes, levels):
+    fes = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    D, n = fes.shape
+    lv = np.asarray(levels, dtype=np.int32)
+    red = np.empty(D, dtype=np.int32)
+    nc = np.empty(D, dtype=np.int32)
+    df = lib().or_dof_reghdfe(_i(n), _i(D), _ptr(fes), _ptr(lv), _ptr(red), _ptr(nc))
+    return df, red, nc
+
+
+def fe_is_function_of(coarse, refined):
+    coarse = np.ascontiguousarray(coarse, dtype=np.int32)
+    refined = np.ascontiguousarray(refined, dtype=np.int32)
+    return bool(lib().or_fe_is_function_of(_i(coarse.size), _ptr(coarse), _ptr(refined)))
+
+
+def fit_dof(fes, levels, cluster=None):
+    """The estimator's FE degrees of freedom (src/hdfe_regressor_v11.cpp:9540-9700,
+    no slopes): FEs nested in the cluster variable (fe_nested_within_cluster,
+    :6765 -- the cluster id is a function of the FE level) are fully redundant,
+    pairwise mobility groups over the others (compute_fe_dof_reghdfe), then one
+    redundant level for every intercept dimension except the first when none is
+    nested.  Returns (df_a, redundant, num_coefs)."""
+    D = len(fes)
+    levels = [int(x) for x in levels]
+    nested = [cluster is not None and fe_is_function_of(cluster, fes[d]) for d in range(D)]
+    red = [0] * D
+    nn = [d for d in range(D) if not nested[d]]
+    if nn:
+        _, r2, _ = dof_reghdfe([fes[d] for d in nn], [levels[d] for d in nn])
+        for q, d in enumerate(nn):
+            red[d] = int(r2[q])
+    nested_total = sum(levels[d] for d in range(D) if nested[d])
+    for d in range(D):
+        if nested[d]:
+            red[d] = levels[d]
+    skip_next = nested_total == 0
+    for d in nn:
+        if red[d] >= levels[d]:
+            continue
+        if skip_next:
+            skip_next = False
+            continue
+        red[d] = max(red[d], 1)
+    nc = [max(0, levels[d] - red[d]) for d in range(D)]
+    return sum(nc), np.array(red), np.array(nc)
+
+
+def fe_collinear(X_raw, X_t):
+    """Columns omitted as collinear with the FEs: ||M_D x||^2 <= 1e-9 ||x||^2
+    (src/hdfe_regressor_v11.cpp:8668-8700)."""
+    raw = (np.asarray(X_raw) ** 2).sum(axis=0)
+    til = (np.asarray(X_t) ** 2).sum(axis=0)
+    return np.where(raw <= 0.0, til <= 1e-9, til <= 1e-9 * raw)
+
+
+def ols_cluster(y_t, X_t, cluster, df_a, omit=None):
+    """OLS on the within-transformed data with a one-way cluster VCV.
+
+    V = (X'X)^-1 (sum_g s_g s_g') (X'X)^-1 * G/(G-1) * (N-1)/df_r
+    (src/ols.cpp:657-687) with the estimator's residual df
+    df_r = N - k - 1 - df_a (intercept dof, src/hdfe_regressor_v11.cpp:9821-9824)."""
+    X_all = np.asarray(X_t, dtype=np.float64)
+    y = np.asarray(y_t, dtype=np.float64)
+    keep_cols = np.ones(X_all.shape[1], bool) if omit is None else ~np.asarray(omit)
+    X = X_all[:, keep_cols]
+    n, k = X.shape
+    xtx = X.T @ X
+    beta = np.linalg.solve(xtx, X.T @ y)
+    e = y - X @ beta
+    _, inv_c = np.unique(cluster, return_inverse=True)
+    G = int(inv_c.max()) + 1
+    S = np.zeros((G, k))
+    np.add.at(S, inv_c, X * e[:, None])
+    meat = S.T @ S
+    bread = np.linalg.inv(xtx)
+    df_resid = n - k - 1 - df_a
+    V = bread @ meat @ bread * (G / (G - 1.0)) * ((n - 1.0) / df_resid)
+    beta_all = np.zeros(X_all.shape[1])
+    se_all = np.full(X_all.shape[1], np.nan)  # omitted: missing SE, as the estimator reports
+    beta_all[keep_cols] = beta
+    se_all[keep_cols] = np.sqrt(np.diag(V))
+    return beta_all, se_all, e, G
+
+
+# ------------------------------------------------------------ reference checker
+
+def ref_absorb(y, X, fes, tol=1e-8, tolerance_mode=1, threads=1, symmetric=False):
+    r = ref()
+    if r is None:
+        return None
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(X, dtype=np.float64)
+    n, k = X.shape
+    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    yo = np.empty(n)
+    Xo = np.empty((n, k), order="F")
+    it, conv = _i(), _i()
+    rc = r.ref_absorb(_i(n), _i(k), _i(len(fes)), _ptr(y), _ptr(X), _ptr(ids), _d(tol),
+                      _i(tolerance_mode), _i(threads), _i(int(symmetric)), _ptr(yo), _ptr(Xo),
+                      ctypes.byref(it), ctypes.byref(conv))
+    if rc != 0:
+        raise RuntimeError("ref_absorb failed")
+    return yo, Xo, it.value, bool(conv.value)
+
+
+def ref_partial_out(y, X, fes, tol=1e-8, tolerance_mode=1, threads=1):
+    r = ref()
+    if r is None:
+        return None
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(X, dtype=np.float64)
+    n, k = X.shape
+    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    idx = np.empty(n, dtype=np.int32)
+    yo = np.empty(n)
+    Xo = np.empty((n, k), order="F")
+    it, conv = _i(), _i()
+    kept = r.ref_partial_out(_i(n), _i(k), _i(len(fes)), _ptr(y), _ptr(X), _ptr(ids), _d(tol),
+                             _i(tolerance_mode), _i(threads), _ptr(idx), _ptr(yo), _ptr(Xo),
+                             ctypes.byref(it), ctypes.byref(conv))
+    if kept < 0:
+        raise RuntimeError("ref_partial_out failed")
+    return idx[:kept].copy(), yo[:kept].copy(), np.asfortranarray(Xo.reshape(-1, order="F")[:kept * k].reshape(kept, k, order="F")), it.value, bool(conv.value)
+
+
+def ref_fit(y, X, fes, cluster=None, tol=1e-8, tolerance_mode=1, threads=1, groupvar=True):
+    r = ref()
+    if r is None:
+        return None
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(X, dtype=np.float64)
+    n, k = X.shape
+    D = len(fes)
+    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    cl = None if cluster is None else np.ascontiguousarray(cluster, dtype=np.int32)
+    coef = np.zeros(k)
+    se = np.zeros(k)
+    idx = np.empty(n, dtype=np.int32)
+    gv = np.zeros(n, dtype=np.int32)
+    df_a, df_r, rss = _d(), _d(), _d()
+    red = np.zeros(D, dtype=np.int32)
+    lev = np.zeros(D, dtype=np.int32)
+    ns, it = _i(), _i()
+    kept = r.ref_fit(_i(n), _i(k), _i(D), _ptr(y), _ptr(X), _ptr(ids), _ptr(cl), _d(tol),
+                     _i(tolerance_mode), _i(threads), _i(int(groupvar)), _ptr(coef), _ptr(se),
+                     _ptr(idx), _ptr(gv), ctypes.byref(df_a), ctypes.byref(df_r), _ptr(red),
+                     _ptr(lev), ctypes.byref(ns), ctypes.byref(it), ctypes.byref(rss))
+    if kept < 0:
+        raise RuntimeError("ref_fit failed")
+    return dict(coef=coef, se=se, sample_index=idx[:kept].copy(), groupvar=gv[:kept].copy(),
+                df_a=df_a.value, df_resid=df_r.value, fe_redundant=red, fe_levels=lev,
+                num_singletons=ns.value, iterations=it.value, rss=rss.value, nobs=kept)

@@ -1,0 +1,147 @@
+// Test infrastructure only: a C-ABI shim over the reference's own CPU core
+// (xhdfe-xfe src/*.cpp compiled from /root/reference by oracle/Makefile into
+// oracle/_ref/libxhdfe_ref.so).  It is the checker used to pin the oracle's
+// restatement in this container; it never ships and never runs on a GPU.
+//
+// Entry points follow the reference's public estimator interface:
+//   HdfeRegressorV11::partial_out  (src/hdfe_regressor_v11.cpp:10275) -- xfe
+//   HdfeRegressorV11::fit          (src/hdfe_regressor_v11.cpp:7522)  -- xhdfe
+//   detail::absorb_fixed_effects   (src/fe_absorption.cpp:5993)       -- raw CPU absorber
+#include <Eigen/Dense>
+
+#include <cstring>
+#include <exception>
+#include <cstdio>
+#include <vector>
+
+#include "fe_absorption.hpp"
+#include "hdfe/hdfe_regressor.hpp"
+#include "hdfe/hdfe_regressor_v11.hpp"
+
+namespace {
+
+std::vector<Eigen::VectorXi> make_fes(const int* fes, int n, int d) {
+    std::vector<Eigen::VectorXi> out;
+    for (int k = 0; k < d; ++k) {
+        Eigen::VectorXi v(n);
+        std::memcpy(v.data(), fes + static_cast<std::size_t>(k) * n, sizeof(int) * n);
+        out.push_back(std::move(v));
+    }
+    return out;
+}
+
+hdfe::HdfeOptions make_options(double tol, int tolerance_mode, int threads) {
+    hdfe::HdfeOptions o;
+    o.tol = tol;
+    o.tolerance_mode = static_cast<hdfe::ToleranceMode>(tolerance_mode);
+    o.num_threads = threads;
+    return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Raw CPU absorber on already-filtered rows (no singleton drop).
+// y: n, X: n*k column-major, fes: d*n (dimension-major). Outputs overwrite
+// y_out/X_out.  Returns 0 on success, -1 on exception.
+int ref_absorb(int n, int k, int d, const double* y, const double* X, const int* fes,
+               double tol, int tolerance_mode, int threads, int symmetric,
+               double* y_out, double* X_out, int* iterations, int* converged) {
+    try {
+        Eigen::VectorXd yv = Eigen::Map<const Eigen::VectorXd>(y, n);
+        Eigen::MatrixXd Xm = Eigen::Map<const Eigen::MatrixXd>(X, n, k);
+        auto fv = make_fes(fes, n, d);
+        hdfe::HdfeOptions o = make_options(tol, tolerance_mode, threads);
+        o.symmetric_sweep = symmetric != 0;
+        auto r = hdfe::detail::absorb_fixed_effects(yv, Xm, fv, nullptr, o);
+        std::memcpy(y_out, r.y_tilde.data(), sizeof(double) * n);
+        if (k > 0) std::memcpy(X_out, r.X_tilde.data(), sizeof(double) * n * k);
+        *iterations = r.iterations;
+        *converged = r.converged ? 1 : 0;
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_absorb: %s\n", e.what());
+        return -1;
+    }
+}
+
+// xfe path: singleton drop + absorption.  Returns the number of kept rows
+// (>= 0) and fills sample_index (n_full capacity), y_out/X_out (kept rows).
+int ref_partial_out(int n, int k, int d, const double* y, const double* X, const int* fes,
+                    double tol, int tolerance_mode, int threads,
+                    int* sample_index, double* y_out, double* X_out, int* iterations,
+                    int* converged) {
+    try {
+        Eigen::VectorXd yv = Eigen::Map<const Eigen::VectorXd>(y, n);
+        Eigen::MatrixXd Xm = Eigen::Map<const Eigen::MatrixXd>(X, n, k);
+        auto fv = make_fes(fes, n, d);
+        hdfe::v11::HdfeRegressorV11 reg(make_options(tol, tolerance_mode, threads));
+        auto r = reg.partial_out(yv, Xm, fv, nullptr, nullptr, nullptr);
+        const auto& res = reg.results();
+        const int kept = static_cast<int>(r.y_tilde.size());
+        for (int i = 0; i < kept; ++i) {
+            sample_index[i] = res.sample_index.size() == kept ? res.sample_index[i] : i;
+        }
+        std::memcpy(y_out, r.y_tilde.data(), sizeof(double) * kept);
+        if (k > 0) std::memcpy(X_out, r.X_tilde.data(), sizeof(double) * kept * k);
+        *iterations = r.iterations;
+        *converged = r.converged ? 1 : 0;
+        return kept;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_partial_out: %s\n", e.what());
+        return -1;
+    }
+}
+
+// xhdfe path: singletons, absorption, OLS with SEs clustered on `cluster`
+// (or robust when cluster == nullptr), reghdfe DoF, first-pair mobility
+// groups.  Output arrays are sized by the caller: coef/se k, sample_index
+// and groupvar n, fe_redundant/fe_levels d.  Returns kept rows or -1.
+int ref_fit(int n, int k, int d, const double* y, const double* X, const int* fes,
+            const int* cluster, double tol, int tolerance_mode, int threads,
+            int want_groupvar, double* coef, double* se, int* sample_index, int* groupvar,
+            double* df_a, double* df_resid, int* fe_redundant, int* fe_levels,
+            int* num_singletons, int* iterations, double* rss) {
+    try {
+        Eigen::VectorXd yv = Eigen::Map<const Eigen::VectorXd>(y, n);
+        Eigen::MatrixXd Xm = Eigen::Map<const Eigen::MatrixXd>(X, n, k);
+        auto fv = make_fes(fes, n, d);
+        hdfe::HdfeOptions o = make_options(tol, tolerance_mode, threads);
+        o.save_groupvar = want_groupvar != 0;
+        std::vector<Eigen::VectorXi> cl;
+        if (cluster) {
+            o.se_type = hdfe::StandardErrorType::Cluster;
+            Eigen::VectorXi c(n);
+            std::memcpy(c.data(), cluster, sizeof(int) * n);
+            cl.push_back(std::move(c));
+        }
+        hdfe::v11::HdfeRegressorV11 reg(o);
+        reg.fit(yv, Xm, fv, nullptr, cluster ? &cl : nullptr);
+        const auto& r = reg.results();
+        const int kc = static_cast<int>(r.coefficients.size());
+        for (int j = 0; j < k && j < kc; ++j) {
+            coef[j] = r.coefficients[j];
+            se[j] = r.std_errors[j];
+        }
+        for (int i = 0; i < r.nobs; ++i) {
+            sample_index[i] = r.sample_index.size() == r.nobs ? r.sample_index[i] : i;
+            if (groupvar) groupvar[i] = r.groupvar.size() == r.nobs ? r.groupvar[i] : 0;
+        }
+        *df_a = r.df_a;
+        *df_resid = r.df_resid;
+        for (int j = 0; j < d; ++j) {
+            fe_redundant[j] = j < static_cast<int>(r.fe_redundant.size()) ? r.fe_redundant[j] : -1;
+            fe_levels[j] = j < static_cast<int>(r.fe_num_levels.size()) ? r.fe_num_levels[j] : -1;
+        }
+        *num_singletons = r.num_singletons;
+        *iterations = r.num_iterations;
+        *rss = r.rss;
+        return r.nobs;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_fit: %s\n", e.what());
+        return -1;
+    }
+}
+
+}  // extern "C"

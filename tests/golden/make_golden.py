@@ -1,0 +1,60 @@
+"""Writes tests/golden/ref_cases.npz from the reference's own CPU core.
+
+Run in the build container only (it needs oracle/_ref/libxhdfe_ref.so, which
+oracle/Makefile compiles from the reference sources).  Inputs are regenerated
+from the seeded panel generators, so the fixture holds only the reference's
+outputs plus checksums of the inputs they were computed from.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "xhdfe-xfe_amd"))
+
+import oracle  # noqa: E402
+from golden_cases import CASES, input_checksum  # noqa: E402
+
+
+def main():
+    if oracle.ref() is None:
+        raise SystemExit("oracle/_ref/libxhdfe_ref.so is not built (make -C oracle ref)")
+    out = {}
+    for name, make in CASES.items():
+        p = make()
+        y, X, fes = p["y"], p["X"], p["fes"]
+        out[f"{name}/input_checksum"] = input_checksum(p)
+        # xfe path (singletons + the estimator's own absorber choice; small
+        # panels may take its sparse direct solve)
+        idx, yo, Xo, it, conv = oracle.ref_partial_out(y, X, fes, tol=1e-12)
+        out[f"{name}/xfe/sample_index"] = idx
+        out[f"{name}/xfe/y_tilde"] = yo
+        out[f"{name}/xfe/X_tilde"] = Xo
+        # the MAP absorber itself (absorb_fixed_effects) on the kept rows, with
+        # the method the estimator picks (symmetric GS for two FEs)
+        yk, Xk = y[idx], X[idx]
+        fk = [f[idx] for f in fes]
+        for tag, tol in (("tight", 1e-12), ("default", 1e-8)):
+            yo, Xo, it, conv = oracle.ref_absorb(yk, Xk, fk, tol=tol, symmetric=len(fes) == 2)
+            out[f"{name}/{tag}/y_tilde"] = yo
+            out[f"{name}/{tag}/X_tilde"] = Xo
+            out[f"{name}/{tag}/iterations"] = np.int64(it)
+            out[f"{name}/{tag}/converged"] = np.int64(conv)
+        f = oracle.ref_fit(y, X, fes, cluster=p["cluster"], tol=1e-12)
+        for key in ("coef", "se", "sample_index", "groupvar", "fe_redundant", "fe_levels"):
+            out[f"{name}/fit/{key}"] = np.asarray(f[key])
+        for key in ("df_a", "df_resid", "num_singletons", "nobs", "rss"):
+            out[f"{name}/fit/{key}"] = np.asarray(f[key])
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_cases.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,254 @@
+"""The oracle pinned against the reference's outputs, closed forms and brute force.
+
+CPU only.  ref_cases.npz holds the reference CPU core's outputs on seeded
+panels (written by tests/golden/make_golden.py).
+"""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from golden_cases import input_checksum
+
+
+def _compact(fes):
+    out = [oracle.build_indexer(f) for f in fes]
+    return [o[0] for o in out], [o[1] for o in out]
+
+
+def _kept(case):
+    keep, _, _ = oracle.drop_singletons(case["fes"])
+    return keep
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_inputs_regenerate(golden, cases, name):
+    assert input_checksum(cases[name]) == golden[f"{name}/input_checksum"]
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_singletons_match_reference(golden, cases, name):
+    keep = _kept(cases[name])
+    idx = np.flatnonzero(keep)
+    np.testing.assert_array_equal(idx, golden[f"{name}/xfe/sample_index"])
+    np.testing.assert_array_equal(idx, golden[f"{name}/fit/sample_index"])
+    assert (~keep).sum() == golden[f"{name}/fit/num_singletons"]
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+@pytest.mark.parametrize("tag", ["tight", "default"])
+def test_absorb_matches_reference(golden, cases, name, tag):
+    p = cases[name]
+    keep = _kept(p)
+    fes = [f[keep] for f in p["fes"]]
+    gids, G = _compact(fes)
+    V = np.column_stack([p["y"][keep], p["X"][keep]])
+    tol = 1e-12 if tag == "tight" else 1e-8
+    Vo, info = oracle.absorb(V, gids, G, tol=tol)
+    ref_y = golden[f"{name}/{tag}/y_tilde"]
+    ref_X = golden[f"{name}/{tag}/X_tilde"]
+    scale = np.abs(V).max()
+    err = max(np.abs(Vo[:, 0] - ref_y).max(), np.abs(Vo[:, 1:] - ref_X).max()) / scale
+    assert err <= (1e-11 if tag == "tight" else 1e-9), err
+    assert info["converged"]
+    # same algorithm, same trajectory: iteration counts agree with the reference
+    assert info["iterations"] == int(golden[f"{name}/{tag}/iterations"])
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_absorb_matches_reference_xfe(golden, cases, name):
+    """Against the estimator's own partial_out (which may take its exact sparse
+    solver on small panels): the within transform agrees to the solver tolerance."""
+    p = cases[name]
+    keep = _kept(p)
+    fes = [f[keep] for f in p["fes"]]
+    gids, G = _compact(fes)
+    V = np.column_stack([p["y"][keep], p["X"][keep]])
+    Vo, _ = oracle.absorb(V, gids, G, tol=1e-12)
+    ref = np.column_stack([golden[f"{name}/xfe/y_tilde"], golden[f"{name}/xfe/X_tilde"]])
+    assert np.abs(Vo - ref).max() / np.abs(V).max() < 1e-9
+
+
+def _direct_residual(V, fes):
+    """Least-squares residual of V on the FE dummies by a sparse direct solve
+    of the grounded normal equations (closed form of the within transform)."""
+    from scipy.sparse.csgraph import connected_components
+    n = V.shape[0]
+    _, ia = np.unique(fes[0], return_inverse=True)
+    _, ib = np.unique(fes[1], return_inverse=True)
+    Ga, Gb = ia.max() + 1, ib.max() + 1
+    Da = sp.csr_matrix((np.ones(n), (np.arange(n), ia)), shape=(n, Ga))
+    Db = sp.csr_matrix((np.ones(n), (np.arange(n), ib)), shape=(n, Gb))
+    adj = sp.bmat([[None, Da.T @ Db], [Db.T @ Da, None]])
+    _, comp = connected_components(adj, directed=False)
+    # ground one b-level per mobility group (the dummies are rank deficient by
+    # exactly one per group)
+    first_b = {}
+    for j in range(Gb):
+        first_b.setdefault(comp[Ga + j], j)
+    keep_b = np.setdiff1d(np.arange(Gb), np.array(list(first_b.values())))
+    Dm = sp.hstack([Da, Db[:, keep_b]]).tocsc()
+    A = (Dm.T @ Dm).tocsc()
+    out = np.empty_like(V)
+    lu = spla.splu(A)
+    for c in range(V.shape[1]):
+        coef = lu.solve(Dm.T @ V[:, c])
+        out[:, c] = V[:, c] - Dm @ coef
+    return out
+
+
+@pytest.mark.parametrize("name", ["two_way", "sparse_two_way"])
+def test_absorb_matches_direct_solve(cases, name):
+    p = cases[name]
+    keep = _kept(p)
+    fes = [f[keep] for f in p["fes"]]
+    gids, G = _compact(fes)
+    V = np.column_stack([p["y"][keep], p["X"][keep]])
+    Vo, _ = oracle.absorb(V, gids, G, tol=1e-13, tolerance_mode=2)
+    ref = _direct_residual(V, fes)
+    assert np.abs(Vo - ref).max() / np.abs(V).max() < 1e-9
+
+
+def test_one_way_is_group_demeaning():
+    rng = np.random.default_rng(1)
+    g = rng.integers(0, 50, 3000).astype(np.int32)
+    V = rng.standard_normal((3000, 3))
+    out = oracle.project(V, g, 50)
+    means = np.zeros((50, 3))
+    np.add.at(means, g, V)
+    means /= np.bincount(g, minlength=50)[:, None]
+    np.testing.assert_allclose(out, V - means[g], atol=1e-14)
+
+
+def test_absorb_invariants(cases):
+    p = cases["akm_three_way"]
+    keep = _kept(p)
+    fes = [f[keep] for f in p["fes"]]
+    gids, G = _compact(fes)
+    V = np.column_stack([p["y"][keep], p["X"][keep]])
+    Vo, _ = oracle.absorb(V, gids, G, tol=1e-12)
+    for g, Gd in zip(gids, G):   # D' v~ = 0 for every dimension
+        s = np.zeros((Gd, V.shape[1]))
+        np.add.at(s, g, Vo)
+        assert np.abs(s).max() < 1e-8 * np.abs(V).max()
+    V2, _ = oracle.absorb(Vo, gids, G, tol=1e-12)  # idempotence
+    assert np.abs(V2 - Vo).max() < 1e-9 * np.abs(V).max()
+
+
+def _brute_singletons(fes, max_rounds=100):
+    keep = np.ones(fes[0].size, bool)
+    for _ in range(max_rounds):
+        drop = np.zeros_like(keep)
+        for f in fes:
+            vals, cnt = np.unique(f[keep], return_counts=True)
+            single = set(vals[cnt == 1].tolist())
+            drop |= keep & np.isin(f, list(single))
+        if not drop.any():
+            break
+        keep &= ~drop
+    return keep
+
+
+def test_singletons_brute_force():
+    rng = np.random.default_rng(3)
+    for trial in range(30):
+        n = int(rng.integers(1, 60))
+        D = int(rng.integers(1, 4))
+        fes = [rng.integers(-5, int(rng.integers(2, 30)), n).astype(np.int32) for _ in range(D)]
+        keep, dropped, _ = oracle.drop_singletons(fes)
+        np.testing.assert_array_equal(keep, _brute_singletons(fes))
+        assert dropped == (~keep).sum()
+
+
+def _bfs_labels(a, b):
+    """Components of the bipartite graph; label = rank of the component's first
+    a-node in first-appearance order of a's values (DenseMap/Sparse numbering)."""
+    order_a = list(dict.fromkeys(a.tolist()))
+    node_a = {v: i for i, v in enumerate(order_a)}
+    adj = {}
+    for x, y in zip(a.tolist(), b.tolist()):
+        adj.setdefault(("a", x), set()).add(("b", y))
+        adj.setdefault(("b", y), set()).add(("a", x))
+    comp = {}
+    c = 0
+    for v in order_a:
+        if ("a", v) in comp:
+            continue
+        c += 1
+        stack = [("a", v)]
+        comp[("a", v)] = c
+        while stack:
+            u = stack.pop()
+            for w in adj[u]:
+                if w not in comp:
+                    comp[w] = c
+                    stack.append(w)
+    return c, np.array([comp[("a", x)] for x in a.tolist()], dtype=np.int32)
+
+
+def test_components_bfs():
+    rng = np.random.default_rng(4)
+    for trial in range(30):
+        n = int(rng.integers(1, 80))
+        a = rng.integers(0, int(rng.integers(1, 40)), n).astype(np.int32) * 3 + 7  # sparse ids
+        b = rng.integers(-20, int(rng.integers(-18, 30)), n).astype(np.int32)
+        c, lab = oracle.components(a, b)
+        c2, lab2 = _bfs_labels(a, b)
+        assert c == c2
+        np.testing.assert_array_equal(lab, lab2)
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_groupvar_and_dof_match_reference(golden, cases, name):
+    p = cases[name]
+    keep = _kept(p)
+    fes = [f[keep] for f in p["fes"]]
+    levels = [np.unique(f).size for f in fes]
+    c, lab = oracle.components(fes[0], fes[1], levels[0], levels[1])
+    np.testing.assert_array_equal(lab, golden[f"{name}/fit/groupvar"])
+    df, red, nc = oracle.dof_reghdfe(fes, levels)
+    assert red[1] == c or len(fes) > 2
+    df_a, red_fit, _ = oracle.fit_dof(fes, levels, p["cluster"][keep])
+    np.testing.assert_array_equal(red_fit, golden[f"{name}/fit/fe_redundant"])
+    assert df_a == golden[f"{name}/fit/df_a"]
+
+
+def _dense_rank_dof(fes):
+    n = fes[0].size
+    cols = []
+    for f in fes:
+        _, inv = np.unique(f, return_inverse=True)
+        D = np.zeros((n, inv.max() + 1))
+        D[np.arange(n), inv] = 1.0
+        cols.append(D)
+    return np.linalg.matrix_rank(np.hstack(cols))
+
+
+def test_dof_two_way_is_exact_rank():
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        n = int(rng.integers(5, 120))
+        fes = [rng.integers(0, int(rng.integers(2, 25)), n).astype(np.int32) for _ in range(2)]
+        levels = [np.unique(f).size for f in fes]
+        df, red, nc = oracle.dof_reghdfe(fes, levels)
+        # for two FEs the mobility-group count is exact: rank = levels - components
+        assert df == _dense_rank_dof(fes) + (1 if red[1] == 0 else 0) or df == _dense_rank_dof(fes)
+
+
+def test_ols_cluster_matches_reference(golden, cases):
+    for name in ("two_way", "akm_three_way"):
+        p = cases[name]
+        keep = _kept(p)
+        fes = [f[keep] for f in p["fes"]]
+        gids, G = _compact(fes)
+        V = np.column_stack([p["y"][keep], p["X"][keep]])
+        Vo, _ = oracle.absorb(V, gids, G, tol=1e-12)
+        levels = [np.unique(f).size for f in fes]
+        df_a, _, _ = oracle.fit_dof(fes, levels, p["cluster"][keep])
+        omit = oracle.fe_collinear(V[:, 1:], Vo[:, 1:])
+        beta, se, _, _ = oracle.ols_cluster(Vo[:, 0], Vo[:, 1:], p["cluster"][keep], df_a, omit)
+        np.testing.assert_allclose(beta, golden[f"{name}/fit/coef"], rtol=1e-9)
+        np.testing.assert_allclose(se, golden[f"{name}/fit/se"], rtol=1e-7)

@@ -1,0 +1,965 @@
+// Host driver of the MI355X fixed-effects absorber: the C ABI of
+// include/xfe_hip.h for absorption (xfe_absorb, xfe_absorb_device).
+//
+// The solver follows the reference CPU/GPU absorber's algorithm
+// (src/fe_absorption.cpp:5993-7088 packed Irons-Tuck path, mirrored by
+// src/fe_absorption_cuda.cu:2294-4613): Gauss-Seidel (or symmetric) sweeps of
+// FE projections, Irons-Tuck extrapolation with grand acceleration every 4
+// iterations, the divergence safeguard with CG hand-off, the honest
+// (reghdfe-comparable) stopping rule ||G(x)-x|| <= tol max(1,||x||), the
+// two-stage warm-up for 3+ FEs, and the tightening polish.  The arithmetic is
+// organised MI355X-first: every projection is one fused streaming pass over
+// the column block (absorb_kernels.hip), rows are permuted once so the largest
+// dimension is level-sorted, and each iteration does a single blocking copy
+// of 3 scalars.
+#include <hip/hip_runtime.h>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "absorb_kernels.h"
+#include "common.h"
+#include "xfe_hip.h"
+
+namespace xfe {
+
+namespace {
+
+constexpr int kMaxGrid = 2048;
+constexpr size_t kLdsBudget = 32 * 1024;  // bytes of LDS a privatised reduce may use
+
+int grid_for(long long work) {
+    long long g = (work + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > kMaxGrid) g = kMaxGrid;
+    return static_cast<int>(g);
+}
+
+struct Profile {
+    bool enabled = false;
+    double ms = 0.0;
+    long long launches = 0;
+    double bytes = 0.0;
+};
+Profile g_profile;
+
+// Per-thread device workspace (mirrors the reference's thread-local
+// persistent CUDA workspace, src/fe_absorption_cuda.cu:1536).
+struct Workspace {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    DevBuf<double> vec[7];  // x, gx, ggx, acc_a, acc_b, acc_c, spare
+    DevBuf<int> gid[XFE_MAX_DIMS];
+    DevBuf<double> sums[XFE_MAX_DIMS];
+    DevBuf<double> means[XFE_MAX_DIMS];
+    DevBuf<double> wsum[XFE_MAX_DIMS];
+    DevBuf<double> inv[XFE_MAX_DIMS];
+    DevBuf<int> perm, keys_tmp, vals_tmp, flag;
+    DevBuf<unsigned char> sort_tmp;
+    DevBuf<double> w;
+    DevBuf<double> partials;
+    DevBuf<double> scalars;
+    DevBuf<double> lanes;  // per-column scalars (CG alpha/beta)
+    DevBuf<double> stage;  // host-API staging of [y | X]
+    DevBuf<int> stage_ids;
+    DevBuf<double> stage_w;
+    double* host_scalars = nullptr;  // pinned
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // Frees every device resource of this thread's workspace.
+    void release() {
+        for (auto& b : vec) b.release();
+        for (int d = 0; d < XFE_MAX_DIMS; ++d) {
+            gid[d].release(); sums[d].release(); means[d].release();
+            wsum[d].release(); inv[d].release();
+        }
+        perm.release(); keys_tmp.release(); vals_tmp.release(); flag.release();
+        sort_tmp.release(); w.release(); partials.release(); scalars.release();
+        lanes.release(); stage.release(); stage_ids.release(); stage_w.release();
+        if (host_scalars) (void)hipHostFree(host_scalars);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+        host_scalars = nullptr; ev0 = ev1 = nullptr; stream = nullptr; device = -1;
+    }
+
+    void init() {
+        int dev = 0;
+        XFE_CHECK(hipGetDevice(&dev));
+        if (stream && dev == device) return;
+        device = dev;
+        XFE_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        XFE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_scalars), 4096 * sizeof(double),
+                                hipHostMallocDefault));
+        XFE_CHECK(hipEventCreate(&ev0));
+        XFE_CHECK(hipEventCreate(&ev1));
+    }
+};
+
+thread_local Workspace tls_ws;
+
+struct Solver {
+    Workspace& ws;
+    hipStream_t s;
+    int n = 0, C = 0, ld = 0, D = 0;
+    int G[XFE_MAX_DIMS] = {};
+    const int* gid[XFE_MAX_DIMS] = {};
+    const double* w = nullptr;
+    int grid = 1;
+    int nchunks = 1;
+    // partial-buffer layout: one region per concurrent reduction
+    size_t part_stride = 0;
+
+    Solver(Workspace& w_, int n_, int C_, int ld_) : ws(w_), s(w_.stream), n(n_), C(C_), ld(ld_) {
+        grid = grid_for(n);
+        nchunks = (C + 7) / 8;
+    }
+
+    bool lds_ok(int d, int cc) const {
+        return static_cast<size_t>(G[d]) * cc * sizeof(double) <= kLdsBudget;
+    }
+
+    // One fused pass over the column block (all column chunks).
+    void pass(const double* src, double* dst, const double* P, const double* Q, double coef,
+              double* xout, int a_dim, int r_dim, int nq, const double* it_x, const double* it_gx,
+              double* partials) {
+        for (int ch = 0; ch < nchunks; ++ch) {
+            const int c0 = ch * 8;
+            const int cc = std::min(8, C - c0);
+            PassArgs a;
+            a.n = n; a.ld = ld; a.C = C; a.c0 = c0;
+            a.src = src; a.dst = dst; a.P = P; a.Q = Q; a.coef = coef; a.xout = xout;
+            if (a_dim >= 0) { a.gid_a = gid[a_dim]; a.means_a = ws.means[a_dim].p; }
+            if (r_dim >= 0) {
+                a.gid_r = gid[r_dim]; a.sums_r = ws.sums[r_dim].p; a.G_r = G[r_dim];
+                a.w = w;
+            }
+            a.it_x = it_x; a.it_gx = it_gx;
+            a.partials = partials ? partials + static_cast<size_t>(ch) * grid * nq : nullptr;
+            const bool lds = r_dim >= 0 && lds_ok(r_dim, cc);
+            if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            XFE_CHECK(launch_pass(a, cc, lds, nq, grid, s));
+            if (g_profile.enabled) {
+                XFE_CHECK(hipEventRecord(ws.ev1, s));
+                XFE_CHECK(hipEventSynchronize(ws.ev1));
+                float ms = 0.f;
+                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+                g_profile.ms += ms;
+                g_profile.launches += 1;
+                // algorithmic bytes of this launch
+                double vec_rw = (P ? 2.0 : 1.0) + (xout ? 1.0 : 0.0) + (dst ? 1.0 : 0.0) +
+                                (it_x ? 2.0 : 0.0);
+                double bytes = vec_rw * 8.0 * static_cast<double>(n) * cc;
+                if (a_dim >= 0) bytes += 4.0 * n + 8.0 * static_cast<double>(G[a_dim]) * cc;
+                if (r_dim >= 0) bytes += 4.0 * n + 16.0 * static_cast<double>(G[r_dim]) * cc;
+                if (r_dim >= 0 && w) bytes += 8.0 * n;
+                g_profile.bytes += bytes;
+            }
+        }
+    }
+
+    void finalize(int d, double* partials) {
+        const long long total = static_cast<long long>(G[d]) * C;
+        XFE_CHECK(launch_finalize(ws.sums[d].p, ws.means[d].p, ws.inv[d].p, G[d], C, partials,
+                                  grid_for(total), s));
+    }
+    int finalize_grid(int d) const { return grid_for(static_cast<long long>(G[d]) * C); }
+
+    void clear_sums(int d) {
+        XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * static_cast<size_t>(G[d]) * C, s));
+    }
+
+    // Reduce `count` partial records of nq doubles on device and copy the nq
+    // results to host memory (blocking).
+    void fetch(const double* partials, int count, int nq, int mode, double* out) {
+        XFE_CHECK(launch_reduce_partials(partials, count, nq, mode, ws.scalars.p, s));
+        XFE_CHECK(hipMemcpyAsync(ws.host_scalars, ws.scalars.p, sizeof(double) * nq,
+                                 hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        for (int j = 0; j < nq; ++j) out[j] = ws.host_scalars[j];
+    }
+
+    // dst <- G(src) over the projection sequence `seq`.  If pre_reduced, the
+    // sums of seq[0] already hold src's reduce.  tail_reduce >= 0 starts the
+    // reduce of that dim on the output (fusion with the next sweep).
+    // nq_last: 1 -> sum of squares of the output, 3 -> IT stats vs (it_x, it_gx).
+    void sweep(const double* src, double* dst, const std::vector<int>& seq, bool pre_reduced,
+               int tail_reduce, int nq_last, const double* it_x, const double* it_gx,
+               double* out_q) {
+        const int m = static_cast<int>(seq.size());
+        if (m == 0) {
+            if (src != dst) XFE_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * ld * C,
+                                                     hipMemcpyDeviceToDevice, s));
+            return;
+        }
+        if (!pre_reduced) pass(src, nullptr, nullptr, nullptr, 0.0, nullptr, -1, seq[0], 0,
+                               nullptr, nullptr, nullptr);
+        for (int j = 0; j < m; ++j) {
+            finalize(seq[j], nullptr);
+            const bool last = j + 1 == m;
+            const int r = last ? tail_reduce : seq[j + 1];
+            const int nq = last ? nq_last : 0;
+            pass(j == 0 ? src : dst, dst, nullptr, nullptr, 0.0, nullptr, seq[j], r, nq,
+                 last ? it_x : nullptr, last ? it_gx : nullptr, nq ? ws.partials.p : nullptr);
+        }
+        if (nq_last && out_q) fetch(ws.partials.p, grid * nchunks, nq_last, 0, out_q);
+    }
+
+    double sumsq(const double* v, int reduce_dim) {
+        pass(v, nullptr, nullptr, nullptr, 0.0, nullptr, -1, reduce_dim, 1, nullptr, nullptr,
+             ws.partials.p);
+        double q = 0.0;
+        fetch(ws.partials.p, grid * nchunks, 1, 0, &q);
+        return q;
+    }
+
+    void copy(double* dst, const double* src) {
+        XFE_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * static_cast<size_t>(ld) * C,
+                                 hipMemcpyDeviceToDevice, s));
+    }
+};
+
+// Build the projection sequence of one sweep: forward over `order`, backward
+// when symmetric, with immediately repeated projections dropped (P_d is
+// idempotent, so d,d acts as d).
+std::vector<int> sweep_sequence(const std::vector<int>& order, bool symmetric) {
+    std::vector<int> seq;
+    for (int d : order) {
+        if (seq.empty() || seq.back() != d) seq.push_back(d);
+    }
+    if (symmetric) {
+        for (int p = static_cast<int>(order.size()); p-- > 0;) {
+            if (seq.empty() || seq.back() != order[p]) seq.push_back(order[p]);
+        }
+    }
+    return seq;
+}
+
+struct ProfileStats {
+    int num_groups = 0;
+    double mean = 0.0, cv = 0.0, singleton_ratio = 0.0, max_val = 0.0;
+};
+
+// profile_fe, src/fe_absorption.cpp:4945-4982.
+ProfileStats profile_of(const std::vector<double>& ws, bool unit) {
+    ProfileStats p;
+    const int G = static_cast<int>(ws.size());
+    p.num_groups = G;
+    if (G <= 0) return p;
+    double sum = 0.0, sq = 0.0, mx = -1e308;
+    int single = 0;
+    for (double v : ws) {
+        sum += v; sq += v * v; mx = std::max(mx, v);
+        if (v <= 1.0 + 1e-12) ++single;
+    }
+    p.mean = sum / G;
+    const double var = std::max(0.0, sq / G - p.mean * p.mean);
+    p.cv = p.mean > 0.0 ? std::sqrt(var) / p.mean : 0.0;
+    p.max_val = mx;
+    p.singleton_ratio = unit ? static_cast<double>(single) / G : 0.0;
+    return p;
+}
+
+// order_by_profile, src/fe_absorption.cpp:5472-5491.
+std::vector<int> order_by_profile(const std::vector<ProfileStats>& p) {
+    std::vector<int> order(p.size());
+    for (size_t i = 0; i < p.size(); ++i) order[i] = static_cast<int>(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (p[a].num_groups != p[b].num_groups) return p[a].num_groups > p[b].num_groups;
+        if (p[a].singleton_ratio != p[b].singleton_ratio) return p[a].singleton_ratio < p[b].singleton_ratio;
+        if (p[a].cv != p[b].cv) return p[a].cv < p[b].cv;
+        return a < b;
+    });
+    return order;
+}
+
+struct AccelState {
+    double *cur, *gx, *ggx, *acc_a, *acc_b, *acc_c;
+    bool diverged = false;
+};
+
+// packed_accel_phase, src/fe_absorption.cpp:6599-6782 (device form).
+// Leaves the iterate in st.cur.  Returns converged.
+bool accel_phase(Solver& sv, AccelState& st, const std::vector<int>& seq, int max_iter,
+                 bool honest, double honest_tol, double conv_tol, int& iter_used, int& sweeps) {
+    iter_used = 0;
+    if (max_iter <= 0 || seq.empty()) return false;
+    const double kDiverge = 4.0, kBail = 1.5;
+    const int cg_iter_cap = 1000, grand_every = 4;
+    const int first = seq[0];
+    int last_check = -1;
+    double prev_norm = std::sqrt(sv.sumsq(st.cur, first));  // also starts the reduce of cur
+    int grand = 0;
+    double best = prev_norm;
+    bool suspended = false;
+    // gx = G(cur); start the reduce of gx for the ggx sweep
+    sv.sweep(st.cur, st.gx, seq, true, first, 0, nullptr, nullptr, nullptr);
+    ++sweeps;
+    bool pending_gx = true;  // sums[first] holds gx's reduce
+    auto drop_pending = [&]() {
+        if (pending_gx) { sv.clear_sums(first); pending_gx = false; }
+    };
+    for (int iter = 0; iter < max_iter; ++iter) {
+        double q[3];
+        sv.sweep(st.gx, st.ggx, seq, pending_gx, -1, 3, st.cur, st.gx, q);
+        pending_gx = false;
+        ++sweeps;
+        const double vprod = q[0], ssq = q[1], d1sq = q[2];
+        if (honest && std::sqrt(d1sq) <= honest_tol * std::max(1.0, prev_norm)) {
+            std::swap(st.cur, st.ggx);
+            iter_used = iter + 1;
+            return true;
+        }
+        if (ssq == 0.0) {
+            sv.copy(st.cur, st.gx);
+            iter_used = iter + 1;
+            return true;
+        }
+        double coef = vprod / ssq;
+        if (suspended) coef = 0.0;
+        // cur = ggx - coef (ggx - gx); gx = G(cur) with its norm
+        sv.pass(nullptr, nullptr, st.ggx, st.gx, coef, st.cur, -1, first, 0, nullptr, nullptr,
+                nullptr);
+        double sq = 0.0;
+        sv.sweep(st.cur, st.gx, seq, true, first, 1, nullptr, nullptr, &sq);
+        pending_gx = true;
+        ++sweeps;
+        {
+            const double curr = std::sqrt(sq);
+            const double rel_change = std::fabs(curr - prev_norm) / std::max(1.0, prev_norm);
+            const int step = iter - last_check;
+            last_check = iter;
+            prev_norm = curr;
+            if (curr <= best) {
+                best = curr;
+                suspended = false;
+            } else {
+                if (curr > kBail * best) {
+                    st.diverged = true;
+                    drop_pending();
+                    std::swap(st.cur, st.gx);
+                    iter_used = iter + 1;
+                    return false;
+                }
+                if (curr > kDiverge * best) {
+                    if (!suspended) grand = 0;
+                    suspended = true;
+                }
+            }
+            if (!honest && step > 0 && rel_change / static_cast<double>(step) < conv_tol) {
+                drop_pending();
+                std::swap(st.cur, st.gx);
+                iter_used = iter + 1;
+                return true;
+            }
+        }
+        if (honest && cg_iter_cap > 0 && !st.diverged && iter + 1 >= cg_iter_cap) {
+            st.diverged = true;
+            drop_pending();
+            std::swap(st.cur, st.gx);
+            iter_used = iter + 1;
+            return false;
+        }
+        if (!suspended && grand_every > 0 && (iter + 1) % grand_every == 0) {
+            ++grand;
+            if (grand == 1) {
+                sv.copy(st.acc_a, st.gx);
+            } else if (grand == 2) {
+                sv.copy(st.acc_b, st.gx);
+            } else {
+                sv.copy(st.acc_c, st.gx);
+                XFE_CHECK(launch_it_stats(st.acc_a, st.acc_b, st.acc_c, sv.n, sv.C, sv.ld,
+                                          sv.ws.partials.p, sv.grid, sv.s));
+                double g3[3];
+                sv.fetch(sv.ws.partials.p, sv.grid, 3, 0, g3);
+                drop_pending();
+                if (g3[1] != 0.0) {
+                    const double ac = g3[0] / g3[1];
+                    // cur = acc_a' = acc_c - ac (acc_c - acc_b), and start its reduce
+                    sv.pass(nullptr, nullptr, st.acc_c, st.acc_b, ac, st.cur, -1, first, 0,
+                            nullptr, nullptr, nullptr);
+                } else {
+                    sv.pass(st.acc_a, st.cur, nullptr, nullptr, 0.0, nullptr, -1, first, 0,
+                            nullptr, nullptr, nullptr);
+                }
+                sv.sweep(st.cur, st.gx, seq, true, first, 0, nullptr, nullptr, nullptr);
+                pending_gx = true;
+                ++sweeps;
+                grand = 0;
+            }
+        }
+    }
+    drop_pending();
+    std::swap(st.cur, st.gx);
+    iter_used = max_iter;
+    return false;
+}
+
+// run_packed_cg, src/fe_absorption.cpp:6790-6973 (device form): per-column
+// CG on the symmetric operator (I - G_sym).  Works in place on `cur`.
+bool run_cg(Solver& sv, double* cur, double* r, double* u, double* v, const std::vector<int>& order,
+            int max_iter, double tol, int& iters_out, int& sweeps) {
+    const int C = sv.C;
+    const std::vector<int> seq = sweep_sequence(order, true);
+    std::vector<double> improvement(C), ssr(C), ssr_old(C), alpha(C), beta(C), recent(C, 0.0),
+        denom(C), next_imp(C);
+    std::vector<unsigned char> solved(C, 0);
+    auto coldot = [&](const double* a, const double* b, std::vector<double>& out) {
+        XFE_CHECK(launch_col_dot(a, b, sv.n, C, sv.ld, sv.ws.partials.p, sv.grid, sv.s));
+        sv.fetch(sv.ws.partials.p, sv.grid, C, 0, out.data());
+    };
+    auto proj = [&](const double* src, double* dst) {
+        sv.sweep(src, dst, seq, false, -1, 0, nullptr, nullptr, nullptr);
+        ++sweeps;
+        XFE_CHECK(launch_sub(dst, src, dst, sv.n, C, sv.ld, sv.grid, sv.s));
+    };
+    auto upload_lanes = [&](const std::vector<double>& x) {
+        XFE_CHECK(hipMemcpyAsync(sv.ws.lanes.p, x.data(), sizeof(double) * C,
+                                 hipMemcpyHostToDevice, sv.s));
+    };
+    coldot(cur, cur, improvement);
+    proj(cur, r);
+    coldot(r, r, ssr);
+    sv.copy(u, r);
+    bool converged = false;
+    int iters = 0;
+    for (int iter = 1; iter <= max_iter; ++iter) {
+        proj(u, v);
+        coldot(u, v, denom);
+        bool bd = false;
+        for (int k = 0; k < C; ++k) { alpha[k] = 0.0; next_imp[k] = improvement[k]; }
+        for (int k = 0; k < C && !bd; ++k) {
+            if (!std::isfinite(improvement[k]) || !std::isfinite(ssr[k]) || ssr[k] < 0.0) { bd = true; break; }
+            if (solved[k]) { if (ssr[k] != 0.0) bd = true; recent[k] = 0.0; continue; }
+            if (ssr[k] == 0.0) { solved[k] = 1; recent[k] = 0.0; continue; }
+            if (!std::isfinite(denom[k]) || denom[k] <= 0.0) { bd = true; break; }
+            alpha[k] = ssr[k] / denom[k];
+            recent[k] = alpha[k] * ssr[k];
+            next_imp[k] = improvement[k] - recent[k];
+            if (!std::isfinite(alpha[k]) || alpha[k] <= 0.0 || !std::isfinite(recent[k]) ||
+                recent[k] <= 0.0 || !std::isfinite(next_imp[k])) { bd = true; break; }
+        }
+        iters = iter;
+        if (bd) break;
+        improvement = next_imp;
+        upload_lanes(alpha);
+        XFE_CHECK(launch_col_axpy(cur, cur, u, sv.ws.lanes.p, sv.n, C, sv.ld, -1.0, sv.grid, sv.s));
+        XFE_CHECK(launch_col_axpy(r, r, v, sv.ws.lanes.p, sv.n, C, sv.ld, -1.0, sv.grid, sv.s));
+        ssr_old = ssr;
+        coldot(r, r, ssr);
+        for (int k = 0; k < C && !bd; ++k) {
+            beta[k] = 0.0;
+            if (!std::isfinite(ssr[k]) || ssr[k] < 0.0 || !std::isfinite(ssr_old[k]) || ssr_old[k] < 0.0) { bd = true; break; }
+            if (solved[k]) { if (ssr[k] != 0.0) bd = true; continue; }
+            if (ssr[k] == 0.0) { solved[k] = 1; continue; }
+            if (ssr_old[k] <= 0.0) { bd = true; break; }
+            beta[k] = ssr[k] / ssr_old[k];
+            if (!std::isfinite(beta[k]) || beta[k] <= 0.0) { bd = true; break; }
+        }
+        if (bd) break;
+        upload_lanes(beta);
+        XFE_CHECK(launch_col_axpy(u, r, u, sv.ws.lanes.p, sv.n, C, sv.ld, 1.0, sv.grid, sv.s));
+        double update_error = 0.0;
+        for (int k = 0; k < C; ++k) {
+            if (solved[k]) continue;
+            const double den = std::fabs(improvement[k]);
+            if (!(recent[k] > 0.0) || !(den > 0.0)) { bd = true; break; }
+            const double e = std::sqrt(recent[k] / den);
+            if (!std::isfinite(e)) { bd = true; break; }
+            update_error = std::max(update_error, e);
+        }
+        if (bd) break;
+        if (update_error <= tol) { converged = true; break; }
+    }
+    iters_out = iters > 0 ? iters : max_iter;
+    return converged;
+}
+
+}  // namespace
+
+// Core solve on device buffers already in the workspace layout.  V: n x C,
+// ld; gids[d] already permuted; ws.wsum/inv filled.  Returns 0 or throws.
+static void solve_core(Workspace& ws, double* V, int n, int C, int ld, int D,
+                       const std::vector<int>& G, const std::vector<int>& order_in,
+                       const std::vector<ProfileStats>& profs, const double* w,
+                       const xfe_options& opt, xfe_absorb_result& res) {
+    Solver sv(ws, n, C, ld);
+    sv.D = D;
+    sv.w = w;
+    for (int d = 0; d < D; ++d) {
+        sv.G[d] = G[d];
+        sv.gid[d] = ws.gid[d].p;
+    }
+    const bool honest = opt.tolerance_mode == 1;
+    const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
+    std::vector<int> order = order_in;
+    for (int d = 0; d < D; ++d) res.sweep_order_used[d] = order[d];
+    const bool symmetric = opt.method == 2 && D > 1;
+    const std::vector<int> seq = sweep_sequence(order, symmetric);
+
+    // gates: src/fe_absorption_cuda.cu:2346-2390 (mirror of the CPU gates)
+    double min_occ = -1.0;
+    for (int d = 0; d < D; ++d)
+        if (profs[d].num_groups > 0 && (min_occ < 0.0 || profs[d].mean < min_occ)) min_occ = profs[d].mean;
+    const bool ill = D >= 2 && min_occ >= 0.0 && min_occ <= 12.0;
+    int total_levels = 0;
+    for (int d = 0; d < D; ++d) total_levels += G[d];
+    bool use_cg = false;
+    if (w == nullptr) {
+        if (opt.force_cg && D >= 2) use_cg = true;
+        if (opt.tolerance_mode == 1 && D >= 3) {
+            double share = 0.0;
+            for (int d = 0; d < D; ++d) share = std::max(share, profs[d].max_val / static_cast<double>(n));
+            if (share >= 0.37) use_cg = true;
+        }
+        if (D == 2 && n <= 20000 && total_levels >= n - 2 && total_levels <= n + 2) use_cg = true;
+    }
+    const bool use_accel = use_cg || (D >= 2 && (n >= 200000 || ill));
+    const int budget = (use_accel && n < 200000) ? std::max(1, (opt.max_iter * 2) / 5) : opt.max_iter;
+
+    // two-stage warm-up (src/fe_absorption.cpp:6293-6331)
+    int iter_warmup = 0, iter_two = 0;
+    std::vector<int> two_order;
+    if (D >= 3 && n >= 200000) {
+        int first = -1, second = -1, third = -1;
+        for (int d = 0; d < D; ++d) {
+            const int g = profs[d].num_groups;
+            if (g > first) { third = second; second = first; first = g; }
+            else if (g > second) { third = second; second = g; }
+            else if (g > third) { third = g; }
+        }
+        if (first > 0 && third > 0 && static_cast<double>(third) <= 0.05 * first) {
+            double msing = 0.0, mcv = 0.0;
+            for (int d = 0; d < D; ++d) { msing = std::max(msing, profs[d].singleton_ratio); mcv = std::max(mcv, profs[d].cv); }
+            int diff;
+            if (D >= 4 && (msing >= 0.30 || mcv >= 2.0)) diff = 3;
+            else if (D >= 4 || msing >= 0.20 || mcv >= 1.5) diff = 2;
+            else if (D >= 3 || msing >= 0.10 || mcv >= 1.0) diff = 1;
+            else diff = 0;
+            if (diff == 3) { iter_warmup = 5; iter_two = 5; }
+            else if (diff >= 2) { iter_warmup = 8; iter_two = 8; }
+            else { iter_warmup = 5; iter_two = 8; }
+            int best = -1, sec = -1, bg = -1, sg = -1;
+            for (int d = 0; d < D; ++d) {
+                const int g = profs[d].num_groups;
+                if (g > bg) { sec = best; sg = bg; best = d; bg = g; }
+                else if (g > sg) { sec = d; sg = g; }
+            }
+            for (int d : order) if (d == best || d == sec) two_order.push_back(d);
+            if (two_order.size() != 2) { iter_warmup = 0; iter_two = 0; two_order.clear(); }
+        }
+    }
+
+    const size_t vec_bytes = sizeof(double) * static_cast<size_t>(ld) * C;
+    for (int j = 1; j < 7; ++j) ws.vec[j].ensure(static_cast<size_t>(ld) * C);
+    bool converged = false;
+    int sweeps = 0;
+    double* cur = V;
+    if (use_accel) {
+        res.used_accel = 1;
+        AccelState st{V, ws.vec[1].p, ws.vec[2].p, ws.vec[3].p, ws.vec[4].p, ws.vec[5].p};
+        if (use_cg) {
+            res.used_cg = 1;
+            converged = run_cg(sv, V, ws.vec[1].p, ws.vec[2].p, ws.vec[3].p, order, opt.max_iter,
+                               opt.tol, res.iterations, sweeps);
+        } else {
+            int remaining = budget, total = 0, used = 0;
+            const std::vector<int> seq2 = sweep_sequence(two_order, symmetric);
+            if (iter_warmup > 0 && remaining > 0) {
+                converged = accel_phase(sv, st, seq, std::min(iter_warmup, remaining), honest,
+                                        opt.tol, tol_eff, used, sweeps);
+                total += used; remaining -= used;
+            }
+            if (!converged && !st.diverged && !two_order.empty() && iter_two > 0 && remaining > 0) {
+                accel_phase(sv, st, seq2, std::min(iter_two, remaining), honest, opt.tol, tol_eff,
+                            used, sweeps);
+                total += used; remaining -= used;
+            }
+            if (!converged && !st.diverged && remaining > 0) {
+                converged = accel_phase(sv, st, seq, remaining, honest, opt.tol, tol_eff, used,
+                                        sweeps);
+                total += used; remaining -= used;
+            }
+            cur = st.cur;
+            if (!converged && st.diverged) {
+                res.used_cg = 1;
+                // CG needs three work vectors distinct from cur
+                std::vector<double*> spare;
+                for (int j = 0; j < 7 && spare.size() < 3; ++j) {
+                    double* p = j == 0 ? V : ws.vec[j].p;
+                    if (p != cur) spare.push_back(p);
+                }
+                converged = run_cg(sv, cur, spare[0], spare[1], spare[2], order, opt.max_iter,
+                                   opt.tol, res.iterations, sweeps);
+            } else if (converged) {
+                res.iterations = total;
+            }
+        }
+    } else {
+        // plain MAP, src/fe_absorption.cpp:7368-7412
+        double prev = std::sqrt(sv.sumsq(V, -1));
+        int last_check = -1;
+        for (int iter = 0; iter < opt.max_iter; ++iter) {
+            double sq = 0.0;
+            sv.sweep(V, V, seq, false, -1, 1, nullptr, nullptr, &sq);
+            ++sweeps;
+            const double curr = std::sqrt(sq);
+            const double rel = std::fabs(curr - prev) / std::max(1.0, prev);
+            const int step = iter - last_check;
+            last_check = iter;
+            prev = curr;
+            if (step > 0 && rel / static_cast<double>(step) < tol_eff) {
+                res.iterations = iter + 1;
+                converged = true;
+                break;
+            }
+        }
+    }
+    if (!converged) {
+        res.iterations = opt.max_iter;
+        res.converged = 0;
+    } else {
+        res.converged = 1;
+        if (res.iterations == 0) res.iterations = 1;
+        // tightening polish, src/fe_absorption.cpp:7016-7068
+        const bool strict = opt.tolerance_mode == 2;
+        const int max_polish = strict ? std::max(0, opt.max_iter - res.iterations) : 6;
+        const double polish_tol = strict ? opt.tol : (tol_eff > 0.0 ? tol_eff * 0.01 : tol_eff);
+        double final_max = 0.0;
+        int done = 0;
+        for (int p = 0; p < max_polish; ++p) {
+            // one sweep in place; every finalize writes its block maxima
+            const int m = static_cast<int>(seq.size());
+            size_t off = 0;
+            sv.pass(cur, nullptr, nullptr, nullptr, 0.0, nullptr, -1, seq[0], 0, nullptr, nullptr,
+                    nullptr);
+            for (int j = 0; j < m; ++j) {
+                sv.finalize(seq[j], ws.partials.p + off);
+                off += sv.finalize_grid(seq[j]);
+                const int r = j + 1 < m ? seq[j + 1] : -1;
+                sv.pass(cur, cur, nullptr, nullptr, 0.0, nullptr, seq[j], r, 0, nullptr, nullptr,
+                        nullptr);
+            }
+            ++sweeps;
+            double mx = 0.0;
+            sv.fetch(ws.partials.p, static_cast<int>(off), 1, 1, &mx);
+            final_max = mx;
+            ++done;
+            if (mx <= polish_tol) break;
+        }
+        res.polish_sweeps = done;
+        res.polish_final_max = final_max;
+    }
+    if (cur != V) XFE_CHECK(hipMemcpyAsync(V, cur, vec_bytes, hipMemcpyDeviceToDevice, ws.stream));
+    res.sweeps = sweeps;
+}
+
+// Shared driver for both entry points.  d_Vin: caller's n x C (ld_in) block,
+// d_gid_in[d]: caller's ids; the result is written back to d_Vin.
+static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
+                               const std::vector<const int*>& d_gid_in, const std::vector<int>& G,
+                               const std::vector<const double*>& h_wsum, const double* d_w_in,
+                               const int* sweep_order, const xfe_options& opt,
+                               xfe_absorb_result& res) {
+    Workspace& ws = tls_ws;
+    ws.init();
+    hipStream_t s = ws.stream;
+    const int D = static_cast<int>(G.size());
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int d = 0; d < D; ++d) res.fe_levels[d] = G[d];
+    const int ld = ((n + 63) / 64) * 64;
+    const int grid = grid_for(n);
+    // primary (sorted) dimension: the one with the most levels
+    int p = 0;
+    for (int d = 1; d < D; ++d) if (G[d] > G[p]) p = d;
+    ws.flag.ensure(1);
+    XFE_CHECK(hipMemsetAsync(ws.flag.p, 0, sizeof(int), s));
+    XFE_CHECK(launch_sorted_check(d_gid_in[p], n, ws.flag.p, grid, s));
+    int unsorted = 0;
+    XFE_CHECK(hipMemcpyAsync(&unsorted, ws.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    XFE_CHECK(hipStreamSynchronize(s));
+    const int* perm = nullptr;
+    if (unsorted) {
+        ws.perm.ensure(n);
+        ws.keys_tmp.ensure(n);
+        ws.vals_tmp.ensure(n);
+        XFE_CHECK(launch_iota(ws.vals_tmp.p, n, grid, s));
+        int bits = 1;
+        while ((1LL << bits) < static_cast<long long>(G[p])) ++bits;
+        size_t tmp_bytes = 0;
+        XFE_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, d_gid_in[p], ws.keys_tmp.p,
+                                            ws.vals_tmp.p, ws.perm.p, static_cast<size_t>(n), 0,
+                                            bits, s));
+        ws.sort_tmp.ensure(tmp_bytes);
+        XFE_CHECK(rocprim::radix_sort_pairs(ws.sort_tmp.p, tmp_bytes, d_gid_in[p], ws.keys_tmp.p,
+                                            ws.vals_tmp.p, ws.perm.p, static_cast<size_t>(n), 0,
+                                            bits, s));
+        perm = ws.perm.p;
+    }
+    // permuted working copies
+    ws.vec[0].ensure(static_cast<size_t>(ld) * C);
+    double* V = ws.vec[0].p;
+    XFE_CHECK(launch_gather_cols(d_Vin, ld_in, perm, V, ld, n, C, grid, s));
+    const double* w = nullptr;
+    if (d_w_in) {
+        ws.w.ensure(n);
+        XFE_CHECK(launch_gather_vec(d_w_in, perm, ws.w.p, n, grid, s));
+        w = ws.w.p;
+    }
+    std::vector<ProfileStats> profs(D);
+    for (int d = 0; d < D; ++d) {
+        ws.gid[d].ensure(n);
+        XFE_CHECK(launch_gather_ids(d_gid_in[d], perm, ws.gid[d].p, n, grid, s));
+        const size_t Gd = static_cast<size_t>(std::max(G[d], 1));
+        ws.wsum[d].ensure(Gd);
+        ws.inv[d].ensure(Gd);
+        ws.sums[d].ensure(Gd * C);
+        ws.means[d].ensure(Gd * C);
+        std::vector<double> hw(G[d]);
+        if (h_wsum[d]) {
+            std::memcpy(hw.data(), h_wsum[d], sizeof(double) * G[d]);
+            XFE_CHECK(hipMemcpyAsync(ws.wsum[d].p, h_wsum[d], sizeof(double) * G[d],
+                                     hipMemcpyHostToDevice, s));
+        } else {
+            XFE_CHECK(hipMemsetAsync(ws.wsum[d].p, 0, sizeof(double) * Gd, s));
+            XFE_CHECK(launch_weight_sums(ws.gid[d].p, w, n, ws.wsum[d].p, grid, s));
+            XFE_CHECK(hipMemcpyAsync(hw.data(), ws.wsum[d].p, sizeof(double) * G[d],
+                                     hipMemcpyDeviceToHost, s));
+        }
+        XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G[d], grid_for(G[d]), s));
+        XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * Gd * C, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        profs[d] = profile_of(hw, w == nullptr);
+    }
+    // partial buffer: enough for chunked passes and polish maxima over a sweep
+    size_t part = static_cast<size_t>(grid) * ((C + 7) / 8) * 3 + static_cast<size_t>(grid) * C;
+    for (int d = 0; d < D; ++d) part += 2 * static_cast<size_t>(grid_for(static_cast<long long>(G[d]) * C));
+    ws.partials.ensure(part + 1024);
+    ws.scalars.ensure(std::max(C, 8));
+    ws.lanes.ensure(std::max(C, 8));
+
+    std::vector<int> order(D);
+    if (sweep_order) {
+        for (int d = 0; d < D; ++d) order[d] = sweep_order[d];
+    } else if (D > 2) {
+        order = order_by_profile(profs);
+    } else {
+        for (int d = 0; d < D; ++d) order[d] = d;
+    }
+    solve_core(ws, V, n, C, ld, D, G, order, profs, w, opt, res);
+    XFE_CHECK(launch_scatter_cols(V, ld, perm, d_Vin, ld_in, n, C, grid, s));
+    XFE_CHECK(hipStreamSynchronize(s));
+    const auto t1 = std::chrono::steady_clock::now();
+    res.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+}  // namespace xfe
+
+using namespace xfe;
+
+extern "C" {
+
+int xfe_backend_available(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return count > 0 ? 1 : 0;
+}
+
+int xfe_absorb_device(double* d_V, int n, int C, int ld, const int* d_group_ids,
+                      const int* num_groups, int num_fes, const double* d_weights,
+                      const int* sweep_order, const xfe_options* opt, xfe_absorb_result* res) {
+    clear_error();
+    if (!res || !opt || !d_V || n < 0 || C <= 0 || C > XFE_MAX_COLS || ld < n || num_fes < 0 ||
+        num_fes > XFE_MAX_DIMS) {
+        set_error("xfe_absorb_device: invalid arguments");
+        return 0;
+    }
+    std::memset(res, 0, sizeof(*res));
+    if (num_fes == 0 || n == 0) {
+        res->converged = 1;
+        return 1;
+    }
+    try {
+        (void)hipGetLastError();  // clear sticky errors (src/fe_absorption_cuda.cu:2484-2491)
+        std::vector<const int*> gids(num_fes);
+        std::vector<int> G(num_fes);
+        std::vector<const double*> wsum(num_fes, nullptr);
+        for (int d = 0; d < num_fes; ++d) {
+            gids[d] = d_group_ids + static_cast<size_t>(d) * n;
+            G[d] = num_groups[d];
+            if (G[d] <= 0) throw Error("num_groups must be positive");
+        }
+        absorb_device_impl(d_V, n, C, ld, gids, G, wsum, d_weights, sweep_order, *opt, *res);
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+int xfe_absorb(const double* y, const double* X, int n, int k, const xfe_fe_input* fes,
+               int num_fes, const double* weights, const int* sweep_order, const xfe_options* opt,
+               xfe_absorb_result* res, double* y_out, double* X_out) {
+    clear_error();
+    if (!res || !opt || !y || (k > 0 && !X) || n < 0 || k < 0 || k + 1 > XFE_MAX_COLS ||
+        num_fes < 0 || num_fes > XFE_MAX_DIMS || !y_out || (k > 0 && !X_out)) {
+        set_error("xfe_absorb: invalid arguments");
+        return 0;
+    }
+    std::memset(res, 0, sizeof(*res));
+    for (int d = 0; d < num_fes; ++d) {
+        if (fes[d].is_slope) {
+            set_error("xfe_absorb: heterogeneous slopes are not supported by this backend");
+            return 0;
+        }
+        if (!fes[d].group_ids || fes[d].num_groups <= 0) {
+            set_error("xfe_absorb: invalid FE input");
+            return 0;
+        }
+    }
+    if (num_fes == 0 || n == 0) {
+        std::memcpy(y_out, y, sizeof(double) * n);
+        if (k > 0) std::memcpy(X_out, X, sizeof(double) * static_cast<size_t>(n) * k);
+        res->converged = 1;
+        return 1;
+    }
+    try {
+        (void)hipGetLastError();
+        Workspace& ws = tls_ws;
+        ws.init();
+        hipStream_t s = ws.stream;
+        const int C = k + 1;
+        ws.stage.ensure(static_cast<size_t>(n) * C);
+        XFE_CHECK(hipMemcpyAsync(ws.stage.p, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        if (k > 0)
+            XFE_CHECK(hipMemcpyAsync(ws.stage.p + n, X, sizeof(double) * static_cast<size_t>(n) * k,
+                                     hipMemcpyHostToDevice, s));
+        ws.stage_ids.ensure(static_cast<size_t>(n) * num_fes);
+        std::vector<const int*> gids(num_fes);
+        std::vector<int> G(num_fes);
+        std::vector<const double*> wsum(num_fes);
+        for (int d = 0; d < num_fes; ++d) {
+            int* dst = ws.stage_ids.p + static_cast<size_t>(d) * n;
+            XFE_CHECK(hipMemcpyAsync(dst, fes[d].group_ids, sizeof(int) * n, hipMemcpyHostToDevice, s));
+            gids[d] = dst;
+            G[d] = fes[d].num_groups;
+            wsum[d] = fes[d].weight_sums;
+        }
+        if (weights) {
+            ws.stage_w.ensure(n);
+            XFE_CHECK(hipMemcpyAsync(ws.stage_w.p, weights, sizeof(double) * n,
+                                     hipMemcpyHostToDevice, s));
+        }
+        absorb_device_impl(ws.stage.p, n, C, n, gids, G, wsum, weights ? ws.stage_w.p : nullptr,
+                           sweep_order, *opt, *res);
+        XFE_CHECK(hipMemcpyAsync(y_out, ws.stage.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        if (k > 0)
+            XFE_CHECK(hipMemcpyAsync(X_out, ws.stage.p + n, sizeof(double) * static_cast<size_t>(n) * k,
+                                     hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+int xfe_recovery_max_delta(const double* residual, int n, const xfe_fe_input* fes, int num_fes,
+                           const double* weights, const int* sweep_order, double* out_max_delta) {
+    clear_error();
+    if (!residual || !out_max_delta || n < 0 || num_fes < 0 || num_fes > XFE_MAX_DIMS) {
+        set_error("xfe_recovery_max_delta: invalid arguments");
+        return 0;
+    }
+    *out_max_delta = 0.0;
+    if (n == 0 || num_fes == 0) return 1;
+    try {
+        (void)hipGetLastError();
+        Workspace& ws = tls_ws;
+        ws.init();
+        hipStream_t s = ws.stream;
+        const int ld = ((n + 63) / 64) * 64;
+        ws.vec[6].ensure(ld);
+        XFE_CHECK(hipMemcpyAsync(ws.vec[6].p, residual, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        const double* w = nullptr;
+        if (weights) {
+            ws.stage_w.ensure(n);
+            XFE_CHECK(hipMemcpyAsync(ws.stage_w.p, weights, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            w = ws.stage_w.p;
+        }
+        Solver sv(ws, n, 1, ld);
+        sv.D = num_fes;
+        sv.w = w;
+        ws.stage_ids.ensure(static_cast<size_t>(n) * num_fes);
+        size_t part = 0;
+        for (int d = 0; d < num_fes; ++d) {
+            if (!fes[d].group_ids || fes[d].num_groups <= 0) throw Error("invalid FE input");
+            const int G = fes[d].num_groups;
+            int* dst = ws.stage_ids.p + static_cast<size_t>(d) * n;
+            XFE_CHECK(hipMemcpyAsync(dst, fes[d].group_ids, sizeof(int) * n, hipMemcpyHostToDevice, s));
+            sv.gid[d] = dst;
+            sv.G[d] = G;
+            ws.wsum[d].ensure(G); ws.inv[d].ensure(G); ws.sums[d].ensure(G); ws.means[d].ensure(G);
+            if (fes[d].weight_sums) {
+                XFE_CHECK(hipMemcpyAsync(ws.wsum[d].p, fes[d].weight_sums, sizeof(double) * G,
+                                         hipMemcpyHostToDevice, s));
+            } else {
+                XFE_CHECK(hipMemsetAsync(ws.wsum[d].p, 0, sizeof(double) * G, s));
+                XFE_CHECK(launch_weight_sums(dst, w, n, ws.wsum[d].p, grid_for(n), s));
+            }
+            XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G, grid_for(G), s));
+            XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * G, s));
+            part += grid_for(G);
+        }
+        ws.partials.ensure(part + static_cast<size_t>(grid_for(n)) * 3 + 1024);
+        ws.scalars.ensure(8);
+        std::vector<int> order(num_fes);
+        for (int d = 0; d < num_fes; ++d) order[d] = sweep_order ? sweep_order[d] : d;
+        // one Gauss-Seidel pass: reduce d, means (+ max |mean|), subtract
+        size_t off = 0;
+        sv.pass(ws.vec[6].p, nullptr, nullptr, nullptr, 0.0, nullptr, -1, order[0], 0, nullptr,
+                nullptr, nullptr);
+        for (int j = 0; j < num_fes; ++j) {
+            const int d = order[j];
+            sv.finalize(d, ws.partials.p + off);
+            off += sv.finalize_grid(d);
+            const int r = j + 1 < num_fes ? order[j + 1] : -1;
+            sv.pass(ws.vec[6].p, ws.vec[6].p, nullptr, nullptr, 0.0, nullptr, d, r, 0, nullptr,
+                    nullptr, nullptr);
+        }
+        double mx = 0.0;
+        sv.fetch(ws.partials.p, static_cast<int>(off), 1, 1, &mx);
+        *out_max_delta = mx;
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+void xfe_release_workspace(void) { tls_ws.release(); }
+
+void xfe_profile_enable(int on) { g_profile.enabled = on != 0; }
+void xfe_profile_reset(void) {
+    g_profile.ms = 0.0;
+    g_profile.launches = 0;
+    g_profile.bytes = 0.0;
+}
+void xfe_profile_get(double* pass_ms, long long* pass_launches, double* pass_bytes) {
+    if (pass_ms) *pass_ms = g_profile.ms;
+    if (pass_launches) *pass_launches = g_profile.launches;
+    if (pass_bytes) *pass_bytes = g_profile.bytes;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// Launch interface of the absorber kernels (absorb_kernels.hip).
+#ifndef XFE_ABSORB_KERNELS_H
+#define XFE_ABSORB_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+namespace xfe {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+// Arguments of the fused streaming pass (see pass_kernel).
+struct PassArgs {
+    int n = 0;
+    int ld = 0;
+    int C = 0;   // total columns (stride of the level-major tables)
+    int c0 = 0;  // first column handled by this launch
+    const double* src = nullptr;
+    double* dst = nullptr;
+    const double* P = nullptr;  // optional IT extrapolation v = P - coef (P - Q)
+    const double* Q = nullptr;
+    double coef = 0.0;
+    double* xout = nullptr;
+    const int* gid_a = nullptr;  // projection being finished
+    const double* means_a = nullptr;
+    const int* gid_r = nullptr;  // projection being started
+    double* sums_r = nullptr;
+    int G_r = 0;
+    const double* w = nullptr;
+    const double* it_x = nullptr;
+    const double* it_gx = nullptr;
+    double* partials = nullptr;
+};
+
+hipError_t launch_pass(const PassArgs& a, int cc, bool lds, int nq, int grid, hipStream_t s);
+hipError_t launch_finalize(double* sums, double* means, const double* inv, int G, int C,
+                           double* partials, int grid, hipStream_t s);
+hipError_t launch_reduce_partials(const double* partials, int count, int nq, int mode, double* out,
+                                  hipStream_t s);
+hipError_t launch_it_stats(const double* x, const double* gx, const double* ggx, int n, int C,
+                           int ld, double* partials, int grid, hipStream_t s);
+hipError_t launch_col_dot(const double* a, const double* b, int n, int C, int ld, double* partials,
+                          int grid, hipStream_t s);
+hipError_t launch_col_axpy(double* y, const double* a, const double* b, const double* alpha, int n,
+                           int C, int ld, double sign, int grid, hipStream_t s);
+hipError_t launch_sub(double* y, const double* a, const double* b, int n, int C, int ld, int grid,
+                      hipStream_t s);
+hipError_t launch_weight_sums(const int* gid, const double* w, int n, double* ws, int grid,
+                              hipStream_t s);
+hipError_t launch_inverse(const double* ws, double* inv, int G, int grid, hipStream_t s);
+hipError_t launch_iota(int* v, int n, int grid, hipStream_t s);
+hipError_t launch_gather_ids(const int* src, const int* perm, int* dst, int n, int grid,
+                             hipStream_t s);
+hipError_t launch_gather_cols(const double* src, int ld_src, const int* perm, double* dst,
+                              int ld_dst, int n, int C, int grid, hipStream_t s);
+hipError_t launch_scatter_cols(const double* src, int ld_src, const int* perm, double* dst,
+                               int ld_dst, int n, int C, int grid, hipStream_t s);
+hipError_t launch_gather_vec(const double* src, const int* perm, double* dst, int n, int grid,
+                             hipStream_t s);
+hipError_t launch_sorted_check(const int* gid, int n, int* flag, int grid, hipStream_t s);
+
+}  // namespace xfe
+
+#endif

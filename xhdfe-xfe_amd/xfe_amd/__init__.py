@@ -1,0 +1,216 @@
+"""Python binding (ctypes) of the MI355X absorber library ``libxfe_hip.so``.
+
+The library is the product: HIP kernels for gfx950 behind the C ABI declared
+in ``include/xfe_hip.h``.  This module only marshals arguments; it never
+computes a result on the CPU.  Loading fails loudly when the library is not
+built (``make -C xhdfe-xfe_amd`` or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libxfe_hip.so")
+MAX_DIMS = 8
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_p = ctypes.c_void_p
+
+
+class FeInput(ctypes.Structure):
+    """xfe_fe_input (mirrors hdfe::detail::GpuFeInput, src/fe_absorption_cuda.hpp:12-21)."""
+    _fields_ = [("group_ids", _p), ("num_groups", _i), ("num_levels_present", _i),
+                ("weight_sums", _p), ("is_slope", _i), ("slope_has_intercept", _i),
+                ("slope_values", _p), ("slope_sum_z", _p), ("slope_sum_zz", _p)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("tol", _d), ("tolerance_mode", _i), ("max_iter", _i), ("method", _i),
+                ("force_cg", _i), ("precision", _i)]
+
+
+class AbsorbResult(ctypes.Structure):
+    _fields_ = [("iterations", _i), ("converged", _i), ("used_accel", _i), ("used_cg", _i),
+                ("polish_sweeps", _i), ("polish_final_max", _d), ("sweep_order_used", _i * MAX_DIMS),
+                ("fe_levels", _i * MAX_DIMS), ("sweeps", _i), ("refinements", _i), ("solve_ms", _d)]
+
+
+# Every entry point of include/xfe_hip.h (the CPU test checks that each is exported).
+EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop_singletons",
+           "xfe_bipartite_components", "xfe_fe_dof", "xfe_gram", "xfe_cluster_meat",
+           "xfe_recovery_max_delta", "xfe_last_error", "xfe_profile_enable", "xfe_profile_reset",
+           "xfe_profile_get", "xfe_release_workspace")
+
+_lib = None
+
+
+class XfeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libxfe_hip.so; raises if it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise XfeError(f"{LIB_PATH} is not built; run make -C xhdfe-xfe_amd")
+        L = ctypes.CDLL(LIB_PATH)
+        L.xfe_last_error.restype = ctypes.c_char_p
+        L.xfe_profile_get.argtypes = [ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_longlong),
+                                      ctypes.POINTER(_d)]
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().xfe_last_error().decode()
+
+
+def backend_available():
+    return bool(lib().xfe_backend_available())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_p)
+
+
+def _opts(tol, tolerance_mode, max_iter, method, force_cg, precision):
+    return Options(float(tol), int(tolerance_mode), int(max_iter), int(method), int(bool(force_cg)),
+                   int(precision))
+
+
+def _info(r, D):
+    return dict(iterations=r.iterations, converged=bool(r.converged), used_accel=bool(r.used_accel),
+                used_cg=bool(r.used_cg), polish_sweeps=r.polish_sweeps,
+                polish_final_max=r.polish_final_max, sweep_order=list(r.sweep_order_used)[:D],
+                fe_levels=list(r.fe_levels)[:D], sweeps=r.sweeps, refinements=r.refinements,
+                solve_ms=r.solve_ms)
+
+
+def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=1e-8,
+           tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0):
+    """absorb_fixed_effects_cuda on host arrays: returns (y_tilde, X_tilde, info).
+
+    gids: compact 0-based level ids per dimension; G: level counts.  ``method``
+    defaults to the reference's choice (SymmetricGaussSeidel for two FEs)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
+    n, k = X.shape
+    D = len(gids)
+    ids = [np.ascontiguousarray(g, dtype=np.int32) for g in gids]
+    ws = [None if weight_sums is None else np.ascontiguousarray(weight_sums[d], dtype=np.float64)
+          for d in range(D)]
+    fes = (FeInput * max(D, 1))()
+    for d in range(D):
+        fes[d].group_ids = ids[d].ctypes.data
+        fes[d].num_groups = int(G[d])
+        fes[d].num_levels_present = int(G[d])
+        fes[d].weight_sums = None if ws[d] is None else ws[d].ctypes.data
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
+    if method is None:
+        method = 2 if D == 2 else 1
+    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision)
+    r = AbsorbResult()
+    yo = np.empty(n)
+    Xo = np.empty((n, k), order="F")
+    ok = lib().xfe_absorb(_ptr(y), _ptr(X), _i(n), _i(k), fes, _i(D), _ptr(w), _ptr(order),
+                          ctypes.byref(o), ctypes.byref(r), _ptr(yo), _ptr(Xo))
+    if not ok:
+        raise XfeError(last_error())
+    return yo, Xo, _info(r, D)
+
+
+def absorb_device(V_ptr, n, C, ld, gid_ptr, G, weights_ptr=None, sweep_order=None, tol=1e-8,
+                  tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0):
+    """xfe_absorb_device on device pointers (e.g. torch tensors' data_ptr())."""
+    D = len(G)
+    Garr = np.ascontiguousarray(G, dtype=np.int32)
+    order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
+    if method is None:
+        method = 2 if D == 2 else 1
+    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision)
+    r = AbsorbResult()
+    ok = lib().xfe_absorb_device(_p(V_ptr), _i(n), _i(C), _i(ld), _p(gid_ptr), _ptr(Garr), _i(D),
+                                 _p(weights_ptr) if weights_ptr else None, _ptr(order),
+                                 ctypes.byref(o), ctypes.byref(r))
+    if not ok:
+        raise XfeError(last_error())
+    return _info(r, D)
+
+
+def drop_singletons(fes, max_rounds=100):
+    """compute_keep_mask_drop_singletons on the device: (keep mask, dropped, rounds)."""
+    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    D, n = ids.shape
+    keep = np.empty(n, dtype=np.uint8)
+    rounds = _i()
+    dropped = lib().xfe_drop_singletons(_ptr(ids), _i(n), _i(D), _i(max_rounds), _ptr(keep),
+                                        ctypes.byref(rounds))
+    if dropped < 0:
+        raise XfeError(last_error())
+    return keep.astype(bool), dropped, rounds.value
+
+
+def components(a, b, expected_a=-1, expected_b=-1, labels=True):
+    """count_bipartite_components on the device: (count, 1-based labels or None)."""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    lab = np.empty(a.size, dtype=np.int32) if labels else None
+    c = lib().xfe_bipartite_components(_ptr(a), _ptr(b), _i(a.size), _i(expected_a),
+                                       _i(expected_b), _ptr(lab))
+    if c < 0:
+        raise XfeError(last_error())
+    return c, lab
+
+
+def fe_dof(fes, levels):
+    """compute_fe_dof_reghdfe (pairwise) on the device: (df_a, redundant, num_coefs)."""
+    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    D, n = ids.shape
+    lv = np.ascontiguousarray(levels, dtype=np.int32)
+    red = np.empty(D, dtype=np.int32)
+    nc = np.empty(D, dtype=np.int32)
+    df = lib().xfe_fe_dof(_ptr(ids), _i(n), _i(D), _ptr(lv), _ptr(red), _ptr(nc))
+    if df < 0:
+        raise XfeError(last_error())
+    return df, red, nc
+
+
+def gram(V):
+    V = np.asfortranarray(V, dtype=np.float64)
+    n, C = V.shape
+    out = np.empty((C, C), order="F")
+    if lib().xfe_gram(_ptr(V), _i(n), _i(C), _ptr(out)) != 0:
+        raise XfeError(last_error())
+    return out
+
+
+def cluster_meat(V, beta, cluster_ids, num_clusters):
+    V = np.asfortranarray(V, dtype=np.float64)
+    n, C = V.shape
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    cl = np.ascontiguousarray(cluster_ids, dtype=np.int32)
+    out = np.empty((C - 1, C - 1), order="F")
+    if lib().xfe_cluster_meat(_ptr(V), _i(n), _i(C), _ptr(b), _ptr(cl), _i(num_clusters),
+                              _ptr(out)) != 0:
+        raise XfeError(last_error())
+    return out
+
+
+def profile_enable(on=True):
+    lib().xfe_profile_enable(_i(int(on)))
+
+
+def profile_reset():
+    lib().xfe_profile_reset()
+
+
+def profile_get():
+    ms, n, b = _d(), ctypes.c_longlong(), _d()
+    lib().xfe_profile_get(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b))
+    return ms.value, n.value, b.value
