@@ -1,0 +1,256 @@
+"""Parity of the HIP path with the oracle, through the C ABI, on an MI355X.
+
+Tolerances: integer results (singleton masks, component labels, DoF) are
+compared exactly; demeaned columns within 1e-8 relative to the column scale
+(max |v~_gpu - v~_oracle| / max |v|) at the default tolerance 1e-8 and within
+1e-10 at tol 1e-12; Gram / meat within 1e-12 relative.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import xfe_amd
+from xfe_amd import panels
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _prep(p):
+    keep, _, _ = oracle.drop_singletons(p["fes"])
+    fes = [f[keep] for f in p["fes"]]
+    idx = [oracle.build_indexer(f) for f in fes]
+    V = np.column_stack([p["y"][keep], p["X"][keep]])
+    return keep, fes, [i[0] for i in idx], [i[1] for i in idx], V
+
+
+def _rel(a, b, scale):
+    return np.abs(a - b).max() / scale
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+@pytest.mark.parametrize("tol", [1e-8, 1e-12])
+def test_absorb_matches_oracle(cases, name, tol):
+    _log(f"absorb {name} tol={tol}")
+    p = cases[name]
+    _, fes, gids, G, V = _prep(p)
+    Vo, info_o = oracle.absorb(V, gids, G, tol=tol)
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=tol)
+    assert info["converged"]
+    scale = np.abs(V).max()
+    err = max(_rel(yg, Vo[:, 0], scale), _rel(Xg, Vo[:, 1:], scale))
+    assert err <= (1e-8 if tol == 1e-8 else 1e-10), (err, info, info_o)
+
+
+def test_absorb_config0_full(golden):
+    _log("absorb config0 200k")
+    p = panels.config0()
+    _, fes, gids, G, V = _prep(p)
+    Vo, info_o = oracle.absorb(V, gids, G, tol=1e-8)
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8)
+    scale = np.abs(V).max()
+    err = max(_rel(yg, Vo[:, 0], scale), _rel(Xg, Vo[:, 1:], scale))
+    _log(f"  gpu iters {info['iterations']} oracle iters {info_o['iterations']} err {err:.3e}")
+    assert info["converged"] and err <= 1e-8
+
+
+def test_absorb_akm_scaled_three_way():
+    _log("absorb akm 1% scale")
+    p = panels.akm_small(scale=0.01)
+    _, fes, gids, G, V = _prep(p)
+    Vo, info_o = oracle.absorb(V, gids, G, tol=1e-8)
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8)
+    scale = np.abs(V).max()
+    err = max(_rel(yg, Vo[:, 0], scale), _rel(Xg, Vo[:, 1:], scale))
+    _log(f"  gpu iters {info['iterations']} oracle iters {info_o['iterations']} err {err:.3e}")
+    assert info["converged"] and err <= 1e-8
+
+
+def test_absorb_weighted():
+    _log("absorb weighted")
+    p = panels.config0(seed=11, workers=800, years=6, firms=90)
+    _, fes, gids, G, V = _prep(p)
+    w = np.random.default_rng(2).uniform(0.5, 2.0, V.shape[0])
+    Vo, _ = oracle.absorb(V, gids, G, weights=w, tol=1e-12)
+    ws = [np.bincount(g, weights=w, minlength=Gd) for g, Gd in zip(gids, G)]
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, weight_sums=ws, weights=w, tol=1e-12)
+    scale = np.abs(V).max()
+    assert max(_rel(yg, Vo[:, 0], scale), _rel(Xg, Vo[:, 1:], scale)) <= 1e-10
+
+
+def test_absorb_cg_path():
+    _log("absorb forced CG")
+    p = panels.config0(seed=12, workers=700, years=5, firms=60)
+    _, fes, gids, G, V = _prep(p)
+    Vo, info_o = oracle.absorb(V, gids, G, tol=1e-12, force_cg=True)
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-12, force_cg=True)
+    assert info["used_cg"] and info_o["used_cg"]
+    scale = np.abs(V).max()
+    assert max(_rel(yg, Vo[:, 0], scale), _rel(Xg, Vo[:, 1:], scale)) <= 1e-9
+
+
+def test_absorb_plain_map_path():
+    _log("absorb plain MAP (no acceleration)")
+    rng = np.random.default_rng(13)
+    n = 5000
+    a = rng.integers(0, 50, n).astype(np.int32)   # 100 obs per level -> no acceleration
+    b = rng.integers(0, 40, n).astype(np.int32)
+    V = rng.standard_normal((n, 3))
+    Vo, info_o = oracle.absorb(V, [a, b], [50, 40], tol=1e-10)
+    assert not info_o["used_accel"]
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], [a, b], [50, 40], tol=1e-10)
+    assert not info["used_accel"] and info["converged"]
+    assert info["iterations"] == info_o["iterations"]
+    assert max(_rel(yg, Vo[:, 0], 1.0), _rel(Xg, Vo[:, 1:], 1.0)) <= 1e-10
+
+
+def test_absorb_many_columns():
+    _log("absorb 20 columns (chunked passes)")
+    p = panels.config0(seed=14, workers=500, years=8, firms=70)
+    _, fes, gids, G, V = _prep(p)
+    rng = np.random.default_rng(3)
+    V = np.column_stack([V, rng.standard_normal((V.shape[0], 16))])
+    Vo, _ = oracle.absorb(V, gids, G, tol=1e-12)
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-12)
+    scale = np.abs(V).max()
+    assert max(_rel(yg, Vo[:, 0], scale), _rel(Xg, Vo[:, 1:], scale)) <= 1e-10
+
+
+def test_singletons_exact(cases):
+    _log("singletons")
+    rng = np.random.default_rng(21)
+    for trial in range(20):
+        n = int(rng.integers(1, 400))
+        D = int(rng.integers(1, 4))
+        fes = [rng.integers(-3, int(rng.integers(2, 200)), n).astype(np.int32) for _ in range(D)]
+        k1, d1, r1 = oracle.drop_singletons(fes)
+        k2, d2, r2 = xfe_amd.drop_singletons(fes)
+        np.testing.assert_array_equal(k1, k2)
+        assert d1 == d2 and r1 == r2
+    for p in list(cases.values()) + [panels.config0()]:
+        k1, d1, r1 = oracle.drop_singletons(p["fes"])
+        k2, d2, r2 = xfe_amd.drop_singletons(p["fes"])
+        np.testing.assert_array_equal(k1, k2)
+        assert (d1, r1) == (d2, r2)
+    # round cap: a chain that needs many rounds stops at the same place
+    n = 300
+    a = np.arange(n, dtype=np.int32) // 2
+    b = (np.arange(n, dtype=np.int32) + 1) // 2
+    for cap in (1, 5, 100):
+        k1, d1, r1 = oracle.drop_singletons([a, b], max_rounds=cap)
+        k2, d2, r2 = xfe_amd.drop_singletons([a, b], max_rounds=cap)
+        np.testing.assert_array_equal(k1, k2)
+        assert (d1, r1) == (d2, r2)
+
+
+def test_components_exact(cases):
+    _log("components")
+    rng = np.random.default_rng(22)
+    for trial in range(20):
+        n = int(rng.integers(1, 500))
+        a = (rng.integers(0, int(rng.integers(1, 80)), n) * int(rng.integers(1, 4))).astype(np.int32)
+        b = rng.integers(-30, int(rng.integers(-28, 60)), n).astype(np.int32)
+        la = np.unique(a).size
+        for exp in (-1, la):
+            c1, l1 = oracle.components(a, b, exp, -1)
+            c2, l2 = xfe_amd.components(a, b, exp, -1)
+            assert c1 == c2
+            np.testing.assert_array_equal(l1, l2)
+    for p in list(cases.values()) + [panels.config0()]:
+        keep, _, _ = oracle.drop_singletons(p["fes"])
+        a, b = p["fes"][0][keep], p["fes"][1][keep]
+        la, lb = np.unique(a).size, np.unique(b).size
+        c1, l1 = oracle.components(a, b, la, lb)
+        c2, l2 = xfe_amd.components(a, b, la, lb)
+        assert c1 == c2
+        np.testing.assert_array_equal(l1, l2)
+
+
+def test_dof_exact(cases):
+    _log("dof")
+    for p in cases.values():
+        keep, _, _ = oracle.drop_singletons(p["fes"])
+        fes = [f[keep] for f in p["fes"]]
+        levels = [np.unique(f).size for f in fes]
+        df1, r1, c1 = oracle.dof_reghdfe(fes, levels)
+        df2, r2, c2 = xfe_amd.fe_dof(fes, levels)
+        assert df1 == df2
+        np.testing.assert_array_equal(r1, r2)
+        np.testing.assert_array_equal(c1, c2)
+    # coarsening: an FE that is a function of a finer one
+    rng = np.random.default_rng(23)
+    fine = rng.integers(0, 60, 2000).astype(np.int32)
+    coarse = (fine // 7).astype(np.int32)
+    other = rng.integers(0, 30, 2000).astype(np.int32)
+    fes = [other, coarse, fine]
+    levels = [np.unique(f).size for f in fes]
+    assert oracle.dof_reghdfe(fes, levels)[0] == xfe_amd.fe_dof(fes, levels)[0]
+
+
+def test_gram_and_cluster_meat():
+    _log("gram / meat")
+    rng = np.random.default_rng(24)
+    n, C = 10000, 11
+    V = rng.standard_normal((n, C))
+    G = xfe_amd.gram(V)
+    ref = V.T @ V
+    assert np.abs(G - ref).max() <= 1e-12 * np.abs(ref).max()
+    cl = rng.integers(0, 300, n).astype(np.int32)
+    beta = rng.standard_normal(C - 1)
+    M = xfe_amd.cluster_meat(V, beta, cl, 300)
+    e = V[:, 0] - V[:, 1:] @ beta
+    S = np.zeros((300, C - 1))
+    np.add.at(S, cl, V[:, 1:] * e[:, None])
+    ref = S.T @ S
+    assert np.abs(M - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def test_recovery_max_delta():
+    _log("recovery max delta")
+    rng = np.random.default_rng(25)
+    n = 3000
+    a = rng.integers(0, 40, n).astype(np.int32)
+    b = rng.integers(0, 25, n).astype(np.int32)
+    r = rng.standard_normal(n)
+    # one GS pass: max over dims of max |mean| at the time the dim is projected
+    v = r.copy()
+    mx = 0.0
+    for g, Gd in ((a, 40), (b, 25)):
+        m = np.bincount(g, weights=v, minlength=Gd) / np.bincount(g, minlength=Gd)
+        mx = max(mx, np.abs(m).max())
+        v = v - m[g]
+    fe = (xfe_amd.FeInput * 2)()
+    for d, (g, Gd) in enumerate(((a, 40), (b, 25))):
+        fe[d].group_ids = g.ctypes.data
+        fe[d].num_groups = Gd
+    import ctypes
+    out = ctypes.c_double()
+    ok = xfe_amd.lib().xfe_recovery_max_delta(r.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(n),
+                                             fe, ctypes.c_int(2), None, None, ctypes.byref(out))
+    assert ok == 1
+    assert abs(out.value - mx) <= 1e-13 * max(1.0, mx)
+
+
+def test_device_entry_point_matches_host_entry_point():
+    _log("device entry point")
+    torch = pytest.importorskip("torch")
+    p = panels.config0(seed=15, workers=1000, years=7, firms=100)
+    _, fes, gids, G, V = _prep(p)
+    yg, Xg, _ = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-10)
+    n, C = V.shape
+    ld = ((n + 63) // 64) * 64
+    dV = torch.zeros((C, ld), dtype=torch.float64, device="cuda")
+    dV[:, :n] = torch.from_numpy(np.ascontiguousarray(V.T)).cuda()
+    ids = torch.from_numpy(np.ascontiguousarray(np.stack(gids))).cuda()
+    torch.cuda.synchronize()
+    info = xfe_amd.absorb_device(dV.data_ptr(), n, C, ld, ids.data_ptr(), G, tol=1e-10)
+    out = dV[:, :n].cpu().numpy().T
+    scale = np.abs(V).max()
+    assert info["converged"]
+    assert max(_rel(out[:, 0], yg, scale), _rel(out[:, 1:], Xg, scale)) <= 1e-12
