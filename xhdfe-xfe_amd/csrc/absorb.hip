@@ -662,6 +662,58 @@ static void solve_core(Workspace& ws, double* V, int n, int C, int ld, int D,
     res.sweeps = sweeps;
 }
 
+// select_sweep_order (src/fe_absorption.cpp:5396-5470), on the device: for
+// 2 <= D <= 4 dimensions try every permutation for two Gauss-Seidel passes over
+// the first min(n, 200000) rows of y (caller's row order, weight sums over those
+// rows) and keep the first permutation with the smallest remaining norm.
+static std::vector<int> select_sweep_order_device(Workspace& ws, const double* d_y, int n, int C,
+                                                  const std::vector<const int*>& gids,
+                                                  const std::vector<int>& G, const double* w) {
+    const int D = static_cast<int>(G.size());
+    std::vector<int> order(D);
+    for (int d = 0; d < D; ++d) order[d] = d;
+    if (D <= 1 || D > 4) return order;
+    const int np = std::min(n, 200000);
+    if (np <= 0) return order;
+    const int ldp = ((np + 63) / 64) * 64;
+    hipStream_t s = ws.stream;
+    Solver sv(ws, np, 1, ldp);
+    sv.D = D;
+    sv.w = w;
+    for (int d = 0; d < D; ++d) {
+        const size_t Gd = static_cast<size_t>(std::max(G[d], 1));
+        sv.G[d] = G[d];
+        sv.gid[d] = gids[d];
+        ws.wsum[d].ensure(Gd);
+        ws.inv[d].ensure(Gd);
+        ws.sums[d].ensure(Gd * C);
+        ws.means[d].ensure(Gd * C);
+        XFE_CHECK(hipMemsetAsync(ws.wsum[d].p, 0, sizeof(double) * Gd, s));
+        XFE_CHECK(launch_weight_sums(gids[d], w, np, ws.wsum[d].p, grid_for(np), s));
+        XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G[d], grid_for(G[d]), s));
+        XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * Gd, s));
+    }
+    ws.partials.ensure(static_cast<size_t>(sv.grid) * 3 + 1024);
+    ws.scalars.ensure(8);
+    ws.vec[6].ensure(ldp);
+    double* tmp = ws.vec[6].p;
+    std::vector<int> perm = order;
+    double best = INFINITY;
+    do {
+        XFE_CHECK(hipMemcpyAsync(tmp, d_y, sizeof(double) * np, hipMemcpyDeviceToDevice, s));
+        std::vector<int> seq(perm);
+        seq.insert(seq.end(), perm.begin(), perm.end());  // two passes
+        double sq = 0.0;
+        sv.sweep(tmp, tmp, seq, false, -1, 1, nullptr, nullptr, &sq);
+        const double norm = std::sqrt(sq);
+        if (norm < best) {
+            best = norm;
+            order = perm;
+        }
+    } while (std::next_permutation(perm.begin(), perm.end()));
+    return order;
+}
+
 // Shared driver for both entry points.  d_Vin: caller's n x C (ld_in) block,
 // d_gid_in[d]: caller's ids; the result is written back to d_Vin.
 static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
@@ -677,6 +729,11 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
     for (int d = 0; d < D; ++d) res.fe_levels[d] = G[d];
     const int ld = ((n + 63) / 64) * 64;
     const int grid = grid_for(n);
+    std::vector<int> probe_order(D);
+    for (int d = 0; d < D; ++d) probe_order[d] = d;
+    if (!sweep_order && D == 2) {
+        probe_order = select_sweep_order_device(ws, d_Vin, n, C, d_gid_in, G, d_w_in);
+    }
     // primary (sorted) dimension: the one with the most levels
     int p = 0;
     for (int d = 1; d < D; ++d) if (G[d] > G[p]) p = d;
@@ -752,7 +809,7 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
     } else if (D > 2) {
         order = order_by_profile(profs);
     } else {
-        for (int d = 0; d < D; ++d) order[d] = d;
+        order = probe_order;
     }
     solve_core(ws, V, n, C, ld, D, G, order, profs, w, opt, res);
     XFE_CHECK(launch_scatter_cols(V, ld, perm, d_Vin, ld_in, n, C, grid, s));

@@ -53,9 +53,19 @@ class XfeError(RuntimeError):
 
 
 def lib():
-    """Load libxfe_hip.so; raises if it is missing (no CPU fallback exists)."""
+    """Load libxfe_hip.so; raises if it is missing (no CPU fallback exists).
+
+    PyTorch-ROCm ships its own libamdhip64.so.7 with the same soname as the
+    system runtime this library links; a process can hold only one.  torch is
+    imported first (when installed) so that torch and this library share
+    torch's runtime; loading the system one first leaves torch without a
+    usable device."""
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise XfeError(f"{LIB_PATH} is not built; run make -C xhdfe-xfe_amd")
         L = ctypes.CDLL(LIB_PATH)
