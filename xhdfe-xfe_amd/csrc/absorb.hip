@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -115,16 +116,52 @@ struct Solver {
     const double* w = nullptr;
     int grid = 1;
     int nchunks = 1;
-    // partial-buffer layout: one region per concurrent reduction
-    size_t part_stride = 0;
+    int sorted_dim = -1;          // the dimension whose levels are contiguous runs
+    int xcd_copies = kXcdCopies;  // private accumulators for unsorted dimensions
 
     Solver(Workspace& w_, int n_, int C_, int ld_) : ws(w_), s(w_.stream), n(n_), C(C_), ld(ld_) {
-        grid = grid_for(n);
+        grid = pass_grid(n);
         nchunks = (C + 7) / 8;
+        if (const char* e = std::getenv("XFE_XCD_COPIES")) xcd_copies = std::max(1, std::min(8, std::atoi(e)));
     }
 
-    bool lds_ok(int d, int cc) const {
-        return static_cast<size_t>(G[d]) * cc * sizeof(double) <= kLdsBudget;
+    static int pass_grid(int rows) {
+        const long long tiles = (static_cast<long long>(rows) + kBlock * kRowSlots - 1) / (kBlock * kRowSlots);
+        return static_cast<int>(std::max(1LL, std::min<long long>(tiles, kMaxGrid)));
+    }
+
+    // accumulation mode of dimension d for a pass over `cols` columns per launch
+    int mode_of(int d, int cols) const {
+        if (static_cast<size_t>(G[d]) * std::min(8, cols) * sizeof(double) <= kLdsBudget) return kRedLds;
+        if (d == sorted_dim) return kRedSorted;
+        return kRedGlobal;
+    }
+    int copies_of(int d, int cols) const { return mode_of(d, cols) == kRedGlobal ? xcd_copies : 1; }
+
+    // size and clear the accumulators of dimension d (call once per solve)
+    void prepare_dim(int d) {
+        const size_t Gd = static_cast<size_t>(std::max(G[d], 1));
+        const size_t cs = static_cast<size_t>(std::max(copies_of(d, C), copies_of(d, 1)));
+        ws.sums[d].ensure(Gd * C * cs);
+        ws.means[d].ensure(Gd * C);
+        ws.wsum[d].ensure(Gd);
+        ws.inv[d].ensure(Gd);
+        XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * Gd * C * cs, s));
+    }
+
+    // sum of weights (counts when w == nullptr) per level of d into ws.wsum[d]
+    // and its inverse into ws.inv[d]; runs the fused pass on a column of ones
+    void weight_sums(int d) {
+        PassArgs a;
+        a.n = n; a.ld = ld; a.C = 1; a.c0 = 0; a.one = 1;
+        a.gid_r = gid[d]; a.sums_r = ws.sums[d].p; a.G_r = G[d]; a.w = w;
+        const int mode = mode_of(d, 1);
+        a.xcd_copies = copies_of(d, 1);
+        a.copy_stride = static_cast<size_t>(G[d]);
+        XFE_CHECK(launch_pass(a, 1, mode, 0, grid, s));
+        XFE_CHECK(launch_finalize(ws.sums[d].p, ws.wsum[d].p, nullptr, G[d], 1, a.xcd_copies,
+                                  nullptr, grid_for(G[d]), s));
+        XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G[d], grid_for(G[d]), s));
     }
 
     // One fused pass over the column block (all column chunks).
@@ -144,9 +181,13 @@ struct Solver {
             }
             a.it_x = it_x; a.it_gx = it_gx;
             a.partials = partials ? partials + static_cast<size_t>(ch) * grid * nq : nullptr;
-            const bool lds = r_dim >= 0 && lds_ok(r_dim, cc);
+            const int mode = r_dim >= 0 ? mode_of(r_dim, C) : kRedGlobal;
+            if (r_dim >= 0) {
+                a.xcd_copies = copies_of(r_dim, C);
+                a.copy_stride = static_cast<size_t>(G[r_dim]) * C;
+            }
             if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
-            XFE_CHECK(launch_pass(a, cc, lds, nq, grid, s));
+            XFE_CHECK(launch_pass(a, cc, mode, nq, grid, s));
             if (g_profile.enabled) {
                 XFE_CHECK(hipEventRecord(ws.ev1, s));
                 XFE_CHECK(hipEventSynchronize(ws.ev1));
@@ -168,13 +209,14 @@ struct Solver {
 
     void finalize(int d, double* partials) {
         const long long total = static_cast<long long>(G[d]) * C;
-        XFE_CHECK(launch_finalize(ws.sums[d].p, ws.means[d].p, ws.inv[d].p, G[d], C, partials,
-                                  grid_for(total), s));
+        XFE_CHECK(launch_finalize(ws.sums[d].p, ws.means[d].p, ws.inv[d].p, G[d], C,
+                                  copies_of(d, C), partials, grid_for(total), s));
     }
     int finalize_grid(int d) const { return grid_for(static_cast<long long>(G[d]) * C); }
 
     void clear_sums(int d) {
-        XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * static_cast<size_t>(G[d]) * C, s));
+        XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0,
+                                 sizeof(double) * static_cast<size_t>(G[d]) * C * copies_of(d, C), s));
     }
 
     // Reduce `count` partial records of nq doubles on device and copy the nq
@@ -487,17 +529,13 @@ bool run_cg(Solver& sv, double* cur, double* r, double* u, double* v, const std:
 
 // Core solve on device buffers already in the workspace layout.  V: n x C,
 // ld; gids[d] already permuted; ws.wsum/inv filled.  Returns 0 or throws.
-static void solve_core(Workspace& ws, double* V, int n, int C, int ld, int D,
-                       const std::vector<int>& G, const std::vector<int>& order_in,
-                       const std::vector<ProfileStats>& profs, const double* w,
-                       const xfe_options& opt, xfe_absorb_result& res) {
-    Solver sv(ws, n, C, ld);
-    sv.D = D;
-    sv.w = w;
-    for (int d = 0; d < D; ++d) {
-        sv.G[d] = G[d];
-        sv.gid[d] = ws.gid[d].p;
-    }
+static void solve_core(Solver& sv, double* V, const std::vector<int>& order_in,
+                       const std::vector<ProfileStats>& profs, const xfe_options& opt,
+                       xfe_absorb_result& res) {
+    Workspace& ws = sv.ws;
+    const int n = sv.n, C = sv.C, ld = sv.ld, D = sv.D;
+    const double* w = sv.w;
+    std::vector<int> G(sv.G, sv.G + D);
     const bool honest = opt.tolerance_mode == 1;
     const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
     std::vector<int> order = order_in;
@@ -680,18 +718,14 @@ static std::vector<int> select_sweep_order_device(Workspace& ws, const double* d
     Solver sv(ws, np, 1, ldp);
     sv.D = D;
     sv.w = w;
+    (void)C;
     for (int d = 0; d < D; ++d) {
-        const size_t Gd = static_cast<size_t>(std::max(G[d], 1));
         sv.G[d] = G[d];
         sv.gid[d] = gids[d];
-        ws.wsum[d].ensure(Gd);
-        ws.inv[d].ensure(Gd);
-        ws.sums[d].ensure(Gd * C);
-        ws.means[d].ensure(Gd * C);
-        XFE_CHECK(hipMemsetAsync(ws.wsum[d].p, 0, sizeof(double) * Gd, s));
-        XFE_CHECK(launch_weight_sums(gids[d], w, np, ws.wsum[d].p, grid_for(np), s));
-        XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G[d], grid_for(G[d]), s));
-        XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * Gd, s));
+    }
+    for (int d = 0; d < D; ++d) {
+        sv.prepare_dim(d);
+        sv.weight_sums(d);
     }
     ws.partials.ensure(static_cast<size_t>(sv.grid) * 3 + 1024);
     ws.scalars.ensure(8);
@@ -771,28 +805,28 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
         XFE_CHECK(launch_gather_vec(d_w_in, perm, ws.w.p, n, grid, s));
         w = ws.w.p;
     }
+    Solver sv(ws, n, C, ld);
+    sv.D = D;
+    sv.w = w;
+    sv.sorted_dim = p;  // level-sorted after the permutation
     std::vector<ProfileStats> profs(D);
     for (int d = 0; d < D; ++d) {
         ws.gid[d].ensure(n);
         XFE_CHECK(launch_gather_ids(d_gid_in[d], perm, ws.gid[d].p, n, grid, s));
-        const size_t Gd = static_cast<size_t>(std::max(G[d], 1));
-        ws.wsum[d].ensure(Gd);
-        ws.inv[d].ensure(Gd);
-        ws.sums[d].ensure(Gd * C);
-        ws.means[d].ensure(Gd * C);
+        sv.G[d] = G[d];
+        sv.gid[d] = ws.gid[d].p;
+        sv.prepare_dim(d);
         std::vector<double> hw(G[d]);
         if (h_wsum[d]) {
             std::memcpy(hw.data(), h_wsum[d], sizeof(double) * G[d]);
             XFE_CHECK(hipMemcpyAsync(ws.wsum[d].p, h_wsum[d], sizeof(double) * G[d],
                                      hipMemcpyHostToDevice, s));
+            XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G[d], grid_for(G[d]), s));
         } else {
-            XFE_CHECK(hipMemsetAsync(ws.wsum[d].p, 0, sizeof(double) * Gd, s));
-            XFE_CHECK(launch_weight_sums(ws.gid[d].p, w, n, ws.wsum[d].p, grid, s));
+            sv.weight_sums(d);
             XFE_CHECK(hipMemcpyAsync(hw.data(), ws.wsum[d].p, sizeof(double) * G[d],
                                      hipMemcpyDeviceToHost, s));
         }
-        XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G[d], grid_for(G[d]), s));
-        XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * Gd * C, s));
         XFE_CHECK(hipStreamSynchronize(s));
         profs[d] = profile_of(hw, w == nullptr);
     }
@@ -811,7 +845,7 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
     } else {
         order = probe_order;
     }
-    solve_core(ws, V, n, C, ld, D, G, order, profs, w, opt, res);
+    solve_core(sv, V, order, profs, opt, res);
     XFE_CHECK(launch_scatter_cols(V, ld, perm, d_Vin, ld_in, n, C, grid, s));
     XFE_CHECK(hipStreamSynchronize(s));
     const auto t1 = std::chrono::steady_clock::now();
@@ -967,16 +1001,14 @@ int xfe_recovery_max_delta(const double* residual, int n, const xfe_fe_input* fe
             XFE_CHECK(hipMemcpyAsync(dst, fes[d].group_ids, sizeof(int) * n, hipMemcpyHostToDevice, s));
             sv.gid[d] = dst;
             sv.G[d] = G;
-            ws.wsum[d].ensure(G); ws.inv[d].ensure(G); ws.sums[d].ensure(G); ws.means[d].ensure(G);
+            sv.prepare_dim(d);
             if (fes[d].weight_sums) {
                 XFE_CHECK(hipMemcpyAsync(ws.wsum[d].p, fes[d].weight_sums, sizeof(double) * G,
                                          hipMemcpyHostToDevice, s));
+                XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G, grid_for(G), s));
             } else {
-                XFE_CHECK(hipMemsetAsync(ws.wsum[d].p, 0, sizeof(double) * G, s));
-                XFE_CHECK(launch_weight_sums(dst, w, n, ws.wsum[d].p, grid_for(n), s));
+                sv.weight_sums(d);
             }
-            XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G, grid_for(G), s));
-            XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * G, s));
             part += grid_for(G);
         }
         ws.partials.ensure(part + static_cast<size_t>(grid_for(n)) * 3 + 1024);

@@ -8,6 +8,11 @@ namespace xfe {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
+constexpr int kRowSlots = 4;  // row slots per thread in the fused pass
+
+// how a pass accumulates level sums of the dimension it starts
+enum { kRedLds = 1, kRedSorted = 2, kRedGlobal = 3 };
+constexpr int kXcdCopies = 8;  // XCD-private accumulator copies (MI355X: 8 XCDs)
 
 // Arguments of the fused streaming pass (see pass_kernel).
 struct PassArgs {
@@ -30,11 +35,14 @@ struct PassArgs {
     const double* it_x = nullptr;
     const double* it_gx = nullptr;
     double* partials = nullptr;
+    int one = 0;           // values are 1 (weight sums)
+    int xcd_copies = 1;    // kRedGlobal: private accumulator copies
+    size_t copy_stride = 0;
 };
 
-hipError_t launch_pass(const PassArgs& a, int cc, bool lds, int nq, int grid, hipStream_t s);
+hipError_t launch_pass(const PassArgs& a, int cc, int rmode, int nq, int grid, hipStream_t s);
 hipError_t launch_finalize(double* sums, double* means, const double* inv, int G, int C,
-                           double* partials, int grid, hipStream_t s);
+                           int copies, double* partials, int grid, hipStream_t s);
 hipError_t launch_reduce_partials(const double* partials, int count, int nq, int mode, double* out,
                                   hipStream_t s);
 hipError_t launch_it_stats(const double* x, const double* gx, const double* ggx, int n, int C,
@@ -45,8 +53,6 @@ hipError_t launch_col_axpy(double* y, const double* a, const double* b, const do
                            int C, int ld, double sign, int grid, hipStream_t s);
 hipError_t launch_sub(double* y, const double* a, const double* b, int n, int C, int ld, int grid,
                       hipStream_t s);
-hipError_t launch_weight_sums(const int* gid, const double* w, int n, double* ws, int grid,
-                              hipStream_t s);
 hipError_t launch_inverse(const double* ws, double* inv, int G, int grid, hipStream_t s);
 hipError_t launch_iota(int* v, int n, int grid, hipStream_t s);
 hipError_t launch_gather_ids(const int* src, const int* perm, int* dst, int n, int grid,

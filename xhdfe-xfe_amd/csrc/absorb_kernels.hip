@@ -23,6 +23,7 @@
 // (e.g. 36 years x C) are privatised in LDS and flushed once per block.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 
 #include "absorb_kernels.h"
@@ -100,19 +101,30 @@ __device__ __forceinline__ double run_sum(double v, int end) {
 
 // ---------------------------------------------------------------------------
 // The fused streaming pass.  For each row i and column c in [c0, c0+CC):
-//   v = P ? P - coef (P - Q) : src            (IT extrapolation fused in)
+//   v = one ? 1 : P ? P - coef (P - Q) : src      (IT extrapolation fused in)
 //   if xout: xout = v
-//   if gid_a: v -= means_a[gid_a[i], c]       (finish projection a)
+//   if gid_a: v -= means_a[gid_a[i], c]           (finish projection a)
 //   if dst:  dst = v
 //   reductions: sum v^2, or IT stats against (it_x, it_gx)
-//   if gid_r: sums_r[gid_r[i], c] += w_i v    (start projection r)
+//   if gid_r: sums_r[gid_r[i], c] += w_i v        (start projection r)
+//
+// A block handles tiles of RS x 256 consecutive rows; row slot r of thread t
+// is row tile + r*256 + t, so every slot is a coalesced wave64 access and the
+// loads of all RS slots are issued before any reduction (memory-level
+// parallelism).  RM selects how level sums are accumulated:
+//   kRedLds    -- LDS-privatised, flushed once per block (small dimensions);
+//   kRedSorted -- the level-sorted dimension: a run that is the whole level
+//                 inside the wave's 64 rows is stored with a plain store, only
+//                 levels cut by a wave boundary use an atomic;
+//   kRedGlobal -- FP64 atomics on run heads, into the private copy of the
+//                 XCD the wave runs on when xcd_copies > 1.
 // ---------------------------------------------------------------------------
-template <int CC, bool LDS_R, int NQ>
+template <int CC, int RM, int NQ, int RS>
 __global__ void __launch_bounds__(kBlock) pass_kernel(PassArgs a) {
     extern __shared__ double lsum[];
     const int n = a.n;
     const size_t ld = static_cast<size_t>(a.ld);
-    if (LDS_R) {
+    if constexpr (RM == kRedLds) {
         for (int j = threadIdx.x; j < a.G_r * CC; j += kBlock) lsum[j] = 0.0;
         __syncthreads();
     }
@@ -120,73 +132,112 @@ __global__ void __launch_bounds__(kBlock) pass_kernel(PassArgs a) {
     double q[NQA];
 #pragma unroll
     for (int j = 0; j < NQA; ++j) q[j] = 0.0;
+    double* sums_r = a.sums_r;
+    if constexpr (RM == kRedGlobal) {
+        if (a.xcd_copies > 1) {
+            // hardware XCC id of this wave (gfx94x/gfx950 HW_REG_XCC_ID, 4 bits)
+            const unsigned xcc = __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11));
+            sums_r += static_cast<size_t>(xcc % static_cast<unsigned>(a.xcd_copies)) * a.copy_stride;
+        }
+    }
+    const int lane = lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
-    const int stride = gridDim.x * kBlock;
-    for (int base = blockIdx.x * kBlock; base < n; base += stride) {
-        const int i = base + threadIdx.x;
-        const bool valid = i < n;
-        double v[CC];
-        if (a.P) {
+    constexpr int TILE = RS * kBlock;
+    for (int tile = blockIdx.x * TILE; tile < n; tile += gridDim.x * TILE) {
+        double v[RS][CC];
+        int ga[RS], gr[RS];
+        bool valid[RS];
+#pragma unroll
+        for (int r = 0; r < RS; ++r) {
+            const int i = tile + r * kBlock + threadIdx.x;
+            valid[r] = i < n;
+            ga[r] = (a.gid_a && valid[r]) ? a.gid_a[i] : 0;
+            gr[r] = (a.gid_r && valid[r]) ? a.gid_r[i] : -1 - lane;
 #pragma unroll
             for (int c = 0; c < CC; ++c) {
                 const size_t off = static_cast<size_t>(a.c0 + c) * ld + i;
-                const double p = valid ? a.P[off] : 0.0;
-                const double qq = valid ? a.Q[off] : 0.0;
-                v[c] = p - a.coef * (p - qq);
-                if (valid && a.xout) a.xout[off] = v[c];
+                if (a.one) {
+                    v[r][c] = 1.0;
+                } else if (a.P) {
+                    const double p = valid[r] ? a.P[off] : 0.0;
+                    const double qq = valid[r] ? a.Q[off] : 0.0;
+                    v[r][c] = p - a.coef * (p - qq);
+                } else {
+                    v[r][c] = valid[r] ? a.src[off] : 0.0;
+                }
             }
-        } else {
+        }
 #pragma unroll
-            for (int c = 0; c < CC; ++c) {
-                const size_t off = static_cast<size_t>(a.c0 + c) * ld + i;
-                v[c] = valid ? a.src[off] : 0.0;
+        for (int r = 0; r < RS; ++r) {
+            const int i = tile + r * kBlock + threadIdx.x;
+            if (a.xout && valid[r]) {
+#pragma unroll
+                for (int c = 0; c < CC; ++c) a.xout[static_cast<size_t>(a.c0 + c) * ld + i] = v[r][c];
             }
-        }
-        if (a.gid_a && valid) {
-            const double* m = a.means_a + static_cast<size_t>(a.gid_a[i]) * a.C + a.c0;
+            if (a.gid_a && valid[r]) {
+                const double* m = a.means_a + static_cast<size_t>(ga[r]) * a.C + a.c0;
 #pragma unroll
-            for (int c = 0; c < CC; ++c) v[c] -= m[c];
-        }
-        if (a.dst && valid) {
+                for (int c = 0; c < CC; ++c) v[r][c] -= m[c];
+            }
+            if (a.dst && valid[r]) {
 #pragma unroll
-            for (int c = 0; c < CC; ++c) a.dst[static_cast<size_t>(a.c0 + c) * ld + i] = v[c];
-        }
-        if constexpr (NQ == 1) {
+                for (int c = 0; c < CC; ++c) a.dst[static_cast<size_t>(a.c0 + c) * ld + i] = v[r][c];
+            }
+            if constexpr (NQ == 1) {
 #pragma unroll
-            for (int c = 0; c < CC; ++c) q[0] += v[c] * v[c];
-        } else if constexpr (NQ == 3) {
-            if (valid) {
+                for (int c = 0; c < CC; ++c) q[0] += v[r][c] * v[r][c];
+            } else if constexpr (NQ == 3) {
+                if (valid[r]) {
 #pragma unroll
-                for (int c = 0; c < CC; ++c) {
-                    const size_t off = static_cast<size_t>(a.c0 + c) * ld + i;
-                    const double gx = a.it_gx[off];
-                    const double dg = v[c] - gx;
-                    const double d2 = dg - gx + a.it_x[off];
-                    q[0] += dg * d2;
-                    q[1] += d2 * d2;
-                    q[2] += dg * dg;
+                    for (int c = 0; c < CC; ++c) {
+                        const size_t off = static_cast<size_t>(a.c0 + c) * ld + i;
+                        const double gx = a.it_gx[off];
+                        const double dg = v[r][c] - gx;
+                        const double d2 = dg - gx + a.it_x[off];
+                        q[0] += dg * d2;
+                        q[1] += d2 * d2;
+                        q[2] += dg * dg;
+                    }
                 }
             }
         }
         if (a.gid_r) {
-            const int key = valid ? a.gid_r[i] : -1 - lane_id();
-            const double wi = (a.w && valid) ? a.w[i] : 1.0;
-            const int end = run_end_lane(key);
-            const bool head = run_head(key) && valid;
 #pragma unroll
-            for (int c = 0; c < CC; ++c) {
-                const double s = run_sum(a.w ? wi * v[c] : v[c], end);
-                if (head) {
-                    if (LDS_R) {
-                        atomicAdd(&lsum[key * CC + c], s);
-                    } else {
-                        atomicAdd(&a.sums_r[static_cast<size_t>(key) * a.C + a.c0 + c], s);
+            for (int r = 0; r < RS; ++r) {
+                const int i = tile + r * kBlock + threadIdx.x;
+                const int key = gr[r];
+                const double wi = (a.w && valid[r]) ? a.w[i] : 1.0;
+                const int end = run_end_lane(key);
+                const bool head = run_head(key) && valid[r];
+                bool owned = false;
+                if constexpr (RM == kRedSorted) {
+                    // the 64 rows of this wave are [wbase, wbase + 64)
+                    const int wbase = tile + r * kBlock + wave * kWave;
+                    const int before = (wbase > 0 && wbase <= n) ? a.gid_r[wbase - 1] : INT_MIN;
+                    const int after = wbase + kWave < n ? a.gid_r[wbase + kWave] : INT_MIN;
+                    owned = (lane > 0 || before != key) && (end < kWave - 1 || after != key);
+                }
+#pragma unroll
+                for (int c = 0; c < CC; ++c) {
+                    const double s = run_sum(a.w ? wi * v[r][c] : v[r][c], end);
+                    if (head) {
+                        if constexpr (RM == kRedLds) {
+                            atomicAdd(&lsum[key * CC + c], s);
+                        } else {
+                            double* t = &sums_r[static_cast<size_t>(key) * a.C + a.c0 + c];
+                            if (RM == kRedSorted && owned) {
+                                *t = s;
+                            } else {
+                                atomicAdd(t, s);
+                            }
+                        }
                     }
                 }
             }
         }
     }
-    if (LDS_R) {
+    if constexpr (RM == kRedLds) {
         __syncthreads();
         for (int j = threadIdx.x; j < a.G_r * CC; j += kBlock) {
             const double s = lsum[j];
@@ -200,17 +251,22 @@ __global__ void __launch_bounds__(kBlock) pass_kernel(PassArgs a) {
     }
 }
 
-// means = sums * inv, sums = 0; optional block max |mean| into partials.
+// means = (sum of the `copies` accumulator copies) * inv (inv == nullptr:
+// the raw sums), every copy reset to 0; optional block max |mean| into partials.
 __global__ void __launch_bounds__(kBlock)
-finalize_kernel(double* sums, double* means, const double* inv, int G, int C, double* partials) {
+finalize_kernel(double* sums, double* means, const double* inv, int G, int C, int copies,
+                double* partials) {
     double mx = 0.0;
     const size_t total = static_cast<size_t>(G) * C;
     for (size_t j = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; j < total;
          j += static_cast<size_t>(gridDim.x) * kBlock) {
-        const int g = static_cast<int>(j / C);
-        const double m = sums[j] * inv[g];
+        double acc = 0.0;
+        for (int x = 0; x < copies; ++x) {
+            acc += sums[static_cast<size_t>(x) * total + j];
+            sums[static_cast<size_t>(x) * total + j] = 0.0;
+        }
+        const double m = inv ? acc * inv[j / C] : acc;
         means[j] = m;
-        sums[j] = 0.0;
         mx = fmax(mx, fabs(m));
     }
     if (partials) {
@@ -316,14 +372,7 @@ sub_kernel(double* y, const double* a, const double* b, int n, int C, int ld) {
     }
 }
 
-// Weight sums per level (counts when w == nullptr) and inverse.
-__global__ void __launch_bounds__(kBlock)
-weight_sums_kernel(const int* gid, const double* w, int n, double* ws) {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        atomicAdd(&ws[gid[i]], w ? w[i] : 1.0);
-    }
-}
-
+// Inverse weight sums (0 for empty levels).
 __global__ void __launch_bounds__(kBlock) inverse_kernel(const double* ws, double* inv, int G) {
     for (int g = blockIdx.x * kBlock + threadIdx.x; g < G; g += gridDim.x * kBlock) {
         const double s = ws[g];
@@ -378,43 +427,47 @@ __global__ void __launch_bounds__(kBlock) sorted_check_kernel(const int* gid, in
 // Launchers (host side, same translation unit so the templates instantiate).
 // ---------------------------------------------------------------------------
 
-template <int CC, bool LDS, int NQ>
+template <int CC, int RM, int NQ>
 static hipError_t launch_pass_t(const PassArgs& a, int grid, hipStream_t s) {
-    const size_t shmem = LDS ? static_cast<size_t>(a.G_r) * CC * sizeof(double) : 0;
-    hipLaunchKernelGGL((pass_kernel<CC, LDS, NQ>), dim3(grid), dim3(kBlock), shmem, s, a);
+    const size_t shmem = RM == kRedLds ? static_cast<size_t>(a.G_r) * CC * sizeof(double) : 0;
+    hipLaunchKernelGGL((pass_kernel<CC, RM, NQ, kRowSlots>), dim3(grid), dim3(kBlock), shmem, s, a);
     return hipGetLastError();
 }
 
-template <int CC>
-static hipError_t launch_pass_cc(const PassArgs& a, bool lds, int nq, int grid, hipStream_t s) {
-    if (lds) {
-        if (nq == 0) return launch_pass_t<CC, true, 0>(a, grid, s);
-        if (nq == 1) return launch_pass_t<CC, true, 1>(a, grid, s);
-        return launch_pass_t<CC, true, 3>(a, grid, s);
-    }
-    if (nq == 0) return launch_pass_t<CC, false, 0>(a, grid, s);
-    if (nq == 1) return launch_pass_t<CC, false, 1>(a, grid, s);
-    return launch_pass_t<CC, false, 3>(a, grid, s);
+template <int CC, int RM>
+static hipError_t launch_pass_rm(const PassArgs& a, int nq, int grid, hipStream_t s) {
+    if (nq == 0) return launch_pass_t<CC, RM, 0>(a, grid, s);
+    if (nq == 1) return launch_pass_t<CC, RM, 1>(a, grid, s);
+    return launch_pass_t<CC, RM, 3>(a, grid, s);
 }
 
-hipError_t launch_pass(const PassArgs& a, int cc, bool lds, int nq, int grid, hipStream_t s) {
+template <int CC>
+static hipError_t launch_pass_cc(const PassArgs& a, int rmode, int nq, int grid, hipStream_t s) {
+    switch (rmode) {
+        case kRedLds: return launch_pass_rm<CC, kRedLds>(a, nq, grid, s);
+        case kRedSorted: return launch_pass_rm<CC, kRedSorted>(a, nq, grid, s);
+        default: return launch_pass_rm<CC, kRedGlobal>(a, nq, grid, s);
+    }
+}
+
+hipError_t launch_pass(const PassArgs& a, int cc, int rmode, int nq, int grid, hipStream_t s) {
     switch (cc) {
-        case 1: return launch_pass_cc<1>(a, lds, nq, grid, s);
-        case 2: return launch_pass_cc<2>(a, lds, nq, grid, s);
-        case 3: return launch_pass_cc<3>(a, lds, nq, grid, s);
-        case 4: return launch_pass_cc<4>(a, lds, nq, grid, s);
-        case 5: return launch_pass_cc<5>(a, lds, nq, grid, s);
-        case 6: return launch_pass_cc<6>(a, lds, nq, grid, s);
-        case 7: return launch_pass_cc<7>(a, lds, nq, grid, s);
-        case 8: return launch_pass_cc<8>(a, lds, nq, grid, s);
+        case 1: return launch_pass_cc<1>(a, rmode, nq, grid, s);
+        case 2: return launch_pass_cc<2>(a, rmode, nq, grid, s);
+        case 3: return launch_pass_cc<3>(a, rmode, nq, grid, s);
+        case 4: return launch_pass_cc<4>(a, rmode, nq, grid, s);
+        case 5: return launch_pass_cc<5>(a, rmode, nq, grid, s);
+        case 6: return launch_pass_cc<6>(a, rmode, nq, grid, s);
+        case 7: return launch_pass_cc<7>(a, rmode, nq, grid, s);
+        case 8: return launch_pass_cc<8>(a, rmode, nq, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_finalize(double* sums, double* means, const double* inv, int G, int C,
-                           double* partials, int grid, hipStream_t s) {
+                           int copies, double* partials, int grid, hipStream_t s) {
     hipLaunchKernelGGL(finalize_kernel, dim3(grid), dim3(kBlock), 0, s, sums, means, inv, G, C,
-                       partials);
+                       copies, partials);
     return hipGetLastError();
 }
 
@@ -448,12 +501,6 @@ hipError_t launch_col_axpy(double* y, const double* a, const double* b, const do
 hipError_t launch_sub(double* y, const double* a, const double* b, int n, int C, int ld, int grid,
                       hipStream_t s) {
     hipLaunchKernelGGL(sub_kernel, dim3(grid), dim3(kBlock), 0, s, y, a, b, n, C, ld);
-    return hipGetLastError();
-}
-
-hipError_t launch_weight_sums(const int* gid, const double* w, int n, double* ws, int grid,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(weight_sums_kernel, dim3(grid), dim3(kBlock), 0, s, gid, w, n, ws);
     return hipGetLastError();
 }
 
