@@ -126,8 +126,13 @@ struct Solver {
     }
 
     static int pass_grid(int rows) {
-        const long long tiles = (static_cast<long long>(rows) + kBlock * kRowSlots - 1) / (kBlock * kRowSlots);
-        return static_cast<int>(std::max(1LL, std::min<long long>(tiles, kMaxGrid)));
+        static const int max_grid = [] {
+            const char* e = std::getenv("XFE_MAX_GRID");
+            const int v = e ? std::atoi(e) : 0;
+            return v > 0 ? std::min(v, 65536) : kMaxGrid;
+        }();
+        const long long tiles = (static_cast<long long>(rows) + kBlock - 1) / kBlock;
+        return static_cast<int>(std::max(1LL, std::min<long long>(tiles, max_grid)));
     }
 
     // accumulation mode of dimension d for a pass over `cols` columns per launch
@@ -181,7 +186,7 @@ struct Solver {
             }
             a.it_x = it_x; a.it_gx = it_gx;
             a.partials = partials ? partials + static_cast<size_t>(ch) * grid * nq : nullptr;
-            const int mode = r_dim >= 0 ? mode_of(r_dim, C) : kRedGlobal;
+            const int mode = r_dim >= 0 ? mode_of(r_dim, C) : kRedNone;
             if (r_dim >= 0) {
                 a.xcd_copies = copies_of(r_dim, C);
                 a.copy_stride = static_cast<size_t>(G[r_dim]) * C;
@@ -831,7 +836,8 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
         profs[d] = profile_of(hw, w == nullptr);
     }
     // partial buffer: enough for chunked passes and polish maxima over a sweep
-    size_t part = static_cast<size_t>(grid) * ((C + 7) / 8) * 3 + static_cast<size_t>(grid) * C;
+    const size_t pg = static_cast<size_t>(std::max(sv.grid, grid));
+    size_t part = pg * ((C + 7) / 8) * 3 + pg * C;
     for (int d = 0; d < D; ++d) part += 2 * static_cast<size_t>(grid_for(static_cast<long long>(G[d]) * C));
     ws.partials.ensure(part + 1024);
     ws.scalars.ensure(std::max(C, 8));
@@ -1011,7 +1017,7 @@ int xfe_recovery_max_delta(const double* residual, int n, const xfe_fe_input* fe
             }
             part += grid_for(G);
         }
-        ws.partials.ensure(part + static_cast<size_t>(grid_for(n)) * 3 + 1024);
+        ws.partials.ensure(part + static_cast<size_t>(std::max(sv.grid, grid_for(n))) * 3 + 1024);
         ws.scalars.ensure(8);
         std::vector<int> order(num_fes);
         for (int d = 0; d < num_fes; ++d) order[d] = sweep_order ? sweep_order[d] : d;

@@ -8,10 +8,9 @@ namespace xfe {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
-constexpr int kRowSlots = 4;  // row slots per thread in the fused pass
 
-// how a pass accumulates level sums of the dimension it starts
-enum { kRedLds = 1, kRedSorted = 2, kRedGlobal = 3 };
+// how a pass accumulates level sums of the dimension it starts (0: no reduce)
+enum { kRedNone = 0, kRedLds = 1, kRedSorted = 2, kRedGlobal = 3 };
 constexpr int kXcdCopies = 8;  // XCD-private accumulator copies (MI355X: 8 XCDs)
 
 // Arguments of the fused streaming pass (see pass_kernel).

@@ -101,25 +101,24 @@ __device__ __forceinline__ double run_sum(double v, int end) {
 
 // ---------------------------------------------------------------------------
 // The fused streaming pass.  For each row i and column c in [c0, c0+CC):
-//   v = one ? 1 : P ? P - coef (P - Q) : src      (IT extrapolation fused in)
-//   if xout: xout = v
-//   if gid_a: v -= means_a[gid_a[i], c]           (finish projection a)
+//   v = SRC==2 ? 1 : SRC==1 ? P - coef (P - Q) : src   (IT extrapolation fused in)
+//   SRC==1: xout = v
+//   if gid_a: v -= means_a[gid_a[i], c]                  (finish projection a)
 //   if dst:  dst = v
-//   reductions: sum v^2, or IT stats against (it_x, it_gx)
-//   if gid_r: sums_r[gid_r[i], c] += w_i v        (start projection r)
+//   reductions: sum v^2 (NQ 1), or IT stats against (it_x, it_gx) (NQ 3)
+//   if gid_r: sums_r[gid_r[i], c] += w_i v               (start projection r)
 //
-// A block handles tiles of RS x 256 consecutive rows; row slot r of thread t
-// is row tile + r*256 + t, so every slot is a coalesced wave64 access and the
-// loads of all RS slots are issued before any reduction (memory-level
-// parallelism).  RM selects how level sums are accumulated:
+// One row per thread per grid-stride step: loads of the whole block row are
+// issued back to back and coalesce into 512-byte wave accesses per column.
+// RM selects how the level sums of projection r are accumulated:
 //   kRedLds    -- LDS-privatised, flushed once per block (small dimensions);
-//   kRedSorted -- the level-sorted dimension: a run that is the whole level
-//                 inside the wave's 64 rows is stored with a plain store, only
-//                 levels cut by a wave boundary use an atomic;
-//   kRedGlobal -- FP64 atomics on run heads, into the private copy of the
+//   kRedSorted -- the level-sorted dimension: a run that neither touches lane
+//                 0 nor lane 63 is a whole level owned by this wave and is
+//                 written with a plain store; other runs use an atomic;
+//   kRedGlobal -- FP64 atomics from run heads, into the private copy of the
 //                 XCD the wave runs on when xcd_copies > 1.
 // ---------------------------------------------------------------------------
-template <int CC, int RM, int NQ, int RS>
+template <int CC, int RM, int NQ, int SRC>
 __global__ void __launch_bounds__(kBlock) pass_kernel(PassArgs a) {
     extern __shared__ double lsum[];
     const int n = a.n;
@@ -141,96 +140,86 @@ __global__ void __launch_bounds__(kBlock) pass_kernel(PassArgs a) {
         }
     }
     const int lane = lane_id();
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const double* __restrict__ src = SRC == 0 ? a.src + static_cast<size_t>(a.c0) * ld : nullptr;
+    const double* __restrict__ P = SRC == 1 ? a.P + static_cast<size_t>(a.c0) * ld : nullptr;
+    const double* __restrict__ Q = SRC == 1 ? a.Q + static_cast<size_t>(a.c0) * ld : nullptr;
+    const bool apply = a.gid_a != nullptr;
+    const bool store = a.dst != nullptr;
+    const bool weighted = a.w != nullptr;
 
-    constexpr int TILE = RS * kBlock;
-    for (int tile = blockIdx.x * TILE; tile < n; tile += gridDim.x * TILE) {
-        double v[RS][CC];
-        int ga[RS], gr[RS];
-        bool valid[RS];
-#pragma unroll
-        for (int r = 0; r < RS; ++r) {
-            const int i = tile + r * kBlock + threadIdx.x;
-            valid[r] = i < n;
-            ga[r] = (a.gid_a && valid[r]) ? a.gid_a[i] : 0;
-            gr[r] = (a.gid_r && valid[r]) ? a.gid_r[i] : -1 - lane;
+    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const int i = base + threadIdx.x;
+        const bool valid = i < n;
+        double v[CC];
+        int ga = 0;
+        int key = -1 - lane;
+        if (valid) {
+            if (apply) ga = a.gid_a[i];
+            if (RM != 0) key = a.gid_r[i];
 #pragma unroll
             for (int c = 0; c < CC; ++c) {
-                const size_t off = static_cast<size_t>(a.c0 + c) * ld + i;
-                if (a.one) {
-                    v[r][c] = 1.0;
-                } else if (a.P) {
-                    const double p = valid[r] ? a.P[off] : 0.0;
-                    const double qq = valid[r] ? a.Q[off] : 0.0;
-                    v[r][c] = p - a.coef * (p - qq);
+                if constexpr (SRC == 2) {
+                    v[c] = 1.0;
+                } else if constexpr (SRC == 1) {
+                    const double p = P[c * ld + i];
+                    v[c] = p - a.coef * (p - Q[c * ld + i]);
                 } else {
-                    v[r][c] = valid[r] ? a.src[off] : 0.0;
+                    v[c] = src[c * ld + i];
                 }
             }
-        }
+            if constexpr (SRC == 1) {
 #pragma unroll
-        for (int r = 0; r < RS; ++r) {
-            const int i = tile + r * kBlock + threadIdx.x;
-            if (a.xout && valid[r]) {
-#pragma unroll
-                for (int c = 0; c < CC; ++c) a.xout[static_cast<size_t>(a.c0 + c) * ld + i] = v[r][c];
+                for (int c = 0; c < CC; ++c) a.xout[(a.c0 + c) * ld + i] = v[c];
             }
-            if (a.gid_a && valid[r]) {
-                const double* m = a.means_a + static_cast<size_t>(ga[r]) * a.C + a.c0;
+            if (apply) {
+                const double* m = a.means_a + static_cast<size_t>(ga) * a.C + a.c0;
 #pragma unroll
-                for (int c = 0; c < CC; ++c) v[r][c] -= m[c];
+                for (int c = 0; c < CC; ++c) v[c] -= m[c];
             }
-            if (a.dst && valid[r]) {
+            if (store) {
 #pragma unroll
-                for (int c = 0; c < CC; ++c) a.dst[static_cast<size_t>(a.c0 + c) * ld + i] = v[r][c];
+                for (int c = 0; c < CC; ++c) a.dst[(a.c0 + c) * ld + i] = v[c];
             }
             if constexpr (NQ == 1) {
 #pragma unroll
-                for (int c = 0; c < CC; ++c) q[0] += v[r][c] * v[r][c];
+                for (int c = 0; c < CC; ++c) q[0] += v[c] * v[c];
             } else if constexpr (NQ == 3) {
-                if (valid[r]) {
-#pragma unroll
-                    for (int c = 0; c < CC; ++c) {
-                        const size_t off = static_cast<size_t>(a.c0 + c) * ld + i;
-                        const double gx = a.it_gx[off];
-                        const double dg = v[r][c] - gx;
-                        const double d2 = dg - gx + a.it_x[off];
-                        q[0] += dg * d2;
-                        q[1] += d2 * d2;
-                        q[2] += dg * dg;
-                    }
-                }
-            }
-        }
-        if (a.gid_r) {
-#pragma unroll
-            for (int r = 0; r < RS; ++r) {
-                const int i = tile + r * kBlock + threadIdx.x;
-                const int key = gr[r];
-                const double wi = (a.w && valid[r]) ? a.w[i] : 1.0;
-                const int end = run_end_lane(key);
-                const bool head = run_head(key) && valid[r];
-                bool owned = false;
-                if constexpr (RM == kRedSorted) {
-                    // the 64 rows of this wave are [wbase, wbase + 64)
-                    const int wbase = tile + r * kBlock + wave * kWave;
-                    const int before = (wbase > 0 && wbase <= n) ? a.gid_r[wbase - 1] : INT_MIN;
-                    const int after = wbase + kWave < n ? a.gid_r[wbase + kWave] : INT_MIN;
-                    owned = (lane > 0 || before != key) && (end < kWave - 1 || after != key);
-                }
 #pragma unroll
                 for (int c = 0; c < CC; ++c) {
-                    const double s = run_sum(a.w ? wi * v[r][c] : v[r][c], end);
-                    if (head) {
-                        if constexpr (RM == kRedLds) {
-                            atomicAdd(&lsum[key * CC + c], s);
+                    const size_t off = (a.c0 + c) * ld + i;
+                    const double gx = a.it_gx[off];
+                    const double dg = v[c] - gx;
+                    const double d2 = dg - gx + a.it_x[off];
+                    q[0] += dg * d2;
+                    q[1] += d2 * d2;
+                    q[2] += dg * dg;
+                }
+            }
+            if (weighted && RM != 0) {
+                const double wi = a.w[i];
+#pragma unroll
+                for (int c = 0; c < CC; ++c) v[c] *= wi;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < CC; ++c) v[c] = 0.0;
+        }
+        if constexpr (RM != 0) {
+            const int end = run_end_lane(key);
+            const bool head = run_head(key) && valid;
+            const bool owned = RM == kRedSorted && lane > 0 && end < kWave - 1;
+#pragma unroll
+            for (int c = 0; c < CC; ++c) {
+                const double s = run_sum(v[c], end);
+                if (head) {
+                    if constexpr (RM == kRedLds) {
+                        atomicAdd(&lsum[key * CC + c], s);
+                    } else {
+                        double* t = &sums_r[static_cast<size_t>(key) * a.C + a.c0 + c];
+                        if (owned) {
+                            *t = s;
                         } else {
-                            double* t = &sums_r[static_cast<size_t>(key) * a.C + a.c0 + c];
-                            if (RM == kRedSorted && owned) {
-                                *t = s;
-                            } else {
-                                atomicAdd(t, s);
-                            }
+                            atomicAdd(t, s);
                         }
                     }
                 }
@@ -427,39 +416,70 @@ __global__ void __launch_bounds__(kBlock) sorted_check_kernel(const int* gid, in
 // Launchers (host side, same translation unit so the templates instantiate).
 // ---------------------------------------------------------------------------
 
-template <int CC, int RM, int NQ>
+template <int CC, int RM, int NQ, int SRC>
 static hipError_t launch_pass_t(const PassArgs& a, int grid, hipStream_t s) {
     const size_t shmem = RM == kRedLds ? static_cast<size_t>(a.G_r) * CC * sizeof(double) : 0;
-    hipLaunchKernelGGL((pass_kernel<CC, RM, NQ, kRowSlots>), dim3(grid), dim3(kBlock), shmem, s, a);
+    hipLaunchKernelGGL((pass_kernel<CC, RM, NQ, SRC>), dim3(grid), dim3(kBlock), shmem, s, a);
     return hipGetLastError();
 }
 
-template <int CC, int RM>
-static hipError_t launch_pass_rm(const PassArgs& a, int nq, int grid, hipStream_t s) {
-    if (nq == 0) return launch_pass_t<CC, RM, 0>(a, grid, s);
-    if (nq == 1) return launch_pass_t<CC, RM, 1>(a, grid, s);
-    return launch_pass_t<CC, RM, 3>(a, grid, s);
-}
-
+// Instantiated variants: plain source with any reduce mode and NQ in {0,1}
+// (NQ 3 only without a reduce), the IT extrapolation source with a reduce,
+// and the constant source (weight sums) with one column.
 template <int CC>
-static hipError_t launch_pass_cc(const PassArgs& a, int rmode, int nq, int grid, hipStream_t s) {
-    switch (rmode) {
-        case kRedLds: return launch_pass_rm<CC, kRedLds>(a, nq, grid, s);
-        case kRedSorted: return launch_pass_rm<CC, kRedSorted>(a, nq, grid, s);
-        default: return launch_pass_rm<CC, kRedGlobal>(a, nq, grid, s);
+static hipError_t launch_pass_cc(const PassArgs& a, int src, int rmode, int nq, int grid,
+                                 hipStream_t s) {
+    if (src == 0) {
+        if (nq == 3) {
+            if (rmode != 0) return hipErrorInvalidValue;
+            return launch_pass_t<CC, 0, 3, 0>(a, grid, s);
+        }
+        switch (rmode * 2 + (nq == 1)) {
+            case 0: return launch_pass_t<CC, 0, 0, 0>(a, grid, s);
+            case 1: return launch_pass_t<CC, 0, 1, 0>(a, grid, s);
+            case 2: return launch_pass_t<CC, kRedLds, 0, 0>(a, grid, s);
+            case 3: return launch_pass_t<CC, kRedLds, 1, 0>(a, grid, s);
+            case 4: return launch_pass_t<CC, kRedSorted, 0, 0>(a, grid, s);
+            case 5: return launch_pass_t<CC, kRedSorted, 1, 0>(a, grid, s);
+            case 6: return launch_pass_t<CC, kRedGlobal, 0, 0>(a, grid, s);
+            case 7: return launch_pass_t<CC, kRedGlobal, 1, 0>(a, grid, s);
+            default: return hipErrorInvalidValue;
+        }
     }
+    if (src == 1) {
+        if (nq != 0) return hipErrorInvalidValue;
+        switch (rmode) {
+            case 0: return launch_pass_t<CC, 0, 0, 1>(a, grid, s);
+            case kRedLds: return launch_pass_t<CC, kRedLds, 0, 1>(a, grid, s);
+            case kRedSorted: return launch_pass_t<CC, kRedSorted, 0, 1>(a, grid, s);
+            default: return launch_pass_t<CC, kRedGlobal, 0, 1>(a, grid, s);
+        }
+    }
+    if constexpr (CC == 1) {
+        if (nq != 0) return hipErrorInvalidValue;
+        switch (rmode) {
+            case kRedLds: return launch_pass_t<1, kRedLds, 0, 2>(a, grid, s);
+            case kRedSorted: return launch_pass_t<1, kRedSorted, 0, 2>(a, grid, s);
+            case kRedGlobal: return launch_pass_t<1, kRedGlobal, 0, 2>(a, grid, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_pass(const PassArgs& a, int cc, int rmode, int nq, int grid, hipStream_t s) {
+    const int src = a.one ? 2 : (a.P ? 1 : 0);
+    if (rmode != 0 && a.gid_r == nullptr) return hipErrorInvalidValue;
+    if (src == 1 && a.xout == nullptr) return hipErrorInvalidValue;
     switch (cc) {
-        case 1: return launch_pass_cc<1>(a, rmode, nq, grid, s);
-        case 2: return launch_pass_cc<2>(a, rmode, nq, grid, s);
-        case 3: return launch_pass_cc<3>(a, rmode, nq, grid, s);
-        case 4: return launch_pass_cc<4>(a, rmode, nq, grid, s);
-        case 5: return launch_pass_cc<5>(a, rmode, nq, grid, s);
-        case 6: return launch_pass_cc<6>(a, rmode, nq, grid, s);
-        case 7: return launch_pass_cc<7>(a, rmode, nq, grid, s);
-        case 8: return launch_pass_cc<8>(a, rmode, nq, grid, s);
+        case 1: return launch_pass_cc<1>(a, src, rmode, nq, grid, s);
+        case 2: return launch_pass_cc<2>(a, src, rmode, nq, grid, s);
+        case 3: return launch_pass_cc<3>(a, src, rmode, nq, grid, s);
+        case 4: return launch_pass_cc<4>(a, src, rmode, nq, grid, s);
+        case 5: return launch_pass_cc<5>(a, src, rmode, nq, grid, s);
+        case 6: return launch_pass_cc<6>(a, src, rmode, nq, grid, s);
+        case 7: return launch_pass_cc<7>(a, src, rmode, nq, grid, s);
+        case 8: return launch_pass_cc<8>(a, src, rmode, nq, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
