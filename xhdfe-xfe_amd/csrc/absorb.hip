@@ -72,6 +72,10 @@ struct Workspace {
     DevBuf<double> stage;  // host-API staging of [y | X]
     DevBuf<int> stage_ids;
     DevBuf<double> stage_w;
+    // level-aligned tiles of the sorted dimension and spell structures
+    DevBuf<int> tile_start, level_count, tmp_flags, tmp_inc, tmp_iota, tmp_key;
+    DevBuf<int> sp_key[XFE_MAX_DIMS], sp_order[XFE_MAX_DIMS], sp_base[XFE_MAX_DIMS];
+    DevBuf<double> sp_sums[XFE_MAX_DIMS];
     double* host_scalars = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -85,6 +89,11 @@ struct Workspace {
         perm.release(); keys_tmp.release(); vals_tmp.release(); flag.release();
         sort_tmp.release(); w.release(); partials.release(); scalars.release();
         lanes.release(); stage.release(); stage_ids.release(); stage_w.release();
+        tile_start.release(); level_count.release(); tmp_flags.release(); tmp_inc.release();
+        tmp_iota.release(); tmp_key.release();
+        for (int d = 0; d < XFE_MAX_DIMS; ++d) {
+            sp_key[d].release(); sp_order[d].release(); sp_base[d].release(); sp_sums[d].release();
+        }
         if (host_scalars) (void)hipHostFree(host_scalars);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -107,6 +116,19 @@ struct Workspace {
 
 thread_local Workspace tls_ws;
 
+// Source of a sweep: a stored block, or the Irons-Tuck extrapolation
+// P - coef (P - Q) computed on the fly and written to xout.
+struct SweepSrc {
+    const double* src = nullptr;
+    const double* P = nullptr;
+    const double* Q = nullptr;
+    double coef = 0.0;
+    double* xout = nullptr;
+    SweepSrc() = default;
+    explicit SweepSrc(const double* s_) : src(s_) {}
+    SweepSrc(const double* p_, const double* q_, double c_, double* x_) : P(p_), Q(q_), coef(c_), xout(x_) {}
+};
+
 struct Solver {
     Workspace& ws;
     hipStream_t s;
@@ -118,6 +140,19 @@ struct Solver {
     int nchunks = 1;
     int sorted_dim = -1;          // the dimension whose levels are contiguous runs
     int xcd_copies = kXcdCopies;  // private accumulators for unsorted dimensions
+    // tiled mode: level-aligned tiles of sorted_dim, spells for large dimensions
+    bool tiled = false;
+    const int* tile_start = nullptr;
+    int num_tiles = 0;
+    int tile_grid = 1;
+    struct Spells {
+        int M = 0;
+        double* S = nullptr;
+        const int* order = nullptr;
+        const int* key = nullptr;
+        const int* base = nullptr;
+    } spell[XFE_MAX_DIMS];
+    double pull_ms = 0.0;
 
     Solver(Workspace& w_, int n_, int C_, int ld_) : ws(w_), s(w_.stream), n(n_), C(C_), ld(ld_) {
         grid = pass_grid(n);
@@ -272,6 +307,160 @@ struct Solver {
         XFE_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * static_cast<size_t>(ld) * C,
                                  hipMemcpyDeviceToDevice, s));
     }
+
+    // ------------------------------------------------------------ tiled mode
+
+    int tile_mode_of(int d) const {
+        if (static_cast<size_t>(G[d]) * std::min(8, C) * sizeof(double) <= kLdsBudget) return kRedLds;
+        return kRedSpell;
+    }
+
+    // One tiled pass over all column chunks: [source] -> [apply a] -> [project
+    // sorted_dim inside the tile if pp] -> [start reduce r] -> [dst, NQ].
+    void tile_pass(const SweepSrc& in, double* dst, int a_dim, bool pp, int r_dim, int nq,
+                   const double* it_x, const double* it_gx, double* partials, double* maxp) {
+        const int rr = r_dim >= 0 ? tile_mode_of(r_dim) : 0;
+        for (int ch = 0; ch < nchunks; ++ch) {
+            const int c0 = ch * 8;
+            const int cc = std::min(8, C - c0);
+            TileArgs a;
+            a.n = n; a.ld = ld; a.C = C; a.c0 = c0;
+            a.src = in.src; a.P = in.P; a.Q = in.Q; a.coef = in.coef; a.xout = in.xout;
+            a.dst = dst;
+            if (a_dim >= 0) { a.gid_a = gid[a_dim]; a.means_a = ws.means[a_dim].p; }
+            a.tile_start = tile_start; a.num_tiles = num_tiles;
+            a.gid_p = gid[sorted_dim]; a.inv_p = ws.inv[sorted_dim].p;
+            if (r_dim >= 0) {
+                a.gid_r = gid[r_dim]; a.G_r = G[r_dim]; a.sums_r = ws.sums[r_dim].p;
+                a.spell_sums = spell[r_dim].S; a.tile_spell_base = spell[r_dim].base;
+            }
+            a.w = w;
+            a.it_x = it_x; a.it_gx = it_gx;
+            a.partials = partials ? partials + static_cast<size_t>(ch) * tile_grid * nq : nullptr;
+            a.maxp_partials = maxp ? maxp + static_cast<size_t>(ch) * tile_grid : nullptr;
+            if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            XFE_CHECK(launch_tile_pass(a, cc, rr, pp, nq, tile_grid, s));
+            if (g_profile.enabled) {
+                XFE_CHECK(hipEventRecord(ws.ev1, s));
+                XFE_CHECK(hipEventSynchronize(ws.ev1));
+                float ms = 0.f;
+                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+                g_profile.ms += ms;
+                g_profile.launches += 1;
+                // algorithmic bytes: vectors read/written, id arrays, level tables
+                const double vec_rw = (in.P ? 2.0 : 1.0) + (in.xout ? 1.0 : 0.0) +
+                                      (dst ? 1.0 : 0.0) + (it_x ? 2.0 : 0.0);
+                double bytes = vec_rw * 8.0 * static_cast<double>(n) * cc;
+                if (a_dim >= 0) bytes += 4.0 * n + 8.0 * static_cast<double>(G[a_dim]) * cc;
+                if (pp || rr == kRedSpell) bytes += 4.0 * n;
+                if (pp) bytes += 8.0 * static_cast<double>(G[sorted_dim]);
+                if (rr != 0) bytes += 4.0 * n;
+                if (rr == kRedSpell) bytes += 8.0 * static_cast<double>(spell[r_dim].M) * cc;
+                if (w && (pp || rr != 0)) bytes += 8.0 * n;
+                g_profile.bytes += bytes;
+            }
+        }
+    }
+
+    // Level sums of r from its spell sums, then means.
+    void pull_and_finalize(int r, double* partials) {
+        if (tile_mode_of(r) == kRedSpell) {
+            for (int ch = 0; ch < nchunks; ++ch) {
+                const int c0 = ch * 8;
+                const int cc = std::min(8, C - c0);
+                XFE_CHECK(launch_spell_pull(spell[r].S, C, c0, cc, spell[r].order, spell[r].key,
+                                            spell[r].M, ws.sums[r].p, grid_for(spell[r].M), s));
+            }
+        }
+        XFE_CHECK(launch_finalize(ws.sums[r].p, ws.means[r].p, ws.inv[r].p, G[r], C, 1, partials,
+                                  grid_for(static_cast<long long>(G[r]) * C), s));
+    }
+
+    // dst <- G(in) over the projection sequence seq (see sweep()).  In tiled
+    // mode each pass is [apply pending] [project sorted_dim in the tile] [start
+    // the next reduce]; otherwise the chain of reduce / apply passes.  When
+    // maxp_out is given it receives the largest |mean| of all projections of
+    // the sweep (polish rule).
+    void sweep2(SweepSrc in, double* dst, const std::vector<int>& seq, int nq_last,
+                const double* it_x, const double* it_gx, double* out_q, double* maxp_out = nullptr) {
+        const int m = static_cast<int>(seq.size());
+        if (m == 0) {
+            if (in.P) {
+                pass(nullptr, nullptr, in.P, in.Q, in.coef, in.xout, -1, -1, 0, nullptr, nullptr, nullptr);
+                in = SweepSrc{in.xout};
+            }
+            if (in.src != dst) copy(dst, in.src);
+            return;
+        }
+        double* part_max = maxp_out ? ws.partials.p + maxp_offset() : nullptr;
+        size_t max_off = 0;
+        if (!tiled) {
+            if (in.P) {
+                pass(nullptr, nullptr, in.P, in.Q, in.coef, in.xout, -1, seq[0], 0, nullptr, nullptr,
+                     nullptr);
+                in = SweepSrc{in.xout};
+            } else {
+                pass(in.src, nullptr, nullptr, nullptr, 0.0, nullptr, -1, seq[0], 0, nullptr,
+                     nullptr, nullptr);
+            }
+            for (int j = 0; j < m; ++j) {
+                finalize(seq[j], part_max ? part_max + max_off : nullptr);
+                if (part_max) max_off += finalize_grid(seq[j]);
+                const bool last = j + 1 == m;
+                const int r = last ? -1 : seq[j + 1];
+                const int nq = last ? nq_last : 0;
+                pass(j == 0 ? in.src : dst, dst, nullptr, nullptr, 0.0, nullptr, seq[j], r, nq,
+                     last ? it_x : nullptr, last ? it_gx : nullptr, nq ? ws.partials.p : nullptr);
+            }
+            if (nq_last && out_q) fetch(ws.partials.p, grid * nchunks, nq_last, 0, out_q);
+        } else {
+            const int p = sorted_dim;
+            int i = 0, pending = -1;
+            bool wrote = false;
+            while (true) {
+                const int a = pending;
+                bool pp = false;
+                if (i < m && seq[i] == p) { pp = true; ++i; }
+                int r = -1;
+                if (i < m) { r = seq[i]; ++i; }
+                const bool last = r < 0;
+                const int nq = last ? nq_last : 0;
+                const bool writes = a >= 0 || pp;
+                SweepSrc src_now = wrote ? SweepSrc{dst} : in;
+                double* mp = nullptr;
+                if (part_max && pp) {
+                    mp = part_max + max_off;
+                    max_off += static_cast<size_t>(tile_grid) * nchunks;
+                }
+                tile_pass(src_now, writes ? dst : nullptr, a, pp, r, nq, last ? it_x : nullptr,
+                          last ? it_gx : nullptr, nq ? ws.partials.p : nullptr, mp);
+                if (!wrote && in.P) in = SweepSrc{in.xout};  // the extrapolated values now live in xout
+                wrote = wrote || writes;
+                if (last) break;
+                double* fp = nullptr;
+                if (part_max) {
+                    fp = part_max + max_off;
+                    max_off += finalize_grid(r);
+                }
+                pull_and_finalize(r, fp);
+                pending = r;
+            }
+            if (!wrote) {  // a sweep that never applied a projection (not reachable with m > 0)
+                if (in.src != dst) copy(dst, in.src);
+            }
+            if (nq_last && out_q) fetch(ws.partials.p, tile_grid * nchunks, nq_last, 0, out_q);
+        }
+        if (maxp_out) {
+            double mx = 0.0;
+            fetch(part_max, static_cast<int>(max_off), 1, 1, &mx);
+            *maxp_out = mx;
+        }
+    }
+
+    // partial-buffer offset reserved for polish maxima (after the pass partials)
+    size_t maxp_offset() const {
+        return static_cast<size_t>(std::max(grid, tile_grid)) * nchunks * 3 + 64;
+    }
 };
 
 // Build the projection sequence of one sweep: forward over `order`, backward
@@ -341,23 +530,16 @@ bool accel_phase(Solver& sv, AccelState& st, const std::vector<int>& seq, int ma
     if (max_iter <= 0 || seq.empty()) return false;
     const double kDiverge = 4.0, kBail = 1.5;
     const int cg_iter_cap = 1000, grand_every = 4;
-    const int first = seq[0];
     int last_check = -1;
-    double prev_norm = std::sqrt(sv.sumsq(st.cur, first));  // also starts the reduce of cur
+    double prev_norm = std::sqrt(sv.sumsq(st.cur, -1));
     int grand = 0;
     double best = prev_norm;
     bool suspended = false;
-    // gx = G(cur); start the reduce of gx for the ggx sweep
-    sv.sweep(st.cur, st.gx, seq, true, first, 0, nullptr, nullptr, nullptr);
+    sv.sweep2(SweepSrc(st.cur), st.gx, seq, 0, nullptr, nullptr, nullptr);  // gx = G(cur)
     ++sweeps;
-    bool pending_gx = true;  // sums[first] holds gx's reduce
-    auto drop_pending = [&]() {
-        if (pending_gx) { sv.clear_sums(first); pending_gx = false; }
-    };
     for (int iter = 0; iter < max_iter; ++iter) {
         double q[3];
-        sv.sweep(st.gx, st.ggx, seq, pending_gx, -1, 3, st.cur, st.gx, q);
-        pending_gx = false;
+        sv.sweep2(SweepSrc(st.gx), st.ggx, seq, 3, st.cur, st.gx, q);  // ggx = G(gx), IT stats
         ++sweeps;
         const double vprod = q[0], ssq = q[1], d1sq = q[2];
         if (honest && std::sqrt(d1sq) <= honest_tol * std::max(1.0, prev_norm)) {
@@ -373,11 +555,8 @@ bool accel_phase(Solver& sv, AccelState& st, const std::vector<int>& seq, int ma
         double coef = vprod / ssq;
         if (suspended) coef = 0.0;
         // cur = ggx - coef (ggx - gx); gx = G(cur) with its norm
-        sv.pass(nullptr, nullptr, st.ggx, st.gx, coef, st.cur, -1, first, 0, nullptr, nullptr,
-                nullptr);
         double sq = 0.0;
-        sv.sweep(st.cur, st.gx, seq, true, first, 1, nullptr, nullptr, &sq);
-        pending_gx = true;
+        sv.sweep2(SweepSrc(st.ggx, st.gx, coef, st.cur), st.gx, seq, 1, nullptr, nullptr, &sq);
         ++sweeps;
         {
             const double curr = std::sqrt(sq);
@@ -391,7 +570,6 @@ bool accel_phase(Solver& sv, AccelState& st, const std::vector<int>& seq, int ma
             } else {
                 if (curr > kBail * best) {
                     st.diverged = true;
-                    drop_pending();
                     std::swap(st.cur, st.gx);
                     iter_used = iter + 1;
                     return false;
@@ -402,7 +580,6 @@ bool accel_phase(Solver& sv, AccelState& st, const std::vector<int>& seq, int ma
                 }
             }
             if (!honest && step > 0 && rel_change / static_cast<double>(step) < conv_tol) {
-                drop_pending();
                 std::swap(st.cur, st.gx);
                 iter_used = iter + 1;
                 return true;
@@ -410,7 +587,6 @@ bool accel_phase(Solver& sv, AccelState& st, const std::vector<int>& seq, int ma
         }
         if (honest && cg_iter_cap > 0 && !st.diverged && iter + 1 >= cg_iter_cap) {
             st.diverged = true;
-            drop_pending();
             std::swap(st.cur, st.gx);
             iter_used = iter + 1;
             return false;
@@ -427,24 +603,20 @@ bool accel_phase(Solver& sv, AccelState& st, const std::vector<int>& seq, int ma
                                           sv.ws.partials.p, sv.grid, sv.s));
                 double g3[3];
                 sv.fetch(sv.ws.partials.p, sv.grid, 3, 0, g3);
-                drop_pending();
                 if (g3[1] != 0.0) {
+                    // cur = acc_c - ac (acc_c - acc_b); gx = G(cur)
                     const double ac = g3[0] / g3[1];
-                    // cur = acc_a' = acc_c - ac (acc_c - acc_b), and start its reduce
-                    sv.pass(nullptr, nullptr, st.acc_c, st.acc_b, ac, st.cur, -1, first, 0,
-                            nullptr, nullptr, nullptr);
+                    sv.sweep2(SweepSrc(st.acc_c, st.acc_b, ac, st.cur), st.gx, seq, 0, nullptr,
+                              nullptr, nullptr);
                 } else {
-                    sv.pass(st.acc_a, st.cur, nullptr, nullptr, 0.0, nullptr, -1, first, 0,
-                            nullptr, nullptr, nullptr);
+                    sv.copy(st.cur, st.acc_a);
+                    sv.sweep2(SweepSrc(st.cur), st.gx, seq, 0, nullptr, nullptr, nullptr);
                 }
-                sv.sweep(st.cur, st.gx, seq, true, first, 0, nullptr, nullptr, nullptr);
-                pending_gx = true;
                 ++sweeps;
                 grand = 0;
             }
         }
     }
-    drop_pending();
     std::swap(st.cur, st.gx);
     iter_used = max_iter;
     return false;
@@ -464,7 +636,7 @@ bool run_cg(Solver& sv, double* cur, double* r, double* u, double* v, const std:
         sv.fetch(sv.ws.partials.p, sv.grid, C, 0, out.data());
     };
     auto proj = [&](const double* src, double* dst) {
-        sv.sweep(src, dst, seq, false, -1, 0, nullptr, nullptr, nullptr);
+        sv.sweep2(SweepSrc(src), dst, seq, 0, nullptr, nullptr, nullptr);
         ++sweeps;
         XFE_CHECK(launch_sub(dst, src, dst, sv.n, C, sv.ld, sv.grid, sv.s));
     };
@@ -652,7 +824,7 @@ static void solve_core(Solver& sv, double* V, const std::vector<int>& order_in,
         int last_check = -1;
         for (int iter = 0; iter < opt.max_iter; ++iter) {
             double sq = 0.0;
-            sv.sweep(V, V, seq, false, -1, 1, nullptr, nullptr, &sq);
+            sv.sweep2(SweepSrc(V), V, seq, 1, nullptr, nullptr, &sq);
             ++sweeps;
             const double curr = std::sqrt(sq);
             const double rel = std::fabs(curr - prev) / std::max(1.0, prev);
@@ -679,21 +851,10 @@ static void solve_core(Solver& sv, double* V, const std::vector<int>& order_in,
         double final_max = 0.0;
         int done = 0;
         for (int p = 0; p < max_polish; ++p) {
-            // one sweep in place; every finalize writes its block maxima
-            const int m = static_cast<int>(seq.size());
-            size_t off = 0;
-            sv.pass(cur, nullptr, nullptr, nullptr, 0.0, nullptr, -1, seq[0], 0, nullptr, nullptr,
-                    nullptr);
-            for (int j = 0; j < m; ++j) {
-                sv.finalize(seq[j], ws.partials.p + off);
-                off += sv.finalize_grid(seq[j]);
-                const int r = j + 1 < m ? seq[j + 1] : -1;
-                sv.pass(cur, cur, nullptr, nullptr, 0.0, nullptr, seq[j], r, 0, nullptr, nullptr,
-                        nullptr);
-            }
-            ++sweeps;
+            // one sweep in place; the largest |mean| over its projections
             double mx = 0.0;
-            sv.fetch(ws.partials.p, static_cast<int>(off), 1, 1, &mx);
+            sv.sweep2(SweepSrc(cur), cur, seq, 0, nullptr, nullptr, nullptr, &mx);
+            ++sweeps;
             final_max = mx;
             ++done;
             if (mx <= polish_tol) break;
@@ -751,6 +912,75 @@ static std::vector<int> select_sweep_order_device(Workspace& ws, const double* d
         }
     } while (std::next_permutation(perm.begin(), perm.end()));
     return order;
+}
+
+// Level-aligned tiles of the sorted dimension p (tile_kernels.hip) and, for
+// every other dimension too large for LDS, its spells: maximal runs of equal
+// level inside a level of p, with their order by level for the pull.
+static void setup_tiles(Solver& sv, Workspace& ws, int n, int D, const std::vector<int>& G) {
+    sv.tiled = false;
+    const int p = sv.sorted_dim;
+    if (p < 0 || n < kTileRows || std::getenv("XFE_NO_TILES")) return;
+    hipStream_t s = ws.stream;
+    const int grid = grid_for(n);
+    ws.level_count.ensure(static_cast<size_t>(G[p]) + 1);
+    XFE_CHECK(hipMemsetAsync(ws.level_count.p, 0, sizeof(int) * (static_cast<size_t>(G[p]) + 1), s));
+    XFE_CHECK(launch_level_count(sv.gid[p], n, ws.level_count.p, grid, s));
+    XFE_CHECK(launch_max_int(ws.level_count.p, G[p], ws.level_count.p + G[p], grid_for(G[p]), s));
+    int lmax = 0;
+    XFE_CHECK(hipMemcpyAsync(&lmax, ws.level_count.p + G[p], sizeof(int), hipMemcpyDeviceToHost, s));
+    XFE_CHECK(hipStreamSynchronize(s));
+    if (lmax <= 0 || lmax > kTileRows / 2) return;  // a level too large for one tile
+    const int step = kTileRows - lmax;
+    const int tiles = static_cast<int>((static_cast<long long>(n) + step - 1) / step);
+    ws.tile_start.ensure(static_cast<size_t>(tiles) + 1);
+    XFE_CHECK(launch_tile_start(sv.gid[p], n, step, tiles, ws.tile_start.p, grid_for(tiles + 1), s));
+    sv.tile_start = ws.tile_start.p;
+    sv.num_tiles = tiles;
+    sv.tile_grid = std::max(1, std::min(tiles, 2048));
+    for (int d = 0; d < D; ++d) {
+        if (d == p || sv.tile_mode_of(d) != kRedSpell) continue;
+        // spell heads -> ids -> keys; sort spells by level of d
+        ws.tmp_flags.ensure(n);
+        ws.tmp_inc.ensure(n);
+        XFE_CHECK(launch_spell_heads(sv.gid[p], sv.gid[d], n, ws.tmp_flags.p, grid, s));
+        size_t bytes = 0;
+        XFE_CHECK(rocprim::inclusive_scan(nullptr, bytes, ws.tmp_flags.p, ws.tmp_inc.p,
+                                          static_cast<size_t>(n), rocprim::plus<int>(), s));
+        ws.sort_tmp.ensure(bytes);
+        XFE_CHECK(rocprim::inclusive_scan(ws.sort_tmp.p, bytes, ws.tmp_flags.p, ws.tmp_inc.p,
+                                          static_cast<size_t>(n), rocprim::plus<int>(), s));
+        int M = 0;
+        XFE_CHECK(hipMemcpyAsync(&M, ws.tmp_inc.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        ws.tmp_key.ensure(M);
+        ws.tmp_iota.ensure(M);
+        ws.sp_key[d].ensure(M);
+        ws.sp_order[d].ensure(M);
+        ws.sp_base[d].ensure(tiles);
+        ws.sp_sums[d].ensure(static_cast<size_t>(M) * sv.C);
+        XFE_CHECK(launch_spell_keys(ws.tmp_flags.p, ws.tmp_inc.p, sv.gid[d], n, ws.tmp_key.p, grid, s));
+        XFE_CHECK(launch_tile_spell_base(ws.tile_start.p, tiles, ws.tmp_inc.p, n, ws.sp_base[d].p,
+                                         grid_for(tiles), s));
+        XFE_CHECK(launch_iota(ws.tmp_iota.p, M, grid_for(M), s));
+        int bits = 1;
+        while ((1LL << bits) < static_cast<long long>(G[d])) ++bits;
+        bytes = 0;
+        XFE_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, ws.tmp_key.p, ws.sp_key[d].p,
+                                            ws.tmp_iota.p, ws.sp_order[d].p, static_cast<size_t>(M),
+                                            0, bits, s));
+        ws.sort_tmp.ensure(bytes);
+        XFE_CHECK(rocprim::radix_sort_pairs(ws.sort_tmp.p, bytes, ws.tmp_key.p, ws.sp_key[d].p,
+                                            ws.tmp_iota.p, ws.sp_order[d].p, static_cast<size_t>(M),
+                                            0, bits, s));
+        sv.spell[d].M = M;
+        sv.spell[d].S = ws.sp_sums[d].p;
+        sv.spell[d].order = ws.sp_order[d].p;
+        sv.spell[d].key = ws.sp_key[d].p;
+        sv.spell[d].base = ws.sp_base[d].p;
+    }
+    XFE_CHECK(hipStreamSynchronize(s));
+    sv.tiled = true;
 }
 
 // Shared driver for both entry points.  d_Vin: caller's n x C (ld_in) block,
@@ -835,10 +1065,14 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
         XFE_CHECK(hipStreamSynchronize(s));
         profs[d] = profile_of(hw, w == nullptr);
     }
-    // partial buffer: enough for chunked passes and polish maxima over a sweep
-    const size_t pg = static_cast<size_t>(std::max(sv.grid, grid));
-    size_t part = pg * ((C + 7) / 8) * 3 + pg * C;
-    for (int d = 0; d < D; ++d) part += 2 * static_cast<size_t>(grid_for(static_cast<long long>(G[d]) * C));
+    setup_tiles(sv, ws, n, D, G);
+    // partial buffer: pass partials (3 per block and chunk), per-column CG dots,
+    // then the polish maxima of one sweep (<= 2D projections)
+    const size_t pg = static_cast<size_t>(std::max(std::max(sv.grid, grid), sv.tile_grid));
+    size_t fgmax = 0;
+    for (int d = 0; d < D; ++d)
+        fgmax = std::max(fgmax, static_cast<size_t>(grid_for(static_cast<long long>(G[d]) * C)));
+    const size_t part = sv.maxp_offset() + pg * C + 2 * static_cast<size_t>(D) * (pg * sv.nchunks + fgmax);
     ws.partials.ensure(part + 1024);
     ws.scalars.ensure(std::max(C, 8));
     ws.lanes.ensure(std::max(C, 8));
