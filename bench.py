@@ -27,6 +27,20 @@ for _p in (ROOT, os.path.join(ROOT, "xhdfe-xfe_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tile_pass_akm56m.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known size),
+    or None when it is absent."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -182,8 +196,9 @@ def main():
         "iterations": info["iterations"],
         "sweeps": info["sweeps"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "pass_kernel (fused apply+reduce sweep pass)",
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(),
+                     "kernel": "tile_pass_kernel (fused apply / in-tile projection / reduce pass)",
                      "avg_launch_ms": round(pass_ms / max(launches, 1), 4),
                      "bytes_per_launch": round(pass_bytes / max(launches, 1), 1)},
         "cpu_baseline": cpu,
