@@ -254,3 +254,37 @@ def test_device_entry_point_matches_host_entry_point():
     scale = np.abs(V).max()
     assert info["converged"]
     assert max(_rel(out[:, 0], yg, scale), _rel(out[:, 1:], Xg, scale)) <= 1e-12
+
+
+# ---------------------------------------------------------------- mixed precision
+# The FP32 Schur-complement CG under FP64 refinement converges to the
+# projection itself; it is compared with the oracle run to tol 1e-12 (error
+# bound: 1e-9 of the column scale) -- a tighter fixed point than either method
+# reaches at the default tolerance.
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_mixed_precision_matches_tight_oracle(cases, name):
+    _log(f"mixed precision {name}")
+    p = cases[name]
+    _, fes, gids, G, V = _prep(p)
+    Vo, _ = oracle.absorb(V, gids, G, tol=1e-12)
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=1)
+    scale = np.abs(V).max()
+    err = max(_rel(yg, Vo[:, 0], scale), _rel(Xg, Vo[:, 1:], scale))
+    _log(f"  inner iterations {info['iterations']} refinements {info['refinements']} err {err:.2e}")
+    assert info["converged"] and info["refinements"] >= 1
+    assert err <= 1e-9
+
+
+@pytest.mark.parametrize("which", ["config0", "akm_1pct"])
+def test_mixed_precision_spell_path(which):
+    _log(f"mixed precision {which}")
+    p = panels.config0() if which == "config0" else panels.akm_small(scale=0.01)
+    _, fes, gids, G, V = _prep(p)
+    Vo, _ = oracle.absorb(V, gids, G, tol=1e-12)
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=1)
+    scale = np.abs(V).max()
+    err = max(_rel(yg, Vo[:, 0], scale), _rel(Xg, Vo[:, 1:], scale))
+    _log(f"  inner iterations {info['iterations']} refinements {info['refinements']} err {err:.2e}")
+    assert info["converged"]
+    assert err <= 1e-9

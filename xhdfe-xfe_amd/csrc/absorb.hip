@@ -76,6 +76,10 @@ struct Workspace {
     DevBuf<int> tile_start, level_count, tmp_flags, tmp_inc, tmp_iota, tmp_key;
     DevBuf<int> sp_key[XFE_MAX_DIMS], sp_order[XFE_MAX_DIMS], sp_base[XFE_MAX_DIMS];
     DevBuf<double> sp_sums[XFE_MAX_DIMS];
+    // mixed-precision solver (coefficient space of the non-sorted dimensions)
+    DevBuf<double> f_beta, f_b, f_diag, f_counts, f_scale;
+    DevBuf<float> f_x, f_r, f_z, f_p, f_q, f_minv, f_invp;
+    DevBuf<float> f_sp[XFE_MAX_DIMS];
     double* host_scalars = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -93,7 +97,11 @@ struct Workspace {
         tmp_iota.release(); tmp_key.release();
         for (int d = 0; d < XFE_MAX_DIMS; ++d) {
             sp_key[d].release(); sp_order[d].release(); sp_base[d].release(); sp_sums[d].release();
+            f_sp[d].release();
         }
+        f_beta.release(); f_b.release(); f_diag.release(); f_counts.release(); f_scale.release();
+        f_x.release(); f_r.release(); f_z.release(); f_p.release(); f_q.release(); f_minv.release();
+        f_invp.release();
         if (host_scalars) (void)hipHostFree(host_scalars);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -704,6 +712,271 @@ bool run_cg(Solver& sv, double* cur, double* r, double* u, double* v, const std:
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Mixed-precision solver (xfe_options.precision = 1).
+//
+// Eliminates the level-sorted dimension p exactly (per-level means inside the
+// tiles) and solves the Schur system S x = b over the coefficients of the other
+// dimensions R by Jacobi-preconditioned CG in FP32 (FP64 dot products, step
+// lengths and CG scalars), inside FP64 iterative refinement: the right-hand side
+// b = D_R' M_p (v - D_R beta) is recomputed from the FP64 data after every inner
+// solve, beta accumulates in FP64, and the final within transform
+// M_p (v - D_R beta) is formed in FP64.  Convergence is judged only on the FP64
+// reduced residual.  Applicable with 2 or 3 dimensions, unit weights, tiles and
+// at most kMaxCgCols columns; returns false otherwise (the caller then runs the
+// plain FP64 method).
+struct FastSolver {
+    Solver& sv;
+    Workspace& ws;
+    hipStream_t s;
+    int p = -1, NR = 0, N = 0, C = 0;
+    int rdim[2] = {-1, -1};
+    int off[2] = {0, 0};
+    int mode[2] = {0, 0};
+    size_t lds_extra32 = 0, lds_extra64 = 0;
+    int lds_off[2] = {0, 0};
+    int vgrid = 1;
+    int inner_total = 0;
+    double pull_apply_ms = 0.0;
+
+    explicit FastSolver(Solver& sv_) : sv(sv_), ws(sv_.ws), s(sv_.s) {}
+
+    bool setup() {
+        if (!sv.tiled || sv.w != nullptr || sv.D < 2 || sv.D > 3 || sv.C > kMaxCgCols) return false;
+        p = sv.sorted_dim;
+        C = sv.C;
+        NR = 0;
+        N = 0;
+        int lo32 = 0, lo64 = 0;
+        for (int d = 0; d < sv.D; ++d) {
+            if (d == p) continue;
+            rdim[NR] = d;
+            off[NR] = N;
+            N += sv.G[d];
+            const int cc = std::min(8, C);
+            const bool lds = static_cast<size_t>(sv.G[d]) * cc * sizeof(double) <= kLdsBudget;
+            mode[NR] = lds ? kRedLds : kRedSpell;
+            if (!lds && sv.spell[d].S == nullptr) return false;
+            if (lds) {
+                lds_off[NR] = lo64;  // same element offset for both precisions
+                lo32 += sv.G[d] * cc;
+                lo64 += sv.G[d] * cc;
+            }
+            ++NR;
+        }
+        lds_extra32 = static_cast<size_t>(lo32) * sizeof(float);
+        lds_extra64 = static_cast<size_t>(lo64) * sizeof(double);
+        const size_t NC = static_cast<size_t>(N) * C;
+        ws.f_beta.ensure(NC); ws.f_b.ensure(NC);
+        ws.f_diag.ensure(N); ws.f_counts.ensure(N); ws.f_scale.ensure(2 * kMaxCgCols);
+        ws.f_x.ensure(NC); ws.f_r.ensure(NC); ws.f_z.ensure(NC); ws.f_p.ensure(NC); ws.f_q.ensure(NC);
+        ws.f_minv.ensure(N); ws.f_invp.ensure(sv.G[p]);
+        for (int k = 0; k < NR; ++k)
+            if (mode[k] == kRedSpell) ws.f_sp[rdim[k]].ensure(static_cast<size_t>(sv.spell[rdim[k]].M) * C);
+        vgrid = std::max(1, std::min(2048, grid_for(N)));
+        ws.partials.ensure(static_cast<size_t>(std::max(vgrid, sv.tile_grid)) * 2 * kMaxCgCols + 64);
+        ws.scalars.ensure(2 * kMaxCgCols);
+        XFE_CHECK(launch_to_float(ws.inv[p].p, ws.f_invp.p, sv.G[p], grid_for(sv.G[p]), s));
+        // Jacobi preconditioner: diag(S) from runs of each R level inside levels of p
+        XFE_CHECK(hipMemsetAsync(ws.f_diag.p, 0, sizeof(double) * N, s));
+        for (int k = 0; k < NR; ++k) {
+            XFE_CHECK(launch_schur_diag(sv.tile_start, sv.num_tiles, sv.gid[p], sv.gid[rdim[k]],
+                                        ws.inv[p].p, ws.f_diag.p + off[k], sv.tile_grid, s));
+            XFE_CHECK(hipMemcpyAsync(ws.f_counts.p + off[k], ws.wsum[rdim[k]].p,
+                                     sizeof(double) * sv.G[rdim[k]], hipMemcpyDeviceToDevice, s));
+        }
+        XFE_CHECK(launch_diag_invert(ws.f_diag.p, ws.f_counts.p, ws.f_minv.p, N, grid_for(N), s));
+        XFE_CHECK(hipMemsetAsync(ws.f_beta.p, 0, sizeof(double) * NC, s));
+        return true;
+    }
+
+    template <typename T>
+    void schur_pass(const T* coef, T* out, const T* V, T* dst, T sign, const T* inv_p,
+                    T* const* spell_sums, size_t lds_extra, double* partials) {
+        XFE_CHECK(hipMemsetAsync(out, 0, sizeof(T) * static_cast<size_t>(N) * C, s));
+        for (int ch = 0; ch < sv.nchunks; ++ch) {
+            const int c0 = ch * 8;
+            const int cc = std::min(8, C - c0);
+            SchurArgs<T> a;
+            a.n = sv.n; a.ld = sv.ld; a.C = C; a.c0 = c0;
+            a.V = V; a.dst = dst;
+            a.tile_start = sv.tile_start; a.num_tiles = sv.num_tiles;
+            a.gid_p = sv.gid[p]; a.inv_p = inv_p; a.sign = sign;
+            for (int k = 0; k < NR; ++k) {
+                const int d = rdim[k];
+                a.gid_r[k] = sv.gid[d];
+                a.coef_r[k] = coef + static_cast<size_t>(off[k]) * C;
+                a.G_r[k] = sv.G[d];
+                a.mode_r[k] = mode[k];
+                a.spell_sums[k] = spell_sums[k];
+                a.spell_base[k] = sv.spell[d].base;
+                a.lds_target[k] = out + static_cast<size_t>(off[k]) * C;
+                a.lds_off[k] = lds_off[k] / 8 * cc;  // element offset for this chunk width
+            }
+            a.partials = partials ? partials + static_cast<size_t>(ch) * sv.tile_grid : nullptr;
+            // LDS offsets were laid out for full 8-wide chunks; recompute for cc
+            int lo = 0;
+            for (int k = 0; k < NR; ++k) {
+                if (mode[k] == kRedLds) { a.lds_off[k] = lo; lo += sv.G[rdim[k]] * cc; }
+            }
+            const size_t extra = static_cast<size_t>(lo) * sizeof(T);
+            (void)lds_extra;
+            if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            XFE_CHECK(launch_schur<T>(a, cc, NR, V != nullptr, dst != nullptr, extra, sv.tile_grid, s));
+            if (g_profile.enabled) {
+                XFE_CHECK(hipEventRecord(ws.ev1, s));
+                XFE_CHECK(hipEventSynchronize(ws.ev1));
+                float ms = 0.f;
+                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+                g_profile.ms += ms;
+                g_profile.launches += 1;
+                const double e = sizeof(T);
+                double bytes = 4.0 * sv.n * (1 + NR);                     // ids
+                if (V) bytes += e * static_cast<double>(sv.n) * cc;       // data read
+                if (dst) bytes += e * static_cast<double>(sv.n) * cc;     // data write
+                for (int k = 0; k < NR; ++k) {
+                    bytes += e * static_cast<double>(sv.G[rdim[k]]) * cc;  // coefficient gather
+                    if (mode[k] == kRedSpell) bytes += e * static_cast<double>(sv.spell[rdim[k]].M) * cc;
+                }
+                bytes += e * static_cast<double>(sv.G[p]);                // 1 / level sizes of p
+                g_profile.bytes += bytes;
+            }
+        }
+        for (int k = 0; k < NR; ++k) {
+            if (mode[k] != kRedSpell) continue;
+            const auto& sp = sv.spell[rdim[k]];
+            for (int ch = 0; ch < sv.nchunks; ++ch) {
+                const int c0 = ch * 8;
+                const int cc = std::min(8, C - c0);
+                XFE_CHECK(launch_pull<T>(spell_sums[k], C, c0, cc, sp.order, sp.key, sp.M,
+                                         out + static_cast<size_t>(off[k]) * C, grid_for(sp.M), s));
+            }
+        }
+    }
+
+    void apply32(const float* x, float* q) {
+        float* sp[2] = {nullptr, nullptr};
+        for (int k = 0; k < NR; ++k) if (mode[k] == kRedSpell) sp[k] = ws.f_sp[rdim[k]].p;
+        schur_pass<float>(x, q, nullptr, nullptr, 1.0f, ws.f_invp.p, sp, lds_extra32, nullptr);
+    }
+
+    // b = D_R' M_p (V - D_R beta) in FP64; with dst, also writes M_p (V - D_R beta).
+    double residual64(double* V, double* dst) {
+        double* sp[2] = {nullptr, nullptr};
+        for (int k = 0; k < NR; ++k) if (mode[k] == kRedSpell) sp[k] = ws.sp_sums[rdim[k]].p;
+        schur_pass<double>(ws.f_beta.p, ws.f_b.p, V, dst, -1.0, ws.inv[p].p, sp, lds_extra64,
+                           dst ? ws.partials.p : nullptr);
+        double sq = 0.0;
+        if (dst) sv.fetch(ws.partials.p, sv.tile_grid * sv.nchunks, 1, 0, &sq);
+        return sq;
+    }
+
+    template <typename T>
+    void coldot(const T* a, const T* b, double* out) {
+        XFE_CHECK(launch_vdot<T>(a, b, N, C, ws.partials.p, vgrid, s));
+        sv.fetch(ws.partials.p, vgrid, C, 0, out);
+    }
+
+    void upload(const double* h, double* d, int cnt) {
+        for (int j = 0; j < cnt; ++j) ws.host_scalars[j] = h[j];
+        XFE_CHECK(hipMemcpyAsync(d, ws.host_scalars, sizeof(double) * cnt, hipMemcpyHostToDevice, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+    }
+
+    // One inner FP32 PCG solve of S x = b / |b| per column; beta64 += |b| x.
+    int inner_solve(double tol_in, int max_inner) {
+        double bb[kMaxCgCols], scale[2 * kMaxCgCols], rz[kMaxCgCols], rr[2 * kMaxCgCols],
+            pq[kMaxCgCols], alpha[kMaxCgCols], beta[kMaxCgCols];
+        bool active[kMaxCgCols];
+        coldot<double>(ws.f_b.p, ws.f_b.p, bb);
+        for (int c = 0; c < C; ++c) {
+            const double nb = std::sqrt(bb[c]);
+            scale[c] = nb > 0.0 ? 1.0 / nb : 0.0;        // rhs scaling into FP32
+            scale[kMaxCgCols + c] = nb;                  // and back
+            active[c] = nb > 0.0;
+        }
+        upload(scale, ws.f_scale.p, 2 * kMaxCgCols);
+        XFE_CHECK(launch_cg_init(ws.f_b.p, ws.f_scale.p, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p,
+                                 ws.f_p.p, N, C, ws.partials.p, vgrid, s));
+        sv.fetch(ws.partials.p, vgrid, C, 0, rz);
+        int it = 0;
+        for (it = 1; it <= max_inner; ++it) {
+            apply32(ws.f_p.p, ws.f_q.p);
+            coldot<float>(ws.f_p.p, ws.f_q.p, pq);
+            bool any = false;
+            for (int c = 0; c < C; ++c) {
+                alpha[c] = 0.0;
+                if (active[c] && pq[c] > 0.0 && rz[c] > 0.0) alpha[c] = rz[c] / pq[c];
+                else active[c] = false;
+            }
+            for (int j = 0; j < C; ++j) ws.host_scalars[j] = alpha[j];
+            XFE_CHECK(hipMemcpyAsync(ws.lanes.p, ws.host_scalars, sizeof(double) * C,
+                                     hipMemcpyHostToDevice, s));
+            XFE_CHECK(launch_cg_step(ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p, ws.f_q.p, ws.f_minv.p,
+                                     ws.lanes.p, N, C, ws.partials.p, vgrid, s));
+            sv.fetch(ws.partials.p, vgrid, 2 * C, 0, rr);
+            for (int c = 0; c < C; ++c) {
+                if (!active[c]) { beta[c] = 0.0; continue; }
+                if (rr[c] <= tol_in * tol_in) { active[c] = false; beta[c] = 0.0; continue; }
+                beta[c] = rr[C + c] / rz[c];
+                rz[c] = rr[C + c];
+                any = true;
+            }
+            if (!any) break;
+            for (int j = 0; j < C; ++j) ws.host_scalars[j] = beta[j];
+            XFE_CHECK(hipMemcpyAsync(ws.lanes.p, ws.host_scalars, sizeof(double) * C,
+                                     hipMemcpyHostToDevice, s));
+            XFE_CHECK(launch_cg_dir(ws.f_p.p, ws.f_z.p, ws.lanes.p, N, C, grid_for(static_cast<long long>(N) * C), s));
+        }
+        XFE_CHECK(launch_refine_update(ws.f_beta.p, ws.f_x.p, ws.f_scale.p + kMaxCgCols, N, C,
+                                       grid_for(static_cast<long long>(N) * C), s));
+        inner_total += std::min(it, max_inner);
+        return it;
+    }
+};
+
+static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb_result& res) {
+    FastSolver fs(sv);
+    if (!fs.setup()) return false;
+    const int C = sv.C;
+    const double target = std::getenv("XFE_FAST_TARGET") ? std::atof(std::getenv("XFE_FAST_TARGET")) : 1e-11;
+    const double tol_in = std::getenv("XFE_FAST_INNER") ? std::atof(std::getenv("XFE_FAST_INNER")) : 1e-5;
+    const int max_inner = std::max(10, std::min(opt.max_iter, 5000));
+    double b0[kMaxCgCols], bb[kMaxCgCols];
+    fs.residual64(V, nullptr);
+    fs.coldot<double>(sv.ws.f_b.p, sv.ws.f_b.p, b0);
+    int refinements = 0;
+    bool converged = false;
+    double prev_rel = INFINITY;
+    for (int ref = 0; ref < 8; ++ref) {
+        fs.coldot<double>(sv.ws.f_b.p, sv.ws.f_b.p, bb);
+        double rel = 0.0;
+        for (int c = 0; c < C; ++c)
+            if (b0[c] > 0.0) rel = std::max(rel, std::sqrt(bb[c] / b0[c]));
+        if (rel <= target) { converged = true; break; }
+        if (rel > 0.5 * prev_rel) break;  // refinement no longer gains: accept what we have
+        prev_rel = rel;
+        fs.inner_solve(tol_in, max_inner);
+        ++refinements;
+        fs.residual64(V, nullptr);
+    }
+    if (!converged) {
+        fs.coldot<double>(sv.ws.f_b.p, sv.ws.f_b.p, bb);
+        double rel = 0.0;
+        for (int c = 0; c < C; ++c)
+            if (b0[c] > 0.0) rel = std::max(rel, std::sqrt(bb[c] / b0[c]));
+        converged = rel <= std::max(target, 1e-9);
+    }
+    fs.residual64(V, V);  // final within transform, in place
+    res.iterations = fs.inner_total;
+    res.refinements = refinements;
+    res.converged = converged ? 1 : 0;
+    res.used_accel = 0;
+    res.used_cg = 1;
+    res.sweeps = 2 * refinements + 2;
+    return true;
+}
+
 // Core solve on device buffers already in the workspace layout.  V: n x C,
 // ld; gids[d] already permuted; ws.wsum/inv filled.  Returns 0 or throws.
 static void solve_core(Solver& sv, double* V, const std::vector<int>& order_in,
@@ -713,6 +986,8 @@ static void solve_core(Solver& sv, double* V, const std::vector<int>& order_in,
     const int n = sv.n, C = sv.C, ld = sv.ld, D = sv.D;
     const double* w = sv.w;
     std::vector<int> G(sv.G, sv.G + D);
+    for (int d = 0; d < D; ++d) res.sweep_order_used[d] = order_in[d];
+    if (opt.precision == 1 && solve_fast(sv, V, opt, res)) return;
     const bool honest = opt.tolerance_mode == 1;
     const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
     std::vector<int> order = order_in;

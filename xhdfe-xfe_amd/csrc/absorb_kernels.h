@@ -85,6 +85,58 @@ hipError_t launch_spell_keys(const int* flags, const int* inc, const int* gr, in
 hipError_t launch_tile_spell_base(const int* tile_start, int num_tiles, const int* inc, int n,
                                   int* base, int grid, hipStream_t s);
 
+// Arguments of the Schur-complement pass of the mixed-precision solver
+// (fast_kernels.hip).  T = float for the inner CG, double for the refinement.
+constexpr int kMaxCgCols = 32;  // columns solved together by the mixed-precision path
+template <typename T>
+struct SchurArgs {
+    int n = 0;
+    int ld = 0;
+    int C = 0;
+    int c0 = 0;
+    const T* V = nullptr;   // data block (refinement residual / final transform)
+    T* dst = nullptr;       // final transform output
+    const int* tile_start = nullptr;
+    int num_tiles = 0;
+    const int* gid_p = nullptr;
+    const T* inv_p = nullptr;
+    T sign = T(1);
+    const int* gid_r[2] = {nullptr, nullptr};
+    const T* coef_r[2] = {nullptr, nullptr};  // level-major G_r x C coefficients
+    int G_r[2] = {0, 0};
+    int mode_r[2] = {0, 0};                   // kRedSpell or kRedLds
+    T* spell_sums[2] = {nullptr, nullptr};
+    const int* spell_base[2] = {nullptr, nullptr};
+    T* lds_target[2] = {nullptr, nullptr};
+    int lds_off[2] = {0, 0};
+    double* partials = nullptr;               // STORE: per-block sum of squares
+};
+
+template <typename T>
+hipError_t launch_schur(const SchurArgs<T>& a, int cc, int nr, bool has_v, bool store,
+                        size_t lds_extra, int grid, hipStream_t s);
+template <typename T>
+hipError_t launch_pull(const T* S, int C, int c0, int cc, const int* order, const int* skey, int M,
+                       T* sums, int grid, hipStream_t s);
+template <typename T>
+hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, int grid, hipStream_t s);
+hipError_t launch_cg_step(float* x, float* r, float* z, const float* p, const float* q,
+                          const float* minv, const double* alpha, int N, int C, double* partials,
+                          int grid, hipStream_t s);
+hipError_t launch_cg_dir(float* p, const float* z, const double* beta, int N, int C, int grid,
+                         hipStream_t s);
+hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x,
+                          float* r, float* z, float* p, int N, int C, double* partials, int grid,
+                          hipStream_t s);
+hipError_t launch_refine_update(double* beta, const float* x, const double* inv_scale, int N, int C,
+                                int grid, hipStream_t s);
+hipError_t launch_schur_diag(const int* tile_start, int num_tiles, const int* gid_p,
+                             const int* gid_r, const double* inv_p, double* diag, int grid,
+                             hipStream_t s);
+hipError_t launch_diag_invert(const double* diag, const double* counts, float* minv, int N,
+                              int grid, hipStream_t s);
+hipError_t launch_to_float(const double* a, float* b, long long m, int grid, hipStream_t s);
+
 hipError_t launch_pass(const PassArgs& a, int cc, int rmode, int nq, int grid, hipStream_t s);
 hipError_t launch_finalize(double* sums, double* means, const double* inv, int G, int C,
                            int copies, double* partials, int grid, hipStream_t s);

@@ -1,0 +1,559 @@
+// Kernels of the mixed-precision solver (precision 1): preconditioned CG in
+// coefficient space on the Schur complement of the level-sorted dimension p.
+//
+// With p the dimension with the most levels (workers) and R the others
+// (firms, years), eliminating p exactly leaves the system
+//     S x = D_R' M_p D_R x = b,   M_p = I - D_p (D_p' D_p)^-1 D_p'
+// over the coefficients x of R only.  One application of S is one pass over
+// the row ids with no n x C data: per row gather the coefficients of R and add
+// them, subtract the mean over the row's level of p (inside the level-aligned
+// tile), reduce into the levels of R (spells for firms, LDS for years).  The
+// same kernel in FP64 with the data block V and sign -1 is the refinement
+// residual  u = M_p (v - D_R beta)  and its reduce b = D_R' u, and (STORE)
+// the final within transform.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "absorb_kernels.h"
+#include "wave_ops.h"
+
+namespace xfe {
+
+namespace {
+
+constexpr int kSlots = kTileRows / kBlock;
+constexpr int kChunks = kSlots * (kBlock / kWave);
+
+__device__ __forceinline__ int tile_rank2(const int* cnt, int chunk, uint64_t mask) {
+    int base = 0;
+    for (int k = 0; k < chunk; ++k) base += cnt[k];
+    const int lane = lane_id();
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    return base + __popcll(mask & upto) - 1;
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_down_t(T v, int off) { return __shfl_down(v, off, kWave); }
+
+template <typename T>
+__device__ __forceinline__ T run_sum_t(T v, int end) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const T o = shfl_down_t(v, off);
+        if (lane + off <= end) v += o;
+    }
+    return v;
+}
+
+}  // namespace
+
+template <typename T, int CC, int NR, bool HAS_V, bool STORE>
+__global__ void __launch_bounds__(kBlock) schur_kernel(SchurArgs<T> a) {
+    extern __shared__ unsigned char dyn[];
+    T* sm = reinterpret_cast<T*>(dyn);                 // kTileRows x CC
+    T* lr = sm + kTileRows * CC;                       // LDS-mode sums of the R dims
+    __shared__ int cnt[kChunks];
+    __shared__ int cnt_total;
+    const size_t ld = static_cast<size_t>(a.ld);
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    for (int k = 0; k < NR; ++k) {
+        if (a.mode_r[k] == kRedLds) {
+            for (int j = threadIdx.x; j < a.G_r[k] * CC; j += kBlock) lr[a.lds_off[k] + j] = T(0);
+        }
+    }
+    double q = 0.0;
+    for (int t = blockIdx.x; t < a.num_tiles; t += gridDim.x) {
+        const int t0 = a.tile_start[t];
+        const int t1 = a.tile_start[t + 1];
+        if (t0 >= t1) continue;
+        T u[kSlots][CC];
+        int kp[kSlots], kr[NR][kSlots];
+        bool valid[kSlots];
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+            const int i = t0 + j * kBlock + threadIdx.x;
+            valid[j] = i < t1;
+            kp[j] = -1 - lane;
+#pragma unroll
+            for (int k = 0; k < NR; ++k) kr[k][j] = -1 - lane;
+#pragma unroll
+            for (int c = 0; c < CC; ++c) u[j][c] = T(0);
+            if (valid[j]) {
+                kp[j] = a.gid_p[i];
+                if constexpr (HAS_V) {
+#pragma unroll
+                    for (int c = 0; c < CC; ++c) u[j][c] = a.V[(a.c0 + c) * ld + i];
+                }
+#pragma unroll
+                for (int k = 0; k < NR; ++k) {
+                    kr[k][j] = a.gid_r[k][i];
+                    const T* cf = a.coef_r[k] + static_cast<size_t>(kr[k][j]) * a.C + a.c0;
+#pragma unroll
+                    for (int c = 0; c < CC; ++c) u[j][c] += a.sign * cf[c];
+                }
+            }
+        }
+        int pp[kSlots];
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+            pp[j] = __shfl_up(kp[j], 1, kWave);
+            const int i = t0 + j * kBlock + threadIdx.x;
+            if (lane == 0 && valid[j]) pp[j] = i > t0 ? a.gid_p[i - 1] : INT_MIN;
+        }
+        // ---- demean over the levels of p inside the tile
+        for (int k = threadIdx.x; k < kTileRows * CC; k += kBlock) sm[k] = T(0);
+        uint64_t mask[kSlots];
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+            mask[j] = __ballot(valid[j] && pp[j] != kp[j]);
+            if (lane == 0) cnt[j * (kBlock / kWave) + wave] = __popcll(mask[j]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+            const int rank = tile_rank2(cnt, j * (kBlock / kWave) + wave, mask[j]);
+            const int end = run_end_lane(kp[j]);
+            const bool seg = run_head(kp[j]) && valid[j];
+#pragma unroll
+            for (int c = 0; c < CC; ++c) {
+                const T s = run_sum_t(u[j][c], end);
+                if (seg) atomicAdd(&sm[rank * CC + c], s);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+            if (valid[j]) {
+                const int rank = tile_rank2(cnt, j * (kBlock / kWave) + wave, mask[j]);
+                const T inv = a.inv_p[kp[j]];
+#pragma unroll
+                for (int c = 0; c < CC; ++c) u[j][c] -= sm[rank * CC + c] * inv;
+            }
+        }
+        __syncthreads();
+        if constexpr (STORE) {
+#pragma unroll
+            for (int j = 0; j < kSlots; ++j) {
+                if (!valid[j]) continue;
+                const int i = t0 + j * kBlock + threadIdx.x;
+#pragma unroll
+                for (int c = 0; c < CC; ++c) {
+                    a.dst[(a.c0 + c) * ld + i] = u[j][c];
+                    q += static_cast<double>(u[j][c]) * static_cast<double>(u[j][c]);
+                }
+            }
+        }
+        // ---- reduce into the levels of each R dimension
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            if (a.mode_r[k] == kRedSpell) {
+                for (int m = threadIdx.x; m < kTileRows * CC; m += kBlock) sm[m] = T(0);
+                int pr[kSlots];
+                uint64_t smask[kSlots];
+#pragma unroll
+                for (int j = 0; j < kSlots; ++j) {
+                    pr[j] = __shfl_up(kr[k][j], 1, kWave);
+                    const int i = t0 + j * kBlock + threadIdx.x;
+                    if (lane == 0 && valid[j]) pr[j] = i > t0 ? a.gid_r[k][i - 1] : INT_MIN;
+                    smask[j] = __ballot(valid[j] && (pp[j] != kp[j] || pr[j] != kr[k][j]));
+                    if (lane == 0) cnt[j * (kBlock / kWave) + wave] = __popcll(smask[j]);
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    int tot = 0;
+                    for (int m = 0; m < kChunks; ++m) tot += cnt[m];
+                    cnt_total = tot;
+                }
+#pragma unroll
+                for (int j = 0; j < kSlots; ++j) {
+                    const int rank = tile_rank2(cnt, j * (kBlock / kWave) + wave, smask[j]);
+                    const int end = run_end_lane2(kp[j], kr[k][j]);
+                    const bool seg = run_head2(kp[j], kr[k][j]) && valid[j];
+#pragma unroll
+                    for (int c = 0; c < CC; ++c) {
+                        const T s = run_sum_t(u[j][c], end);
+                        if (seg) atomicAdd(&sm[rank * CC + c], s);
+                    }
+                }
+                __syncthreads();
+                const int nsp = cnt_total;
+                T* out = a.spell_sums[k] + static_cast<size_t>(a.spell_base[k][t]) * a.C + a.c0;
+                for (int m = threadIdx.x; m < nsp * CC; m += kBlock) {
+                    out[static_cast<size_t>(m / CC) * a.C + (m % CC)] = sm[m];
+                }
+                __syncthreads();
+            } else {
+                T* l = lr + a.lds_off[k];
+#pragma unroll
+                for (int j = 0; j < kSlots; ++j) {
+                    const int end = run_end_lane(kr[k][j]);
+                    const bool seg = run_head(kr[k][j]) && valid[j];
+#pragma unroll
+                    for (int c = 0; c < CC; ++c) {
+                        const T s = run_sum_t(u[j][c], end);
+                        if (seg) atomicAdd(&l[kr[k][j] * CC + c], s);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < NR; ++k) {
+        if (a.mode_r[k] != kRedLds) continue;
+        const T* l = lr + a.lds_off[k];
+        for (int j = threadIdx.x; j < a.G_r[k] * CC; j += kBlock) {
+            const T s = l[j];
+            if (s != T(0)) atomicAdd(&a.lds_target[k][static_cast<size_t>(j / CC) * a.C + a.c0 + (j % CC)], s);
+        }
+    }
+    if constexpr (STORE) {
+        double qq[1] = {q};
+        block_sum_store<1>(qq, a.partials + blockIdx.x);
+    }
+}
+
+// Level sums of one R dimension from its spell sums (spells in order of level).
+template <typename T, int CC>
+__global__ void __launch_bounds__(kBlock)
+pull_kernel(const T* S, int C, int c0, const int* order, const int* skey, int M, T* sums) {
+    const int lane = lane_id();
+    for (int base = blockIdx.x * kBlock; base < M; base += gridDim.x * kBlock) {
+        const int j = base + threadIdx.x;
+        const bool valid = j < M;
+        const int key = valid ? skey[j] : -1 - lane;
+        T v[CC];
+        if (valid) {
+            const T* s = S + static_cast<size_t>(order[j]) * C + c0;
+#pragma unroll
+            for (int c = 0; c < CC; ++c) v[c] = s[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < CC; ++c) v[c] = T(0);
+        }
+        const int end = run_end_lane(key);
+        const bool head = run_head(key) && valid;
+        const bool owned = lane > 0 && end < kWave - 1;
+#pragma unroll
+        for (int c = 0; c < CC; ++c) {
+            const T s = run_sum_t(v[c], end);
+            if (head) {
+                T* t = &sums[static_cast<size_t>(key) * C + c0 + c];
+                if (owned) *t = s;
+                else atomicAdd(t, s);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ CG vector ops
+// Vectors are level-major N x C blocks (N levels of all R dims, C columns);
+// scalars per column live in device arrays of C doubles.
+
+// partials[block*C + c] = sum_j a[j,c] * b[j,c] (FP64 accumulation)
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+vdot_kernel(const T* a, const T* b, int N, int C, double* partials) {
+    __shared__ double red[kBlock / kWave][kMaxCgCols];
+    double acc[kMaxCgCols];
+    for (int c = 0; c < C; ++c) acc[c] = 0.0;
+    const long long total = static_cast<long long>(N) * C;
+    // thread-strided over levels so that a thread always owns the same columns
+    for (long long g = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; g < N;
+         g += static_cast<long long>(gridDim.x) * kBlock) {
+        for (int c = 0; c < C; ++c)
+            acc[c] += static_cast<double>(a[g * C + c]) * static_cast<double>(b[g * C + c]);
+    }
+    (void)total;
+    for (int c = 0; c < C; ++c) {
+        const double s = wave_sum(acc[c]);
+        if (lane_id() == 0) red[threadIdx.x / kWave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < C) {
+        double s = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s += red[w][threadIdx.x];
+        partials[static_cast<size_t>(blockIdx.x) * C + threadIdx.x] = s;
+    }
+}
+
+// x += alpha p; r -= alpha q; z = minv r; partials: (r.r, r.z) per column
+__global__ void __launch_bounds__(kBlock)
+cg_step_kernel(float* x, float* r, float* z, const float* p, const float* qv, const float* minv,
+               const double* alpha, int N, int C, double* partials) {
+    __shared__ double red[kBlock / kWave][2 * kMaxCgCols];
+    double rr[kMaxCgCols], rz[kMaxCgCols];
+    for (int c = 0; c < C; ++c) { rr[c] = 0.0; rz[c] = 0.0; }
+    for (long long g = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; g < N;
+         g += static_cast<long long>(gridDim.x) * kBlock) {
+        const float mi = minv[g];
+        for (int c = 0; c < C; ++c) {
+            const long long j = g * C + c;
+            const float al = static_cast<float>(alpha[c]);
+            x[j] += al * p[j];
+            const float rn = r[j] - al * qv[j];
+            r[j] = rn;
+            const float zn = mi * rn;
+            z[j] = zn;
+            rr[c] += static_cast<double>(rn) * rn;
+            rz[c] += static_cast<double>(rn) * zn;
+        }
+    }
+    for (int c = 0; c < C; ++c) {
+        const double s1 = wave_sum(rr[c]);
+        const double s2 = wave_sum(rz[c]);
+        if (lane_id() == 0) {
+            red[threadIdx.x / kWave][c] = s1;
+            red[threadIdx.x / kWave][kMaxCgCols + c] = s2;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * C) {
+        const int c = threadIdx.x < C ? threadIdx.x : threadIdx.x - C + kMaxCgCols;
+        double s = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s += red[w][c];
+        partials[static_cast<size_t>(blockIdx.x) * 2 * C + threadIdx.x] = s;
+    }
+}
+
+// p = z + beta p
+__global__ void __launch_bounds__(kBlock)
+cg_dir_kernel(float* p, const float* z, const double* beta, int N, int C) {
+    const long long total = static_cast<long long>(N) * C;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        p[j] = z[j] + static_cast<float>(beta[j % C]) * p[j];
+    }
+}
+
+// CG start: r = b * scale_c (to FP32), x = 0, z = minv r, p = z; partials r.z
+__global__ void __launch_bounds__(kBlock)
+cg_init_kernel(const double* b, const double* scale, const float* minv, float* x, float* r, float* z,
+               float* p, int N, int C, double* partials) {
+    __shared__ double red[kBlock / kWave][kMaxCgCols];
+    double rz[kMaxCgCols];
+    for (int c = 0; c < C; ++c) rz[c] = 0.0;
+    for (long long g = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; g < N;
+         g += static_cast<long long>(gridDim.x) * kBlock) {
+        const float mi = minv[g];
+        for (int c = 0; c < C; ++c) {
+            const long long j = g * C + c;
+            const float rv = static_cast<float>(b[j] * scale[c]);
+            x[j] = 0.0f;
+            r[j] = rv;
+            const float zv = mi * rv;
+            z[j] = zv;
+            p[j] = zv;
+            rz[c] += static_cast<double>(rv) * zv;
+        }
+    }
+    for (int c = 0; c < C; ++c) {
+        const double s = wave_sum(rz[c]);
+        if (lane_id() == 0) red[threadIdx.x / kWave][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < C) {
+        double s = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s += red[w][threadIdx.x];
+        partials[static_cast<size_t>(blockIdx.x) * C + threadIdx.x] = s;
+    }
+}
+
+// beta64 += x32 / scale_c
+__global__ void __launch_bounds__(kBlock)
+refine_update_kernel(double* beta, const float* x, const double* inv_scale, int N, int C) {
+    const long long total = static_cast<long long>(N) * C;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        beta[j] += static_cast<double>(x[j]) * inv_scale[j % C];
+    }
+}
+
+// Diagonal of S for the Jacobi preconditioner, from runs of equal level of r
+// inside a level of p: sum over runs of n - n^2 / N_p (exact when every
+// (p-level, r-level) pair forms one run).  Result is inverted in place later.
+__global__ void __launch_bounds__(kBlock)
+schur_diag_kernel(const int* tile_start, int num_tiles, const int* gid_p, const int* gid_r,
+                  const double* inv_p, double* diag) {
+    __shared__ int cnt[kChunks];
+    __shared__ int cnt_total;
+    __shared__ int run_n[kTileRows];
+    __shared__ int run_key[kTileRows];
+    __shared__ double run_ip[kTileRows];
+    const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
+    for (int t = blockIdx.x; t < num_tiles; t += gridDim.x) {
+        const int t0 = tile_start[t], t1 = tile_start[t + 1];
+        if (t0 >= t1) continue;
+        for (int k = threadIdx.x; k < kTileRows; k += kBlock) run_n[k] = 0;
+        int kp[kSlots], kr[kSlots], rank[kSlots];
+        bool valid[kSlots];
+        uint64_t mask[kSlots];
+        for (int j = 0; j < kSlots; ++j) {
+            const int i = t0 + j * kBlock + threadIdx.x;
+            valid[j] = i < t1;
+            kp[j] = valid[j] ? gid_p[i] : -1 - lane;
+            kr[j] = valid[j] ? gid_r[i] : -1 - lane;
+            int pp = __shfl_up(kp[j], 1, kWave), pr = __shfl_up(kr[j], 1, kWave);
+            if (lane == 0 && valid[j]) {
+                pp = i > t0 ? gid_p[i - 1] : INT_MIN;
+                pr = i > t0 ? gid_r[i - 1] : INT_MIN;
+            }
+            mask[j] = __ballot(valid[j] && (pp != kp[j] || pr != kr[j]));
+            if (lane == 0) cnt[j * (kBlock / kWave) + wave] = __popcll(mask[j]);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int m = 0; m < kChunks; ++m) tot += cnt[m];
+            cnt_total = tot;
+        }
+        for (int j = 0; j < kSlots; ++j) {
+            rank[j] = tile_rank2(cnt, j * (kBlock / kWave) + wave, mask[j]);
+            if (valid[j]) {
+                atomicAdd(&run_n[rank[j]], 1);
+                run_key[rank[j]] = kr[j];
+                run_ip[rank[j]] = inv_p[kp[j]];
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < cnt_total; k += kBlock) {
+            const double nn = run_n[k];
+            atomicAdd(&diag[run_key[k]], nn - nn * nn * run_ip[k]);
+        }
+        __syncthreads();
+    }
+}
+
+// minv = 1 / max(diag, floor * count) (levels without any variation keep a finite entry)
+__global__ void __launch_bounds__(kBlock)
+diag_invert_kernel(const double* diag, const double* counts, float* minv, int N) {
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < N; g += gridDim.x * kBlock) {
+        const double c = counts[g];
+        double d = diag[g];
+        const double floor = 1e-6 * (c > 0.0 ? c : 1.0);
+        if (!(d > floor)) d = c > 0.0 ? c : 1.0;
+        minv[g] = static_cast<float>(1.0 / d);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) to_float_kernel(const double* a, float* b, long long m) {
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < m;
+         j += static_cast<long long>(gridDim.x) * kBlock)
+        b[j] = static_cast<float>(a[j]);
+}
+
+// ------------------------------------------------------------- launchers
+
+template <typename T, int CC, int NR, bool HAS_V, bool STORE>
+static hipError_t launch_schur_t(const SchurArgs<T>& a, size_t lds_extra, int grid, hipStream_t s) {
+    const size_t shmem = static_cast<size_t>(kTileRows) * CC * sizeof(T) + lds_extra;
+    hipLaunchKernelGGL((schur_kernel<T, CC, NR, HAS_V, STORE>), dim3(grid), dim3(kBlock), shmem, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, int CC>
+static hipError_t launch_schur_cc(const SchurArgs<T>& a, int nr, bool has_v, bool store,
+                                  size_t lds_extra, int grid, hipStream_t s) {
+    if (nr == 1) {
+        if (!has_v) return launch_schur_t<T, CC, 1, false, false>(a, lds_extra, grid, s);
+        if (!store) return launch_schur_t<T, CC, 1, true, false>(a, lds_extra, grid, s);
+        return launch_schur_t<T, CC, 1, true, true>(a, lds_extra, grid, s);
+    }
+    if (!has_v) return launch_schur_t<T, CC, 2, false, false>(a, lds_extra, grid, s);
+    if (!store) return launch_schur_t<T, CC, 2, true, false>(a, lds_extra, grid, s);
+    return launch_schur_t<T, CC, 2, true, true>(a, lds_extra, grid, s);
+}
+
+template <typename T>
+hipError_t launch_schur(const SchurArgs<T>& a, int cc, int nr, bool has_v, bool store,
+                        size_t lds_extra, int grid, hipStream_t s) {
+    if (nr < 1 || nr > 2) return hipErrorInvalidValue;
+    switch (cc) {
+        case 1: return launch_schur_cc<T, 1>(a, nr, has_v, store, lds_extra, grid, s);
+        case 2: return launch_schur_cc<T, 2>(a, nr, has_v, store, lds_extra, grid, s);
+        case 3: return launch_schur_cc<T, 3>(a, nr, has_v, store, lds_extra, grid, s);
+        case 4: return launch_schur_cc<T, 4>(a, nr, has_v, store, lds_extra, grid, s);
+        case 5: return launch_schur_cc<T, 5>(a, nr, has_v, store, lds_extra, grid, s);
+        case 6: return launch_schur_cc<T, 6>(a, nr, has_v, store, lds_extra, grid, s);
+        case 7: return launch_schur_cc<T, 7>(a, nr, has_v, store, lds_extra, grid, s);
+        case 8: return launch_schur_cc<T, 8>(a, nr, has_v, store, lds_extra, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+template hipError_t launch_schur<float>(const SchurArgs<float>&, int, int, bool, bool, size_t, int, hipStream_t);
+template hipError_t launch_schur<double>(const SchurArgs<double>&, int, int, bool, bool, size_t, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_pull(const T* S, int C, int c0, int cc, const int* order, const int* skey, int M,
+                       T* sums, int grid, hipStream_t s) {
+    switch (cc) {
+#define XFE_PULLT(K) case K: hipLaunchKernelGGL((pull_kernel<T, K>), dim3(grid), dim3(kBlock), 0, s, S, C, c0, order, skey, M, sums); break;
+        XFE_PULLT(1) XFE_PULLT(2) XFE_PULLT(3) XFE_PULLT(4) XFE_PULLT(5) XFE_PULLT(6) XFE_PULLT(7) XFE_PULLT(8)
+#undef XFE_PULLT
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_pull<float>(const float*, int, int, int, const int*, const int*, int, float*, int, hipStream_t);
+template hipError_t launch_pull<double>(const double*, int, int, int, const int*, const int*, int, double*, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((vdot_kernel<T>), dim3(grid), dim3(kBlock), 0, s, a, b, N, C, partials);
+    return hipGetLastError();
+}
+template hipError_t launch_vdot<float>(const float*, const float*, int, int, double*, int, hipStream_t);
+template hipError_t launch_vdot<double>(const double*, const double*, int, int, double*, int, hipStream_t);
+
+hipError_t launch_cg_step(float* x, float* r, float* z, const float* p, const float* q,
+                          const float* minv, const double* alpha, int N, int C, double* partials,
+                          int grid, hipStream_t s) {
+    hipLaunchKernelGGL(cg_step_kernel, dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, alpha, N,
+                       C, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_dir(float* p, const float* z, const double* beta, int N, int C, int grid,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(cg_dir_kernel, dim3(grid), dim3(kBlock), 0, s, p, z, beta, N, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x,
+                          float* r, float* z, float* p, int N, int C, double* partials, int grid,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(cg_init_kernel, dim3(grid), dim3(kBlock), 0, s, b, scale, minv, x, r, z, p,
+                       N, C, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_refine_update(double* beta, const float* x, const double* inv_scale, int N, int C,
+                                int grid, hipStream_t s) {
+    hipLaunchKernelGGL(refine_update_kernel, dim3(grid), dim3(kBlock), 0, s, beta, x, inv_scale, N, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_schur_diag(const int* tile_start, int num_tiles, const int* gid_p,
+                             const int* gid_r, const double* inv_p, double* diag, int grid,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(schur_diag_kernel, dim3(grid), dim3(kBlock), 0, s, tile_start, num_tiles,
+                       gid_p, gid_r, inv_p, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_invert(const double* diag, const double* counts, float* minv, int N,
+                              int grid, hipStream_t s) {
+    hipLaunchKernelGGL(diag_invert_kernel, dim3(grid), dim3(kBlock), 0, s, diag, counts, minv, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_float(const double* a, float* b, long long m, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(to_float_kernel, dim3(grid), dim3(kBlock), 0, s, a, b, m);
+    return hipGetLastError();
+}
+
+}  // namespace xfe
