@@ -53,7 +53,7 @@ namespace xfe {
 // ---------------------------------------------------------------------------
 template <int CC, int RM, int NQ, int SRC>
 __global__ void __launch_bounds__(kBlock) pass_kernel(PassArgs a) {
-    extern __shared__ double lsum[];
+    extern __shared__ __align__(16) double lsum[];
     const int n = a.n;
     const size_t ld = static_cast<size_t>(a.ld);
     if constexpr (RM == kRedLds) {

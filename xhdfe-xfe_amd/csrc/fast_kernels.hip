@@ -52,7 +52,9 @@ __device__ __forceinline__ T run_sum_t(T v, int end) {
 
 template <typename T, int CC, int NR, bool HAS_V, bool STORE>
 __global__ void __launch_bounds__(kBlock) schur_kernel(SchurArgs<T> a) {
-    extern __shared__ unsigned char dyn[];
+    // 16-byte aligned: the dynamic block follows the static LDS, and the FP64
+    // LDS atomics / 128-bit LDS loads below need naturally aligned addresses
+    extern __shared__ __align__(16) unsigned char dyn[];
     T* sm = reinterpret_cast<T*>(dyn);                 // kTileRows x CC
     T* lr = sm + kTileRows * CC;                       // LDS-mode sums of the R dims
     __shared__ int cnt[kChunks];

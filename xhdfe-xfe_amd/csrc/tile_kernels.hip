@@ -49,7 +49,7 @@ __device__ __forceinline__ int tile_rank(const int* cnt, int chunk, uint64_t mas
 
 template <int CC, int NQ, int SRC, int RR, int PP>
 __global__ void __launch_bounds__(kBlock) tile_pass_kernel(TileArgs a) {
-    extern __shared__ double dsm[];
+    extern __shared__ __align__(16) double dsm[];
     double* sm = dsm;                       // kTileRows x CC run sums (levels, then spells)
     double* lr = dsm + kTileRows * CC;      // G_r x CC (RR == kRedLds)
     __shared__ int cnt[kChunks];
