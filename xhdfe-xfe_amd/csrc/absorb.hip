@@ -80,6 +80,10 @@ struct Workspace {
     DevBuf<double> f_beta, f_b, f_diag, f_counts, f_scale;
     DevBuf<float> f_x, f_r, f_z, f_p, f_q, f_minv, f_invp;
     DevBuf<float> f_sp[XFE_MAX_DIMS];
+    // segment form of the Schur operator (worker-major and firm-major records)
+    DevBuf<int> seg_start, seg_w, seg_r1, seg_len, seg_a, wseg, f_key, f_order, f_w, f_len, f_a;
+    DevBuf<float> f_wmean, f_hpart;
+    DevBuf<double> d_wmean, d_hpart, seg_tot, f_b0, f_tmp;
     double* host_scalars = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -102,6 +106,10 @@ struct Workspace {
         f_beta.release(); f_b.release(); f_diag.release(); f_counts.release(); f_scale.release();
         f_x.release(); f_r.release(); f_z.release(); f_p.release(); f_q.release(); f_minv.release();
         f_invp.release();
+        seg_start.release(); seg_w.release(); seg_r1.release(); seg_len.release(); seg_a.release();
+        wseg.release(); f_key.release(); f_order.release(); f_w.release(); f_len.release(); f_a.release();
+        f_wmean.release(); f_hpart.release(); d_wmean.release(); d_hpart.release(); seg_tot.release();
+        f_b0.release(); f_tmp.release();
         if (host_scalars) (void)hipHostFree(host_scalars);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -738,8 +746,142 @@ struct FastSolver {
     int vgrid = 1;
     int inner_total = 0;
     double pull_apply_ms = 0.0;
+    // segment form: k1 indexes the spell dimension r1 in rdim, k2 the range
+    // dimension r2 (-1 without one)
+    bool seg = false;
+    int k1 = 0, k2 = -1, M = 0, Nw = 0, G2 = 0, seg_grid_w = 1, seg_grid_f = 1;
 
     explicit FastSolver(Solver& sv_) : sv(sv_), ws(sv_.ws), s(sv_.s) {}
+
+    // Segment records of the sorted rows (see seg_kernels.hip).  Returns false
+    // when the range dimension has too many levels for the LDS tables.
+    bool setup_segments() {
+        if (std::getenv("XFE_NO_SEGMENTS")) return false;
+        if (NR == 2) {
+            k1 = sv.G[rdim[0]] >= sv.G[rdim[1]] ? 0 : 1;
+            k2 = 1 - k1;
+            if (sv.G[rdim[k2]] > kSegMaxRange) return false;
+            G2 = sv.G[rdim[k2]];
+        } else {
+            k1 = 0;
+            k2 = -1;
+            G2 = 0;
+        }
+        const int n = sv.n;
+        Nw = sv.G[p];
+        const int* gp = sv.gid[p];
+        const int* g1 = sv.gid[rdim[k1]];
+        const int* g2 = k2 >= 0 ? sv.gid[rdim[k2]] : nullptr;
+        const int grid = grid_for(n);
+        ws.tmp_flags.ensure(n);
+        ws.tmp_inc.ensure(n);
+        XFE_CHECK(launch_seg_heads(gp, g1, g2, n, ws.tmp_flags.p, grid, s));
+        size_t bytes = 0;
+        XFE_CHECK(rocprim::inclusive_scan(nullptr, bytes, ws.tmp_flags.p, ws.tmp_inc.p,
+                                          static_cast<size_t>(n), rocprim::plus<int>(), s));
+        ws.sort_tmp.ensure(bytes);
+        XFE_CHECK(rocprim::inclusive_scan(ws.sort_tmp.p, bytes, ws.tmp_flags.p, ws.tmp_inc.p,
+                                          static_cast<size_t>(n), rocprim::plus<int>(), s));
+        XFE_CHECK(hipMemcpyAsync(&M, ws.tmp_inc.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        if (M <= 0) return false;
+        ws.seg_start.ensure(M); ws.seg_w.ensure(M); ws.seg_r1.ensure(M); ws.seg_len.ensure(M);
+        ws.seg_a.ensure(M); ws.wseg.ensure(static_cast<size_t>(Nw) + 1);
+        ws.f_key.ensure(M); ws.f_order.ensure(M); ws.f_w.ensure(M); ws.f_len.ensure(M); ws.f_a.ensure(M);
+        ws.tmp_iota.ensure(M);
+        const int mgrid = grid_for(M);
+        XFE_CHECK(launch_seg_fill(ws.tmp_flags.p, ws.tmp_inc.p, gp, g1, g2, n, ws.seg_start.p, ws.seg_w.p,
+                                  ws.seg_r1.p, ws.seg_a.p, grid, s));
+        XFE_CHECK(launch_seg_len(ws.seg_start.p, ws.seg_w.p, M, n, Nw, ws.seg_len.p, ws.wseg.p, mgrid, s));
+        // firm-major order of the segments (stable: worker order within a firm)
+        XFE_CHECK(launch_iota(ws.tmp_iota.p, M, mgrid, s));
+        int bits = 1;
+        while ((1LL << bits) < static_cast<long long>(sv.G[rdim[k1]])) ++bits;
+        bytes = 0;
+        XFE_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, ws.seg_r1.p, ws.f_key.p, ws.tmp_iota.p,
+                                            ws.f_order.p, static_cast<size_t>(M), 0, bits, s));
+        ws.sort_tmp.ensure(bytes);
+        XFE_CHECK(rocprim::radix_sort_pairs(ws.sort_tmp.p, bytes, ws.seg_r1.p, ws.f_key.p, ws.tmp_iota.p,
+                                            ws.f_order.p, static_cast<size_t>(M), 0, bits, s));
+        XFE_CHECK(launch_gather_ids(ws.seg_w.p, ws.f_order.p, ws.f_w.p, M, mgrid, s));
+        XFE_CHECK(launch_gather_ids(ws.seg_len.p, ws.f_order.p, ws.f_len.p, M, mgrid, s));
+        if (k2 >= 0) XFE_CHECK(launch_gather_ids(ws.seg_a.p, ws.f_order.p, ws.f_a.p, M, mgrid, s));
+        seg_grid_w = grid_for(Nw);
+        seg_grid_f = mgrid;
+        const size_t tabs = static_cast<size_t>(seg_grid_f) * (G2 + 1) * 8;
+        ws.f_wmean.ensure(static_cast<size_t>(Nw) * 8);
+        ws.d_wmean.ensure(static_cast<size_t>(Nw) * 8);
+        ws.f_hpart.ensure(tabs);
+        ws.d_hpart.ensure(tabs);
+        ws.seg_tot.ensure(static_cast<size_t>(G2 + 1) * 8);
+        const size_t NC = static_cast<size_t>(N) * C;
+        ws.f_b0.ensure(NC);
+        ws.f_tmp.ensure(NC);
+        return true;
+    }
+
+    // out = S x through the segment records (T = float for the inner CG,
+    // double for the refinement residual).
+    template <typename T>
+    void seg_apply(const T* x, T* out, const T* inv_w, T* wmean, T* hpart) {
+        XFE_CHECK(hipMemsetAsync(out, 0, sizeof(T) * static_cast<size_t>(N) * C, s));
+        for (int ch = 0; ch < sv.nchunks; ++ch) {
+            const int c0 = ch * 8;
+            const int cc = std::min(8, C - c0);
+            SegArgs<T> a;
+            a.C = C; a.c0 = c0; a.Nw = Nw; a.M = M; a.G2 = G2;
+            a.x1 = x + static_cast<size_t>(off[k1]) * C;
+            a.x2 = k2 >= 0 ? x + static_cast<size_t>(off[k2]) * C : nullptr;
+            a.inv_w = inv_w;
+            a.wseg = ws.wseg.p; a.seg_r1 = ws.seg_r1.p; a.seg_len = ws.seg_len.p; a.seg_a = ws.seg_a.p;
+            a.wmean = wmean;
+            a.f_key = ws.f_key.p; a.f_w = ws.f_w.p; a.f_len = ws.f_len.p; a.f_a = ws.f_a.p;
+            a.out1 = out + static_cast<size_t>(off[k1]) * C;
+            a.hpart = hpart;
+            if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            XFE_CHECK(launch_seg_apply<T>(a, cc, k2 >= 0, false, seg_grid_w, seg_grid_f, s));
+            if (k2 >= 0)
+                XFE_CHECK(launch_seg_year<T>(hpart, seg_grid_f, G2, cc, ws.seg_tot.p, ws.wsum[rdim[k2]].p,
+                                             a.x2, C, c0, out + static_cast<size_t>(off[k2]) * C, s));
+            if (g_profile.enabled) {
+                XFE_CHECK(hipEventRecord(ws.ev1, s));
+                XFE_CHECK(hipEventSynchronize(ws.ev1));
+                float ms = 0.f;
+                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+                g_profile.ms += ms;
+                g_profile.launches += 1;
+                const double e = sizeof(T);
+                // worker pass: offsets, 3 ints per segment, coefficient gathers, means;
+                // firm pass: 4 ints per segment, coefficients, mean gathers, output
+                double bytes = 4.0 * (Nw + 1) + 12.0 * M + e * cc * (static_cast<double>(M) + Nw);
+                bytes += 16.0 * M + e * cc * (static_cast<double>(sv.G[rdim[k1]]) + M + sv.G[rdim[k1]]);
+                g_profile.bytes += bytes;
+            }
+        }
+    }
+
+    // Jacobi diagonal from the segments: firms L - L^2 / n_w per segment,
+    // years n_y - sum of 1 / n_w over the segments covering the year.
+    void seg_diag() {
+        XFE_CHECK(hipMemsetAsync(ws.f_diag.p, 0, sizeof(double) * N, s));
+        SegArgs<double> a;
+        a.C = 1; a.c0 = 0; a.Nw = Nw; a.M = M; a.G2 = G2;
+        a.inv_w = ws.inv[p].p;
+        a.f_key = ws.f_key.p; a.f_w = ws.f_w.p; a.f_len = ws.f_len.p; a.f_a = ws.f_a.p;
+        a.out1 = ws.f_diag.p + off[k1];
+        a.hpart = ws.d_hpart.p;
+        XFE_CHECK(launch_seg_apply<double>(a, 1, k2 >= 0, true, seg_grid_w, seg_grid_f, s));
+        if (k2 >= 0)
+            XFE_CHECK(launch_seg_year<double>(ws.d_hpart.p, seg_grid_f, G2, 1, ws.seg_tot.p,
+                                              ws.wsum[rdim[k2]].p, nullptr, 1, 0, ws.f_diag.p + off[k2], s));
+    }
+
+    // Refinement residual b = b0 - S beta in FP64 through the segments.
+    void residual_seg() {
+        seg_apply<double>(ws.f_beta.p, ws.f_tmp.p, ws.inv[p].p, ws.d_wmean.p, ws.d_hpart.p);
+        const int NC = N * C;
+        XFE_CHECK(launch_sub(ws.f_b.p, ws.f_b0.p, ws.f_tmp.p, NC, 1, NC, grid_for(NC), s));
+    }
 
     bool setup() {
         if (!sv.tiled || sv.w != nullptr || sv.D < 2 || sv.D > 3 || sv.C > kMaxCgCols) return false;
@@ -777,11 +919,18 @@ struct FastSolver {
         ws.partials.ensure(static_cast<size_t>(std::max(vgrid, sv.tile_grid)) * 2 * kMaxCgCols + 64);
         ws.scalars.ensure(2 * kMaxCgCols);
         XFE_CHECK(launch_to_float(ws.inv[p].p, ws.f_invp.p, sv.G[p], grid_for(sv.G[p]), s));
-        // Jacobi preconditioner: diag(S) from runs of each R level inside levels of p
-        XFE_CHECK(hipMemsetAsync(ws.f_diag.p, 0, sizeof(double) * N, s));
+        seg = setup_segments();
+        // Jacobi preconditioner: diag(S) from the segments, or from runs of each
+        // R level inside levels of p
+        if (seg) {
+            seg_diag();
+        } else {
+            XFE_CHECK(hipMemsetAsync(ws.f_diag.p, 0, sizeof(double) * N, s));
+            for (int k = 0; k < NR; ++k)
+                XFE_CHECK(launch_schur_diag(sv.tile_start, sv.num_tiles, sv.gid[p], sv.gid[rdim[k]],
+                                            ws.inv[p].p, ws.f_diag.p + off[k], sv.tile_grid, s));
+        }
         for (int k = 0; k < NR; ++k) {
-            XFE_CHECK(launch_schur_diag(sv.tile_start, sv.num_tiles, sv.gid[p], sv.gid[rdim[k]],
-                                        ws.inv[p].p, ws.f_diag.p + off[k], sv.tile_grid, s));
             XFE_CHECK(hipMemcpyAsync(ws.f_counts.p + off[k], ws.wsum[rdim[k]].p,
                                      sizeof(double) * sv.G[rdim[k]], hipMemcpyDeviceToDevice, s));
         }
@@ -855,6 +1004,10 @@ struct FastSolver {
     }
 
     void apply32(const float* x, float* q) {
+        if (seg) {
+            seg_apply<float>(x, q, ws.f_invp.p, ws.f_wmean.p, ws.f_hpart.p);
+            return;
+        }
         float* sp[2] = {nullptr, nullptr};
         for (int k = 0; k < NR; ++k) if (mode[k] == kRedSpell) sp[k] = ws.f_sp[rdim[k]].p;
         schur_pass<float>(x, q, nullptr, nullptr, 1.0f, ws.f_invp.p, sp, lds_extra32, nullptr);
@@ -944,6 +1097,10 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
     const int max_inner = std::max(10, std::min(opt.max_iter, 5000));
     double b0[kMaxCgCols], bb[kMaxCgCols];
     fs.residual64(V, nullptr);
+    if (fs.seg) {
+        XFE_CHECK(hipMemcpyAsync(sv.ws.f_b0.p, sv.ws.f_b.p, sizeof(double) * static_cast<size_t>(fs.N) * C,
+                                 hipMemcpyDeviceToDevice, sv.s));
+    }
     fs.coldot<double>(sv.ws.f_b.p, sv.ws.f_b.p, b0);
     int refinements = 0;
     bool converged = false;
@@ -958,7 +1115,8 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
         prev_rel = rel;
         fs.inner_solve(tol_in, max_inner);
         ++refinements;
-        fs.residual64(V, nullptr);
+        if (fs.seg) fs.residual_seg();
+        else fs.residual64(V, nullptr);
     }
     if (!converged) {
         fs.coldot<double>(sv.ws.f_b.p, sv.ws.f_b.p, bb);

@@ -112,6 +112,45 @@ struct SchurArgs {
     double* partials = nullptr;               // STORE: per-block sum of squares
 };
 
+// Arguments of the segment form of the Schur operator (seg_kernels.hip).
+constexpr int kSegMaxRange = 511;  // most levels of the range dimension r2 (LDS tables)
+template <typename T>
+struct SegArgs {
+    int C = 0;                    // stride of the level-major coefficient tables
+    int c0 = 0;                   // first column of this chunk
+    int Nw = 0;                   // levels of the eliminated dimension p
+    int M = 0;                    // segments
+    int G2 = 0;                   // levels of r2 (0: no r2)
+    const T* x1 = nullptr;        // coefficients of r1 (G1 x C)
+    const T* x2 = nullptr;        // coefficients of r2 (G2 x C)
+    const T* inv_w = nullptr;     // 1 / n_w
+    const int* wseg = nullptr;    // Nw + 1 segment offsets per level of p
+    const int* seg_r1 = nullptr;  // worker-major segment records
+    const int* seg_len = nullptr;
+    const int* seg_a = nullptr;
+    T* wmean = nullptr;           // Nw x cc means of u over p
+    const int* f_key = nullptr;   // firm-major segment records (sorted by r1)
+    const int* f_w = nullptr;
+    const int* f_len = nullptr;
+    const int* f_a = nullptr;
+    T* out1 = nullptr;            // (S x)_1, zeroed before the pass
+    T* hpart = nullptr;           // per-block (G2 + 1) x cc year differences
+};
+
+hipError_t launch_seg_heads(const int* gp, const int* g1, const int* g2, int n, int* flags, int grid,
+                            hipStream_t s);
+hipError_t launch_seg_fill(const int* flags, const int* inc, const int* gp, const int* g1, const int* g2,
+                           int n, int* seg_start, int* seg_w, int* seg_r1, int* seg_a, int grid,
+                           hipStream_t s);
+hipError_t launch_seg_len(const int* seg_start, const int* seg_w, int M, int n, int Nw, int* seg_len,
+                          int* wseg, int grid, hipStream_t s);
+template <typename T>
+hipError_t launch_seg_apply(const SegArgs<T>& a, int cc, bool has_y, bool diag, int grid_w, int grid_f,
+                            hipStream_t s);
+template <typename T>
+hipError_t launch_seg_year(const T* hpart, int nblk, int G2, int cc, double* tot, const double* counts,
+                           const T* x2, int C, int c0, T* out2, hipStream_t s);
+
 template <typename T>
 hipError_t launch_schur(const SchurArgs<T>& a, int cc, int nr, bool has_v, bool store,
                         size_t lds_extra, int grid, hipStream_t s);

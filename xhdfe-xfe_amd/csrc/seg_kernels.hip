@@ -1,0 +1,325 @@
+// Segment form of the Schur-complement operator of the mixed-precision solver.
+//
+// Rows are level-sorted by the eliminated dimension p (workers).  A segment is
+// a maximal run of rows with the same level of p, the same level of r1 (firms)
+// and, when a second dimension r2 (years) is kept, consecutive levels of r2
+// (r2 of the next row = r2 of this row + 1).  On an AKM panel a segment is a
+// job spell: about 7.5M segments for 54M rows.
+//
+// For coefficients x = (x1, x2) and u_i = x1[r1_i] + x2[r2_i], the operator
+// S x = D_R' M_p D_R x is, with P the prefix sums of x2 over the levels of r2,
+//     U_s  = L_s x1[f_s] + P[a_s + L_s] - P[a_s]        (sum of u over segment s)
+//     m_w  = (1 / n_w) sum_{s in w} U_s                 (mean of u over worker w)
+//     (S x)_1[f] = sum_{s: f_s = f} L_s (x1[f] - m_w(s)) + P[a_s + L_s] - P[a_s]
+//     (S x)_2[y] = n_y x2[y] + sum_{s covering y} (x1[f_s] - m_w(s))
+// The last sum is a difference array over the segment's range [a_s, a_s + L_s)
+// followed by a prefix sum.  One application is two passes over the segments
+// (worker-major for m, firm-major for (S x)_1 and the year differences) with
+// no per-row work.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "absorb_kernels.h"
+#include "wave_ops.h"
+
+namespace xfe {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T seg_run_sum(T v, int end) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const T o = __shfl_down(v, off, kWave);
+        if (lane + off <= end) v += o;
+    }
+    return v;
+}
+
+// P[k][c] = sum_{y < k} x2[y][c0 + c] for k = 0..G2, built in LDS by one
+// thread per column (G2 <= kSegMaxRange).
+template <typename T, int CC>
+__device__ __forceinline__ void build_prefix(const T* x2, int G2, int C, int c0, T* P) {
+    if (threadIdx.x < CC) {
+        const int c = threadIdx.x;
+        T acc = T(0);
+        P[c] = acc;
+        for (int y = 0; y < G2; ++y) {
+            acc += x2[static_cast<size_t>(y) * C + c0 + c];
+            P[(y + 1) * CC + c] = acc;
+        }
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ set-up
+
+// 1 where a row starts a segment.
+__global__ void __launch_bounds__(kBlock)
+seg_head_kernel(const int* gp, const int* g1, const int* g2, int n, int* flags) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        int h = 1;
+        if (i > 0) {
+            h = gp[i] != gp[i - 1] || g1[i] != g1[i - 1] || (g2 != nullptr && g2[i] != g2[i - 1] + 1);
+        }
+        flags[i] = h;
+    }
+}
+
+// Segment records from the head rows (inc = inclusive scan of the flags).
+__global__ void __launch_bounds__(kBlock)
+seg_fill_kernel(const int* flags, const int* inc, const int* gp, const int* g1, const int* g2, int n,
+                int* seg_start, int* seg_w, int* seg_r1, int* seg_a) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        if (!flags[i]) continue;
+        const int s = inc[i] - 1;
+        seg_start[s] = i;
+        seg_w[s] = gp[i];
+        seg_r1[s] = g1[i];
+        if (g2 != nullptr) seg_a[s] = g2[i];
+    }
+}
+
+// Segment lengths, and wseg[w] = first segment of worker w (wseg[Nw] = M;
+// workers without rows get the start of the next worker).
+__global__ void __launch_bounds__(kBlock)
+seg_len_kernel(const int* seg_start, const int* seg_w, int M, int n, int Nw, int* seg_len, int* wseg) {
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < M; s += gridDim.x * kBlock) {
+        const int e = s + 1 < M ? seg_start[s + 1] : n;
+        seg_len[s] = e - seg_start[s];
+        const int w = seg_w[s];
+        const int prev = s > 0 ? seg_w[s - 1] : -1;
+        if (w != prev) {
+            for (int v = prev + 1; v <= w; ++v) wseg[v] = s;
+        }
+        if (s == M - 1) {
+            for (int v = w + 1; v <= Nw; ++v) wseg[v] = M;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ apply
+
+// Pass 1 (one thread per worker): m_w = (1 / n_w) sum_{s in w} U_s.
+template <typename T, int CC, bool HAS_Y>
+__global__ void __launch_bounds__(kBlock) seg_worker_kernel(SegArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char seg_dyn[];
+    T* P = reinterpret_cast<T*>(seg_dyn);
+    if constexpr (HAS_Y) {
+        build_prefix<T, CC>(a.x2, a.G2, a.C, a.c0, P);
+        __syncthreads();
+    }
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < a.Nw; w += gridDim.x * kBlock) {
+        const int s0 = a.wseg[w], s1 = a.wseg[w + 1];
+        T acc[CC];
+#pragma unroll
+        for (int c = 0; c < CC; ++c) acc[c] = T(0);
+        for (int s = s0; s < s1; ++s) {
+            const int f = a.seg_r1[s];
+            const int L = a.seg_len[s];
+            const T* xf = a.x1 + static_cast<size_t>(f) * a.C + a.c0;
+            const T tl = static_cast<T>(L);
+#pragma unroll
+            for (int c = 0; c < CC; ++c) acc[c] += tl * xf[c];
+            if constexpr (HAS_Y) {
+                const int y0 = a.seg_a[s];
+#pragma unroll
+                for (int c = 0; c < CC; ++c) acc[c] += P[(y0 + L) * CC + c] - P[y0 * CC + c];
+            }
+        }
+        const T iw = a.inv_w[w];
+        T* m = a.wmean + static_cast<size_t>(w) * CC;
+#pragma unroll
+        for (int c = 0; c < CC; ++c) m[c] = acc[c] * iw;
+    }
+}
+
+// Pass 2 (one thread per segment, segments in firm order): (S x)_1 by a
+// segmented sum over equal firms, and the year differences into a per-block
+// LDS histogram written to hpart[block].  DIAG: the Jacobi diagonal instead
+// (T = double, CC = 1): L - L^2 / n_w per segment, and -1 / n_w over its years.
+template <typename T, int CC, bool HAS_Y, bool DIAG>
+__global__ void __launch_bounds__(kBlock) seg_firm_kernel(SegArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char seg_dyn[];
+    T* P = reinterpret_cast<T*>(seg_dyn);          // (G2 + 1) x CC prefix sums
+    T* H = P + (a.G2 + 1) * CC;                    // (G2 + 1) x CC differences
+    const int lane = lane_id();
+    if constexpr (HAS_Y) {
+        if constexpr (!DIAG) build_prefix<T, CC>(a.x2, a.G2, a.C, a.c0, P);
+        for (int k = threadIdx.x; k < (a.G2 + 1) * CC; k += kBlock) H[k] = T(0);
+        __syncthreads();
+    }
+    for (int base = blockIdx.x * kBlock; base < a.M; base += gridDim.x * kBlock) {
+        const int j = base + threadIdx.x;
+        const bool valid = j < a.M;
+        const int key = valid ? a.f_key[j] : -1 - lane;
+        T v[CC];
+#pragma unroll
+        for (int c = 0; c < CC; ++c) v[c] = T(0);
+        if (valid) {
+            const int w = a.f_w[j];
+            const int L = a.f_len[j];
+            const T tl = static_cast<T>(L);
+            if constexpr (DIAG) {
+                const T iw = a.inv_w[w];
+                v[0] = tl - tl * tl * iw;
+                if constexpr (HAS_Y) {
+                    const int y0 = a.f_a[j];
+                    atomicAdd(&H[y0], -iw);
+                    atomicAdd(&H[y0 + L], iw);
+                }
+            } else {
+                const T* xf = a.x1 + static_cast<size_t>(key) * a.C + a.c0;
+                const T* m = a.wmean + static_cast<size_t>(w) * CC;
+                T d[CC];
+#pragma unroll
+                for (int c = 0; c < CC; ++c) {
+                    d[c] = xf[c] - m[c];
+                    v[c] = tl * d[c];
+                }
+                if constexpr (HAS_Y) {
+                    const int y0 = a.f_a[j];
+#pragma unroll
+                    for (int c = 0; c < CC; ++c) {
+                        v[c] += P[(y0 + L) * CC + c] - P[y0 * CC + c];
+                        atomicAdd(&H[y0 * CC + c], d[c]);
+                        atomicAdd(&H[(y0 + L) * CC + c], -d[c]);
+                    }
+                }
+            }
+        }
+        const int end = run_end_lane(key);
+        const bool head = run_head(key) && valid;
+        const bool owned = lane > 0 && end < kWave - 1;  // the firm lies inside this wave
+#pragma unroll
+        for (int c = 0; c < CC; ++c) {
+            const T s = seg_run_sum<T>(v[c], end);
+            if (head) {
+                T* t = a.out1 + static_cast<size_t>(key) * a.C + a.c0 + c;
+                if (owned) *t = s;
+                else atomicAdd(t, s);
+            }
+        }
+    }
+    if constexpr (HAS_Y) {
+        __syncthreads();
+        T* hp = a.hpart + static_cast<size_t>(blockIdx.x) * (a.G2 + 1) * CC;
+        for (int k = threadIdx.x; k < (a.G2 + 1) * CC; k += kBlock) hp[k] = H[k];
+    }
+}
+
+// tot[e] = sum over blocks of hpart[b][e], one block per entry e.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) seg_hsum_kernel(const T* hpart, int nblk, int E, double* tot) {
+    __shared__ double red[kBlock / kWave];
+    const int e = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) acc += static_cast<double>(hpart[static_cast<size_t>(b) * E + e]);
+    acc = wave_sum(acc);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) s += red[k];
+        tot[e] = s;
+    }
+}
+
+// (S x)_2[y] = n_y x2[y] + prefix_y(tot); with x2 == nullptr the result is
+// n_y + prefix_y(tot) (the Jacobi diagonal of the year block).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+seg_year_kernel(const double* tot, int G2, int cc, const double* counts, const T* x2, int C, int c0,
+                T* out2) {
+    const int c = threadIdx.x;
+    if (c >= cc) return;
+    double run = 0.0;
+    for (int y = 0; y < G2; ++y) {
+        run += tot[y * cc + c];
+        const double xv = x2 ? static_cast<double>(x2[static_cast<size_t>(y) * C + c0 + c]) : 1.0;
+        out2[static_cast<size_t>(y) * C + c0 + c] = static_cast<T>(counts[y] * xv + run);
+    }
+}
+
+// ------------------------------------------------------------- launchers
+
+hipError_t launch_seg_heads(const int* gp, const int* g1, const int* g2, int n, int* flags, int grid,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(seg_head_kernel, dim3(grid), dim3(kBlock), 0, s, gp, g1, g2, n, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_fill(const int* flags, const int* inc, const int* gp, const int* g1, const int* g2,
+                           int n, int* seg_start, int* seg_w, int* seg_r1, int* seg_a, int grid,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(seg_fill_kernel, dim3(grid), dim3(kBlock), 0, s, flags, inc, gp, g1, g2, n,
+                       seg_start, seg_w, seg_r1, seg_a);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_len(const int* seg_start, const int* seg_w, int M, int n, int Nw, int* seg_len,
+                          int* wseg, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(seg_len_kernel, dim3(grid), dim3(kBlock), 0, s, seg_start, seg_w, M, n, Nw,
+                       seg_len, wseg);
+    return hipGetLastError();
+}
+
+template <typename T, int CC>
+static hipError_t seg_apply_cc(const SegArgs<T>& a, bool has_y, bool diag, int grid_w, int grid_f,
+                               hipStream_t s) {
+    const size_t tab = has_y ? static_cast<size_t>(a.G2 + 1) * CC * sizeof(T) : 0;
+    if (diag) {
+        if constexpr (CC == 1) {
+            if (has_y) hipLaunchKernelGGL((seg_firm_kernel<T, 1, true, true>), dim3(grid_f), dim3(kBlock), 2 * tab, s, a);
+            else hipLaunchKernelGGL((seg_firm_kernel<T, 1, false, true>), dim3(grid_f), dim3(kBlock), 0, s, a);
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
+    if (has_y) {
+        hipLaunchKernelGGL((seg_worker_kernel<T, CC, true>), dim3(grid_w), dim3(kBlock), tab, s, a);
+        hipLaunchKernelGGL((seg_firm_kernel<T, CC, true, false>), dim3(grid_f), dim3(kBlock), 2 * tab, s, a);
+    } else {
+        hipLaunchKernelGGL((seg_worker_kernel<T, CC, false>), dim3(grid_w), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((seg_firm_kernel<T, CC, false, false>), dim3(grid_f), dim3(kBlock), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_seg_apply(const SegArgs<T>& a, int cc, bool has_y, bool diag, int grid_w, int grid_f,
+                            hipStream_t s) {
+    switch (cc) {
+        case 1: return seg_apply_cc<T, 1>(a, has_y, diag, grid_w, grid_f, s);
+        case 2: return seg_apply_cc<T, 2>(a, has_y, diag, grid_w, grid_f, s);
+        case 3: return seg_apply_cc<T, 3>(a, has_y, diag, grid_w, grid_f, s);
+        case 4: return seg_apply_cc<T, 4>(a, has_y, diag, grid_w, grid_f, s);
+        case 5: return seg_apply_cc<T, 5>(a, has_y, diag, grid_w, grid_f, s);
+        case 6: return seg_apply_cc<T, 6>(a, has_y, diag, grid_w, grid_f, s);
+        case 7: return seg_apply_cc<T, 7>(a, has_y, diag, grid_w, grid_f, s);
+        case 8: return seg_apply_cc<T, 8>(a, has_y, diag, grid_w, grid_f, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+template hipError_t launch_seg_apply<float>(const SegArgs<float>&, int, bool, bool, int, int, hipStream_t);
+template hipError_t launch_seg_apply<double>(const SegArgs<double>&, int, bool, bool, int, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_seg_year(const T* hpart, int nblk, int G2, int cc, double* tot, const double* counts,
+                           const T* x2, int C, int c0, T* out2, hipStream_t s) {
+    const int E = (G2 + 1) * cc;
+    hipLaunchKernelGGL((seg_hsum_kernel<T>), dim3(E), dim3(kBlock), 0, s, hpart, nblk, E, tot);
+    hipLaunchKernelGGL((seg_year_kernel<T>), dim3(1), dim3(kBlock), 0, s, tot, G2, cc, counts, x2, C, c0,
+                       out2);
+    return hipGetLastError();
+}
+template hipError_t launch_seg_year<float>(const float*, int, int, int, double*, const double*, const float*,
+                                           int, int, float*, hipStream_t);
+template hipError_t launch_seg_year<double>(const double*, int, int, int, double*, const double*,
+                                            const double*, int, int, double*, hipStream_t);
+
+}  // namespace xfe
