@@ -77,13 +77,14 @@ struct Workspace {
     DevBuf<int> sp_key[XFE_MAX_DIMS], sp_order[XFE_MAX_DIMS], sp_base[XFE_MAX_DIMS];
     DevBuf<double> sp_sums[XFE_MAX_DIMS];
     // mixed-precision solver (coefficient space of the non-sorted dimensions)
-    DevBuf<double> f_beta, f_b, f_diag, f_counts, f_scale;
+    DevBuf<double> f_beta, f_b, f_diag, f_counts, f_scale, cg;
     DevBuf<float> f_x, f_r, f_z, f_p, f_q, f_minv, f_invp;
     DevBuf<float> f_sp[XFE_MAX_DIMS];
     // segment form of the Schur operator (worker-major and firm-major records)
     DevBuf<int> seg_start, seg_w, seg_r1, seg_len, seg_a, wseg, f_key, f_order, f_w, f_len, f_a;
-    DevBuf<float> f_wmean, f_hpart;
-    DevBuf<double> d_wmean, d_hpart, seg_tot, f_b0, f_tmp;
+    DevBuf<int> w_pf, w_pi, w_rank, seg_rk, p_order, p_rk, p_r1, p_len, p_a, p_wseg;
+    DevBuf<float> f_wmean, f_hpart, f_invw;
+    DevBuf<double> d_wmean, d_hpart, seg_tot, f_b0, f_tmp, d_invw;
     double* host_scalars = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -103,13 +104,16 @@ struct Workspace {
             sp_key[d].release(); sp_order[d].release(); sp_base[d].release(); sp_sums[d].release();
             f_sp[d].release();
         }
-        f_beta.release(); f_b.release(); f_diag.release(); f_counts.release(); f_scale.release();
+        cg.release(); f_beta.release(); f_b.release(); f_diag.release(); f_counts.release(); f_scale.release();
         f_x.release(); f_r.release(); f_z.release(); f_p.release(); f_q.release(); f_minv.release();
         f_invp.release();
         seg_start.release(); seg_w.release(); seg_r1.release(); seg_len.release(); seg_a.release();
         wseg.release(); f_key.release(); f_order.release(); f_w.release(); f_len.release(); f_a.release();
         f_wmean.release(); f_hpart.release(); d_wmean.release(); d_hpart.release(); seg_tot.release();
         f_b0.release(); f_tmp.release();
+        w_pf.release(); w_pi.release(); w_rank.release(); seg_rk.release(); p_order.release();
+        p_rk.release(); p_r1.release(); p_len.release(); p_a.release(); p_wseg.release();
+        f_invw.release(); d_invw.release();
         if (host_scalars) (void)hipHostFree(host_scalars);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -753,6 +757,18 @@ struct FastSolver {
 
     explicit FastSolver(Solver& sv_) : sv(sv_), ws(sv_.ws), s(sv_.s) {}
 
+    // Stable radix sort of m (key, value) pairs with keys in [0, max_key].
+    void sort_pairs(const int* kin, int* kout, const int* vin, int* vout, int m, int max_key) {
+        int bits = 1;
+        while ((1LL << bits) <= static_cast<long long>(max_key)) ++bits;
+        size_t bytes = 0;
+        XFE_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, static_cast<size_t>(m), 0,
+                                            bits, s));
+        ws.sort_tmp.ensure(bytes);
+        XFE_CHECK(rocprim::radix_sort_pairs(ws.sort_tmp.p, bytes, kin, kout, vin, vout, static_cast<size_t>(m),
+                                            0, bits, s));
+    }
+
     // Segment records of the sorted rows (see seg_kernels.hip).  Returns false
     // when the range dimension has too many levels for the LDS tables.
     bool setup_segments() {
@@ -788,24 +804,41 @@ struct FastSolver {
         ws.seg_start.ensure(M); ws.seg_w.ensure(M); ws.seg_r1.ensure(M); ws.seg_len.ensure(M);
         ws.seg_a.ensure(M); ws.wseg.ensure(static_cast<size_t>(Nw) + 1);
         ws.f_key.ensure(M); ws.f_order.ensure(M); ws.f_w.ensure(M); ws.f_len.ensure(M); ws.f_a.ensure(M);
-        ws.tmp_iota.ensure(M);
+        ws.tmp_iota.ensure(std::max(M, Nw));
+        ws.w_pf.ensure(Nw); ws.w_pi.ensure(Nw); ws.w_rank.ensure(Nw); ws.seg_rk.ensure(M);
+        ws.p_order.ensure(M); ws.p_rk.ensure(M); ws.p_r1.ensure(M); ws.p_len.ensure(M); ws.p_a.ensure(M);
+        ws.p_wseg.ensure(static_cast<size_t>(Nw) + 1);
+        ws.f_invw.ensure(Nw); ws.d_invw.ensure(Nw);
         const int mgrid = grid_for(M);
+        const int wgrid = grid_for(Nw);
+        const int G1 = sv.G[rdim[k1]];
         XFE_CHECK(launch_seg_fill(ws.tmp_flags.p, ws.tmp_inc.p, gp, g1, g2, n, ws.seg_start.p, ws.seg_w.p,
                                   ws.seg_r1.p, ws.seg_a.p, grid, s));
-        XFE_CHECK(launch_seg_len(ws.seg_start.p, ws.seg_w.p, M, n, Nw, ws.seg_len.p, ws.wseg.p, mgrid, s));
-        // firm-major order of the segments (stable: worker order within a firm)
+        XFE_CHECK(launch_seg_len(ws.seg_start.p, M, n, ws.seg_len.p, mgrid, s));
+        XFE_CHECK(launch_seg_offsets(ws.seg_w.p, M, Nw, ws.wseg.p, mgrid, s));
+        // Renumber workers by their primary firm (stable), so that the workers
+        // of one firm are adjacent: the worker pass then gathers firm
+        // coefficients of neighbouring firms and the firm pass reads the means
+        // of neighbouring workers.
+        XFE_CHECK(launch_seg_primary(ws.wseg.p, ws.seg_r1.p, Nw, G1, ws.w_pf.p, wgrid, s));
+        XFE_CHECK(launch_iota(ws.tmp_iota.p, Nw, wgrid, s));
+        sort_pairs(ws.w_pf.p, ws.w_rank.p, ws.tmp_iota.p, ws.w_pi.p, Nw, G1);  // sorted keys unused
+        XFE_CHECK(launch_seg_rank(ws.w_pi.p, Nw, ws.w_rank.p, wgrid, s));
+        XFE_CHECK(launch_seg_gather<int>(ws.w_rank.p, ws.seg_w.p, ws.seg_rk.p, M, mgrid, s));
         XFE_CHECK(launch_iota(ws.tmp_iota.p, M, mgrid, s));
-        int bits = 1;
-        while ((1LL << bits) < static_cast<long long>(sv.G[rdim[k1]])) ++bits;
-        bytes = 0;
-        XFE_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, ws.seg_r1.p, ws.f_key.p, ws.tmp_iota.p,
-                                            ws.f_order.p, static_cast<size_t>(M), 0, bits, s));
-        ws.sort_tmp.ensure(bytes);
-        XFE_CHECK(rocprim::radix_sort_pairs(ws.sort_tmp.p, bytes, ws.seg_r1.p, ws.f_key.p, ws.tmp_iota.p,
-                                            ws.f_order.p, static_cast<size_t>(M), 0, bits, s));
-        XFE_CHECK(launch_gather_ids(ws.seg_w.p, ws.f_order.p, ws.f_w.p, M, mgrid, s));
-        XFE_CHECK(launch_gather_ids(ws.seg_len.p, ws.f_order.p, ws.f_len.p, M, mgrid, s));
-        if (k2 >= 0) XFE_CHECK(launch_gather_ids(ws.seg_a.p, ws.f_order.p, ws.f_a.p, M, mgrid, s));
+        sort_pairs(ws.seg_rk.p, ws.p_rk.p, ws.tmp_iota.p, ws.p_order.p, M, Nw);
+        XFE_CHECK(launch_seg_gather<int>(ws.seg_r1.p, ws.p_order.p, ws.p_r1.p, M, mgrid, s));
+        XFE_CHECK(launch_seg_gather<int>(ws.seg_len.p, ws.p_order.p, ws.p_len.p, M, mgrid, s));
+        if (k2 >= 0) XFE_CHECK(launch_seg_gather<int>(ws.seg_a.p, ws.p_order.p, ws.p_a.p, M, mgrid, s));
+        XFE_CHECK(launch_seg_offsets(ws.p_rk.p, M, Nw, ws.p_wseg.p, mgrid, s));
+        XFE_CHECK(launch_seg_gather<float>(ws.f_invp.p, ws.w_pi.p, ws.f_invw.p, Nw, wgrid, s));
+        XFE_CHECK(launch_seg_gather<double>(ws.inv[p].p, ws.w_pi.p, ws.d_invw.p, Nw, wgrid, s));
+        // firm-major order of the renumbered segments (stable: by worker rank within a firm)
+        XFE_CHECK(launch_iota(ws.tmp_iota.p, M, mgrid, s));
+        sort_pairs(ws.p_r1.p, ws.f_key.p, ws.tmp_iota.p, ws.f_order.p, M, G1);
+        XFE_CHECK(launch_seg_gather<int>(ws.p_rk.p, ws.f_order.p, ws.f_w.p, M, mgrid, s));
+        XFE_CHECK(launch_seg_gather<int>(ws.p_len.p, ws.f_order.p, ws.f_len.p, M, mgrid, s));
+        if (k2 >= 0) XFE_CHECK(launch_seg_gather<int>(ws.p_a.p, ws.f_order.p, ws.f_a.p, M, mgrid, s));
         seg_grid_w = grid_for(Nw);
         seg_grid_f = mgrid;
         const size_t tabs = static_cast<size_t>(seg_grid_f) * (G2 + 1) * 8;
@@ -833,7 +866,7 @@ struct FastSolver {
             a.x1 = x + static_cast<size_t>(off[k1]) * C;
             a.x2 = k2 >= 0 ? x + static_cast<size_t>(off[k2]) * C : nullptr;
             a.inv_w = inv_w;
-            a.wseg = ws.wseg.p; a.seg_r1 = ws.seg_r1.p; a.seg_len = ws.seg_len.p; a.seg_a = ws.seg_a.p;
+            a.wseg = ws.p_wseg.p; a.seg_r1 = ws.p_r1.p; a.seg_len = ws.p_len.p; a.seg_a = ws.p_a.p;
             a.wmean = wmean;
             a.f_key = ws.f_key.p; a.f_w = ws.f_w.p; a.f_len = ws.f_len.p; a.f_a = ws.f_a.p;
             a.out1 = out + static_cast<size_t>(off[k1]) * C;
@@ -866,7 +899,7 @@ struct FastSolver {
         XFE_CHECK(hipMemsetAsync(ws.f_diag.p, 0, sizeof(double) * N, s));
         SegArgs<double> a;
         a.C = 1; a.c0 = 0; a.Nw = Nw; a.M = M; a.G2 = G2;
-        a.inv_w = ws.inv[p].p;
+        a.inv_w = ws.d_invw.p;
         a.f_key = ws.f_key.p; a.f_w = ws.f_w.p; a.f_len = ws.f_len.p; a.f_a = ws.f_a.p;
         a.out1 = ws.f_diag.p + off[k1];
         a.hpart = ws.d_hpart.p;
@@ -878,7 +911,7 @@ struct FastSolver {
 
     // Refinement residual b = b0 - S beta in FP64 through the segments.
     void residual_seg() {
-        seg_apply<double>(ws.f_beta.p, ws.f_tmp.p, ws.inv[p].p, ws.d_wmean.p, ws.d_hpart.p);
+        seg_apply<double>(ws.f_beta.p, ws.f_tmp.p, ws.d_invw.p, ws.d_wmean.p, ws.d_hpart.p);
         const int NC = N * C;
         XFE_CHECK(launch_sub(ws.f_b.p, ws.f_b0.p, ws.f_tmp.p, NC, 1, NC, grid_for(NC), s));
     }
@@ -918,6 +951,8 @@ struct FastSolver {
         vgrid = std::max(1, std::min(2048, grid_for(N)));
         ws.partials.ensure(static_cast<size_t>(std::max(vgrid, sv.tile_grid)) * 2 * kMaxCgCols + 64);
         ws.scalars.ensure(2 * kMaxCgCols);
+        ws.cg.ensure(kCgSlots);
+        XFE_CHECK(hipMemsetAsync(ws.cg.p, 0, sizeof(double) * kCgSlots, s));
         XFE_CHECK(launch_to_float(ws.inv[p].p, ws.f_invp.p, sv.G[p], grid_for(sv.G[p]), s));
         seg = setup_segments();
         // Jacobi preconditioner: diag(S) from the segments, or from runs of each
@@ -1005,7 +1040,7 @@ struct FastSolver {
 
     void apply32(const float* x, float* q) {
         if (seg) {
-            seg_apply<float>(x, q, ws.f_invp.p, ws.f_wmean.p, ws.f_hpart.p);
+            seg_apply<float>(x, q, ws.f_invw.p, ws.f_wmean.p, ws.f_hpart.p);
             return;
         }
         float* sp[2] = {nullptr, nullptr};
@@ -1037,52 +1072,39 @@ struct FastSolver {
     }
 
     // One inner FP32 PCG solve of S x = b / |b| per column; beta64 += |b| x.
+    // The CG scalars live on the device; the host polls the count of active
+    // columns every kPoll iterations (iterations after the last column stops
+    // are no-ops: alpha = 0).
     int inner_solve(double tol_in, int max_inner) {
-        double bb[kMaxCgCols], scale[2 * kMaxCgCols], rz[kMaxCgCols], rr[2 * kMaxCgCols],
-            pq[kMaxCgCols], alpha[kMaxCgCols], beta[kMaxCgCols];
-        bool active[kMaxCgCols];
+        constexpr int kPoll = 4;
+        double bb[kMaxCgCols], scale[2 * kMaxCgCols];
         coldot<double>(ws.f_b.p, ws.f_b.p, bb);
         for (int c = 0; c < C; ++c) {
             const double nb = std::sqrt(bb[c]);
             scale[c] = nb > 0.0 ? 1.0 / nb : 0.0;        // rhs scaling into FP32
             scale[kMaxCgCols + c] = nb;                  // and back
-            active[c] = nb > 0.0;
         }
+        for (int c = C; c < kMaxCgCols; ++c) scale[c] = scale[kMaxCgCols + c] = 0.0;
         upload(scale, ws.f_scale.p, 2 * kMaxCgCols);
-        XFE_CHECK(launch_cg_init(ws.f_b.p, ws.f_scale.p, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p,
-                                 ws.f_p.p, N, C, ws.partials.p, vgrid, s));
-        sv.fetch(ws.partials.p, vgrid, C, 0, rz);
+        const long long total = static_cast<long long>(N) * C;
+        const int g = cg_grid(total, C, kMaxGrid);
+        double* cg = ws.cg.p;
+        XFE_CHECK(launch_cg_init(ws.f_b.p, ws.f_scale.p, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p,
+                                 total, C, cg, g, s));
         int it = 0;
         for (it = 1; it <= max_inner; ++it) {
             apply32(ws.f_p.p, ws.f_q.p);
-            coldot<float>(ws.f_p.p, ws.f_q.p, pq);
-            bool any = false;
-            for (int c = 0; c < C; ++c) {
-                alpha[c] = 0.0;
-                if (active[c] && pq[c] > 0.0 && rz[c] > 0.0) alpha[c] = rz[c] / pq[c];
-                else active[c] = false;
+            XFE_CHECK(launch_cg_dot(ws.f_p.p, ws.f_q.p, total, C, cg, g, s));
+            XFE_CHECK(launch_cg_step(ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p, ws.f_q.p, ws.f_minv.p, total, C,
+                                     cg, tol_in * tol_in, g, s));
+            if (it % kPoll == 0 || it == max_inner) {
+                XFE_CHECK(hipMemcpyAsync(ws.host_scalars, cg + kCgCount, sizeof(double), hipMemcpyDeviceToHost, s));
+                XFE_CHECK(hipStreamSynchronize(s));
+                if (ws.host_scalars[0] == 0.0) break;
             }
-            for (int j = 0; j < C; ++j) ws.host_scalars[j] = alpha[j];
-            XFE_CHECK(hipMemcpyAsync(ws.lanes.p, ws.host_scalars, sizeof(double) * C,
-                                     hipMemcpyHostToDevice, s));
-            XFE_CHECK(launch_cg_step(ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p, ws.f_q.p, ws.f_minv.p,
-                                     ws.lanes.p, N, C, ws.partials.p, vgrid, s));
-            sv.fetch(ws.partials.p, vgrid, 2 * C, 0, rr);
-            for (int c = 0; c < C; ++c) {
-                if (!active[c]) { beta[c] = 0.0; continue; }
-                if (rr[c] <= tol_in * tol_in) { active[c] = false; beta[c] = 0.0; continue; }
-                beta[c] = rr[C + c] / rz[c];
-                rz[c] = rr[C + c];
-                any = true;
-            }
-            if (!any) break;
-            for (int j = 0; j < C; ++j) ws.host_scalars[j] = beta[j];
-            XFE_CHECK(hipMemcpyAsync(ws.lanes.p, ws.host_scalars, sizeof(double) * C,
-                                     hipMemcpyHostToDevice, s));
-            XFE_CHECK(launch_cg_dir(ws.f_p.p, ws.f_z.p, ws.lanes.p, N, C, grid_for(static_cast<long long>(N) * C), s));
         }
         XFE_CHECK(launch_refine_update(ws.f_beta.p, ws.f_x.p, ws.f_scale.p + kMaxCgCols, N, C,
-                                       grid_for(static_cast<long long>(N) * C), s));
+                                       grid_for(total), s));
         inner_total += std::min(it, max_inner);
         return it;
     }

@@ -142,8 +142,13 @@ hipError_t launch_seg_heads(const int* gp, const int* g1, const int* g2, int n, 
 hipError_t launch_seg_fill(const int* flags, const int* inc, const int* gp, const int* g1, const int* g2,
                            int n, int* seg_start, int* seg_w, int* seg_r1, int* seg_a, int grid,
                            hipStream_t s);
-hipError_t launch_seg_len(const int* seg_start, const int* seg_w, int M, int n, int Nw, int* seg_len,
-                          int* wseg, int grid, hipStream_t s);
+hipError_t launch_seg_len(const int* seg_start, int M, int n, int* seg_len, int grid, hipStream_t s);
+hipError_t launch_seg_offsets(const int* key, int M, int Nw, int* off, int grid, hipStream_t s);
+hipError_t launch_seg_primary(const int* wseg, const int* seg_r1, int Nw, int G1, int* pf, int grid,
+                              hipStream_t s);
+hipError_t launch_seg_rank(const int* perm, int m, int* rank, int grid, hipStream_t s);
+template <typename T>
+hipError_t launch_seg_gather(const T* src, const int* perm, T* dst, int m, int grid, hipStream_t s);
 template <typename T>
 hipError_t launch_seg_apply(const SegArgs<T>& a, int cc, bool has_y, bool diag, int grid_w, int grid_f,
                             hipStream_t s);
@@ -159,14 +164,28 @@ hipError_t launch_pull(const T* S, int C, int c0, int cc, const int* order, cons
                        T* sums, int grid, hipStream_t s);
 template <typename T>
 hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, int grid, hipStream_t s);
-hipError_t launch_cg_step(float* x, float* r, float* z, const float* p, const float* q,
-                          const float* minv, const double* alpha, int N, int C, double* partials,
-                          int grid, hipStream_t s);
-hipError_t launch_cg_dir(float* p, const float* z, const double* beta, int N, int C, int grid,
+// Scalar block of the device-side CG (doubles, kMaxCgCols per field).
+enum {
+    kCgAlpha = 0 * kMaxCgCols, kCgBeta = 1 * kMaxCgCols, kCgRz = 2 * kMaxCgCols,
+    kCgActive = 3 * kMaxCgCols, kCgPq = 4 * kMaxCgCols, kCgRr = 5 * kMaxCgCols,
+    kCgRzNew = 6 * kMaxCgCols, kCgCount = 7 * kMaxCgCols, kCgSlots = 7 * kMaxCgCols + 8
+};
+// grid for the CG vector kernels: a multiple of C (each thread keeps one column)
+inline int cg_grid(long long total, int C, int max_grid) {
+    long long per = (total + static_cast<long long>(kBlock) * C - 1) / (static_cast<long long>(kBlock) * C);
+    const long long cap = max_grid / C > 0 ? max_grid / C : 1;
+    if (per > cap) per = cap;
+    if (per < 1) per = 1;
+    return static_cast<int>(per * C);
+}
+// pq -> alpha (cg_dot + cg_alpha)
+hipError_t launch_cg_dot(const float* a, const float* b, long long total, int C, double* cg, int grid,
                          hipStream_t s);
-hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x,
-                          float* r, float* z, float* p, int N, int C, double* partials, int grid,
-                          hipStream_t s);
+// x, r, z update, r.r and r.z -> beta and the active count, then p = z + beta p
+hipError_t launch_cg_step(float* x, float* r, float* z, float* p, const float* q, const float* minv,
+                          long long total, int C, double* cg, double tol2, int grid, hipStream_t s);
+hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x, float* r,
+                          float* z, float* p, long long total, int C, double* cg, int grid, hipStream_t s);
 hipError_t launch_refine_update(double* beta, const float* x, const double* inv_scale, int N, int C,
                                 int grid, hipStream_t s);
 hipError_t launch_schur_diag(const int* tile_start, int num_tiles, const int* gid_p,

@@ -282,86 +282,146 @@ vdot_kernel(const T* a, const T* b, int N, int C, double* partials) {
     }
 }
 
-// x += alpha p; r -= alpha q; z = minv r; partials: (r.r, r.z) per column
+// Device-side CG over level-major N x C blocks.  The vector kernels use a
+// grid whose thread stride is a multiple of C, so every thread always meets
+// the same column c = (global thread id) % C and keeps one FP64 accumulator;
+// block sums go through LDS into FP64 accumulators of the scalar block `cg`
+// (layout kCg* in absorb_kernels.h), which the scalar kernels consume and clear.
+
+namespace {
+__device__ __forceinline__ void cg_flush(double v, int c, int C, double* gacc) {
+    __shared__ double red[kMaxCgCols];
+    if (threadIdx.x < C) red[threadIdx.x] = 0.0;
+    __syncthreads();
+    atomicAdd(&red[c], v);
+    __syncthreads();
+    if (threadIdx.x < C) atomicAdd(&gacc[threadIdx.x], red[threadIdx.x]);
+}
+}  // namespace
+
+// cg[kCgPq + c] += sum_j a[j, c] b[j, c]
+__global__ void __launch_bounds__(kBlock)
+cg_dot_kernel(const float* a, const float* b, long long total, int C, double* cg) {
+    const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const int c = static_cast<int>(g0 % C);
+    double acc = 0.0;
+    for (long long j = g0; j < total; j += stride) acc += static_cast<double>(a[j]) * static_cast<double>(b[j]);
+    cg_flush(acc, c, C, cg + kCgPq);
+}
+
+// alpha = rz / pq for active columns (a column with pq <= 0 or rz <= 0 stops)
+__global__ void cg_alpha_kernel(double* cg, int C) {
+    const int c = threadIdx.x;
+    if (c >= C) return;
+    const double pq = cg[kCgPq + c];
+    const double rz = cg[kCgRz + c];
+    cg[kCgPq + c] = 0.0;
+    double al = 0.0;
+    if (cg[kCgActive + c] != 0.0) {
+        if (pq > 0.0 && rz > 0.0) al = rz / pq;
+        else cg[kCgActive + c] = 0.0;
+    }
+    cg[kCgAlpha + c] = al;
+}
+
+// x += alpha p; r -= alpha q; z = minv r; cg[kCgRr] += r.r, cg[kCgRzNew] += r.z
 __global__ void __launch_bounds__(kBlock)
 cg_step_kernel(float* x, float* r, float* z, const float* p, const float* qv, const float* minv,
-               const double* alpha, int N, int C, double* partials) {
-    __shared__ double red[kBlock / kWave][2 * kMaxCgCols];
-    double rr[kMaxCgCols], rz[kMaxCgCols];
-    for (int c = 0; c < C; ++c) { rr[c] = 0.0; rz[c] = 0.0; }
-    for (long long g = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; g < N;
-         g += static_cast<long long>(gridDim.x) * kBlock) {
-        const float mi = minv[g];
-        for (int c = 0; c < C; ++c) {
-            const long long j = g * C + c;
-            const float al = static_cast<float>(alpha[c]);
-            x[j] += al * p[j];
-            const float rn = r[j] - al * qv[j];
-            r[j] = rn;
-            const float zn = mi * rn;
-            z[j] = zn;
-            rr[c] += static_cast<double>(rn) * rn;
-            rz[c] += static_cast<double>(rn) * zn;
-        }
+               long long total, int C, double* cg) {
+    const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const int c = static_cast<int>(g0 % C);
+    const float al = static_cast<float>(cg[kCgAlpha + c]);
+    double rr = 0.0, rz = 0.0;
+    for (long long j = g0; j < total; j += stride) {
+        x[j] += al * p[j];
+        const float rn = r[j] - al * qv[j];
+        r[j] = rn;
+        const float zn = minv[j / C] * rn;
+        z[j] = zn;
+        rr += static_cast<double>(rn) * rn;
+        rz += static_cast<double>(rn) * zn;
     }
-    for (int c = 0; c < C; ++c) {
-        const double s1 = wave_sum(rr[c]);
-        const double s2 = wave_sum(rz[c]);
-        if (lane_id() == 0) {
-            red[threadIdx.x / kWave][c] = s1;
-            red[threadIdx.x / kWave][kMaxCgCols + c] = s2;
+    cg_flush(rr, c, C, cg + kCgRr);
+    cg_flush(rz, c, C, cg + kCgRzNew);
+}
+
+// beta = rz_new / rz; a column whose r.r <= tol2 stops; cg[kCgCount] = active columns
+__global__ void cg_beta_kernel(double* cg, int C, double tol2) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int c = threadIdx.x;
+    if (c < C) {
+        const double rr = cg[kCgRr + c];
+        const double rzn = cg[kCgRzNew + c];
+        cg[kCgRr + c] = 0.0;
+        cg[kCgRzNew + c] = 0.0;
+        double be = 0.0;
+        if (cg[kCgActive + c] != 0.0) {
+            if (rr <= tol2) {
+                cg[kCgActive + c] = 0.0;
+            } else {
+                be = rzn / cg[kCgRz + c];
+                cg[kCgRz + c] = rzn;
+                atomicAdd(&cnt, 1);
+            }
         }
+        cg[kCgBeta + c] = be;
     }
     __syncthreads();
-    if (threadIdx.x < 2 * C) {
-        const int c = threadIdx.x < C ? threadIdx.x : threadIdx.x - C + kMaxCgCols;
-        double s = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) s += red[w][c];
-        partials[static_cast<size_t>(blockIdx.x) * 2 * C + threadIdx.x] = s;
-    }
+    if (threadIdx.x == 0) cg[kCgCount] = cnt;
 }
 
 // p = z + beta p
 __global__ void __launch_bounds__(kBlock)
-cg_dir_kernel(float* p, const float* z, const double* beta, int N, int C) {
-    const long long total = static_cast<long long>(N) * C;
-    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
-         j += static_cast<long long>(gridDim.x) * kBlock) {
-        p[j] = z[j] + static_cast<float>(beta[j % C]) * p[j];
-    }
+cg_dir_kernel(float* p, const float* z, const double* cg, long long total, int C) {
+    const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const float be = static_cast<float>(cg[kCgBeta + static_cast<int>(g0 % C)]);
+    for (long long j = g0; j < total; j += stride) p[j] = z[j] + be * p[j];
 }
 
-// CG start: r = b * scale_c (to FP32), x = 0, z = minv r, p = z; partials r.z
+// CG start: r = b * scale_c (to FP32), x = 0, z = minv r, p = z; cg[kCgRzNew] += r.z
 __global__ void __launch_bounds__(kBlock)
 cg_init_kernel(const double* b, const double* scale, const float* minv, float* x, float* r, float* z,
-               float* p, int N, int C, double* partials) {
-    __shared__ double red[kBlock / kWave][kMaxCgCols];
-    double rz[kMaxCgCols];
-    for (int c = 0; c < C; ++c) rz[c] = 0.0;
-    for (long long g = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; g < N;
-         g += static_cast<long long>(gridDim.x) * kBlock) {
-        const float mi = minv[g];
-        for (int c = 0; c < C; ++c) {
-            const long long j = g * C + c;
-            const float rv = static_cast<float>(b[j] * scale[c]);
-            x[j] = 0.0f;
-            r[j] = rv;
-            const float zv = mi * rv;
-            z[j] = zv;
-            p[j] = zv;
-            rz[c] += static_cast<double>(rv) * zv;
-        }
+               float* p, long long total, int C, double* cg) {
+    const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const int c = static_cast<int>(g0 % C);
+    const double sc = scale[c];
+    double rz = 0.0;
+    for (long long j = g0; j < total; j += stride) {
+        const float rv = static_cast<float>(b[j] * sc);
+        x[j] = 0.0f;
+        r[j] = rv;
+        const float zv = minv[j / C] * rv;
+        z[j] = zv;
+        p[j] = zv;
+        rz += static_cast<double>(rv) * zv;
     }
-    for (int c = 0; c < C; ++c) {
-        const double s = wave_sum(rz[c]);
-        if (lane_id() == 0) red[threadIdx.x / kWave][c] = s;
+    cg_flush(rz, c, C, cg + kCgRzNew);
+}
+
+// rz = r.z; columns with a zero right-hand side or r.z <= 0 start inactive
+__global__ void cg_start_kernel(double* cg, const double* scale, int C) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int c = threadIdx.x;
+    if (c < C) {
+        const double rz = cg[kCgRzNew + c];
+        cg[kCgRzNew + c] = 0.0;
+        cg[kCgRz + c] = rz;
+        const bool act = scale[c] > 0.0 && rz > 0.0;
+        cg[kCgActive + c] = act ? 1.0 : 0.0;
+        cg[kCgAlpha + c] = 0.0;
+        cg[kCgBeta + c] = 0.0;
+        if (act) atomicAdd(&cnt, 1);
     }
     __syncthreads();
-    if (threadIdx.x < C) {
-        double s = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) s += red[w][threadIdx.x];
-        partials[static_cast<size_t>(blockIdx.x) * C + threadIdx.x] = s;
-    }
+    if (threadIdx.x == 0) cg[kCgCount] = cnt;
 }
 
 // beta64 += x32 / scale_c
@@ -511,25 +571,26 @@ hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, i
 template hipError_t launch_vdot<float>(const float*, const float*, int, int, double*, int, hipStream_t);
 template hipError_t launch_vdot<double>(const double*, const double*, int, int, double*, int, hipStream_t);
 
-hipError_t launch_cg_step(float* x, float* r, float* z, const float* p, const float* q,
-                          const float* minv, const double* alpha, int N, int C, double* partials,
-                          int grid, hipStream_t s) {
-    hipLaunchKernelGGL(cg_step_kernel, dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, alpha, N,
-                       C, partials);
-    return hipGetLastError();
-}
-
-hipError_t launch_cg_dir(float* p, const float* z, const double* beta, int N, int C, int grid,
+hipError_t launch_cg_dot(const float* a, const float* b, long long total, int C, double* cg, int grid,
                          hipStream_t s) {
-    hipLaunchKernelGGL(cg_dir_kernel, dim3(grid), dim3(kBlock), 0, s, p, z, beta, N, C);
+    hipLaunchKernelGGL(cg_dot_kernel, dim3(grid), dim3(kBlock), 0, s, a, b, total, C, cg);
+    hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kMaxCgCols), 0, s, cg, C);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x,
-                          float* r, float* z, float* p, int N, int C, double* partials, int grid,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(cg_init_kernel, dim3(grid), dim3(kBlock), 0, s, b, scale, minv, x, r, z, p,
-                       N, C, partials);
+hipError_t launch_cg_step(float* x, float* r, float* z, float* p, const float* q, const float* minv,
+                          long long total, int C, double* cg, double tol2, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(cg_step_kernel, dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, total, C, cg);
+    hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kMaxCgCols), 0, s, cg, C, tol2);
+    hipLaunchKernelGGL(cg_dir_kernel, dim3(grid), dim3(kBlock), 0, s, p, z, cg, total, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x, float* r,
+                          float* z, float* p, long long total, int C, double* cg, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(cg_init_kernel, dim3(grid), dim3(kBlock), 0, s, b, scale, minv, x, r, z, p, total, C,
+                       cg);
+    hipLaunchKernelGGL(cg_start_kernel, dim3(1), dim3(kMaxCgCols), 0, s, cg, scale, C);
     return hipGetLastError();
 }
 

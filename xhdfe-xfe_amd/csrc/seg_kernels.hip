@@ -84,22 +84,45 @@ seg_fill_kernel(const int* flags, const int* inc, const int* gp, const int* g1, 
     }
 }
 
-// Segment lengths, and wseg[w] = first segment of worker w (wseg[Nw] = M;
-// workers without rows get the start of the next worker).
-__global__ void __launch_bounds__(kBlock)
-seg_len_kernel(const int* seg_start, const int* seg_w, int M, int n, int Nw, int* seg_len, int* wseg) {
+// Segment lengths from their start rows.
+__global__ void __launch_bounds__(kBlock) seg_len_kernel(const int* seg_start, int M, int n, int* seg_len) {
     for (int s = blockIdx.x * kBlock + threadIdx.x; s < M; s += gridDim.x * kBlock) {
         const int e = s + 1 < M ? seg_start[s + 1] : n;
         seg_len[s] = e - seg_start[s];
-        const int w = seg_w[s];
-        const int prev = s > 0 ? seg_w[s - 1] : -1;
+    }
+}
+
+// off[w] = first segment whose (ascending) key is >= w, for w = 0..Nw
+// (off[Nw] = M): segment offsets per worker, empty workers included.
+__global__ void __launch_bounds__(kBlock) seg_offsets_kernel(const int* key, int M, int Nw, int* off) {
+    for (int s = blockIdx.x * kBlock + threadIdx.x; s < M; s += gridDim.x * kBlock) {
+        const int w = key[s];
+        const int prev = s > 0 ? key[s - 1] : -1;
         if (w != prev) {
-            for (int v = prev + 1; v <= w; ++v) wseg[v] = s;
+            for (int v = prev + 1; v <= w; ++v) off[v] = s;
         }
         if (s == M - 1) {
-            for (int v = w + 1; v <= Nw; ++v) wseg[v] = M;
+            for (int v = w + 1; v <= Nw; ++v) off[v] = M;
         }
     }
+}
+
+// Primary firm of each worker (the firm of its first segment; G1 when the
+// worker has no rows, so that it sorts last).
+__global__ void __launch_bounds__(kBlock)
+seg_primary_kernel(const int* wseg, const int* seg_r1, int Nw, int G1, int* pf) {
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < Nw; w += gridDim.x * kBlock)
+        pf[w] = wseg[w] < wseg[w + 1] ? seg_r1[wseg[w]] : G1;
+}
+
+// rank[perm[t]] = t
+__global__ void __launch_bounds__(kBlock) seg_rank_kernel(const int* perm, int m, int* rank) {
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < m; t += gridDim.x * kBlock) rank[perm[t]] = t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) seg_gather_kernel(const T* src, const int* perm, T* dst, int m) {
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < m; t += gridDim.x * kBlock) dst[t] = src[perm[t]];
 }
 
 // ------------------------------------------------------------------ apply
@@ -261,12 +284,35 @@ hipError_t launch_seg_fill(const int* flags, const int* inc, const int* gp, cons
     return hipGetLastError();
 }
 
-hipError_t launch_seg_len(const int* seg_start, const int* seg_w, int M, int n, int Nw, int* seg_len,
-                          int* wseg, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(seg_len_kernel, dim3(grid), dim3(kBlock), 0, s, seg_start, seg_w, M, n, Nw,
-                       seg_len, wseg);
+hipError_t launch_seg_len(const int* seg_start, int M, int n, int* seg_len, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(seg_len_kernel, dim3(grid), dim3(kBlock), 0, s, seg_start, M, n, seg_len);
     return hipGetLastError();
 }
+
+hipError_t launch_seg_offsets(const int* key, int M, int Nw, int* off, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(seg_offsets_kernel, dim3(grid), dim3(kBlock), 0, s, key, M, Nw, off);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_primary(const int* wseg, const int* seg_r1, int Nw, int G1, int* pf, int grid,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(seg_primary_kernel, dim3(grid), dim3(kBlock), 0, s, wseg, seg_r1, Nw, G1, pf);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_rank(const int* perm, int m, int* rank, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(seg_rank_kernel, dim3(grid), dim3(kBlock), 0, s, perm, m, rank);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_seg_gather(const T* src, const int* perm, T* dst, int m, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((seg_gather_kernel<T>), dim3(grid), dim3(kBlock), 0, s, src, perm, dst, m);
+    return hipGetLastError();
+}
+template hipError_t launch_seg_gather<float>(const float*, const int*, float*, int, int, hipStream_t);
+template hipError_t launch_seg_gather<double>(const double*, const int*, double*, int, int, hipStream_t);
+template hipError_t launch_seg_gather<int>(const int*, const int*, int*, int, int, hipStream_t);
 
 template <typename T, int CC>
 static hipError_t seg_apply_cc(const SegArgs<T>& a, bool has_y, bool diag, int grid_w, int grid_f,
