@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "absorb_kernels.h"
@@ -77,7 +78,7 @@ struct Workspace {
     DevBuf<int> sp_key[XFE_MAX_DIMS], sp_order[XFE_MAX_DIMS], sp_base[XFE_MAX_DIMS];
     DevBuf<double> sp_sums[XFE_MAX_DIMS];
     // mixed-precision solver (coefficient space of the non-sorted dimensions)
-    DevBuf<double> f_beta, f_b, f_diag, f_counts, f_scale, cg;
+    DevBuf<double> f_beta, f_b, f_diag, f_counts, f_scale, cg, cg_part;
     DevBuf<float> f_x, f_r, f_z, f_p, f_q, f_minv, f_invp;
     DevBuf<float> f_sp[XFE_MAX_DIMS];
     // segment form of the Schur operator (worker-major and firm-major records)
@@ -85,6 +86,9 @@ struct Workspace {
     DevBuf<int> w_pf, w_pi, w_rank, seg_rk, p_order, p_rk, p_r1, p_len, p_a, p_wseg;
     DevBuf<float> f_wmean, f_hpart, f_invw;
     DevBuf<double> d_wmean, d_hpart, seg_tot, f_b0, f_tmp, d_invw;
+    DevBuf<int> seg_bk[2];
+    DevBuf<float> f_bv[2];
+    DevBuf<double> d_bv[2];
     double* host_scalars = nullptr;  // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -104,7 +108,7 @@ struct Workspace {
             sp_key[d].release(); sp_order[d].release(); sp_base[d].release(); sp_sums[d].release();
             f_sp[d].release();
         }
-        cg.release(); f_beta.release(); f_b.release(); f_diag.release(); f_counts.release(); f_scale.release();
+        cg.release(); cg_part.release(); f_beta.release(); f_b.release(); f_diag.release(); f_counts.release(); f_scale.release();
         f_x.release(); f_r.release(); f_z.release(); f_p.release(); f_q.release(); f_minv.release();
         f_invp.release();
         seg_start.release(); seg_w.release(); seg_r1.release(); seg_len.release(); seg_a.release();
@@ -114,6 +118,7 @@ struct Workspace {
         w_pf.release(); w_pi.release(); w_rank.release(); seg_rk.release(); p_order.release();
         p_rk.release(); p_r1.release(); p_len.release(); p_a.release(); p_wseg.release();
         f_invw.release(); d_invw.release();
+        for (int k = 0; k < 2; ++k) { seg_bk[k].release(); f_bv[k].release(); d_bv[k].release(); }
         if (host_scalars) (void)hipHostFree(host_scalars);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -847,16 +852,44 @@ struct FastSolver {
         ws.f_hpart.ensure(tabs);
         ws.d_hpart.ensure(tabs);
         ws.seg_tot.ensure(static_cast<size_t>(G2 + 1) * 8);
+        const size_t nb = 2 * ((static_cast<size_t>(M) + kWave - 1) / kWave);
+        for (int k = 0; k < 2; ++k) {
+            ws.seg_bk[k].ensure(nb);
+            ws.f_bv[k].ensure(nb * 8);
+            ws.d_bv[k].ensure(nb * 8);
+        }
         const size_t NC = static_cast<size_t>(N) * C;
         ws.f_b0.ensure(NC);
         ws.f_tmp.ensure(NC);
         return true;
     }
 
+    template <typename T>
+    T* bv_buf(int k) {
+        if constexpr (std::is_same<T, float>::value) return ws.f_bv[k].p;
+        else return ws.d_bv[k].p;
+    }
+
+    // Finish the firm reduction: reduce the boundary lists level by level
+    // (stride: the row stride of out1).
+    template <typename T>
+    void seg_boundaries(T* out1, int stride, int c0, int cc) {
+        int m = static_cast<int>(2 * ((static_cast<long long>(M) + kWave - 1) / kWave));
+        int cur = 0;
+        for (;;) {
+            const int fin = m <= kWave ? 1 : 0;
+            XFE_CHECK(launch_seg_bnd<T>(ws.seg_bk[cur].p, bv_buf<T>(cur), m, out1, stride, c0, cc,
+                                        ws.seg_bk[1 - cur].p, bv_buf<T>(1 - cur), fin, grid_for(m), s));
+            if (fin) break;
+            m = static_cast<int>(2 * ((static_cast<long long>(m) + kWave - 1) / kWave));
+            cur = 1 - cur;
+        }
+    }
+
     // out = S x through the segment records (T = float for the inner CG,
     // double for the refinement residual).
     template <typename T>
-    void seg_apply(const T* x, T* out, const T* inv_w, T* wmean, T* hpart) {
+    void seg_apply(const T* x, T* out, const T* inv_w, T* wmean, double* hpart) {
         XFE_CHECK(hipMemsetAsync(out, 0, sizeof(T) * static_cast<size_t>(N) * C, s));
         for (int ch = 0; ch < sv.nchunks; ++ch) {
             const int c0 = ch * 8;
@@ -871,8 +904,11 @@ struct FastSolver {
             a.f_key = ws.f_key.p; a.f_w = ws.f_w.p; a.f_len = ws.f_len.p; a.f_a = ws.f_a.p;
             a.out1 = out + static_cast<size_t>(off[k1]) * C;
             a.hpart = hpart;
+            a.bkey = ws.seg_bk[0].p;
+            a.bval = bv_buf<T>(0);
             if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
             XFE_CHECK(launch_seg_apply<T>(a, cc, k2 >= 0, false, seg_grid_w, seg_grid_f, s));
+            seg_boundaries<T>(a.out1, C, c0, cc);
             if (k2 >= 0)
                 XFE_CHECK(launch_seg_year<T>(hpart, seg_grid_f, G2, cc, ws.seg_tot.p, ws.wsum[rdim[k2]].p,
                                              a.x2, C, c0, out + static_cast<size_t>(off[k2]) * C, s));
@@ -903,7 +939,10 @@ struct FastSolver {
         a.f_key = ws.f_key.p; a.f_w = ws.f_w.p; a.f_len = ws.f_len.p; a.f_a = ws.f_a.p;
         a.out1 = ws.f_diag.p + off[k1];
         a.hpart = ws.d_hpart.p;
+        a.bkey = ws.seg_bk[0].p;
+        a.bval = bv_buf<double>(0);
         XFE_CHECK(launch_seg_apply<double>(a, 1, k2 >= 0, true, seg_grid_w, seg_grid_f, s));
+        seg_boundaries<double>(a.out1, 1, 0, 1);
         if (k2 >= 0)
             XFE_CHECK(launch_seg_year<double>(ws.d_hpart.p, seg_grid_f, G2, 1, ws.seg_tot.p,
                                               ws.wsum[rdim[k2]].p, nullptr, 1, 0, ws.f_diag.p + off[k2], s));
@@ -952,6 +991,7 @@ struct FastSolver {
         ws.partials.ensure(static_cast<size_t>(std::max(vgrid, sv.tile_grid)) * 2 * kMaxCgCols + 64);
         ws.scalars.ensure(2 * kMaxCgCols);
         ws.cg.ensure(kCgSlots);
+        ws.cg_part.ensure(2 * static_cast<size_t>(kMaxGrid) * kMaxCgCols);  // 2 x grid x C partials
         XFE_CHECK(hipMemsetAsync(ws.cg.p, 0, sizeof(double) * kCgSlots, s));
         XFE_CHECK(launch_to_float(ws.inv[p].p, ws.f_invp.p, sv.G[p], grid_for(sv.G[p]), s));
         seg = setup_segments();
@@ -1040,7 +1080,7 @@ struct FastSolver {
 
     void apply32(const float* x, float* q) {
         if (seg) {
-            seg_apply<float>(x, q, ws.f_invw.p, ws.f_wmean.p, ws.f_hpart.p);
+            seg_apply<float>(x, q, ws.f_invw.p, ws.f_wmean.p, ws.d_hpart.p);
             return;
         }
         float* sp[2] = {nullptr, nullptr};
@@ -1089,14 +1129,15 @@ struct FastSolver {
         const long long total = static_cast<long long>(N) * C;
         const int g = cg_grid(total, C, kMaxGrid);
         double* cg = ws.cg.p;
+        double* part = ws.cg_part.p;
         XFE_CHECK(launch_cg_init(ws.f_b.p, ws.f_scale.p, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p,
-                                 total, C, cg, g, s));
+                                 total, C, cg, part, g, s));
         int it = 0;
         for (it = 1; it <= max_inner; ++it) {
             apply32(ws.f_p.p, ws.f_q.p);
-            XFE_CHECK(launch_cg_dot(ws.f_p.p, ws.f_q.p, total, C, cg, g, s));
+            XFE_CHECK(launch_cg_dot(ws.f_p.p, ws.f_q.p, total, C, cg, part, g, s));
             XFE_CHECK(launch_cg_step(ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p, ws.f_q.p, ws.f_minv.p, total, C,
-                                     cg, tol_in * tol_in, g, s));
+                                     cg, part, tol_in * tol_in, g, s));
             if (it % kPoll == 0 || it == max_inner) {
                 XFE_CHECK(hipMemcpyAsync(ws.host_scalars, cg + kCgCount, sizeof(double), hipMemcpyDeviceToHost, s));
                 XFE_CHECK(hipStreamSynchronize(s));

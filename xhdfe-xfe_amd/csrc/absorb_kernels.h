@@ -113,7 +113,7 @@ struct SchurArgs {
 };
 
 // Arguments of the segment form of the Schur operator (seg_kernels.hip).
-constexpr int kSegMaxRange = 511;  // most levels of the range dimension r2 (LDS tables)
+constexpr int kSegMaxRange = 200;  // most levels of the range dimension r2 (LDS tables <= 64 KB)
 template <typename T>
 struct SegArgs {
     int C = 0;                    // stride of the level-major coefficient tables
@@ -134,7 +134,9 @@ struct SegArgs {
     const int* f_len = nullptr;
     const int* f_a = nullptr;
     T* out1 = nullptr;            // (S x)_1, zeroed before the pass
-    T* hpart = nullptr;           // per-block (G2 + 1) x cc year differences
+    double* hpart = nullptr;      // per-block (G2 + 1) x cc year differences
+    int* bkey = nullptr;          // boundary list of the firm reduction:
+    T* bval = nullptr;            // 2 entries (key, cc values) per wave of segments
 };
 
 hipError_t launch_seg_heads(const int* gp, const int* g1, const int* g2, int n, int* flags, int grid,
@@ -153,7 +155,10 @@ template <typename T>
 hipError_t launch_seg_apply(const SegArgs<T>& a, int cc, bool has_y, bool diag, int grid_w, int grid_f,
                             hipStream_t s);
 template <typename T>
-hipError_t launch_seg_year(const T* hpart, int nblk, int G2, int cc, double* tot, const double* counts,
+hipError_t launch_seg_bnd(const int* kin, const T* vin, int m, T* out, int C, int c0, int cc, int* kout, T* vout,
+                          int final, int grid, hipStream_t s);
+template <typename T>
+hipError_t launch_seg_year(const double* hpart, int nblk, int G2, int cc, double* tot, const double* counts,
                            const T* x2, int C, int c0, T* out2, hipStream_t s);
 
 template <typename T>
@@ -178,14 +183,15 @@ inline int cg_grid(long long total, int C, int max_grid) {
     if (per < 1) per = 1;
     return static_cast<int>(per * C);
 }
-// pq -> alpha (cg_dot + cg_alpha)
-hipError_t launch_cg_dot(const float* a, const float* b, long long total, int C, double* cg, int grid,
-                         hipStream_t s);
+// pq -> alpha (cg_dot + cg_alpha); part: grid x C doubles (2 x for cg_step)
+hipError_t launch_cg_dot(const float* a, const float* b, long long total, int C, double* cg, double* part,
+                         int grid, hipStream_t s);
 // x, r, z update, r.r and r.z -> beta and the active count, then p = z + beta p
 hipError_t launch_cg_step(float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                          long long total, int C, double* cg, double tol2, int grid, hipStream_t s);
+                          long long total, int C, double* cg, double* part, double tol2, int grid, hipStream_t s);
 hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x, float* r,
-                          float* z, float* p, long long total, int C, double* cg, int grid, hipStream_t s);
+                          float* z, float* p, long long total, int C, double* cg, double* part, int grid,
+                          hipStream_t s);
 hipError_t launch_refine_update(double* beta, const float* x, const double* inv_scale, int N, int C,
                                 int grid, hipStream_t s);
 hipError_t launch_schur_diag(const int* tile_start, int num_tiles, const int* gid_p,

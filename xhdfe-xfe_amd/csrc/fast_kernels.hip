@@ -285,38 +285,60 @@ vdot_kernel(const T* a, const T* b, int N, int C, double* partials) {
 // Device-side CG over level-major N x C blocks.  The vector kernels use a
 // grid whose thread stride is a multiple of C, so every thread always meets
 // the same column c = (global thread id) % C and keeps one FP64 accumulator;
-// block sums go through LDS into FP64 accumulators of the scalar block `cg`
-// (layout kCg* in absorb_kernels.h), which the scalar kernels consume and clear.
+// each block stores its column sums in a partials array (plain stores), and
+// the single-block scalar kernels reduce the partials and update the scalar
+// block `cg` (layout kCg* in absorb_kernels.h).
 
 namespace {
-__device__ __forceinline__ void cg_flush(double v, int c, int C, double* gacc) {
+// part[block * C + c] = sum of v over the block's threads of column c
+__device__ __forceinline__ void cg_block_store(double v, int c, int C, double* part) {
     __shared__ double red[kMaxCgCols];
     if (threadIdx.x < C) red[threadIdx.x] = 0.0;
     __syncthreads();
     atomicAdd(&red[c], v);
     __syncthreads();
-    if (threadIdx.x < C) atomicAdd(&gacc[threadIdx.x], red[threadIdx.x]);
+    if (threadIdx.x < C) part[static_cast<size_t>(blockIdx.x) * C + threadIdx.x] = red[threadIdx.x];
+    __syncthreads();
+}
+
+// out[c] = sum over nblk blocks of part[b * C + c], c < C (one block of kBlock)
+__device__ __forceinline__ void cg_reduce_parts(const double* part, int nblk, int C, double* out) {
+    __shared__ double red[kBlock / kWave];
+    for (int c = 0; c < C; ++c) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += kBlock) acc += part[static_cast<size_t>(b) * C + c];
+        acc = wave_sum(acc);
+        if (lane_id() == 0) red[threadIdx.x / kWave] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
+            out[c] = t;
+        }
+        __syncthreads();
+    }
 }
 }  // namespace
 
-// cg[kCgPq + c] += sum_j a[j, c] b[j, c]
+// part1[block, c] = sum_j a[j, c] b[j, c]
 __global__ void __launch_bounds__(kBlock)
-cg_dot_kernel(const float* a, const float* b, long long total, int C, double* cg) {
+cg_dot_kernel(const float* a, const float* b, long long total, int C, double* part1) {
     const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     const int c = static_cast<int>(g0 % C);
     double acc = 0.0;
     for (long long j = g0; j < total; j += stride) acc += static_cast<double>(a[j]) * static_cast<double>(b[j]);
-    cg_flush(acc, c, C, cg + kCgPq);
+    cg_block_store(acc, c, C, part1);
 }
 
 // alpha = rz / pq for active columns (a column with pq <= 0 or rz <= 0 stops)
-__global__ void cg_alpha_kernel(double* cg, int C) {
+__global__ void __launch_bounds__(kBlock) cg_alpha_kernel(double* cg, const double* part1, int nblk, int C) {
+    __shared__ double pqs[kMaxCgCols];
+    cg_reduce_parts(part1, nblk, C, pqs);
     const int c = threadIdx.x;
     if (c >= C) return;
-    const double pq = cg[kCgPq + c];
+    const double pq = pqs[c];
     const double rz = cg[kCgRz + c];
-    cg[kCgPq + c] = 0.0;
     double al = 0.0;
     if (cg[kCgActive + c] != 0.0) {
         if (pq > 0.0 && rz > 0.0) al = rz / pq;
@@ -325,10 +347,10 @@ __global__ void cg_alpha_kernel(double* cg, int C) {
     cg[kCgAlpha + c] = al;
 }
 
-// x += alpha p; r -= alpha q; z = minv r; cg[kCgRr] += r.r, cg[kCgRzNew] += r.z
+// x += alpha p; r -= alpha q; z = minv r; part1 = r.r, part2 = r.z per block and column
 __global__ void __launch_bounds__(kBlock)
 cg_step_kernel(float* x, float* r, float* z, const float* p, const float* qv, const float* minv,
-               long long total, int C, double* cg) {
+               long long total, int C, const double* cg, double* part1, double* part2) {
     const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     const int c = static_cast<int>(g0 % C);
@@ -343,21 +365,23 @@ cg_step_kernel(float* x, float* r, float* z, const float* p, const float* qv, co
         rr += static_cast<double>(rn) * rn;
         rz += static_cast<double>(rn) * zn;
     }
-    cg_flush(rr, c, C, cg + kCgRr);
-    cg_flush(rz, c, C, cg + kCgRzNew);
+    cg_block_store(rr, c, C, part1);
+    cg_block_store(rz, c, C, part2);
 }
 
 // beta = rz_new / rz; a column whose r.r <= tol2 stops; cg[kCgCount] = active columns
-__global__ void cg_beta_kernel(double* cg, int C, double tol2) {
+__global__ void __launch_bounds__(kBlock)
+cg_beta_kernel(double* cg, const double* part1, const double* part2, int nblk, int C, double tol2) {
+    __shared__ double rrs[kMaxCgCols], rzs[kMaxCgCols];
     __shared__ int cnt;
+    cg_reduce_parts(part1, nblk, C, rrs);
+    cg_reduce_parts(part2, nblk, C, rzs);
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
     const int c = threadIdx.x;
     if (c < C) {
-        const double rr = cg[kCgRr + c];
-        const double rzn = cg[kCgRzNew + c];
-        cg[kCgRr + c] = 0.0;
-        cg[kCgRzNew + c] = 0.0;
+        const double rr = rrs[c];
+        const double rzn = rzs[c];
         double be = 0.0;
         if (cg[kCgActive + c] != 0.0) {
             if (rr <= tol2) {
@@ -383,10 +407,10 @@ cg_dir_kernel(float* p, const float* z, const double* cg, long long total, int C
     for (long long j = g0; j < total; j += stride) p[j] = z[j] + be * p[j];
 }
 
-// CG start: r = b * scale_c (to FP32), x = 0, z = minv r, p = z; cg[kCgRzNew] += r.z
+// CG start: r = b * scale_c (to FP32), x = 0, z = minv r, p = z; part1 = r.z
 __global__ void __launch_bounds__(kBlock)
 cg_init_kernel(const double* b, const double* scale, const float* minv, float* x, float* r, float* z,
-               float* p, long long total, int C, double* cg) {
+               float* p, long long total, int C, double* part1) {
     const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     const int c = static_cast<int>(g0 % C);
@@ -401,18 +425,20 @@ cg_init_kernel(const double* b, const double* scale, const float* minv, float* x
         p[j] = zv;
         rz += static_cast<double>(rv) * zv;
     }
-    cg_flush(rz, c, C, cg + kCgRzNew);
+    cg_block_store(rz, c, C, part1);
 }
 
 // rz = r.z; columns with a zero right-hand side or r.z <= 0 start inactive
-__global__ void cg_start_kernel(double* cg, const double* scale, int C) {
+__global__ void __launch_bounds__(kBlock)
+cg_start_kernel(double* cg, const double* part1, int nblk, const double* scale, int C) {
+    __shared__ double rzs[kMaxCgCols];
     __shared__ int cnt;
+    cg_reduce_parts(part1, nblk, C, rzs);
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
     const int c = threadIdx.x;
     if (c < C) {
-        const double rz = cg[kCgRzNew + c];
-        cg[kCgRzNew + c] = 0.0;
+        const double rz = rzs[c];
         cg[kCgRz + c] = rz;
         const bool act = scale[c] > 0.0 && rz > 0.0;
         cg[kCgActive + c] = act ? 1.0 : 0.0;
@@ -571,26 +597,29 @@ hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, i
 template hipError_t launch_vdot<float>(const float*, const float*, int, int, double*, int, hipStream_t);
 template hipError_t launch_vdot<double>(const double*, const double*, int, int, double*, int, hipStream_t);
 
-hipError_t launch_cg_dot(const float* a, const float* b, long long total, int C, double* cg, int grid,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(cg_dot_kernel, dim3(grid), dim3(kBlock), 0, s, a, b, total, C, cg);
-    hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kMaxCgCols), 0, s, cg, C);
+hipError_t launch_cg_dot(const float* a, const float* b, long long total, int C, double* cg, double* part,
+                         int grid, hipStream_t s) {
+    hipLaunchKernelGGL(cg_dot_kernel, dim3(grid), dim3(kBlock), 0, s, a, b, total, C, part);
+    hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, grid, C);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_step(float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                          long long total, int C, double* cg, double tol2, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(cg_step_kernel, dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, total, C, cg);
-    hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kMaxCgCols), 0, s, cg, C, tol2);
+                          long long total, int C, double* cg, double* part, double tol2, int grid, hipStream_t s) {
+    double* part2 = part + static_cast<size_t>(grid) * C;
+    hipLaunchKernelGGL(cg_step_kernel, dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, total, C, cg, part,
+                       part2);
+    hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, part2, grid, C, tol2);
     hipLaunchKernelGGL(cg_dir_kernel, dim3(grid), dim3(kBlock), 0, s, p, z, cg, total, C);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x, float* r,
-                          float* z, float* p, long long total, int C, double* cg, int grid, hipStream_t s) {
+                          float* z, float* p, long long total, int C, double* cg, double* part, int grid,
+                          hipStream_t s) {
     hipLaunchKernelGGL(cg_init_kernel, dim3(grid), dim3(kBlock), 0, s, b, scale, minv, x, r, z, p, total, C,
-                       cg);
-    hipLaunchKernelGGL(cg_start_kernel, dim3(1), dim3(kMaxCgCols), 0, s, cg, scale, C);
+                       part);
+    hipLaunchKernelGGL(cg_start_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, grid, scale, C);
     return hipGetLastError();
 }
 

@@ -39,19 +39,69 @@ __device__ __forceinline__ T seg_run_sum(T v, int end) {
     return v;
 }
 
-// P[k][c] = sum_{y < k} x2[y][c0 + c] for k = 0..G2, built in LDS by one
-// thread per column (G2 <= kSegMaxRange).
+// Segmented sum over a wave of sorted keys, without atomics.  A run that lies
+// strictly inside the wave (head lane > 0, end lane < 63, or ending at the
+// last entry of the list) holds every entry of its key: its sum is stored in
+// out.  The run at lane 0 and the run reaching lane 63 may continue in the
+// neighbouring waves: they go to the boundary list at entries 2w and 2w + 1
+// (a wave that is one run, or ends inside the list's tail, stores a zero
+// second entry under its first key), which stays sorted and is reduced the
+// same way (seg_bnd_kernel).  With `final` (the list is one wave) every run is
+// complete and stored in out.
+template <typename T, int CC>
+__device__ __forceinline__ void seg_wave_reduce(int key, bool valid, const T (&v)[CC], long long wv,
+                                                bool lane63_valid, bool final, T* out, int C, int c0,
+                                                int* bkey, T* bval) {
+    const int lane = lane_id();
+    const int prev = __shfl_up(key, 1, kWave);
+    const bool is_head = lane == 0 || prev != key;
+    const uint64_t heads = __ballot(is_head);
+    const uint64_t above = lane == 63 ? 0ull : (heads & ~((2ull << lane) - 1ull));
+    const int end = above ? (__builtin_ctzll(above) - 1) : 63;
+    const int h63 = 63 - __builtin_clzll(heads);
+    const bool head = is_head && valid;
+    const bool inner = final || (lane > 0 && end < kWave - 1);
+#pragma unroll
+    for (int c = 0; c < CC; ++c) {
+        const T s = seg_run_sum<T>(v[c], end);
+        if (head) {
+            if (inner) out[static_cast<size_t>(key) * C + c0 + c] = s;
+            else if (lane == 0) bval[(2 * wv) * CC + c] = s;
+            else bval[(2 * wv + 1) * CC + c] = s;
+        }
+        if (!final && lane == 0 && valid && (h63 == 0 || !lane63_valid)) bval[(2 * wv + 1) * CC + c] = T(0);
+    }
+    if (!final && head && !inner) bkey[2 * wv + (lane == 0 ? 0 : 1)] = key;
+    if (!final && lane == 0 && valid && (h63 == 0 || !lane63_valid)) bkey[2 * wv + 1] = key;
+}
+
+// bytes of an LDS table of e values of T, rounded up to 16
+template <typename T>
+__host__ __device__ constexpr size_t seg_tab_bytes(int e) {
+    return (static_cast<size_t>(e) * sizeof(T) + 15) / 16 * 16;
+}
+
+// P[k][c] = sum_{y < k} x2[y][c0 + c] for k = 0..G2, built in LDS: the block
+// stages x2 with coalesced loads, then one thread per column scans in LDS
+// (G2 <= kSegMaxRange).  Ends with a barrier.
 template <typename T, int CC>
 __device__ __forceinline__ void build_prefix(const T* x2, int G2, int C, int c0, T* P) {
+    for (int k = threadIdx.x; k < G2 * CC; k += kBlock) {
+        const int y = k / CC;
+        const int c = k - y * CC;
+        P[k + CC] = x2[static_cast<size_t>(y) * C + c0 + c];
+    }
+    __syncthreads();
     if (threadIdx.x < CC) {
         const int c = threadIdx.x;
         T acc = T(0);
         P[c] = acc;
-        for (int y = 0; y < G2; ++y) {
-            acc += x2[static_cast<size_t>(y) * C + c0 + c];
-            P[(y + 1) * CC + c] = acc;
+        for (int y = 1; y <= G2; ++y) {
+            acc += P[y * CC + c];
+            P[y * CC + c] = acc;
         }
     }
+    __syncthreads();
 }
 
 }  // namespace
@@ -132,10 +182,7 @@ template <typename T, int CC, bool HAS_Y>
 __global__ void __launch_bounds__(kBlock) seg_worker_kernel(SegArgs<T> a) {
     extern __shared__ __align__(16) unsigned char seg_dyn[];
     T* P = reinterpret_cast<T*>(seg_dyn);
-    if constexpr (HAS_Y) {
-        build_prefix<T, CC>(a.x2, a.G2, a.C, a.c0, P);
-        __syncthreads();
-    }
+    if constexpr (HAS_Y) build_prefix<T, CC>(a.x2, a.G2, a.C, a.c0, P);
     for (int w = blockIdx.x * kBlock + threadIdx.x; w < a.Nw; w += gridDim.x * kBlock) {
         const int s0 = a.wseg[w], s1 = a.wseg[w + 1];
         T acc[CC];
@@ -162,18 +209,22 @@ __global__ void __launch_bounds__(kBlock) seg_worker_kernel(SegArgs<T> a) {
 }
 
 // Pass 2 (one thread per segment, segments in firm order): (S x)_1 by a
-// segmented sum over equal firms, and the year differences into a per-block
-// LDS histogram written to hpart[block].  DIAG: the Jacobi diagonal instead
-// (T = double, CC = 1): L - L^2 / n_w per segment, and -1 / n_w over its years.
+// segmented sum over equal firms, and the year differences into an FP64 LDS
+// histogram with one copy per wave (FP32 LDS atomics on the few year slots
+// stall the LDS pipe), summed into hpart[block].  DIAG: the Jacobi diagonal
+// instead (T = double, CC = 1): L - L^2 / n_w per segment, and -1 / n_w over
+// its years.
 template <typename T, int CC, bool HAS_Y, bool DIAG>
 __global__ void __launch_bounds__(kBlock) seg_firm_kernel(SegArgs<T> a) {
     extern __shared__ __align__(16) unsigned char seg_dyn[];
     T* P = reinterpret_cast<T*>(seg_dyn);          // (G2 + 1) x CC prefix sums
-    T* H = P + (a.G2 + 1) * CC;                    // (G2 + 1) x CC differences
+    const int E = (a.G2 + 1) * CC;
+    double* H = reinterpret_cast<double*>(seg_dyn + seg_tab_bytes<T>(E));  // kBlock / kWave copies of E
     const int lane = lane_id();
+    double* Hw = H + (threadIdx.x / kWave) * E;
     if constexpr (HAS_Y) {
         if constexpr (!DIAG) build_prefix<T, CC>(a.x2, a.G2, a.C, a.c0, P);
-        for (int k = threadIdx.x; k < (a.G2 + 1) * CC; k += kBlock) H[k] = T(0);
+        for (int k = threadIdx.x; k < (kBlock / kWave) * E; k += kBlock) H[k] = 0.0;
         __syncthreads();
     }
     for (int base = blockIdx.x * kBlock; base < a.M; base += gridDim.x * kBlock) {
@@ -192,8 +243,8 @@ __global__ void __launch_bounds__(kBlock) seg_firm_kernel(SegArgs<T> a) {
                 v[0] = tl - tl * tl * iw;
                 if constexpr (HAS_Y) {
                     const int y0 = a.f_a[j];
-                    atomicAdd(&H[y0], -iw);
-                    atomicAdd(&H[y0 + L], iw);
+                    atomicAdd(&Hw[y0], -static_cast<double>(iw));
+                    atomicAdd(&Hw[y0 + L], static_cast<double>(iw));
                 }
             } else {
                 const T* xf = a.x1 + static_cast<size_t>(key) * a.C + a.c0;
@@ -209,39 +260,57 @@ __global__ void __launch_bounds__(kBlock) seg_firm_kernel(SegArgs<T> a) {
 #pragma unroll
                     for (int c = 0; c < CC; ++c) {
                         v[c] += P[(y0 + L) * CC + c] - P[y0 * CC + c];
-                        atomicAdd(&H[y0 * CC + c], d[c]);
-                        atomicAdd(&H[(y0 + L) * CC + c], -d[c]);
+                        atomicAdd(&Hw[y0 * CC + c], static_cast<double>(d[c]));
+                        atomicAdd(&Hw[(y0 + L) * CC + c], -static_cast<double>(d[c]));
                     }
                 }
             }
         }
-        const int end = run_end_lane(key);
-        const bool head = run_head(key) && valid;
-        const bool owned = lane > 0 && end < kWave - 1;  // the firm lies inside this wave
-#pragma unroll
-        for (int c = 0; c < CC; ++c) {
-            const T s = seg_run_sum<T>(v[c], end);
-            if (head) {
-                T* t = a.out1 + static_cast<size_t>(key) * a.C + a.c0 + c;
-                if (owned) *t = s;
-                else atomicAdd(t, s);
-            }
+        const long long wbase = base + (threadIdx.x & ~(kWave - 1));
+        if (wbase < a.M) {
+            seg_wave_reduce<T, CC>(key, valid, v, wbase / kWave, wbase + kWave - 1 < a.M, false, a.out1, a.C,
+                                   a.c0, a.bkey, a.bval);
         }
     }
     if constexpr (HAS_Y) {
         __syncthreads();
-        T* hp = a.hpart + static_cast<size_t>(blockIdx.x) * (a.G2 + 1) * CC;
-        for (int k = threadIdx.x; k < (a.G2 + 1) * CC; k += kBlock) hp[k] = H[k];
+        double* hp = a.hpart + static_cast<size_t>(blockIdx.x) * E;
+        for (int k = threadIdx.x; k < E; k += kBlock) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < kBlock / kWave; ++w) t += H[w * E + k];
+            hp[k] = t;
+        }
+    }
+}
+
+// One level of the boundary reduction: the sorted list (kin, vin) of m
+// entries -> complete runs into out, boundary runs into (kout, vout).
+template <typename T, int CC>
+__global__ void __launch_bounds__(kBlock)
+seg_bnd_kernel(const int* kin, const T* vin, int m, T* out, int C, int c0, int* kout, T* vout, int final) {
+    const int lane = lane_id();
+    for (int base = blockIdx.x * kBlock; base < m; base += gridDim.x * kBlock) {
+        const int j = base + threadIdx.x;
+        const bool valid = j < m;
+        const int key = valid ? kin[j] : -1 - lane;
+        T v[CC];
+#pragma unroll
+        for (int c = 0; c < CC; ++c) v[c] = valid ? vin[static_cast<size_t>(j) * CC + c] : T(0);
+        const long long wbase = base + (threadIdx.x & ~(kWave - 1));
+        if (wbase < m) {
+            seg_wave_reduce<T, CC>(key, valid, v, wbase / kWave, wbase + kWave - 1 < m, final != 0, out, C, c0,
+                                   kout, vout);
+        }
     }
 }
 
 // tot[e] = sum over blocks of hpart[b][e], one block per entry e.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) seg_hsum_kernel(const T* hpart, int nblk, int E, double* tot) {
+__global__ void __launch_bounds__(kBlock) seg_hsum_kernel(const double* hpart, int nblk, int E, double* tot) {
     __shared__ double red[kBlock / kWave];
     const int e = blockIdx.x;
     double acc = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += kBlock) acc += static_cast<double>(hpart[static_cast<size_t>(b) * E + e]);
+    for (int b = threadIdx.x; b < nblk; b += kBlock) acc += hpart[static_cast<size_t>(b) * E + e];
     acc = wave_sum(acc);
     if (lane_id() == 0) red[threadIdx.x / kWave] = acc;
     __syncthreads();
@@ -317,10 +386,11 @@ template hipError_t launch_seg_gather<int>(const int*, const int*, int*, int, in
 template <typename T, int CC>
 static hipError_t seg_apply_cc(const SegArgs<T>& a, bool has_y, bool diag, int grid_w, int grid_f,
                                hipStream_t s) {
-    const size_t tab = has_y ? static_cast<size_t>(a.G2 + 1) * CC * sizeof(T) : 0;
+    const size_t tab = has_y ? seg_tab_bytes<T>((a.G2 + 1) * CC) : 0;
+    const size_t ftab = has_y ? tab + static_cast<size_t>(kBlock / kWave) * (a.G2 + 1) * CC * sizeof(double) : 0;
     if (diag) {
         if constexpr (CC == 1) {
-            if (has_y) hipLaunchKernelGGL((seg_firm_kernel<T, 1, true, true>), dim3(grid_f), dim3(kBlock), 2 * tab, s, a);
+            if (has_y) hipLaunchKernelGGL((seg_firm_kernel<T, 1, true, true>), dim3(grid_f), dim3(kBlock), ftab, s, a);
             else hipLaunchKernelGGL((seg_firm_kernel<T, 1, false, true>), dim3(grid_f), dim3(kBlock), 0, s, a);
             return hipGetLastError();
         }
@@ -328,7 +398,7 @@ static hipError_t seg_apply_cc(const SegArgs<T>& a, bool has_y, bool diag, int g
     }
     if (has_y) {
         hipLaunchKernelGGL((seg_worker_kernel<T, CC, true>), dim3(grid_w), dim3(kBlock), tab, s, a);
-        hipLaunchKernelGGL((seg_firm_kernel<T, CC, true, false>), dim3(grid_f), dim3(kBlock), 2 * tab, s, a);
+        hipLaunchKernelGGL((seg_firm_kernel<T, CC, true, false>), dim3(grid_f), dim3(kBlock), ftab, s, a);
     } else {
         hipLaunchKernelGGL((seg_worker_kernel<T, CC, false>), dim3(grid_w), dim3(kBlock), 0, s, a);
         hipLaunchKernelGGL((seg_firm_kernel<T, CC, false, false>), dim3(grid_f), dim3(kBlock), 0, s, a);
@@ -354,16 +424,44 @@ hipError_t launch_seg_apply(const SegArgs<T>& a, int cc, bool has_y, bool diag, 
 template hipError_t launch_seg_apply<float>(const SegArgs<float>&, int, bool, bool, int, int, hipStream_t);
 template hipError_t launch_seg_apply<double>(const SegArgs<double>&, int, bool, bool, int, int, hipStream_t);
 
+template <typename T, int CC>
+static void seg_bnd_cc(const int* kin, const T* vin, int m, T* out, int C, int c0, int* kout, T* vout, int final,
+                       int grid, hipStream_t s) {
+    hipLaunchKernelGGL((seg_bnd_kernel<T, CC>), dim3(grid), dim3(kBlock), 0, s, kin, vin, m, out, C, c0, kout,
+                       vout, final);
+}
+
 template <typename T>
-hipError_t launch_seg_year(const T* hpart, int nblk, int G2, int cc, double* tot, const double* counts,
+hipError_t launch_seg_bnd(const int* kin, const T* vin, int m, T* out, int C, int c0, int cc, int* kout, T* vout,
+                          int final, int grid, hipStream_t s) {
+    switch (cc) {
+        case 1: seg_bnd_cc<T, 1>(kin, vin, m, out, C, c0, kout, vout, final, grid, s); break;
+        case 2: seg_bnd_cc<T, 2>(kin, vin, m, out, C, c0, kout, vout, final, grid, s); break;
+        case 3: seg_bnd_cc<T, 3>(kin, vin, m, out, C, c0, kout, vout, final, grid, s); break;
+        case 4: seg_bnd_cc<T, 4>(kin, vin, m, out, C, c0, kout, vout, final, grid, s); break;
+        case 5: seg_bnd_cc<T, 5>(kin, vin, m, out, C, c0, kout, vout, final, grid, s); break;
+        case 6: seg_bnd_cc<T, 6>(kin, vin, m, out, C, c0, kout, vout, final, grid, s); break;
+        case 7: seg_bnd_cc<T, 7>(kin, vin, m, out, C, c0, kout, vout, final, grid, s); break;
+        case 8: seg_bnd_cc<T, 8>(kin, vin, m, out, C, c0, kout, vout, final, grid, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_seg_bnd<float>(const int*, const float*, int, float*, int, int, int, int*, float*, int,
+                                          int, hipStream_t);
+template hipError_t launch_seg_bnd<double>(const int*, const double*, int, double*, int, int, int, int*, double*,
+                                           int, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_seg_year(const double* hpart, int nblk, int G2, int cc, double* tot, const double* counts,
                            const T* x2, int C, int c0, T* out2, hipStream_t s) {
     const int E = (G2 + 1) * cc;
-    hipLaunchKernelGGL((seg_hsum_kernel<T>), dim3(E), dim3(kBlock), 0, s, hpart, nblk, E, tot);
+    hipLaunchKernelGGL(seg_hsum_kernel, dim3(E), dim3(kBlock), 0, s, hpart, nblk, E, tot);
     hipLaunchKernelGGL((seg_year_kernel<T>), dim3(1), dim3(kBlock), 0, s, tot, G2, cc, counts, x2, C, c0,
                        out2);
     return hipGetLastError();
 }
-template hipError_t launch_seg_year<float>(const float*, int, int, int, double*, const double*, const float*,
+template hipError_t launch_seg_year<float>(const double*, int, int, int, double*, const double*, const float*,
                                            int, int, float*, hipStream_t);
 template hipError_t launch_seg_year<double>(const double*, int, int, int, double*, const double*,
                                             const double*, int, int, double*, hipStream_t);
