@@ -5,9 +5,13 @@ capped at 36 years, 800k firms with Pareto(1.1) sizes, 36 year FE, mover
 probability 0.05, k = 4 regressors: tenure, tenure^2/100, two N(0,1)),
 synthetic and seeded.  Singletons are dropped on the device once (reported,
 not timed); one step is the full within transform of [y | X] (C = 5 columns,
-3-way FE, FP64, reghdfe-comparable tolerance 1e-8) starting from the raw
-columns already resident in HBM: the level sort, the Irons-Tuck accelerated
-Gauss-Seidel solve, the polish and the scatter back to the caller's layout.
+3-way FE, tolerance 1e-8) starting from the raw columns already resident in
+HBM, through the level sort, the solve and the scatter back to the caller's
+layout.  The timed method is the mixed-precision absorber (precision 1: FP32
+Schur-complement PCG under FP64 refinement, final transform in FP64); the
+plain FP64 absorber (precision 0: Irons-Tuck accelerated Gauss-Seidel with the
+reference's gates and stopping rule) is timed in the same run, and the two
+results are compared.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -27,15 +31,22 @@ for _p in (ROOT, os.path.join(ROOT, "xhdfe-xfe_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tile_pass_akm56m.md")
+METRIC = "s to absorb 56M-obs AKM at 1e-8 tol: learned+FP32/FP64 vs plain FP64; HBM GB/s"
+# dominant kernel of each method and the committed rocprofv3 summary holding its HBM traffic
+KERNELS = {
+    1: ("seg_firm_kernel (firm-major segment pass of the FP32 Schur operator)",
+        os.path.join(ROOT, "profiles", "seg_firm_akm56m.md")),
+    0: ("tile_pass_kernel (fused apply / in-tile projection / reduce pass)",
+        os.path.join(ROOT, "profiles", "tile_pass_akm56m.md")),
+}
 
 
-def measured_traffic():
+def measured_traffic(path):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy of known size),
     or None when it is absent."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             if line.startswith("traffic_bytes_per_launch"):
                 return float(line.split(":")[1])
     except OSError:
@@ -92,7 +103,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="panel scale (1.0 = 56M obs)")
-    ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 under FP64 refinement")
+    ap.add_argument("--precision", type=int, default=1, help="1 FP32 under FP64 refinement, 0 FP64")
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain FP64 absorber")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -139,34 +151,54 @@ def main():
     del X, y
     torch.cuda.synchronize()
 
-    def step():
+    def step(precision):
         V.copy_(V0)
         return xfe_amd.absorb_device(V.data_ptr(), n, C, ld, ids.data_ptr(), G, tol=1e-8,
-                                     tolerance_mode=1, precision=args.precision)
+                                     tolerance_mode=1, precision=precision)
 
-    for w in range(args.warmup):
-        t0 = time.perf_counter()
-        info = step()
+    def timed(precision, steps, warmup, profile):
+        for w in range(warmup):
+            t0 = time.perf_counter()
+            info = step(precision)
+            torch.cuda.synchronize()
+            log(f"precision {precision} warmup {w}: {time.perf_counter() - t0:.3f} s, "
+                f"iterations {info['iterations']}, converged {info['converged']}")
+        if profile:
+            xfe_amd.profile_reset()
+            xfe_amd.profile_enable(True)
         torch.cuda.synchronize()
-        log(f"warmup {w}: {time.perf_counter() - t0:.3f} s, iterations {info['iterations']}, "
-            f"sweeps {info['sweeps']}, converged {info['converged']}")
+        t0 = time.perf_counter()
+        infos = []
+        for k in range(steps):
+            infos.append(step(precision))
+            log(f"precision {precision} step {k}: iterations {infos[-1]['iterations']} "
+                f"refinements {infos[-1]['refinements']} converged {infos[-1]['converged']}")
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        if profile:
+            xfe_amd.profile_enable(False)
+        return elapsed / steps, infos[-1]
 
-    xfe_amd.profile_reset()
-    xfe_amd.profile_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    infos = []
-    for k in range(args.steps):
-        infos.append(step())
-        log(f"step {k}: iterations {infos[-1]['iterations']} sweeps {infos[-1]['sweeps']}")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    xfe_amd.profile_enable(False)
+    # the plain FP64 absorber first (its result is the reference for the comparison)
+    plain_s = None
+    V_plain = None
+    if args.plain_steps > 0 and args.precision != 0:
+        plain_s, plain_info = timed(0, args.plain_steps, 1, False)
+        V_plain = V.clone()
+        log(f"plain FP64: {plain_s:.3f} s/step, {plain_info['iterations']} iterations")
+
+    sec, info = timed(args.precision, args.steps, args.warmup, True)
     pass_ms, launches, pass_bytes = xfe_amd.profile_get()
-    sec = elapsed / args.steps
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
-    log(f"{sec:.3f} s/step; fused pass: {launches} launches, {pass_ms / max(launches, 1):.3f} ms avg, "
-        f"{achieved:.0f} GB/s algorithmic")
+    kernel, traffic_file = KERNELS[args.precision]
+    log(f"{sec:.3f} s/step; {kernel.split()[0]}: {launches} launches, "
+        f"{pass_ms / max(launches, 1):.3f} ms avg, {achieved:.0f} GB/s algorithmic")
+    diff = None
+    if V_plain is not None:
+        scale = float(V0[:, :n].abs().max())
+        diff = float((V[:, :n] - V_plain[:, :n]).abs().max()) / scale
+        log(f"max |mixed - plain FP64| / max |v| = {diff:.3e}")
+        del V_plain
 
     cpu = None
     if not args.no_cpu:
@@ -175,9 +207,8 @@ def main():
         cpu = cpu_baseline(n, C, threads)
         log(f"cpu baseline {cpu['value']} s (extrapolated)")
 
-    info = infos[-1]
     out = {
-        "metric": "s to absorb 56M-obs AKM at 1e-8 tol (plain FP64 GPU absorber)",
+        "metric": METRIC,
         "value": round(sec, 4),
         "unit": "s",
         "n_gpus": 1,
@@ -187,18 +218,23 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == 0 else "f32+f64-refinement",
+        "dtype": "f32+f64" if args.precision == 1 else "f64",
         "data": "synthetic",
+        "method": ("mixed precision: FP32 Jacobi-PCG on the Schur complement of the worker FE under FP64 "
+                   "iterative refinement (no learned preconditioner)") if args.precision == 1
+                  else "plain FP64: Irons-Tuck accelerated Gauss-Seidel",
         "config": {"workload": "akm_56m_3way", "obs": n, "workers": G[0], "firms": G[1],
                    "years": G[2], "columns": C, "tol": 1e-8,
                    "tolerance_mode": "reghdfe-comparable", "singletons_dropped": dropped},
         "obs_cols_per_s": round(n * C / sec, 1),
         "iterations": info["iterations"],
-        "sweeps": info["sweeps"],
+        "refinements": info["refinements"],
+        "plain_fp64_s": None if plain_s is None else round(plain_s, 4),
+        "speedup_vs_plain_fp64": None if plain_s is None else round(plain_s / sec, 2),
+        "max_rel_diff_vs_plain_fp64": diff,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(),
-                     "kernel": "tile_pass_kernel (fused apply / in-tile projection / reduce pass)",
+                     "traffic": measured_traffic(traffic_file), "kernel": kernel,
                      "avg_launch_ms": round(pass_ms / max(launches, 1), 4),
                      "bytes_per_launch": round(pass_bytes / max(launches, 1), 1)},
         "cpu_baseline": cpu,

@@ -85,7 +85,10 @@ int xfe_absorb(const double* y, const double* X, int n, int k,
  * overwritten with the within transform; d_group_ids is num_fes x n
  * (dimension-major, ld n) compact ids; d_weight_sums may be NULL (computed on
  * the device).  No counterpart in the reference (its backend always stages
- * host memory); it serves callers that keep their panel on the device. */
+ * host memory); it serves callers that keep their panel on the device.
+ * Ordering: the call first waits for all work already queued on the device
+ * (any stream), so buffers filled asynchronously by the caller are complete,
+ * and it returns after its own work has finished. */
 int xfe_absorb_device(double* d_V, int n, int C, int ld,
                       const int* d_group_ids, const int* num_groups, int num_fes,
                       const double* d_weights, const int* sweep_order,

@@ -256,6 +256,36 @@ def test_device_entry_point_matches_host_entry_point():
     assert max(_rel(out[:, 0], yg, scale), _rel(out[:, 1:], Xg, scale)) <= 1e-12
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_entry_point_waits_for_queued_work(precision):
+    """The device entry point runs on the library's own stream; a caller that
+    fills d_V asynchronously (torch's stream) and calls at once must still get
+    the transform of the filled data."""
+    _log(f"device entry point ordering, precision {precision}")
+    torch = pytest.importorskip("torch")
+    p = panels.akm_small(scale=0.02)
+    _, fes, gids, G, V = _prep(p)
+    yg, Xg, _ = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=precision)
+    n, C = V.shape
+    ld = ((n + 63) // 64) * 64
+    V0 = torch.zeros((C, ld), dtype=torch.float64, device="cuda")
+    V0[:, :n] = torch.from_numpy(np.ascontiguousarray(V.T)).cuda()
+    ids = torch.from_numpy(np.ascontiguousarray(np.stack(gids))).cuda()
+    dV = torch.full_like(V0, 1.0e6)
+    torch.cuda.synchronize()
+    big = torch.empty(64 << 20, dtype=torch.float64, device="cuda")
+    for _ in range(8):            # a backlog of ~4 GB of copies on torch's stream
+        big.copy_(big.flip(0))
+    dV.copy_(V0)                  # no synchronize before the call
+    info = xfe_amd.absorb_device(dV.data_ptr(), n, C, ld, ids.data_ptr(), G, tol=1e-8, precision=precision)
+    out = dV[:, :n].cpu().numpy().T
+    scale = np.abs(V).max()
+    assert info["converged"]
+    # the two entry points run the same method on the same data (FP32 inner
+    # solves may round differently between runs)
+    assert max(_rel(out[:, 0], yg, scale), _rel(out[:, 1:], Xg, scale)) <= (1e-12 if precision == 0 else 1e-9)
+
+
 # ---------------------------------------------------------------- mixed precision
 # The FP32 Schur-complement CG under FP64 refinement converges to the
 # projection itself; it is compared with the oracle run to tol 1e-12 (error

@@ -906,26 +906,27 @@ struct FastSolver {
             a.hpart = hpart;
             a.bkey = ws.seg_bk[0].p;
             a.bval = bv_buf<T>(0);
+            XFE_CHECK(launch_seg_worker<T>(a, cc, k2 >= 0, seg_grid_w, s));
             if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
-            XFE_CHECK(launch_seg_apply<T>(a, cc, k2 >= 0, false, seg_grid_w, seg_grid_f, s));
-            seg_boundaries<T>(a.out1, C, c0, cc);
-            if (k2 >= 0)
-                XFE_CHECK(launch_seg_year<T>(hpart, seg_grid_f, G2, cc, ws.seg_tot.p, ws.wsum[rdim[k2]].p,
-                                             a.x2, C, c0, out + static_cast<size_t>(off[k2]) * C, s));
+            XFE_CHECK(launch_seg_firm<T>(a, cc, k2 >= 0, false, seg_grid_f, s));
             if (g_profile.enabled) {
+                // the firm pass alone: segment records (key, worker, length, start), one worker
+                // mean per segment, the firm coefficients read and written once
                 XFE_CHECK(hipEventRecord(ws.ev1, s));
                 XFE_CHECK(hipEventSynchronize(ws.ev1));
                 float ms = 0.f;
                 XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+                const double e = sizeof(T);
+                const double G1 = sv.G[rdim[k1]];
                 g_profile.ms += ms;
                 g_profile.launches += 1;
-                const double e = sizeof(T);
-                // worker pass: offsets, 3 ints per segment, coefficient gathers, means;
-                // firm pass: 4 ints per segment, coefficients, mean gathers, output
-                double bytes = 4.0 * (Nw + 1) + 12.0 * M + e * cc * (static_cast<double>(M) + Nw);
-                bytes += 16.0 * M + e * cc * (static_cast<double>(sv.G[rdim[k1]]) + M + sv.G[rdim[k1]]);
-                g_profile.bytes += bytes;
+                g_profile.bytes += static_cast<double>(M) * (12.0 + (k2 >= 0 ? 4.0 : 0.0) + e * cc) +
+                                   2.0 * e * cc * G1;
             }
+            seg_boundaries<T>(a.out1, C, c0, cc);
+            if (k2 >= 0)
+                XFE_CHECK(launch_seg_year<T>(hpart, seg_grid_f, G2, cc, ws.seg_tot.p, ws.wsum[rdim[k2]].p,
+                                             a.x2, C, c0, out + static_cast<size_t>(off[k2]) * C, s));
         }
     }
 
@@ -941,7 +942,7 @@ struct FastSolver {
         a.hpart = ws.d_hpart.p;
         a.bkey = ws.seg_bk[0].p;
         a.bval = bv_buf<double>(0);
-        XFE_CHECK(launch_seg_apply<double>(a, 1, k2 >= 0, true, seg_grid_w, seg_grid_f, s));
+        XFE_CHECK(launch_seg_firm<double>(a, 1, k2 >= 0, true, seg_grid_f, s));
         seg_boundaries<double>(a.out1, 1, 0, 1);
         if (k2 >= 0)
             XFE_CHECK(launch_seg_year<double>(ws.d_hpart.p, seg_grid_f, G2, 1, ws.seg_tot.p,
@@ -1045,9 +1046,9 @@ struct FastSolver {
             }
             const size_t extra = static_cast<size_t>(lo) * sizeof(T);
             (void)lds_extra;
-            if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            if (g_profile.enabled && !seg) XFE_CHECK(hipEventRecord(ws.ev0, s));
             XFE_CHECK(launch_schur<T>(a, cc, NR, V != nullptr, dst != nullptr, extra, sv.tile_grid, s));
-            if (g_profile.enabled) {
+            if (g_profile.enabled && !seg) {
                 XFE_CHECK(hipEventRecord(ws.ev1, s));
                 XFE_CHECK(hipEventSynchronize(ws.ev1));
                 float ms = 0.f;
@@ -1619,6 +1620,10 @@ int xfe_absorb_device(double* d_V, int n, int C, int ld, const int* d_group_ids,
     }
     try {
         (void)hipGetLastError();  // clear sticky errors (src/fe_absorption_cuda.cu:2484-2491)
+        // The library works on its own non-blocking stream: wait for the
+        // caller's pending work on the device (e.g. the copy that filled d_V)
+        // before reading its buffers.
+        XFE_CHECK(hipDeviceSynchronize());
         std::vector<const int*> gids(num_fes);
         std::vector<int> G(num_fes);
         std::vector<const double*> wsum(num_fes, nullptr);

@@ -151,9 +151,11 @@ hipError_t launch_seg_primary(const int* wseg, const int* seg_r1, int Nw, int G1
 hipError_t launch_seg_rank(const int* perm, int m, int* rank, int grid, hipStream_t s);
 template <typename T>
 hipError_t launch_seg_gather(const T* src, const int* perm, T* dst, int m, int grid, hipStream_t s);
+// pass 1 (worker means) and pass 2 (firm sums, year differences; diag: the Jacobi diagonal)
 template <typename T>
-hipError_t launch_seg_apply(const SegArgs<T>& a, int cc, bool has_y, bool diag, int grid_w, int grid_f,
-                            hipStream_t s);
+hipError_t launch_seg_worker(const SegArgs<T>& a, int cc, bool has_y, int grid, hipStream_t s);
+template <typename T>
+hipError_t launch_seg_firm(const SegArgs<T>& a, int cc, bool has_y, bool diag, int grid, hipStream_t s);
 template <typename T>
 hipError_t launch_seg_bnd(const int* kin, const T* vin, int m, T* out, int C, int c0, int cc, int* kout, T* vout,
                           int final, int grid, hipStream_t s);

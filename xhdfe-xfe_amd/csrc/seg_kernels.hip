@@ -384,45 +384,61 @@ template hipError_t launch_seg_gather<double>(const double*, const int*, double*
 template hipError_t launch_seg_gather<int>(const int*, const int*, int*, int, int, hipStream_t);
 
 template <typename T, int CC>
-static hipError_t seg_apply_cc(const SegArgs<T>& a, bool has_y, bool diag, int grid_w, int grid_f,
-                               hipStream_t s) {
+static hipError_t seg_worker_cc(const SegArgs<T>& a, bool has_y, int grid, hipStream_t s) {
+    const size_t tab = has_y ? seg_tab_bytes<T>((a.G2 + 1) * CC) : 0;
+    if (has_y) hipLaunchKernelGGL((seg_worker_kernel<T, CC, true>), dim3(grid), dim3(kBlock), tab, s, a);
+    else hipLaunchKernelGGL((seg_worker_kernel<T, CC, false>), dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, int CC>
+static hipError_t seg_firm_cc(const SegArgs<T>& a, bool has_y, bool diag, int grid, hipStream_t s) {
     const size_t tab = has_y ? seg_tab_bytes<T>((a.G2 + 1) * CC) : 0;
     const size_t ftab = has_y ? tab + static_cast<size_t>(kBlock / kWave) * (a.G2 + 1) * CC * sizeof(double) : 0;
     if (diag) {
         if constexpr (CC == 1) {
-            if (has_y) hipLaunchKernelGGL((seg_firm_kernel<T, 1, true, true>), dim3(grid_f), dim3(kBlock), ftab, s, a);
-            else hipLaunchKernelGGL((seg_firm_kernel<T, 1, false, true>), dim3(grid_f), dim3(kBlock), 0, s, a);
+            if (has_y) hipLaunchKernelGGL((seg_firm_kernel<T, 1, true, true>), dim3(grid), dim3(kBlock), ftab, s, a);
+            else hipLaunchKernelGGL((seg_firm_kernel<T, 1, false, true>), dim3(grid), dim3(kBlock), 0, s, a);
             return hipGetLastError();
         }
         return hipErrorInvalidValue;
     }
-    if (has_y) {
-        hipLaunchKernelGGL((seg_worker_kernel<T, CC, true>), dim3(grid_w), dim3(kBlock), tab, s, a);
-        hipLaunchKernelGGL((seg_firm_kernel<T, CC, true, false>), dim3(grid_f), dim3(kBlock), ftab, s, a);
-    } else {
-        hipLaunchKernelGGL((seg_worker_kernel<T, CC, false>), dim3(grid_w), dim3(kBlock), 0, s, a);
-        hipLaunchKernelGGL((seg_firm_kernel<T, CC, false, false>), dim3(grid_f), dim3(kBlock), 0, s, a);
-    }
+    if (has_y) hipLaunchKernelGGL((seg_firm_kernel<T, CC, true, false>), dim3(grid), dim3(kBlock), ftab, s, a);
+    else hipLaunchKernelGGL((seg_firm_kernel<T, CC, false, false>), dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_seg_apply(const SegArgs<T>& a, int cc, bool has_y, bool diag, int grid_w, int grid_f,
-                            hipStream_t s) {
-    switch (cc) {
-        case 1: return seg_apply_cc<T, 1>(a, has_y, diag, grid_w, grid_f, s);
-        case 2: return seg_apply_cc<T, 2>(a, has_y, diag, grid_w, grid_f, s);
-        case 3: return seg_apply_cc<T, 3>(a, has_y, diag, grid_w, grid_f, s);
-        case 4: return seg_apply_cc<T, 4>(a, has_y, diag, grid_w, grid_f, s);
-        case 5: return seg_apply_cc<T, 5>(a, has_y, diag, grid_w, grid_f, s);
-        case 6: return seg_apply_cc<T, 6>(a, has_y, diag, grid_w, grid_f, s);
-        case 7: return seg_apply_cc<T, 7>(a, has_y, diag, grid_w, grid_f, s);
-        case 8: return seg_apply_cc<T, 8>(a, has_y, diag, grid_w, grid_f, s);
-        default: return hipErrorInvalidValue;
+#define XFE_SEG_SWITCH(CALL)                                \
+    switch (cc) {                                           \
+        case 1: return CALL(1);                             \
+        case 2: return CALL(2);                             \
+        case 3: return CALL(3);                             \
+        case 4: return CALL(4);                             \
+        case 5: return CALL(5);                             \
+        case 6: return CALL(6);                             \
+        case 7: return CALL(7);                             \
+        case 8: return CALL(8);                             \
+        default: return hipErrorInvalidValue;               \
     }
+
+template <typename T>
+hipError_t launch_seg_worker(const SegArgs<T>& a, int cc, bool has_y, int grid, hipStream_t s) {
+#define XFE_W(K) seg_worker_cc<T, K>(a, has_y, grid, s)
+    XFE_SEG_SWITCH(XFE_W)
+#undef XFE_W
 }
-template hipError_t launch_seg_apply<float>(const SegArgs<float>&, int, bool, bool, int, int, hipStream_t);
-template hipError_t launch_seg_apply<double>(const SegArgs<double>&, int, bool, bool, int, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_seg_firm(const SegArgs<T>& a, int cc, bool has_y, bool diag, int grid, hipStream_t s) {
+#define XFE_F(K) seg_firm_cc<T, K>(a, has_y, diag, grid, s)
+    XFE_SEG_SWITCH(XFE_F)
+#undef XFE_F
+}
+#undef XFE_SEG_SWITCH
+template hipError_t launch_seg_worker<float>(const SegArgs<float>&, int, bool, int, hipStream_t);
+template hipError_t launch_seg_worker<double>(const SegArgs<double>&, int, bool, int, hipStream_t);
+template hipError_t launch_seg_firm<float>(const SegArgs<float>&, int, bool, bool, int, hipStream_t);
+template hipError_t launch_seg_firm<double>(const SegArgs<double>&, int, bool, bool, int, hipStream_t);
 
 template <typename T, int CC>
 static void seg_bnd_cc(const int* kin, const T* vin, int m, T* out, int C, int c0, int* kout, T* vout, int final,
