@@ -47,6 +47,8 @@ typedef struct {
     int method;                 /* AbsorptionMethod: 0 Auto, 1 GaussSeidel, 2 SymmetricGaussSeidel */
     int force_cg;               /* explicit reghdfe convergence criterion -> CG accelerator */
     int precision;              /* 0: FP64 throughout (the plain method); 1: FP32 sweeps under FP64 refinement */
+    int certify;                /* 1: compute the normal-equation certificate of the result on the device
+                                   (certify_absorption_result, src/fe_absorption.cpp:14942-15190) */
 } xfe_options;
 
 /* Outputs of one absorption (the fields of hdfe::detail::AbsorptionResult,
@@ -63,7 +65,32 @@ typedef struct {
     int sweeps;                 /* projection sweeps executed (diagnostic) */
     int refinements;            /* FP64 refinement rounds (precision 1) */
     double solve_ms;            /* device wall time of the solve */
+    /* certificate (options.certify): AbsorptionResult::abs_residual,
+     * abs_residual_rel and precision_certified (include/fe_absorption.hpp:38-40) */
+    double abs_residual;
+    double abs_residual_rel;
+    int precision_certified;
+    int certificate_computed;
 } xfe_absorb_result;
+
+/* The certificate on its own (certify_cuda_absorption_result,
+ * src/fe_absorption_cuda_certificate.cpp:318; certify_absorption_result,
+ * src/fe_absorption.cpp:14942-15190, standard FEs): for every column c,
+ * abs_c = ||D'W v~_c||_2 over all FE dimensions, ||D'W||_F^2 = sum_d sum_i w_i^2,
+ * rel_c = abs_c / (||D'W||_F ||v_c||_2) with v the untransformed column;
+ * certified = converged && max_c rel_c <= max(8 tol_eff, 64 eps), tol_eff =
+ * min(tol, 1e-9) in reghdfe-comparable mode.  Host buffers: y, X (original),
+ * y_t, X_t (transformed), all n rows, X column-major (ld n); weights may be
+ * NULL.  Returns 1 on success, 0 on failure. */
+typedef struct {
+    double abs_residual;        /* max_c abs_c */
+    double abs_residual_rel;    /* max_c rel_c */
+    int precision_certified;
+} xfe_certificate;
+
+int xfe_certify(const double* y, const double* X, const double* y_t, const double* X_t, int n, int k,
+                const xfe_fe_input* fes, int num_fes, const double* weights,
+                const xfe_options* opt, int converged, xfe_certificate* out);
 
 /* cuda_backend_available (src/fe_absorption_cuda.hpp:125,
  * impl src/fe_absorption_cuda.cu:2281).  1 if a gfx950 device is usable. */

@@ -11,6 +11,9 @@ under ``xhdfe-xfe_amd/``) never imports it and never falls back to it.
   is; used to pin the restatement).
 * ``ols_cluster``: numpy restatement of the OLS + one-way cluster VCV
   (src/ols.cpp:1237-1556, scale G/(G-1) (N-1)/df_resid at :682-687).
+* ``certify``: numpy restatement of the normal-equation certificate
+  (src/fe_absorption.cpp:14942-15190); pinned against the reference's own
+  certificate (``ref_certify``) and a dense ||D'W v|| on tiny cases.
 """
 from __future__ import annotations
 
@@ -216,6 +219,67 @@ def ols_cluster(y_t, X_t, cluster, df_a, omit=None):
 
 
 # ------------------------------------------------------------ reference checker
+
+def effective_tolerance(tol, tolerance_mode):
+    """effective_absorption_tolerance (src/fe_absorption.cpp:339-345)."""
+    if tol > 0.0 and tolerance_mode == 1:
+        return min(tol, 1.0e-9)
+    return tol
+
+
+def certify(V, Vt, gids, G, weights=None, tol=1e-8, tolerance_mode=1, converged=True):
+    """certify_absorption_result (src/fe_absorption.cpp:14942-15190) for
+    standard FEs: per column c, abs_c = ||D'W vt_c|| over all dimensions
+    (level sums of w * vt, squared and summed), ||D'W||_F^2 = sum_d sum_i w_i^2,
+    rel_c = abs_c / (||D'W||_F ||v_c||) with v the untransformed column
+    (unweighted norm); certified iff converged and max_c rel_c <=
+    max(8 tol_eff, 64 eps) (:15181-15189).  Accumulates squares in extended
+    precision like the reference (long double)."""
+    V = np.asarray(V, dtype=np.float64)
+    Vt = np.asarray(Vt, dtype=np.float64)
+    if V.ndim == 1:
+        V, Vt = V[:, None], Vt[:, None]
+    n, C = V.shape
+    w = np.ones(n) if weights is None else np.asarray(weights, dtype=np.float64)
+    orig = (V.astype(np.longdouble) ** 2).sum(axis=0)
+    resid = np.zeros(C, dtype=np.longdouble)
+    op = np.longdouble(0.0)
+    for g, Gd in zip(gids, G):
+        g = np.asarray(g)
+        op += (w.astype(np.longdouble) ** 2).sum()
+        for c in range(C):
+            sums = np.bincount(g, weights=w * Vt[:, c], minlength=Gd)
+            resid[c] += (sums.astype(np.longdouble) ** 2).sum()
+    absolute = np.sqrt(np.maximum(resid, 0.0)).astype(np.float64)
+    scale = float(np.sqrt(max(op, 0.0))) * np.sqrt(np.maximum(orig, 0.0)).astype(np.float64)
+    rel = np.where(scale > 0.0, absolute / np.where(scale > 0.0, scale, 1.0),
+                   np.where(absolute > 0.0, np.finfo(np.float64).max, 0.0))
+    tol_eff = effective_tolerance(tol, tolerance_mode)
+    limit = max(8.0 * max(0.0, tol_eff), 64.0 * np.finfo(np.float64).eps)
+    return dict(abs_residual=float(absolute.max()), abs_residual_rel=float(rel.max()),
+                certified=bool(converged and rel.max() <= limit), per_column_rel=rel)
+
+
+def ref_certify(y, X, yt, Xt, fes, weights=None, tol=1e-8, tolerance_mode=1, converged=True):
+    """The reference's certify_absorption_result through oracle/_ref (checker)."""
+    r = ref()
+    if r is None:
+        return None
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
+    yt = np.ascontiguousarray(yt, dtype=np.float64)
+    Xt = np.asfortranarray(np.asarray(Xt, dtype=np.float64).reshape(y.size, -1))
+    n, k = X.shape
+    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    a, rl, cert = _d(), _d(), _i()
+    rc = r.ref_certify(_i(n), _i(k), _i(len(fes)), _ptr(y), _ptr(X), _ptr(ids), _ptr(w), _d(tol),
+                       _i(tolerance_mode), _i(int(bool(converged))), _ptr(yt), _ptr(Xt),
+                       ctypes.byref(a), ctypes.byref(rl), ctypes.byref(cert))
+    if rc != 0:
+        raise RuntimeError("ref_certify failed")
+    return dict(abs_residual=a.value, abs_residual_rel=rl.value, certified=bool(cert.value))
+
 
 def ref_absorb(y, X, fes, tol=1e-8, tolerance_mode=1, threads=1, symmetric=False):
     r = ref()

@@ -7,6 +7,7 @@
 //   HdfeRegressorV11::partial_out  (src/hdfe_regressor_v11.cpp:10275) -- xfe
 //   HdfeRegressorV11::fit          (src/hdfe_regressor_v11.cpp:7522)  -- xhdfe
 //   detail::absorb_fixed_effects   (src/fe_absorption.cpp:5993)       -- raw CPU absorber
+//   detail::certify_absorption_result (src/fe_absorption.cpp:14942)   -- normal-equation certificate
 #include <Eigen/Dense>
 
 #include <cstring>
@@ -140,6 +141,33 @@ int ref_fit(int n, int k, int d, const double* y, const double* X, const int* fe
         return r.nobs;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "ref_fit: %s\n", e.what());
+        return -1;
+    }
+}
+
+// Certificate of a given within transform (y_t, X_t) of (y, X): weights may be
+// NULL.  Outputs abs_residual, abs_residual_rel and precision_certified.
+int ref_certify(int n, int k, int d, const double* y, const double* X, const int* fes, const double* w,
+                double tol, int tolerance_mode, int converged, const double* y_t, const double* X_t,
+                double* abs_residual, double* abs_residual_rel, int* certified) {
+    try {
+        Eigen::VectorXd yv = Eigen::Map<const Eigen::VectorXd>(y, n);
+        Eigen::MatrixXd Xm = Eigen::Map<const Eigen::MatrixXd>(X, n, k);
+        auto fv = make_fes(fes, n, d);
+        hdfe::HdfeOptions o = make_options(tol, tolerance_mode, 1);
+        hdfe::detail::AbsorptionResult r;
+        r.y_tilde = Eigen::Map<const Eigen::VectorXd>(y_t, n);
+        r.X_tilde = Eigen::Map<const Eigen::MatrixXd>(X_t, n, k);
+        r.converged = converged != 0;
+        Eigen::VectorXd wv;
+        if (w) wv = Eigen::Map<const Eigen::VectorXd>(w, n);
+        hdfe::detail::certify_absorption_result(yv, Xm, fv, w ? &wv : nullptr, o, {}, r);
+        *abs_residual = r.abs_residual;
+        *abs_residual_rel = r.abs_residual_rel;
+        *certified = r.precision_certified ? 1 : 0;
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_certify: %s\n", e.what());
         return -1;
     }
 }

@@ -37,3 +37,8 @@ def input_checksum(p):
     for f in p["fes"]:
         h.update(np.ascontiguousarray(f, dtype=np.int32).tobytes())
     return np.frombuffer(h.digest()[:8], dtype=np.int64)[0]
+
+
+def certificate_weights(n):
+    """Deterministic row weights for the weighted certificate fixtures."""
+    return 0.5 + (np.arange(n) % 7) / 4.0

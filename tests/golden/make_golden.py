@@ -19,7 +19,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "xhdfe-xfe_amd"))
 
 import oracle  # noqa: E402
-from golden_cases import CASES, input_checksum  # noqa: E402
+from golden_cases import CASES, certificate_weights, input_checksum  # noqa: E402
 
 
 def main():
@@ -46,6 +46,18 @@ def main():
             out[f"{name}/{tag}/X_tilde"] = Xo
             out[f"{name}/{tag}/iterations"] = np.int64(it)
             out[f"{name}/{tag}/converged"] = np.int64(conv)
+            # the reference's certificate of its own output (certify_absorption_result)
+            c = oracle.ref_certify(yk, Xk, yo, Xo, fk, tol=tol, converged=conv)
+            out[f"{name}/{tag}/cert"] = np.array([c["abs_residual"], c["abs_residual_rel"],
+                                                  float(c["certified"])])
+            # and with row weights (the certificate's operator norm and sums use w)
+            w = certificate_weights(yk.size)
+            c = oracle.ref_certify(yk, Xk, yo, Xo, fk, weights=w, tol=tol, converged=conv)
+            out[f"{name}/{tag}/cert_w"] = np.array([c["abs_residual"], c["abs_residual_rel"],
+                                                    float(c["certified"])])
+        # certificate of the untransformed columns (large residual, not certified)
+        c = oracle.ref_certify(yk, Xk, yk, Xk, fk, tol=1e-8, converged=True)
+        out[f"{name}/raw/cert"] = np.array([c["abs_residual"], c["abs_residual_rel"], float(c["certified"])])
         f = oracle.ref_fit(y, X, fes, cluster=p["cluster"], tol=1e-12)
         for key in ("coef", "se", "sample_index", "groupvar", "fe_redundant", "fe_levels"):
             out[f"{name}/fit/{key}"] = np.asarray(f[key])

@@ -318,3 +318,39 @@ def test_mixed_precision_spell_path(which):
     _log(f"  inner iterations {info['iterations']} refinements {info['refinements']} err {err:.2e}")
     assert info["converged"]
     assert err <= 1e-9
+
+
+# ---------------------------------------------------------------- certificate
+
+def _rel_close(got, want, rtol):
+    # values at the rounding floor (about 1e-17 relative) carry no digits
+    return abs(got - want) <= rtol * want + 64 * np.finfo(np.float64).eps
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_certify_matches_oracle(cases, name):
+    from golden_cases import certificate_weights
+    _log(f"certify {name}")
+    p = cases[name]
+    _, fes, gids, G, V = _prep(p)
+    Vo, info = oracle.absorb(V, gids, G, tol=1e-8)
+    w = certificate_weights(V.shape[0])
+    for Vt, weights, rtol in ((V, None, 1e-11), (Vo, None, 1e-5), (Vo, w, 1e-11)):
+        g = xfe_amd.certify(V[:, 0], V[:, 1:], Vt[:, 0], Vt[:, 1:], gids, G, weights=weights, tol=1e-8)
+        o = oracle.certify(V, Vt, gids, G, weights=weights, tol=1e-8)
+        assert _rel_close(g["abs_residual"], o["abs_residual"], rtol), (g, o)
+        assert _rel_close(g["abs_residual_rel"], o["abs_residual_rel"], rtol), (g, o)
+        assert g["certified"] == o["certified"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way"])
+def test_absorb_reports_certificate(cases, name, precision):
+    _log(f"absorb certificate {name} precision {precision}")
+    p = cases[name]
+    _, fes, gids, G, V = _prep(p)
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=precision)
+    assert info["certificate_computed"] and info["precision_certified"]
+    o = oracle.certify(V, np.column_stack([yg, Xg]), gids, G, tol=1e-8)
+    assert o["certified"]
+    assert _rel_close(info["abs_residual_rel"], o["abs_residual_rel"], 1e-5), (info, o)

@@ -252,3 +252,59 @@ def test_ols_cluster_matches_reference(golden, cases):
         beta, se, _, _ = oracle.ols_cluster(Vo[:, 0], Vo[:, 1:], p["cluster"][keep], df_a, omit)
         np.testing.assert_allclose(beta, golden[f"{name}/fit/coef"], rtol=1e-9)
         np.testing.assert_allclose(se, golden[f"{name}/fit/se"], rtol=1e-7)
+
+
+# ---------------------------------------------------------------- certificate
+# certify_absorption_result (src/fe_absorption.cpp:14942-15190): the oracle's
+# restatement against the reference's own certificate of its own outputs
+# (fixtures), with and without row weights, and against a dense ||D'W v||.
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+@pytest.mark.parametrize("tag", ["tight", "default", "raw"])
+def test_certificate_matches_reference(golden, cases, name, tag):
+    from golden_cases import certificate_weights
+    p = cases[name]
+    keep = _kept(p)
+    fes = [f[keep] for f in p["fes"]]
+    gids, G = _compact(fes)
+    V = np.column_stack([p["y"][keep], p["X"][keep]])
+    if tag == "raw":
+        Vt, tol, variants = V, 1e-8, (("cert", None),)
+    else:
+        Vt = np.column_stack([golden[f"{name}/{tag}/y_tilde"], golden[f"{name}/{tag}/X_tilde"]])
+        tol = 1e-12 if tag == "tight" else 1e-8
+        variants = (("cert", None), ("cert_w", certificate_weights(V.shape[0])))
+    for key, w in variants:
+        ref_abs, ref_rel, ref_cert = golden[f"{name}/{tag}/{key}"]
+        o = oracle.certify(V, Vt, gids, G, weights=w, tol=tol, converged=True)
+        # converged outputs: the certificate sums cancel to a small residue, so
+        # summation order shows at ~1e-7; large residuals agree to rounding
+        rtol = 1e-5 if (tag != "raw" and key == "cert") else 1e-11
+        assert abs(o["abs_residual"] - ref_abs) <= rtol * ref_abs, (key, o, ref_abs)
+        assert abs(o["abs_residual_rel"] - ref_rel) <= rtol * ref_rel, (key, o, ref_rel)
+        assert o["certified"] == bool(ref_cert)
+
+
+def test_certificate_dense_formula():
+    """||D'W v~_c|| with a dense D (one indicator block per dimension),
+    ||D'W||_F and the untransformed norm, on a tiny weighted case."""
+    rng = np.random.default_rng(11)
+    n, G = 60, [7, 5, 3]
+    gids = [rng.integers(0, g, n) for g in G]
+    V = rng.normal(size=(n, 3))
+    Vt = rng.normal(size=(n, 3)) * 1e-3
+    w = rng.uniform(0.5, 2.0, n)
+    D = np.hstack([np.eye(g)[ids] for g, ids in zip(G, gids)])
+    A = D.T * w                          # D'W
+    resid = np.linalg.norm(A @ Vt, axis=0)
+    rel = resid / (np.linalg.norm(A) * np.linalg.norm(V, axis=0))
+    o = oracle.certify(V, Vt, gids, G, weights=w, tol=1e-8)
+    assert abs(o["abs_residual"] - resid.max()) <= 1e-12 * resid.max()
+    assert abs(o["abs_residual_rel"] - rel.max()) <= 1e-12 * rel.max()
+    np.testing.assert_allclose(o["per_column_rel"], rel, rtol=1e-12)
+    # the certification limit max(8 tol_eff, 64 eps), tol_eff = min(tol, 1e-9) (reghdfe-comparable)
+    small = Vt * (5e-9 / rel.max())
+    assert oracle.certify(V, small, gids, G, weights=w, tol=1e-8)["certified"]          # 5e-9 <= 8e-9
+    assert not oracle.certify(V, small * 2, gids, G, weights=w, tol=1e-8)["certified"]  # 1e-8 > 8e-9
+    assert not oracle.certify(V, small, gids, G, weights=w, tol=1e-8, converged=False)["certified"]
+    assert oracle.certify(V, small * 2, gids, G, weights=w, tol=1e-8, tolerance_mode=0)["certified"]

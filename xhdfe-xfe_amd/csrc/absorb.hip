@@ -18,6 +18,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1480,6 +1481,63 @@ static void setup_tiles(Solver& sv, Workspace& ws, int n, int D, const std::vect
     sv.tiled = true;
 }
 
+// Normal-equation certificate of the transformed block Vt (n x C, ld), the
+// device form of certify_absorption_result (src/fe_absorption.cpp:14942-15190)
+// for standard FEs: per column, the squared level sums of w * vt summed over
+// all dimensions (the fused reduce pass into level sums, then per-column sums
+// of squares of the G x C table); ||D'W||_F^2 = D sum_i w_i^2; orig_sq holds
+// the per-column sums of squares of the untransformed columns.
+static void certificate(Solver& sv, const double* Vt, const std::vector<double>& orig_sq,
+                        const xfe_options& opt, int converged, double& abs_out, double& rel_out,
+                        int& certified) {
+    Workspace& ws = sv.ws;
+    const int C = sv.C;
+    const bool prof = g_profile.enabled;
+    g_profile.enabled = false;  // the profile covers the solver's passes only
+    std::vector<double> resid(C, 0.0), tmp(C, 0.0);
+    for (int d = 0; d < sv.D; ++d) {
+        sv.clear_sums(d);
+        sv.pass(Vt, nullptr, nullptr, nullptr, 0.0, nullptr, -1, d, 0, nullptr, nullptr, nullptr);
+        const long long total = static_cast<long long>(sv.G[d]) * C;
+        XFE_CHECK(launch_finalize(ws.sums[d].p, ws.means[d].p, nullptr, sv.G[d], C, sv.copies_of(d, C),
+                                  nullptr, grid_for(total), sv.s));
+        const int g = cg_grid(total, C, kMaxGrid);
+        ws.partials.ensure(static_cast<size_t>(g) * C);
+        XFE_CHECK(launch_table_sumsq(ws.means[d].p, total, C, ws.partials.p, g, sv.s));
+        sv.fetch(ws.partials.p, g, C, 0, tmp.data());
+        for (int c = 0; c < C; ++c) resid[c] += tmp[c];
+    }
+    g_profile.enabled = prof;
+    double wsq = static_cast<double>(sv.n);
+    if (sv.w) {
+        XFE_CHECK(launch_col_dot(sv.w, sv.w, sv.n, 1, sv.n, ws.partials.p, sv.grid, sv.s));
+        sv.fetch(ws.partials.p, sv.grid, 1, 0, &wsq);
+    }
+    const double op = std::sqrt(std::max(0.0, sv.D * wsq));
+    double amax = 0.0, rmax = 0.0;
+    for (int c = 0; c < C; ++c) {
+        const double a = std::sqrt(std::max(0.0, resid[c]));
+        const double scale = op * std::sqrt(std::max(0.0, orig_sq[c]));
+        const double rel = scale > 0.0 ? a / scale : (a > 0.0 ? DBL_MAX : 0.0);
+        amax = std::max(amax, a);
+        rmax = std::max(rmax, rel);
+    }
+    const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
+    const double limit = std::max(8.0 * std::max(0.0, tol_eff), 64.0 * DBL_EPSILON);
+    abs_out = amax;
+    rel_out = rmax;
+    certified = (converged && rmax <= limit) ? 1 : 0;
+}
+
+// per-column sums of squares of an n x C block (ld)
+static std::vector<double> column_sumsq(Solver& sv, const double* V, int ld) {
+    std::vector<double> out(sv.C, 0.0);
+    sv.ws.partials.ensure(static_cast<size_t>(sv.grid) * sv.C);
+    XFE_CHECK(launch_col_dot(V, V, sv.n, sv.C, ld, sv.ws.partials.p, sv.grid, sv.s));
+    sv.fetch(sv.ws.partials.p, sv.grid, sv.C, 0, out.data());
+    return out;
+}
+
 // Shared driver for both entry points.  d_Vin: caller's n x C (ld_in) block,
 // d_gid_in[d]: caller's ids; the result is written back to d_Vin.
 static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
@@ -1582,7 +1640,14 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
     } else {
         order = probe_order;
     }
+    std::vector<double> orig_sq;
+    if (opt.certify) orig_sq = column_sumsq(sv, V, ld);
     solve_core(sv, V, order, profs, opt, res);
+    if (opt.certify) {
+        certificate(sv, V, orig_sq, opt, res.converged, res.abs_residual, res.abs_residual_rel,
+                    res.precision_certified);
+        res.certificate_computed = 1;
+    }
     XFE_CHECK(launch_scatter_cols(V, ld, perm, d_Vin, ld_in, n, C, grid, s));
     XFE_CHECK(hipStreamSynchronize(s));
     const auto t1 = std::chrono::steady_clock::now();
@@ -1700,6 +1765,66 @@ int xfe_absorb(const double* y, const double* X, int n, int k, const xfe_fe_inpu
             XFE_CHECK(hipMemcpyAsync(X_out, ws.stage.p + n, sizeof(double) * static_cast<size_t>(n) * k,
                                      hipMemcpyDeviceToHost, s));
         XFE_CHECK(hipStreamSynchronize(s));
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+int xfe_certify(const double* y, const double* X, const double* y_t, const double* X_t, int n, int k,
+                const xfe_fe_input* fes, int num_fes, const double* weights, const xfe_options* opt,
+                int converged, xfe_certificate* out) {
+    clear_error();
+    if (!out || !opt || !y || !y_t || (k > 0 && (!X || !X_t)) || n < 0 || k < 0 || k + 1 > XFE_MAX_COLS ||
+        num_fes < 0 || num_fes > XFE_MAX_DIMS || (num_fes > 0 && !fes)) {
+        set_error("xfe_certify: invalid arguments");
+        return 0;
+    }
+    std::memset(out, 0, sizeof(*out));
+    out->precision_certified = converged ? 1 : 0;
+    if (num_fes == 0 || n == 0) return 1;  // nothing to certify (src/fe_absorption.cpp:14971-14973)
+    for (int d = 0; d < num_fes; ++d) {
+        if (fes[d].is_slope || !fes[d].group_ids || fes[d].num_groups <= 0) {
+            set_error("xfe_certify: invalid FE input");
+            return 0;
+        }
+    }
+    try {
+        (void)hipGetLastError();
+        Workspace& ws = tls_ws;
+        ws.init();
+        hipStream_t s = ws.stream;
+        const int C = k + 1;
+        const size_t nc = static_cast<size_t>(n) * C;
+        ws.stage.ensure(nc);
+        ws.vec[1].ensure(nc);
+        XFE_CHECK(hipMemcpyAsync(ws.stage.p, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        XFE_CHECK(hipMemcpyAsync(ws.vec[1].p, y_t, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        if (k > 0) {
+            XFE_CHECK(hipMemcpyAsync(ws.stage.p + n, X, sizeof(double) * n * k, hipMemcpyHostToDevice, s));
+            XFE_CHECK(hipMemcpyAsync(ws.vec[1].p + n, X_t, sizeof(double) * n * k, hipMemcpyHostToDevice, s));
+        }
+        ws.stage_ids.ensure(static_cast<size_t>(n) * num_fes);
+        Solver sv(ws, n, C, n);
+        sv.D = num_fes;
+        for (int d = 0; d < num_fes; ++d) {
+            int* dst = ws.stage_ids.p + static_cast<size_t>(d) * n;
+            XFE_CHECK(hipMemcpyAsync(dst, fes[d].group_ids, sizeof(int) * n, hipMemcpyHostToDevice, s));
+            sv.G[d] = fes[d].num_groups;
+            sv.gid[d] = dst;
+            sv.prepare_dim(d);
+        }
+        if (weights) {
+            ws.stage_w.ensure(n);
+            XFE_CHECK(hipMemcpyAsync(ws.stage_w.p, weights, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            sv.w = ws.stage_w.p;
+        }
+        ws.scalars.ensure(std::max(C, 8));
+        ws.partials.ensure(static_cast<size_t>(sv.grid) * C + 1024);
+        const std::vector<double> orig_sq = column_sumsq(sv, ws.stage.p, n);
+        certificate(sv, ws.vec[1].p, orig_sq, *opt, converged, out->abs_residual, out->abs_residual_rel,
+                    out->precision_certified);
         return 1;
     } catch (const std::exception& e) {
         set_error(e.what());

@@ -212,6 +212,9 @@ hipError_t launch_it_stats(const double* x, const double* gx, const double* ggx,
                            int ld, double* partials, int grid, hipStream_t s);
 hipError_t launch_col_dot(const double* a, const double* b, int n, int C, int ld, double* partials,
                           int grid, hipStream_t s);
+// per-column sums of squares of a level-major table (certificate), grid from cg_grid
+constexpr int kMaxTableCols = 256;
+hipError_t launch_table_sumsq(const double* a, long long total, int C, double* part, int grid, hipStream_t s);
 hipError_t launch_col_axpy(double* y, const double* a, const double* b, const double* alpha, int n,
                            int C, int ld, double sign, int grid, hipStream_t s);
 hipError_t launch_sub(double* y, const double* a, const double* b, int n, int C, int ld, int grid,

@@ -271,6 +271,24 @@ col_dot_kernel(const double* a, const double* b, int n, int C, int ld, double* p
     }
 }
 
+// part[block * C + c] = sum of a[j]^2 over the block's elements of column c of
+// a level-major table (j = level * C + c).  The grid's thread stride must be a
+// multiple of C (every thread keeps one column); C <= kMaxTableCols.
+__global__ void __launch_bounds__(kBlock)
+table_sumsq_kernel(const double* a, long long total, int C, double* part) {
+    __shared__ double red[kMaxTableCols];
+    const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const int c = static_cast<int>(g0 % C);
+    double acc = 0.0;
+    for (long long j = g0; j < total; j += stride) acc += a[j] * a[j];
+    for (int k = threadIdx.x; k < C; k += kBlock) red[k] = 0.0;
+    __syncthreads();
+    atomicAdd(&red[c], acc);
+    __syncthreads();
+    for (int k = threadIdx.x; k < C; k += kBlock) part[static_cast<size_t>(blockIdx.x) * C + k] = red[k];
+}
+
 // y = a - alpha_c * b per column (alpha on device, C entries); in place allowed.
 __global__ void __launch_bounds__(kBlock)
 col_axpy_kernel(double* y, const double* a, const double* b, const double* alpha, int n, int C,
@@ -441,6 +459,12 @@ hipError_t launch_it_stats(const double* x, const double* gx, const double* ggx,
 hipError_t launch_col_dot(const double* a, const double* b, int n, int C, int ld, double* partials,
                           int grid, hipStream_t s) {
     hipLaunchKernelGGL(col_dot_kernel, dim3(grid), dim3(kBlock), 0, s, a, b, n, C, ld, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_table_sumsq(const double* a, long long total, int C, double* part, int grid, hipStream_t s) {
+    if (C <= 0 || C > kMaxTableCols) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(table_sumsq_kernel, dim3(grid), dim3(kBlock), 0, s, a, total, C, part);
     return hipGetLastError();
 }
 

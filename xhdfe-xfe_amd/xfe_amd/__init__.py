@@ -30,20 +30,26 @@ class FeInput(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("tol", _d), ("tolerance_mode", _i), ("max_iter", _i), ("method", _i),
-                ("force_cg", _i), ("precision", _i)]
+                ("force_cg", _i), ("precision", _i), ("certify", _i)]
 
 
 class AbsorbResult(ctypes.Structure):
     _fields_ = [("iterations", _i), ("converged", _i), ("used_accel", _i), ("used_cg", _i),
                 ("polish_sweeps", _i), ("polish_final_max", _d), ("sweep_order_used", _i * MAX_DIMS),
-                ("fe_levels", _i * MAX_DIMS), ("sweeps", _i), ("refinements", _i), ("solve_ms", _d)]
+                ("fe_levels", _i * MAX_DIMS), ("sweeps", _i), ("refinements", _i), ("solve_ms", _d),
+                ("abs_residual", _d), ("abs_residual_rel", _d), ("precision_certified", _i),
+                ("certificate_computed", _i)]
+
+
+class Certificate(ctypes.Structure):
+    _fields_ = [("abs_residual", _d), ("abs_residual_rel", _d), ("precision_certified", _i)]
 
 
 # Every entry point of include/xfe_hip.h (the CPU test checks that each is exported).
 EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop_singletons",
            "xfe_bipartite_components", "xfe_fe_dof", "xfe_gram", "xfe_cluster_meat",
            "xfe_recovery_max_delta", "xfe_last_error", "xfe_profile_enable", "xfe_profile_reset",
-           "xfe_profile_get", "xfe_release_workspace")
+           "xfe_profile_get", "xfe_release_workspace", "xfe_certify")
 
 _lib = None
 
@@ -88,9 +94,9 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(_p)
 
 
-def _opts(tol, tolerance_mode, max_iter, method, force_cg, precision):
+def _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify=True):
     return Options(float(tol), int(tolerance_mode), int(max_iter), int(method), int(bool(force_cg)),
-                   int(precision))
+                   int(precision), int(bool(certify)))
 
 
 def _info(r, D):
@@ -98,11 +104,13 @@ def _info(r, D):
                 used_cg=bool(r.used_cg), polish_sweeps=r.polish_sweeps,
                 polish_final_max=r.polish_final_max, sweep_order=list(r.sweep_order_used)[:D],
                 fe_levels=list(r.fe_levels)[:D], sweeps=r.sweeps, refinements=r.refinements,
-                solve_ms=r.solve_ms)
+                solve_ms=r.solve_ms, abs_residual=r.abs_residual, abs_residual_rel=r.abs_residual_rel,
+                precision_certified=bool(r.precision_certified),
+                certificate_computed=bool(r.certificate_computed))
 
 
 def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=1e-8,
-           tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0):
+           tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0, certify=True):
     """absorb_fixed_effects_cuda on host arrays: returns (y_tilde, X_tilde, info).
 
     gids: compact 0-based level ids per dimension; G: level counts.  ``method``
@@ -124,7 +132,7 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
     order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
     if method is None:
         method = 2 if D == 2 else 1
-    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision)
+    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify)
     r = AbsorbResult()
     yo = np.empty(n)
     Xo = np.empty((n, k), order="F")
@@ -136,14 +144,15 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
 
 
 def absorb_device(V_ptr, n, C, ld, gid_ptr, G, weights_ptr=None, sweep_order=None, tol=1e-8,
-                  tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0):
+                  tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0,
+                  certify=True):
     """xfe_absorb_device on device pointers (e.g. torch tensors' data_ptr())."""
     D = len(G)
     Garr = np.ascontiguousarray(G, dtype=np.int32)
     order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
     if method is None:
         method = 2 if D == 2 else 1
-    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision)
+    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify)
     r = AbsorbResult()
     ok = lib().xfe_absorb_device(_p(V_ptr), _i(n), _i(C), _i(ld), _p(gid_ptr), _ptr(Garr), _i(D),
                                  _p(weights_ptr) if weights_ptr else None, _ptr(order),
@@ -151,6 +160,33 @@ def absorb_device(V_ptr, n, C, ld, gid_ptr, G, weights_ptr=None, sweep_order=Non
     if not ok:
         raise XfeError(last_error())
     return _info(r, D)
+
+
+def certify(y, X, y_t, X_t, gids, G, weights=None, tol=1e-8, tolerance_mode=1, converged=True):
+    """xfe_certify (certify_cuda_absorption_result / certify_absorption_result,
+    src/fe_absorption.cpp:14942-15190) on host arrays: dict(abs_residual,
+    abs_residual_rel, certified)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
+    yt = np.ascontiguousarray(y_t, dtype=np.float64)
+    Xt = np.asfortranarray(np.asarray(X_t, dtype=np.float64).reshape(y.size, -1))
+    n, k = X.shape
+    D = len(gids)
+    ids = [np.ascontiguousarray(g, dtype=np.int32) for g in gids]
+    fes = (FeInput * max(D, 1))()
+    for d in range(D):
+        fes[d].group_ids = ids[d].ctypes.data
+        fes[d].num_groups = int(G[d])
+        fes[d].num_levels_present = int(G[d])
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    o = _opts(tol, tolerance_mode, 0, 0, False, 0, True)
+    out = Certificate()
+    ok = lib().xfe_certify(_ptr(y), _ptr(X), _ptr(yt), _ptr(Xt), _i(n), _i(k), fes, _i(D), _ptr(w),
+                           ctypes.byref(o), _i(int(bool(converged))), ctypes.byref(out))
+    if not ok:
+        raise XfeError(last_error())
+    return dict(abs_residual=out.abs_residual, abs_residual_rel=out.abs_residual_rel,
+                certified=bool(out.precision_certified))
 
 
 def drop_singletons(fes, max_rounds=100):
