@@ -760,6 +760,7 @@ struct FastSolver {
     // dimension r2 (-1 without one)
     bool seg = false;
     int k1 = 0, k2 = -1, M = 0, Nw = 0, G2 = 0, seg_grid_w = 1, seg_grid_f = 1;
+    bool empty_firms = true;  // some level of r1 has no segment
 
     explicit FastSolver(Solver& sv_) : sv(sv_), ws(sv_.ws), s(sv_.s) {}
 
@@ -845,6 +846,16 @@ struct FastSolver {
         XFE_CHECK(launch_seg_gather<int>(ws.p_rk.p, ws.f_order.p, ws.f_w.p, M, mgrid, s));
         XFE_CHECK(launch_seg_gather<int>(ws.p_len.p, ws.f_order.p, ws.f_len.p, M, mgrid, s));
         if (k2 >= 0) XFE_CHECK(launch_seg_gather<int>(ws.p_a.p, ws.f_order.p, ws.f_a.p, M, mgrid, s));
+        // firms without segments (their sums must be cleared before each apply)
+        ws.w_pf.ensure(std::max(Nw, G1 + 1));
+        XFE_CHECK(launch_seg_offsets(ws.f_key.p, M, G1, ws.w_pf.p, mgrid, s));
+        ws.flag.ensure(1);
+        XFE_CHECK(hipMemsetAsync(ws.flag.p, 0, sizeof(int), s));
+        XFE_CHECK(launch_seg_empty(ws.w_pf.p, G1, ws.flag.p, grid_for(G1), s));
+        int nempty = 0;
+        XFE_CHECK(hipMemcpyAsync(&nempty, ws.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        empty_firms = nempty > 0;
         seg_grid_w = grid_for(Nw);
         seg_grid_f = mgrid;
         const size_t tabs = static_cast<size_t>(seg_grid_f) * (G2 + 1) * 8;
@@ -872,26 +883,28 @@ struct FastSolver {
     }
 
     // Finish the firm reduction: reduce the boundary lists level by level
-    // (stride: the row stride of out1).
+    // (stride: the row stride of out1); large levels with the full grid, the
+    // rest in one block.
     template <typename T>
     void seg_boundaries(T* out1, int stride, int c0, int cc) {
         int m = static_cast<int>(2 * ((static_cast<long long>(M) + kWave - 1) / kWave));
         int cur = 0;
-        for (;;) {
-            const int fin = m <= kWave ? 1 : 0;
+        while (m > kBndTail) {
             XFE_CHECK(launch_seg_bnd<T>(ws.seg_bk[cur].p, bv_buf<T>(cur), m, out1, stride, c0, cc,
-                                        ws.seg_bk[1 - cur].p, bv_buf<T>(1 - cur), fin, grid_for(m), s));
-            if (fin) break;
+                                        ws.seg_bk[1 - cur].p, bv_buf<T>(1 - cur), 0, grid_for(m), s));
             m = static_cast<int>(2 * ((static_cast<long long>(m) + kWave - 1) / kWave));
             cur = 1 - cur;
         }
+        XFE_CHECK(launch_seg_bnd_tail<T>(ws.seg_bk[cur].p, bv_buf<T>(cur), ws.seg_bk[1 - cur].p,
+                                         bv_buf<T>(1 - cur), m, out1, stride, c0, cc, s));
     }
 
     // out = S x through the segment records (T = float for the inner CG,
     // double for the refinement residual).
     template <typename T>
     void seg_apply(const T* x, T* out, const T* inv_w, T* wmean, double* hpart) {
-        XFE_CHECK(hipMemsetAsync(out, 0, sizeof(T) * static_cast<size_t>(N) * C, s));
+        // every firm with a segment and every year is written; clear only when some firm has none
+        if (empty_firms) XFE_CHECK(hipMemsetAsync(out, 0, sizeof(T) * static_cast<size_t>(N) * C, s));
         for (int ch = 0; ch < sv.nchunks; ++ch) {
             const int c0 = ch * 8;
             const int cc = std::min(8, C - c0);
@@ -1118,7 +1131,7 @@ struct FastSolver {
     // columns every kPoll iterations (iterations after the last column stops
     // are no-ops: alpha = 0).
     int inner_solve(double tol_in, int max_inner) {
-        constexpr int kPoll = 4;
+        constexpr int kPoll = 8;
         double bb[kMaxCgCols], scale[2 * kMaxCgCols];
         coldot<double>(ws.f_b.p, ws.f_b.p, bb);
         for (int c = 0; c < C; ++c) {
@@ -1415,16 +1428,23 @@ static std::vector<int> select_sweep_order_device(Workspace& ws, const double* d
 // Level-aligned tiles of the sorted dimension p (tile_kernels.hip) and, for
 // every other dimension too large for LDS, its spells: maximal runs of equal
 // level inside a level of p, with their order by level for the pull.
-static void setup_tiles(Solver& sv, Workspace& ws, int n, int D, const std::vector<int>& G) {
+// counts_known: ws.wsum[p] holds the level sizes (unit weights, sums computed here)
+static void setup_tiles(Solver& sv, Workspace& ws, int n, int D, const std::vector<int>& G, bool counts_known) {
     sv.tiled = false;
     const int p = sv.sorted_dim;
     if (p < 0 || n < kTileRows || std::getenv("XFE_NO_TILES")) return;
     hipStream_t s = ws.stream;
     const int grid = grid_for(n);
     ws.level_count.ensure(static_cast<size_t>(G[p]) + 1);
-    XFE_CHECK(hipMemsetAsync(ws.level_count.p, 0, sizeof(int) * (static_cast<size_t>(G[p]) + 1), s));
-    XFE_CHECK(launch_level_count(sv.gid[p], n, ws.level_count.p, grid, s));
-    XFE_CHECK(launch_max_int(ws.level_count.p, G[p], ws.level_count.p + G[p], grid_for(G[p]), s));
+    if (counts_known) {
+        // unit weights: the level sizes are the weight sums already on the device
+        XFE_CHECK(hipMemsetAsync(ws.level_count.p + G[p], 0, sizeof(int), s));
+        XFE_CHECK(launch_max_count(ws.wsum[p].p, G[p], ws.level_count.p + G[p], grid_for(G[p]), s));
+    } else {
+        XFE_CHECK(hipMemsetAsync(ws.level_count.p, 0, sizeof(int) * (static_cast<size_t>(G[p]) + 1), s));
+        XFE_CHECK(launch_level_count(sv.gid[p], n, ws.level_count.p, grid, s));
+        XFE_CHECK(launch_max_int(ws.level_count.p, G[p], ws.level_count.p + G[p], grid_for(G[p]), s));
+    }
     int lmax = 0;
     XFE_CHECK(hipMemcpyAsync(&lmax, ws.level_count.p + G[p], sizeof(int), hipMemcpyDeviceToHost, s));
     XFE_CHECK(hipStreamSynchronize(s));
@@ -1620,7 +1640,7 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
         XFE_CHECK(hipStreamSynchronize(s));
         profs[d] = profile_of(hw, w == nullptr);
     }
-    setup_tiles(sv, ws, n, D, G);
+    setup_tiles(sv, ws, n, D, G, w == nullptr && h_wsum[p] == nullptr);
     // partial buffer: pass partials (3 per block and chunk), per-column CG dots,
     // then the polish maxima of one sweep (<= 2D projections)
     const size_t pg = static_cast<size_t>(std::max(std::max(sv.grid, grid), sv.tile_grid));

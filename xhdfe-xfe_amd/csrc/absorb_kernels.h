@@ -76,6 +76,8 @@ hipError_t launch_spell_pull(const double* S, int C, int c0, int cc, const int* 
                              const int* skey, int M, double* sums, int grid, hipStream_t s);
 hipError_t launch_level_count(const int* gid, int n, int* cnt, int grid, hipStream_t s);
 hipError_t launch_max_int(const int* v, int m, int* out, int grid, hipStream_t s);
+// out = max over m of (int) v (level sizes from unit-weight sums)
+hipError_t launch_max_count(const double* v, int m, int* out, int grid, hipStream_t s);
 hipError_t launch_tile_start(const int* gid, int n, int step, int num_tiles, int* tile_start,
                              int grid, hipStream_t s);
 hipError_t launch_spell_heads(const int* gp, const int* gr, int n, int* flags, int grid,
@@ -149,6 +151,7 @@ hipError_t launch_seg_offsets(const int* key, int M, int Nw, int* off, int grid,
 hipError_t launch_seg_primary(const int* wseg, const int* seg_r1, int Nw, int G1, int* pf, int grid,
                               hipStream_t s);
 hipError_t launch_seg_rank(const int* perm, int m, int* rank, int grid, hipStream_t s);
+hipError_t launch_seg_empty(const int* off, int G, int* flag, int grid, hipStream_t s);
 template <typename T>
 hipError_t launch_seg_gather(const T* src, const int* perm, T* dst, int m, int grid, hipStream_t s);
 // pass 1 (worker means) and pass 2 (firm sums, year differences; diag: the Jacobi diagonal)
@@ -159,6 +162,11 @@ hipError_t launch_seg_firm(const SegArgs<T>& a, int cc, bool has_y, bool diag, i
 template <typename T>
 hipError_t launch_seg_bnd(const int* kin, const T* vin, int m, T* out, int C, int c0, int cc, int* kout, T* vout,
                           int final, int grid, hipStream_t s);
+// the remaining levels (list of m entries) in one block of kBndTail threads
+constexpr int kBndTail = 1024;
+template <typename T>
+hipError_t launch_seg_bnd_tail(int* k0, T* v0, int* k1, T* v1, int m, T* out, int C, int c0, int cc,
+                               hipStream_t s);
 template <typename T>
 hipError_t launch_seg_year(const double* hpart, int nblk, int G2, int cc, double* tot, const double* counts,
                            const T* x2, int C, int c0, T* out2, hipStream_t s);

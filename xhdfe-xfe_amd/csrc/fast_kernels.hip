@@ -286,8 +286,10 @@ vdot_kernel(const T* a, const T* b, int N, int C, double* partials) {
 // grid whose thread stride is a multiple of C, so every thread always meets
 // the same column c = (global thread id) % C and keeps one FP64 accumulator;
 // each block stores its column sums in a partials array (plain stores), and
-// the single-block scalar kernels reduce the partials and update the scalar
-// block `cg` (layout kCg* in absorb_kernels.h).
+// single-block scalar kernels reduce the partials and update the scalar block
+// `cg` (layout kCg* in absorb_kernels.h).  (A last-block reduction inside the
+// vector kernels needs a device-scope release per block, which writes back
+// the XCD's L2 and costs more than the extra launch.)
 
 namespace {
 // part[block * C + c] = sum of v over the block's threads of column c
@@ -301,22 +303,45 @@ __device__ __forceinline__ void cg_block_store(double v, int c, int C, double* p
     __syncthreads();
 }
 
-// out[c] = sum over nblk blocks of part[b * C + c], c < C (one block of kBlock)
-__device__ __forceinline__ void cg_reduce_parts(const double* part, int nblk, int C, double* out) {
-    __shared__ double red[kBlock / kWave];
-    for (int c = 0; c < C; ++c) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += kBlock) acc += part[static_cast<size_t>(b) * C + c];
-        acc = wave_sum(acc);
-        if (lane_id() == 0) red[threadIdx.x / kWave] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
-            out[c] = t;
+// out[c] = sum over nblk blocks of part[b * C + c], c < C (one block of kBlock):
+// T = floor(kBlock / C) * C threads read the array linearly with stride T, so
+// thread t always meets column t % C, with 8 independent accumulators to keep
+// 8 loads in flight; then one thread per column adds the T / C thread sums.
+// Two arrays (part2 may be null) are reduced in the same pass.
+__device__ __forceinline__ void cg_reduce_parts(const double* part, const double* part2, int nblk, int C,
+                                                double* out, double* out2) {
+    __shared__ double red[2][kBlock];
+    const int T = (kBlock / C) * C;
+    const int total = nblk * C;
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double b[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (static_cast<int>(threadIdx.x) < T) {
+        int e = threadIdx.x;
+        for (; e + 7 * T < total; e += 8 * T) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                a[u] += part[e + u * T];
+                if (part2) b[u] += part2[e + u * T];
+            }
         }
-        __syncthreads();
+        for (; e < total; e += T) {
+            a[0] += part[e];
+            if (part2) b[0] += part2[e];
+        }
     }
+    red[0][threadIdx.x] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    red[1][threadIdx.x] = ((b[0] + b[1]) + (b[2] + b[3])) + ((b[4] + b[5]) + (b[6] + b[7]));
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < C) {
+        double t = 0.0, t2 = 0.0;
+        for (int k = threadIdx.x; k < T; k += C) {
+            t += red[0][k];
+            t2 += red[1][k];
+        }
+        out[threadIdx.x] = t;
+        if (out2) out2[threadIdx.x] = t2;
+    }
+    __syncthreads();
 }
 }  // namespace
 
@@ -334,7 +359,7 @@ cg_dot_kernel(const float* a, const float* b, long long total, int C, double* pa
 // alpha = rz / pq for active columns (a column with pq <= 0 or rz <= 0 stops)
 __global__ void __launch_bounds__(kBlock) cg_alpha_kernel(double* cg, const double* part1, int nblk, int C) {
     __shared__ double pqs[kMaxCgCols];
-    cg_reduce_parts(part1, nblk, C, pqs);
+    cg_reduce_parts(part1, nullptr, nblk, C, pqs, nullptr);
     const int c = threadIdx.x;
     if (c >= C) return;
     const double pq = pqs[c];
@@ -374,21 +399,18 @@ __global__ void __launch_bounds__(kBlock)
 cg_beta_kernel(double* cg, const double* part1, const double* part2, int nblk, int C, double tol2) {
     __shared__ double rrs[kMaxCgCols], rzs[kMaxCgCols];
     __shared__ int cnt;
-    cg_reduce_parts(part1, nblk, C, rrs);
-    cg_reduce_parts(part2, nblk, C, rzs);
+    cg_reduce_parts(part1, part2, nblk, C, rrs, rzs);
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
     const int c = threadIdx.x;
     if (c < C) {
-        const double rr = rrs[c];
-        const double rzn = rzs[c];
         double be = 0.0;
         if (cg[kCgActive + c] != 0.0) {
-            if (rr <= tol2) {
+            if (rrs[c] <= tol2) {
                 cg[kCgActive + c] = 0.0;
             } else {
-                be = rzn / cg[kCgRz + c];
-                cg[kCgRz + c] = rzn;
+                be = rzs[c] / cg[kCgRz + c];
+                cg[kCgRz + c] = rzs[c];
                 atomicAdd(&cnt, 1);
             }
         }
@@ -433,14 +455,13 @@ __global__ void __launch_bounds__(kBlock)
 cg_start_kernel(double* cg, const double* part1, int nblk, const double* scale, int C) {
     __shared__ double rzs[kMaxCgCols];
     __shared__ int cnt;
-    cg_reduce_parts(part1, nblk, C, rzs);
+    cg_reduce_parts(part1, nullptr, nblk, C, rzs, nullptr);
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
     const int c = threadIdx.x;
     if (c < C) {
-        const double rz = rzs[c];
-        cg[kCgRz + c] = rz;
-        const bool act = scale[c] > 0.0 && rz > 0.0;
+        cg[kCgRz + c] = rzs[c];
+        const bool act = scale[c] > 0.0 && rzs[c] > 0.0;
         cg[kCgActive + c] = act ? 1.0 : 0.0;
         cg[kCgAlpha + c] = 0.0;
         cg[kCgBeta + c] = 0.0;

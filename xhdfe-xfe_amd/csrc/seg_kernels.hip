@@ -165,6 +165,12 @@ seg_primary_kernel(const int* wseg, const int* seg_r1, int Nw, int G1, int* pf) 
         pf[w] = wseg[w] < wseg[w + 1] ? seg_r1[wseg[w]] : G1;
 }
 
+// flag += number of levels f < G with off[f] == off[f + 1] (no segment)
+__global__ void __launch_bounds__(kBlock) seg_empty_kernel(const int* off, int G, int* flag) {
+    for (int f = blockIdx.x * kBlock + threadIdx.x; f < G; f += gridDim.x * kBlock)
+        if (off[f] == off[f + 1]) atomicAdd(flag, 1);
+}
+
 // rank[perm[t]] = t
 __global__ void __launch_bounds__(kBlock) seg_rank_kernel(const int* perm, int m, int* rank) {
     for (int t = blockIdx.x * kBlock + threadIdx.x; t < m; t += gridDim.x * kBlock) rank[perm[t]] = t;
@@ -305,7 +311,41 @@ seg_bnd_kernel(const int* kin, const T* vin, int m, T* out, int C, int c0, int* 
     }
 }
 
-// tot[e] = sum over blocks of hpart[b][e], one block per entry e.
+// The last levels of the boundary reduction in one block of kBndTail threads:
+// levels are reduced in place of two ping-pong lists until one wave holds the
+// list, which is then stored completely.  (kin, vin) is the first list.
+template <typename T, int CC>
+__global__ void __launch_bounds__(kBndTail)
+seg_bnd_tail_kernel(int* k0, T* v0, int* k1, T* v1, int m, T* out, int C, int c0) {
+    const int lane = lane_id();
+    int* kin = k0;
+    T* vin = v0;
+    int* kout = k1;
+    T* vout = v1;
+    for (;;) {
+        const bool final = m <= kWave;
+        for (int base = 0; base < m; base += kBndTail) {
+            const int j = base + threadIdx.x;
+            const bool valid = j < m;
+            const int key = valid ? kin[j] : -1 - lane;
+            T v[CC];
+#pragma unroll
+            for (int c = 0; c < CC; ++c) v[c] = valid ? vin[static_cast<size_t>(j) * CC + c] : T(0);
+            const long long wbase = base + (threadIdx.x & ~(kWave - 1));
+            if (wbase < m) {
+                seg_wave_reduce<T, CC>(key, valid, v, wbase / kWave, wbase + kWave - 1 < m, final, out, C, c0,
+                                       kout, vout);
+            }
+        }
+        if (final) break;
+        __syncthreads();  // the next level reads what other waves wrote
+        m = static_cast<int>(2 * ((static_cast<long long>(m) + kWave - 1) / kWave));
+        int* tk = kin; kin = kout; kout = tk;
+        T* tv = vin; vin = vout; vout = tv;
+    }
+}
+
+// tot[e] = sum over blocks of hpart[b][e], one block per entry e.// tot[e] = sum over blocks of hpart[b][e], one block per entry e.
 __global__ void __launch_bounds__(kBlock) seg_hsum_kernel(const double* hpart, int nblk, int E, double* tot) {
     __shared__ double red[kBlock / kWave];
     const int e = blockIdx.x;
@@ -322,18 +362,29 @@ __global__ void __launch_bounds__(kBlock) seg_hsum_kernel(const double* hpart, i
 }
 
 // (S x)_2[y] = n_y x2[y] + prefix_y(tot); with x2 == nullptr the result is
-// n_y + prefix_y(tot) (the Jacobi diagonal of the year block).
+// n_y + prefix_y(tot) (the Jacobi diagonal of the year block).  The block
+// stages tot, n_y and x2 in LDS, one thread per column scans.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 seg_year_kernel(const double* tot, int G2, int cc, const double* counts, const T* x2, int C, int c0,
                 T* out2) {
-    const int c = threadIdx.x;
-    if (c >= cc) return;
-    double run = 0.0;
-    for (int y = 0; y < G2; ++y) {
-        run += tot[y * cc + c];
-        const double xv = x2 ? static_cast<double>(x2[static_cast<size_t>(y) * C + c0 + c]) : 1.0;
-        out2[static_cast<size_t>(y) * C + c0 + c] = static_cast<T>(counts[y] * xv + run);
+    __shared__ double st[(kSegMaxRange + 1) * 8];
+    __shared__ double sx[kSegMaxRange * 8];
+    __shared__ double sn[kSegMaxRange];
+    for (int k = threadIdx.x; k < G2 * cc; k += kBlock) {
+        st[k] = tot[k];
+        const int y = k / cc;
+        sx[k] = x2 ? static_cast<double>(x2[static_cast<size_t>(y) * C + c0 + (k - y * cc)]) : 1.0;
+    }
+    for (int y = threadIdx.x; y < G2; y += kBlock) sn[y] = counts[y];
+    __syncthreads();
+    for (int k = threadIdx.x; k < G2 * cc; k += kBlock) {
+        // the prefix of column c up to level y, by one thread per entry (G2 <= kSegMaxRange)
+        const int y = k / cc;
+        const int c = k - y * cc;
+        double run = 0.0;
+        for (int t = 0; t <= y; ++t) run += st[t * cc + c];
+        out2[static_cast<size_t>(y) * C + c0 + c] = static_cast<T>(sn[y] * sx[k] + run);
     }
 }
 
@@ -366,6 +417,11 @@ hipError_t launch_seg_offsets(const int* key, int M, int Nw, int* off, int grid,
 hipError_t launch_seg_primary(const int* wseg, const int* seg_r1, int Nw, int G1, int* pf, int grid,
                               hipStream_t s) {
     hipLaunchKernelGGL(seg_primary_kernel, dim3(grid), dim3(kBlock), 0, s, wseg, seg_r1, Nw, G1, pf);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_empty(const int* off, int G, int* flag, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(seg_empty_kernel, dim3(grid), dim3(kBlock), 0, s, off, G, flag);
     return hipGetLastError();
 }
 
@@ -467,6 +523,21 @@ template hipError_t launch_seg_bnd<float>(const int*, const float*, int, float*,
                                           int, hipStream_t);
 template hipError_t launch_seg_bnd<double>(const int*, const double*, int, double*, int, int, int, int*, double*,
                                            int, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_seg_bnd_tail(int* k0, T* v0, int* k1, T* v1, int m, T* out, int C, int c0, int cc,
+                               hipStream_t s) {
+    switch (cc) {
+#define XFE_TAIL(K) case K: hipLaunchKernelGGL((seg_bnd_tail_kernel<T, K>), dim3(1), dim3(kBndTail), 0, s, k0, v0, k1, v1, m, out, C, c0); break;
+        XFE_TAIL(1) XFE_TAIL(2) XFE_TAIL(3) XFE_TAIL(4) XFE_TAIL(5) XFE_TAIL(6) XFE_TAIL(7) XFE_TAIL(8)
+#undef XFE_TAIL
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_seg_bnd_tail<float>(int*, float*, int*, float*, int, float*, int, int, int, hipStream_t);
+template hipError_t launch_seg_bnd_tail<double>(int*, double*, int*, double*, int, double*, int, int, int,
+                                                hipStream_t);
 
 template <typename T>
 hipError_t launch_seg_year(const double* hpart, int nblk, int G2, int cc, double* tot, const double* counts,

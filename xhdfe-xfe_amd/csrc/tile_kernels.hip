@@ -317,6 +317,14 @@ __global__ void __launch_bounds__(kBlock) max_int_kernel(const int* v, int m, in
     if ((threadIdx.x & 63) == 0) atomicMax(out, mx);
 }
 
+__global__ void __launch_bounds__(kBlock) max_count_kernel(const double* v, int m, int* out) {
+    int mx = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock)
+        mx = max(mx, static_cast<int>(v[i] + 0.5));
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, mx);
+}
+
 // tile_start[k] = first level start of p at or after row k * step (n past the end).
 __global__ void __launch_bounds__(kBlock)
 tile_start_kernel(const int* gid, int n, int step, int num_tiles, int* tile_start) {
@@ -430,6 +438,11 @@ hipError_t launch_spell_pull(const double* S, int C, int c0, int cc, const int* 
 
 hipError_t launch_level_count(const int* gid, int n, int* cnt, int grid, hipStream_t s) {
     hipLaunchKernelGGL(level_count_kernel, dim3(grid), dim3(kBlock), 0, s, gid, n, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_count(const double* v, int m, int* out, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(max_count_kernel, dim3(grid), dim3(kBlock), 0, s, v, m, out);
     return hipGetLastError();
 }
 
