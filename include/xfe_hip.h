@@ -161,6 +161,52 @@ int xfe_recovery_max_delta(const double* residual, int n, const xfe_fe_input* fe
                            int num_fes, const double* weights, const int* sweep_order,
                            double* out_max_delta);
 
+/* ---- AKM/KSS leverage solver (include/hdfe/akm_kss_cuda.hpp:19-74, impl
+ * src/akm_kss_cuda.cu:164-722): the grounded worker-partialled firm Laplacian
+ * S = Df - B' Dw^{-1} B over n_firms levels (ground: the last firm), B the
+ * worker x firm match counts.  Same arguments and error convention as the
+ * reference's akm_cuda_* functions. */
+typedef struct xfe_akm_context xfe_akm_context;
+
+/* akm_cuda_available (:21). */
+int xfe_akm_available(void);
+
+/* akm_cuda_stable_sort_u64 (:27-29): order_out receives the 0-based row
+ * indices of the host keys in ascending key order, equal keys in input order.
+ * Returns 1 on success, 0 on failure. */
+int xfe_akm_stable_sort_u64(const uint64_t* keys, int64_t* order_out, size_t n);
+
+/* akm_cuda_create (:33-40): uploads the match structure (matches grouped by
+ * worker as w_ptr says; f_ptr / f_matches list each firm's matches).  Match
+ * counts m_c must be positive integers.  Returns NULL on failure (the caller
+ * falls back to its CPU solver). */
+xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, const int* m_f,
+                                const double* m_c, size_t n_matches, const int64_t* w_ptr,
+                                const int64_t* f_ptr, const int64_t* f_matches,
+                                const double* Dw, const double* Df, const double* inv_diagS);
+
+/* akm_cuda_destroy (:42). */
+void xfe_akm_destroy(xfe_akm_context* ctx);
+
+/* akm_cuda_solve_S (:47-48): Jacobi-PCG of S z = rhs (length n_firms, ground
+ * entry pinned to zero), stop at |r|^2 <= tol^2 |rhs|^2.  Returns the
+ * iteration count, or -1 when not converged within max_iter. */
+int xfe_akm_solve_S(xfe_akm_context* ctx, const double* rhs, double* z, double tol, int max_iter);
+
+/* akm_cuda_solve_S_multi (:59-61): nb <= xfe_akm_max_lanes() independent
+ * lanes at once; rhs_pack / z_pack hold nb contiguous length-n_firms columns
+ * (may alias).  iters_out[l]: iterations, or -1 (that lane's z is zero).
+ * Returns 0, or -1 on a device error. */
+int xfe_akm_solve_S_multi(xfe_akm_context* ctx, const double* rhs_pack, double* z_pack, int nb,
+                          double tol, int max_iter, int* iters_out);
+
+/* akm_cuda_max_lanes (:65). */
+int xfe_akm_max_lanes(void);
+
+/* akm_cuda_pack_buffer (:74): pinned host buffer of nb x n_firms doubles owned
+ * by the context, or NULL (then the caller uses its own buffer). */
+double* xfe_akm_pack_buffer(xfe_akm_context* ctx, int nb);
+
 /* Frees the calling thread's device workspace (it otherwise lives until the
  * process ends).  No counterpart in the reference. */
 void xfe_release_workspace(void);

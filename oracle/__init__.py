@@ -11,6 +11,9 @@ under ``xhdfe-xfe_amd/``) never imports it and never falls back to it.
   is; used to pin the restatement).
 * ``ols_cluster``: numpy restatement of the OLS + one-way cluster VCV
   (src/ols.cpp:1237-1556, scale G/(G-1) (N-1)/df_resid at :682-687).
+* ``akm_build`` / ``akm_solve_S``: numpy restatement of the AKM/KSS grounded
+  firm system and its Jacobi-PCG (src/akm_kss.cpp:1042-1091,
+  src/akm_kss_cuda.cu:532-695); pinned by a dense grounded solve.
 * ``certify``: numpy restatement of the normal-equation certificate
   (src/fe_absorption.cpp:14942-15190); pinned against the reference's own
   certificate (``ref_certify``) and a dense ||D'W v|| on tiny cases.
@@ -279,6 +282,75 @@ def ref_certify(y, X, yt, Xt, fes, weights=None, tol=1e-8, tolerance_mode=1, con
     if rc != 0:
         raise RuntimeError("ref_certify failed")
     return dict(abs_residual=a.value, abs_residual_rel=rl.value, certified=bool(cert.value))
+
+
+def akm_build(mw, mf, mc, N, J):
+    """TwoWaySolver::build (src/akm_kss.cpp:1042-1091): from matches (worker,
+    firm, count) grouped by worker, the CSR offsets w_ptr, the firm offsets
+    f_ptr and per-firm match lists f_matches (ascending match index), the
+    degrees Dw, Df and the Jacobi inverse diagonal of the grounded S
+    (diag S = Df - sum c^2 / Dw; 0 for the ground firm J-1 and non-positive
+    entries)."""
+    mw = np.asarray(mw, dtype=np.int32)
+    mf = np.asarray(mf, dtype=np.int32)
+    c = np.asarray(mc, dtype=np.float64)
+    Dw = np.bincount(mw, weights=c, minlength=N)
+    Df = np.bincount(mf, weights=c, minlength=J)
+    w_ptr = np.concatenate([[0], np.cumsum(np.bincount(mw, minlength=N))]).astype(np.int64)
+    f_ptr = np.concatenate([[0], np.cumsum(np.bincount(mf, minlength=J))]).astype(np.int64)
+    f_matches = np.argsort(mf, kind="stable").astype(np.int64)
+    diag = Df - np.bincount(mf, weights=c * c / Dw[mw], minlength=J)
+    inv_diag = np.where((np.arange(J) < J - 1) & (diag > 0.0), 1.0 / np.where(diag > 0.0, diag, 1.0), 0.0)
+    return dict(N=N, J=J, m_w=mw, m_f=mf, m_c=c, w_ptr=w_ptr, f_ptr=f_ptr, f_matches=f_matches,
+                Dw=Dw, Df=Df, inv_diag=inv_diag)
+
+
+def akm_apply_S(st, p):
+    """Grounded S p = Df p - B'(Dw^-1 B p), S p[ground] = 0
+    (src/akm_kss_cuda.cu:347-354 with the kernels at :33-57)."""
+    t = np.bincount(st["m_w"], weights=st["m_c"] * p[st["m_f"]], minlength=st["N"]) / st["Dw"]
+    out = st["Df"] * p - np.bincount(st["m_f"], weights=st["m_c"] * t[st["m_w"]], minlength=st["J"])
+    out[st["J"] - 1] = 0.0
+    return out
+
+
+def akm_solve_S(st, rhs, tol, max_iter):
+    """Jacobi-PCG on the grounded S for one lane (src/akm_kss_cuda.cu:532-695,
+    the per-lane rules of akm_cuda_solve_S_multi): ground entry of rhs pinned
+    to 0, x = 0, stop when r.r <= tol^2 |rhs|^2 (after the update of that
+    iteration), give up on pAp outside (0, 1e300) or after max_iter iterations.
+    Returns (z, iterations) with z = 0 and iterations = -1 when not converged."""
+    J = st["J"]
+    r = np.array(rhs, dtype=np.float64)
+    r[J - 1] = 0.0
+    x = np.zeros(J)
+    rhs2 = float(r @ r)
+    if rhs2 == 0.0:
+        return x, 0
+    tol2 = tol * tol * rhs2
+    z = st["inv_diag"] * r
+    p = z.copy()
+    rz = float(r @ z)
+    its = 0
+    while True:
+        Ap = akm_apply_S(st, p)
+        pAp = float(p @ Ap)
+        if not (pAp > 0.0) or not (pAp < 1e300):
+            return np.zeros(J), -1
+        a = rz / pAp
+        x += a * p
+        r -= a * Ap
+        z = st["inv_diag"] * r
+        rr = float(r @ r)
+        rz_new = float(r @ z)
+        its += 1
+        if rr <= tol2:
+            x[J - 1] = 0.0
+            return x, its
+        if its >= max_iter:
+            return np.zeros(J), -1
+        p = z + (rz_new / rz) * p
+        rz = rz_new
 
 
 def ref_absorb(y, X, fes, tol=1e-8, tolerance_mode=1, threads=1, symmetric=False):

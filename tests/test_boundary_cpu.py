@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     text = open(os.path.join(ROOT, "include", "xfe_hip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(xfe_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(xfe_[A-Za-z_0-9]+)\s*\(", text)))
 
 
 def test_header_symbols_exported():

@@ -354,3 +354,80 @@ def test_absorb_reports_certificate(cases, name, precision):
     o = oracle.certify(V, np.column_stack([yg, Xg]), gids, G, tol=1e-8)
     assert o["certified"]
     assert _rel_close(info["abs_residual_rel"], o["abs_residual_rel"], 1e-5), (info, o)
+
+
+# ---------------------------------------------------------------- AKM/KSS solver
+
+def _akm_graph(seed, N, J, movers):
+    rng = np.random.default_rng(seed)
+    mw, mf, mc = [], [], []
+    for w in range(N):
+        k = 1 + int(rng.random() < movers) + int(rng.random() < movers / 3)
+        firms = np.unique(np.minimum(J - 1, (rng.pareto(1.2, size=k) * 3).astype(int)))
+        for f in firms:
+            mw.append(w); mf.append(int(f)); mc.append(int(rng.integers(1, 6)))
+    # a chain of movers between consecutive firms keeps the firm graph connected
+    # (the reference solves on the leave-out connected set)
+    for f in range(1, J):
+        w = N + f - 1
+        mw += [w, w]; mf += [f - 1, f]; mc += [1, 1]
+    N = N + J - 1
+    return oracle.akm_build(mw, mf, mc, N, J)
+
+
+@pytest.mark.parametrize("nb", [1, 9, 32])
+def test_akm_solve_matches_oracle(nb):
+    _log(f"akm solve, {nb} lanes")
+    st = _akm_graph(7, N=20000, J=1500, movers=0.3)
+    J = st["J"]
+    rng = np.random.default_rng(nb)
+    rhs = rng.normal(size=(nb, J))
+    solver = xfe_amd.AkmSolver(st)
+    z, its = solver.solve_multi(rhs, tol=1e-10, max_iter=5000)
+    for l in range(nb):
+        zo, ito = oracle.akm_solve_S(st, rhs[l], 1e-10, 5000)
+        assert ito > 0 and its[l] > 0
+        assert abs(int(its[l]) - ito) <= 2, (l, its[l], ito)
+        scale = np.abs(zo).max()
+        assert np.abs(z[l] - zo).max() <= 1e-8 * scale, (l, np.abs(z[l] - zo).max() / scale)
+        assert z[l, J - 1] == 0.0
+    if nb == 1:
+        z1, it1 = solver.solve(rhs[0], tol=1e-10, max_iter=5000)
+        assert it1 == its[0] and np.array_equal(z1, z[0])
+    solver.close()
+
+
+def test_akm_lane_rules():
+    _log("akm lane rules")
+    st = _akm_graph(8, N=3000, J=300, movers=0.3)
+    J = st["J"]
+    rng = np.random.default_rng(3)
+    rhs = rng.normal(size=(4, J))
+    rhs[1] = 0.0                       # zero right-hand side: converged at once, z = 0
+    rhs[2] = 0.0; rhs[2, J - 1] = 7.0  # only the ground entry: pinned away
+    solver = xfe_amd.AkmSolver(st)
+    z, its = solver.solve_multi(rhs, tol=1e-10, max_iter=5000)
+    assert its[0] > 0 and its[1] == 0 and its[2] == 0 and its[3] > 0
+    assert not z[1].any() and not z[2].any()
+    z, its = solver.solve_multi(rhs[[0, 3]], tol=1e-14, max_iter=2)   # too few iterations
+    assert list(its) == [-1, -1] and not z.any()
+    assert solver.solve(rhs[0], tol=1e-14, max_iter=2)[1] == -1
+    solver.close()
+
+
+def test_akm_stable_sort_u64():
+    _log("akm stable sort")
+    rng = np.random.default_rng(9)
+    keys = rng.integers(0, 1 << 62, size=100000, dtype=np.uint64)
+    keys[::7] = keys[3]                # many equal keys: input order must be kept
+    keys[-5:] = np.uint64(2**64 - 1)
+    np.testing.assert_array_equal(xfe_amd.akm_stable_sort_u64(keys), np.argsort(keys, kind="stable"))
+
+
+def test_akm_create_rejects_bad_input():
+    st = _akm_graph(10, N=200, J=30, movers=0.3)
+    bad = dict(st)
+    bad["m_c"] = st["m_c"].copy()
+    bad["m_c"][0] = 1.5                # counts must be integers
+    with pytest.raises(xfe_amd.XfeError):
+        xfe_amd.AkmSolver(bad)

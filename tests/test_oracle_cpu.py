@@ -308,3 +308,62 @@ def test_certificate_dense_formula():
     assert not oracle.certify(V, small * 2, gids, G, weights=w, tol=1e-8)["certified"]  # 1e-8 > 8e-9
     assert not oracle.certify(V, small, gids, G, weights=w, tol=1e-8, converged=False)["certified"]
     assert oracle.certify(V, small * 2, gids, G, weights=w, tol=1e-8, tolerance_mode=0)["certified"]
+
+
+# ---------------------------------------------------------------- AKM/KSS grounded solve
+# TwoWaySolver::build (src/akm_kss.cpp:1042-1091) and the grounded Jacobi-PCG
+# (src/akm_kss_cuda.cu:532-695), restated in the oracle, against a dense solve.
+
+def akm_graph(seed, N=400, J=60, movers=0.4):
+    rng = np.random.default_rng(seed)
+    mw, mf, mc = [], [], []
+    for w in range(N):
+        k = 1 + int(rng.random() < movers) + int(rng.random() < movers / 3)
+        firms = np.unique(np.minimum(J - 1, (rng.pareto(1.2, size=k) * 3).astype(int)))
+        for f in firms:
+            mw.append(w); mf.append(int(f)); mc.append(int(rng.integers(1, 6)))
+    # a chain of movers between consecutive firms keeps the firm graph connected
+    # (the reference solves on the leave-out connected set)
+    for f in range(1, J):
+        w = N + f - 1
+        mw += [w, w]; mf += [f - 1, f]; mc += [1, 1]
+    N = N + J - 1
+    return oracle.akm_build(mw, mf, mc, N, J)
+
+
+def _dense_S(st):
+    B = np.zeros((st["N"], st["J"]))
+    np.add.at(B, (st["m_w"], st["m_f"]), st["m_c"])
+    return np.diag(st["Df"]) - B.T @ (B / st["Dw"][:, None])
+
+
+def test_akm_oracle_matches_dense_solve():
+    for seed in (1, 2, 3):
+        st = akm_graph(seed)
+        J = st["J"]
+        S = _dense_S(st)
+        rng = np.random.default_rng(seed + 10)
+        p = rng.normal(size=J); p[J - 1] = 0.0
+        Sp = S @ p; Sp[J - 1] = 0.0
+        np.testing.assert_allclose(oracle.akm_apply_S(st, p), Sp, rtol=1e-12, atol=1e-12 * np.abs(Sp).max())
+        rhs = rng.normal(size=J)
+        z, it = oracle.akm_solve_S(st, rhs, 1e-12, 10000)
+        # firms unreachable from the rest make S singular: solve on the reachable block by lstsq
+        zd = np.zeros(J)
+        zd[:J - 1] = np.linalg.lstsq(S[:J - 1, :J - 1], rhs[:J - 1], rcond=None)[0]
+        assert it > 0
+        assert np.abs(z - zd).max() <= 1e-8 * np.abs(zd).max(), (seed, np.abs(z - zd).max())
+        assert z[J - 1] == 0.0
+
+
+def test_akm_oracle_lane_rules():
+    st = akm_graph(4)
+    J = st["J"]
+    z, it = oracle.akm_solve_S(st, np.zeros(J), 1e-10, 100)
+    assert it == 0 and not z.any()
+    rhs = np.zeros(J); rhs[J - 1] = 5.0              # only the ground entry: pinned away
+    z, it = oracle.akm_solve_S(st, rhs, 1e-10, 100)
+    assert it == 0 and not z.any()
+    rhs = np.random.default_rng(5).normal(size=J)
+    z, it = oracle.akm_solve_S(st, rhs, 1e-14, 2)    # cannot converge in 2 iterations
+    assert it == -1 and not z.any()

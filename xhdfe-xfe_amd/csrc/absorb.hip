@@ -1145,14 +1145,14 @@ struct FastSolver {
         const int g = cg_grid(total, C, kMaxGrid);
         double* cg = ws.cg.p;
         double* part = ws.cg_part.p;
-        XFE_CHECK(launch_cg_init(ws.f_b.p, ws.f_scale.p, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p,
-                                 total, C, cg, part, g, s));
+        XFE_CHECK(launch_cg_init<float>(ws.f_b.p, ws.f_scale.p, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p,
+                                        ws.f_p.p, total, C, cg, part, g, s));
         int it = 0;
         for (it = 1; it <= max_inner; ++it) {
             apply32(ws.f_p.p, ws.f_q.p);
-            XFE_CHECK(launch_cg_dot(ws.f_p.p, ws.f_q.p, total, C, cg, part, g, s));
-            XFE_CHECK(launch_cg_step(ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p, ws.f_q.p, ws.f_minv.p, total, C,
-                                     cg, part, tol_in * tol_in, g, s));
+            XFE_CHECK(launch_cg_dot<float>(ws.f_p.p, ws.f_q.p, total, C, cg, part, g, s));
+            XFE_CHECK(launch_cg_step<float>(ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p, ws.f_q.p, ws.f_minv.p, total, C,
+                                            cg, part, tol_in * tol_in, max_inner + 1, g, s));
             if (it % kPoll == 0 || it == max_inner) {
                 XFE_CHECK(hipMemcpyAsync(ws.host_scalars, cg + kCgCount, sizeof(double), hipMemcpyDeviceToHost, s));
                 XFE_CHECK(hipStreamSynchronize(s));

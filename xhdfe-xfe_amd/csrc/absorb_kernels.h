@@ -183,7 +183,8 @@ hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, i
 enum {
     kCgAlpha = 0 * kMaxCgCols, kCgBeta = 1 * kMaxCgCols, kCgRz = 2 * kMaxCgCols,
     kCgActive = 3 * kMaxCgCols, kCgPq = 4 * kMaxCgCols, kCgRr = 5 * kMaxCgCols,
-    kCgRzNew = 6 * kMaxCgCols, kCgCount = 7 * kMaxCgCols, kCgSlots = 7 * kMaxCgCols + 8
+    kCgRzNew = 6 * kMaxCgCols, kCgIters = 7 * kMaxCgCols, kCgOk = 8 * kMaxCgCols,
+    kCgCount = 9 * kMaxCgCols, kCgSlots = 9 * kMaxCgCols + 8
 };
 // grid for the CG vector kernels: a multiple of C (each thread keeps one column)
 inline int cg_grid(long long total, int C, int max_grid) {
@@ -193,15 +194,19 @@ inline int cg_grid(long long total, int C, int max_grid) {
     if (per < 1) per = 1;
     return static_cast<int>(per * C);
 }
-// pq -> alpha (cg_dot + cg_alpha); part: grid x C doubles (2 x for cg_step)
-hipError_t launch_cg_dot(const float* a, const float* b, long long total, int C, double* cg, double* part,
-                         int grid, hipStream_t s);
-// x, r, z update, r.r and r.z -> beta and the active count, then p = z + beta p
-hipError_t launch_cg_step(float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                          long long total, int C, double* cg, double* part, double tol2, int grid, hipStream_t s);
-hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x, float* r,
-                          float* z, float* p, long long total, int C, double* cg, double* part, int grid,
-                          hipStream_t s);
+// pq -> alpha (cg_dot + cg_alpha); part: grid x C doubles (2 x for cg_step).
+// T = float (mixed-precision inner solves) or double (AKM/KSS solves).
+template <typename T>
+hipError_t launch_cg_dot(const T* a, const T* b, long long total, int C, double* cg, double* part, int grid,
+                         hipStream_t s);
+// x, r, z update, r.r and r.z -> beta, per-column iteration counts, the
+// max_iter cap and the active count, then p = z + beta p
+template <typename T>
+hipError_t launch_cg_step(T* x, T* r, T* z, T* p, const T* q, const T* minv, long long total, int C, double* cg,
+                          double* part, double tol2, int max_iter, int grid, hipStream_t s);
+template <typename T>
+hipError_t launch_cg_init(const double* b, const double* scale, const T* minv, T* x, T* r, T* z, T* p,
+                          long long total, int C, double* cg, double* part, int grid, hipStream_t s);
 hipError_t launch_refine_update(double* beta, const float* x, const double* inv_scale, int N, int C,
                                 int grid, hipStream_t s);
 hipError_t launch_schur_diag(const int* tile_start, int num_tiles, const int* gid_p,

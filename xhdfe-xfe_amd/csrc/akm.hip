@@ -1,0 +1,389 @@
+// AKM/KSS leverage solver: the grounded worker-partialled firm Laplacian
+//     S = Df - B' Dw^{-1} B
+// (B: worker x firm match counts m_c) and its Jacobi-preconditioned CG for
+// up to kMaxCgCols right-hand sides at once.  The drop-in for the reference's
+// akm_cuda_* functions (include/hdfe/akm_kss_cuda.hpp:19-74, impl
+// src/akm_kss_cuda.cu:164-722).
+//
+// S is the two-dimensional case of the segment operator of the mixed-precision
+// absorber: a match (worker w, firm f, count c) is a segment of length c with
+// no year range, so  S p = Df p - B'(Dw^{-1} B p)  is one worker-major pass
+// (t_w = Dw^{-1} sum_a c_a p[f_a]) and one firm-major pass (sum over the
+// firm's matches of c_a (p[f] - t_w(a)) with the atomic-free hierarchical
+// reduction), in FP64, for all lanes of a chunk together (level-major J x nb
+// vectors: one gather reads every lane of a firm).  The ground firm
+// (J - 1) is pinned to zero as in the reference.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "absorb_kernels.h"
+#include "common.h"
+#include "xfe_hip.h"
+
+namespace xfe {
+namespace {
+
+// level-major (f * nb + l) <- lane-major (l * J + f), ground rows zeroed
+__global__ void __launch_bounds__(kBlock)
+akm_to_levels(const double* lanes, int J, int nb, int ground, double* levels) {
+    const long long total = static_cast<long long>(J) * nb;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int f = static_cast<int>(j / nb);
+        const int l = static_cast<int>(j - static_cast<long long>(f) * nb);
+        levels[j] = f == ground ? 0.0 : lanes[static_cast<size_t>(l) * J + f];
+    }
+}
+
+// lane-major z = unscale_l * x, ground zero, lanes that did not converge zero
+__global__ void __launch_bounds__(kBlock)
+akm_to_lanes(const double* x, int J, int nb, int ground, const double* unscale, const double* cg,
+             double* lanes) {
+    const long long total = static_cast<long long>(J) * nb;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int l = static_cast<int>(j / J);
+        const int f = static_cast<int>(j - static_cast<long long>(l) * J);
+        const bool ok = cg[kCgOk + l] != 0.0;
+        lanes[j] = (f == ground || !ok) ? 0.0 : x[static_cast<size_t>(f) * nb + l] * unscale[l];
+    }
+}
+
+// out = Df p - B' t: the firm pass gave sum_a c_a (p[f] - t_w(a)) for firms
+// with matches; add (Df - deg) p, take Df p for firms without matches, pin the ground
+__global__ void __launch_bounds__(kBlock)
+akm_finish(double* out, const double* p, const double* corr, const int* has, int J, int nb, int ground) {
+    const long long total = static_cast<long long>(J) * nb;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int f = static_cast<int>(j / nb);
+        double v = has[f] ? out[j] : 0.0;
+        v += corr[f] * p[j];
+        out[j] = f == ground ? 0.0 : v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) akm_iota64(int64_t* v, size_t n) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * kBlock)
+        v[i] = static_cast<int64_t>(i);
+}
+
+constexpr int kAkmMaxGrid = 2048;
+
+int akm_grid(long long work) {
+    long long g = (work + kBlock - 1) / kBlock;
+    return static_cast<int>(std::max(1LL, std::min<long long>(g, 2048)));
+}
+
+}  // namespace
+}  // namespace xfe
+
+using namespace xfe;
+
+struct xfe_akm_context {
+    int N = 0, J = 0, ground = 0, M = 0;
+    hipStream_t s = nullptr;
+    // worker-major matches
+    DevBuf<int> wseg, r1, len;
+    // firm-major matches
+    DevBuf<int> fkey, fw, flen, has;
+    DevBuf<double> inv_dw, inv_diag, corr;
+    // workspace
+    DevBuf<double> wmean, lanes, b, x, r, z, p, q, cg, part, scale;
+    DevBuf<int> bk[2];
+    DevBuf<double> bv[2];
+    double* h_pack = nullptr;   // pinned J x lanes staging
+    int h_pack_lanes = 0;
+    double* h_small = nullptr;  // pinned scalars
+
+    ~xfe_akm_context() {
+        if (h_pack) (void)hipHostFree(h_pack);
+        if (h_small) (void)hipHostFree(h_small);
+        if (s) (void)hipStreamDestroy(s);
+    }
+
+    // out = S p for level-major J x nb vectors (ground pinned)
+    void apply(const double* pv, double* out, int nb) {
+        for (int c0 = 0; c0 < nb; c0 += 8) {
+            const int cc = std::min(8, nb - c0);
+            SegArgs<double> a;
+            a.C = nb; a.c0 = c0; a.Nw = N; a.M = M; a.G2 = 0;
+            a.x1 = pv; a.x2 = nullptr; a.inv_w = inv_dw.p;
+            a.wseg = wseg.p; a.seg_r1 = r1.p; a.seg_len = len.p; a.seg_a = nullptr;
+            a.wmean = wmean.p;
+            a.f_key = fkey.p; a.f_w = fw.p; a.f_len = flen.p; a.f_a = nullptr;
+            a.out1 = out; a.hpart = nullptr;
+            a.bkey = bk[0].p; a.bval = bv[0].p;
+            XFE_CHECK(launch_seg_worker<double>(a, cc, false, akm_grid(N), s));
+            XFE_CHECK(launch_seg_firm<double>(a, cc, false, false, akm_grid(M), s));
+            int m = static_cast<int>(2 * ((static_cast<long long>(M) + kWave - 1) / kWave));
+            int cur = 0;
+            while (m > kBndTail) {
+                XFE_CHECK(launch_seg_bnd<double>(bk[cur].p, bv[cur].p, m, out, nb, c0, cc, bk[1 - cur].p,
+                                                 bv[1 - cur].p, 0, akm_grid(m), s));
+                m = static_cast<int>(2 * ((static_cast<long long>(m) + kWave - 1) / kWave));
+                cur = 1 - cur;
+            }
+            XFE_CHECK(launch_seg_bnd_tail<double>(bk[cur].p, bv[cur].p, bk[1 - cur].p, bv[1 - cur].p, m, out, nb,
+                                                  c0, cc, s));
+        }
+        hipLaunchKernelGGL(akm_finish, dim3(akm_grid(static_cast<long long>(J) * nb)), dim3(kBlock), 0, s, out, pv,
+                           corr.p, has.p, J, nb, ground);
+        XFE_CHECK(hipGetLastError());
+    }
+
+    void ensure_lanes(int nb) {
+        const size_t JL = static_cast<size_t>(J) * nb;
+        lanes.ensure(JL); b.ensure(JL); x.ensure(JL); r.ensure(JL); z.ensure(JL); p.ensure(JL); q.ensure(JL);
+    }
+
+    // Batched Jacobi-PCG (src/akm_kss_cuda.cu:532-695): per lane, the
+    // right-hand side with its ground entry pinned is scaled to unit norm, so
+    // the reference's relative stop r.r <= tol^2 |rhs|^2 becomes r.r <= tol^2;
+    // a lane stops on convergence, on pAp outside (0, 1e300) or after max_iter
+    // iterations.  Returns per-lane iteration counts (-1: not converged, the
+    // lane's solution is zero).
+    void solve(const double* rhs_pack, double* z_pack, int nb, double tol, int max_iter, int* iters_out) {
+        ensure_lanes(nb);
+        const size_t JL = static_cast<size_t>(J) * nb;
+        XFE_CHECK(hipMemcpyAsync(lanes.p, rhs_pack, sizeof(double) * JL, hipMemcpyHostToDevice, s));
+        const int gj = akm_grid(static_cast<long long>(JL));
+        hipLaunchKernelGGL(akm_to_levels, dim3(gj), dim3(kBlock), 0, s, lanes.p, J, nb, ground, b.p);
+        XFE_CHECK(hipGetLastError());
+        // per-lane norms of the pinned right-hand sides
+        const int vg = akm_grid(J);
+        part.ensure(std::max<size_t>(static_cast<size_t>(vg) * nb, 2 * static_cast<size_t>(kAkmMaxGrid) * kMaxCgCols));
+        XFE_CHECK(launch_vdot<double>(b.p, b.p, J, nb, part.p, vg, s));
+        XFE_CHECK(launch_reduce_partials(part.p, vg, nb, 0, scale.p + 2 * kMaxCgCols, s));
+        XFE_CHECK(hipMemcpyAsync(h_small, scale.p + 2 * kMaxCgCols, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        double sc[2 * kMaxCgCols] = {};
+        for (int l = 0; l < nb; ++l) {
+            const double nrm = std::sqrt(h_small[l]);
+            sc[l] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+            sc[kMaxCgCols + l] = nrm;
+        }
+        std::memcpy(h_small, sc, sizeof(sc));
+        XFE_CHECK(hipMemcpyAsync(scale.p, h_small, sizeof(sc), hipMemcpyHostToDevice, s));
+        const long long total = static_cast<long long>(JL);
+        const int g = cg_grid(total, nb, kAkmMaxGrid);
+        XFE_CHECK(launch_cg_init<double>(b.p, scale.p, inv_diag.p, x.p, r.p, z.p, p.p, total, nb, cg.p, part.p, g, s));
+        constexpr int kPoll = 8;
+        const double tol2 = tol * tol;
+        for (int it = 1; it <= max_iter; ++it) {
+            apply(p.p, q.p, nb);
+            XFE_CHECK(launch_cg_dot<double>(p.p, q.p, total, nb, cg.p, part.p, g, s));
+            XFE_CHECK(launch_cg_step<double>(x.p, r.p, z.p, p.p, q.p, inv_diag.p, total, nb, cg.p, part.p, tol2,
+                                             max_iter, g, s));
+            if (it % kPoll == 0 || it == max_iter) {
+                XFE_CHECK(hipMemcpyAsync(h_small, cg.p + kCgCount, sizeof(double), hipMemcpyDeviceToHost, s));
+                XFE_CHECK(hipStreamSynchronize(s));
+                if (h_small[0] == 0.0) break;
+            }
+        }
+        hipLaunchKernelGGL(akm_to_lanes, dim3(gj), dim3(kBlock), 0, s, x.p, J, nb, ground, scale.p + kMaxCgCols,
+                           cg.p, lanes.p);
+        XFE_CHECK(hipGetLastError());
+        XFE_CHECK(hipMemcpyAsync(z_pack, lanes.p, sizeof(double) * JL, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipMemcpyAsync(h_small, cg.p + kCgIters, sizeof(double) * 2 * kMaxCgCols, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        for (int l = 0; l < nb; ++l) {
+            const bool ok = h_small[kMaxCgCols + l] != 0.0;
+            iters_out[l] = ok ? static_cast<int>(h_small[l]) : -1;
+        }
+    }
+};
+
+extern "C" {
+
+int xfe_akm_available(void) { return xfe_backend_available(); }
+
+int xfe_akm_max_lanes(void) { return kMaxCgCols; }
+
+int xfe_akm_stable_sort_u64(const uint64_t* keys, int64_t* order_out, size_t n) {
+    clear_error();
+    if (n == 0) return 1;
+    if (!keys || !order_out || n > static_cast<size_t>(INT32_MAX)) {
+        set_error("xfe_akm_stable_sort_u64: invalid arguments");
+        return 0;
+    }
+    try {
+        (void)hipGetLastError();
+        hipStream_t s = nullptr;
+        XFE_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        DevBuf<uint64_t> kin, kout;
+        DevBuf<int64_t> vin, vout;
+        DevBuf<unsigned char> tmp;
+        kin.ensure(n); kout.ensure(n); vin.ensure(n); vout.ensure(n);
+        XFE_CHECK(hipMemcpyAsync(kin.p, keys, sizeof(uint64_t) * n, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(akm_iota64, dim3(akm_grid(static_cast<long long>(n))), dim3(kBlock), 0, s, vin.p, n);
+        XFE_CHECK(hipGetLastError());
+        size_t bytes = 0;
+        XFE_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin.p, kout.p, vin.p, vout.p, n, 0, 64, s));
+        tmp.ensure(bytes);
+        XFE_CHECK(rocprim::radix_sort_pairs(tmp.p, bytes, kin.p, kout.p, vin.p, vout.p, n, 0, 64, s));
+        XFE_CHECK(hipMemcpyAsync(order_out, vout.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        kin.release(); kout.release(); vin.release(); vout.release(); tmp.release();
+        (void)hipStreamDestroy(s);
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, const int* m_f, const double* m_c,
+                                size_t n_matches, const int64_t* w_ptr, const int64_t* f_ptr,
+                                const int64_t* f_matches, const double* Dw, const double* Df,
+                                const double* inv_diagS) {
+    clear_error();
+    if (n_workers <= 0 || n_firms <= 0 || n_matches == 0 || n_matches > static_cast<size_t>(INT32_MAX) || !m_w ||
+        !m_f || !m_c || !w_ptr || !f_ptr || !f_matches || !Dw || !Df || !inv_diagS) {
+        set_error("xfe_akm_create: invalid arguments");
+        return nullptr;
+    }
+    const int N = n_workers, J = n_firms, M = static_cast<int>(n_matches);
+    if (w_ptr[0] != 0 || w_ptr[N] != M || f_ptr[0] != 0 || f_ptr[J] != M) {
+        set_error("xfe_akm_create: inconsistent match structure");
+        return nullptr;
+    }
+    // host restatement of the structure into the segment records
+    std::vector<int> wseg(N + 1), len(M), fkey(M), fw(M), flen(M), has(J, 0);
+    std::vector<double> inv_dw(N), corr(J);
+    for (int w = 0; w <= N; ++w) wseg[w] = static_cast<int>(w_ptr[w]);
+    for (int w = 0; w < N; ++w) {
+        if (w_ptr[w + 1] < w_ptr[w]) {
+            set_error("xfe_akm_create: w_ptr is not non-decreasing");
+            return nullptr;
+        }
+        for (int64_t a = w_ptr[w]; a < w_ptr[w + 1]; ++a) {
+            if (m_w[a] != w) {
+                set_error("xfe_akm_create: matches are not grouped by worker as w_ptr says");
+                return nullptr;
+            }
+        }
+    }
+    for (int a = 0; a < M; ++a) {
+        const double c = m_c[a];
+        if (!(c >= 1.0) || c != std::floor(c) || c > 2.0e9 || m_f[a] < 0 || m_f[a] >= J || m_w[a] < 0 ||
+            m_w[a] >= N) {
+            set_error("xfe_akm_create: match counts must be positive integers and ids in range");
+            return nullptr;
+        }
+        len[a] = static_cast<int>(c);
+    }
+    std::vector<double> deg(J, 0.0);
+    for (int f = 0; f < J; ++f) {
+        for (int64_t j = f_ptr[f]; j < f_ptr[f + 1]; ++j) {
+            const int64_t m = f_matches[j];
+            if (m < 0 || m >= M || m_f[m] != f) {
+                set_error("xfe_akm_create: f_matches does not list the firm's matches");
+                return nullptr;
+            }
+            fkey[j] = f;
+            fw[j] = m_w[m];
+            flen[j] = len[m];
+            deg[f] += m_c[m];
+        }
+        has[f] = f_ptr[f + 1] > f_ptr[f] ? 1 : 0;
+        corr[f] = Df[f] - deg[f];
+    }
+    for (int w = 0; w < N; ++w) inv_dw[w] = Dw[w] != 0.0 ? 1.0 / Dw[w] : 0.0;
+    xfe_akm_context* c = nullptr;
+    try {
+        (void)hipGetLastError();
+        c = new xfe_akm_context();
+        c->N = N; c->J = J; c->M = M; c->ground = J - 1;
+        XFE_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        auto up_i = [&](DevBuf<int>& d, const std::vector<int>& h) {
+            d.ensure(h.size());
+            XFE_CHECK(hipMemcpyAsync(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, c->s));
+        };
+        auto up_d = [&](DevBuf<double>& d, const double* h, size_t n) {
+            d.ensure(n);
+            XFE_CHECK(hipMemcpyAsync(d.p, h, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+        };
+        up_i(c->wseg, wseg);
+        c->r1.ensure(M);
+        XFE_CHECK(hipMemcpyAsync(c->r1.p, m_f, sizeof(int) * M, hipMemcpyHostToDevice, c->s));
+        up_i(c->len, len);
+        up_i(c->fkey, fkey);
+        up_i(c->fw, fw);
+        up_i(c->flen, flen);
+        up_i(c->has, has);
+        up_d(c->inv_dw, inv_dw.data(), N);
+        up_d(c->inv_diag, inv_diagS, J);
+        up_d(c->corr, corr.data(), J);
+        c->wmean.ensure(static_cast<size_t>(N) * 8);
+        const size_t nb2 = 2 * ((static_cast<size_t>(M) + kWave - 1) / kWave);
+        for (int k = 0; k < 2; ++k) {
+            c->bk[k].ensure(nb2);
+            c->bv[k].ensure(nb2 * 8);
+        }
+        c->cg.ensure(kCgSlots);
+        XFE_CHECK(hipMemsetAsync(c->cg.p, 0, sizeof(double) * kCgSlots, c->s));
+        c->part.ensure(2 * static_cast<size_t>(kAkmMaxGrid) * kMaxCgCols);
+        c->scale.ensure(3 * kMaxCgCols);
+        XFE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_small), 4 * kMaxCgCols * sizeof(double),
+                                hipHostMallocDefault));
+        XFE_CHECK(hipStreamSynchronize(c->s));  // the host vectors above go out of scope
+        return c;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        delete c;
+        return nullptr;
+    }
+}
+
+void xfe_akm_destroy(xfe_akm_context* ctx) { delete ctx; }
+
+double* xfe_akm_pack_buffer(xfe_akm_context* c, int nb) {
+    clear_error();
+    if (!c || nb < 1 || nb > kMaxCgCols) return nullptr;
+    if (c->h_pack_lanes >= nb) return c->h_pack;
+    if (c->h_pack) (void)hipHostFree(c->h_pack);
+    c->h_pack = nullptr;
+    c->h_pack_lanes = 0;
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_pack), sizeof(double) * static_cast<size_t>(c->J) * nb,
+                      hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c->h_pack = nullptr;  // the caller uses its own buffer
+        return nullptr;
+    }
+    c->h_pack_lanes = nb;
+    return c->h_pack;
+}
+
+int xfe_akm_solve_S_multi(xfe_akm_context* c, const double* rhs_pack, double* z_pack, int nb, double tol,
+                          int max_iter, int* iters_out) {
+    clear_error();
+    if (!c || !rhs_pack || !z_pack || !iters_out || nb < 1 || nb > kMaxCgCols || max_iter < 1) {
+        set_error("xfe_akm_solve_S_multi: invalid arguments");
+        return -1;
+    }
+    try {
+        (void)hipGetLastError();
+        c->solve(rhs_pack, z_pack, nb, tol, max_iter, iters_out);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int xfe_akm_solve_S(xfe_akm_context* c, const double* rhs, double* z, double tol, int max_iter) {
+    int it = -1;
+    if (xfe_akm_solve_S_multi(c, rhs, z, 1, tol, max_iter, &it) != 0) return -1;
+    return it;
+}
+
+}  // extern "C"

@@ -292,14 +292,19 @@ vdot_kernel(const T* a, const T* b, int N, int C, double* partials) {
 // the XCD's L2 and costs more than the extra launch.)
 
 namespace {
-// part[block * C + c] = sum of v over the block's threads of column c
+// part[block * C + c] = sum of v over the block's threads of column c, added
+// in thread order (thread t holds column (block offset + t) % C): the same
+// inputs give the same bits on every run
 __device__ __forceinline__ void cg_block_store(double v, int c, int C, double* part) {
-    __shared__ double red[kMaxCgCols];
-    if (threadIdx.x < C) red[threadIdx.x] = 0.0;
+    __shared__ double red[kBlock];
+    red[threadIdx.x] = v;
     __syncthreads();
-    atomicAdd(&red[c], v);
-    __syncthreads();
-    if (threadIdx.x < C) part[static_cast<size_t>(blockIdx.x) * C + threadIdx.x] = red[threadIdx.x];
+    if (static_cast<int>(threadIdx.x) < C) {
+        // threads k = t, t + C, t + 2C, ... hold this thread's column c (= column of thread t)
+        double t = 0.0;
+        for (int k = threadIdx.x; k < kBlock; k += C) t += red[k];
+        part[static_cast<size_t>(blockIdx.x) * C + c] = t;
+    }
     __syncthreads();
 }
 
@@ -346,8 +351,9 @@ __device__ __forceinline__ void cg_reduce_parts(const double* part, const double
 }  // namespace
 
 // part1[block, c] = sum_j a[j, c] b[j, c]
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-cg_dot_kernel(const float* a, const float* b, long long total, int C, double* part1) {
+cg_dot_kernel(const T* a, const T* b, long long total, int C, double* part1) {
     const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     const int c = static_cast<int>(g0 % C);
@@ -356,7 +362,8 @@ cg_dot_kernel(const float* a, const float* b, long long total, int C, double* pa
     cg_block_store(acc, c, C, part1);
 }
 
-// alpha = rz / pq for active columns (a column with pq <= 0 or rz <= 0 stops)
+// alpha = rz / pq for active columns; a column with pq not in (0, 1e300) or
+// rz <= 0 stops (without converging)
 __global__ void __launch_bounds__(kBlock) cg_alpha_kernel(double* cg, const double* part1, int nblk, int C) {
     __shared__ double pqs[kMaxCgCols];
     cg_reduce_parts(part1, nullptr, nblk, C, pqs, nullptr);
@@ -366,37 +373,40 @@ __global__ void __launch_bounds__(kBlock) cg_alpha_kernel(double* cg, const doub
     const double rz = cg[kCgRz + c];
     double al = 0.0;
     if (cg[kCgActive + c] != 0.0) {
-        if (pq > 0.0 && rz > 0.0) al = rz / pq;
+        if (pq > 0.0 && pq < 1e300 && rz > 0.0) al = rz / pq;
         else cg[kCgActive + c] = 0.0;
     }
     cg[kCgAlpha + c] = al;
 }
 
 // x += alpha p; r -= alpha q; z = minv r; part1 = r.r, part2 = r.z per block and column
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-cg_step_kernel(float* x, float* r, float* z, const float* p, const float* qv, const float* minv,
-               long long total, int C, const double* cg, double* part1, double* part2) {
+cg_step_kernel(T* x, T* r, T* z, const T* p, const T* qv, const T* minv, long long total, int C,
+               const double* cg, double* part1, double* part2) {
     const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     const int c = static_cast<int>(g0 % C);
-    const float al = static_cast<float>(cg[kCgAlpha + c]);
+    const T al = static_cast<T>(cg[kCgAlpha + c]);
     double rr = 0.0, rz = 0.0;
     for (long long j = g0; j < total; j += stride) {
         x[j] += al * p[j];
-        const float rn = r[j] - al * qv[j];
+        const T rn = r[j] - al * qv[j];
         r[j] = rn;
-        const float zn = minv[j / C] * rn;
+        const T zn = minv[j / C] * rn;
         z[j] = zn;
-        rr += static_cast<double>(rn) * rn;
-        rz += static_cast<double>(rn) * zn;
+        rr += static_cast<double>(rn) * static_cast<double>(rn);
+        rz += static_cast<double>(rn) * static_cast<double>(zn);
     }
     cg_block_store(rr, c, C, part1);
     cg_block_store(rz, c, C, part2);
 }
 
-// beta = rz_new / rz; a column whose r.r <= tol2 stops; cg[kCgCount] = active columns
+// per active column: one more iteration; r.r <= tol2 converges it, reaching
+// max_iter stops it; otherwise beta = rz_new / rz.  cg[kCgCount] = active columns
 __global__ void __launch_bounds__(kBlock)
-cg_beta_kernel(double* cg, const double* part1, const double* part2, int nblk, int C, double tol2) {
+cg_beta_kernel(double* cg, const double* part1, const double* part2, int nblk, int C, double tol2,
+               int max_iter) {
     __shared__ double rrs[kMaxCgCols], rzs[kMaxCgCols];
     __shared__ int cnt;
     cg_reduce_parts(part1, part2, nblk, C, rrs, rzs);
@@ -406,7 +416,12 @@ cg_beta_kernel(double* cg, const double* part1, const double* part2, int nblk, i
     if (c < C) {
         double be = 0.0;
         if (cg[kCgActive + c] != 0.0) {
+            const double its = cg[kCgIters + c] + 1.0;
+            cg[kCgIters + c] = its;
             if (rrs[c] <= tol2) {
+                cg[kCgActive + c] = 0.0;
+                cg[kCgOk + c] = 1.0;
+            } else if (its >= max_iter) {
                 cg[kCgActive + c] = 0.0;
             } else {
                 be = rzs[c] / cg[kCgRz + c];
@@ -421,36 +436,39 @@ cg_beta_kernel(double* cg, const double* part1, const double* part2, int nblk, i
 }
 
 // p = z + beta p
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-cg_dir_kernel(float* p, const float* z, const double* cg, long long total, int C) {
+cg_dir_kernel(T* p, const T* z, const double* cg, long long total, int C) {
     const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
-    const float be = static_cast<float>(cg[kCgBeta + static_cast<int>(g0 % C)]);
+    const T be = static_cast<T>(cg[kCgBeta + static_cast<int>(g0 % C)]);
     for (long long j = g0; j < total; j += stride) p[j] = z[j] + be * p[j];
 }
 
-// CG start: r = b * scale_c (to FP32), x = 0, z = minv r, p = z; part1 = r.z
+// CG start: r = b * scale_c (rounded to T), x = 0, z = minv r, p = z; part1 = r.z
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
-cg_init_kernel(const double* b, const double* scale, const float* minv, float* x, float* r, float* z,
-               float* p, long long total, int C, double* part1) {
+cg_init_kernel(const double* b, const double* scale, const T* minv, T* x, T* r, T* z, T* p, long long total,
+               int C, double* part1) {
     const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
     const long long stride = static_cast<long long>(gridDim.x) * kBlock;
     const int c = static_cast<int>(g0 % C);
     const double sc = scale[c];
     double rz = 0.0;
     for (long long j = g0; j < total; j += stride) {
-        const float rv = static_cast<float>(b[j] * sc);
-        x[j] = 0.0f;
+        const T rv = static_cast<T>(b[j] * sc);
+        x[j] = T(0);
         r[j] = rv;
-        const float zv = minv[j / C] * rv;
+        const T zv = minv[j / C] * rv;
         z[j] = zv;
         p[j] = zv;
-        rz += static_cast<double>(rv) * zv;
+        rz += static_cast<double>(rv) * static_cast<double>(zv);
     }
     cg_block_store(rz, c, C, part1);
 }
 
-// rz = r.z; columns with a zero right-hand side or r.z <= 0 start inactive
+// rz = r.z; a column with a zero right-hand side is converged at once (x = 0),
+// one with r.z <= 0 starts inactive
 __global__ void __launch_bounds__(kBlock)
 cg_start_kernel(double* cg, const double* part1, int nblk, const double* scale, int C) {
     __shared__ double rzs[kMaxCgCols];
@@ -461,8 +479,11 @@ cg_start_kernel(double* cg, const double* part1, int nblk, const double* scale, 
     const int c = threadIdx.x;
     if (c < C) {
         cg[kCgRz + c] = rzs[c];
-        const bool act = scale[c] > 0.0 && rzs[c] > 0.0;
+        const bool zero = !(scale[c] > 0.0);
+        const bool act = !zero && rzs[c] > 0.0;
         cg[kCgActive + c] = act ? 1.0 : 0.0;
+        cg[kCgOk + c] = zero ? 1.0 : 0.0;
+        cg[kCgIters + c] = 0.0;
         cg[kCgAlpha + c] = 0.0;
         cg[kCgBeta + c] = 0.0;
         if (act) atomicAdd(&cnt, 1);
@@ -618,31 +639,43 @@ hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, i
 template hipError_t launch_vdot<float>(const float*, const float*, int, int, double*, int, hipStream_t);
 template hipError_t launch_vdot<double>(const double*, const double*, int, int, double*, int, hipStream_t);
 
-hipError_t launch_cg_dot(const float* a, const float* b, long long total, int C, double* cg, double* part,
-                         int grid, hipStream_t s) {
-    hipLaunchKernelGGL(cg_dot_kernel, dim3(grid), dim3(kBlock), 0, s, a, b, total, C, part);
+template <typename T>
+hipError_t launch_cg_dot(const T* a, const T* b, long long total, int C, double* cg, double* part, int grid,
+                         hipStream_t s) {
+    hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(grid), dim3(kBlock), 0, s, a, b, total, C, part);
     hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, grid, C);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_step(float* x, float* r, float* z, float* p, const float* q, const float* minv,
-                          long long total, int C, double* cg, double* part, double tol2, int grid, hipStream_t s) {
+template <typename T>
+hipError_t launch_cg_step(T* x, T* r, T* z, T* p, const T* q, const T* minv, long long total, int C, double* cg,
+                          double* part, double tol2, int max_iter, int grid, hipStream_t s) {
     double* part2 = part + static_cast<size_t>(grid) * C;
-    hipLaunchKernelGGL(cg_step_kernel, dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, total, C, cg, part,
-                       part2);
-    hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, part2, grid, C, tol2);
-    hipLaunchKernelGGL(cg_dir_kernel, dim3(grid), dim3(kBlock), 0, s, p, z, cg, total, C);
+    hipLaunchKernelGGL((cg_step_kernel<T>), dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, total, C, cg,
+                       part, part2);
+    hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, part2, grid, C, tol2, max_iter);
+    hipLaunchKernelGGL((cg_dir_kernel<T>), dim3(grid), dim3(kBlock), 0, s, p, z, cg, total, C);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_init(const double* b, const double* scale, const float* minv, float* x, float* r,
-                          float* z, float* p, long long total, int C, double* cg, double* part, int grid,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(cg_init_kernel, dim3(grid), dim3(kBlock), 0, s, b, scale, minv, x, r, z, p, total, C,
+template <typename T>
+hipError_t launch_cg_init(const double* b, const double* scale, const T* minv, T* x, T* r, T* z, T* p,
+                          long long total, int C, double* cg, double* part, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((cg_init_kernel<T>), dim3(grid), dim3(kBlock), 0, s, b, scale, minv, x, r, z, p, total, C,
                        part);
     hipLaunchKernelGGL(cg_start_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, grid, scale, C);
     return hipGetLastError();
 }
+
+#define XFE_CG_INST(T)                                                                                         \
+    template hipError_t launch_cg_dot<T>(const T*, const T*, long long, int, double*, double*, int, hipStream_t); \
+    template hipError_t launch_cg_step<T>(T*, T*, T*, T*, const T*, const T*, long long, int, double*, double*,   \
+                                          double, int, int, hipStream_t);                                      \
+    template hipError_t launch_cg_init<T>(const double*, const double*, const T*, T*, T*, T*, T*, long long, int, \
+                                          double*, double*, int, hipStream_t);
+XFE_CG_INST(float)
+XFE_CG_INST(double)
+#undef XFE_CG_INST
 
 hipError_t launch_refine_update(double* beta, const float* x, const double* inv_scale, int N, int C,
                                 int grid, hipStream_t s) {

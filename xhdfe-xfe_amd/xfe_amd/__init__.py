@@ -49,7 +49,9 @@ class Certificate(ctypes.Structure):
 EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop_singletons",
            "xfe_bipartite_components", "xfe_fe_dof", "xfe_gram", "xfe_cluster_meat",
            "xfe_recovery_max_delta", "xfe_last_error", "xfe_profile_enable", "xfe_profile_reset",
-           "xfe_profile_get", "xfe_release_workspace", "xfe_certify")
+           "xfe_profile_get", "xfe_release_workspace", "xfe_certify", "xfe_akm_available",
+           "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
+           "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer")
 
 _lib = None
 
@@ -187,6 +189,67 @@ def certify(y, X, y_t, X_t, gids, G, weights=None, tol=1e-8, tolerance_mode=1, c
         raise XfeError(last_error())
     return dict(abs_residual=out.abs_residual, abs_residual_rel=out.abs_residual_rel,
                 certified=bool(out.precision_certified))
+
+
+class AkmSolver:
+    """xfe_akm_create / _solve_S(_multi) / _destroy (the reference's akm_cuda_*
+    context, include/hdfe/akm_kss_cuda.hpp) from a structure dict with keys
+    N, J, m_w, m_f, m_c, w_ptr, f_ptr, f_matches, Dw, Df, inv_diag."""
+
+    def __init__(self, st):
+        L = lib()
+        L.xfe_akm_create.restype = _p
+        L.xfe_akm_pack_buffer.restype = _p
+        self._keep = {k: np.ascontiguousarray(st[k], dtype=t) for k, t in (
+            ("m_w", np.int32), ("m_f", np.int32), ("m_c", np.float64), ("w_ptr", np.int64),
+            ("f_ptr", np.int64), ("f_matches", np.int64), ("Dw", np.float64), ("Df", np.float64),
+            ("inv_diag", np.float64))}
+        k = self._keep
+        self.J = int(st["J"])
+        self._ctx = L.xfe_akm_create(_i(int(st["N"])), _i(self.J), _ptr(k["m_w"]), _ptr(k["m_f"]),
+                                     _ptr(k["m_c"]), ctypes.c_size_t(k["m_w"].size), _ptr(k["w_ptr"]),
+                                     _ptr(k["f_ptr"]), _ptr(k["f_matches"]), _ptr(k["Dw"]), _ptr(k["Df"]),
+                                     _ptr(k["inv_diag"]))
+        if not self._ctx:
+            raise XfeError(last_error())
+
+    def solve(self, rhs, tol=1e-10, max_iter=1000):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        z = np.empty_like(rhs)
+        it = lib().xfe_akm_solve_S(_p(self._ctx), _ptr(rhs), _ptr(z), _d(tol), _i(max_iter))
+        return z, it
+
+    def solve_multi(self, rhs_lanes, tol=1e-10, max_iter=1000):
+        """rhs_lanes: (nb, J) array, one lane per row; returns (z (nb, J), iterations)."""
+        rhs = np.ascontiguousarray(rhs_lanes, dtype=np.float64)
+        nb = rhs.shape[0]
+        z = np.empty_like(rhs)
+        its = np.empty(nb, dtype=np.int32)
+        rc = lib().xfe_akm_solve_S_multi(_p(self._ctx), _ptr(rhs), _ptr(z), _i(nb), _d(tol), _i(max_iter),
+                                         _ptr(its))
+        if rc != 0:
+            raise XfeError(last_error())
+        return z, its
+
+    def close(self):
+        if self._ctx:
+            lib().xfe_akm_destroy(_p(self._ctx))
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def akm_stable_sort_u64(keys):
+    """xfe_akm_stable_sort_u64: stable ascending order of uint64 keys."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    out = np.empty(keys.size, dtype=np.int64)
+    if not lib().xfe_akm_stable_sort_u64(_ptr(keys), _ptr(out), ctypes.c_size_t(keys.size)):
+        raise XfeError(last_error())
+    return out
 
 
 def drop_singletons(fes, max_rounds=100):
