@@ -49,6 +49,8 @@ typedef struct {
     int precision;              /* 0: FP64 throughout (the plain method); 1: FP32 sweeps under FP64 refinement */
     int certify;                /* 1: compute the normal-equation certificate of the result on the device
                                    (certify_absorption_result, src/fe_absorption.cpp:14942-15190) */
+    double refine_tol;          /* precision 1: FP64 refinement target, ||b - S beta|| / ||b|| of the
+                                   Schur system over the non-primary levels; <= 0 selects 1e-11 */
 } xfe_options;
 
 /* Outputs of one absorption (the fields of hdfe::detail::AbsorptionResult,
@@ -120,6 +122,22 @@ int xfe_absorb_device(double* d_V, int n, int C, int ld,
                       const int* d_group_ids, const int* num_groups, int num_fes,
                       const double* d_weights, const int* sweep_order,
                       const xfe_options* opt, xfe_absorb_result* res);
+
+/* xhdfe::schwarz_demean_raw (include/schwarz_demean.hpp:30, impl
+ * src/schwarz_demean.cpp:419-438; called by absorb_fixed_effects,
+ * src/fe_absorption.cpp:5185-5215): unweighted within transform of Y (n) and
+ * X (n x k, column-major, ld n) over ndim >= 2 dimensions, in place.  fe[d]
+ * holds n ids in [0, nlev[d]).  Converged means, for every column c with
+ * ||D'v_c|| > 1e-300, ||D'(v_c - D u_c)|| / ||D'v_c|| < tol over all
+ * dimensions (the reference's stopping rule, evaluated on the result).
+ * Returns 1 when converged (Y, X overwritten with the residuals), 0 otherwise
+ * (ndim < 2, n <= 0, k < 0, k + 1 > 16, invalid ids, tolerance not met or a
+ * device failure; Y and X are left unchanged).  iters_out (optional)
+ * receives the inner Schur PCG iterations.  threads and collapse are
+ * accepted for the signature and do not change the result. */
+int xfe_schwarz_demean(double* Y, double* X, int64_t n, int k, const int32_t* const* fe,
+                       const int* nlev, int ndim, int threads, double tol, int max_iter,
+                       int* iters_out, int collapse);
 
 /* compute_keep_mask_drop_singletons (src/hdfe_regressor_v11.cpp:4056-4302):
  * iterative singleton drop on the device, bit-exact.  fes: num_fes x n raw

@@ -353,6 +353,57 @@ def akm_solve_S(st, rhs, tol, max_iter):
         rz = rz_new
 
 
+def schwarz_rule(V, Vt, gids, G):
+    """The stopping rule of schwarz_demean_raw evaluated on a result
+    (src/schwarz_demean.cpp:283-305, :358-378): per column c,
+    rel_c = ||D'vt_c|| / ||D'v_c|| over the normal equations of all
+    dimensions, leaving out the pinned levels -- level 0 of every dimension
+    except the one with the most levels (the first such, :65-69; pins at
+    :224-225, :283-284).  Columns with ||D'v_c|| <= 1e-300 are inactive (rel 0).
+    Returns the per-column rel array."""
+    V = np.asarray(V, dtype=np.float64)
+    Vt = np.asarray(Vt, dtype=np.float64)
+    if V.ndim == 1:
+        V, Vt = V[:, None], Vt[:, None]
+    C = V.shape[1]
+    pa = int(np.argmax(np.asarray(G)))
+    bn = np.zeros(C)
+    rn = np.zeros(C)
+    for d, (g, Gd) in enumerate(zip(gids, G)):
+        g = np.asarray(g)
+        for c in range(C):
+            b = np.bincount(g, weights=V[:, c], minlength=Gd)
+            r = np.bincount(g, weights=Vt[:, c], minlength=Gd)
+            if d != pa:
+                b[0] = 0.0
+                r[0] = 0.0
+            bn[c] += b @ b
+            rn[c] += r @ r
+    bn = np.sqrt(bn)
+    return np.where(bn > 1e-300, np.sqrt(rn) / np.where(bn > 1e-300, bn, 1.0), 0.0)
+
+
+def ref_schwarz(y, X, gids, G, tol=1e-10, max_iter=10000):
+    """The reference's schwarz_demean_raw through oracle/_ref (checker):
+    (ok, y_t, X_t, iterations)."""
+    r = ref()
+    if r is None:
+        return None
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
+    n, k = X.shape
+    ids = np.ascontiguousarray(np.stack([np.asarray(g, dtype=np.int32) for g in gids]))
+    lev = np.ascontiguousarray(G, dtype=np.int32)
+    yo = np.empty(n)
+    Xo = np.empty((n, k), order="F")
+    it = _i()
+    rc = r.ref_schwarz(_i(n), _i(k), _i(len(gids)), _ptr(y), _ptr(X), _ptr(ids), _ptr(lev), _d(tol),
+                       _i(max_iter), _ptr(yo), _ptr(Xo), ctypes.byref(it))
+    if rc < 0:
+        raise RuntimeError("ref_schwarz failed")
+    return bool(rc), yo, Xo, it.value
+
+
 def ref_absorb(y, X, fes, tol=1e-8, tolerance_mode=1, threads=1, symmetric=False):
     r = ref()
     if r is None:

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "fe_absorption.hpp"
+#include "schwarz_demean.hpp"
 #include "hdfe/hdfe_regressor.hpp"
 #include "hdfe/hdfe_regressor_v11.hpp"
 
@@ -168,6 +169,28 @@ int ref_certify(int n, int k, int d, const double* y, const double* X, const int
         return 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "ref_certify: %s\n", e.what());
+        return -1;
+    }
+}
+
+// xhdfe::schwarz_demean_raw (src/schwarz_demean.cpp:419): PCG within transform.
+// Returns 1 (converged; y_out/X_out hold the residuals), 0 (not converged or
+// unsupported), -1 on exception.
+int ref_schwarz(int n, int k, int d, const double* y, const double* X, const int* fes, const int* nlev,
+                double tol, int max_iter, double* y_out, double* X_out, int* iters) {
+    try {
+        std::memcpy(y_out, y, sizeof(double) * n);
+        if (k > 0) std::memcpy(X_out, X, sizeof(double) * static_cast<std::size_t>(n) * k);
+        std::vector<const int32_t*> fp;
+        std::vector<int> lev;
+        for (int j = 0; j < d; ++j) {
+            fp.push_back(reinterpret_cast<const int32_t*>(fes + static_cast<std::size_t>(j) * n));
+            lev.push_back(nlev[j]);
+        }
+        const bool ok = xhdfe::schwarz_demean_raw(y_out, X_out, n, k, fp, lev, 1, tol, max_iter, iters, -1);
+        return ok ? 1 : 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_schwarz: %s\n", e.what());
         return -1;
     }
 }

@@ -42,3 +42,36 @@ def input_checksum(p):
 def certificate_weights(n):
     """Deterministic row weights for the weighted certificate fixtures."""
     return 0.5 + (np.arange(n) % 7) / 4.0
+
+
+# Edge cases of schwarz_demean_raw (src/schwarz_demean.cpp:290-438), as
+# ((y, X, ids, levels), max_iter) on one seeded 2000-row panel:
+# 0 the plain two-way case, 1 an empty level in the first dimension, 2 an
+# empty level in a third dimension, 3 one dimension only, 4 all-zero columns,
+# 5 one zero column among non-zero ones, 6 max_iter 0, 7 k = 15, 8 k = 16.
+SCHWARZ_EDGE_NAMES = ("two_way", "empty_level_dim0", "empty_level_extra", "one_dim", "zero_columns",
+                      "one_zero_column", "max_iter_0", "k15", "k16")
+
+
+def schwarz_edge_cases():
+    rng = np.random.default_rng(20261203)
+    n = 2000
+    a = rng.integers(0, 100, n).astype(np.int32)
+    b = rng.integers(0, 30, n).astype(np.int32)
+    c = rng.integers(0, 7, n).astype(np.int32)
+    y = rng.normal(size=n)
+    X = np.asfortranarray(rng.normal(size=(n, 2)))
+    X1 = np.asfortranarray(np.column_stack([X[:, 0], np.zeros(n)]))
+    X15 = np.asfortranarray(rng.normal(size=(n, 15)))
+    X16 = np.asfortranarray(rng.normal(size=(n, 16)))
+    return [
+        ((y, X, [a, b], [100, 30]), 10000),
+        ((y, X, [a, b], [101, 30]), 10000),
+        ((y, X, [a, b, c], [100, 30, 8]), 10000),
+        ((y, X, [a], [100]), 10000),
+        ((np.zeros(n), np.zeros((n, 2), order="F"), [a, b], [100, 30]), 10000),
+        ((y, X1, [a, b], [100, 30]), 10000),
+        ((y, X, [a, b], [100, 30]), 0),
+        ((y, X15, [a, b], [100, 30]), 10000),
+        ((y, X16, [a, b], [100, 30]), 10000),
+    ]

@@ -19,7 +19,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "xhdfe-xfe_amd"))
 
 import oracle  # noqa: E402
-from golden_cases import CASES, certificate_weights, input_checksum  # noqa: E402
+from golden_cases import CASES, certificate_weights, input_checksum, schwarz_edge_cases  # noqa: E402
 
 
 def main():
@@ -55,6 +55,14 @@ def main():
             c = oracle.ref_certify(yk, Xk, yo, Xo, fk, weights=w, tol=tol, converged=conv)
             out[f"{name}/{tag}/cert_w"] = np.array([c["abs_residual"], c["abs_residual_rel"],
                                                     float(c["certified"])])
+        # schwarz_demean_raw (the PCG demeaner) on the kept rows with compact ids
+        ck = [oracle.build_indexer(f) for f in fk]
+        for tag, tol in (("tight", 1e-12), ("default", 1e-8)):
+            ok, yo, Xo, it = oracle.ref_schwarz(yk, Xk, [c[0] for c in ck], [c[1] for c in ck], tol=tol)
+            out[f"{name}/schwarz_{tag}/y_tilde"] = yo
+            out[f"{name}/schwarz_{tag}/X_tilde"] = Xo
+            out[f"{name}/schwarz_{tag}/iterations"] = np.int64(it)
+            out[f"{name}/schwarz_{tag}/ok"] = np.int64(ok)
         # certificate of the untransformed columns (large residual, not certified)
         c = oracle.ref_certify(yk, Xk, yk, Xk, fk, tol=1e-8, converged=True)
         out[f"{name}/raw/cert"] = np.array([c["abs_residual"], c["abs_residual_rel"], float(c["certified"])])
@@ -63,6 +71,10 @@ def main():
             out[f"{name}/fit/{key}"] = np.asarray(f[key])
         for key in ("df_a", "df_resid", "num_singletons", "nobs", "rss"):
             out[f"{name}/fit/{key}"] = np.asarray(f[key])
+    # schwarz_demean_raw's gates and edge cases on the two-way panel: the
+    # returned flag and iteration count per case (order of SCHWARZ_EDGE_CASES)
+    out["schwarz_edges"] = np.array([list(oracle.ref_schwarz(*args, tol=1e-10, max_iter=mi)[0::3])
+                                     for args, mi in schwarz_edge_cases()], dtype=np.int64)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_cases.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

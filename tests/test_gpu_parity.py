@@ -431,3 +431,74 @@ def test_akm_create_rejects_bad_input():
     bad["m_c"][0] = 1.5                # counts must be integers
     with pytest.raises(xfe_amd.XfeError):
         xfe_amd.AkmSolver(bad)
+
+
+# ------------------------------------------------------------ schwarz_demean_raw
+# xfe_schwarz_demean returns the projection (mixed Schur PCG under FP64
+# refinement) once the reference's stopping rule, evaluated on the result,
+# holds for every column; it is compared with the oracle at tol 1e-14 and with
+# the reference's own output (golden fixture).
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+@pytest.mark.parametrize("tag", ["tight", "default"])
+def test_schwarz_matches_reference(golden, cases, name, tag):
+    _log(f"schwarz {name} {tag}")
+    _, fes, gids, G, V = _prep(cases[name])
+    tol = 1e-12 if tag == "tight" else 1e-8
+    y = V[:, 0].copy()
+    X = np.asfortranarray(V[:, 1:])
+    ok, it = xfe_amd.schwarz_demean(y, X, gids, G, tol=tol)
+    assert ok and it > 0, xfe_amd.last_error()
+    Vg = np.column_stack([y, X])
+    assert oracle.schwarz_rule(V, Vg, gids, G).max() < tol
+    scale = np.abs(V).max()
+    Vo, _ = oracle.absorb(V, gids, G, tol=1e-14)
+    Vr = np.column_stack([golden[f"{name}/schwarz_{tag}/y_tilde"], golden[f"{name}/schwarz_{tag}/X_tilde"]])
+    err_o = np.abs(Vg - Vo).max() / scale
+    err_r = np.abs(Vg - Vr).max() / scale
+    _log(f"  iterations {it} vs oracle {err_o:.2e} vs reference {err_r:.2e}")
+    # the rule bounds the normal-equation residual; the error itself may be
+    # larger by the conditioning of the panel (the module's 1e-10 at tol 1e-12)
+    assert err_o <= (1e-10 if tag == "tight" else 1e-8)
+    assert err_r <= (1e-10 if tag == "tight" else 1e-7)
+
+
+def test_schwarz_tight_tolerance(cases):
+    """A tolerance below the default refinement target takes the FP64 polish sweeps."""
+    _log("schwarz tol 1e-13")
+    _, fes, gids, G, V = _prep(cases["akm_three_way"])
+    y = V[:, 0].copy()
+    X = np.asfortranarray(V[:, 1:])
+    ok, _ = xfe_amd.schwarz_demean(y, X, gids, G, tol=1e-13)
+    assert ok, xfe_amd.last_error()
+    assert oracle.schwarz_rule(V, np.column_stack([y, X]), gids, G).max() < 1e-13
+
+
+def test_schwarz_edge_cases(golden):
+    """The reference's gates and edge cases (golden flags, same inputs).  Where
+    the reference fails on an empty level of a third dimension (its
+    preconditioner divides by the level's zero count) this path returns the
+    projection; iteration counts are this solver's own."""
+    from golden_cases import SCHWARZ_EDGE_NAMES, schwarz_edge_cases
+    ref_flags = golden["schwarz_edges"]
+    for j, ((y0, X0, ids, lev), mi) in enumerate(schwarz_edge_cases()):
+        name = SCHWARZ_EDGE_NAMES[j]
+        _log(f"schwarz edge {name}")
+        y = y0.copy()
+        X = np.asfortranarray(X0.copy())
+        ok, it = xfe_amd.schwarz_demean(y, X, ids, lev, tol=1e-10, max_iter=mi)
+        want = True if name == "empty_level_extra" else bool(ref_flags[j][0])
+        assert ok == want, (name, xfe_amd.last_error())
+        if not ok:
+            assert np.array_equal(y, y0) and np.array_equal(X, X0), name
+            continue
+        V = np.column_stack([y0, X0])
+        Vg = np.column_stack([y, X])
+        if name == "zero_columns":
+            assert it == 0 and np.array_equal(Vg, V)
+            continue
+        assert oracle.schwarz_rule(V, Vg, ids, lev).max() < 1e-10, name
+        Vo, _ = oracle.absorb(V, ids, lev, tol=1e-14)
+        assert np.abs(Vg - Vo).max() / np.abs(V).max() <= 1e-9, name
+        if name == "one_zero_column":
+            assert np.array_equal(X[:, 1], X0[:, 1])

@@ -30,7 +30,8 @@ class FeInput(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("tol", _d), ("tolerance_mode", _i), ("max_iter", _i), ("method", _i),
-                ("force_cg", _i), ("precision", _i), ("certify", _i)]
+                ("force_cg", _i), ("precision", _i), ("certify", _i),
+                ("refine_tol", _d)]
 
 
 class AbsorbResult(ctypes.Structure):
@@ -51,7 +52,7 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_recovery_max_delta", "xfe_last_error", "xfe_profile_enable", "xfe_profile_reset",
            "xfe_profile_get", "xfe_release_workspace", "xfe_certify", "xfe_akm_available",
            "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
-           "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer")
+           "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer", "xfe_schwarz_demean")
 
 _lib = None
 
@@ -96,9 +97,9 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(_p)
 
 
-def _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify=True):
+def _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify=True, refine_tol=0.0):
     return Options(float(tol), int(tolerance_mode), int(max_iter), int(method), int(bool(force_cg)),
-                   int(precision), int(bool(certify)))
+                   int(precision), int(bool(certify)), float(refine_tol))
 
 
 def _info(r, D):
@@ -112,7 +113,8 @@ def _info(r, D):
 
 
 def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=1e-8,
-           tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0, certify=True):
+           tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0, certify=True,
+           refine_tol=0.0):
     """absorb_fixed_effects_cuda on host arrays: returns (y_tilde, X_tilde, info).
 
     gids: compact 0-based level ids per dimension; G: level counts.  ``method``
@@ -134,7 +136,7 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
     order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
     if method is None:
         method = 2 if D == 2 else 1
-    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify)
+    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify, refine_tol)
     r = AbsorbResult()
     yo = np.empty(n)
     Xo = np.empty((n, k), order="F")
@@ -189,6 +191,35 @@ def certify(y, X, y_t, X_t, gids, G, weights=None, tol=1e-8, tolerance_mode=1, c
         raise XfeError(last_error())
     return dict(abs_residual=out.abs_residual, abs_residual_rel=out.abs_residual_rel,
                 certified=bool(out.precision_certified))
+
+
+def schwarz_demean(Y, X, fe, nlev, tol=1e-8, max_iter=10000, threads=1, collapse=-1):
+    """xhdfe::schwarz_demean_raw (src/schwarz_demean.cpp:419-438) through
+    xfe_schwarz_demean: demeans Y (n) and X (n x k, column-major) in place
+    when it converges.  fe: one int32 id array per dimension, ids in
+    [0, nlev[d]).  Returns (converged, iterations), like the reference's bool
+    and iters_out."""
+    Y = np.asarray(Y)
+    if Y.dtype != np.float64 or not Y.flags.c_contiguous:
+        raise XfeError("Y must be a contiguous float64 array")
+    n = Y.size
+    if X is None:
+        k = 0
+        Xp = None
+    else:
+        if X.dtype != np.float64 or not X.flags.f_contiguous or X.shape[0] != n:
+            raise XfeError("X must be a column-major float64 (n, k) array")
+        k = X.shape[1] if X.ndim == 2 else 1
+        Xp = _ptr(X)
+    ids = [np.ascontiguousarray(f, dtype=np.int32) for f in fe]
+    ptrs = (_p * max(len(ids), 1))(*[a.ctypes.data for a in ids])
+    lev = np.ascontiguousarray(nlev, dtype=np.int32)
+    it = _i(0)
+    L = lib()
+    L.xfe_schwarz_demean.argtypes = [_p, _p, ctypes.c_int64, _i, _p, _p, _i, _i, _d, _i, _p, _i]
+    ok = L.xfe_schwarz_demean(_ptr(Y), Xp, n, k, ptrs, _ptr(lev), len(ids), int(threads), float(tol),
+                              int(max_iter), ctypes.byref(it), int(collapse))
+    return bool(ok), it.value
 
 
 class AkmSolver:
