@@ -97,6 +97,95 @@ def cpu_baseline(n_full, C, threads):
                 obs_cols_per_s=rate)
 
 
+AKM_METRIC = "s per 256-RHS grounded firm-Laplacian PCG solve (KSS JLA draws), 10M-obs KSS panel"
+
+
+def akm_cpu_baseline(st, gpu_iters, lanes):
+    """The oracle's single-lane Jacobi-PCG (numpy) on the same structure for a
+    bounded number of iterations; value = seconds for all lanes at the GPU's
+    iteration counts and the sample's seconds per lane-iteration."""
+    import oracle
+    rng = np.random.default_rng(1)
+    rhs = rng.choice([-1.0, 1.0], size=st["J"])
+    t0 = time.perf_counter()
+    _, its = oracle.akm_solve_S(st, rhs, 1e-10, 60)
+    dt = time.perf_counter() - t0
+    per = dt / 60.0
+    return dict(value=round(per * float(np.sum(gpu_iters)), 2), unit="s", cores=1, kind="port",
+                sample=(f"oracle single-lane Jacobi-PCG (numpy bincount), 60 iterations on the full structure "
+                        f"in {dt:.2f} s; value = {per * 1e3:.1f} ms per lane-iteration x the GPU's "
+                        f"{int(np.sum(gpu_iters))} lane-iterations over {lanes} lanes"))
+
+
+def bench_akm(args):
+    """The leverage solver (akm_cuda_solve_S_multi drop-in): 256 Rademacher
+    right-hand sides, as in the JLA draws, solved at cg_tol 1e-10 in batches of
+    32 lanes on the largest mobility group of the KSS-shape panel."""
+    import torch  # noqa: F401  (shares torch's HIP runtime)
+
+    import xfe_amd
+    from xfe_amd import akm, panels
+
+    if not xfe_amd.backend_available():
+        raise SystemExit("no HIP device")
+    t0 = time.perf_counter()
+    p = panels.kss_panel()
+    n = p["worker"].size
+    mw, mf, mc, N, J = akm.panel_matches(p["worker"], p["firm"])
+    st = akm.build_structure(mw, mf, mc, N, J)
+    M = mw.size
+    log(f"KSS panel {n} obs -> {N} workers, {J} firms, {M} matches in {time.perf_counter() - t0:.1f} s")
+    solver = xfe_amd.AkmSolver(st)
+    lanes, batch = 256, 32
+    rng = np.random.default_rng(20261503)
+    rhs = rng.choice([-1.0, 1.0], size=(lanes, J))
+    max_iter = 20000
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        _, its = solver.solve_multi(rhs[:batch], tol=1e-10, max_iter=max_iter)
+        log(f"warmup {w}: {time.perf_counter() - t0:.3f} s, iterations {its.min()}..{its.max()}")
+    xfe_amd.profile_reset()
+    xfe_amd.profile_enable(True)
+    t0 = time.perf_counter()
+    all_its = []
+    for k in range(args.steps):
+        for b0 in range(0, lanes, batch):
+            _, its = solver.solve_multi(rhs[b0:b0 + batch], tol=1e-10, max_iter=max_iter)
+            if (its < 0).any():
+                raise SystemExit(f"lanes did not converge: {its}")
+            all_its.append(its)
+        log(f"step {k}: {time.perf_counter() - t0:.3f} s so far")
+    sec = (time.perf_counter() - t0) / args.steps
+    xfe_amd.profile_enable(False)
+    pass_ms, launches, pass_bytes = xfe_amd.profile_get()
+    solver.close()
+    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
+    its = np.concatenate(all_its[:lanes // batch])
+    log(f"{sec:.3f} s per {lanes} lanes; seg_firm: {launches} launches, "
+        f"{pass_ms / max(launches, 1):.4f} ms avg, {achieved:.0f} GB/s algorithmic")
+    cpu = None
+    if not args.no_cpu:
+        cpu = akm_cpu_baseline(st, its, lanes)
+        log(f"cpu baseline {cpu['value']} s (extrapolated)")
+    out = {
+        "metric": AKM_METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "method": "FP64 Jacobi-PCG over the match segments, 32 lanes per launch, per-lane retire",
+        "config": {"workload": "kss_leverage_256rhs", "obs": int(n), "workers": N, "firms": J, "matches": int(M),
+                   "lanes": lanes, "lanes_per_launch": batch, "cg_tol": 1e-10},
+        "iterations": {"min": int(its.min()), "max": int(its.max()), "mean": round(float(its.mean()), 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(os.path.join(ROOT, "profiles", "akm_seg_firm_kss10m.md")),
+                     "kernel": "seg_firm_kernel<double> (firm-major match pass of S)",
+                     "avg_launch_ms": round(pass_ms / max(launches, 1), 4),
+                     "bytes_per_launch": round(pass_bytes / max(launches, 1), 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,11 +195,16 @@ def main():
     ap.add_argument("--precision", type=int, default=1, help="1 FP32 under FP64 refinement, 0 FP64")
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain FP64 absorber")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--path", choices=("absorb", "akm"), default="absorb",
+                    help="absorb: the 56M-obs absorber (default); akm: the KSS leverage solver")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               f"(the absorber runs on a single device)", file=sys.stderr)
         sys.exit(2)
+    if args.path == "akm":
+        bench_akm(args)
+        return
 
     import torch
 

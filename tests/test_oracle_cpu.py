@@ -433,3 +433,18 @@ def test_schwarz_edge_fixture_regenerates(golden):
     from golden_cases import schwarz_edge_cases
     got = [list(oracle.ref_schwarz(*args, tol=1e-10, max_iter=mi)[0::3]) for args, mi in schwarz_edge_cases()]
     np.testing.assert_array_equal(np.array(got, dtype=np.int64), golden["schwarz_edges"])
+
+
+def test_akm_host_structure_matches_oracle():
+    """The package's TwoWaySolver::build restatement (what the device solver is
+    handed) equals the oracle's on a random match list."""
+    from xfe_amd import akm
+    rng = np.random.default_rng(5)
+    worker = rng.integers(0, 300, 4000)
+    firm = rng.integers(0, 40, 4000)
+    mw, mf, mc, N, J = akm.panel_matches(worker, firm, connected=False)
+    assert np.all(np.diff(mw) >= 0) and mc.sum() == 4000
+    a = akm.build_structure(mw, mf, mc, N, J)
+    b = oracle.akm_build(mw, mf, mc, N, J)
+    for k in b:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)

@@ -34,6 +34,8 @@
 
 namespace xfe {
 
+Profile g_profile;
+
 namespace {
 
 constexpr int kMaxGrid = 2048;
@@ -45,14 +47,6 @@ int grid_for(long long work) {
     if (g > kMaxGrid) g = kMaxGrid;
     return static_cast<int>(g);
 }
-
-struct Profile {
-    bool enabled = false;
-    double ms = 0.0;
-    long long launches = 0;
-    double bytes = 0.0;
-};
-Profile g_profile;
 
 // Per-thread device workspace (mirrors the reference's thread-local
 // persistent CUDA workspace, src/fe_absorption_cuda.cu:1536).

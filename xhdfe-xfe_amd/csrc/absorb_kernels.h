@@ -244,6 +244,16 @@ hipError_t launch_gather_vec(const double* src, const int* perm, double* dst, in
                              hipStream_t s);
 hipError_t launch_sorted_check(const int* gid, int n, int* flag, int grid, hipStream_t s);
 
+// Device time and algorithmic bytes of the dominant kernel of the last solves
+// (xfe_profile_*): HIP events around each launch when enabled.
+struct Profile {
+    bool enabled = false;
+    double ms = 0.0;
+    long long launches = 0;
+    double bytes = 0.0;
+};
+extern Profile g_profile;
+
 }  // namespace xfe
 
 #endif

@@ -5,14 +5,14 @@
 // akm_cuda_* functions (include/hdfe/akm_kss_cuda.hpp:19-74, impl
 // src/akm_kss_cuda.cu:164-722).
 //
-// S is the two-dimensional case of the segment operator of the mixed-precision
-// absorber: a match (worker w, firm f, count c) is a segment of length c with
-// no year range, so  S p = Df p - B'(Dw^{-1} B p)  is one worker-major pass
-// (t_w = Dw^{-1} sum_a c_a p[f_a]) and one firm-major pass (sum over the
-// firm's matches of c_a (p[f] - t_w(a)) with the atomic-free hierarchical
-// reduction), in FP64, for all lanes of a chunk together (level-major J x nb
-// vectors: one gather reads every lane of a firm).  The ground firm
-// (J - 1) is pinned to zero as in the reference.
+// S p = Df p - B'(Dw^{-1} B p) is applied to up to 32 lanes at once in FP64
+// (level-major J x nb vectors): one pass over the worker-major matches forms
+// the worker means t_w = Dw^{-1} sum_a c_a p[f_a], one pass over the
+// firm-major matches sums c_a (p[f] - t_w(a)) per firm with a deterministic
+// hierarchical reduction of the runs that cross piece boundaries, and a last
+// kernel adds (Df - deg) p and pins the ground firm (J - 1) to zero as in the
+// reference.  Lanes are spread over threads (32 per worker or piece), so every
+// gather reads a worker's or a firm's lanes as one contiguous block.
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -76,6 +76,93 @@ __global__ void __launch_bounds__(kBlock) akm_iota64(int64_t* v, size_t n) {
         v[i] = static_cast<int64_t>(i);
 }
 
+// S p = Df p - B'(Dw^{-1} B p) for nb <= 32 lanes at once, lane-parallel: a
+// group of 32 threads takes one worker (or one piece of firm-major matches)
+// and thread c holds lane c, so every gather of a worker's or firm's lanes is
+// one contiguous 8 nb-byte read.
+constexpr int kLaneGroup = 32;
+constexpr int kGroups = kBlock / kLaneGroup;  // lane groups per block
+constexpr int kPiece = 64;                    // firm-major matches (or list entries) per group
+
+// t[w * nb + c] = (1 / Dw[w]) sum over the worker's matches a of c_a p[f_a * nb + c]
+__global__ void __launch_bounds__(kBlock)
+akm_wmean_kernel(const int* wseg, const int* mf, const int* len, const double* inv_dw, const double* p, int N,
+                 int nb, double* t) {
+    const int c = threadIdx.x % kLaneGroup;
+    const int stride = gridDim.x * kGroups;
+    for (int w = blockIdx.x * kGroups + threadIdx.x / kLaneGroup; w < N; w += stride) {
+        const int a0 = wseg[w], a1 = wseg[w + 1];
+        double acc = 0.0;
+        if (c < nb) {
+            for (int a = a0; a < a1; ++a)
+                acc += static_cast<double>(len[a]) * p[static_cast<size_t>(mf[a]) * nb + c];
+            t[static_cast<size_t>(w) * nb + c] = acc * inv_dw[w];
+        }
+    }
+}
+
+// Runs of equal key over pieces of kPiece elements, one lane group per piece.
+// A run strictly inside its piece is complete and stored in out[key]; the
+// first and the last run of a piece go to the boundary list as entries 2q and
+// 2q + 1 (a piece with one run adds (key, 0) as its second entry), so every
+// key's total is either stored once or split over boundary entries only, and
+// the list stays sorted by key.  With final (the input is one piece) every run
+// is stored.  FIRM: elements are firm-major matches, value c_a (p[f] - t[w_a]);
+// otherwise elements are list entries (key, value).
+template <bool FIRM>
+__global__ void __launch_bounds__(kBlock)
+akm_runs_kernel(const int* key_in, const int* fw, const int* flen, const double* p, const double* vin, int m, int nb,
+                double* out, int* kout, double* vout, int final) {
+    const int c = threadIdx.x % kLaneGroup;
+    const int stride = gridDim.x * kGroups;
+    const int pieces = (m + kPiece - 1) / kPiece;
+    for (int q = blockIdx.x * kGroups + threadIdx.x / kLaneGroup; q < pieces; q += stride) {
+        const int j0 = q * kPiece;
+        const int j1 = min(m, j0 + kPiece);
+        int key = key_in[j0];
+        double acc = 0.0;
+        bool first = true;
+        for (int j = j0; j < j1; ++j) {
+            const int k = key_in[j];
+            if (k != key) {
+                if (c < nb) {
+                    if (first && !final) vout[static_cast<size_t>(2 * q) * nb + c] = acc;
+                    else out[static_cast<size_t>(key) * nb + c] = acc;
+                }
+                if (first && !final && c == 0) kout[2 * q] = key;
+                first = false;
+                key = k;
+                acc = 0.0;
+            }
+            if (c < nb) {
+                if constexpr (FIRM) {
+                    acc += static_cast<double>(flen[j]) *
+                           (p[static_cast<size_t>(k) * nb + c] - vin[static_cast<size_t>(fw[j]) * nb + c]);
+                } else {
+                    acc += vin[static_cast<size_t>(j) * nb + c];
+                }
+            }
+        }
+        if (final) {
+            if (c < nb) out[static_cast<size_t>(key) * nb + c] = acc;
+        } else {
+            const size_t e0 = static_cast<size_t>(2 * q) * nb, e1 = static_cast<size_t>(2 * q + 1) * nb;
+            if (c < nb) {
+                if (first) {
+                    vout[e0 + c] = acc;
+                    vout[e1 + c] = 0.0;
+                } else {
+                    vout[e1 + c] = acc;
+                }
+            }
+            if (c == 0) {
+                if (first) kout[2 * q] = key;
+                kout[2 * q + 1] = key;
+            }
+        }
+    }
+}
+
 constexpr int kAkmMaxGrid = 2048;
 
 int akm_grid(long long work) {
@@ -103,37 +190,50 @@ struct xfe_akm_context {
     double* h_pack = nullptr;   // pinned J x lanes staging
     int h_pack_lanes = 0;
     double* h_small = nullptr;  // pinned scalars
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // kernel timing (xfe_profile_*)
 
     ~xfe_akm_context() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
         if (h_pack) (void)hipHostFree(h_pack);
         if (h_small) (void)hipHostFree(h_small);
         if (s) (void)hipStreamDestroy(s);
     }
 
-    // out = S p for level-major J x nb vectors (ground pinned)
+    // out = S p for level-major J x nb vectors (ground pinned): the worker
+    // means, the firm-major runs with the boundary list reduced level by level,
+    // then the degree correction and the ground pin.
     void apply(const double* pv, double* out, int nb) {
-        for (int c0 = 0; c0 < nb; c0 += 8) {
-            const int cc = std::min(8, nb - c0);
-            SegArgs<double> a;
-            a.C = nb; a.c0 = c0; a.Nw = N; a.M = M; a.G2 = 0;
-            a.x1 = pv; a.x2 = nullptr; a.inv_w = inv_dw.p;
-            a.wseg = wseg.p; a.seg_r1 = r1.p; a.seg_len = len.p; a.seg_a = nullptr;
-            a.wmean = wmean.p;
-            a.f_key = fkey.p; a.f_w = fw.p; a.f_len = flen.p; a.f_a = nullptr;
-            a.out1 = out; a.hpart = nullptr;
-            a.bkey = bk[0].p; a.bval = bv[0].p;
-            XFE_CHECK(launch_seg_worker<double>(a, cc, false, akm_grid(N), s));
-            XFE_CHECK(launch_seg_firm<double>(a, cc, false, false, akm_grid(M), s));
-            int m = static_cast<int>(2 * ((static_cast<long long>(M) + kWave - 1) / kWave));
-            int cur = 0;
-            while (m > kBndTail) {
-                XFE_CHECK(launch_seg_bnd<double>(bk[cur].p, bv[cur].p, m, out, nb, c0, cc, bk[1 - cur].p,
-                                                 bv[1 - cur].p, 0, akm_grid(m), s));
-                m = static_cast<int>(2 * ((static_cast<long long>(m) + kWave - 1) / kWave));
-                cur = 1 - cur;
-            }
-            XFE_CHECK(launch_seg_bnd_tail<double>(bk[cur].p, bv[cur].p, bk[1 - cur].p, bv[1 - cur].p, m, out, nb,
-                                                  c0, cc, s));
+        hipLaunchKernelGGL(akm_wmean_kernel, dim3(akm_grid(static_cast<long long>(N) * kLaneGroup)), dim3(kBlock), 0,
+                           s, wseg.p, r1.p, len.p, inv_dw.p, pv, N, nb, wmean.p);
+        XFE_CHECK(hipGetLastError());
+        int m = M;
+        int pieces = (m + kPiece - 1) / kPiece;
+        if (g_profile.enabled) XFE_CHECK(hipEventRecord(ev0, s));
+        hipLaunchKernelGGL(akm_runs_kernel<true>, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
+                           dim3(kBlock), 0, s, fkey.p, fw.p, flen.p, pv, wmean.p, m, nb, out, bk[0].p, bv[0].p,
+                           pieces == 1 ? 1 : 0);
+        XFE_CHECK(hipGetLastError());
+        if (g_profile.enabled) {
+            // the firm pass alone: match records (firm key, worker, count), the worker's
+            // nb lanes per match, the firm's lanes read and written once
+            XFE_CHECK(hipEventRecord(ev1, s));
+            XFE_CHECK(hipEventSynchronize(ev1));
+            float ms = 0.f;
+            XFE_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            g_profile.ms += ms;
+            g_profile.launches += 1;
+            g_profile.bytes += static_cast<double>(M) * (12.0 + 8.0 * nb) + 16.0 * nb * J;
+        }
+        int cur = 0;
+        while (pieces > 1) {
+            m = 2 * pieces;
+            pieces = (m + kPiece - 1) / kPiece;
+            hipLaunchKernelGGL(akm_runs_kernel<false>, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
+                               dim3(kBlock), 0, s, bk[cur].p, nullptr, nullptr, nullptr, bv[cur].p, m, nb, out,
+                               bk[1 - cur].p, bv[1 - cur].p, pieces == 1 ? 1 : 0);
+            XFE_CHECK(hipGetLastError());
+            cur = 1 - cur;
         }
         hipLaunchKernelGGL(akm_finish, dim3(akm_grid(static_cast<long long>(J) * nb)), dim3(kBlock), 0, s, out, pv,
                            corr.p, has.p, J, nb, ground);
@@ -304,6 +404,8 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
         c = new xfe_akm_context();
         c->N = N; c->J = J; c->M = M; c->ground = J - 1;
         XFE_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        XFE_CHECK(hipEventCreate(&c->ev0));
+        XFE_CHECK(hipEventCreate(&c->ev1));
         auto up_i = [&](DevBuf<int>& d, const std::vector<int>& h) {
             d.ensure(h.size());
             XFE_CHECK(hipMemcpyAsync(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, c->s));
@@ -323,11 +425,11 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
         up_d(c->inv_dw, inv_dw.data(), N);
         up_d(c->inv_diag, inv_diagS, J);
         up_d(c->corr, corr.data(), J);
-        c->wmean.ensure(static_cast<size_t>(N) * 8);
-        const size_t nb2 = 2 * ((static_cast<size_t>(M) + kWave - 1) / kWave);
+        c->wmean.ensure(static_cast<size_t>(N) * kMaxCgCols);
+        const size_t nb2 = 2 * ((static_cast<size_t>(M) + kPiece - 1) / kPiece);
         for (int k = 0; k < 2; ++k) {
             c->bk[k].ensure(nb2);
-            c->bv[k].ensure(nb2 * 8);
+            c->bv[k].ensure(nb2 * kMaxCgCols);
         }
         c->cg.ensure(kCgSlots);
         XFE_CHECK(hipMemsetAsync(c->cg.p, 0, sizeof(double) * kCgSlots, c->s));

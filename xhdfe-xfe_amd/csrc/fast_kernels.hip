@@ -308,15 +308,20 @@ __device__ __forceinline__ void cg_block_store(double v, int c, int C, double* p
     __syncthreads();
 }
 
-// out[c] = sum over nblk blocks of part[b * C + c], c < C (one block of kBlock):
-// T = floor(kBlock / C) * C threads read the array linearly with stride T, so
+// Threads of the single-block scalar kernels: the partials (up to 2048 blocks
+// x 32 columns per array) are read by 16 waves, each thread with 8 loads in
+// flight (a 256-thread block is latency-bound on them).
+constexpr int kRed = 1024;
+
+// out[c] = sum over nblk blocks of part[b * C + c], c < C (one block of kRed):
+// T = floor(kRed / C) * C threads read the array linearly with stride T, so
 // thread t always meets column t % C, with 8 independent accumulators to keep
 // 8 loads in flight; then one thread per column adds the T / C thread sums.
 // Two arrays (part2 may be null) are reduced in the same pass.
 __device__ __forceinline__ void cg_reduce_parts(const double* part, const double* part2, int nblk, int C,
                                                 double* out, double* out2) {
-    __shared__ double red[2][kBlock];
-    const int T = (kBlock / C) * C;
+    __shared__ double red[2][kRed];
+    const int T = (kRed / C) * C;
     const int total = nblk * C;
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double b[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -364,7 +369,7 @@ cg_dot_kernel(const T* a, const T* b, long long total, int C, double* part1) {
 
 // alpha = rz / pq for active columns; a column with pq not in (0, 1e300) or
 // rz <= 0 stops (without converging)
-__global__ void __launch_bounds__(kBlock) cg_alpha_kernel(double* cg, const double* part1, int nblk, int C) {
+__global__ void __launch_bounds__(kRed) cg_alpha_kernel(double* cg, const double* part1, int nblk, int C) {
     __shared__ double pqs[kMaxCgCols];
     cg_reduce_parts(part1, nullptr, nblk, C, pqs, nullptr);
     const int c = threadIdx.x;
@@ -404,7 +409,7 @@ cg_step_kernel(T* x, T* r, T* z, const T* p, const T* qv, const T* minv, long lo
 
 // per active column: one more iteration; r.r <= tol2 converges it, reaching
 // max_iter stops it; otherwise beta = rz_new / rz.  cg[kCgCount] = active columns
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kRed)
 cg_beta_kernel(double* cg, const double* part1, const double* part2, int nblk, int C, double tol2,
                int max_iter) {
     __shared__ double rrs[kMaxCgCols], rzs[kMaxCgCols];
@@ -469,7 +474,7 @@ cg_init_kernel(const double* b, const double* scale, const T* minv, T* x, T* r, 
 
 // rz = r.z; a column with a zero right-hand side is converged at once (x = 0),
 // one with r.z <= 0 starts inactive
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kRed)
 cg_start_kernel(double* cg, const double* part1, int nblk, const double* scale, int C) {
     __shared__ double rzs[kMaxCgCols];
     __shared__ int cnt;
@@ -643,7 +648,7 @@ template <typename T>
 hipError_t launch_cg_dot(const T* a, const T* b, long long total, int C, double* cg, double* part, int grid,
                          hipStream_t s) {
     hipLaunchKernelGGL((cg_dot_kernel<T>), dim3(grid), dim3(kBlock), 0, s, a, b, total, C, part);
-    hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, grid, C);
+    hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(kRed), 0, s, cg, part, grid, C);
     return hipGetLastError();
 }
 
@@ -653,7 +658,7 @@ hipError_t launch_cg_step(T* x, T* r, T* z, T* p, const T* q, const T* minv, lon
     double* part2 = part + static_cast<size_t>(grid) * C;
     hipLaunchKernelGGL((cg_step_kernel<T>), dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, total, C, cg,
                        part, part2);
-    hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, part2, grid, C, tol2, max_iter);
+    hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(kRed), 0, s, cg, part, part2, grid, C, tol2, max_iter);
     hipLaunchKernelGGL((cg_dir_kernel<T>), dim3(grid), dim3(kBlock), 0, s, p, z, cg, total, C);
     return hipGetLastError();
 }
@@ -663,7 +668,7 @@ hipError_t launch_cg_init(const double* b, const double* scale, const T* minv, T
                           long long total, int C, double* cg, double* part, int grid, hipStream_t s) {
     hipLaunchKernelGGL((cg_init_kernel<T>), dim3(grid), dim3(kBlock), 0, s, b, scale, minv, x, r, z, p, total, C,
                        part);
-    hipLaunchKernelGGL(cg_start_kernel, dim3(1), dim3(kBlock), 0, s, cg, part, grid, scale, C);
+    hipLaunchKernelGGL(cg_start_kernel, dim3(1), dim3(kRed), 0, s, cg, part, grid, scale, C);
     return hipGetLastError();
 }
 

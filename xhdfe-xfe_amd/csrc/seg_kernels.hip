@@ -345,7 +345,7 @@ seg_bnd_tail_kernel(int* k0, T* v0, int* k1, T* v1, int m, T* out, int C, int c0
     }
 }
 
-// tot[e] = sum over blocks of hpart[b][e], one block per entry e.// tot[e] = sum over blocks of hpart[b][e], one block per entry e.
+// tot[e] = sum over blocks of hpart[b][e], one block per entry e.
 __global__ void __launch_bounds__(kBlock) seg_hsum_kernel(const double* hpart, int nblk, int E, double* tot) {
     __shared__ double red[kBlock / kWave];
     const int e = blockIdx.x;

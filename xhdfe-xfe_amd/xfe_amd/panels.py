@@ -53,13 +53,8 @@ def config0(seed=SEED0 + 0, workers=20000, years=10, firms=2000):
                 name="c0_two_way_200k")
 
 
-def akm_panel(seed=SEED0 + 100, workers=5_950_000, firms=800_000, years=36, tenure_mean=9.4,
-              pareto_alpha=1.1, p_move=0.05, beta=(0.5, -0.2, 0.1, 0.05)):
-    """AKM-shape panel: geometric tenure (mean 9.4, capped at `years`), start
-    years spread so every spell fits the window, Pareto(alpha) firm sizes,
-    size-weighted moves with probability p_move per year.  Regressors:
-    tenure, tenure^2/100 and two N(0,1).  FE = (worker, firm, year)."""
-    rng = np.random.default_rng(seed)
+def _akm_ids(rng, workers, firms, years, tenure_mean, pareto_alpha, p_move):
+    """Worker, firm and year ids (and the within-spell index) of the AKM shape."""
     T = np.minimum(rng.geometric(1.0 / tenure_mean, workers), years).astype(np.int64)
     start = (rng.random(workers) * (years - T + 1)).astype(np.int64)
     n = int(T.sum())
@@ -73,6 +68,18 @@ def akm_panel(seed=SEED0 + 100, workers=5_950_000, firms=800_000, years=36, tenu
     size = (1.0 - rng.random(firms)) ** (-1.0 / pareto_alpha)
     cdf = np.cumsum(size)
     firm = _firm_path(rng, cdf, first, p_move)
+    return worker, firm, year, within
+
+
+def akm_panel(seed=SEED0 + 100, workers=5_950_000, firms=800_000, years=36, tenure_mean=9.4,
+              pareto_alpha=1.1, p_move=0.05, beta=(0.5, -0.2, 0.1, 0.05)):
+    """AKM-shape panel: geometric tenure (mean 9.4, capped at `years`), start
+    years spread so every spell fits the window, Pareto(alpha) firm sizes,
+    size-weighted moves with probability p_move per year.  Regressors:
+    tenure, tenure^2/100 and two N(0,1).  FE = (worker, firm, year)."""
+    rng = np.random.default_rng(seed)
+    worker, firm, year, within = _akm_ids(rng, workers, firms, years, tenure_mean, pareto_alpha, p_move)
+    n = worker.size
     tenure = (within + 1).astype(np.float64)
     X = np.empty((n, 4), order="F")
     X[:, 0] = tenure
@@ -98,3 +105,13 @@ def akm_small(seed=SEED0 + 100, scale=0.01):
     """A config-1-shaped panel scaled down by `scale` (workers and firms)."""
     return akm_panel(seed=seed, workers=max(10, int(5_950_000 * scale)),
                      firms=max(5, int(800_000 * scale)))
+
+
+def kss_panel(seed=SEED0 + 300, workers=1_200_000, firms=150_000, years=36, tenure_mean=8.4,
+              pareto_alpha=1.3, p_move=0.08):
+    """The KSS leverage shape: 1.2M workers, 150k firms with Pareto(1.3)
+    sizes, mover probability 0.08 per year, geometric tenure of mean 8.4
+    (about 10M person-years).  Returns worker and firm ids per row."""
+    rng = np.random.default_rng(seed)
+    worker, firm, _, _ = _akm_ids(rng, workers, firms, years, tenure_mean, pareto_alpha, p_move)
+    return dict(worker=worker, firm=firm, name=f"kss_{worker.size}obs_{workers}w_{firms}f")
