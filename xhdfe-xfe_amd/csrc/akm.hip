@@ -18,6 +18,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -84,35 +85,167 @@ constexpr int kLaneGroup = 32;
 constexpr int kGroups = kBlock / kLaneGroup;  // lane groups per block
 constexpr int kPiece = 64;                    // firm-major matches (or list entries) per group
 
-// t[w * nb + c] = (1 / Dw[w]) sum over the worker's matches a of c_a p[f_a * nb + c]
+constexpr int kBatch = 16;  // gathers issued together by a lane group
+
+// A group's records of a piece of at most kPiece consecutive matches: thread c
+// loads entries j0 + c and j0 + 32 + c (coalesced); entry e is then read by
+// every thread of the group with a shuffle.
+template <typename T>
+__device__ __forceinline__ T group_entry(T a, T b, int e) {
+    return e < kLaneGroup ? __shfl(a, e, kLaneGroup) : __shfl(b, e - kLaneGroup, kLaneGroup);
+}
+
+// Worker means over pieces of whole workers (piece boundaries pstart fall on
+// worker boundaries; a piece holds at most kPiece matches unless one worker has
+// more): t[w * nb + c] = sum over the worker's matches a of
+// (c_a / Dw[w]) p[f_a * nb + c].  Records come in kPiece at a time with two
+// coalesced loads, the p gathers of kBatch matches are issued together, then
+// the runs of equal worker are summed.
 __global__ void __launch_bounds__(kBlock)
-akm_wmean_kernel(const int* wseg, const int* mf, const int* len, const double* inv_dw, const double* p, int N,
-                 int nb, double* t) {
+akm_wmean_kernel(const int* __restrict__ pstart, int npieces, const int* __restrict__ rw,
+                 const int* __restrict__ rf, const double* __restrict__ rwt, const double* __restrict__ p, int nb,
+                 double* __restrict__ t) {
     const int c = threadIdx.x % kLaneGroup;
     const int stride = gridDim.x * kGroups;
-    for (int w = blockIdx.x * kGroups + threadIdx.x / kLaneGroup; w < N; w += stride) {
-        const int a0 = wseg[w], a1 = wseg[w + 1];
+    // both groups of a wave run the same number of rounds and chunks (shuffles
+    // need every lane of the wave)
+    const int q0 = blockIdx.x * kGroups + threadIdx.x / kLaneGroup;
+    const int rounds = (npieces - (q0 - q0 % 2) + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        const int q = q0 + r * stride;
+        const bool live = q < npieces;
+        const int j0 = live ? pstart[q] : 0;
+        const int len = live ? pstart[q + 1] - j0 : 0;
+        const int cl = c < nb ? c : 0;
+        const int chunks = (len + kPiece - 1) / kPiece;
+        const int cmax = max(chunks, __shfl_xor(chunks, kLaneGroup));
+        int cur = -1;
         double acc = 0.0;
-        if (c < nb) {
-            for (int a = a0; a < a1; ++a)
-                acc += static_cast<double>(len[a]) * p[static_cast<size_t>(mf[a]) * nb + c];
-            t[static_cast<size_t>(w) * nb + c] = acc * inv_dw[w];
+        for (int ch = 0; ch < cmax; ++ch) {
+            const int base = j0 + ch * kPiece;
+            const int n = min(max(len - ch * kPiece, 0), kPiece);
+            const int wA = c < n ? rw[base + c] : -1, wB = c + kLaneGroup < n ? rw[base + kLaneGroup + c] : -1;
+            const int fA = c < n ? rf[base + c] : 0, fB = c + kLaneGroup < n ? rf[base + kLaneGroup + c] : 0;
+            const double xA = c < n ? rwt[base + c] : 0.0;
+            const double xB = c + kLaneGroup < n ? rwt[base + kLaneGroup + c] : 0.0;
+            const int nmax = max(n, __shfl_xor(n, kLaneGroup));
+            if (ch == 0) cur = group_entry(wA, wB, 0);
+            for (int e0 = 0; e0 < nmax; e0 += kBatch) {
+                double v[kBatch];
+                // unconditional loads (an entry past n reads firm 0, a lane past nb
+                // lane 0; both are masked below), so the kBatch gathers are in flight
+                // together
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) {
+                    const int f = group_entry(fA, fB, e0 + u);
+                    v[u] = p[static_cast<size_t>(f) * nb + cl];
+                }
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) {
+                    const int w = group_entry(wA, wB, e0 + u);
+                    const double x = group_entry(xA, xB, e0 + u);
+                    if (e0 + u < n) {
+                        if (w != cur) {
+                            if (c < nb) t[static_cast<size_t>(cur) * nb + c] = acc;
+                            cur = w;
+                            acc = 0.0;
+                        }
+                        acc += x * v[u];  // x = 0 past n
+                    }
+                }
+            }
+        }
+        if (len > 0 && c < nb) t[static_cast<size_t>(cur) * nb + c] = acc;
+    }
+}
+
+// Firm pass over fixed pieces of kPiece firm-major matches (records: firm key,
+// worker, count): value c_a (p[f] - t[w_a]) per match with the t and p gathers
+// of kBatch matches issued together, runs of equal firm summed; a run strictly
+// inside its piece is stored in out, the first and the last run of the piece go
+// to the boundary list (entries 2q and 2q + 1, a one-run piece adds (key, 0) as
+// its second entry), unless final (the matches are one piece: every run stored).
+__global__ void __launch_bounds__(kBlock)
+akm_firm_kernel(const int* __restrict__ fkey, const int* __restrict__ fw, const double* __restrict__ fc,
+                const double* __restrict__ p, const double* __restrict__ t, int m, int nb, double* __restrict__ out,
+                int* __restrict__ kout, double* __restrict__ vout, int final) {
+    const int c = threadIdx.x % kLaneGroup;
+    const int stride = gridDim.x * kGroups;
+    const int pieces = (m + kPiece - 1) / kPiece;
+    const int q0 = blockIdx.x * kGroups + threadIdx.x / kLaneGroup;
+    const int rounds = (pieces - (q0 - q0 % 2) + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        const int q = q0 + r * stride;
+        const bool live = q < pieces;
+        const int j0 = q * kPiece;
+        const int n = live ? min(m - j0, kPiece) : 0;
+        const int kA = c < n ? fkey[j0 + c] : 0, kB = c + kLaneGroup < n ? fkey[j0 + kLaneGroup + c] : 0;
+        const int wA = c < n ? fw[j0 + c] : 0, wB = c + kLaneGroup < n ? fw[j0 + kLaneGroup + c] : 0;
+        const double xA = c < n ? fc[j0 + c] : 0.0, xB = c + kLaneGroup < n ? fc[j0 + kLaneGroup + c] : 0.0;
+        const int nmax = max(n, __shfl_xor(n, kLaneGroup));
+        const int cl = c < nb ? c : 0;
+        int key = group_entry(kA, kB, 0);
+        double acc = 0.0;
+        bool first = true;
+        for (int e0 = 0; e0 < nmax; e0 += kBatch) {
+            double v[kBatch];
+            // unconditional loads (an entry past n reads firm 0 and worker 0, a lane
+            // past nb lane 0; both are masked below), so the 2 kBatch gathers are in
+            // flight together
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int k = group_entry(kA, kB, e0 + u);
+                const int w = group_entry(wA, wB, e0 + u);
+                v[u] = p[static_cast<size_t>(k) * nb + cl] - t[static_cast<size_t>(w) * nb + cl];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int k = group_entry(kA, kB, e0 + u);
+                const double x = group_entry(xA, xB, e0 + u);
+                if (e0 + u < n) {
+                    if (k != key) {
+                        if (c < nb) {
+                            if (first && !final) vout[static_cast<size_t>(2 * q) * nb + c] = acc;
+                            else out[static_cast<size_t>(key) * nb + c] = acc;
+                        }
+                        if (first && !final && c == 0) kout[2 * q] = key;
+                        first = false;
+                        key = k;
+                        acc = 0.0;
+                    }
+                    acc += x * v[u];
+                }
+            }
+        }
+        if (n > 0) {
+            if (final) {
+                if (c < nb) out[static_cast<size_t>(key) * nb + c] = acc;
+            } else {
+                const size_t e0 = static_cast<size_t>(2 * q) * nb, e1 = static_cast<size_t>(2 * q + 1) * nb;
+                if (c < nb) {
+                    if (first) {
+                        vout[e0 + c] = acc;
+                        vout[e1 + c] = 0.0;
+                    } else {
+                        vout[e1 + c] = acc;
+                    }
+                }
+                if (c == 0) {
+                    if (first) kout[2 * q] = key;
+                    kout[2 * q + 1] = key;
+                }
+            }
         }
     }
 }
 
-// Runs of equal key over pieces of kPiece elements, one lane group per piece.
-// A run strictly inside its piece is complete and stored in out[key]; the
-// first and the last run of a piece go to the boundary list as entries 2q and
-// 2q + 1 (a piece with one run adds (key, 0) as its second entry), so every
-// key's total is either stored once or split over boundary entries only, and
-// the list stays sorted by key.  With final (the input is one piece) every run
-// is stored.  FIRM: elements are firm-major matches, value c_a (p[f] - t[w_a]);
-// otherwise elements are list entries (key, value).
-template <bool FIRM>
+// One level of the boundary list reduction: entries (key, nb values) sorted by
+// key, in pieces of kPiece entries per lane group, with the same run rule as
+// akm_firm_kernel (a key's total is stored once or split over the next
+// level's entries only).
 __global__ void __launch_bounds__(kBlock)
-akm_runs_kernel(const int* key_in, const int* fw, const int* flen, const double* p, const double* vin, int m, int nb,
-                double* out, int* kout, double* vout, int final) {
+akm_list_kernel(const int* __restrict__ key_in, const double* __restrict__ vin, int m, int nb,
+                double* __restrict__ out, int* __restrict__ kout, double* __restrict__ vout, int final) {
     const int c = threadIdx.x % kLaneGroup;
     const int stride = gridDim.x * kGroups;
     const int pieces = (m + kPiece - 1) / kPiece;
@@ -134,14 +267,7 @@ akm_runs_kernel(const int* key_in, const int* fw, const int* flen, const double*
                 key = k;
                 acc = 0.0;
             }
-            if (c < nb) {
-                if constexpr (FIRM) {
-                    acc += static_cast<double>(flen[j]) *
-                           (p[static_cast<size_t>(k) * nb + c] - vin[static_cast<size_t>(fw[j]) * nb + c]);
-                } else {
-                    acc += vin[static_cast<size_t>(j) * nb + c];
-                }
-            }
+            if (c < nb) acc += vin[static_cast<size_t>(j) * nb + c];
         }
         if (final) {
             if (c < nb) out[static_cast<size_t>(key) * nb + c] = acc;
@@ -179,10 +305,12 @@ struct xfe_akm_context {
     int N = 0, J = 0, ground = 0, M = 0;
     hipStream_t s = nullptr;
     // worker-major matches
-    DevBuf<int> wseg, r1, len;
+    int PW = 0;  // worker pieces
+    DevBuf<int> wpiece, rw, r1;
+    DevBuf<double> rwt, fc;
     // firm-major matches
-    DevBuf<int> fkey, fw, flen, has;
-    DevBuf<double> inv_dw, inv_diag, corr;
+    DevBuf<int> fkey, fw, has;
+    DevBuf<double> inv_diag, corr;
     // workspace
     DevBuf<double> wmean, lanes, b, x, r, z, p, q, cg, part, scale;
     DevBuf<int> bk[2];
@@ -204,14 +332,14 @@ struct xfe_akm_context {
     // means, the firm-major runs with the boundary list reduced level by level,
     // then the degree correction and the ground pin.
     void apply(const double* pv, double* out, int nb) {
-        hipLaunchKernelGGL(akm_wmean_kernel, dim3(akm_grid(static_cast<long long>(N) * kLaneGroup)), dim3(kBlock), 0,
-                           s, wseg.p, r1.p, len.p, inv_dw.p, pv, N, nb, wmean.p);
+        hipLaunchKernelGGL(akm_wmean_kernel, dim3(akm_grid(static_cast<long long>(PW) * kLaneGroup)), dim3(kBlock),
+                           0, s, wpiece.p, PW, rw.p, r1.p, rwt.p, pv, nb, wmean.p);
         XFE_CHECK(hipGetLastError());
         int m = M;
         int pieces = (m + kPiece - 1) / kPiece;
         if (g_profile.enabled) XFE_CHECK(hipEventRecord(ev0, s));
-        hipLaunchKernelGGL(akm_runs_kernel<true>, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
-                           dim3(kBlock), 0, s, fkey.p, fw.p, flen.p, pv, wmean.p, m, nb, out, bk[0].p, bv[0].p,
+        hipLaunchKernelGGL(akm_firm_kernel, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
+                           dim3(kBlock), 0, s, fkey.p, fw.p, fc.p, pv, wmean.p, m, nb, out, bk[0].p, bv[0].p,
                            pieces == 1 ? 1 : 0);
         XFE_CHECK(hipGetLastError());
         if (g_profile.enabled) {
@@ -229,9 +357,9 @@ struct xfe_akm_context {
         while (pieces > 1) {
             m = 2 * pieces;
             pieces = (m + kPiece - 1) / kPiece;
-            hipLaunchKernelGGL(akm_runs_kernel<false>, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
-                               dim3(kBlock), 0, s, bk[cur].p, nullptr, nullptr, nullptr, bv[cur].p, m, nb, out,
-                               bk[1 - cur].p, bv[1 - cur].p, pieces == 1 ? 1 : 0);
+            hipLaunchKernelGGL(akm_list_kernel, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
+                               dim3(kBlock), 0, s, bk[cur].p, bv[cur].p, m, nb, out, bk[1 - cur].p, bv[1 - cur].p,
+                               pieces == 1 ? 1 : 0);
             XFE_CHECK(hipGetLastError());
             cur = 1 - cur;
         }
@@ -356,10 +484,9 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
         set_error("xfe_akm_create: inconsistent match structure");
         return nullptr;
     }
-    // host restatement of the structure into the segment records
-    std::vector<int> wseg(N + 1), len(M), fkey(M), fw(M), flen(M), has(J, 0);
-    std::vector<double> inv_dw(N), corr(J);
-    for (int w = 0; w <= N; ++w) wseg[w] = static_cast<int>(w_ptr[w]);
+    // host restatement of the structure into the device records
+    std::vector<int> rw(M), fkey(M), fw(M), has(J, 0), wpiece;
+    std::vector<double> rwt(M), fc(M), corr(J);
     for (int w = 0; w < N; ++w) {
         if (w_ptr[w + 1] < w_ptr[w]) {
             set_error("xfe_akm_create: w_ptr is not non-decreasing");
@@ -379,25 +506,64 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
             set_error("xfe_akm_create: match counts must be positive integers and ids in range");
             return nullptr;
         }
-        len[a] = static_cast<int>(c);
     }
+    // Workers are renumbered by their primary firm (most person-years, then the
+    // lowest firm id), so that the firm pass reads the worker means of a firm's
+    // stayers as one contiguous block.  S is a firm-by-firm operator: the
+    // numbering of workers does not change it.
+    std::vector<int> primary(N, -1), order(N), newid(N);
+    for (int w = 0; w < N; ++w) {
+        double best = -1.0;
+        for (int64_t a = w_ptr[w]; a < w_ptr[w + 1]; ++a)
+            if (m_c[a] > best || (m_c[a] == best && m_f[a] < primary[w])) {
+                best = m_c[a];
+                primary[w] = m_f[a];
+            }
+    }
+    for (int w = 0; w < N; ++w) order[w] = w;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return primary[a] < primary[b]; });
+    for (int k = 0; k < N; ++k) newid[order[k]] = k;
+    // worker-major records in the new order, and pieces of whole workers of at
+    // most kPiece matches
+    std::vector<int> rf(M);
+    wpiece.push_back(0);
+    int pos = 0;
+    for (int k = 0; k < N; ++k) {
+        const int w = order[k];
+        const int a0 = pos;
+        const double iw = Dw[w] != 0.0 ? 1.0 / Dw[w] : 0.0;
+        for (int64_t a = w_ptr[w]; a < w_ptr[w + 1]; ++a, ++pos) {
+            rw[pos] = k;
+            rf[pos] = m_f[a];
+            rwt[pos] = m_c[a] * iw;
+        }
+        if (pos > a0 && pos - wpiece.back() > kPiece && a0 > wpiece.back()) wpiece.push_back(a0);
+    }
+    wpiece.push_back(M);
+    // firm-major records, a firm's matches in the new worker order
     std::vector<double> deg(J, 0.0);
+    std::vector<std::pair<int, double>> run;
     for (int f = 0; f < J; ++f) {
+        run.clear();
         for (int64_t j = f_ptr[f]; j < f_ptr[f + 1]; ++j) {
             const int64_t m = f_matches[j];
             if (m < 0 || m >= M || m_f[m] != f) {
                 set_error("xfe_akm_create: f_matches does not list the firm's matches");
                 return nullptr;
             }
-            fkey[j] = f;
-            fw[j] = m_w[m];
-            flen[j] = len[m];
+            run.emplace_back(newid[m_w[m]], m_c[m]);
             deg[f] += m_c[m];
+        }
+        std::sort(run.begin(), run.end());
+        for (size_t i = 0; i < run.size(); ++i) {
+            const int64_t j = f_ptr[f] + static_cast<int64_t>(i);
+            fkey[j] = f;
+            fw[j] = run[i].first;
+            fc[j] = run[i].second;
         }
         has[f] = f_ptr[f + 1] > f_ptr[f] ? 1 : 0;
         corr[f] = Df[f] - deg[f];
     }
-    for (int w = 0; w < N; ++w) inv_dw[w] = Dw[w] != 0.0 ? 1.0 / Dw[w] : 0.0;
     xfe_akm_context* c = nullptr;
     try {
         (void)hipGetLastError();
@@ -414,15 +580,15 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
             d.ensure(n);
             XFE_CHECK(hipMemcpyAsync(d.p, h, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
         };
-        up_i(c->wseg, wseg);
-        c->r1.ensure(M);
-        XFE_CHECK(hipMemcpyAsync(c->r1.p, m_f, sizeof(int) * M, hipMemcpyHostToDevice, c->s));
-        up_i(c->len, len);
+        c->PW = static_cast<int>(wpiece.size()) - 1;
+        up_i(c->wpiece, wpiece);
+        up_i(c->rw, rw);
+        up_d(c->rwt, rwt.data(), M);
+        up_d(c->fc, fc.data(), M);
+        up_i(c->r1, rf);
         up_i(c->fkey, fkey);
         up_i(c->fw, fw);
-        up_i(c->flen, flen);
         up_i(c->has, has);
-        up_d(c->inv_dw, inv_dw.data(), N);
         up_d(c->inv_diag, inv_diagS, J);
         up_d(c->corr, corr.data(), J);
         c->wmean.ensure(static_cast<size_t>(N) * kMaxCgCols);
