@@ -161,7 +161,7 @@ def bench_akm(args):
     solver.close()
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     its = np.concatenate(all_its[:lanes // batch])
-    log(f"{sec:.3f} s per {lanes} lanes; seg_firm: {launches} launches, "
+    log(f"{sec:.3f} s per {lanes} lanes; akm_firm_kernel: {launches} launches, "
         f"{pass_ms / max(launches, 1):.4f} ms avg, {achieved:.0f} GB/s algorithmic")
     cpu = None
     if not args.no_cpu:
@@ -177,8 +177,8 @@ def bench_akm(args):
         "iterations": {"min": int(its.min()), "max": int(its.max()), "mean": round(float(its.mean()), 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(os.path.join(ROOT, "profiles", "akm_seg_firm_kss10m.md")),
-                     "kernel": "seg_firm_kernel<double> (firm-major match pass of S)",
+                     "traffic": measured_traffic(os.path.join(ROOT, "profiles", "akm_firm_kss10m.md")),
+                     "kernel": "akm_firm_kernel (firm-major match pass of S, 32 lanes)",
                      "avg_launch_ms": round(pass_ms / max(launches, 1), 4),
                      "bytes_per_launch": round(pass_bytes / max(launches, 1), 1)},
         "cpu_baseline": cpu,

@@ -242,48 +242,71 @@ akm_firm_kernel(const int* __restrict__ fkey, const int* __restrict__ fw, const 
 // One level of the boundary list reduction: entries (key, nb values) sorted by
 // key, in pieces of kPiece entries per lane group, with the same run rule as
 // akm_firm_kernel (a key's total is stored once or split over the next
-// level's entries only).
+// level's entries only); keys come in with two coalesced loads and the values
+// of kBatch entries are loaded together.
 __global__ void __launch_bounds__(kBlock)
 akm_list_kernel(const int* __restrict__ key_in, const double* __restrict__ vin, int m, int nb,
                 double* __restrict__ out, int* __restrict__ kout, double* __restrict__ vout, int final) {
     const int c = threadIdx.x % kLaneGroup;
     const int stride = gridDim.x * kGroups;
     const int pieces = (m + kPiece - 1) / kPiece;
-    for (int q = blockIdx.x * kGroups + threadIdx.x / kLaneGroup; q < pieces; q += stride) {
+    const int q0 = blockIdx.x * kGroups + threadIdx.x / kLaneGroup;
+    const int rounds = (pieces - (q0 - q0 % 2) + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        const int q = q0 + r * stride;
+        const bool live = q < pieces;
         const int j0 = q * kPiece;
-        const int j1 = min(m, j0 + kPiece);
-        int key = key_in[j0];
+        const int n = live ? min(m - j0, kPiece) : 0;
+        const int kA = c < n ? key_in[j0 + c] : 0, kB = c + kLaneGroup < n ? key_in[j0 + kLaneGroup + c] : 0;
+        const int nmax = max(n, __shfl_xor(n, kLaneGroup));
+        const int cl = c < nb ? c : 0;
+        int key = group_entry(kA, kB, 0);
         double acc = 0.0;
         bool first = true;
-        for (int j = j0; j < j1; ++j) {
-            const int k = key_in[j];
-            if (k != key) {
-                if (c < nb) {
-                    if (first && !final) vout[static_cast<size_t>(2 * q) * nb + c] = acc;
-                    else out[static_cast<size_t>(key) * nb + c] = acc;
-                }
-                if (first && !final && c == 0) kout[2 * q] = key;
-                first = false;
-                key = k;
-                acc = 0.0;
+        for (int e0 = 0; e0 < nmax; e0 += kBatch) {
+            // unconditional loads (an entry past n reads entry 0, a lane past nb
+            // lane 0; both are masked below)
+            double v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int j = e0 + u < n ? j0 + e0 + u : 0;
+                v[u] = vin[static_cast<size_t>(j) * nb + cl];
             }
-            if (c < nb) acc += vin[static_cast<size_t>(j) * nb + c];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int k = group_entry(kA, kB, e0 + u);
+                if (e0 + u < n) {
+                    if (k != key) {
+                        if (c < nb) {
+                            if (first && !final) vout[static_cast<size_t>(2 * q) * nb + c] = acc;
+                            else out[static_cast<size_t>(key) * nb + c] = acc;
+                        }
+                        if (first && !final && c == 0) kout[2 * q] = key;
+                        first = false;
+                        key = k;
+                        acc = 0.0;
+                    }
+                    acc += v[u];
+                }
+            }
         }
-        if (final) {
-            if (c < nb) out[static_cast<size_t>(key) * nb + c] = acc;
-        } else {
-            const size_t e0 = static_cast<size_t>(2 * q) * nb, e1 = static_cast<size_t>(2 * q + 1) * nb;
-            if (c < nb) {
-                if (first) {
-                    vout[e0 + c] = acc;
-                    vout[e1 + c] = 0.0;
-                } else {
-                    vout[e1 + c] = acc;
+        if (n > 0) {
+            if (final) {
+                if (c < nb) out[static_cast<size_t>(key) * nb + c] = acc;
+            } else {
+                const size_t e0 = static_cast<size_t>(2 * q) * nb, e1 = static_cast<size_t>(2 * q + 1) * nb;
+                if (c < nb) {
+                    if (first) {
+                        vout[e0 + c] = acc;
+                        vout[e1 + c] = 0.0;
+                    } else {
+                        vout[e1 + c] = acc;
+                    }
                 }
-            }
-            if (c == 0) {
-                if (first) kout[2 * q] = key;
-                kout[2 * q + 1] = key;
+                if (c == 0) {
+                    if (first) kout[2 * q] = key;
+                    kout[2 * q + 1] = key;
+                }
             }
         }
     }
