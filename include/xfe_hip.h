@@ -31,7 +31,9 @@ typedef struct {
     int num_groups;
     int num_levels_present;
     const double* weight_sums;
-    int is_slope;              /* heterogeneous slope: rejected (returns 0) */
+    int is_slope;              /* heterogeneous slope term on this dimension (xfe_absorb only): each
+                                  level is projected onto span{1, z} (with slope_has_intercept) or
+                                  span{z}, z = slope_values (n, host) */
     int slope_has_intercept;
     const double* slope_values;
     const double* slope_sum_z;
@@ -99,7 +101,9 @@ int xfe_certify(const double* y, const double* X, const double* y_t, const doubl
 int xfe_backend_available(void);
 
 /* absorb_fixed_effects_cuda (src/fe_absorption_cuda.hpp:127-134,
- * impl src/fe_absorption_cuda.cu:2294-4738).  Host buffers: y (n), X (n x k,
+ * impl src/fe_absorption_cuda.cu:2294-4738; slope dimensions as
+ * slope_project_inplace, src/fe_absorption.cpp:4096-4300, by FP64 alternating
+ * projections without the certificate).  Host buffers: y (n), X (n x k,
  * column-major, ld = n), optional weights (n).  sweep_order: num_fes entries
  * (0-based dims) or NULL for the default.  Writes y_out (n) and X_out (n x k).
  * Returns 1 on success (check res->converged, as the reference does), 0 on a

@@ -383,6 +383,34 @@ def schwarz_rule(V, Vt, gids, G):
     return np.where(bn > 1e-300, np.sqrt(rn) / np.where(bn > 1e-300, bn, 1.0), 0.0)
 
 
+def slope_projection(V, gids, G, slope_dim, z, intercept=True, weights=None):
+    """The within transform with a heterogeneous slope term on dimension
+    slope_dim (slope_project_inplace, src/fe_absorption.cpp:4096-4300, as the
+    fixed point of the alternating projections): v - A (A'WA)^+ A'W v, where A
+    holds one indicator column per level of every other dimension and, for
+    slope_dim, z times the level indicator (and the indicator itself with the
+    intercept).  Dense least squares: small panels only."""
+    V = np.asarray(V, dtype=np.float64)
+    if V.ndim == 1:
+        V = V[:, None]
+    n = V.shape[0]
+    cols = []
+    for d, (g, Gd) in enumerate(zip(gids, G)):
+        g = np.asarray(g)
+        ind = np.zeros((n, Gd))
+        ind[np.arange(n), g] = 1.0
+        if d == slope_dim:
+            if intercept:
+                cols.append(ind)
+            cols.append(ind * np.asarray(z, dtype=np.float64)[:, None])
+        else:
+            cols.append(ind)
+    A = np.hstack(cols)
+    sw = np.ones(n) if weights is None else np.sqrt(np.asarray(weights, dtype=np.float64))
+    coef, *_ = np.linalg.lstsq(A * sw[:, None], V * sw[:, None], rcond=1e-12)
+    return V - A @ coef
+
+
 def ref_schwarz(y, X, gids, G, tol=1e-10, max_iter=10000):
     """The reference's schwarz_demean_raw through oracle/_ref (checker):
     (ok, y_t, X_t, iterations)."""
@@ -402,6 +430,28 @@ def ref_schwarz(y, X, gids, G, tol=1e-10, max_iter=10000):
     if rc < 0:
         raise RuntimeError("ref_schwarz failed")
     return bool(rc), yo, Xo, it.value
+
+
+def ref_absorb_slope(y, X, fes, slope_dim, z, intercept=True, tol=1e-8, tolerance_mode=1, threads=1):
+    """The reference's absorb_fixed_effects_v6 with one heterogeneous slope
+    term through oracle/_ref (checker): (y_t, X_t, iterations, converged)."""
+    r = ref()
+    if r is None:
+        return None
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
+    n, k = X.shape
+    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    zz = np.ascontiguousarray(z, dtype=np.float64)
+    yo = np.empty(n)
+    Xo = np.empty((n, k), order="F")
+    it, cv = _i(), _i()
+    rc = r.ref_absorb_slope(_i(n), _i(k), _i(len(fes)), _ptr(y), _ptr(X), _ptr(ids), _i(slope_dim), _ptr(zz),
+                            _i(int(bool(intercept))), _d(tol), _i(tolerance_mode), _i(threads), _ptr(yo),
+                            _ptr(Xo), ctypes.byref(it), ctypes.byref(cv))
+    if rc != 0:
+        raise RuntimeError("ref_absorb_slope failed")
+    return yo, Xo, it.value, bool(cv.value)
 
 
 def ref_absorb(y, X, fes, tol=1e-8, tolerance_mode=1, threads=1, symmetric=False):

@@ -195,4 +195,32 @@ int ref_schwarz(int n, int k, int d, const double* y, const double* X, const int
     }
 }
 
+// absorb_fixed_effects_v6 (src/fe_absorption.cpp:8750) with one heterogeneous
+// slope term on dimension slope_dim (values z, with or without the level
+// intercept), CPU backend.  Returns 0 on success, -1 on exception.
+int ref_absorb_slope(int n, int k, int d, const double* y, const double* X, const int* fes, int slope_dim,
+                     const double* z, int include_intercept, double tol, int tolerance_mode, int threads,
+                     double* y_out, double* X_out, int* iterations, int* converged) {
+    try {
+        Eigen::VectorXd yv = Eigen::Map<const Eigen::VectorXd>(y, n);
+        Eigen::MatrixXd Xm = Eigen::Map<const Eigen::MatrixXd>(X, n, k);
+        auto fv = make_fes(fes, n, d);
+        hdfe::HdfeOptions o = make_options(tol, tolerance_mode, threads);
+        std::vector<hdfe::detail::HeterogeneousSlopeTerm> slopes(1);
+        slopes[0].fe_index = slope_dim;
+        slopes[0].values = Eigen::Map<const Eigen::VectorXd>(z, n);
+        slopes[0].include_intercept = include_intercept != 0;
+        auto r = hdfe::detail::absorb_fixed_effects_v6(yv, Xm, fv, nullptr, o, hdfe::AbsorptionMethod::Auto,
+                                                       slopes);
+        std::memcpy(y_out, r.y_tilde.data(), sizeof(double) * n);
+        if (k > 0) std::memcpy(X_out, r.X_tilde.data(), sizeof(double) * n * k);
+        *iterations = r.iterations;
+        *converged = r.converged ? 1 : 0;
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_absorb_slope: %s\n", e.what());
+        return -1;
+    }
+}
+
 }  // extern "C"

@@ -75,3 +75,13 @@ def schwarz_edge_cases():
         ((y, X15, [a, b], [100, 30]), 10000),
         ((y, X16, [a, b], [100, 30]), 10000),
     ]
+
+
+# Heterogeneous-slope cases: (case name, slope dimension, with intercept) on
+# the kept rows of the golden panels, slope variable slope_values(n).
+SLOPE_CASES = (("two_way", 1, True), ("two_way", 1, False), ("akm_three_way", 1, True))
+
+
+def slope_values(n):
+    """The slope variable of the slope cases: N(1, 1), seeded."""
+    return np.random.default_rng(20261213).normal(1.0, 1.0, n)

@@ -19,7 +19,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "xhdfe-xfe_amd"))
 
 import oracle  # noqa: E402
-from golden_cases import CASES, certificate_weights, input_checksum, schwarz_edge_cases  # noqa: E402
+from golden_cases import (CASES, SLOPE_CASES, certificate_weights, input_checksum,  # noqa: E402
+                          schwarz_edge_cases, slope_values)
 
 
 def main():
@@ -63,6 +64,17 @@ def main():
             out[f"{name}/schwarz_{tag}/X_tilde"] = Xo
             out[f"{name}/schwarz_{tag}/iterations"] = np.int64(it)
             out[f"{name}/schwarz_{tag}/ok"] = np.int64(ok)
+        # one heterogeneous slope term (absorb_fixed_effects_v6 with slopes)
+        for sname, sdim, icpt in SLOPE_CASES:
+            if sname != name:
+                continue
+            z = slope_values(yk.size)
+            yo, Xo, it, conv = oracle.ref_absorb_slope(yk, Xk, fk, sdim, z, intercept=icpt, tol=1e-12)
+            tag = f"slope{sdim}_{'icpt' if icpt else 'noicpt'}"
+            out[f"{name}/{tag}/y_tilde"] = yo
+            out[f"{name}/{tag}/X_tilde"] = Xo
+            out[f"{name}/{tag}/iterations"] = np.int64(it)
+            out[f"{name}/{tag}/converged"] = np.int64(conv)
         # certificate of the untransformed columns (large residual, not certified)
         c = oracle.ref_certify(yk, Xk, yk, Xk, fk, tol=1e-8, converged=True)
         out[f"{name}/raw/cert"] = np.array([c["abs_residual"], c["abs_residual_rel"], float(c["certified"])])

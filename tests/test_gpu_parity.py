@@ -5,6 +5,7 @@ compared exactly; demeaned columns within 1e-8 relative to the column scale
 (max |v~_gpu - v~_oracle| / max |v|) at the default tolerance 1e-8 and within
 1e-10 at tol 1e-12; Gram / meat within 1e-12 relative.
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -502,3 +503,64 @@ def test_schwarz_edge_cases(golden):
         assert np.abs(Vg - Vo).max() / np.abs(V).max() <= 1e-9, name
         if name == "one_zero_column":
             assert np.array_equal(X[:, 1], X0[:, 1])
+
+
+# ------------------------------------------------------------ heterogeneous slopes
+# The slope path runs FP64 alternating projections to a per-sweep change of
+# tol_eff max(1, ||v||); it is compared with the exact projection (oracle) and
+# with the reference's absorber (fixtures at tol 1e-12, themselves 9e-9 of the
+# column scale from the projection on these slowly converging panels).
+
+@pytest.mark.parametrize("name,sdim,icpt", [("two_way", 1, True), ("two_way", 1, False),
+                                             ("akm_three_way", 1, True)])
+def test_absorb_slopes_matches_oracle(golden, cases, name, sdim, icpt):
+    from golden_cases import slope_values
+    _log(f"slopes {name} dim {sdim} intercept {icpt}")
+    _, fes, gids, G, V = _prep(cases[name])
+    z = slope_values(V.shape[0])
+    yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-12, slopes={sdim: (z, icpt)})
+    assert info["converged"], info
+    Vg = np.column_stack([yg, Xg])
+    Vo = oracle.slope_projection(V, gids, G, sdim, z, intercept=icpt)
+    tag = f"slope{sdim}_{'icpt' if icpt else 'noicpt'}"
+    Vr = np.column_stack([golden[f"{name}/{tag}/y_tilde"], golden[f"{name}/{tag}/X_tilde"]])
+    scale = np.abs(V).max()
+    err_o = np.abs(Vg - Vo).max() / scale
+    err_r = np.abs(Vg - Vr).max() / scale
+    _log(f"  sweeps {info['iterations']} vs oracle {err_o:.2e} vs reference {err_r:.2e}")
+    assert err_o <= 1e-8 and err_r <= 2e-8
+
+
+def test_absorb_slope_single_dimension_weighted():
+    """One weighted slope dimension is exact after one sweep: each level's
+    weighted least-squares line of v on z."""
+    _log("slopes single dimension weighted")
+    rng = np.random.default_rng(11)
+    n = 900
+    g = rng.integers(0, 30, n).astype(np.int32)
+    z = rng.normal(size=n)
+    w = rng.uniform(0.5, 2.0, n)
+    V = rng.normal(size=(n, 3))
+    for icpt in (True, False):
+        yg, Xg, info = xfe_amd.absorb(V[:, 0], V[:, 1:], [g], [30], weights=w, tol=1e-12,
+                                      slopes={0: (z, icpt)})
+        Vo = oracle.slope_projection(V, [g], [30], 0, z, intercept=icpt, weights=w)
+        assert info["converged"] and info["iterations"] <= 2
+        assert np.abs(np.column_stack([yg, Xg]) - Vo).max() <= 1e-12
+
+
+def test_absorb_slope_requires_values():
+    rng = np.random.default_rng(1)
+    g = rng.integers(0, 5, 50).astype(np.int32)
+    fes = (xfe_amd.FeInput * 1)()
+    fes[0].group_ids = g.ctypes.data
+    fes[0].num_groups = 5
+    fes[0].num_levels_present = 5
+    fes[0].is_slope = 1
+    y = rng.normal(size=50)
+    yo = np.empty(50)
+    o = xfe_amd._opts(1e-8, 1, 1000, 1, False, 0)
+    r = xfe_amd.AbsorbResult()
+    ok = xfe_amd.lib().xfe_absorb(xfe_amd._ptr(y), None, 50, 0, fes, 1, None, None,
+                                  ctypes.byref(o), ctypes.byref(r), xfe_amd._ptr(yo), None)
+    assert ok == 0 and "slope_values" in xfe_amd.last_error()

@@ -448,3 +448,69 @@ def test_akm_host_structure_matches_oracle():
     b = oracle.akm_build(mw, mf, mc, N, J)
     for k in b:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+# ------------------------------------------------------------ heterogeneous slopes
+
+def _slope_inputs(case):
+    keep = _kept(case)
+    gids, G = _compact([f[keep] for f in case["fes"]])
+    return gids, G, np.column_stack([case["y"][keep], case["X"][keep]])
+
+
+@pytest.mark.parametrize("name,sdim,icpt", [("two_way", 1, True), ("two_way", 1, False),
+                                             ("akm_three_way", 1, True)])
+def test_slope_projection_matches_reference(golden, cases, name, sdim, icpt):
+    """The exact slope projection against the reference's absorber with the
+    slope term at tol 1e-12 (its alternating projections converge slowly on
+    these panels: 53 - 2,508 iterations; 9e-9 of the column scale left)."""
+    from golden_cases import slope_values
+    gids, G, V = _slope_inputs(cases[name])
+    z = slope_values(V.shape[0])
+    tag = f"slope{sdim}_{'icpt' if icpt else 'noicpt'}"
+    assert golden[f"{name}/{tag}/converged"] == 1
+    Vr = np.column_stack([golden[f"{name}/{tag}/y_tilde"], golden[f"{name}/{tag}/X_tilde"]])
+    Vo = oracle.slope_projection(V, gids, G, sdim, z, intercept=icpt)
+    assert np.abs(Vr - Vo).max() / np.abs(V).max() <= 2e-8
+
+
+def test_slope_projection_matches_reference_tight():
+    """Where the reference converges fast it lands on the oracle's projection
+    to rounding (slope on either dimension, with and without the intercept)."""
+    if oracle.ref() is None:
+        pytest.skip("oracle/_ref is not built (needs the reference sources)")
+    rng = np.random.default_rng(3)
+    n = 3000
+    a = rng.integers(0, 200, n)
+    b = rng.integers(0, 40, n)
+    z = rng.normal(size=n) + 0.5
+    y = rng.normal(size=n)
+    X = rng.normal(size=(n, 2))
+    gids, G = _compact([a, b])
+    V = np.column_stack([y, X])
+    for sdim in (0, 1):
+        for icpt in (True, False):
+            yo, Xo, _, conv = oracle.ref_absorb_slope(y, X, [a, b], sdim, z, intercept=icpt, tol=1e-13)
+            Vo = oracle.slope_projection(V, gids, G, sdim, z, intercept=icpt)
+            assert conv and np.abs(np.column_stack([yo, Xo]) - Vo).max() <= 1e-12
+
+
+def test_slope_projection_single_dimension():
+    """One dimension with a slope: per level, the residual of the least-squares
+    line of v on z (numpy polyfit); without the intercept, of v on z alone."""
+    rng = np.random.default_rng(11)
+    n = 900
+    g = rng.integers(0, 30, n)
+    z = rng.normal(size=n)
+    V = rng.normal(size=(n, 2))
+    for icpt in (True, False):
+        Vo = oracle.slope_projection(V, [g], [30], 0, z, intercept=icpt)
+        for lev in range(30):
+            m = g == lev
+            for c in range(2):
+                if icpt:
+                    slope, inter = np.polyfit(z[m], V[m, c], 1)
+                    want = V[m, c] - (inter + slope * z[m])
+                else:
+                    want = V[m, c] - z[m] * (z[m] @ V[m, c]) / (z[m] @ z[m])
+                np.testing.assert_allclose(Vo[m, c], want, atol=1e-12)

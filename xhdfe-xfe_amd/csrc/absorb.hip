@@ -68,6 +68,7 @@ struct Workspace {
     DevBuf<double> stage;  // host-API staging of [y | X]
     DevBuf<int> stage_ids;
     DevBuf<double> stage_w;
+    DevBuf<double> stage_z[XFE_MAX_DIMS];  // host-API staging of slope variables
     // level-aligned tiles of the sorted dimension and spell structures
     DevBuf<int> tile_start, level_count, tmp_flags, tmp_inc, tmp_iota, tmp_key;
     DevBuf<int> sp_key[XFE_MAX_DIMS], sp_order[XFE_MAX_DIMS], sp_base[XFE_MAX_DIMS];
@@ -101,7 +102,7 @@ struct Workspace {
         tmp_iota.release(); tmp_key.release();
         for (int d = 0; d < XFE_MAX_DIMS; ++d) {
             sp_key[d].release(); sp_order[d].release(); sp_base[d].release(); sp_sums[d].release();
-            f_sp[d].release();
+            f_sp[d].release(); stage_z[d].release();
         }
         cg.release(); cg_part.release(); f_beta.release(); f_b.release(); f_diag.release(); f_counts.release(); f_scale.release();
         f_x.release(); f_r.release(); f_z.release(); f_p.release(); f_q.release(); f_minv.release();
@@ -1782,6 +1783,70 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
     res.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 }
 
+// Absorption with heterogeneous slopes (absorb_fixed_effects_v6_mixed,
+// src/fe_absorption.cpp:8162, with slope_project_inplace :4096-4300 for a slope
+// dimension): alternating projections in FP64 over the dimensions, a slope
+// dimension projecting each level onto span{1, z} (or span{z} without the
+// intercept).  Sweeps follow sweep_order (symmetric for two dimensions, as the
+// plain path) until every column's change over a sweep is at most
+// tol_eff max(1, ||v||).  Works on the caller's row order; V is n x C (ld).
+static void absorb_slopes_impl(double* V, int n, int C, int ld, const std::vector<const int*>& gid,
+                               const std::vector<int>& G, const std::vector<const double*>& z,
+                               const std::vector<int>& mode, const double* w, const std::vector<int>& order,
+                               const xfe_options& opt, xfe_absorb_result& res) {
+    Workspace& ws = tls_ws;
+    hipStream_t s = ws.stream;
+    const int D = static_cast<int>(G.size());
+    const int grid = grid_for(n);
+    for (int d = 0; d < D; ++d) {
+        ws.wsum[d].ensure(3 * static_cast<size_t>(G[d]));
+        ws.sums[d].ensure(2 * static_cast<size_t>(G[d]) * C);
+        ws.means[d].ensure(2 * static_cast<size_t>(G[d]) * C);
+        XFE_CHECK(hipMemsetAsync(ws.wsum[d].p, 0, sizeof(double) * 3 * G[d], s));
+        XFE_CHECK(launch_slope_moments(gid[d], z[d], w, n, ws.wsum[d].p, grid, s));
+        res.sweep_order_used[d] = order[d];
+    }
+    ws.vec[1].ensure(static_cast<size_t>(ld) * C);
+    double* old = ws.vec[1].p;
+    ws.partials.ensure(static_cast<size_t>(grid) * C);
+    ws.scalars.ensure(std::max(C, 8));
+    auto column_sums = [&](const double* a, const double* b, double* out) {
+        XFE_CHECK(launch_col_dot(a, b, n, C, ld, ws.partials.p, grid, s));
+        XFE_CHECK(launch_reduce_partials(ws.partials.p, grid, C, 0, ws.scalars.p, s));
+        XFE_CHECK(hipMemcpyAsync(out, ws.scalars.p, sizeof(double) * C, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+    };
+    const std::vector<int> seq = sweep_sequence(order, opt.method == 2 && D > 1);
+    const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
+    std::vector<double> dd(C), xx(C);
+    bool converged = false;
+    int it = 0;
+    while (it < opt.max_iter) {
+        ++it;
+        XFE_CHECK(hipMemcpyAsync(old, V, sizeof(double) * static_cast<size_t>(ld) * C, hipMemcpyDeviceToDevice, s));
+        for (int d : seq) {
+            XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * 2 * static_cast<size_t>(G[d]) * C, s));
+            XFE_CHECK(launch_slope_reduce(V, ld, C, n, gid[d], z[d], w, ws.sums[d].p, grid, s));
+            XFE_CHECK(launch_slope_coef(ws.sums[d].p, ws.wsum[d].p, G[d], C, mode[d], ws.means[d].p,
+                                        grid_for(static_cast<long long>(G[d]) * C), s));
+            XFE_CHECK(launch_slope_apply(V, ld, C, n, gid[d], z[d], ws.means[d].p, grid, s));
+        }
+        XFE_CHECK(launch_sub(old, V, old, n, C, ld, grid, s));
+        column_sums(old, old, dd.data());
+        column_sums(V, V, xx.data());
+        bool done = true;
+        for (int c = 0; c < C; ++c)
+            if (std::sqrt(std::max(0.0, dd[c])) > tol_eff * std::max(1.0, std::sqrt(std::max(0.0, xx[c])))) done = false;
+        if (done) {
+            converged = true;
+            break;
+        }
+    }
+    res.iterations = it;
+    res.converged = converged ? 1 : 0;
+    res.sweeps = it;
+}
+
 }  // namespace xfe
 
 using namespace xfe;
@@ -1843,10 +1908,14 @@ int xfe_absorb(const double* y, const double* X, int n, int k, const xfe_fe_inpu
         return 0;
     }
     std::memset(res, 0, sizeof(*res));
+    bool slopes = false;
     for (int d = 0; d < num_fes; ++d) {
         if (fes[d].is_slope) {
-            set_error("xfe_absorb: heterogeneous slopes are not supported by this backend");
-            return 0;
+            if (!fes[d].slope_values) {
+                set_error("xfe_absorb: a slope dimension needs slope_values");
+                return 0;
+            }
+            slopes = true;
         }
         if (!fes[d].group_ids || fes[d].num_groups <= 0) {
             set_error("xfe_absorb: invalid FE input");
@@ -1886,8 +1955,26 @@ int xfe_absorb(const double* y, const double* X, int n, int k, const xfe_fe_inpu
             XFE_CHECK(hipMemcpyAsync(ws.stage_w.p, weights, sizeof(double) * n,
                                      hipMemcpyHostToDevice, s));
         }
-        absorb_device_impl(ws.stage.p, n, C, n, gids, G, wsum, weights ? ws.stage_w.p : nullptr,
-                           sweep_order, *opt, *res);
+        if (slopes) {
+            std::vector<const double*> z(num_fes, nullptr);
+            std::vector<int> mode(num_fes, 0), order(num_fes);
+            for (int d = 0; d < num_fes; ++d) {
+                res->fe_levels[d] = G[d];
+                order[d] = sweep_order ? sweep_order[d] : d;
+                if (order[d] < 0 || order[d] >= num_fes) throw Error("invalid sweep_order");
+                if (!fes[d].is_slope) continue;
+                ws.stage_z[d].ensure(n);
+                XFE_CHECK(hipMemcpyAsync(ws.stage_z[d].p, fes[d].slope_values, sizeof(double) * n,
+                                         hipMemcpyHostToDevice, s));
+                z[d] = ws.stage_z[d].p;
+                mode[d] = fes[d].slope_has_intercept ? 1 : 2;
+            }
+            absorb_slopes_impl(ws.stage.p, n, C, n, gids, G, z, mode, weights ? ws.stage_w.p : nullptr, order,
+                               *opt, *res);
+        } else {
+            absorb_device_impl(ws.stage.p, n, C, n, gids, G, wsum, weights ? ws.stage_w.p : nullptr,
+                               sweep_order, *opt, *res);
+        }
         XFE_CHECK(hipMemcpyAsync(y_out, ws.stage.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         if (k > 0)
             XFE_CHECK(hipMemcpyAsync(X_out, ws.stage.p + n, sizeof(double) * static_cast<size_t>(n) * k,

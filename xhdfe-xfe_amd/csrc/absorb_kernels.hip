@@ -481,6 +481,106 @@ hipError_t launch_sub(double* y, const double* a, const double* b, int n, int C,
     return hipGetLastError();
 }
 
+// ---- heterogeneous slopes (slope_project_inplace, src/fe_absorption.cpp:4096-4300)
+
+// mom[3 g + {0, 1, 2}] += w, w z, w z^2 over the level's rows (z may be null)
+__global__ void __launch_bounds__(kBlock)
+slope_moments_kernel(const int* gid, const double* z, const double* w, int n, double* mom) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double wi = w ? w[i] : 1.0;
+        double* m = mom + 3 * static_cast<size_t>(gid[i]);
+        atomicAdd(&m[0], wi);
+        if (z) {
+            const double zi = z[i];
+            atomicAdd(&m[1], wi * zi);
+            atomicAdd(&m[2], wi * zi * zi);
+        }
+    }
+}
+
+// sums[2 (g C + c) + {0, 1}] += w v, w z v (the second only with z)
+__global__ void __launch_bounds__(kBlock)
+slope_reduce_kernel(const double* V, int ld, int C, int n, const int* gid, const double* z, const double* w,
+                    double* sums) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double wi = w ? w[i] : 1.0;
+        const double zi = z ? z[i] : 0.0;
+        double* sg = sums + 2 * static_cast<size_t>(gid[i]) * C;
+        for (int c = 0; c < C; ++c) {
+            const double v = wi * V[static_cast<size_t>(c) * ld + i];
+            atomicAdd(&sg[2 * c], v);
+            if (z) atomicAdd(&sg[2 * c + 1], zi * v);
+        }
+    }
+}
+
+// Per level and column: coef[2 (g C + c) + {0, 1}] = (alpha, gamma).  mode 0:
+// the level mean; 1: intercept and slope, the 2 x 2 solve when det > 1e-12
+// max(1, sw szz), else the mean; 2: slope only, szy / szz when
+// szz > 1e-12 max(1, szz), else 0 (:4186-4240).
+__global__ void __launch_bounds__(kBlock)
+slope_coef_kernel(const double* sums, const double* mom, int G, int C, int mode, double* coef) {
+    const long long total = static_cast<long long>(G) * C;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int g = static_cast<int>(j / C);
+        const double sw = mom[3 * g], sz = mom[3 * g + 1], szz = mom[3 * g + 2];
+        const double sv = sums[2 * j], szv = sums[2 * j + 1];
+        double alpha = 0.0, gamma = 0.0;
+        if (mode == 0) {
+            if (sw > 0.0) alpha = sv / sw;
+        } else if (mode == 1) {
+            if (sw > 0.0) {
+                const double det = sw * szz - sz * sz;
+                if (det > 1e-12 * fmax(1.0, sw * szz)) {
+                    alpha = (szz * sv - sz * szv) / det;
+                    gamma = (-sz * sv + sw * szv) / det;
+                } else {
+                    alpha = sv / sw;
+                }
+            }
+        } else if (szz > 1e-12 * fmax(1.0, szz)) {
+            gamma = szv / szz;
+        }
+        coef[2 * j] = alpha;
+        coef[2 * j + 1] = gamma;
+    }
+}
+
+// v -= alpha[g] + gamma[g] z (z may be null)
+__global__ void __launch_bounds__(kBlock)
+slope_apply_kernel(double* V, int ld, int C, int n, const int* gid, const double* z, const double* coef) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double zi = z ? z[i] : 0.0;
+        const double* cg = coef + 2 * static_cast<size_t>(gid[i]) * C;
+        for (int c = 0; c < C; ++c) V[static_cast<size_t>(c) * ld + i] -= cg[2 * c] + cg[2 * c + 1] * zi;
+    }
+}
+
+hipError_t launch_slope_moments(const int* gid, const double* z, const double* w, int n, double* mom, int grid,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(slope_moments_kernel, dim3(grid), dim3(kBlock), 0, s, gid, z, w, n, mom);
+    return hipGetLastError();
+}
+
+hipError_t launch_slope_reduce(const double* V, int ld, int C, int n, const int* gid, const double* z,
+                               const double* w, double* sums, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(slope_reduce_kernel, dim3(grid), dim3(kBlock), 0, s, V, ld, C, n, gid, z, w, sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_slope_coef(const double* sums, const double* mom, int G, int C, int mode, double* coef, int grid,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(slope_coef_kernel, dim3(grid), dim3(kBlock), 0, s, sums, mom, G, C, mode, coef);
+    return hipGetLastError();
+}
+
+hipError_t launch_slope_apply(double* V, int ld, int C, int n, const int* gid, const double* z, const double* coef,
+                              int grid, hipStream_t s) {
+    hipLaunchKernelGGL(slope_apply_kernel, dim3(grid), dim3(kBlock), 0, s, V, ld, C, n, gid, z, coef);
+    return hipGetLastError();
+}
+
 hipError_t launch_inverse(const double* ws, double* inv, int G, int grid, hipStream_t s) {
     hipLaunchKernelGGL(inverse_kernel, dim3(grid), dim3(kBlock), 0, s, ws, inv, G);
     return hipGetLastError();

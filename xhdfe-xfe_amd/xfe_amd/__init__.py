@@ -114,11 +114,12 @@ def _info(r, D):
 
 def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=1e-8,
            tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0, certify=True,
-           refine_tol=0.0):
+           refine_tol=0.0, slopes=None):
     """absorb_fixed_effects_cuda on host arrays: returns (y_tilde, X_tilde, info).
 
     gids: compact 0-based level ids per dimension; G: level counts.  ``method``
-    defaults to the reference's choice (SymmetricGaussSeidel for two FEs)."""
+    defaults to the reference's choice (SymmetricGaussSeidel for two FEs).
+    slopes: {dimension: (z, with_intercept)} heterogeneous slope terms."""
     y = np.ascontiguousarray(y, dtype=np.float64)
     X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
     n, k = X.shape
@@ -132,6 +133,12 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
         fes[d].num_groups = int(G[d])
         fes[d].num_levels_present = int(G[d])
         fes[d].weight_sums = None if ws[d] is None else ws[d].ctypes.data
+    zs = {}
+    for d, (z, icpt) in (slopes or {}).items():
+        zs[d] = np.ascontiguousarray(z, dtype=np.float64)
+        fes[d].is_slope = 1
+        fes[d].slope_has_intercept = int(bool(icpt))
+        fes[d].slope_values = zs[d].ctypes.data
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
     order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
     if method is None:
