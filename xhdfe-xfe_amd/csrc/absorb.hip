@@ -1787,9 +1787,12 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
 // src/fe_absorption.cpp:8162, with slope_project_inplace :4096-4300 for a slope
 // dimension): alternating projections in FP64 over the dimensions, a slope
 // dimension projecting each level onto span{1, z} (or span{z} without the
-// intercept).  Sweeps follow sweep_order (symmetric for two dimensions, as the
-// plain path) until every column's change over a sweep is at most
-// tol_eff max(1, ||v||).  Works on the caller's row order; V is n x C (ld).
+// intercept), accelerated by Irons-Tuck steps per column: from x, Gx and GGx
+// (two sweeps), x <- GGx - (<d2, dG> / <d2, d2>) dG with dG = GGx - Gx and
+// d2 = dG - (Gx - x).  Sweeps follow sweep_order (symmetric for two
+// dimensions, as the plain path); the solve stops when every column's change
+// over the last sweep is at most tol_eff max(1, ||v||).  Works on the caller's
+// row order; V is n x C (ld).
 static void absorb_slopes_impl(double* V, int n, int C, int ld, const std::vector<const int*>& gid,
                                const std::vector<int>& G, const std::vector<const double*>& z,
                                const std::vector<int>& mode, const double* w, const std::vector<int>& order,
@@ -1806,10 +1809,15 @@ static void absorb_slopes_impl(double* V, int n, int C, int ld, const std::vecto
         XFE_CHECK(launch_slope_moments(gid[d], z[d], w, n, ws.wsum[d].p, grid, s));
         res.sweep_order_used[d] = order[d];
     }
-    ws.vec[1].ensure(static_cast<size_t>(ld) * C);
-    double* old = ws.vec[1].p;
+    const size_t block = static_cast<size_t>(ld) * C;
+    for (int k = 1; k <= 4; ++k) ws.vec[k].ensure(block);
+    double* gx = ws.vec[1].p;
+    double* ggx = ws.vec[2].p;
+    double* dg = ws.vec[3].p;
+    double* d2 = ws.vec[4].p;
     ws.partials.ensure(static_cast<size_t>(grid) * C);
     ws.scalars.ensure(std::max(C, 8));
+    ws.lanes.ensure(std::max(C, 8));
     auto column_sums = [&](const double* a, const double* b, double* out) {
         XFE_CHECK(launch_col_dot(a, b, n, C, ld, ws.partials.p, grid, s));
         XFE_CHECK(launch_reduce_partials(ws.partials.p, grid, C, 0, ws.scalars.p, s));
@@ -1817,34 +1825,52 @@ static void absorb_slopes_impl(double* V, int n, int C, int ld, const std::vecto
         XFE_CHECK(hipStreamSynchronize(s));
     };
     const std::vector<int> seq = sweep_sequence(order, opt.method == 2 && D > 1);
-    const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
-    std::vector<double> dd(C), xx(C);
-    bool converged = false;
-    int it = 0;
-    while (it < opt.max_iter) {
-        ++it;
-        XFE_CHECK(hipMemcpyAsync(old, V, sizeof(double) * static_cast<size_t>(ld) * C, hipMemcpyDeviceToDevice, s));
+    auto sweep = [&](double* X) {
         for (int d : seq) {
             XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * 2 * static_cast<size_t>(G[d]) * C, s));
-            XFE_CHECK(launch_slope_reduce(V, ld, C, n, gid[d], z[d], w, ws.sums[d].p, grid, s));
+            XFE_CHECK(launch_slope_reduce(X, ld, C, n, gid[d], z[d], w, ws.sums[d].p, grid, s));
             XFE_CHECK(launch_slope_coef(ws.sums[d].p, ws.wsum[d].p, G[d], C, mode[d], ws.means[d].p,
                                         grid_for(static_cast<long long>(G[d]) * C), s));
-            XFE_CHECK(launch_slope_apply(V, ld, C, n, gid[d], z[d], ws.means[d].p, grid, s));
+            XFE_CHECK(launch_slope_apply(X, ld, C, n, gid[d], z[d], ws.means[d].p, grid, s));
         }
-        XFE_CHECK(launch_sub(old, V, old, n, C, ld, grid, s));
-        column_sums(old, old, dd.data());
-        column_sums(V, V, xx.data());
+    };
+    const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
+    std::vector<double> num(C), den(C), ch(C), nn(C), coef(C);
+    bool converged = false;
+    int it = 0, sweeps = 0;
+    while (sweeps < opt.max_iter) {
+        ++it;
+        XFE_CHECK(hipMemcpyAsync(gx, V, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
+        sweep(gx);
+        XFE_CHECK(hipMemcpyAsync(ggx, gx, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
+        sweep(ggx);
+        sweeps += 2;
+        XFE_CHECK(launch_sub(dg, ggx, gx, n, C, ld, grid, s));
+        XFE_CHECK(launch_sub(d2, gx, V, n, C, ld, grid, s));
+        XFE_CHECK(launch_sub(d2, dg, d2, n, C, ld, grid, s));
+        column_sums(dg, dg, ch.data());
+        column_sums(ggx, ggx, nn.data());
         bool done = true;
         for (int c = 0; c < C; ++c)
-            if (std::sqrt(std::max(0.0, dd[c])) > tol_eff * std::max(1.0, std::sqrt(std::max(0.0, xx[c])))) done = false;
+            if (std::sqrt(std::max(0.0, ch[c])) > tol_eff * std::max(1.0, std::sqrt(std::max(0.0, nn[c]))))
+                done = false;
         if (done) {
+            XFE_CHECK(hipMemcpyAsync(V, ggx, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
             converged = true;
             break;
         }
+        column_sums(d2, dg, num.data());
+        column_sums(d2, d2, den.data());
+        for (int c = 0; c < C; ++c) coef[c] = den[c] > 0.0 ? num[c] / den[c] : 0.0;
+        XFE_CHECK(hipMemcpyAsync(ws.lanes.p, coef.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+        XFE_CHECK(launch_col_axpy(V, ggx, dg, ws.lanes.p, n, C, ld, -1.0, grid, s));
     }
+    if (!converged) XFE_CHECK(hipMemcpyAsync(V, ggx, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
+    XFE_CHECK(hipStreamSynchronize(s));
     res.iterations = it;
     res.converged = converged ? 1 : 0;
-    res.sweeps = it;
+    res.used_accel = 1;
+    res.sweeps = sweeps;
 }
 
 }  // namespace xfe
