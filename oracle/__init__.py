@@ -17,6 +17,13 @@ under ``xhdfe-xfe_amd/``) never imports it and never falls back to it.
 * ``certify``: numpy restatement of the normal-equation certificate
   (src/fe_absorption.cpp:14942-15190); pinned against the reference's own
   certificate (``ref_certify``) and a dense ||D'W v|| on tiny cases.
+* ``schwarz_rule``: the stopping rule of schwarz_demean_raw
+  (src/schwarz_demean.cpp:283-378) evaluated on a result; pinned against the
+  reference's own convergence decisions (``ref_schwarz``).
+* ``slope_projection``: the exact within transform with a heterogeneous slope
+  term (the fixed point of slope_project_inplace, src/fe_absorption.cpp:
+  4096-4300) by dense least squares; pinned against the reference's slope
+  absorber (``ref_absorb_slope``) and per-level polyfit.
 """
 from __future__ import annotations
 
