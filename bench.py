@@ -186,6 +186,64 @@ def bench_akm(args):
     print(json.dumps(out), flush=True)
 
 
+SLOPE_METRIC = "s to absorb a heterogeneous-slope panel (firm intercept + slope on x1, worker, firm x year) at 1e-8"
+
+
+def bench_slope(args):
+    """The heterogeneous-slope path (xfe_absorb with a slope dimension) on the
+    slope shape at --scale of 30M obs (default 0.1): singletons dropped on the
+    device, then the host entry point (upload, absorb, download) timed."""
+    import torch  # noqa: F401  (shares torch's HIP runtime)
+
+    import xfe_amd
+    from xfe_amd import panels
+
+    if not xfe_amd.backend_available():
+        raise SystemExit("no HIP device")
+    scale = args.scale if args.scale < 1.0 else 0.1
+    t0 = time.perf_counter()
+    p = panels.slope_panel(scale=scale)
+    keep, dropped, _ = xfe_amd.drop_singletons([f.astype(np.int32) for f in p["fes"]])
+    comp = [compact(f[keep]) for f in p["fes"]]
+    gids = [c[0] for c in comp]
+    G = [c[1] for c in comp]
+    y, X, z = p["y"][keep], np.asfortranarray(p["X"][keep]), p["z"][keep]
+    n, C = y.size, X.shape[1] + 1
+    del p
+    log(f"slope panel: n={n} C={C} levels={G} ({dropped} singletons) in {time.perf_counter() - t0:.1f} s")
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        _, _, info = xfe_amd.absorb(y, X, gids, G, tol=1e-8, slopes={1: (z, True)})
+        log(f"warmup {w}: {time.perf_counter() - t0:.3f} s, steps {info['iterations']}, sweeps {info['sweeps']}")
+    xfe_amd.profile_reset()
+    xfe_amd.profile_enable(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        _, _, info = xfe_amd.absorb(y, X, gids, G, tol=1e-8, slopes={1: (z, True)})
+    sec = (time.perf_counter() - t0) / args.steps
+    xfe_amd.profile_enable(False)
+    pass_ms, launches, pass_bytes = xfe_amd.profile_get()
+    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
+    log(f"{sec:.3f} s per absorption; slope_reduce_kernel: {launches} launches, "
+        f"{pass_ms / max(launches, 1):.3f} ms avg, {achieved:.0f} GB/s algorithmic")
+    out = {
+        "metric": SLOPE_METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "method": "FP64 alternating projections with per-column Irons-Tuck steps (host entry: PCIe included)",
+        "config": {"workload": f"slope_{scale:g}x30m", "obs": int(n), "columns": C, "levels": G,
+                   "slope": "firm intercept + slope", "tol": 1e-8, "singletons_dropped": int(dropped)},
+        "iterations": info["iterations"], "sweeps": info["sweeps"], "converged": info["converged"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "slope_reduce_kernel (level sums of w v and w z v, FP64 atomics)",
+                     "avg_launch_ms": round(pass_ms / max(launches, 1), 4),
+                     "bytes_per_launch": round(pass_bytes / max(launches, 1), 1)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,8 +253,9 @@ def main():
     ap.add_argument("--precision", type=int, default=1, help="1 FP32 under FP64 refinement, 0 FP64")
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain FP64 absorber")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--path", choices=("absorb", "akm"), default="absorb",
-                    help="absorb: the 56M-obs absorber (default); akm: the KSS leverage solver")
+    ap.add_argument("--path", choices=("absorb", "akm", "slope"), default="absorb",
+                    help="absorb: the 56M-obs absorber (default); akm: the KSS leverage solver; "
+                         "slope: the heterogeneous-slope path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -204,6 +263,9 @@ def main():
         sys.exit(2)
     if args.path == "akm":
         bench_akm(args)
+        return
+    if args.path == "slope":
+        bench_slope(args)
         return
 
     import torch

@@ -1785,14 +1785,17 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
 
 // Absorption with heterogeneous slopes (absorb_fixed_effects_v6_mixed,
 // src/fe_absorption.cpp:8162, with slope_project_inplace :4096-4300 for a slope
-// dimension): alternating projections in FP64 over the dimensions, a slope
-// dimension projecting each level onto span{1, z} (or span{z} without the
-// intercept), accelerated by Irons-Tuck steps per column: from x, Gx and GGx
-// (two sweeps), x <- GGx - (<d2, dG> / <d2, d2>) dG with dG = GGx - Gx and
-// d2 = dG - (Gx - x).  Sweeps follow sweep_order (symmetric for two
-// dimensions, as the plain path); the solve stops when every column's change
-// over the last sweep is at most tol_eff max(1, ||v||).  Works on the caller's
-// row order; V is n x C (ld).
+// dimension), in FP64.  Every dimension's map v -> v - P_d v is a W-orthogonal
+// projection (a slope dimension projects each level onto span{1, z}, or span{z}
+// without the intercept), so the symmetric sweep T = (I - P_1)...(I - P_D)...
+// (I - P_1) is self-adjoint and the within transform is v - y with y the
+// solution of (I - T) y = (I - T) v in the span of the fixed effects: CG on
+// that system per column (the reference's accelerator for slow alternating
+// projections, src/fe_absorption.cpp:6790-6973), from y = 0, in the W inner
+// product.  The residual r = (I - T)(v - y) is the change one symmetric sweep
+// would make to the current transform; the solve stops when every column has
+// ||r||_W <= tol_eff max(1, ||v||_W).  Works on the caller's row order; V is
+// n x C (ld).
 static void absorb_slopes_impl(double* V, int n, int C, int ld, const std::vector<const int*>& gid,
                                const std::vector<int>& G, const std::vector<const double*>& z,
                                const std::vector<int>& mode, const double* w, const std::vector<int>& order,
@@ -1801,75 +1804,105 @@ static void absorb_slopes_impl(double* V, int n, int C, int ld, const std::vecto
     hipStream_t s = ws.stream;
     const int D = static_cast<int>(G.size());
     const int grid = grid_for(n);
+    // copies of each level-sum table: a level's rows spread their atomics over
+    // up to 8 copies (at most 64 MB per table)
+    std::vector<int> copies(D);
     for (int d = 0; d < D; ++d) {
+        const size_t tab = 2 * static_cast<size_t>(G[d]) * C;
+        copies[d] = static_cast<int>(std::max<size_t>(1, std::min<size_t>(8, (size_t(8) << 20) / tab)));
         ws.wsum[d].ensure(3 * static_cast<size_t>(G[d]));
-        ws.sums[d].ensure(2 * static_cast<size_t>(G[d]) * C);
-        ws.means[d].ensure(2 * static_cast<size_t>(G[d]) * C);
+        ws.sums[d].ensure(tab * copies[d]);
+        ws.means[d].ensure(tab);
         XFE_CHECK(hipMemsetAsync(ws.wsum[d].p, 0, sizeof(double) * 3 * G[d], s));
         XFE_CHECK(launch_slope_moments(gid[d], z[d], w, n, ws.wsum[d].p, grid, s));
         res.sweep_order_used[d] = order[d];
     }
     const size_t block = static_cast<size_t>(ld) * C;
-    for (int k = 1; k <= 4; ++k) ws.vec[k].ensure(block);
-    double* gx = ws.vec[1].p;
-    double* ggx = ws.vec[2].p;
-    double* dg = ws.vec[3].p;
-    double* d2 = ws.vec[4].p;
+    for (int k = 1; k <= 5; ++k) ws.vec[k].ensure(block);
+    double* y = ws.vec[1].p;
+    double* r = ws.vec[2].p;
+    double* pv = ws.vec[3].p;
+    double* q = ws.vec[4].p;
+    double* t = ws.vec[5].p;
     ws.partials.ensure(static_cast<size_t>(grid) * C);
     ws.scalars.ensure(std::max(C, 8));
     ws.lanes.ensure(std::max(C, 8));
-    auto column_sums = [&](const double* a, const double* b, double* out) {
-        XFE_CHECK(launch_col_dot(a, b, n, C, ld, ws.partials.p, grid, s));
+    auto wdot = [&](const double* a, const double* b, double* out) {
+        XFE_CHECK(launch_slope_wdot(a, b, w, n, C, ld, ws.partials.p, grid, s));
         XFE_CHECK(launch_reduce_partials(ws.partials.p, grid, C, 0, ws.scalars.p, s));
         XFE_CHECK(hipMemcpyAsync(out, ws.scalars.p, sizeof(double) * C, hipMemcpyDeviceToHost, s));
         XFE_CHECK(hipStreamSynchronize(s));
     };
-    const std::vector<int> seq = sweep_sequence(order, opt.method == 2 && D > 1);
-    auto sweep = [&](double* X) {
+    auto upload = [&](const std::vector<double>& h) {
+        XFE_CHECK(hipMemcpyAsync(ws.lanes.p, h.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+    };
+    const std::vector<int> seq = sweep_sequence(order, true);
+    int sweeps = 0;
+    // out = (I - T) x, through t
+    auto apply_A = [&](const double* x, double* out) {
+        XFE_CHECK(hipMemcpyAsync(t, x, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
         for (int d : seq) {
-            XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * 2 * static_cast<size_t>(G[d]) * C, s));
-            XFE_CHECK(launch_slope_reduce(X, ld, C, n, gid[d], z[d], w, ws.sums[d].p, grid, s));
-            XFE_CHECK(launch_slope_coef(ws.sums[d].p, ws.wsum[d].p, G[d], C, mode[d], ws.means[d].p,
+            XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0,
+                                     sizeof(double) * 2 * static_cast<size_t>(G[d]) * C * copies[d], s));
+            if (g_profile.enabled && z[d]) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            XFE_CHECK(launch_slope_reduce(t, ld, C, n, gid[d], z[d], w, ws.sums[d].p, G[d], copies[d], grid, s));
+            if (g_profile.enabled && z[d]) {
+                // the slope dimension's reduce: ids, z, (w,) C values per row, and the
+                // 2 C level sums of each row's level
+                XFE_CHECK(hipEventRecord(ws.ev1, s));
+                XFE_CHECK(hipEventSynchronize(ws.ev1));
+                float ms = 0.f;
+                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+                g_profile.ms += ms;
+                g_profile.launches += 1;
+                g_profile.bytes += static_cast<double>(n) * (12.0 + (w ? 8.0 : 0.0) + 8.0 * C) +
+                                   16.0 * static_cast<double>(G[d]) * C;
+            }
+            XFE_CHECK(launch_slope_coef(ws.sums[d].p, ws.wsum[d].p, G[d], C, mode[d], copies[d], ws.means[d].p,
                                         grid_for(static_cast<long long>(G[d]) * C), s));
-            XFE_CHECK(launch_slope_apply(X, ld, C, n, gid[d], z[d], ws.means[d].p, grid, s));
+            XFE_CHECK(launch_slope_apply(t, ld, C, n, gid[d], z[d], ws.means[d].p, grid, s));
         }
+        ++sweeps;
+        XFE_CHECK(launch_sub(out, x, t, n, C, ld, grid, s));
     };
     const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
-    std::vector<double> num(C), den(C), ch(C), nn(C), coef(C);
+    std::vector<double> vv(C), rr(C), rr_new(C), pq(C), alpha(C), beta(C), limit(C);
+    wdot(V, V, vv.data());
+    for (int c = 0; c < C; ++c) limit[c] = tol_eff * std::max(1.0, std::sqrt(std::max(0.0, vv[c])));
+    // y = 0, r = p = (I - T) v
+    XFE_CHECK(hipMemsetAsync(y, 0, sizeof(double) * block, s));
+    apply_A(V, r);
+    XFE_CHECK(hipMemcpyAsync(pv, r, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
+    wdot(r, r, rr.data());
     bool converged = false;
-    int it = 0, sweeps = 0;
-    while (sweeps < opt.max_iter) {
-        ++it;
-        XFE_CHECK(hipMemcpyAsync(gx, V, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
-        sweep(gx);
-        XFE_CHECK(hipMemcpyAsync(ggx, gx, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
-        sweep(ggx);
-        sweeps += 2;
-        XFE_CHECK(launch_sub(dg, ggx, gx, n, C, ld, grid, s));
-        XFE_CHECK(launch_sub(d2, gx, V, n, C, ld, grid, s));
-        XFE_CHECK(launch_sub(d2, dg, d2, n, C, ld, grid, s));
-        column_sums(dg, dg, ch.data());
-        column_sums(ggx, ggx, nn.data());
+    int it = 0;
+    while (true) {
         bool done = true;
         for (int c = 0; c < C; ++c)
-            if (std::sqrt(std::max(0.0, ch[c])) > tol_eff * std::max(1.0, std::sqrt(std::max(0.0, nn[c]))))
-                done = false;
+            if (std::sqrt(std::max(0.0, rr[c])) > limit[c]) done = false;
         if (done) {
-            XFE_CHECK(hipMemcpyAsync(V, ggx, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
             converged = true;
             break;
         }
-        column_sums(d2, dg, num.data());
-        column_sums(d2, d2, den.data());
-        for (int c = 0; c < C; ++c) coef[c] = den[c] > 0.0 ? num[c] / den[c] : 0.0;
-        XFE_CHECK(hipMemcpyAsync(ws.lanes.p, coef.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
-        XFE_CHECK(launch_col_axpy(V, ggx, dg, ws.lanes.p, n, C, ld, -1.0, grid, s));
+        if (it >= opt.max_iter) break;
+        ++it;
+        apply_A(pv, q);
+        wdot(pv, q, pq.data());
+        for (int c = 0; c < C; ++c) alpha[c] = (pq[c] > 0.0 && rr[c] > 0.0) ? rr[c] / pq[c] : 0.0;
+        upload(alpha);
+        XFE_CHECK(launch_col_axpy(y, y, pv, ws.lanes.p, n, C, ld, 1.0, grid, s));
+        XFE_CHECK(launch_col_axpy(r, r, q, ws.lanes.p, n, C, ld, -1.0, grid, s));
+        wdot(r, r, rr_new.data());
+        for (int c = 0; c < C; ++c) beta[c] = rr[c] > 0.0 ? rr_new[c] / rr[c] : 0.0;
+        upload(beta);
+        XFE_CHECK(launch_col_axpy(pv, r, pv, ws.lanes.p, n, C, ld, 1.0, grid, s));
+        rr = rr_new;
     }
-    if (!converged) XFE_CHECK(hipMemcpyAsync(V, ggx, sizeof(double) * block, hipMemcpyDeviceToDevice, s));
+    XFE_CHECK(launch_sub(V, V, y, n, C, ld, grid, s));
     XFE_CHECK(hipStreamSynchronize(s));
     res.iterations = it;
     res.converged = converged ? 1 : 0;
-    res.used_accel = 1;
+    res.used_cg = 1;
     res.sweeps = sweeps;
 }
 

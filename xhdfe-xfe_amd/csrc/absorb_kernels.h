@@ -234,12 +234,14 @@ hipError_t launch_sub(double* y, const double* a, const double* b, int n, int C,
                       hipStream_t s);
 // heterogeneous slopes: level moments, level sums of w v and w z v, the
 // per-level (alpha, gamma) with the reference's rank rule, and the apply
+hipError_t launch_slope_wdot(const double* a, const double* b, const double* w, int n, int C, int ld,
+                             double* partials, int grid, hipStream_t s);
 hipError_t launch_slope_moments(const int* gid, const double* z, const double* w, int n, double* mom, int grid,
                                 hipStream_t s);
 hipError_t launch_slope_reduce(const double* V, int ld, int C, int n, const int* gid, const double* z,
-                               const double* w, double* sums, int grid, hipStream_t s);
-hipError_t launch_slope_coef(const double* sums, const double* mom, int G, int C, int mode, double* coef, int grid,
-                             hipStream_t s);
+                               const double* w, double* sums, int G, int copies, int grid, hipStream_t s);
+hipError_t launch_slope_coef(const double* sums, const double* mom, int G, int C, int mode, int copies,
+                             double* coef, int grid, hipStream_t s);
 hipError_t launch_slope_apply(double* V, int ld, int C, int n, const int* gid, const double* z, const double* coef,
                               int grid, hipStream_t s);
 hipError_t launch_inverse(const double* ws, double* inv, int G, int grid, hipStream_t s);

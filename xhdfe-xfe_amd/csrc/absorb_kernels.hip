@@ -498,14 +498,17 @@ slope_moments_kernel(const int* gid, const double* z, const double* w, int n, do
     }
 }
 
-// sums[2 (g C + c) + {0, 1}] += w v, w z v (the second only with z)
+// sums[k][2 (g C + c) + {0, 1}] += w v, w z v (the second only with z), into
+// copy k = block % copies of the G x C x 2 table (spreads the atomics on a
+// level's few addresses over the copies)
 __global__ void __launch_bounds__(kBlock)
 slope_reduce_kernel(const double* V, int ld, int C, int n, const int* gid, const double* z, const double* w,
-                    double* sums) {
+                    double* sums, int G, int copies) {
+    double* tab = sums + static_cast<size_t>(blockIdx.x % copies) * 2 * static_cast<size_t>(G) * C;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const double wi = w ? w[i] : 1.0;
         const double zi = z ? z[i] : 0.0;
-        double* sg = sums + 2 * static_cast<size_t>(gid[i]) * C;
+        double* sg = tab + 2 * static_cast<size_t>(gid[i]) * C;
         for (int c = 0; c < C; ++c) {
             const double v = wi * V[static_cast<size_t>(c) * ld + i];
             atomicAdd(&sg[2 * c], v);
@@ -519,13 +522,17 @@ slope_reduce_kernel(const double* V, int ld, int C, int n, const int* gid, const
 // max(1, sw szz), else the mean; 2: slope only, szy / szz when
 // szz > 1e-12 max(1, szz), else 0 (:4186-4240).
 __global__ void __launch_bounds__(kBlock)
-slope_coef_kernel(const double* sums, const double* mom, int G, int C, int mode, double* coef) {
+slope_coef_kernel(const double* sums, const double* mom, int G, int C, int mode, int copies, double* coef) {
     const long long total = static_cast<long long>(G) * C;
     for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
          j += static_cast<long long>(gridDim.x) * kBlock) {
         const int g = static_cast<int>(j / C);
         const double sw = mom[3 * g], sz = mom[3 * g + 1], szz = mom[3 * g + 2];
-        const double sv = sums[2 * j], szv = sums[2 * j + 1];
+        double sv = 0.0, szv = 0.0;
+        for (int k = 0; k < copies; ++k) {
+            sv += sums[static_cast<size_t>(k) * 2 * total + 2 * j];
+            szv += sums[static_cast<size_t>(k) * 2 * total + 2 * j + 1];
+        }
         double alpha = 0.0, gamma = 0.0;
         if (mode == 0) {
             if (sw > 0.0) alpha = sv / sw;
@@ -557,6 +564,34 @@ slope_apply_kernel(double* V, int ld, int C, int n, const int* gid, const double
     }
 }
 
+// partials[block * C + c] = sum_i w_i a_ic b_ic over the block's rows (w may be null)
+__global__ void __launch_bounds__(kBlock)
+slope_wdot_kernel(const double* a, const double* b, const double* w, int n, int C, int ld, double* partials) {
+    __shared__ double red[kBlock / kWave];
+    for (int c = 0; c < C; ++c) {
+        double acc = 0.0;
+        const double* ac = a + static_cast<size_t>(c) * ld;
+        const double* bc = b + static_cast<size_t>(c) * ld;
+        for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+            acc += (w ? w[i] : 1.0) * ac[i] * bc[i];
+        const double s = wave_sum(acc);
+        if (lane_id() == 0) red[threadIdx.x / kWave] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 0.0;
+            for (int wv = 0; wv < kBlock / kWave; ++wv) r += red[wv];
+            partials[static_cast<size_t>(blockIdx.x) * C + c] = r;
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_slope_wdot(const double* a, const double* b, const double* w, int n, int C, int ld,
+                             double* partials, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(slope_wdot_kernel, dim3(grid), dim3(kBlock), 0, s, a, b, w, n, C, ld, partials);
+    return hipGetLastError();
+}
+
 hipError_t launch_slope_moments(const int* gid, const double* z, const double* w, int n, double* mom, int grid,
                                 hipStream_t s) {
     hipLaunchKernelGGL(slope_moments_kernel, dim3(grid), dim3(kBlock), 0, s, gid, z, w, n, mom);
@@ -564,14 +599,15 @@ hipError_t launch_slope_moments(const int* gid, const double* z, const double* w
 }
 
 hipError_t launch_slope_reduce(const double* V, int ld, int C, int n, const int* gid, const double* z,
-                               const double* w, double* sums, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(slope_reduce_kernel, dim3(grid), dim3(kBlock), 0, s, V, ld, C, n, gid, z, w, sums);
+                               const double* w, double* sums, int G, int copies, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(slope_reduce_kernel, dim3(grid), dim3(kBlock), 0, s, V, ld, C, n, gid, z, w, sums, G,
+                       copies);
     return hipGetLastError();
 }
 
-hipError_t launch_slope_coef(const double* sums, const double* mom, int G, int C, int mode, double* coef, int grid,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(slope_coef_kernel, dim3(grid), dim3(kBlock), 0, s, sums, mom, G, C, mode, coef);
+hipError_t launch_slope_coef(const double* sums, const double* mom, int G, int C, int mode, int copies,
+                             double* coef, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(slope_coef_kernel, dim3(grid), dim3(kBlock), 0, s, sums, mom, G, C, mode, copies, coef);
     return hipGetLastError();
 }
 

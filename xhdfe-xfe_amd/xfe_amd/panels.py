@@ -115,3 +115,24 @@ def kss_panel(seed=SEED0 + 300, workers=1_200_000, firms=150_000, years=36, tenu
     rng = np.random.default_rng(seed)
     worker, firm, _, _ = _akm_ids(rng, workers, firms, years, tenure_mean, pareto_alpha, p_move)
     return dict(worker=worker, firm=firm, name=f"kss_{worker.size}obs_{workers}w_{firms}f")
+
+
+def slope_panel(seed=SEED0 + 200, scale=0.1, regressors=31, p_move=0.08):
+    """The heterogeneous-slope shape at `scale` of 30M obs: 3M workers and
+    400k firms (Pareto 1.1) times scale, geometric tenure of mean 10 capped at
+    36 years, mover probability 0.08; FE = (worker, firm with a slope on x1,
+    firm x year); y and `regressors` N(0, 1) columns besides x1."""
+    rng = np.random.default_rng(seed)
+    workers = max(10, int(3_000_000 * scale))
+    firms = max(5, int(400_000 * scale))
+    worker, firm, year, _ = _akm_ids(rng, workers, firms, 36, 10.0, 1.1, p_move)
+    n = worker.size
+    x1 = rng.standard_normal(n)
+    cell = firm.astype(np.int64) * 36 + year
+    X = np.empty((n, regressors), order="F")
+    for j in range(regressors):
+        X[:, j] = rng.standard_normal(n)
+    psi = rng.normal(0.0, 0.2, firms)
+    slope = rng.normal(0.5, 0.1, firms)
+    y = X[:, 0] * 0.3 + psi[firm] + slope[firm] * x1 + rng.normal(0.0, 0.25, n)
+    return dict(y=y, X=X, z=x1, fes=[worker, firm, cell], name=f"slope_{n}obs_{workers}w_{firms}f")
