@@ -230,7 +230,7 @@ def bench_slope(args):
         "metric": SLOPE_METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "method": "FP64 alternating projections with per-column Irons-Tuck steps (host entry: PCIe included)",
+        "method": "FP64 CG on the symmetric projection sweep, rows in slope-level order (host entry: PCIe included)",
         "config": {"workload": f"slope_{scale:g}x30m", "obs": int(n), "columns": C, "levels": G,
                    "slope": "firm intercept + slope", "tol": 1e-8, "singletons_dropped": int(dropped)},
         "iterations": info["iterations"], "sweeps": info["sweeps"], "converged": info["converged"],

@@ -69,6 +69,7 @@ struct Workspace {
     DevBuf<int> stage_ids;
     DevBuf<double> stage_w;
     DevBuf<double> stage_z[XFE_MAX_DIMS];  // host-API staging of slope variables
+    DevBuf<double> slope_z[XFE_MAX_DIMS];  // slope variables in the slope path's row order
     // level-aligned tiles of the sorted dimension and spell structures
     DevBuf<int> tile_start, level_count, tmp_flags, tmp_inc, tmp_iota, tmp_key;
     DevBuf<int> sp_key[XFE_MAX_DIMS], sp_order[XFE_MAX_DIMS], sp_base[XFE_MAX_DIMS];
@@ -102,7 +103,7 @@ struct Workspace {
         tmp_iota.release(); tmp_key.release();
         for (int d = 0; d < XFE_MAX_DIMS; ++d) {
             sp_key[d].release(); sp_order[d].release(); sp_base[d].release(); sp_sums[d].release();
-            f_sp[d].release(); stage_z[d].release();
+            f_sp[d].release(); stage_z[d].release(); slope_z[d].release();
         }
         cg.release(); cg_part.release(); f_beta.release(); f_b.release(); f_diag.release(); f_counts.release(); f_scale.release();
         f_x.release(); f_r.release(); f_z.release(); f_p.release(); f_q.release(); f_minv.release();
@@ -1794,16 +1795,60 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
 // projections, src/fe_absorption.cpp:6790-6973), from y = 0, in the W inner
 // product.  The residual r = (I - T)(v - y) is the change one symmetric sweep
 // would make to the current transform; the solve stops when every column has
-// ||r||_W <= tol_eff max(1, ||v||_W).  Works on the caller's row order; V is
-// n x C (ld).
-static void absorb_slopes_impl(double* V, int n, int C, int ld, const std::vector<const int*>& gid,
-                               const std::vector<int>& G, const std::vector<const double*>& z,
-                               const std::vector<int>& mode, const double* w, const std::vector<int>& order,
+// ||r||_W <= tol_eff max(1, ||v||_W).  V is n x C (ld), in the caller's row
+// order on entry and exit.
+static void absorb_slopes_impl(double* V_in, int n, int C, int ld, const std::vector<const int*>& gid_in,
+                               const std::vector<int>& G, const std::vector<const double*>& z_in,
+                               const std::vector<int>& mode, const double* w_in, const std::vector<int>& order,
                                const xfe_options& opt, xfe_absorb_result& res) {
     Workspace& ws = tls_ws;
     hipStream_t s = ws.stream;
     const int D = static_cast<int>(G.size());
     const int grid = grid_for(n);
+    // Rows are put in the order of the (first) slope dimension's levels (a
+    // stable radix sort), so its level sums reduce runs of rows inside the
+    // waves; the result is scattered back at the end.
+    int sd = -1;
+    for (int d = 0; d < D && sd < 0; ++d)
+        if (z_in[d]) sd = d;
+    double* V = V_in;
+    std::vector<const int*> gid = gid_in;
+    std::vector<const double*> z = z_in;
+    const double* w = w_in;
+    const int* perm = nullptr;
+    if (sd >= 0) {
+        ws.perm.ensure(n);
+        ws.keys_tmp.ensure(n);
+        ws.vals_tmp.ensure(n);
+        XFE_CHECK(launch_iota(ws.vals_tmp.p, n, grid, s));
+        int bits = 1;
+        while ((1LL << bits) < static_cast<long long>(G[sd])) ++bits;
+        size_t tmp_bytes = 0;
+        XFE_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, gid_in[sd], ws.keys_tmp.p, ws.vals_tmp.p, ws.perm.p,
+                                            static_cast<size_t>(n), 0, bits, s));
+        ws.sort_tmp.ensure(tmp_bytes);
+        XFE_CHECK(rocprim::radix_sort_pairs(ws.sort_tmp.p, tmp_bytes, gid_in[sd], ws.keys_tmp.p, ws.vals_tmp.p,
+                                            ws.perm.p, static_cast<size_t>(n), 0, bits, s));
+        perm = ws.perm.p;
+        ws.vec[0].ensure(static_cast<size_t>(ld) * C);
+        V = ws.vec[0].p;
+        XFE_CHECK(launch_gather_cols(V_in, ld, perm, V, ld, n, C, grid, s));
+        for (int d = 0; d < D; ++d) {
+            ws.gid[d].ensure(n);
+            XFE_CHECK(launch_gather_ids(gid_in[d], perm, ws.gid[d].p, n, grid, s));
+            gid[d] = ws.gid[d].p;
+            if (z_in[d]) {
+                ws.slope_z[d].ensure(n);
+                XFE_CHECK(launch_gather_vec(z_in[d], perm, ws.slope_z[d].p, n, grid, s));
+                z[d] = ws.slope_z[d].p;
+            }
+        }
+        if (w_in) {
+            ws.w.ensure(n);
+            XFE_CHECK(launch_gather_vec(w_in, perm, ws.w.p, n, grid, s));
+            w = ws.w.p;
+        }
+    }
     // copies of each level-sum table: a level's rows spread their atomics over
     // up to 8 copies (at most 64 MB per table)
     std::vector<int> copies(D);
@@ -1899,6 +1944,7 @@ static void absorb_slopes_impl(double* V, int n, int C, int ld, const std::vecto
         rr = rr_new;
     }
     XFE_CHECK(launch_sub(V, V, y, n, C, ld, grid, s));
+    if (perm) XFE_CHECK(launch_scatter_cols(V, ld, perm, V_in, ld, n, C, grid, s));
     XFE_CHECK(hipStreamSynchronize(s));
     res.iterations = it;
     res.converged = converged ? 1 : 0;

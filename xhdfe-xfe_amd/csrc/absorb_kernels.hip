@@ -499,20 +499,32 @@ slope_moments_kernel(const int* gid, const double* z, const double* w, int n, do
 }
 
 // sums[k][2 (g C + c) + {0, 1}] += w v, w z v (the second only with z), into
-// copy k = block % copies of the G x C x 2 table (spreads the atomics on a
-// level's few addresses over the copies)
+// copy k = block % copies of the G x C x 2 table.  Consecutive rows of equal
+// level are summed in the wave first (segmented sums over lanes), so a run
+// costs one atomic per value: with the rows sorted by this dimension a level
+// of r rows takes about r / 64 atomics per column instead of r.
 __global__ void __launch_bounds__(kBlock)
 slope_reduce_kernel(const double* V, int ld, int C, int n, const int* gid, const double* z, const double* w,
                     double* sums, int G, int copies) {
     double* tab = sums + static_cast<size_t>(blockIdx.x % copies) * 2 * static_cast<size_t>(G) * C;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const double wi = w ? w[i] : 1.0;
-        const double zi = z ? z[i] : 0.0;
-        double* sg = tab + 2 * static_cast<size_t>(gid[i]) * C;
+    const int lane = lane_id();
+    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const int i = base + threadIdx.x;
+        const bool valid = i < n;
+        const int key = valid ? gid[i] : -1 - lane;
+        const int end = run_end_lane(key);
+        const bool head = run_head(key) && valid;
+        const double wi = valid ? (w ? w[i] : 1.0) : 0.0;
+        const double zi = (valid && z) ? z[i] : 0.0;
+        double* sg = tab + 2 * static_cast<size_t>(valid ? key : 0) * C;
         for (int c = 0; c < C; ++c) {
-            const double v = wi * V[static_cast<size_t>(c) * ld + i];
-            atomicAdd(&sg[2 * c], v);
-            if (z) atomicAdd(&sg[2 * c + 1], zi * v);
+            const double v = valid ? wi * V[static_cast<size_t>(c) * ld + i] : 0.0;
+            const double s1 = run_sum(v, end);
+            if (head) atomicAdd(&sg[2 * c], s1);
+            if (z) {
+                const double s2 = run_sum(zi * v, end);
+                if (head) atomicAdd(&sg[2 * c + 1], s2);
+            }
         }
     }
 }
