@@ -506,10 +506,11 @@ def test_schwarz_edge_cases(golden):
 
 
 # ------------------------------------------------------------ heterogeneous slopes
-# The slope path runs FP64 alternating projections to a per-sweep change of
-# tol_eff max(1, ||v||); it is compared with the exact projection (oracle) and
-# with the reference's absorber (fixtures at tol 1e-12, themselves 9e-9 of the
-# column scale from the projection on these slowly converging panels).
+# The slope path runs FP64 CG on the symmetric projection sweep until a sweep
+# would change the transform by at most tol_eff max(1, ||v||); it is compared
+# with the exact projection (oracle) and with the reference's absorber
+# (fixtures at tol 1e-12, themselves 9e-9 of the column scale from the
+# projection on these slowly converging panels).
 
 @pytest.mark.parametrize("name,sdim,icpt", [("two_way", 1, True), ("two_way", 1, False),
                                              ("akm_three_way", 1, True)])
@@ -532,7 +533,7 @@ def test_absorb_slopes_matches_oracle(golden, cases, name, sdim, icpt):
 
 
 def test_absorb_slope_single_dimension_weighted():
-    """One weighted slope dimension is exact after one sweep: each level's
+    """One weighted slope dimension is exact after one CG step: each level's
     weighted least-squares line of v on z."""
     _log("slopes single dimension weighted")
     rng = np.random.default_rng(11)
