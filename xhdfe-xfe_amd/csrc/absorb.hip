@@ -37,6 +37,44 @@ namespace xfe {
 Profile g_profile;
 
 namespace {
+std::vector<hipEvent_t> g_prof_events;  // 2 per slot, reused
+std::vector<double> g_prof_bytes;
+size_t g_prof_used = 0;
+}  // namespace
+
+int profile_begin(hipStream_t s) {
+    if (!g_profile.enabled) return -1;
+    if (2 * (g_prof_used + 1) > g_prof_events.size()) {
+        for (int k = 0; k < 2; ++k) {
+            hipEvent_t e = nullptr;
+            XFE_CHECK(hipEventCreate(&e));
+            g_prof_events.push_back(e);
+        }
+        g_prof_bytes.push_back(0.0);
+    }
+    XFE_CHECK(hipEventRecord(g_prof_events[2 * g_prof_used], s));
+    return static_cast<int>(g_prof_used++);
+}
+
+void profile_end(int slot, hipStream_t s, double bytes) {
+    if (slot < 0) return;
+    XFE_CHECK(hipEventRecord(g_prof_events[2 * static_cast<size_t>(slot) + 1], s));
+    g_prof_bytes[static_cast<size_t>(slot)] = bytes;
+}
+
+void profile_resolve() {
+    for (size_t i = 0; i < g_prof_used; ++i) {
+        XFE_CHECK(hipEventSynchronize(g_prof_events[2 * i + 1]));
+        float ms = 0.f;
+        XFE_CHECK(hipEventElapsedTime(&ms, g_prof_events[2 * i], g_prof_events[2 * i + 1]));
+        g_profile.ms += ms;
+        g_profile.launches += 1;
+        g_profile.bytes += g_prof_bytes[i];
+    }
+    g_prof_used = 0;
+}
+
+namespace {
 
 constexpr int kMaxGrid = 2048;
 constexpr size_t kLdsBudget = 32 * 1024;  // bytes of LDS a privatised reduce may use
@@ -87,7 +125,6 @@ struct Workspace {
     DevBuf<float> f_bv[2];
     DevBuf<double> d_bv[2];
     double* host_scalars = nullptr;  // pinned
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     // Frees every device resource of this thread's workspace.
     void release() {
@@ -117,10 +154,8 @@ struct Workspace {
         f_invw.release(); d_invw.release();
         for (int k = 0; k < 2; ++k) { seg_bk[k].release(); f_bv[k].release(); d_bv[k].release(); }
         if (host_scalars) (void)hipHostFree(host_scalars);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
-        host_scalars = nullptr; ev0 = ev1 = nullptr; stream = nullptr; device = -1;
+        host_scalars = nullptr; stream = nullptr; device = -1;
     }
 
     void init() {
@@ -131,8 +166,6 @@ struct Workspace {
         XFE_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         XFE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_scalars), 4096 * sizeof(double),
                                 hipHostMallocDefault));
-        XFE_CHECK(hipEventCreate(&ev0));
-        XFE_CHECK(hipEventCreate(&ev1));
     }
 };
 
@@ -248,15 +281,9 @@ struct Solver {
                 a.xcd_copies = copies_of(r_dim, C);
                 a.copy_stride = static_cast<size_t>(G[r_dim]) * C;
             }
-            if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            const int pslot = profile_begin(s);
             XFE_CHECK(launch_pass(a, cc, mode, nq, grid, s));
-            if (g_profile.enabled) {
-                XFE_CHECK(hipEventRecord(ws.ev1, s));
-                XFE_CHECK(hipEventSynchronize(ws.ev1));
-                float ms = 0.f;
-                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
-                g_profile.ms += ms;
-                g_profile.launches += 1;
+            if (pslot >= 0) {
                 // algorithmic bytes of this launch
                 double vec_rw = (P ? 2.0 : 1.0) + (xout ? 1.0 : 0.0) + (dst ? 1.0 : 0.0) +
                                 (it_x ? 2.0 : 0.0);
@@ -264,7 +291,7 @@ struct Solver {
                 if (a_dim >= 0) bytes += 4.0 * n + 8.0 * static_cast<double>(G[a_dim]) * cc;
                 if (r_dim >= 0) bytes += 4.0 * n + 16.0 * static_cast<double>(G[r_dim]) * cc;
                 if (r_dim >= 0 && w) bytes += 8.0 * n;
-                g_profile.bytes += bytes;
+                profile_end(pslot, s, bytes);
             }
         }
     }
@@ -360,15 +387,9 @@ struct Solver {
             a.it_x = it_x; a.it_gx = it_gx;
             a.partials = partials ? partials + static_cast<size_t>(ch) * tile_grid * nq : nullptr;
             a.maxp_partials = maxp ? maxp + static_cast<size_t>(ch) * tile_grid : nullptr;
-            if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            const int pslot = profile_begin(s);
             XFE_CHECK(launch_tile_pass(a, cc, rr, pp, nq, tile_grid, s));
-            if (g_profile.enabled) {
-                XFE_CHECK(hipEventRecord(ws.ev1, s));
-                XFE_CHECK(hipEventSynchronize(ws.ev1));
-                float ms = 0.f;
-                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
-                g_profile.ms += ms;
-                g_profile.launches += 1;
+            if (pslot >= 0) {
                 // algorithmic bytes: vectors read/written, id arrays, level tables
                 const double vec_rw = (in.P ? 2.0 : 1.0) + (in.xout ? 1.0 : 0.0) +
                                       (dst ? 1.0 : 0.0) + (it_x ? 2.0 : 0.0);
@@ -379,7 +400,7 @@ struct Solver {
                 if (rr != 0) bytes += 4.0 * n;
                 if (rr == kRedSpell) bytes += 8.0 * static_cast<double>(spell[r_dim].M) * cc;
                 if (w && (pp || rr != 0)) bytes += 8.0 * n;
-                g_profile.bytes += bytes;
+                profile_end(pslot, s, bytes);
             }
         }
     }
@@ -917,21 +938,15 @@ struct FastSolver {
             a.bkey = ws.seg_bk[0].p;
             a.bval = bv_buf<T>(0);
             XFE_CHECK(launch_seg_worker<T>(a, cc, k2 >= 0, seg_grid_w, s));
-            if (g_profile.enabled) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            const int pslot = profile_begin(s);
             XFE_CHECK(launch_seg_firm<T>(a, cc, k2 >= 0, false, seg_grid_f, s));
-            if (g_profile.enabled) {
+            if (pslot >= 0) {
                 // the firm pass alone: segment records (key, worker, length, start), one worker
                 // mean per segment, the firm coefficients read and written once
-                XFE_CHECK(hipEventRecord(ws.ev1, s));
-                XFE_CHECK(hipEventSynchronize(ws.ev1));
-                float ms = 0.f;
-                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
                 const double e = sizeof(T);
                 const double G1 = sv.G[rdim[k1]];
-                g_profile.ms += ms;
-                g_profile.launches += 1;
-                g_profile.bytes += static_cast<double>(M) * (12.0 + (k2 >= 0 ? 4.0 : 0.0) + e * cc) +
-                                   2.0 * e * cc * G1;
+                profile_end(pslot, s,
+                            static_cast<double>(M) * (12.0 + (k2 >= 0 ? 4.0 : 0.0) + e * cc) + 2.0 * e * cc * G1);
             }
             seg_boundaries<T>(a.out1, C, c0, cc);
             if (k2 >= 0)
@@ -1056,15 +1071,9 @@ struct FastSolver {
             }
             const size_t extra = static_cast<size_t>(lo) * sizeof(T);
             (void)lds_extra;
-            if (g_profile.enabled && !seg) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            const int pslot = seg ? -1 : profile_begin(s);
             XFE_CHECK(launch_schur<T>(a, cc, NR, V != nullptr, dst != nullptr, extra, sv.tile_grid, s));
-            if (g_profile.enabled && !seg) {
-                XFE_CHECK(hipEventRecord(ws.ev1, s));
-                XFE_CHECK(hipEventSynchronize(ws.ev1));
-                float ms = 0.f;
-                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
-                g_profile.ms += ms;
-                g_profile.launches += 1;
+            if (pslot >= 0) {
                 const double e = sizeof(T);
                 double bytes = 4.0 * sv.n * (1 + NR);                     // ids
                 if (V) bytes += e * static_cast<double>(sv.n) * cc;       // data read
@@ -1074,7 +1083,7 @@ struct FastSolver {
                     if (mode[k] == kRedSpell) bytes += e * static_cast<double>(sv.spell[rdim[k]].M) * cc;
                 }
                 bytes += e * static_cast<double>(sv.G[p]);                // 1 / level sizes of p
-                g_profile.bytes += bytes;
+                profile_end(pslot, s, bytes);
             }
         }
         for (int k = 0; k < NR; ++k) {
@@ -1889,20 +1898,12 @@ static void absorb_slopes_impl(double* V_in, int n, int C, int ld, const std::ve
         for (int d : seq) {
             XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0,
                                      sizeof(double) * 2 * static_cast<size_t>(G[d]) * C * copies[d], s));
-            if (g_profile.enabled && z[d]) XFE_CHECK(hipEventRecord(ws.ev0, s));
+            const int pslot = z[d] ? profile_begin(s) : -1;
             XFE_CHECK(launch_slope_reduce(t, ld, C, n, gid[d], z[d], w, ws.sums[d].p, G[d], copies[d], grid, s));
-            if (g_profile.enabled && z[d]) {
-                // the slope dimension's reduce: ids, z, (w,) C values per row, and the
-                // 2 C level sums of each row's level
-                XFE_CHECK(hipEventRecord(ws.ev1, s));
-                XFE_CHECK(hipEventSynchronize(ws.ev1));
-                float ms = 0.f;
-                XFE_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
-                g_profile.ms += ms;
-                g_profile.launches += 1;
-                g_profile.bytes += static_cast<double>(n) * (12.0 + (w ? 8.0 : 0.0) + 8.0 * C) +
-                                   16.0 * static_cast<double>(G[d]) * C;
-            }
+            // the slope dimension's reduce: ids, z, (w,) C values per row, and the 2 C
+            // level sums of each row's level
+            profile_end(pslot, s,
+                        static_cast<double>(n) * (12.0 + (w ? 8.0 : 0.0) + 8.0 * C) + 16.0 * static_cast<double>(G[d]) * C);
             XFE_CHECK(launch_slope_coef(ws.sums[d].p, ws.wsum[d].p, G[d], C, mode[d], copies[d], ws.means[d].p,
                                         grid_for(static_cast<long long>(G[d]) * C), s));
             XFE_CHECK(launch_slope_apply(t, ld, C, n, gid[d], z[d], ws.means[d].p, grid, s));
@@ -2307,11 +2308,21 @@ void xfe_release_workspace(void) { tls_ws.release(); }
 
 void xfe_profile_enable(int on) { g_profile.enabled = on != 0; }
 void xfe_profile_reset(void) {
+    try {
+        profile_resolve();
+    } catch (const std::exception& e) {
+        set_error(e.what());
+    }
     g_profile.ms = 0.0;
     g_profile.launches = 0;
     g_profile.bytes = 0.0;
 }
 void xfe_profile_get(double* pass_ms, long long* pass_launches, double* pass_bytes) {
+    try {
+        profile_resolve();
+    } catch (const std::exception& e) {
+        set_error(e.what());
+    }
     if (pass_ms) *pass_ms = g_profile.ms;
     if (pass_launches) *pass_launches = g_profile.launches;
     if (pass_bytes) *pass_bytes = g_profile.bytes;

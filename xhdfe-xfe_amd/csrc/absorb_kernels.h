@@ -257,7 +257,8 @@ hipError_t launch_gather_vec(const double* src, const int* perm, double* dst, in
 hipError_t launch_sorted_check(const int* gid, int n, int* flag, int grid, hipStream_t s);
 
 // Device time and algorithmic bytes of the dominant kernel of the last solves
-// (xfe_profile_*): HIP events around each launch when enabled.
+// (xfe_profile_*): a pair of HIP events around each profiled launch, recorded
+// without waiting; their times are read when the numbers are asked for.
 struct Profile {
     bool enabled = false;
     double ms = 0.0;
@@ -265,6 +266,13 @@ struct Profile {
     double bytes = 0.0;
 };
 extern Profile g_profile;
+// Records the start of a profiled launch on s; returns its slot (-1 when
+// profiling is off).
+int profile_begin(hipStream_t s);
+// Records the end of the launch in `slot` with its algorithmic bytes.
+void profile_end(int slot, hipStream_t s, double bytes);
+// Waits for the recorded launches and adds them to g_profile.
+void profile_resolve();
 
 }  // namespace xfe
 

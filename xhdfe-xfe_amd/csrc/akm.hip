@@ -341,11 +341,8 @@ struct xfe_akm_context {
     double* h_pack = nullptr;   // pinned J x lanes staging
     int h_pack_lanes = 0;
     double* h_small = nullptr;  // pinned scalars
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // kernel timing (xfe_profile_*)
 
     ~xfe_akm_context() {
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
         if (h_pack) (void)hipHostFree(h_pack);
         if (h_small) (void)hipHostFree(h_small);
         if (s) (void)hipStreamDestroy(s);
@@ -360,22 +357,14 @@ struct xfe_akm_context {
         XFE_CHECK(hipGetLastError());
         int m = M;
         int pieces = (m + kPiece - 1) / kPiece;
-        if (g_profile.enabled) XFE_CHECK(hipEventRecord(ev0, s));
+        const int pslot = profile_begin(s);
         hipLaunchKernelGGL(akm_firm_kernel, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
                            dim3(kBlock), 0, s, fkey.p, fw.p, fc.p, pv, wmean.p, m, nb, out, bk[0].p, bv[0].p,
                            pieces == 1 ? 1 : 0);
         XFE_CHECK(hipGetLastError());
-        if (g_profile.enabled) {
-            // the firm pass alone: match records (firm key, worker, count), the worker's
-            // nb lanes per match, the firm's lanes read and written once
-            XFE_CHECK(hipEventRecord(ev1, s));
-            XFE_CHECK(hipEventSynchronize(ev1));
-            float ms = 0.f;
-            XFE_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
-            g_profile.ms += ms;
-            g_profile.launches += 1;
-            g_profile.bytes += static_cast<double>(M) * (12.0 + 8.0 * nb) + 16.0 * nb * J;
-        }
+        // the firm pass alone: match records (firm key, worker, count), the worker's nb
+        // lanes per match, the firm's lanes read and written once
+        profile_end(pslot, s, static_cast<double>(M) * (12.0 + 8.0 * nb) + 16.0 * nb * J);
         int cur = 0;
         while (pieces > 1) {
             m = 2 * pieces;
@@ -593,8 +582,6 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
         c = new xfe_akm_context();
         c->N = N; c->J = J; c->M = M; c->ground = J - 1;
         XFE_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
-        XFE_CHECK(hipEventCreate(&c->ev0));
-        XFE_CHECK(hipEventCreate(&c->ev1));
         auto up_i = [&](DevBuf<int>& d, const std::vector<int>& h) {
             d.ensure(h.size());
             XFE_CHECK(hipMemcpyAsync(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, c->s));
