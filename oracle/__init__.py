@@ -360,6 +360,98 @@ def akm_solve_S(st, rhs, tol, max_iter):
         rz = rz_new
 
 
+_U64 = (1 << 64) - 1
+
+
+def splitmix64(x):
+    """SplitMix64 (Steele, Lea and Flood 2014), as src/akm_kss.cpp:212-217,
+    on numpy uint64 arrays (wrapping arithmetic)."""
+    with np.errstate(over="ignore"):
+        x = np.asarray(x, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def stream_seed(seed, tag, unit):
+    """The per-(draw, unit) stream state of src/akm_kss.cpp:219-222."""
+    with np.errstate(over="ignore"):
+        a = np.uint64((0xA24BAED4963EE407 * (int(tag) + 1)) & _U64)
+        s = splitmix64(np.uint64(seed) ^ a)
+        u = (np.uint64(0x9FB21C651E98DF25) * (np.asarray(unit, dtype=np.uint64) + np.uint64(1)))
+        return splitmix64(s ^ u)
+
+
+def jla_signs(seed, draw, R):
+    """The +-1 vector of draw `draw` over R working rows (src/akm_kss.cpp:
+    3157-3170): 64 rows per stream state stream_seed(seed, 3 draw, block),
+    row r taking bit r mod 64 (1 -> +1)."""
+    nb = (R + 63) // 64
+    with np.errstate(over="ignore"):
+        bits = stream_seed(seed, 3 * draw, np.arange(nb, dtype=np.uint64))
+    r = np.arange(R)
+    b = (bits[r // 64] >> (r % 64).astype(np.uint64)) & np.uint64(1)
+    return np.where(b == 1, 1.0, -1.0)
+
+
+def akm_jla_pii(st, draws, seed):
+    """JLA leverages of the match-level working rows (src/akm_kss.cpp:3153-3285,
+    the P_ii part): per draw, r = +-1 signs, t = A' r with A the rows
+    sqrt(c_m) (e_w + e_f), the two-way normal equations K [zw; zf] = t solved
+    exactly (firms grounded at J-1, a dense solve of the Schur complement S),
+    Z = A [zw; zf]; with P = mean Z^2 and M = mean (r - Z)^2 over the draws,
+    P_ii = P / (P + M).  st: akm_build's structure (rows = matches)."""
+    mw, mf, c = st["m_w"], st["m_f"], st["m_c"]
+    N, J = st["N"], st["J"]
+    R = mw.size
+    sq = np.sqrt(c)
+    S = np.zeros((J, J))
+    for f in range(J):
+        e = np.zeros(J)
+        e[f] = 1.0
+        S[:, f] = akm_apply_S(st, e)
+    Sg = S[:J - 1, :J - 1]
+    Pacc = np.zeros(R)
+    Macc = np.zeros(R)
+    for d in range(draws):
+        r = jla_signs(seed, d, R)
+        v = sq * r
+        tw = np.bincount(mw, weights=v, minlength=N)
+        tf = np.bincount(mf, weights=v, minlength=J)
+        rhs = tf - np.bincount(mf, weights=c * (tw / st["Dw"])[mw], minlength=J)
+        zf = np.zeros(J)
+        zf[:J - 1] = np.linalg.solve(Sg, rhs[:J - 1])
+        zw = (tw - np.bincount(mw, weights=c * zf[mf], minlength=N)) / st["Dw"]
+        Z = sq * (zw[mw] + zf[mf])
+        Pacc += Z * Z / draws
+        Macc += (r - Z) ** 2 / draws
+    den = Pacc + Macc
+    return np.where(den > 0.0, Pacc / np.where(den > 0.0, den, 1.0), 0.0)
+
+
+def ref_akm_jla_pii(y, worker, firm, draws, seed):
+    """The reference's akm_kss_decompose with JLA leverages at match level (no
+    pruning) through oracle/_ref (checker): (row_worker, row_firm,
+    row_weight, pii) of its working rows, in its order."""
+    r = ref()
+    if r is None:
+        return None
+    n = len(y)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    w = np.ascontiguousarray(worker, dtype=np.int32)
+    f = np.ascontiguousarray(firm, dtype=np.int32)
+    rw = np.empty(n, dtype=np.int32)
+    rf = np.empty(n, dtype=np.int32)
+    rc = np.empty(n)
+    pii = np.empty(n)
+    r.ref_akm_jla_pii.argtypes = [_i, _p, _p, _p, _i, ctypes.c_ulonglong, _p, _p, _p, _p]
+    R = r.ref_akm_jla_pii(n, _ptr(y), _ptr(w), _ptr(f), int(draws), int(seed), _ptr(rw), _ptr(rf), _ptr(rc),
+                          _ptr(pii))
+    if R < 0:
+        raise RuntimeError("ref_akm_jla_pii failed")
+    return rw[:R], rf[:R], rc[:R], pii[:R]
+
+
 def schwarz_rule(V, Vt, gids, G):
     """The stopping rule of schwarz_demean_raw evaluated on a result
     (src/schwarz_demean.cpp:283-305, :358-378): per column c,

@@ -19,6 +19,7 @@
 #include "schwarz_demean.hpp"
 #include "hdfe/hdfe_regressor.hpp"
 #include "hdfe/hdfe_regressor_v11.hpp"
+#include "hdfe/akm_kss.hpp"
 
 namespace {
 
@@ -219,6 +220,38 @@ int ref_absorb_slope(int n, int k, int d, const double* y, const double* X, cons
         return 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "ref_absorb_slope: %s\n", e.what());
+        return -1;
+    }
+}
+
+// akm_kss_decompose (src/akm_kss.cpp:2286) with JLA leverages at match level,
+// no pruning (the caller passes a leave-out connected sample): the working
+// rows (original worker / firm ids, person-years) and their P_ii.  Outputs
+// have capacity n; returns the number of working rows, -1 on exception.
+int ref_akm_jla_pii(int n, const double* y, const int* worker, const int* firm, int draws,
+                    unsigned long long seed, int* row_worker, int* row_firm, double* row_weight,
+                    double* pii) {
+    try {
+        Eigen::VectorXd yv = Eigen::Map<const Eigen::VectorXd>(y, n);
+        Eigen::VectorXi wv = Eigen::Map<const Eigen::VectorXi>(worker, n);
+        Eigen::VectorXi fv = Eigen::Map<const Eigen::VectorXi>(firm, n);
+        hdfe::akm::AkmOptions o;
+        o.leverage_method = hdfe::akm::LeverageMethod::Jla;
+        o.jla_draws = draws;
+        o.seed = seed;
+        o.prune = false;
+        o.num_threads = 1;
+        auto r = hdfe::akm::akm_kss_decompose(yv, wv, fv, nullptr, o);
+        const int R = static_cast<int>(r.pii.size());
+        for (int k = 0; k < R && k < n; ++k) {
+            row_worker[k] = r.row_worker[k];
+            row_firm[k] = r.row_firm[k];
+            row_weight[k] = r.row_weight[k];
+            pii[k] = r.pii[k];
+        }
+        return R;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_akm_jla_pii: %s\n", e.what());
         return -1;
     }
 }

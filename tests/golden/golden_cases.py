@@ -85,3 +85,22 @@ SLOPE_CASES = (("two_way", 1, True), ("two_way", 1, False), ("akm_three_way", 1,
 def slope_values(n):
     """The slope variable of the slope cases: N(1, 1), seeded."""
     return np.random.default_rng(20261213).normal(1.0, 1.0, n)
+
+
+def jla_panel():
+    """A small AKM panel whose worker-firm graph is leave-out connected: 600
+    workers x 5 years over 40 firms, a move with probability 0.35 a year."""
+    rng = np.random.default_rng(4)
+    W, J, T = 600, 40, 5
+    worker = np.repeat(np.arange(W), T)
+    firm = np.empty(W * T, dtype=np.int64)
+    for w in range(W):
+        f = rng.integers(0, J)
+        for t in range(T):
+            if rng.random() < 0.35:
+                f = rng.integers(0, J)
+            firm[w * T + t] = f
+    return dict(y=rng.normal(size=W * T), worker=worker, firm=firm)
+
+
+JLA_DRAWS, JLA_SEED = 50, 42

@@ -19,8 +19,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "xhdfe-xfe_amd"))
 
 import oracle  # noqa: E402
-from golden_cases import (CASES, SLOPE_CASES, certificate_weights, input_checksum,  # noqa: E402
-                          schwarz_edge_cases, slope_values)
+from golden_cases import (CASES, JLA_DRAWS, JLA_SEED, SLOPE_CASES, certificate_weights,  # noqa: E402
+                          input_checksum, jla_panel, schwarz_edge_cases, slope_values)
 
 
 def main():
@@ -87,6 +87,10 @@ def main():
     # returned flag and iteration count per case (order of SCHWARZ_EDGE_CASES)
     out["schwarz_edges"] = np.array([list(oracle.ref_schwarz(*args, tol=1e-10, max_iter=mi)[0::3])
                                      for args, mi in schwarz_edge_cases()], dtype=np.int64)
+    # JLA leverages of the reference's KSS decomposition (match level, no pruning)
+    jp = jla_panel()
+    rw, rf, rc, pii = oracle.ref_akm_jla_pii(jp["y"], jp["worker"], jp["firm"], JLA_DRAWS, JLA_SEED)
+    out["jla/row_worker"], out["jla/row_firm"], out["jla/row_weight"], out["jla/pii"] = rw, rf, rc, pii
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_cases.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -565,3 +565,26 @@ def test_absorb_slope_requires_values():
     ok = xfe_amd.lib().xfe_absorb(xfe_amd._ptr(y), None, 50, 0, fes, 1, None, None,
                                   ctypes.byref(o), ctypes.byref(r), xfe_amd._ptr(yo), None)
     assert ok == 0 and "slope_values" in xfe_amd.last_error()
+
+
+# ------------------------------------------------------------ JLA leverages
+
+def test_akm_jla_pii_matches_reference(golden):
+    """xfe_akm_jla_pii with the reference's streams against the reference's
+    P_ii (50 draws, seed 42; the solves stop at 1e-12 here, the reference's
+    direct solve is exact) and against the oracle at 200 draws."""
+    from golden_cases import JLA_DRAWS, JLA_SEED
+    _log("akm jla pii")
+    rw, rf, rc = golden["jla/row_worker"], golden["jla/row_firm"], golden["jla/row_weight"]
+    wu, wi = np.unique(rw, return_inverse=True)
+    fu, fi = np.unique(rf, return_inverse=True)
+    st = oracle.akm_build(wi, fi, rc, wu.size, fu.size)
+    solver = xfe_amd.AkmSolver(st)
+    pii, ok = solver.jla_pii(JLA_DRAWS, JLA_SEED, tol=1e-12, max_iter=5000)
+    assert ok, xfe_amd.last_error()
+    err = np.abs(pii - golden["jla/pii"]).max()
+    _log(f"  vs reference {err:.2e}")
+    assert err <= 1e-8
+    pii2, ok = solver.jla_pii(200, 9, tol=1e-12, max_iter=5000)
+    assert ok and np.abs(pii2 - oracle.akm_jla_pii(st, 200, 9)).max() <= 1e-8
+    solver.close()

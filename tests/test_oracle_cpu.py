@@ -514,3 +514,44 @@ def test_slope_projection_single_dimension():
                 else:
                     want = V[m, c] - z[m] * (z[m] @ V[m, c]) / (z[m] @ z[m])
                 np.testing.assert_allclose(Vo[m, c], want, atol=1e-12)
+
+
+# ------------------------------------------------------------ JLA leverages
+
+def test_splitmix64_known_answers():
+    """SplitMix64's published outputs (Vigna's reference generator, state 0:
+    the first two outputs)."""
+    assert int(oracle.splitmix64(np.uint64(0))) == 0xE220A8397B1DCDAF
+    assert int(oracle.splitmix64(np.uint64(0x9E3779B97F4A7C15))) == 0x6E789E6AA1B965F4
+
+
+def _jla_structure(golden):
+    rw, rf, rc = golden["jla/row_worker"], golden["jla/row_firm"], golden["jla/row_weight"]
+    wu, wi = np.unique(rw, return_inverse=True)
+    fu, fi = np.unique(rf, return_inverse=True)
+    assert np.all(np.diff(wi) >= 0)  # the reference's working rows are worker-major
+    return oracle.akm_build(wi, fi, rc, wu.size, fu.size)
+
+
+def test_jla_pii_matches_reference(golden):
+    """The oracle's JLA leverages with the reference's streams equal the
+    reference's akm_kss_decompose P_ii (50 draws, seed 42) to rounding."""
+    from golden_cases import JLA_DRAWS, JLA_SEED
+    st = _jla_structure(golden)
+    pii = oracle.akm_jla_pii(st, JLA_DRAWS, JLA_SEED)
+    assert np.abs(pii - golden["jla/pii"]).max() <= 1e-12
+
+
+def test_jla_pii_limit_is_exact_leverage(golden):
+    """Many draws approach the exact leverages x_m' (A'A)^+ x_m of the
+    match-level design (dense), within the draws' sampling error."""
+    st = _jla_structure(golden)
+    N, J = st["N"], st["J"]
+    A = np.zeros((st["m_w"].size, N + J))
+    sq = np.sqrt(st["m_c"])
+    A[np.arange(A.shape[0]), st["m_w"]] = sq
+    A[np.arange(A.shape[0]), N + st["m_f"]] = sq
+    H = A @ np.linalg.pinv(A.T @ A) @ A.T
+    exact = np.diag(H)
+    est = oracle.akm_jla_pii(st, 400, 7)
+    assert np.abs(est - exact).mean() < 0.02 and np.corrcoef(est, exact)[0, 1] > 0.95

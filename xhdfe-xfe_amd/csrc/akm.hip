@@ -327,6 +327,9 @@ using namespace xfe;
 struct xfe_akm_context {
     int N = 0, J = 0, ground = 0, M = 0;
     hipStream_t s = nullptr;
+    // the matches as given (worker-major), for the host side of the JLA draws
+    std::vector<int> h_w, h_f;
+    std::vector<double> h_c, h_dw;
     // worker-major matches
     int PW = 0;  // worker pieces
     DevBuf<int> wpiece, rw, r1;
@@ -581,6 +584,10 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
         (void)hipGetLastError();
         c = new xfe_akm_context();
         c->N = N; c->J = J; c->M = M; c->ground = J - 1;
+        c->h_w.assign(m_w, m_w + M);
+        c->h_f.assign(m_f, m_f + M);
+        c->h_c.assign(m_c, m_c + M);
+        c->h_dw.assign(Dw, Dw + N);
         XFE_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         auto up_i = [&](DevBuf<int>& d, const std::vector<int>& h) {
             d.ensure(h.size());
@@ -652,6 +659,101 @@ int xfe_akm_solve_S_multi(xfe_akm_context* c, const double* rhs_pack, double* z_
         (void)hipGetLastError();
         c->solve(rhs_pack, z_pack, nb, tol, max_iter, iters_out);
         return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+namespace {
+// SplitMix64 (Steele, Lea and Flood 2014) and the per-(draw, unit) stream
+// state of the reference's JLA draws (src/akm_kss.cpp:212-222): the same
+// sequence is needed for the same leverages.
+uint64_t jla_mix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+uint64_t jla_stream(uint64_t seed, uint64_t tag, uint64_t unit) {
+    const uint64_t s = jla_mix(seed ^ (0xA24BAED4963EE407ULL * (tag + 1)));
+    return jla_mix(s ^ (0x9FB21C651E98DF25ULL * (unit + 1)));
+}
+
+// signs of draw d over the M working rows: bit r mod 64 of stream (3 d, r / 64)
+void jla_signs(uint64_t seed, int d, int M, std::vector<double>& r) {
+    r.resize(M);
+    for (int b = 0; b * 64 < M; ++b) {
+        const uint64_t bits = jla_stream(seed, 3ULL * static_cast<uint64_t>(d), static_cast<uint64_t>(b));
+        const int hi = std::min(M, 64 * b + 64);
+        for (int k = 64 * b; k < hi; ++k) r[k] = ((bits >> (k - 64 * b)) & 1ULL) ? 1.0 : -1.0;
+    }
+}
+}  // namespace
+
+int xfe_akm_jla_pii(xfe_akm_context* c, int draws, uint64_t seed, double tol, int max_iter, double* pii) {
+    clear_error();
+    if (!c || !pii || draws < 1 || max_iter < 1) {
+        set_error("xfe_akm_jla_pii: invalid arguments");
+        return -1;
+    }
+    try {
+        (void)hipGetLastError();
+        const int M = c->M, N = c->N, J = c->J;
+        std::vector<double> sq(M), pacc(M, 0.0), macc(M, 0.0), r, tw, tf, zw(N);
+        for (int m = 0; m < M; ++m) sq[m] = std::sqrt(c->h_c[m]);
+        // t = A' r (A: rows sqrt(c_m) (e_w + e_f)) for draw d
+        auto scatter = [&](int d) {
+            jla_signs(seed, d, M, r);
+            tw.assign(N, 0.0);
+            tf.assign(J, 0.0);
+            for (int m = 0; m < M; ++m) {
+                const double v = sq[m] * r[m];
+                tw[c->h_w[m]] += v;
+                tf[c->h_f[m]] += v;
+            }
+        };
+        std::vector<double> pack, zpack;
+        std::vector<int> its(kMaxCgCols);
+        bool failed = false;
+        const double inv_p = 1.0 / draws;
+        for (int d0 = 0; d0 < draws; d0 += kMaxCgCols) {
+            const int nb = std::min(kMaxCgCols, draws - d0);
+            pack.assign(static_cast<size_t>(nb) * J, 0.0);
+            zpack.assign(static_cast<size_t>(nb) * J, 0.0);
+            // Schur right-hand sides tf - B' Dw^-1 tw of the two-way normal equations
+            for (int l = 0; l < nb; ++l) {
+                scatter(d0 + l);
+                double* b = pack.data() + static_cast<size_t>(l) * J;
+                for (int f = 0; f < J; ++f) b[f] = tf[f];
+                for (int m = 0; m < M; ++m) {
+                    const int w = c->h_w[m];
+                    b[c->h_f[m]] -= c->h_c[m] * tw[w] / c->h_dw[w];
+                }
+            }
+            c->solve(pack.data(), zpack.data(), nb, tol, max_iter, its.data());
+            for (int l = 0; l < nb; ++l) {
+                if (its[l] < 0) failed = true;
+                scatter(d0 + l);  // the draw's t again (cheaper than keeping nb of them)
+                const double* zf = zpack.data() + static_cast<size_t>(l) * J;
+                for (int w = 0; w < N; ++w) zw[w] = tw[w];
+                for (int m = 0; m < M; ++m) zw[c->h_w[m]] -= c->h_c[m] * zf[c->h_f[m]];
+                for (int w = 0; w < N; ++w) zw[w] = c->h_dw[w] > 0.0 ? zw[w] / c->h_dw[w] : 0.0;
+                for (int m = 0; m < M; ++m) {
+                    const double Z = sq[m] * (zw[c->h_w[m]] + zf[c->h_f[m]]);
+                    const double e = r[m] - Z;
+                    pacc[m] += Z * Z * inv_p;
+                    macc[m] += e * e * inv_p;
+                }
+            }
+        }
+        for (int m = 0; m < M; ++m) {
+            const double den = pacc[m] + macc[m];
+            pii[m] = den > 0.0 ? pacc[m] / den : 0.0;
+        }
+        if (failed) set_error("xfe_akm_jla_pii: some draw solves did not converge");
+        return failed ? 1 : 0;
     } catch (const std::exception& e) {
         set_error(e.what());
         return -1;

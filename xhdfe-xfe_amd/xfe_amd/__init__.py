@@ -52,7 +52,8 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_recovery_max_delta", "xfe_last_error", "xfe_profile_enable", "xfe_profile_reset",
            "xfe_profile_get", "xfe_release_workspace", "xfe_certify", "xfe_akm_available",
            "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
-           "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer", "xfe_schwarz_demean")
+           "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
+           "xfe_akm_jla_pii")
 
 _lib = None
 
@@ -268,6 +269,17 @@ class AkmSolver:
         if rc != 0:
             raise XfeError(last_error())
         return z, its
+
+    def jla_pii(self, draws=200, seed=20260705, tol=1e-10, max_iter=10000):
+        """xfe_akm_jla_pii: JLA leverages of the matches (in the given order)
+        from `draws` Rademacher draws of the reference's streams."""
+        pii = np.empty(self._keep["m_w"].size)
+        L = lib()
+        L.xfe_akm_jla_pii.argtypes = [_p, _i, ctypes.c_uint64, _d, _i, _p]
+        rc = L.xfe_akm_jla_pii(_p(self._ctx), int(draws), int(seed), float(tol), int(max_iter), _ptr(pii))
+        if rc < 0:
+            raise XfeError(last_error())
+        return pii, rc == 0
 
     def close(self):
         if self._ctx:
