@@ -158,6 +158,12 @@ def bench_akm(args):
     sec = (time.perf_counter() - t0) / args.steps
     xfe_amd.profile_enable(False)
     pass_ms, launches, pass_bytes = xfe_amd.profile_get()
+    # the JLA leverages with the reference's default 200 draws (solves on the
+    # device, draws and accumulation on the host)
+    t0 = time.perf_counter()
+    pii, jla_ok = solver.jla_pii(draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
+    jla_s = time.perf_counter() - t0
+    log(f"JLA P_ii, 200 draws: {jla_s:.2f} s, converged {jla_ok}, mean P_ii {pii.mean():.4f}")
     solver.close()
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     its = np.concatenate(all_its[:lanes // batch])
@@ -175,6 +181,7 @@ def bench_akm(args):
         "config": {"workload": "kss_leverage_256rhs", "obs": int(n), "workers": N, "firms": J, "matches": int(M),
                    "lanes": lanes, "lanes_per_launch": batch, "cg_tol": 1e-10},
         "iterations": {"min": int(its.min()), "max": int(its.max()), "mean": round(float(its.mean()), 1)},
+        "jla_pii_200_draws_s": round(jla_s, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(os.path.join(ROOT, "profiles", "akm_firm_kss10m.md")),
