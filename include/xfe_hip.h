@@ -223,13 +223,13 @@ int xfe_akm_solve_S_multi(xfe_akm_context* ctx, const double* rhs_pack, double* 
                           double tol, int max_iter, int* iters_out);
 
 /* JLA leverages of the working rows (the P_ii part of the JLA draws,
- * src/akm_kss.cpp:3153-3285, match level): the rows are the context's
- * matches in the order given to xfe_akm_create, row m standing for
+ * src/akm_kss.cpp:3153-3285, match level), on the device: the rows are the
+ * context's matches in the order given to xfe_akm_create, row m standing for
  * sqrt(m_c[m]) (e_worker + e_firm).  Per draw d, signs r from the reference's
  * SplitMix64 streams (seed, 3 d, block of 64 rows), the two-way normal
  * equations solved for A'r (firm part by the grounded S solver, batches of
  * xfe_akm_max_lanes() draws, stop at tol), Z = A [zw; zf]; then
- * pii = mean Z^2 / (mean Z^2 + mean (r - Z)^2).  Returns 0, 1 when some
+ * pii = mean Z^2 / (mean Z^2 + mean (r - Z)^2) (host array of M).  Returns 0, 1 when some
  * draw's solve did not converge (its z is zero, as in the reference), -1 on
  * failure. */
 int xfe_akm_jla_pii(xfe_akm_context* ctx, int draws, uint64_t seed, double tol, int max_iter,

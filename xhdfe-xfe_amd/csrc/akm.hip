@@ -312,6 +312,172 @@ akm_list_kernel(const int* __restrict__ key_in, const double* __restrict__ vin, 
     }
 }
 
+// ---- JLA draws on the device (src/akm_kss.cpp:3153-3285, the P_ii part)
+
+// SplitMix64 (Steele, Lea and Flood 2014) and the reference's per-(draw,
+// unit) stream state (src/akm_kss.cpp:212-222): the sign of working row m in
+// draw d is bit m mod 64 of stream_seed(seed, 3 d, m / 64).
+__host__ __device__ __forceinline__ uint64_t jla_mix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ double jla_sign(uint64_t seed, int d, int m) {
+    const uint64_t s0 = jla_mix(seed ^ (0xA24BAED4963EE407ULL * (3ULL * static_cast<uint64_t>(d) + 1ULL)));
+    const uint64_t st = jla_mix(s0 ^ (0x9FB21C651E98DF25ULL * (static_cast<uint64_t>(m >> 6) + 1ULL)));
+    return ((st >> (m & 63)) & 1ULL) ? 1.0 : -1.0;
+}
+
+// tw[w nb + l] = sum over the worker's matches a of sqrt(c_a) r_a(d0 + l)
+__global__ void __launch_bounds__(kBlock)
+jla_tw_kernel(const int* wofs, int N, const int* orig, const double* sq, uint64_t seed, int d0, int nb,
+              double* tw) {
+    const long long total = static_cast<long long>(N) * nb;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int w = static_cast<int>(j / nb);
+        const int l = static_cast<int>(j - static_cast<long long>(w) * nb);
+        double acc = 0.0;
+        for (int a = wofs[w]; a < wofs[w + 1]; ++a) acc += sq[a] * jla_sign(seed, d0 + l, orig[a]);
+        tw[j] = acc;
+    }
+}
+
+// Firm pass of the draws' Schur right-hand sides: per firm, the sum over its
+// matches of c (r / sqrt(c) - tw[w] / Dw[w]) = sqrt(c) r - c tw[w] / Dw[w],
+// with the run rule of akm_firm_kernel (pieces of kPiece firm-major matches,
+// runs crossing a piece boundary into the boundary list).
+__global__ void __launch_bounds__(kBlock)
+jla_firm_kernel(const int* __restrict__ fkey, const int* __restrict__ fw, const double* __restrict__ fc,
+                const int* __restrict__ forig, const double* __restrict__ tw, const double* __restrict__ idw,
+                uint64_t seed, int d0, int m, int nb, double* __restrict__ out, int* __restrict__ kout,
+                double* __restrict__ vout, int final) {
+    const int c = threadIdx.x % kLaneGroup;
+    const int stride = gridDim.x * kGroups;
+    const int pieces = (m + kPiece - 1) / kPiece;
+    const int q0 = blockIdx.x * kGroups + threadIdx.x / kLaneGroup;
+    const int rounds = (pieces - (q0 - q0 % 2) + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        const int q = q0 + r * stride;
+        const bool live = q < pieces;
+        const int j0 = q * kPiece;
+        const int n = live ? min(m - j0, kPiece) : 0;
+        const int kA = c < n ? fkey[j0 + c] : 0, kB = c + kLaneGroup < n ? fkey[j0 + kLaneGroup + c] : 0;
+        const int wA = c < n ? fw[j0 + c] : 0, wB = c + kLaneGroup < n ? fw[j0 + kLaneGroup + c] : 0;
+        const int oA = c < n ? forig[j0 + c] : 0, oB = c + kLaneGroup < n ? forig[j0 + kLaneGroup + c] : 0;
+        const double xA = c < n ? fc[j0 + c] : 1.0, xB = c + kLaneGroup < n ? fc[j0 + kLaneGroup + c] : 1.0;
+        const int nmax = max(n, __shfl_xor(n, kLaneGroup));
+        const int cl = c < nb ? c : 0;
+        int key = group_entry(kA, kB, 0);
+        double acc = 0.0;
+        bool first = true;
+        for (int e0 = 0; e0 < nmax; e0 += kBatch) {
+            double v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int w = group_entry(wA, wB, e0 + u);
+                v[u] = tw[static_cast<size_t>(w) * nb + cl] * idw[w];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int k = group_entry(kA, kB, e0 + u);
+                const int o = group_entry(oA, oB, e0 + u);
+                const double x = group_entry(xA, xB, e0 + u);
+                if (e0 + u < n) {
+                    if (k != key) {
+                        if (c < nb) {
+                            if (first && !final) vout[static_cast<size_t>(2 * q) * nb + c] = acc;
+                            else out[static_cast<size_t>(key) * nb + c] = acc;
+                        }
+                        if (first && !final && c == 0) kout[2 * q] = key;
+                        first = false;
+                        key = k;
+                        acc = 0.0;
+                    }
+                    acc += x * (jla_sign(seed, d0 + cl, o) / sqrt(x) - v[u]);
+                }
+            }
+        }
+        if (n > 0) {
+            if (final) {
+                if (c < nb) out[static_cast<size_t>(key) * nb + c] = acc;
+            } else {
+                const size_t e0 = static_cast<size_t>(2 * q) * nb, e1 = static_cast<size_t>(2 * q + 1) * nb;
+                if (c < nb) {
+                    if (first) {
+                        vout[e0 + c] = acc;
+                        vout[e1 + c] = 0.0;
+                    } else {
+                        vout[e1 + c] = acc;
+                    }
+                }
+                if (c == 0) {
+                    if (first) kout[2 * q] = key;
+                    kout[2 * q + 1] = key;
+                }
+            }
+        }
+    }
+}
+
+// zf = unscale_l x (level-major), zero on the ground firm and on lanes that did not converge
+__global__ void __launch_bounds__(kBlock)
+jla_unscale_kernel(const double* x, int J, int nb, int ground, const double* unscale, const double* cg, double* zf) {
+    const long long total = static_cast<long long>(J) * nb;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int f = static_cast<int>(j / nb);
+        const int l = static_cast<int>(j - static_cast<long long>(f) * nb);
+        zf[j] = (f == ground || cg[kCgOk + l] == 0.0) ? 0.0 : x[j] * unscale[l];
+    }
+}
+
+// zw[w nb + l] = (tw[w nb + l] - sum over the worker's matches of c_a zf[f_a nb + l]) / Dw[w]
+__global__ void __launch_bounds__(kBlock)
+jla_zw_kernel(const int* wofs, int N, const int* mf, const double* mc, const double* idw, const double* tw,
+              const double* zf, int nb, double* zw) {
+    const long long total = static_cast<long long>(N) * nb;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int w = static_cast<int>(j / nb);
+        const int l = static_cast<int>(j - static_cast<long long>(w) * nb);
+        double acc = tw[j];
+        for (int a = wofs[w]; a < wofs[w + 1]; ++a) acc -= mc[a] * zf[static_cast<size_t>(mf[a]) * nb + l];
+        zw[j] = acc * idw[w];
+    }
+}
+
+// Per working row (worker-major record a, row orig[a]): Z = sqrt(c)(zw + zf)
+// and r for the nb draws; P += Z^2 / p, M += (r - Z)^2 / p
+__global__ void __launch_bounds__(kBlock)
+jla_accumulate_kernel(int M, const int* mw, const int* mf, const int* orig, const double* sq, const double* zw,
+                      const double* zf, uint64_t seed, int d0, int nb, double inv_p, double* pacc, double* macc) {
+    for (int a = blockIdx.x * kBlock + threadIdx.x; a < M; a += gridDim.x * kBlock) {
+        const int o = orig[a];
+        const double* zwa = zw + static_cast<size_t>(mw[a]) * nb;
+        const double* zfa = zf + static_cast<size_t>(mf[a]) * nb;
+        double pp = 0.0, mm = 0.0;
+        for (int l = 0; l < nb; ++l) {
+            const double Z = sq[a] * (zwa[l] + zfa[l]);
+            const double e = jla_sign(seed, d0 + l, o) - Z;
+            pp += Z * Z;
+            mm += e * e;
+        }
+        pacc[o] += pp * inv_p;
+        macc[o] += mm * inv_p;
+    }
+}
+
+// pii = P / (P + M)
+__global__ void __launch_bounds__(kBlock) jla_ratio_kernel(int M, const double* pacc, const double* macc, double* pii) {
+    for (int m = blockIdx.x * kBlock + threadIdx.x; m < M; m += gridDim.x * kBlock) {
+        const double den = pacc[m] + macc[m];
+        pii[m] = den > 0.0 ? pacc[m] / den : 0.0;
+    }
+}
+
 constexpr int kAkmMaxGrid = 2048;
 
 int akm_grid(long long work) {
@@ -327,9 +493,10 @@ using namespace xfe;
 struct xfe_akm_context {
     int N = 0, J = 0, ground = 0, M = 0;
     hipStream_t s = nullptr;
-    // the matches as given (worker-major), for the host side of the JLA draws
-    std::vector<int> h_w, h_f;
-    std::vector<double> h_c, h_dw;
+    // JLA draws: worker offsets and per-record row index, sqrt(c), c in the
+    // worker-major record order; row index per firm-major record; 1 / Dw
+    DevBuf<int> wofs, rorig, forig;
+    DevBuf<double> rsq, rcv, idw, jtw, pacc, macc, pii;
     // worker-major matches
     int PW = 0;  // worker pieces
     DevBuf<int> wpiece, rw, r1;
@@ -401,6 +568,18 @@ struct xfe_akm_context {
         const int gj = akm_grid(static_cast<long long>(JL));
         hipLaunchKernelGGL(akm_to_levels, dim3(gj), dim3(kBlock), 0, s, lanes.p, J, nb, ground, b.p);
         XFE_CHECK(hipGetLastError());
+        solve_levels(nb, tol, max_iter);
+        hipLaunchKernelGGL(akm_to_lanes, dim3(gj), dim3(kBlock), 0, s, x.p, J, nb, ground, scale.p + kMaxCgCols,
+                           cg.p, lanes.p);
+        XFE_CHECK(hipGetLastError());
+        XFE_CHECK(hipMemcpyAsync(z_pack, lanes.p, sizeof(double) * JL, hipMemcpyDeviceToHost, s));
+        lane_iterations(nb, iters_out);
+    }
+
+    // The batched PCG on the level-major right-hand sides already in b (ground
+    // rows zero); leaves the scaled solutions in x, the scales in scale.
+    void solve_levels(int nb, double tol, int max_iter) {
+        const size_t JL = static_cast<size_t>(J) * nb;
         // per-lane norms of the pinned right-hand sides
         const int vg = akm_grid(J);
         part.ensure(std::max<size_t>(static_cast<size_t>(vg) * nb, 2 * static_cast<size_t>(kAkmMaxGrid) * kMaxCgCols));
@@ -432,10 +611,10 @@ struct xfe_akm_context {
                 if (h_small[0] == 0.0) break;
             }
         }
-        hipLaunchKernelGGL(akm_to_lanes, dim3(gj), dim3(kBlock), 0, s, x.p, J, nb, ground, scale.p + kMaxCgCols,
-                           cg.p, lanes.p);
-        XFE_CHECK(hipGetLastError());
-        XFE_CHECK(hipMemcpyAsync(z_pack, lanes.p, sizeof(double) * JL, hipMemcpyDeviceToHost, s));
+    }
+
+    // per-lane iteration counts (-1: not converged) of the last solve
+    void lane_iterations(int nb, int* iters_out) {
         XFE_CHECK(hipMemcpyAsync(h_small, cg.p + kCgIters, sizeof(double) * 2 * kMaxCgCols, hipMemcpyDeviceToHost, s));
         XFE_CHECK(hipStreamSynchronize(s));
         for (int l = 0; l < nb; ++l) {
@@ -540,24 +719,38 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
     for (int k = 0; k < N; ++k) newid[order[k]] = k;
     // worker-major records in the new order, and pieces of whole workers of at
     // most kPiece matches
-    std::vector<int> rf(M);
+    std::vector<int> rf(M), rorig(M), wofs(N + 1);
+    std::vector<double> rsq(M), rcv(M), idw(N);
     wpiece.push_back(0);
     int pos = 0;
     for (int k = 0; k < N; ++k) {
         const int w = order[k];
         const int a0 = pos;
         const double iw = Dw[w] != 0.0 ? 1.0 / Dw[w] : 0.0;
+        wofs[k] = pos;
+        idw[k] = iw;
         for (int64_t a = w_ptr[w]; a < w_ptr[w + 1]; ++a, ++pos) {
             rw[pos] = k;
             rf[pos] = m_f[a];
             rwt[pos] = m_c[a] * iw;
+            rorig[pos] = static_cast<int>(a);
+            rsq[pos] = std::sqrt(m_c[a]);
+            rcv[pos] = m_c[a];
         }
         if (pos > a0 && pos - wpiece.back() > kPiece && a0 > wpiece.back()) wpiece.push_back(a0);
     }
+    wofs[N] = M;
     wpiece.push_back(M);
     // firm-major records, a firm's matches in the new worker order
     std::vector<double> deg(J, 0.0);
-    std::vector<std::pair<int, double>> run;
+    std::vector<int> forig(M);
+    struct FirmRec {
+        int w;
+        int m;
+        double c;
+        bool operator<(const FirmRec& o) const { return w < o.w || (w == o.w && m < o.m); }
+    };
+    std::vector<FirmRec> run;
     for (int f = 0; f < J; ++f) {
         run.clear();
         for (int64_t j = f_ptr[f]; j < f_ptr[f + 1]; ++j) {
@@ -566,15 +759,16 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
                 set_error("xfe_akm_create: f_matches does not list the firm's matches");
                 return nullptr;
             }
-            run.emplace_back(newid[m_w[m]], m_c[m]);
+            run.push_back({newid[m_w[m]], static_cast<int>(m), m_c[m]});
             deg[f] += m_c[m];
         }
         std::sort(run.begin(), run.end());
         for (size_t i = 0; i < run.size(); ++i) {
             const int64_t j = f_ptr[f] + static_cast<int64_t>(i);
             fkey[j] = f;
-            fw[j] = run[i].first;
-            fc[j] = run[i].second;
+            fw[j] = run[i].w;
+            fc[j] = run[i].c;
+            forig[j] = run[i].m;
         }
         has[f] = f_ptr[f + 1] > f_ptr[f] ? 1 : 0;
         corr[f] = Df[f] - deg[f];
@@ -584,10 +778,6 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
         (void)hipGetLastError();
         c = new xfe_akm_context();
         c->N = N; c->J = J; c->M = M; c->ground = J - 1;
-        c->h_w.assign(m_w, m_w + M);
-        c->h_f.assign(m_f, m_f + M);
-        c->h_c.assign(m_c, m_c + M);
-        c->h_dw.assign(Dw, Dw + N);
         XFE_CHECK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         auto up_i = [&](DevBuf<int>& d, const std::vector<int>& h) {
             d.ensure(h.size());
@@ -600,6 +790,12 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
         c->PW = static_cast<int>(wpiece.size()) - 1;
         up_i(c->wpiece, wpiece);
         up_i(c->rw, rw);
+        up_i(c->wofs, wofs);
+        up_i(c->rorig, rorig);
+        up_i(c->forig, forig);
+        up_d(c->rsq, rsq.data(), M);
+        up_d(c->rcv, rcv.data(), M);
+        up_d(c->idw, idw.data(), N);
         up_d(c->rwt, rwt.data(), M);
         up_d(c->fc, fc.data(), M);
         up_i(c->r1, rf);
@@ -665,33 +861,6 @@ int xfe_akm_solve_S_multi(xfe_akm_context* c, const double* rhs_pack, double* z_
     }
 }
 
-namespace {
-// SplitMix64 (Steele, Lea and Flood 2014) and the per-(draw, unit) stream
-// state of the reference's JLA draws (src/akm_kss.cpp:212-222): the same
-// sequence is needed for the same leverages.
-uint64_t jla_mix(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ULL;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
-    return x ^ (x >> 31);
-}
-
-uint64_t jla_stream(uint64_t seed, uint64_t tag, uint64_t unit) {
-    const uint64_t s = jla_mix(seed ^ (0xA24BAED4963EE407ULL * (tag + 1)));
-    return jla_mix(s ^ (0x9FB21C651E98DF25ULL * (unit + 1)));
-}
-
-// signs of draw d over the M working rows: bit r mod 64 of stream (3 d, r / 64)
-void jla_signs(uint64_t seed, int d, int M, std::vector<double>& r) {
-    r.resize(M);
-    for (int b = 0; b * 64 < M; ++b) {
-        const uint64_t bits = jla_stream(seed, 3ULL * static_cast<uint64_t>(d), static_cast<uint64_t>(b));
-        const int hi = std::min(M, 64 * b + 64);
-        for (int k = 64 * b; k < hi; ++k) r[k] = ((bits >> (k - 64 * b)) & 1ULL) ? 1.0 : -1.0;
-    }
-}
-}  // namespace
-
 int xfe_akm_jla_pii(xfe_akm_context* c, int draws, uint64_t seed, double tol, int max_iter, double* pii) {
     clear_error();
     if (!c || !pii || draws < 1 || max_iter < 1) {
@@ -701,57 +870,62 @@ int xfe_akm_jla_pii(xfe_akm_context* c, int draws, uint64_t seed, double tol, in
     try {
         (void)hipGetLastError();
         const int M = c->M, N = c->N, J = c->J;
-        std::vector<double> sq(M), pacc(M, 0.0), macc(M, 0.0), r, tw, tf, zw(N);
-        for (int m = 0; m < M; ++m) sq[m] = std::sqrt(c->h_c[m]);
-        // t = A' r (A: rows sqrt(c_m) (e_w + e_f)) for draw d
-        auto scatter = [&](int d) {
-            jla_signs(seed, d, M, r);
-            tw.assign(N, 0.0);
-            tf.assign(J, 0.0);
-            for (int m = 0; m < M; ++m) {
-                const double v = sq[m] * r[m];
-                tw[c->h_w[m]] += v;
-                tf[c->h_f[m]] += v;
-            }
-        };
-        std::vector<double> pack, zpack;
+        hipStream_t s = c->s;
+        c->ensure_lanes(kMaxCgCols);
+        c->jtw.ensure(static_cast<size_t>(N) * kMaxCgCols);
+        c->pacc.ensure(M);
+        c->macc.ensure(M);
+        c->pii.ensure(M);
+        XFE_CHECK(hipMemsetAsync(c->pacc.p, 0, sizeof(double) * M, s));
+        XFE_CHECK(hipMemsetAsync(c->macc.p, 0, sizeof(double) * M, s));
         std::vector<int> its(kMaxCgCols);
         bool failed = false;
-        const double inv_p = 1.0 / draws;
         for (int d0 = 0; d0 < draws; d0 += kMaxCgCols) {
             const int nb = std::min(kMaxCgCols, draws - d0);
-            pack.assign(static_cast<size_t>(nb) * J, 0.0);
-            zpack.assign(static_cast<size_t>(nb) * J, 0.0);
-            // Schur right-hand sides tf - B' Dw^-1 tw of the two-way normal equations
-            for (int l = 0; l < nb; ++l) {
-                scatter(d0 + l);
-                double* b = pack.data() + static_cast<size_t>(l) * J;
-                for (int f = 0; f < J; ++f) b[f] = tf[f];
-                for (int m = 0; m < M; ++m) {
-                    const int w = c->h_w[m];
-                    b[c->h_f[m]] -= c->h_c[m] * tw[w] / c->h_dw[w];
-                }
+            const long long NL = static_cast<long long>(N) * nb, JL = static_cast<long long>(J) * nb;
+            // t = A' r: the worker part, then the firm part of the Schur right-hand side
+            hipLaunchKernelGGL(jla_tw_kernel, dim3(akm_grid(NL)), dim3(kBlock), 0, s, c->wofs.p, N, c->rorig.p,
+                               c->rsq.p, seed, d0, nb, c->jtw.p);
+            XFE_CHECK(hipGetLastError());
+            XFE_CHECK(hipMemsetAsync(c->b.p, 0, sizeof(double) * JL, s));
+            int m = M;
+            int pieces = (m + kPiece - 1) / kPiece;
+            hipLaunchKernelGGL(jla_firm_kernel, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
+                               dim3(kBlock), 0, s, c->fkey.p, c->fw.p, c->fc.p, c->forig.p, c->jtw.p, c->idw.p,
+                               seed, d0, m, nb, c->b.p, c->bk[0].p, c->bv[0].p, pieces == 1 ? 1 : 0);
+            XFE_CHECK(hipGetLastError());
+            int cur = 0;
+            while (pieces > 1) {
+                m = 2 * pieces;
+                pieces = (m + kPiece - 1) / kPiece;
+                hipLaunchKernelGGL(akm_list_kernel, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
+                                   dim3(kBlock), 0, s, c->bk[cur].p, c->bv[cur].p, m, nb, c->b.p, c->bk[1 - cur].p,
+                                   c->bv[1 - cur].p, pieces == 1 ? 1 : 0);
+                XFE_CHECK(hipGetLastError());
+                cur = 1 - cur;
             }
-            c->solve(pack.data(), zpack.data(), nb, tol, max_iter, its.data());
-            for (int l = 0; l < nb; ++l) {
+            XFE_CHECK(hipMemsetAsync(c->b.p + static_cast<size_t>(c->ground) * nb, 0, sizeof(double) * nb, s));
+            c->solve_levels(nb, tol, max_iter);
+            c->lane_iterations(nb, its.data());
+            for (int l = 0; l < nb; ++l)
                 if (its[l] < 0) failed = true;
-                scatter(d0 + l);  // the draw's t again (cheaper than keeping nb of them)
-                const double* zf = zpack.data() + static_cast<size_t>(l) * J;
-                for (int w = 0; w < N; ++w) zw[w] = tw[w];
-                for (int m = 0; m < M; ++m) zw[c->h_w[m]] -= c->h_c[m] * zf[c->h_f[m]];
-                for (int w = 0; w < N; ++w) zw[w] = c->h_dw[w] > 0.0 ? zw[w] / c->h_dw[w] : 0.0;
-                for (int m = 0; m < M; ++m) {
-                    const double Z = sq[m] * (zw[c->h_w[m]] + zf[c->h_f[m]]);
-                    const double e = r[m] - Z;
-                    pacc[m] += Z * Z * inv_p;
-                    macc[m] += e * e * inv_p;
-                }
-            }
+            // z = (zw, zf), Z = A z and the draws' sums per working row
+            hipLaunchKernelGGL(jla_unscale_kernel, dim3(akm_grid(JL)), dim3(kBlock), 0, s, c->x.p, J, nb, c->ground,
+                               c->scale.p + kMaxCgCols, c->cg.p, c->lanes.p);
+            XFE_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(jla_zw_kernel, dim3(akm_grid(NL)), dim3(kBlock), 0, s, c->wofs.p, N, c->r1.p,
+                               c->rcv.p, c->idw.p, c->jtw.p, c->lanes.p, nb, c->wmean.p);
+            XFE_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(jla_accumulate_kernel, dim3(akm_grid(M)), dim3(kBlock), 0, s, M, c->rw.p, c->r1.p,
+                               c->rorig.p, c->rsq.p, c->wmean.p, c->lanes.p, seed, d0, nb, 1.0 / draws, c->pacc.p,
+                               c->macc.p);
+            XFE_CHECK(hipGetLastError());
         }
-        for (int m = 0; m < M; ++m) {
-            const double den = pacc[m] + macc[m];
-            pii[m] = den > 0.0 ? pacc[m] / den : 0.0;
-        }
+        hipLaunchKernelGGL(jla_ratio_kernel, dim3(akm_grid(M)), dim3(kBlock), 0, s, M, c->pacc.p, c->macc.p,
+                           c->pii.p);
+        XFE_CHECK(hipGetLastError());
+        XFE_CHECK(hipMemcpyAsync(pii, c->pii.p, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
         if (failed) set_error("xfe_akm_jla_pii: some draw solves did not converge");
         return failed ? 1 : 0;
     } catch (const std::exception& e) {
