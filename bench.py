@@ -158,12 +158,18 @@ def bench_akm(args):
     sec = (time.perf_counter() - t0) / args.steps
     xfe_amd.profile_enable(False)
     pass_ms, launches, pass_bytes = xfe_amd.profile_get()
-    # the JLA leverages with the reference's default 200 draws (solves on the
-    # device, draws and accumulation on the host)
+    # the JLA draws with the reference's default 200 (streams, right-hand
+    # sides, solves and per-row sums on the device): P_ii, then the B_ii
+    # weights of the three components (two solves per draw)
     t0 = time.perf_counter()
     pii, jla_ok = solver.jla_pii(draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
     jla_s = time.perf_counter() - t0
     log(f"JLA P_ii, 200 draws: {jla_s:.2f} s, converged {jla_ok}, mean P_ii {pii.mean():.4f}")
+    t0 = time.perf_counter()
+    bfe, bpe, bcov, bii_ok = solver.jla_bii(None, draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
+    bii_s = time.perf_counter() - t0
+    log(f"JLA B_ii, 200 draws: {bii_s:.2f} s, converged {bii_ok}, sums {bfe.sum():.4g} {bpe.sum():.4g} "
+        f"{bcov.sum():.4g}")
     solver.close()
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     its = np.concatenate(all_its[:lanes // batch])
@@ -181,7 +187,7 @@ def bench_akm(args):
         "config": {"workload": "kss_leverage_256rhs", "obs": int(n), "workers": N, "firms": J, "matches": int(M),
                    "lanes": lanes, "lanes_per_launch": batch, "cg_tol": 1e-10},
         "iterations": {"min": int(its.min()), "max": int(its.max()), "mean": round(float(its.mean()), 1)},
-        "jla_pii_200_draws_s": round(jla_s, 3),
+        "jla_pii_200_draws_s": round(jla_s, 3), "jla_bii_200_draws_s": round(bii_s, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(os.path.join(ROOT, "profiles", "akm_firm_kss10m.md")),

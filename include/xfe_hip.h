@@ -235,6 +235,20 @@ int xfe_akm_solve_S_multi(xfe_akm_context* ctx, const double* rhs_pack, double* 
 int xfe_akm_jla_pii(xfe_akm_context* ctx, int draws, uint64_t seed, double tol, int max_iter,
                     double* pii);
 
+/* JLA estimates of the KSS quadratic-form weights B_ii of the variance
+ * components (src/akm_kss.cpp:2985-3047 and :3128-3146, match level), on the
+ * device, rows as in xfe_akm_jla_pii.  Per draw d, u_m = sum of py_count[m]
+ * Rademacher draws of the reference's stream (seed, 3 d + 1, m), scaled by
+ * 1 / sqrt(draws) and demeaned over person years (u_m - vbar py_count[m],
+ * vbar = sum u / n_py); the fe solve of [0; F'u] and the pe solve of
+ * [W'u; 0] (draws in batches of xfe_akm_max_lanes() / 2, stop at tol) give
+ * W2 = A z_fe and W1 = A z_pe; bfe += W2^2, bpe += W1^2, bcov += W1 W2 (host
+ * arrays of M).  py_count: person-year rows per match; n_py <= 0 selects
+ * sum(py_count) (the reference's n_py is the frequency-weight total).
+ * Returns 0, 1 when some draw's solve did not converge, -1 on failure. */
+int xfe_akm_jla_bii(xfe_akm_context* ctx, int draws, uint64_t seed, double tol, int max_iter,
+                    const int64_t* py_count, double n_py, double* bfe, double* bpe, double* bcov);
+
 /* akm_cuda_max_lanes (:65). */
 int xfe_akm_max_lanes(void);
 

@@ -429,6 +429,99 @@ def akm_jla_pii(st, draws, seed):
     return np.where(den > 0.0, Pacc / np.where(den > 0.0, den, 1.0), 0.0)
 
 
+def rademacher_sums(seed, tag, count):
+    """Per unit m, the sum of count[m] +-1 draws of the stream
+    stream_seed(seed, tag, m) (src/akm_kss.cpp:224-238): the state advances
+    by one SplitMix64 application per 64 draws, and each advance contributes
+    2 popcount(low k bits) - k."""
+    count = np.asarray(count, dtype=np.int64)
+    with np.errstate(over="ignore"):
+        s = stream_seed(seed, tag, np.arange(count.size, dtype=np.uint64))
+        acc = np.zeros(count.size)
+        rem = count.copy()
+        while (rem > 0).any():
+            live = rem > 0
+            s = np.where(live, splitmix64(s), s)
+            k = np.minimum(rem, 64)
+            sh = np.minimum(k, 63).astype(np.uint64)
+            mask = np.where(k >= 64, np.uint64(_U64), (np.uint64(1) << sh) - np.uint64(1))
+            pc = np.bitwise_count(s & mask).astype(np.float64)
+            acc += np.where(live, 2.0 * pc - k, 0.0)
+            rem = np.where(live, rem - k, rem)
+    return acc
+
+
+def _two_way_solve(st, Sg, tw, tf):
+    """(zw, zf) of K [zw; zf] = [tw; tf] for the match-level normal equations,
+    firms grounded at J-1 (dense Schur solve)."""
+    mw, mf, c, J = st["m_w"], st["m_f"], st["m_c"], st["J"]
+    rhs = tf - np.bincount(mf, weights=c * (tw / st["Dw"])[mw], minlength=J)
+    zf = np.zeros(J)
+    zf[:J - 1] = np.linalg.solve(Sg, rhs[:J - 1])
+    zw = (tw - np.bincount(mw, weights=c * zf[mf], minlength=st["N"])) / st["Dw"]
+    return zw, zf
+
+
+def akm_jla_bii(st, draws, seed, count):
+    """JLA estimates of the KSS quadratic-form weights B_ii of the three
+    components, per match-level working row (src/akm_kss.cpp:2985-3047 and
+    :3128-3146): per draw d, u_m = sum of count[m] Rademacher draws of the
+    stream (seed, 3 d + 1, m) scaled by 1/sqrt(p) and demeaned over person
+    years (u_m - vbar count[m], vbar = sum u / n_py), then two solves,
+    fe: K z = [0; F'u] and pe: K z = [W'u; 0], with W2 = A z_fe and
+    W1 = A z_pe on the rows (A rows sqrt(c)(e_w + e_f));
+    Bfe += W2^2, Bpe += W1^2, Bcov += W1 W2.  count: person-year rows per
+    match (the match weights when unweighted); n_py = sum of the weights."""
+    mw, mf, c = st["m_w"], st["m_f"], st["m_c"]
+    N, J = st["N"], st["J"]
+    R = mw.size
+    count = np.asarray(count, dtype=np.int64)
+    sq = np.sqrt(c)
+    S = np.zeros((J, J))
+    for f in range(J):
+        e = np.zeros(J)
+        e[f] = 1.0
+        S[:, f] = akm_apply_S(st, e)
+    Sg = S[:J - 1, :J - 1]
+    scale = 1.0 / np.sqrt(float(draws))
+    n_py = float(c.sum())
+    bfe, bpe, bcov = np.zeros(R), np.zeros(R), np.zeros(R)
+    for d in range(draws):
+        ms = rademacher_sums(seed, 3 * d + 1, count)
+        vbar = (float(ms.sum()) / n_py) * scale
+        u = ms * scale - vbar * count
+        tw = np.bincount(mw, weights=u, minlength=N)
+        tf = np.bincount(mf, weights=u, minlength=J)
+        zw, zf = _two_way_solve(st, Sg, np.zeros(N), tf)
+        W2 = sq * (zw[mw] + zf[mf])
+        zw, zf = _two_way_solve(st, Sg, tw, np.zeros(J))
+        W1 = sq * (zw[mw] + zf[mf])
+        bfe += W2 * W2
+        bpe += W1 * W1
+        bcov += W1 * W2
+    return bfe, bpe, bcov
+
+
+def ref_akm_jla_components(y, worker, firm, draws, seed):
+    """The reference's akm_kss_decompose (JLA, match level, no pruning)
+    through oracle/_ref (checker): (comp, sigma_i) with comp = plug-in, AGSU
+    and KSS (var_alpha, var_psi, cov_alpha_psi) each, sigma2_ho, n_py."""
+    r = ref()
+    if r is None:
+        return None
+    n = len(y)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    w = np.ascontiguousarray(worker, dtype=np.int32)
+    f = np.ascontiguousarray(firm, dtype=np.int32)
+    comp = np.empty(11)
+    sig = np.empty(n)
+    r.ref_akm_jla_components.argtypes = [_i, _p, _p, _p, _i, ctypes.c_ulonglong, _p, _p]
+    R = r.ref_akm_jla_components(n, _ptr(y), _ptr(w), _ptr(f), int(draws), int(seed), _ptr(comp), _ptr(sig))
+    if R < 0:
+        raise RuntimeError("ref_akm_jla_components failed")
+    return comp, sig[:R]
+
+
 def ref_akm_jla_pii(y, worker, firm, draws, seed):
     """The reference's akm_kss_decompose with JLA leverages at match level (no
     pruning) through oracle/_ref (checker): (row_worker, row_firm,

@@ -256,4 +256,38 @@ int ref_akm_jla_pii(int n, const double* y, const int* worker, const int* firm, 
     }
 }
 
+// The same decomposition's variance components and KSS sigma_i
+// (src/akm_kss.cpp:3354-3412): comp = plug-in, AGSU and KSS
+// (var_alpha, var_psi, cov_alpha_psi) each, then sigma2_ho and n_py; sigma
+// has capacity n.  Returns the number of working rows, -1 on exception.
+int ref_akm_jla_components(int n, const double* y, const int* worker, const int* firm, int draws,
+                           unsigned long long seed, double* comp, double* sigma) {
+    try {
+        Eigen::VectorXd yv = Eigen::Map<const Eigen::VectorXd>(y, n);
+        Eigen::VectorXi wv = Eigen::Map<const Eigen::VectorXi>(worker, n);
+        Eigen::VectorXi fv = Eigen::Map<const Eigen::VectorXi>(firm, n);
+        hdfe::akm::AkmOptions o;
+        o.leverage_method = hdfe::akm::LeverageMethod::Jla;
+        o.jla_draws = draws;
+        o.seed = seed;
+        o.prune = false;
+        o.num_threads = 1;
+        auto r = hdfe::akm::akm_kss_decompose(yv, wv, fv, nullptr, o);
+        const hdfe::akm::AkmComponents* cs[3] = {&r.plugin, &r.agsu, &r.kss};
+        for (int i = 0; i < 3; ++i) {
+            comp[3 * i] = cs[i]->var_alpha;
+            comp[3 * i + 1] = cs[i]->var_psi;
+            comp[3 * i + 2] = cs[i]->cov_alpha_psi;
+        }
+        comp[9] = r.sigma2_ho;
+        comp[10] = r.row_weight.sum();
+        const int R = static_cast<int>(r.sigma_i.size());
+        for (int k = 0; k < R && k < n; ++k) sigma[k] = r.sigma_i[k];
+        return R;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_akm_jla_components: %s\n", e.what());
+        return -1;
+    }
+}
+
 }  // extern "C"

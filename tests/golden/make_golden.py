@@ -91,6 +91,10 @@ def main():
     jp = jla_panel()
     rw, rf, rc, pii = oracle.ref_akm_jla_pii(jp["y"], jp["worker"], jp["firm"], JLA_DRAWS, JLA_SEED)
     out["jla/row_worker"], out["jla/row_firm"], out["jla/row_weight"], out["jla/pii"] = rw, rf, rc, pii
+    # its variance components (plug-in, AGSU, KSS; sigma2_ho, n_py) and KSS
+    # sigma_i: linear functionals of its B_ii draws
+    out["jla/components"], out["jla/sigma_i"] = oracle.ref_akm_jla_components(
+        jp["y"], jp["worker"], jp["firm"], JLA_DRAWS, JLA_SEED)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_cases.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -588,3 +588,39 @@ def test_akm_jla_pii_matches_reference(golden):
     pii2, ok = solver.jla_pii(200, 9, tol=1e-12, max_iter=5000)
     assert ok and np.abs(pii2 - oracle.akm_jla_pii(st, 200, 9)).max() <= 1e-8
     solver.close()
+
+
+def test_akm_jla_bii_matches_reference(golden):
+    """xfe_akm_jla_bii (50 draws, seed 42: batches of 16 + 16 + 16 + 2)
+    against the oracle's weights per row, and against the reference: its AGSU
+    components are plug-in - sigma2_ho sum(B) / dof and its KSS components
+    plug-in - sum(B sigma_i) / dof.  Solves stop at 1e-12 (the reference's
+    and the oracle's are direct), so the bound is 1e-8 relative."""
+    from golden_cases import JLA_DRAWS, JLA_SEED
+    _log("akm jla bii")
+    rw, rf, rc = golden["jla/row_worker"], golden["jla/row_firm"], golden["jla/row_weight"]
+    wu, wi = np.unique(rw, return_inverse=True)
+    fu, fi = np.unique(rf, return_inverse=True)
+    st = oracle.akm_build(wi, fi, rc, wu.size, fu.size)
+    cnt = rc.astype(np.int64)
+    solver = xfe_amd.AkmSolver(st)
+    bfe, bpe, bcov, ok = solver.jla_bii(cnt, JLA_DRAWS, JLA_SEED, tol=1e-12, max_iter=5000)
+    assert ok, xfe_amd.last_error()
+    want = oracle.akm_jla_bii(st, JLA_DRAWS, JLA_SEED, cnt)
+    for got, w in zip((bfe, bpe, bcov), want):
+        err = np.abs(got - w).max() / np.abs(w).max()
+        _log(f"  vs oracle {err:.2e}")
+        assert err <= 1e-8
+    comp, sig = golden["jla/components"], golden["jla/sigma_i"]
+    dof = comp[10] - 1.0
+    for i, B in enumerate((bpe, bfe, bcov)):
+        want_sum = (comp[i] - comp[3 + i]) * dof / comp[9]
+        want_sig = (comp[i] - comp[6 + i]) * dof
+        assert abs(B.sum() - want_sum) <= 1e-8 * abs(want_sum)
+        assert abs(B @ sig - want_sig) <= 1e-8 * abs(want_sig)
+    # default counts (the integral match weights), a ragged last batch
+    b2 = solver.jla_bii(None, 21, 5, tol=1e-12, max_iter=5000)
+    assert b2[3]
+    for got, w in zip(b2[:3], oracle.akm_jla_bii(st, 21, 5, cnt)):
+        assert np.abs(got - w).max() <= 1e-8 * np.abs(w).max()
+    solver.close()

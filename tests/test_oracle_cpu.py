@@ -555,3 +555,74 @@ def test_jla_pii_limit_is_exact_leverage(golden):
     exact = np.diag(H)
     est = oracle.akm_jla_pii(st, 400, 7)
     assert np.abs(est - exact).mean() < 0.02 and np.corrcoef(est, exact)[0, 1] > 0.95
+
+
+def test_rademacher_sums_closed_forms():
+    """A count of 0 sums to 0; a count of 1 is the low bit of the advanced
+    state (1 -> +1); a count of 64 + k is the 64-bit popcount of the first
+    advance plus the k-bit one of the second; parity equals count's."""
+    cnt = np.array([0, 1, 64, 70, 129, 5])
+    got = oracle.rademacher_sums(11, 4, cnt)
+    s0 = oracle.stream_seed(11, 4, np.arange(cnt.size, dtype=np.uint64))
+    s1 = oracle.splitmix64(s0)
+    s2 = oracle.splitmix64(s1)
+    s3 = oracle.splitmix64(s2)
+    pop = lambda x, k: bin(int(x) & ((1 << k) - 1)).count("1")
+    assert got[0] == 0.0
+    assert got[1] == (1.0 if int(s1[1]) & 1 else -1.0)
+    assert got[2] == 2 * pop(s1[2], 64) - 64
+    assert got[3] == 2 * pop(s1[3], 64) - 64 + 2 * pop(s2[3], 6) - 6
+    assert got[4] == 2 * (pop(s1[4], 64) + pop(s2[4], 64) + pop(s3[4], 1)) - 129
+    assert np.all((got - cnt) % 2 == 0) and np.all(np.abs(got) <= cnt)
+
+
+def _jla_bii(golden):
+    from golden_cases import JLA_DRAWS, JLA_SEED
+    st = _jla_structure(golden)
+    rc = golden["jla/row_weight"]
+    return st, rc, oracle.akm_jla_bii(st, JLA_DRAWS, JLA_SEED, rc.astype(np.int64))
+
+
+def test_jla_bii_matches_reference_components(golden):
+    """The reference's AGSU components are plug-in - sigma2_ho sum(B)/dof and
+    its KSS components plug-in - sum(B sigma_i)/dof (50 draws, seed 42):
+    the oracle's B_ii draws reproduce both linear functionals of each of the
+    three weight vectors to rounding."""
+    st, rc, (bfe, bpe, bcov) = _jla_bii(golden)
+    comp, sig = golden["jla/components"], golden["jla/sigma_i"]
+    assert comp[10] == rc.sum()
+    dof = comp[10] - 1.0
+    for i, B in enumerate((bpe, bfe, bcov)):  # var_alpha, var_psi, cov
+        want_sum = (comp[i] - comp[3 + i]) * dof / comp[9]
+        want_sig = (comp[i] - comp[6 + i]) * dof
+        assert abs(B.sum() - want_sum) <= 1e-11 * abs(want_sum)
+        assert abs(B @ sig - want_sig) <= 1e-11 * abs(want_sig)
+
+
+def test_jla_bii_limit_is_exact_quadratic_form(golden):
+    """Many draws approach the exact weights: with u's covariance over
+    person years C = diag(count) - count count'/n_py and
+    L = A K^+ [0; F'] (fe) or A K^+ [W'; 0] (pe), E Bfe = diag(L_fe C L_fe'),
+    E Bpe = diag(L_pe C L_pe'), E Bcov = diag(L_pe C L_fe') (dense)."""
+    st, rc, _ = _jla_bii(golden)
+    N, J = st["N"], st["J"]
+    M = st["m_w"].size
+    sq = np.sqrt(st["m_c"])
+    A = np.zeros((M, N + J))
+    A[np.arange(M), st["m_w"]] = sq
+    A[np.arange(M), N + st["m_f"]] = sq
+    Kp = np.linalg.pinv(A.T @ A)
+    Wm = np.zeros((N, M))
+    Wm[st["m_w"], np.arange(M)] = 1.0
+    Fm = np.zeros((J, M))
+    Fm[st["m_f"], np.arange(M)] = 1.0
+    Lfe = A @ Kp @ np.vstack([np.zeros((N, M)), Fm])
+    Lpe = A @ Kp @ np.vstack([Wm, np.zeros((J, M))])
+    C = np.diag(rc) - np.outer(rc, rc) / rc.sum()
+    efe = np.einsum("ij,jk,ik->i", Lfe, C, Lfe)
+    epe = np.einsum("ij,jk,ik->i", Lpe, C, Lpe)
+    eco = np.einsum("ij,jk,ik->i", Lpe, C, Lfe)
+    bfe, bpe, bcov = oracle.akm_jla_bii(st, 600, 3, rc.astype(np.int64))
+    for est, ex in ((bfe, efe), (bpe, epe), (bcov, eco)):
+        assert abs(est.sum() - ex.sum()) <= 0.05 * np.abs(ex).sum()
+        assert np.corrcoef(est, ex)[0, 1] > 0.9

@@ -346,14 +346,15 @@ jla_tw_kernel(const int* wofs, int N, const int* orig, const double* sq, uint64_
 }
 
 // Firm pass of the draws' Schur right-hand sides: per firm, the sum over its
-// matches of c (r / sqrt(c) - tw[w] / Dw[w]) = sqrt(c) r - c tw[w] / Dw[w],
+// matches of c (r / sqrt(c) - tw[w] / Dw[w]) = sqrt(c) r - c tw[w] / Dw[w]
+// (with su: s - c tw[w] / Dw[w], s = su of the row on lanes below h, else 0),
 // with the run rule of akm_firm_kernel (pieces of kPiece firm-major matches,
 // runs crossing a piece boundary into the boundary list).
 __global__ void __launch_bounds__(kBlock)
 jla_firm_kernel(const int* __restrict__ fkey, const int* __restrict__ fw, const double* __restrict__ fc,
                 const int* __restrict__ forig, const double* __restrict__ tw, const double* __restrict__ idw,
                 uint64_t seed, int d0, int m, int nb, double* __restrict__ out, int* __restrict__ kout,
-                double* __restrict__ vout, int final) {
+                double* __restrict__ vout, int final, const double* __restrict__ su, int h) {
     const int c = threadIdx.x % kLaneGroup;
     const int stride = gridDim.x * kGroups;
     const int pieces = (m + kPiece - 1) / kPiece;
@@ -396,7 +397,8 @@ jla_firm_kernel(const int* __restrict__ fkey, const int* __restrict__ fw, const 
                         key = k;
                         acc = 0.0;
                     }
-                    acc += x * (jla_sign(seed, d0 + cl, o) / sqrt(x) - v[u]);
+                    if (su) acc += (cl < h ? su[static_cast<size_t>(o) * h + cl] : 0.0) - x * v[u];
+                    else acc += x * (jla_sign(seed, d0 + cl, o) / sqrt(x) - v[u]);
                 }
             }
         }
@@ -478,6 +480,102 @@ __global__ void __launch_bounds__(kBlock) jla_ratio_kernel(int M, const double* 
     }
 }
 
+// ---- B_ii draws (src/akm_kss.cpp:2985-3047, :3128-3146): per draw, u_m is
+// the sum of count[m] Rademacher draws of the stream (seed, 3 d + 1, m),
+// scaled by 1 / sqrt(p) and demeaned over person years; the fe solve takes
+// [0; F'u], the pe solve [W'u; 0].  A batch holds h draws: lanes [0, h) fe,
+// [h, 2 h) pe.
+
+constexpr int kBiiDraws = kMaxCgCols / 2;
+
+// su[o h + l] = sum of the count[o] draws of row o in draw d0 + l; tot[l] +=
+// their integer total (exact, order-free).  Thread slot l = threadIdx % 16.
+__global__ void __launch_bounds__(kBlock)
+jla_msum_kernel(int M, const long long* __restrict__ cnt, uint64_t seed, int d0, int h, double* __restrict__ su,
+                unsigned long long* __restrict__ tot) {
+    __shared__ long long red[kBlock];
+    const int l = threadIdx.x % kBiiDraws;
+    const int per = kBlock / kBiiDraws;
+    long long part = 0;
+    if (l < h) {
+        const uint64_t s0 = jla_mix(seed ^ (0xA24BAED4963EE407ULL * (3ULL * static_cast<uint64_t>(d0 + l) + 2ULL)));
+        for (int o = blockIdx.x * per + threadIdx.x / kBiiDraws; o < M; o += gridDim.x * per) {
+            uint64_t st = jla_mix(s0 ^ (0x9FB21C651E98DF25ULL * (static_cast<uint64_t>(o) + 1ULL)));
+            long long rem = cnt[o], acc = 0;
+            while (rem > 0) {
+                st = jla_mix(st);
+                const int k = rem >= 64 ? 64 : static_cast<int>(rem);
+                const uint64_t bits = k == 64 ? st : (st & ((1ULL << k) - 1ULL));
+                acc += 2LL * __popcll(bits) - k;
+                rem -= k;
+            }
+            su[static_cast<size_t>(o) * h + l] = static_cast<double>(acc);
+            part += acc;
+        }
+    }
+    red[threadIdx.x] = part;
+    __syncthreads();
+    if (threadIdx.x < h) {
+        long long t = 0;
+        for (int i = threadIdx.x; i < kBlock; i += kBiiDraws) t += red[i];
+        atomicAdd(tot + threadIdx.x, static_cast<unsigned long long>(t));
+    }
+}
+
+// u = msum / sqrt(p) - vbar count, vbar = (total / n_py) / sqrt(p)
+__global__ void __launch_bounds__(kBlock)
+jla_center_kernel(int M, const long long* cnt, int h, double scale, double n_py, const unsigned long long* tot,
+                  double* su) {
+    const long long total = static_cast<long long>(M) * h;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int o = static_cast<int>(j / h);
+        const int l = static_cast<int>(j - static_cast<long long>(o) * h);
+        const double vbar = (static_cast<double>(static_cast<long long>(tot[l])) / n_py) * scale;
+        su[j] = su[j] * scale - vbar * static_cast<double>(cnt[o]);
+    }
+}
+
+// Worker part of the B batch: tw = 0 on the fe lanes, W'u on the pe lanes
+__global__ void __launch_bounds__(kBlock)
+jla_tw_bii_kernel(const int* wofs, int N, const int* orig, const double* su, int h, double* tw) {
+    const int nb = 2 * h;
+    const long long total = static_cast<long long>(N) * nb;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int w = static_cast<int>(j / nb);
+        const int l = static_cast<int>(j - static_cast<long long>(w) * nb);
+        double acc = 0.0;
+        if (l >= h)
+            for (int a = wofs[w]; a < wofs[w + 1]; ++a) acc += su[static_cast<size_t>(orig[a]) * h + (l - h)];
+        tw[j] = acc;
+    }
+}
+
+// Per working row: W2 = sqrt(c)(zw + zf) of the fe lane, W1 of the pe lane;
+// Bfe += W2^2, Bpe += W1^2, Bcov += W1 W2 in ascending draw order
+__global__ void __launch_bounds__(kBlock)
+jla_bii_accumulate_kernel(int M, const int* mw, const int* mf, const int* orig, const double* sq, const double* zw,
+                          const double* zf, int h, double* bfe, double* bpe, double* bcov) {
+    const int nb = 2 * h;
+    for (int a = blockIdx.x * kBlock + threadIdx.x; a < M; a += gridDim.x * kBlock) {
+        const int o = orig[a];
+        const double* zwa = zw + static_cast<size_t>(mw[a]) * nb;
+        const double* zfa = zf + static_cast<size_t>(mf[a]) * nb;
+        double fe = bfe[o], pe = bpe[o], co = bcov[o];
+        for (int l = 0; l < h; ++l) {
+            const double W2 = sq[a] * (zwa[l] + zfa[l]);
+            const double W1 = sq[a] * (zwa[h + l] + zfa[h + l]);
+            fe += W2 * W2;
+            pe += W1 * W1;
+            co += W1 * W2;
+        }
+        bfe[o] = fe;
+        bpe[o] = pe;
+        bcov[o] = co;
+    }
+}
+
 constexpr int kAkmMaxGrid = 2048;
 
 int akm_grid(long long work) {
@@ -497,6 +595,10 @@ struct xfe_akm_context {
     // worker-major record order; row index per firm-major record; 1 / Dw
     DevBuf<int> wofs, rorig, forig;
     DevBuf<double> rsq, rcv, idw, jtw, pacc, macc, pii;
+    // B_ii draws: person-year count per row, u per (row, draw), integer totals, sums
+    DevBuf<long long> jcnt;
+    DevBuf<double> jsu, bfe, bpe, bcov;
+    DevBuf<unsigned long long> jtot;
     // worker-major matches
     int PW = 0;  // worker pieces
     DevBuf<int> wpiece, rw, r1;
@@ -513,6 +615,14 @@ struct xfe_akm_context {
     double* h_small = nullptr;  // pinned scalars
 
     ~xfe_akm_context() {
+        for (DevBuf<int>* d : {&wofs, &rorig, &forig, &wpiece, &rw, &r1, &fkey, &fw, &has, &bk[0], &bk[1]})
+            d->release();
+        for (DevBuf<double>* d : {&rsq, &rcv, &idw, &jtw, &pacc, &macc, &pii, &jsu, &bfe, &bpe, &bcov, &rwt, &fc,
+                                  &inv_diag, &corr, &wmean, &lanes, &b, &x, &r, &z, &p, &q, &cg, &part, &scale,
+                                  &bv[0], &bv[1]})
+            d->release();
+        jcnt.release();
+        jtot.release();
         if (h_pack) (void)hipHostFree(h_pack);
         if (h_small) (void)hipHostFree(h_small);
         if (s) (void)hipStreamDestroy(s);
@@ -553,6 +663,44 @@ struct xfe_akm_context {
     void ensure_lanes(int nb) {
         const size_t JL = static_cast<size_t>(J) * nb;
         lanes.ensure(JL); b.ensure(JL); x.ensure(JL); r.ensure(JL); z.ensure(JL); p.ensure(JL); q.ensure(JL);
+    }
+
+    // One batch of draw solves from the worker part jtw (N x nb): the firm
+    // part of the Schur right-hand sides (P_ii signs, or su on lanes below h),
+    // reduced over the boundary list; the grounded solve; zf into lanes and
+    // zw into wmean.  Returns true when some lane did not converge.
+    bool jla_solve(uint64_t seed, int d0, int nb, const double* su, int h, double tol, int max_iter, int* its) {
+        const long long NL = static_cast<long long>(N) * nb, JL = static_cast<long long>(J) * nb;
+        XFE_CHECK(hipMemsetAsync(b.p, 0, sizeof(double) * JL, s));
+        int m = M;
+        int pieces = (m + kPiece - 1) / kPiece;
+        hipLaunchKernelGGL(jla_firm_kernel, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)), dim3(kBlock),
+                           0, s, fkey.p, fw.p, fc.p, forig.p, jtw.p, idw.p, seed, d0, m, nb, b.p, bk[0].p, bv[0].p,
+                           pieces == 1 ? 1 : 0, su, h);
+        XFE_CHECK(hipGetLastError());
+        int cur = 0;
+        while (pieces > 1) {
+            m = 2 * pieces;
+            pieces = (m + kPiece - 1) / kPiece;
+            hipLaunchKernelGGL(akm_list_kernel, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
+                               dim3(kBlock), 0, s, bk[cur].p, bv[cur].p, m, nb, b.p, bk[1 - cur].p, bv[1 - cur].p,
+                               pieces == 1 ? 1 : 0);
+            XFE_CHECK(hipGetLastError());
+            cur = 1 - cur;
+        }
+        XFE_CHECK(hipMemsetAsync(b.p + static_cast<size_t>(ground) * nb, 0, sizeof(double) * nb, s));
+        solve_levels(nb, tol, max_iter);
+        lane_iterations(nb, its);
+        bool failed = false;
+        for (int l = 0; l < nb; ++l)
+            if (its[l] < 0) failed = true;
+        hipLaunchKernelGGL(jla_unscale_kernel, dim3(akm_grid(JL)), dim3(kBlock), 0, s, x.p, J, nb, ground,
+                           scale.p + kMaxCgCols, cg.p, lanes.p);
+        XFE_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(jla_zw_kernel, dim3(akm_grid(NL)), dim3(kBlock), 0, s, wofs.p, N, r1.p, rcv.p, idw.p,
+                           jtw.p, lanes.p, nb, wmean.p);
+        XFE_CHECK(hipGetLastError());
+        return failed;
     }
 
     // Batched Jacobi-PCG (src/akm_kss_cuda.cu:532-695): per lane, the
@@ -882,40 +1030,12 @@ int xfe_akm_jla_pii(xfe_akm_context* c, int draws, uint64_t seed, double tol, in
         bool failed = false;
         for (int d0 = 0; d0 < draws; d0 += kMaxCgCols) {
             const int nb = std::min(kMaxCgCols, draws - d0);
-            const long long NL = static_cast<long long>(N) * nb, JL = static_cast<long long>(J) * nb;
+            const long long NL = static_cast<long long>(N) * nb;
             // t = A' r: the worker part, then the firm part of the Schur right-hand side
             hipLaunchKernelGGL(jla_tw_kernel, dim3(akm_grid(NL)), dim3(kBlock), 0, s, c->wofs.p, N, c->rorig.p,
                                c->rsq.p, seed, d0, nb, c->jtw.p);
             XFE_CHECK(hipGetLastError());
-            XFE_CHECK(hipMemsetAsync(c->b.p, 0, sizeof(double) * JL, s));
-            int m = M;
-            int pieces = (m + kPiece - 1) / kPiece;
-            hipLaunchKernelGGL(jla_firm_kernel, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
-                               dim3(kBlock), 0, s, c->fkey.p, c->fw.p, c->fc.p, c->forig.p, c->jtw.p, c->idw.p,
-                               seed, d0, m, nb, c->b.p, c->bk[0].p, c->bv[0].p, pieces == 1 ? 1 : 0);
-            XFE_CHECK(hipGetLastError());
-            int cur = 0;
-            while (pieces > 1) {
-                m = 2 * pieces;
-                pieces = (m + kPiece - 1) / kPiece;
-                hipLaunchKernelGGL(akm_list_kernel, dim3(akm_grid(static_cast<long long>(pieces) * kLaneGroup)),
-                                   dim3(kBlock), 0, s, c->bk[cur].p, c->bv[cur].p, m, nb, c->b.p, c->bk[1 - cur].p,
-                                   c->bv[1 - cur].p, pieces == 1 ? 1 : 0);
-                XFE_CHECK(hipGetLastError());
-                cur = 1 - cur;
-            }
-            XFE_CHECK(hipMemsetAsync(c->b.p + static_cast<size_t>(c->ground) * nb, 0, sizeof(double) * nb, s));
-            c->solve_levels(nb, tol, max_iter);
-            c->lane_iterations(nb, its.data());
-            for (int l = 0; l < nb; ++l)
-                if (its[l] < 0) failed = true;
-            // z = (zw, zf), Z = A z and the draws' sums per working row
-            hipLaunchKernelGGL(jla_unscale_kernel, dim3(akm_grid(JL)), dim3(kBlock), 0, s, c->x.p, J, nb, c->ground,
-                               c->scale.p + kMaxCgCols, c->cg.p, c->lanes.p);
-            XFE_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(jla_zw_kernel, dim3(akm_grid(NL)), dim3(kBlock), 0, s, c->wofs.p, N, c->r1.p,
-                               c->rcv.p, c->idw.p, c->jtw.p, c->lanes.p, nb, c->wmean.p);
-            XFE_CHECK(hipGetLastError());
+            if (c->jla_solve(seed, d0, nb, nullptr, 0, tol, max_iter, its.data())) failed = true;
             hipLaunchKernelGGL(jla_accumulate_kernel, dim3(akm_grid(M)), dim3(kBlock), 0, s, M, c->rw.p, c->r1.p,
                                c->rorig.p, c->rsq.p, c->wmean.p, c->lanes.p, seed, d0, nb, 1.0 / draws, c->pacc.p,
                                c->macc.p);
@@ -927,6 +1047,70 @@ int xfe_akm_jla_pii(xfe_akm_context* c, int draws, uint64_t seed, double tol, in
         XFE_CHECK(hipMemcpyAsync(pii, c->pii.p, sizeof(double) * M, hipMemcpyDeviceToHost, s));
         XFE_CHECK(hipStreamSynchronize(s));
         if (failed) set_error("xfe_akm_jla_pii: some draw solves did not converge");
+        return failed ? 1 : 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int xfe_akm_jla_bii(xfe_akm_context* c, int draws, uint64_t seed, double tol, int max_iter, const int64_t* py_count,
+                    double n_py, double* bfe, double* bpe, double* bcov) {
+    clear_error();
+    if (!c || !py_count || !bfe || !bpe || !bcov || draws < 1 || max_iter < 1) {
+        set_error("xfe_akm_jla_bii: invalid arguments");
+        return -1;
+    }
+    try {
+        (void)hipGetLastError();
+        const int M = c->M, N = c->N;
+        hipStream_t s = c->s;
+        long long total = 0;
+        for (int m = 0; m < M; ++m) {
+            if (py_count[m] < 0) {
+                set_error("xfe_akm_jla_bii: negative person-year count");
+                return -1;
+            }
+            total += py_count[m];
+        }
+        if (!(n_py > 0.0)) n_py = static_cast<double>(total);
+        c->ensure_lanes(kMaxCgCols);
+        c->jtw.ensure(static_cast<size_t>(N) * kMaxCgCols);
+        c->jcnt.ensure(M);
+        c->jsu.ensure(static_cast<size_t>(M) * kBiiDraws);
+        c->jtot.ensure(kBiiDraws);
+        c->bfe.ensure(M);
+        c->bpe.ensure(M);
+        c->bcov.ensure(M);
+        XFE_CHECK(hipMemcpyAsync(c->jcnt.p, py_count, sizeof(long long) * M, hipMemcpyHostToDevice, s));
+        XFE_CHECK(hipMemsetAsync(c->bfe.p, 0, sizeof(double) * M, s));
+        XFE_CHECK(hipMemsetAsync(c->bpe.p, 0, sizeof(double) * M, s));
+        XFE_CHECK(hipMemsetAsync(c->bcov.p, 0, sizeof(double) * M, s));
+        const double scale = 1.0 / std::sqrt(static_cast<double>(draws));
+        std::vector<int> its(kMaxCgCols);
+        bool failed = false;
+        for (int d0 = 0; d0 < draws; d0 += kBiiDraws) {
+            const int h = std::min(kBiiDraws, draws - d0), nb = 2 * h;
+            XFE_CHECK(hipMemsetAsync(c->jtot.p, 0, sizeof(unsigned long long) * kBiiDraws, s));
+            hipLaunchKernelGGL(jla_msum_kernel, dim3(akm_grid(static_cast<long long>(M) * kBiiDraws)), dim3(kBlock),
+                               0, s, M, c->jcnt.p, seed, d0, h, c->jsu.p, c->jtot.p);
+            XFE_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(jla_center_kernel, dim3(akm_grid(static_cast<long long>(M) * h)), dim3(kBlock), 0, s,
+                               M, c->jcnt.p, h, scale, n_py, c->jtot.p, c->jsu.p);
+            XFE_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(jla_tw_bii_kernel, dim3(akm_grid(static_cast<long long>(N) * nb)), dim3(kBlock), 0, s,
+                               c->wofs.p, N, c->rorig.p, c->jsu.p, h, c->jtw.p);
+            XFE_CHECK(hipGetLastError());
+            if (c->jla_solve(seed, d0, nb, c->jsu.p, h, tol, max_iter, its.data())) failed = true;
+            hipLaunchKernelGGL(jla_bii_accumulate_kernel, dim3(akm_grid(M)), dim3(kBlock), 0, s, M, c->rw.p, c->r1.p,
+                               c->rorig.p, c->rsq.p, c->wmean.p, c->lanes.p, h, c->bfe.p, c->bpe.p, c->bcov.p);
+            XFE_CHECK(hipGetLastError());
+        }
+        XFE_CHECK(hipMemcpyAsync(bfe, c->bfe.p, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipMemcpyAsync(bpe, c->bpe.p, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipMemcpyAsync(bcov, c->bcov.p, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        if (failed) set_error("xfe_akm_jla_bii: some draw solves did not converge");
         return failed ? 1 : 0;
     } catch (const std::exception& e) {
         set_error(e.what());

@@ -53,7 +53,7 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_profile_get", "xfe_release_workspace", "xfe_certify", "xfe_akm_available",
            "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
            "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
-           "xfe_akm_jla_pii")
+           "xfe_akm_jla_pii", "xfe_akm_jla_bii")
 
 _lib = None
 
@@ -280,6 +280,29 @@ class AkmSolver:
         if rc < 0:
             raise XfeError(last_error())
         return pii, rc == 0
+
+    def jla_bii(self, py_count=None, draws=200, seed=20260705, tol=1e-10, max_iter=10000, n_py=0.0):
+        """xfe_akm_jla_bii: JLA estimates of the KSS weights (Bfe, Bpe, Bcov)
+        of the matches from `draws` draws of the reference's streams.
+        py_count: person-year rows per match (default: the match weights,
+        which must then be integral)."""
+        if py_count is None:
+            c = np.asarray(self._keep["m_c"], dtype=np.float64)
+            py_count = np.rint(c)
+            if np.any(py_count != c):
+                raise XfeError("jla_bii: non-integral match weights need py_count")
+        cnt = np.ascontiguousarray(py_count, dtype=np.int64)
+        M = self._keep["m_w"].size
+        if cnt.shape != (M,):
+            raise XfeError("jla_bii: py_count needs one entry per match")
+        out = [np.empty(M) for _ in range(3)]
+        L = lib()
+        L.xfe_akm_jla_bii.argtypes = [_p, _i, ctypes.c_uint64, _d, _i, _p, _d, _p, _p, _p]
+        rc = L.xfe_akm_jla_bii(_p(self._ctx), int(draws), int(seed), float(tol), int(max_iter), _ptr(cnt),
+                               float(n_py), *(_ptr(a) for a in out))
+        if rc < 0:
+            raise XfeError(last_error())
+        return out[0], out[1], out[2], rc == 0
 
     def close(self):
         if self._ctx:
