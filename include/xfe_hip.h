@@ -183,6 +183,32 @@ int xfe_recovery_max_delta(const double* residual, int n, const xfe_fe_input* fe
                            int num_fes, const double* weights, const int* sweep_order,
                            double* out_max_delta);
 
+/* The GPU branch of recover_fixed_effects_group_ids (src/fe_absorption.cpp:15470-15555,
+ * called for savefe by fit, src/hdfe_regressor_v11.cpp:9335-9345): absorbs
+ * `partial` with the fixed effects retained (absorb_fixed_effects_cuda with
+ * retain_fixed_effects, src/fe_absorption_cuda.cu:4663-4690) and writes each
+ * dimension's level coefficients alpha_out[d] (num_groups doubles, host) and,
+ * when contributions_out is not NULL, contributions_out[d*n + i] =
+ * alpha_out[d][ids_d[i]] (FeRecoveryResult::contributions).  max_delta is
+ * fe_recovery_max_delta of partial - sum_d contributions_d (one Gauss-Seidel
+ * pass, as xfe_recovery_max_delta); converged = absorber converged &&
+ * max_delta <= fe_tol (fe_tol <= 0 selects HdfeOptions::fe_tolerance's 1e-6).
+ * The coefficients are one solution of D alpha = partial - M_D partial; like
+ * the reference's, they are identified up to a constant per dimension and
+ * mobility group.  1 to 3 dimensions; weights only with one dimension.
+ * Returns 1 on success, 0 on failure (then see xfe_last_error). */
+typedef struct {
+    int iterations;        /* FeRecoveryResult::iterations (inner iterations of the solve) */
+    int converged;         /* FeRecoveryResult::converged */
+    double max_delta;      /* FeRecoveryResult::max_delta */
+    int absorb_converged;  /* the absorber's own flag */
+} xfe_recovery_result;
+
+int xfe_recover_fixed_effects(const double* partial, int n, const xfe_fe_input* fes, int num_fes,
+                              const double* weights, const int* sweep_order, const xfe_options* opt,
+                              double fe_tol, double* const* alpha_out, double* contributions_out,
+                              xfe_recovery_result* out);
+
 /* ---- AKM/KSS leverage solver (include/hdfe/akm_kss_cuda.hpp:19-74, impl
  * src/akm_kss_cuda.cu:164-722): the grounded worker-partialled firm Laplacian
  * S = Df - B' Dw^{-1} B over n_firms levels (ground: the last firm), B the

@@ -24,6 +24,11 @@ under ``xhdfe-xfe_amd/``) never imports it and never falls back to it.
   term (the fixed point of slope_project_inplace, src/fe_absorption.cpp:
   4096-4300) by dense least squares; pinned against the reference's slope
   absorber (``ref_absorb_slope``) and per-level polyfit.
+* ``recover_fixed_effects`` / ``fe_recovery_max_delta``: numpy restatement of
+  the savefe recovery's Gauss-Seidel phase with coefficient accumulation and
+  its max-delta check (src/fe_absorption.cpp:14447-14935, :15563-15700);
+  pinned against the reference's own recovery (``ref_recover_fe``) on the
+  golden panels and against the exact fitted component partial - M_D partial.
 """
 from __future__ import annotations
 
@@ -520,6 +525,75 @@ def ref_akm_jla_components(y, worker, firm, draws, seed):
     if R < 0:
         raise RuntimeError("ref_akm_jla_components failed")
     return comp, sig[:R]
+
+
+def _level_means(r, gid, G, w):
+    if w is None:
+        return np.bincount(gid, weights=r, minlength=G) / np.maximum(np.bincount(gid, minlength=G), 1)
+    ws = np.bincount(gid, weights=w, minlength=G)
+    return np.where(ws > 0, np.bincount(gid, weights=w * r, minlength=G) / np.where(ws > 0, ws, 1.0), 0.0)
+
+
+def fe_recovery_max_delta(residual, gids, G, weights=None, order=None):
+    """fe_recovery_max_delta (src/fe_absorption.cpp:15563-15700): one
+    Gauss-Seidel pass over the residual in sweep order; the largest absolute
+    weighted level mean met on the way."""
+    r = np.array(residual, dtype=np.float64)
+    mx = 0.0
+    for d in (order if order is not None else range(len(gids))):
+        m = _level_means(r, gids[d], G[d], weights)
+        mx = max(mx, float(np.abs(m).max()) if m.size else 0.0)
+        r -= m[gids[d]]
+    return mx
+
+
+def recover_fixed_effects(partial, gids, G, weights=None, fe_tol=1e-6, max_iter=100000, order=None):
+    """The savefe recovery (recover_fixed_effects_impl,
+    src/fe_absorption.cpp:14447-14935, its plain Gauss-Seidel phase): from
+    alpha = 0, sweep the dimensions in order, each taking the weighted level
+    means m of the current residual partial - sum_d alpha_d[ids_d] into
+    alpha_d; stop once the largest |m| of a sweep is below fe_tol.  Returns
+    (alphas, iterations, max_delta, converged).  The reference's accelerated
+    phases change the trajectory, not the fixed point D alpha = partial -
+    M_D partial."""
+    r = np.array(partial, dtype=np.float64)
+    D = len(gids)
+    alphas = [np.zeros(int(G[d])) for d in range(D)]
+    seq = list(order if order is not None else range(D))
+    mx = 0.0
+    for it in range(1, max_iter + 1):
+        mx = 0.0
+        for d in seq:
+            m = _level_means(r, gids[d], G[d], weights)
+            alphas[d] += m
+            r -= m[gids[d]]
+            mx = max(mx, float(np.abs(m).max()) if m.size else 0.0)
+        if mx < fe_tol:
+            return alphas, it, mx, True
+    return alphas, max_iter, mx, False
+
+
+def ref_recover_fe(partial, fes, tol=1e-8):
+    """The reference's recover_fixed_effects (src/fe_absorption.cpp:15316)
+    through oracle/_ref (checker): (contributions d x n, iterations,
+    max_delta, converged)."""
+    r = ref()
+    if r is None:
+        return None
+    n, d = len(partial), len(fes)
+    pv = np.ascontiguousarray(partial, dtype=np.float64)
+    F = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    out = np.empty((d, n))
+    it, conv = ctypes.c_int(), ctypes.c_int()
+    md = ctypes.c_double()
+    r.ref_recover_fe.argtypes = [_i, _i, _p, _p, ctypes.c_double, _i, _p,
+                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_int)]
+    rc = r.ref_recover_fe(n, d, _ptr(pv), _ptr(F), float(tol), 1, _ptr(out), ctypes.byref(it),
+                          ctypes.byref(md), ctypes.byref(conv))
+    if rc != 0:
+        raise RuntimeError("ref_recover_fe failed")
+    return out, it.value, md.value, conv.value
 
 
 def ref_akm_jla_pii(y, worker, firm, draws, seed):

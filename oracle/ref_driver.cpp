@@ -290,4 +290,29 @@ int ref_akm_jla_components(int n, const double* y, const int* worker, const int*
     }
 }
 
+// detail::recover_fixed_effects (src/fe_absorption.cpp:15316-15354, impl
+// :14447-14935): FE values of `partial` by the reference's CPU recovery.
+// contrib has capacity d*n (dimension-major): contributions[dim][i] =
+// alpha_dim[fes_dim[i]]; tol is HdfeOptions::fe_tolerance.  Returns 0 on success, -1 on exception.
+int ref_recover_fe(int n, int d, const double* partial, const int* fes, double tol, int threads,
+                   double* contrib, int* iterations, double* max_delta, int* converged) {
+    try {
+        Eigen::VectorXd pv = Eigen::Map<const Eigen::VectorXd>(partial, n);
+        auto fv = make_fes(fes, n, d);
+        hdfe::HdfeOptions o = make_options(1e-8, 1, threads);
+        o.fe_tolerance = tol;
+        o.max_iter = 100000;
+        auto r = hdfe::detail::recover_fixed_effects(pv, fv, nullptr, o);
+        for (int k = 0; k < d; ++k)
+            std::memcpy(contrib + static_cast<std::size_t>(k) * n, r.contributions[k].data(), sizeof(double) * n);
+        *iterations = r.iterations;
+        *max_delta = r.max_delta;
+        *converged = r.converged ? 1 : 0;
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_recover_fe: %s\n", e.what());
+        return -1;
+    }
+}
+
 }  // extern "C"

@@ -56,6 +56,12 @@ def main():
             c = oracle.ref_certify(yk, Xk, yo, Xo, fk, weights=w, tol=tol, converged=conv)
             out[f"{name}/{tag}/cert_w"] = np.array([c["abs_residual"], c["abs_residual_rel"],
                                                     float(c["certified"])])
+        # savefe recovery of y's FE values (recover_fixed_effects), at the
+        # default fe_tolerance and tightly
+        for tag, ftol in (("default", 1e-6), ("tight", 1e-12)):
+            contrib, it, md, conv = oracle.ref_recover_fe(yk, fk, tol=ftol)
+            out[f"{name}/recover_{tag}/contributions"] = contrib
+            out[f"{name}/recover_{tag}/info"] = np.array([it, md, conv], dtype=np.float64)
         # schwarz_demean_raw (the PCG demeaner) on the kept rows with compact ids
         ck = [oracle.build_indexer(f) for f in fk]
         for tag, tol in (("tight", 1e-12), ("default", 1e-8)):

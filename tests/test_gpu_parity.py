@@ -624,3 +624,70 @@ def test_akm_jla_bii_matches_reference(golden):
     for got, w in zip(b2[:3], oracle.akm_jla_bii(st, 21, 5, cnt)):
         assert np.abs(got - w).max() <= 1e-8 * np.abs(w).max()
     solver.close()
+
+
+# ---- savefe FE recovery (xfe_recover_fixed_effects)
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_recover_fixed_effects_matches_reference(golden, cases, name):
+    """The device recovery against the reference's own recovery and the
+    oracle: the per-row sum of the FE values (unique, unlike the values) within
+    1e-8 of the column scale of the reference's tight recovery and of the
+    exact fitted component; contributions expand the level values; max_delta
+    is the oracle's check of the same residual."""
+    _log(f"recover {name}")
+    _, fes, gids, G, V = _prep(cases[name])
+    y = V[:, 0]
+    alphas, contrib, info = xfe_amd.recover_fixed_effects(y, gids, G, fe_tol=1e-6)
+    assert info["converged"] and info["absorb_converged"], (info, xfe_amd.last_error())
+    for d in range(len(gids)):
+        np.testing.assert_array_equal(contrib[d], alphas[d][gids[d]])
+    tot = contrib.sum(0)
+    scale = np.abs(y).max()
+    ref = golden[f"{name}/recover_tight/contributions"]
+    assert _rel(tot, ref.sum(0), scale) < 1e-8
+    Vo, _ = oracle.absorb(V, gids, G, tol=1e-14)
+    assert _rel(tot, y - Vo[:, 0], scale) < 1e-8
+    md = oracle.fe_recovery_max_delta(y - tot, gids, G)
+    assert info["max_delta"] < 1e-8
+    assert abs(info["max_delta"] - md) <= 1e-12 + 1e-6 * md, (info["max_delta"], md)
+
+
+def test_recover_fixed_effects_akm_scaled():
+    """A 1 % AKM-shape panel (three dimensions): the sum of the recovered FE
+    values is the exact fitted component, and the check pass is tiny."""
+    _log("recover akm 1%")
+    p = panels.akm_small(scale=0.01)
+    _, fes, gids, G, V = _prep(p)
+    y = V[:, 0]
+    alphas, contrib, info = xfe_amd.recover_fixed_effects(y, gids, G)
+    assert info["converged"], info
+    Vo, _ = oracle.absorb(V[:, :1], gids, G, tol=1e-13)
+    scale = np.abs(y).max()
+    assert _rel(contrib.sum(0), y - Vo[:, 0], scale) < 1e-8
+    assert info["max_delta"] < 1e-8
+
+
+def test_recover_fixed_effects_one_dimension_weighted():
+    """One weighted dimension: the values are the weighted group means of
+    partial (closed form)."""
+    _log("recover one dim weighted")
+    rng = np.random.default_rng(11)
+    n, G = 20000, 300
+    g = rng.integers(0, G, n).astype(np.int32)
+    w = 0.5 + rng.random(n)
+    y = rng.standard_normal(n)
+    alphas, contrib, info = xfe_amd.recover_fixed_effects(y, [g], [G], weights=w)
+    assert info["converged"]
+    exact = np.bincount(g, w * y, G) / np.bincount(g, w, G)
+    np.testing.assert_allclose(alphas[0], exact, rtol=0, atol=1e-13)
+    assert info["max_delta"] < 1e-13
+
+
+def test_recover_fixed_effects_rejects_weighted_multi_dim():
+    rng = np.random.default_rng(12)
+    n = 1000
+    a = rng.integers(0, 50, n).astype(np.int32)
+    b = rng.integers(0, 20, n).astype(np.int32)
+    with pytest.raises(xfe_amd.XfeError):
+        xfe_amd.recover_fixed_effects(rng.standard_normal(n), [a, b], [50, 20], weights=np.ones(n))

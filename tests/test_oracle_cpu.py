@@ -626,3 +626,71 @@ def test_jla_bii_limit_is_exact_quadratic_form(golden):
     for est, ex in ((bfe, efe), (bpe, epe), (bcov, eco)):
         assert abs(est.sum() - ex.sum()) <= 0.05 * np.abs(ex).sum()
         assert np.corrcoef(est, ex)[0, 1] > 0.9
+
+
+# ---- savefe FE recovery (recover_fixed_effects, src/fe_absorption.cpp:14447-15700)
+
+def _recover_case(cases, name):
+    p = cases[name]
+    keep = _kept(p)
+    fk = [f[keep] for f in p["fes"]]
+    ck = [oracle.build_indexer(f) for f in fk]
+    return p["y"][keep], p["X"][keep], fk, [c[0] for c in ck], [c[1] for c in ck]
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_recover_matches_reference(golden, cases, name):
+    """The oracle's recovery against the reference's own (tight fe_tolerance):
+    the FE values are identified only up to constants per dimension and
+    mobility group, so their sum per row is what both must agree on; and it
+    is the exact fitted component partial - M_D partial."""
+    y, X, fk, gids, G = _recover_case(cases, name)
+    alphas, it, md, conv = oracle.recover_fixed_effects(y, gids, G, fe_tol=1e-12)
+    assert conv and md < 1e-12
+    tot = sum(a[g] for a, g in zip(alphas, gids))
+    ref = golden[f"{name}/recover_tight/contributions"]
+    scale = np.abs(y).max()
+    assert np.abs(tot - ref.sum(0)).max() / scale < 1e-8
+    Vo, _ = oracle.absorb(np.column_stack([y, X]), gids, G, tol=1e-14)
+    assert np.abs(tot - (y - Vo[:, 0])).max() / scale < 1e-9
+    # the reference's default fe_tolerance stops early: its sum is within what
+    # its max_delta allows
+    refd = golden[f"{name}/recover_default/contributions"]
+    it_d, md_d, conv_d = golden[f"{name}/recover_default/info"]
+    assert conv_d == 1 and md_d < 1e-6
+    assert np.abs(tot - refd.sum(0)).max() / scale < 1e-4
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_recovery_max_delta_of_reference_output(golden, cases, name):
+    """fe_recovery_max_delta of the reference's converged recovery: one more
+    Gauss-Seidel pass moves no level by more than the reference's stopping
+    threshold allows."""
+    y, _, _, gids, G = _recover_case(cases, name)
+    for tag, ftol in (("tight", 1e-12), ("default", 1e-6)):
+        C = golden[f"{name}/recover_{tag}/contributions"]
+        md = oracle.fe_recovery_max_delta(y - C.sum(0), gids, G)
+        _, md_ref, _ = golden[f"{name}/recover_{tag}/info"]
+        assert md <= 2.0 * ftol, (tag, md, md_ref)
+
+
+def test_recover_tiny_is_least_squares():
+    """On a tiny weighted two-way design the recovered coefficients fit
+    partial exactly as dense weighted least squares does (the fitted values
+    are unique even where the coefficients are not)."""
+    rng = np.random.default_rng(5)
+    n = 60
+    a = rng.integers(0, 7, n).astype(np.int32)
+    b = rng.integers(0, 5, n).astype(np.int32)
+    w = 0.5 + rng.random(n)
+    y = rng.standard_normal(n)
+    alphas, _, md, conv = oracle.recover_fixed_effects(y, [a, b], [7, 5], weights=w, fe_tol=1e-13)
+    assert conv
+    Dm = np.hstack([np.eye(7)[a], np.eye(5)[b]])
+    sw = np.sqrt(w)
+    coef = np.linalg.lstsq(Dm * sw[:, None], y * sw, rcond=None)[0]
+    np.testing.assert_allclose(alphas[0][a] + alphas[1][b], Dm @ coef, atol=1e-11)
+    # one dimension: the weighted group means, exactly
+    al, it, _, _ = oracle.recover_fixed_effects(y, [a], [7], weights=w)
+    assert it == 2 or it == 1
+    np.testing.assert_allclose(al[0], np.bincount(a, w * y, 7) / np.bincount(a, w, 7), rtol=1e-14)

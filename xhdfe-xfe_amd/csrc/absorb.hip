@@ -124,6 +124,17 @@ struct Workspace {
     DevBuf<int> seg_bk[2];
     DevBuf<float> f_bv[2];
     DevBuf<double> d_bv[2];
+    // FE recovery: the untransformed column and the level coefficients
+    DevBuf<double> rec_p;
+    DevBuf<double> rec_alpha[3];
+    // the last mixed-precision solve of this workspace: primary dimension and
+    // where each other dimension's coefficients sit in f_beta (level-major, C wide)
+    struct {
+        bool used = false;
+        int p = -1, NR = 0, C = 0;
+        int rdim[2] = {-1, -1};
+        int off[2] = {0, 0};
+    } fast_rec;
     double* host_scalars = nullptr;  // pinned
 
     // Frees every device resource of this thread's workspace.
@@ -153,6 +164,9 @@ struct Workspace {
         p_rk.release(); p_r1.release(); p_len.release(); p_a.release(); p_wseg.release();
         f_invw.release(); d_invw.release();
         for (int k = 0; k < 2; ++k) { seg_bk[k].release(); f_bv[k].release(); d_bv[k].release(); }
+        rec_p.release();
+        for (auto& b : rec_alpha) b.release();
+        fast_rec.used = false;
         if (host_scalars) (void)hipHostFree(host_scalars);
         if (stream) (void)hipStreamDestroy(stream);
         host_scalars = nullptr; stream = nullptr; device = -1;
@@ -1209,6 +1223,12 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
         converged = rel <= std::max(target, 1e-9);
     }
     fs.residual64(V, V);  // final within transform, in place
+    auto& rec = sv.ws.fast_rec;
+    rec.used = true;
+    rec.p = fs.p;
+    rec.NR = fs.NR;
+    rec.C = C;
+    for (int k = 0; k < 2; ++k) { rec.rdim[k] = fs.rdim[k]; rec.off[k] = fs.off[k]; }
     res.iterations = fs.inner_total;
     res.refinements = refinements;
     res.converged = converged ? 1 : 0;
@@ -1678,6 +1698,7 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
                                xfe_absorb_result& res, SchwarzRule* rule = nullptr) {
     Workspace& ws = tls_ws;
     ws.init();
+    ws.fast_rec.used = false;
     hipStream_t s = ws.stream;
     const int D = static_cast<int>(G.size());
     const auto t0 = std::chrono::steady_clock::now();
@@ -1956,6 +1977,55 @@ static void absorb_slopes_impl(double* V_in, int n, int C, int ld, const std::ve
 }  // namespace xfe
 
 using namespace xfe;
+
+// One Gauss-Seidel pass over the residual in ws.vec[6] (ld = n rounded up to
+// 64; modified) with the ids staged in ws.stage_ids (dimension-major, n):
+// max_d max_g |weighted mean_g| (fe_recovery_max_delta_cuda,
+// src/fe_absorption_cuda.cu:5227).  w: device weights or nullptr.
+static double max_delta_staged(Workspace& ws, int n, const xfe_fe_input* fes, int num_fes, const double* w,
+                               const int* sweep_order) {
+    hipStream_t s = ws.stream;
+    const int ld = ((n + 63) / 64) * 64;
+    Solver sv(ws, n, 1, ld);
+    sv.D = num_fes;
+    sv.w = w;
+    size_t part = 0;
+    for (int d = 0; d < num_fes; ++d) {
+        if (!fes[d].group_ids || fes[d].num_groups <= 0) throw Error("invalid FE input");
+        const int G = fes[d].num_groups;
+        int* dst = ws.stage_ids.p + static_cast<size_t>(d) * n;
+        sv.gid[d] = dst;
+        sv.G[d] = G;
+        sv.prepare_dim(d);
+        if (fes[d].weight_sums) {
+            XFE_CHECK(hipMemcpyAsync(ws.wsum[d].p, fes[d].weight_sums, sizeof(double) * G,
+                                     hipMemcpyHostToDevice, s));
+            XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G, grid_for(G), s));
+        } else {
+            sv.weight_sums(d);
+        }
+        part += grid_for(G);
+    }
+    ws.partials.ensure(part + static_cast<size_t>(std::max(sv.grid, grid_for(n))) * 3 + 1024);
+    ws.scalars.ensure(8);
+    std::vector<int> order(num_fes);
+    for (int d = 0; d < num_fes; ++d) order[d] = sweep_order ? sweep_order[d] : d;
+    // one Gauss-Seidel pass: reduce d, means (+ max |mean|), subtract
+    size_t off = 0;
+    sv.pass(ws.vec[6].p, nullptr, nullptr, nullptr, 0.0, nullptr, -1, order[0], 0, nullptr,
+            nullptr, nullptr);
+    for (int j = 0; j < num_fes; ++j) {
+        const int d = order[j];
+        sv.finalize(d, ws.partials.p + off);
+        off += sv.finalize_grid(d);
+        const int r = j + 1 < num_fes ? order[j + 1] : -1;
+        sv.pass(ws.vec[6].p, ws.vec[6].p, nullptr, nullptr, 0.0, nullptr, d, r, 0, nullptr,
+                nullptr, nullptr);
+    }
+    double mx = 0.0;
+    sv.fetch(ws.partials.p, static_cast<int>(off), 1, 1, &mx);
+    return mx;
+}
 
 extern "C" {
 
@@ -2256,47 +2326,128 @@ int xfe_recovery_max_delta(const double* residual, int n, const xfe_fe_input* fe
             XFE_CHECK(hipMemcpyAsync(ws.stage_w.p, weights, sizeof(double) * n, hipMemcpyHostToDevice, s));
             w = ws.stage_w.p;
         }
-        Solver sv(ws, n, 1, ld);
-        sv.D = num_fes;
-        sv.w = w;
         ws.stage_ids.ensure(static_cast<size_t>(n) * num_fes);
-        size_t part = 0;
         for (int d = 0; d < num_fes; ++d) {
             if (!fes[d].group_ids || fes[d].num_groups <= 0) throw Error("invalid FE input");
-            const int G = fes[d].num_groups;
+            XFE_CHECK(hipMemcpyAsync(ws.stage_ids.p + static_cast<size_t>(d) * n, fes[d].group_ids,
+                                     sizeof(int) * n, hipMemcpyHostToDevice, s));
+        }
+        const double mx = max_delta_staged(ws, n, fes, num_fes, w, sweep_order);
+        *out_max_delta = mx;
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+int xfe_recover_fixed_effects(const double* partial, int n, const xfe_fe_input* fes, int num_fes,
+                              const double* weights, const int* sweep_order, const xfe_options* opt,
+                              double fe_tol, double* const* alpha_out, double* contributions_out,
+                              xfe_recovery_result* out) {
+    clear_error();
+    if (!partial || !opt || !out || !alpha_out || n < 0 || num_fes < 0 || num_fes > 3 || !fes) {
+        set_error("xfe_recover_fixed_effects: invalid arguments");
+        return 0;
+    }
+    std::memset(out, 0, sizeof(*out));
+    for (int d = 0; d < num_fes; ++d) {
+        if (!fes[d].group_ids || fes[d].num_groups <= 0 || fes[d].is_slope || !alpha_out[d]) {
+            set_error("xfe_recover_fixed_effects: invalid FE input (ids, levels, no slopes, alpha_out)");
+            return 0;
+        }
+    }
+    if (weights && num_fes >= 2) {
+        set_error("xfe_recover_fixed_effects: weighted recovery over two or more dimensions is not supported");
+        return 0;
+    }
+    if (fe_tol <= 0.0) fe_tol = 1e-6;  // HdfeOptions::fe_tolerance
+    if (n == 0 || num_fes == 0) {
+        for (int d = 0; d < num_fes; ++d) std::fill(alpha_out[d], alpha_out[d] + fes[d].num_groups, 0.0);
+        out->converged = 1;
+        out->absorb_converged = 1;
+        return 1;
+    }
+    try {
+        (void)hipGetLastError();
+        Workspace& ws = tls_ws;
+        ws.init();
+        hipStream_t s = ws.stream;
+        const int grid = grid_for(n);
+        ws.stage.ensure(n);
+        ws.rec_p.ensure(n);
+        XFE_CHECK(hipMemcpyAsync(ws.rec_p.p, partial, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        XFE_CHECK(hipMemcpyAsync(ws.stage.p, ws.rec_p.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        ws.stage_ids.ensure(static_cast<size_t>(n) * num_fes);
+        std::vector<const int*> gids(num_fes);
+        std::vector<int> G(num_fes);
+        std::vector<const double*> wsum(num_fes);
+        for (int d = 0; d < num_fes; ++d) {
             int* dst = ws.stage_ids.p + static_cast<size_t>(d) * n;
             XFE_CHECK(hipMemcpyAsync(dst, fes[d].group_ids, sizeof(int) * n, hipMemcpyHostToDevice, s));
-            sv.gid[d] = dst;
-            sv.G[d] = G;
-            sv.prepare_dim(d);
-            if (fes[d].weight_sums) {
-                XFE_CHECK(hipMemcpyAsync(ws.wsum[d].p, fes[d].weight_sums, sizeof(double) * G,
-                                         hipMemcpyHostToDevice, s));
-                XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G, grid_for(G), s));
-            } else {
-                sv.weight_sums(d);
+            gids[d] = dst;
+            G[d] = fes[d].num_groups;
+            wsum[d] = fes[d].weight_sums;
+        }
+        const double* w = nullptr;
+        if (weights) {
+            ws.stage_w.ensure(n);
+            XFE_CHECK(hipMemcpyAsync(ws.stage_w.p, weights, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            w = ws.stage_w.p;
+        }
+        // within transform by the Schur path: its FP64 coefficients are those of
+        // the non-primary dimensions
+        xfe_options o = *opt;
+        o.precision = 1;
+        o.certify = 0;
+        xfe_absorb_result ares;
+        std::memset(&ares, 0, sizeof(ares));
+        absorb_device_impl(ws.stage.p, n, 1, n, gids, G, wsum, w, sweep_order, o, ares);
+        const auto& rec = ws.fast_rec;
+        if (num_fes >= 2 && !rec.used)
+            throw Error("xfe_recover_fixed_effects: the panel does not take the Schur-complement path");
+        RecoverArgs a;
+        a.partial = ws.rec_p.p;
+        a.vt = ws.stage.p;
+        a.w = w;
+        a.D = num_fes;
+        a.p = num_fes >= 2 ? rec.p : 0;
+        for (int d = 0; d < num_fes; ++d) {
+            ws.rec_alpha[d].ensure(G[d]);
+            a.gid[d] = gids[d];
+            a.alpha[d] = ws.rec_alpha[d].p;
+        }
+        for (int k = 0; num_fes >= 2 && k < rec.NR; ++k) {
+            const int d = rec.rdim[k];
+            XFE_CHECK(hipMemcpyAsync(ws.rec_alpha[d].p, ws.f_beta.p + rec.off[k], sizeof(double) * G[d],
+                                     hipMemcpyDeviceToDevice, s));
+        }
+        const int p = a.p;
+        XFE_CHECK(hipMemsetAsync(ws.rec_alpha[p].p, 0, sizeof(double) * G[p], s));
+        XFE_CHECK(launch_recover_primary(a, ws.rec_alpha[p].p, n, grid, s));
+        // ws.wsum[p]: the primary dimension's weight sums, filled by the absorber
+        XFE_CHECK(launch_recover_divide(ws.rec_alpha[p].p, ws.wsum[p].p, G[p], grid_for(G[p]), s));
+        // the reference's check: one Gauss-Seidel pass over partial - sum of contributions
+        const int ld = ((n + 63) / 64) * 64;
+        ws.vec[6].ensure(ld);
+        XFE_CHECK(launch_recover_residual(a, ws.vec[6].p, n, grid, s));
+        for (int d = 0; d < num_fes; ++d)
+            XFE_CHECK(hipMemcpyAsync(alpha_out[d], ws.rec_alpha[d].p, sizeof(double) * G[d],
+                                     hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        const double mx = max_delta_staged(ws, n, fes, num_fes, w, sweep_order);
+        if (contributions_out) {
+            for (int d = 0; d < num_fes; ++d) {
+                const int* g = fes[d].group_ids;
+                const double* al = alpha_out[d];
+                double* c = contributions_out + static_cast<size_t>(d) * n;
+                for (int i = 0; i < n; ++i) c[i] = al[g[i]];
             }
-            part += grid_for(G);
         }
-        ws.partials.ensure(part + static_cast<size_t>(std::max(sv.grid, grid_for(n))) * 3 + 1024);
-        ws.scalars.ensure(8);
-        std::vector<int> order(num_fes);
-        for (int d = 0; d < num_fes; ++d) order[d] = sweep_order ? sweep_order[d] : d;
-        // one Gauss-Seidel pass: reduce d, means (+ max |mean|), subtract
-        size_t off = 0;
-        sv.pass(ws.vec[6].p, nullptr, nullptr, nullptr, 0.0, nullptr, -1, order[0], 0, nullptr,
-                nullptr, nullptr);
-        for (int j = 0; j < num_fes; ++j) {
-            const int d = order[j];
-            sv.finalize(d, ws.partials.p + off);
-            off += sv.finalize_grid(d);
-            const int r = j + 1 < num_fes ? order[j + 1] : -1;
-            sv.pass(ws.vec[6].p, ws.vec[6].p, nullptr, nullptr, 0.0, nullptr, d, r, 0, nullptr,
-                    nullptr, nullptr);
-        }
-        double mx = 0.0;
-        sv.fetch(ws.partials.p, static_cast<int>(off), 1, 1, &mx);
-        *out_max_delta = mx;
+        out->iterations = ares.iterations;
+        out->absorb_converged = ares.converged;
+        out->max_delta = mx;
+        out->converged = (ares.converged && mx <= fe_tol) ? 1 : 0;
         return 1;
     } catch (const std::exception& e) {
         set_error(e.what());

@@ -274,6 +274,22 @@ void profile_end(int slot, hipStream_t s, double bytes);
 // Waits for the recorded launches and adds them to g_profile.
 void profile_resolve();
 
+// FE recovery: rows in the caller's order; gid[d] n ids, alpha[d] level
+// coefficients of dimension d (alpha[p] is being formed by recover_primary).
+struct RecoverArgs {
+    const double* partial = nullptr;
+    const double* vt = nullptr;  // within transform of partial
+    const double* w = nullptr;
+    const int* gid[3] = {nullptr, nullptr, nullptr};
+    const double* alpha[3] = {nullptr, nullptr, nullptr};
+    int D = 0;
+    int p = 0;
+};
+hipError_t launch_recover_primary(const RecoverArgs& a, double* sums, int n, int grid, hipStream_t s);
+hipError_t launch_recover_divide(double* alpha, const double* wsum, int G, int grid, hipStream_t s);
+hipError_t launch_recover_residual(const RecoverArgs& a, double* res, int n, int grid, hipStream_t s);
+
 }  // namespace xfe
+
 
 #endif

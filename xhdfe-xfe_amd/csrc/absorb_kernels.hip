@@ -670,4 +670,55 @@ hipError_t launch_sorted_check(const int* gid, int n, int* flag, int grid, hipSt
     return hipGetLastError();
 }
 
+// FE recovery (savefe).  After the within transform vt = M_D partial by the
+// Schur path, the coefficients of the non-primary dimensions are known; the
+// primary level g gets alpha_p[g] = weighted mean over its rows of
+// partial - vt - sum_k coef_k[gid_k]: the rows of one level carry the same
+// value up to rounding, and the mean absorbs the rounding.
+__global__ void recover_primary_kernel(RecoverArgs a, double* __restrict__ sums, int n) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double t = a.partial[i] - a.vt[i];
+        for (int k = 0; k < a.D; ++k) {
+            if (k == a.p) continue;
+            t -= a.alpha[k][a.gid[k][i]];
+        }
+        if (a.w) t *= a.w[i];
+        atomicAdd(&sums[a.gid[a.p][i]], t);
+    }
+}
+
+// alpha[g] = sums[g] / wsum[g] in place (0 for a level without weight).
+__global__ void recover_divide_kernel(double* __restrict__ alpha, const double* __restrict__ wsum, int G) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += stride)
+        alpha[g] = wsum[g] > 0.0 ? alpha[g] / wsum[g] : 0.0;
+}
+
+// res_i = partial_i - sum_d alpha_d[gid_d[i]] (the residual fe_recovery_max_delta checks).
+__global__ void recover_residual_kernel(RecoverArgs a, double* __restrict__ res, int n) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double t = a.partial[i];
+        for (int k = 0; k < a.D; ++k) t -= a.alpha[k][a.gid[k][i]];
+        res[i] = t;
+    }
+}
+
+hipError_t launch_recover_primary(const RecoverArgs& a, double* sums, int n, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(recover_primary_kernel, dim3(grid), dim3(kBlock), 0, s, a, sums, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_recover_divide(double* alpha, const double* wsum, int G, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(recover_divide_kernel, dim3(grid), dim3(kBlock), 0, s, alpha, wsum, G);
+    return hipGetLastError();
+}
+
+hipError_t launch_recover_residual(const RecoverArgs& a, double* res, int n, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(recover_residual_kernel, dim3(grid), dim3(kBlock), 0, s, a, res, n);
+    return hipGetLastError();
+}
+
 }  // namespace xfe
+

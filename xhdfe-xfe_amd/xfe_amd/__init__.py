@@ -42,6 +42,10 @@ class AbsorbResult(ctypes.Structure):
                 ("certificate_computed", _i)]
 
 
+class RecoveryResult(ctypes.Structure):
+    _fields_ = [("iterations", _i), ("converged", _i), ("max_delta", _d), ("absorb_converged", _i)]
+
+
 class Certificate(ctypes.Structure):
     _fields_ = [("abs_residual", _d), ("abs_residual_rel", _d), ("precision_certified", _i)]
 
@@ -53,7 +57,7 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_profile_get", "xfe_release_workspace", "xfe_certify", "xfe_akm_available",
            "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
            "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
-           "xfe_akm_jla_pii", "xfe_akm_jla_bii")
+           "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects")
 
 _lib = None
 
@@ -153,6 +157,34 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
     if not ok:
         raise XfeError(last_error())
     return yo, Xo, _info(r, D)
+
+
+def recover_fixed_effects(partial, gids, G, weights=None, sweep_order=None, tol=1e-8, fe_tol=1e-6,
+                          tolerance_mode=1, max_iter=100000, refine_tol=0.0, contributions=True):
+    """recover_fixed_effects_group_ids' GPU branch on host arrays: returns
+    (alphas, contributions or None, info); alphas[d] holds the level
+    coefficients of dimension d, contributions is D x n."""
+    partial = np.ascontiguousarray(partial, dtype=np.float64)
+    n, D = partial.size, len(gids)
+    ids = [np.ascontiguousarray(g, dtype=np.int32) for g in gids]
+    fes = (FeInput * max(D, 1))()
+    for d in range(D):
+        fes[d].group_ids = ids[d].ctypes.data
+        fes[d].num_groups = int(G[d])
+        fes[d].num_levels_present = int(G[d])
+    alphas = [np.empty(int(G[d])) for d in range(D)]
+    aptr = (_p * max(D, 1))(*[a.ctypes.data for a in alphas])
+    contrib = np.empty((D, n)) if contributions else None
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
+    o = _opts(tol, tolerance_mode, max_iter, 2 if D == 2 else 1, False, 1, False, refine_tol)
+    r = RecoveryResult()
+    ok = lib().xfe_recover_fixed_effects(_ptr(partial), _i(n), fes, _i(D), _ptr(w), _ptr(order),
+                                         ctypes.byref(o), _d(fe_tol), aptr, _ptr(contrib), ctypes.byref(r))
+    if not ok:
+        raise XfeError(last_error())
+    return alphas, contrib, dict(iterations=r.iterations, converged=bool(r.converged),
+                                 max_delta=r.max_delta, absorb_converged=bool(r.absorb_converged))
 
 
 def absorb_device(V_ptr, n, C, ld, gid_ptr, G, weights_ptr=None, sweep_order=None, tol=1e-8,
