@@ -113,6 +113,37 @@ int xfe_absorb(const double* y, const double* X, int n, int k,
                const int* sweep_order, const xfe_options* opt,
                xfe_absorb_result* res, double* y_out, double* X_out);
 
+/* group()/individual() structure, mirrors hdfe::detail::GroupIndividualStructure
+ * (include/fe_absorption.hpp:13-21): num_groups = n rows (one observation per
+ * group); group g's individuals are group_individual[group_ptr[g] ..
+ * group_ptr[g+1]), individual j's groups individual_group[individual_ptr[j] ..
+ * individual_ptr[j+1]) (the same incidences); group_scale (n) multiplies the
+ * individual effects of a row (1 for sum, 1/size for mean aggregation). */
+typedef struct {
+    int num_groups;
+    int num_individuals;
+    const int* group_ptr;
+    const int* group_individual;
+    const int* individual_ptr;
+    const int* individual_group;
+    const double* group_scale;
+} xfe_group_individual;
+
+/* absorb_fixed_effects_group_individual_cuda (src/fe_absorption_cuda.hpp:136-145,
+ * impl src/fe_absorption_cuda.cu:4740-5219): within transform of [y | X] over
+ * the standard dimensions fes and the individual dimension, by Gauss-Seidel
+ * sweeps (standard dimensions in sweep order, then the individuals colour
+ * class by colour class; mirrored for method 2), the change-of-norm stop at
+ * min(tol_eff, 1e-12), 16 polish sweeps, then the certificate with the
+ * individual dimension (batches of 64 more sweeps until certified or
+ * max_iter).  res->converged = res->precision_certified, as the reference sets
+ * result.converged.  Host buffers as xfe_absorb.  Tolerance modes 0 and 1.
+ * Returns 1 on success, 0 on invalid input or a device failure. */
+int xfe_absorb_group_individual(const double* y, const double* X, int n, int k, const xfe_fe_input* fes,
+                                int num_fes, const xfe_group_individual* gi, const double* weights,
+                                const int* sweep_order, const xfe_options* opt, xfe_absorb_result* res,
+                                double* y_out, double* X_out);
+
 /* Same computation with device-resident buffers (inputs already in HBM):
  * d_V holds [y | X] column-major with leading dimension ld >= n and is
  * overwritten with the within transform; d_group_ids is num_fes x n

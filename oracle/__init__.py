@@ -29,6 +29,12 @@ under ``xhdfe-xfe_amd/``) never imports it and never falls back to it.
   its max-delta check (src/fe_absorption.cpp:14447-14935, :15563-15700);
   pinned against the reference's own recovery (``ref_recover_fe``) on the
   golden panels and against the exact fitted component partial - M_D partial.
+* ``absorb_group_individual`` / ``gi_colour`` / ``certify_group_individual``:
+  numpy restatement of the group()/individual() absorber
+  (src/fe_absorption_cuda.cu:4740-5219, colouring :4913-4947; certificate
+  src/fe_absorption.cpp:15192-15300 with :15092-15190); pinned against the
+  reference's CPU absorber (``ref_absorb_gi``) and the exact weighted least
+  squares projection ``gi_exact``.
 """
 from __future__ import annotations
 
@@ -571,6 +577,183 @@ def recover_fixed_effects(partial, gids, G, weights=None, fe_tol=1e-6, max_iter=
         if mx < fe_tol:
             return alphas, it, mx, True
     return alphas, max_iter, mx, False
+
+
+def gi_colour(gi):
+    """Greedy colouring of the individuals in index order
+    (src/fe_absorption_cuda.cu:4913-4947): (colour per individual, classes as
+    lists of individuals in index order)."""
+    m = int(gi["num_individuals"])
+    gp, gind = gi["group_ptr"], gi["group_individual"]
+    ip, ig = gi["individual_ptr"], gi["individual_group"]
+    colour = np.zeros(m, dtype=np.int64)
+    for j in range(m):
+        taken = set()
+        for g in ig[ip[j]:ip[j + 1]]:
+            for o in gind[gp[g]:gp[g + 1]]:
+                if o < j:
+                    taken.add(int(colour[o]))
+        c = 0
+        while c in taken:
+            c += 1
+        colour[j] = c
+    classes = [np.flatnonzero(colour == c) for c in range(int(colour.max()) + 1)]
+    return colour, classes
+
+
+def gi_design(n, gi):
+    """The individual design A (n x individuals, A[g, j] = group_scale[g])."""
+    gp = np.asarray(gi["group_ptr"])
+    rows = np.repeat(np.arange(n), np.diff(gp))
+    A = np.zeros((n, int(gi["num_individuals"])))
+    A[rows, gi["group_individual"]] = np.asarray(gi["group_scale"])[rows]
+    return A
+
+
+def gi_exact(V, gids, G, gi, weights=None):
+    """The within transform as dense weighted least squares: the residual of
+    each column of V on [D_std, A] (small cases)."""
+    V = np.asarray(V, dtype=np.float64).reshape(len(gids[0]) if gids else len(gi["group_scale"]), -1)
+    n = V.shape[0]
+    blocks = [np.eye(int(G[d]))[gids[d]] for d in range(len(gids))] + [gi_design(n, gi)]
+    M = np.hstack(blocks)
+    sw = np.ones(n) if weights is None else np.sqrt(np.asarray(weights, dtype=np.float64))
+    coef = np.linalg.lstsq(M * sw[:, None], V * sw[:, None], rcond=None)[0]
+    return V - M @ coef
+
+
+def certify_group_individual(V, Vt, gids, G, gi, weights=None, tol=1e-8, tolerance_mode=1):
+    """The certificate with the individual dimension
+    (src/fe_absorption.cpp:15092-15190): (abs, rel, certified)."""
+    V = np.asarray(V, dtype=np.float64)
+    Vt = np.asarray(Vt, dtype=np.float64)
+    n, C = Vt.shape
+    w = np.ones(n) if weights is None else np.asarray(weights, dtype=np.float64)
+    op = 0.0
+    abs2 = np.zeros(C)
+    for d in range(len(gids)):
+        op += float((w * w).sum())
+        for c in range(C):
+            abs2[c] += float((np.bincount(gids[d], weights=w * Vt[:, c], minlength=int(G[d])) ** 2).sum())
+    gp = np.asarray(gi["group_ptr"])
+    rows = np.repeat(np.arange(n), np.diff(gp))
+    mult = (w * np.asarray(gi["group_scale"]))[rows]
+    op += float((mult * mult).sum())
+    m = int(gi["num_individuals"])
+    for c in range(C):
+        abs2[c] += float((np.bincount(gi["group_individual"], weights=mult * Vt[rows, c], minlength=m) ** 2).sum())
+    a = np.sqrt(abs2)
+    rel = a / (np.sqrt(op) * np.sqrt((V * V).sum(0)))
+    lim = max(8.0 * max(0.0, effective_tolerance(tol, tolerance_mode)), 64.0 * np.finfo(float).eps)
+    return float(a.max()), float(rel.max()), bool(rel.max() <= lim)
+
+
+def absorb_group_individual(V, gids, G, gi, weights=None, tol=1e-8, tolerance_mode=1, max_iter=100000,
+                            method=1, sweep_order=None):
+    """The group()/individual() absorber (absorb_fixed_effects_group_individual_cuda,
+    src/fe_absorption_cuda.cu:4740-5219): sweeps of the standard dimensions
+    (weighted level means) in order, then the individuals colour class by
+    colour class (a_j = sum_g w_g s_g v_g / sum_g w_g s_g^2, v_g -= s_g a_j),
+    mirrored for method 2; stop when |norm - previous norm| / max(1,
+    previous) < min(tol_eff, 1e-12) (reghdfe-comparable), then 16 polish
+    sweeps and the certificate, with 64 more sweeps per failed certificate.
+    Returns (V~, iterations, certified)."""
+    V0 = np.asarray(V, dtype=np.float64)
+    V0 = V0.reshape(V0.shape[0], -1)
+    Vt = V0.copy()
+    n, C = Vt.shape
+    w = np.ones(n) if weights is None else np.asarray(weights, dtype=np.float64)
+    order = list(range(len(gids)) if sweep_order is None else sweep_order)
+    wsum = [np.bincount(gids[d], weights=w, minlength=int(G[d])) for d in range(len(gids))]
+    scale = np.asarray(gi["group_scale"], dtype=np.float64)
+    ip, ig = gi["individual_ptr"], gi["individual_group"]
+    m = int(gi["num_individuals"])
+    owner_all = np.repeat(np.arange(m), np.diff(ip))
+    denom = np.bincount(owner_all, weights=w[ig] * scale[ig] ** 2, minlength=m)
+    denom = np.where(denom > 0, denom, 1.0)
+    _, classes = gi_colour(gi)
+    plans = []
+    for cls in classes:
+        pos = np.concatenate([np.arange(ip[j], ip[j + 1]) for j in cls]) if len(cls) else np.zeros(0, int)
+        own = np.repeat(np.arange(len(cls)), [ip[j + 1] - ip[j] for j in cls])
+        plans.append((cls, ig[pos], own))
+
+    def std(d):
+        for c in range(C):
+            s = np.bincount(gids[d], weights=w * Vt[:, c], minlength=int(G[d]))
+            Vt[:, c] -= (s / wsum[d])[gids[d]]
+
+    def indiv():
+        for cls, rows, own in plans:
+            for c in range(C):
+                num = np.bincount(own, weights=w[rows] * scale[rows] * Vt[rows, c], minlength=len(cls))
+                a = num / denom[cls]
+                Vt[rows, c] -= scale[rows] * a[own]
+
+    def sweep():
+        for d in order:
+            std(d)
+        indiv()
+        if method == 2:
+            indiv()
+            for d in reversed(order):
+                std(d)
+
+    tol_eff = effective_tolerance(tol, tolerance_mode)
+    gi_tol = min(tol_eff, 1e-12) if (tol_eff > 0 and tolerance_mode == 1) else tol_eff
+    prev = float(np.sqrt((Vt * Vt).sum()))
+    it = 0
+    conv = False
+    for k in range(max_iter):
+        sweep()
+        cur = float(np.sqrt((Vt * Vt).sum()))
+        rel = abs(cur - prev) / max(1.0, prev)
+        prev = cur
+        if rel < gi_tol:
+            it, conv = k + 1, True
+            break
+    it = max(1, it) if conv else max_iter
+    if conv and it < max_iter:
+        extra = min(16, max_iter - it)
+        for _ in range(extra):
+            sweep()
+        it += extra
+    cert = certify_group_individual(V0, Vt, gids, G, gi, weights, tol, tolerance_mode)[2]
+    while not cert and it < max_iter:
+        b = min(64, max_iter - it)
+        for _ in range(b):
+            sweep()
+        it += b
+        cert = certify_group_individual(V0, Vt, gids, G, gi, weights, tol, tolerance_mode)[2]
+    return Vt, it, cert
+
+
+def ref_absorb_gi(y, X, fes, gi, weights=None, tol=1e-8, method=1):
+    """The reference's CPU group()/individual() absorber through oracle/_ref
+    (checker): (y_tilde, X_tilde, iterations, converged)."""
+    r = ref()
+    if r is None:
+        return None
+    n = len(y)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(n, -1))
+    k = X.shape[1]
+    F = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes])) if fes else np.zeros(1, np.int32)
+    a = {key: np.ascontiguousarray(gi[key], dtype=np.int32)
+         for key in ("group_ptr", "group_individual", "individual_ptr", "individual_group")}
+    sc = np.ascontiguousarray(gi["group_scale"], dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    yo, Xo = np.empty(n), np.empty((n, k), order="F")
+    it, conv = ctypes.c_int(), ctypes.c_int()
+    r.ref_absorb_gi.argtypes = [_i, _i, _i, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, ctypes.c_double, _i, _p, _p,
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    rc = r.ref_absorb_gi(n, k, len(fes), _ptr(y), _ptr(X), _ptr(F), int(gi["num_individuals"]), _ptr(a["group_ptr"]),
+                         _ptr(a["group_individual"]), _ptr(a["individual_ptr"]), _ptr(a["individual_group"]),
+                         _ptr(sc), _ptr(w), float(tol), int(method), _ptr(yo), _ptr(Xo), ctypes.byref(it),
+                         ctypes.byref(conv))
+    if rc != 0:
+        raise RuntimeError("ref_absorb_gi failed")
+    return yo, Xo, it.value, conv.value
 
 
 def ref_recover_fe(partial, fes, tol=1e-8):

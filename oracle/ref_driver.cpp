@@ -315,4 +315,42 @@ int ref_recover_fe(int n, int d, const double* partial, const int* fes, double t
     }
 }
 
+// detail::absorb_fixed_effects_group_individual (src/fe_absorption.cpp:9741):
+// the reference's CPU group()/individual() absorber.  gi arrays as in
+// GroupIndividualStructure (include/fe_absorption.hpp:13-21); w may be NULL;
+// method 1 GaussSeidel, 2 SymmetricGaussSeidel.  Returns 0 on success.
+int ref_absorb_gi(int n, int k, int d, const double* y, const double* X, const int* fes, int m,
+                  const int* group_ptr, const int* group_individual, const int* individual_ptr,
+                  const int* individual_group, const double* group_scale, const double* w, double tol,
+                  int method, double* y_out, double* X_out, int* iterations, int* converged) {
+    try {
+        Eigen::VectorXd yv = Eigen::Map<const Eigen::VectorXd>(y, n);
+        Eigen::MatrixXd Xm = Eigen::Map<const Eigen::MatrixXd>(X, n, k);
+        auto fv = make_fes(fes, n, d);
+        hdfe::detail::GroupIndividualStructure gi;
+        gi.num_groups = n;
+        gi.num_individuals = m;
+        const int E = group_ptr[n];
+        gi.group_ptr.assign(group_ptr, group_ptr + n + 1);
+        gi.group_individual.assign(group_individual, group_individual + E);
+        gi.individual_ptr.assign(individual_ptr, individual_ptr + m + 1);
+        gi.individual_group.assign(individual_group, individual_group + E);
+        gi.group_scale.assign(group_scale, group_scale + n);
+        hdfe::HdfeOptions o = make_options(tol, 1, 1);
+        Eigen::VectorXd wv;
+        if (w) wv = Eigen::Map<const Eigen::VectorXd>(w, n);
+        auto r = hdfe::detail::absorb_fixed_effects_group_individual(
+            yv, Xm, fv, gi, w ? &wv : nullptr, o,
+            method == 2 ? hdfe::AbsorptionMethod::SymmetricGaussSeidel : hdfe::AbsorptionMethod::GaussSeidel);
+        std::memcpy(y_out, r.y_tilde.data(), sizeof(double) * n);
+        if (k > 0) std::memcpy(X_out, r.X_tilde.data(), sizeof(double) * n * k);
+        *iterations = r.iterations;
+        *converged = r.converged ? 1 : 0;
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_absorb_gi: %s\n", e.what());
+        return -1;
+    }
+}
+
 }  // extern "C"

@@ -104,3 +104,44 @@ def jla_panel():
 
 
 JLA_DRAWS, JLA_SEED = 50, 42
+
+
+def gi_panel(seed=20261203 + 500, n=3000, individuals=800, years=20, classes=40):
+    """A group()/individual() panel (patents and their inventors): each row
+    (group) has 1-3 distinct individuals drawn with log-normal activity
+    weights; group_scale = 1 / team size (mean aggregation); standard FEs
+    year and class; y = x b + mean of the team's effects + year + class + e."""
+    rng = np.random.default_rng(seed)
+    act = rng.lognormal(0.0, 1.0, individuals)
+    cdf = np.cumsum(act) / act.sum()
+    size = rng.integers(1, 4, n)
+    gp = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(size, out=gp[1:])
+    members = []
+    for g in range(n):
+        team = set()
+        while len(team) < size[g]:
+            team.add(int(np.searchsorted(cdf, rng.random(), side="right")))
+        members.extend(sorted(team))
+    gind = np.asarray(members, dtype=np.int32)
+    rows = np.repeat(np.arange(n, dtype=np.int32), size)
+    order = np.lexsort((rows, gind))
+    iptr = np.zeros(individuals + 1, dtype=np.int64)
+    np.cumsum(np.bincount(gind, minlength=individuals), out=iptr[1:])
+    scale = 1.0 / size.astype(np.float64)
+    year = rng.integers(0, years, n).astype(np.int32)
+    cls = rng.integers(0, classes, n).astype(np.int32)
+    alpha = rng.normal(0.0, 0.5, individuals)
+    team_effect = np.bincount(rows, weights=alpha[gind] * scale[rows], minlength=n)
+    X = rng.standard_normal((n, 2))
+    y = X @ np.array([0.4, -0.3]) + team_effect + rng.normal(0, 0.2, years)[year] + \
+        rng.normal(0, 0.3, classes)[cls] + rng.normal(0.0, 0.25, n)
+    gi = dict(num_individuals=individuals, group_ptr=gp.astype(np.int32), group_individual=gind,
+              individual_ptr=iptr.astype(np.int32), individual_group=rows[order].astype(np.int32),
+              group_scale=scale)
+    return dict(y=y, X=np.asfortranarray(X), fes=[year, cls], G=[years, classes], gi=gi)
+
+
+def gi_weights(n):
+    """Deterministic row weights for the weighted group()/individual() case."""
+    return 0.5 + (np.arange(n) % 5) / 3.0

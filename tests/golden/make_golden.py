@@ -20,7 +20,7 @@ sys.path.insert(0, os.path.join(ROOT, "xhdfe-xfe_amd"))
 
 import oracle  # noqa: E402
 from golden_cases import (CASES, JLA_DRAWS, JLA_SEED, SLOPE_CASES, certificate_weights,  # noqa: E402
-                          input_checksum, jla_panel, schwarz_edge_cases, slope_values)
+                          gi_panel, gi_weights, input_checksum, jla_panel, schwarz_edge_cases, slope_values)
 
 
 def main():
@@ -101,6 +101,13 @@ def main():
     # sigma_i: linear functionals of its B_ii draws
     out["jla/components"], out["jla/sigma_i"] = oracle.ref_akm_jla_components(
         jp["y"], jp["worker"], jp["firm"], JLA_DRAWS, JLA_SEED)
+    # group()/individual() absorption (absorb_fixed_effects_group_individual)
+    gp = gi_panel()
+    for tag, w, method in (("gs", None, 1), ("sgs_w", gi_weights(gp["y"].size), 2)):
+        yo, Xo, it, conv = oracle.ref_absorb_gi(gp["y"], gp["X"], gp["fes"], gp["gi"], weights=w, tol=1e-8,
+                                                method=method)
+        out[f"gi/{tag}/y_tilde"], out[f"gi/{tag}/X_tilde"] = yo, Xo
+        out[f"gi/{tag}/info"] = np.array([it, conv], dtype=np.int64)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_cases.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -691,3 +691,58 @@ def test_recover_fixed_effects_rejects_weighted_multi_dim():
     b = rng.integers(0, 20, n).astype(np.int32)
     with pytest.raises(xfe_amd.XfeError):
         xfe_amd.recover_fixed_effects(rng.standard_normal(n), [a, b], [50, 20], weights=np.ones(n))
+
+
+# ---- group()/individual() absorption (xfe_absorb_group_individual)
+
+@pytest.mark.parametrize("tag", ["gs", "sgs_w"])
+def test_group_individual_matches_reference(golden, tag):
+    """The device group()/individual() absorber: certified, within 1e-7 of the
+    column scale of the reference's CPU absorber (both stop certified at
+    8 tol_eff), within 1e-8 of the exact weighted least squares projection,
+    and its certificate equals the oracle's on its output."""
+    from golden_cases import gi_panel, gi_weights
+    _log(f"group_individual {tag}")
+    p = gi_panel()
+    V = np.column_stack([p["y"], p["X"]])
+    w = gi_weights(V.shape[0]) if tag == "sgs_w" else None
+    method = 2 if tag == "sgs_w" else 1
+    yg, Xg, info = xfe_amd.absorb_group_individual(p["y"], p["X"], p["fes"], p["G"], p["gi"], weights=w,
+                                                   method=method)
+    assert info["converged"] and info["precision_certified"], info
+    Vg = np.column_stack([yg, Xg])
+    scale = np.abs(V).max()
+    ref = np.column_stack([golden[f"gi/{tag}/y_tilde"], golden[f"gi/{tag}/X_tilde"]])
+    assert _rel(Vg, ref, scale) < 1e-7
+    ex = oracle.gi_exact(V, p["fes"], p["G"], p["gi"], weights=w)
+    assert _rel(Vg, ex, scale) < 1e-8
+    a, rel, cert = oracle.certify_group_individual(V, Vg, p["fes"], p["G"], p["gi"], weights=w)
+    # both at the rounding floor here (~1e-18, the decision limit is 8e-9):
+    # long double on the device host side, double in the oracle
+    assert cert and abs(rel - info["abs_residual_rel"]) <= 1e-6 * rel + 1e-15
+    # same colouring, same sweeps: the oracle's restatement needs the same count
+    Vo, it, _ = oracle.absorb_group_individual(V, p["fes"], p["G"], p["gi"], weights=w, method=method)
+    assert abs(it - info["iterations"]) <= 1, (it, info["iterations"])
+    assert _rel(Vg, Vo, scale) < 1e-10
+
+
+def test_group_individual_only_individuals():
+    """No standard dimension: the individual projection alone (exact lstsq)."""
+    from golden_cases import gi_panel
+    _log("group_individual no standard fes")
+    p = gi_panel(seed=77, n=2000, individuals=500)
+    V = np.column_stack([p["y"], p["X"]])
+    yg, Xg, info = xfe_amd.absorb_group_individual(p["y"], p["X"], [], [], p["gi"])
+    assert info["converged"]
+    ex = oracle.gi_exact(V, [], [], p["gi"])
+    assert _rel(np.column_stack([yg, Xg]), ex, np.abs(V).max()) < 1e-8
+
+
+def test_group_individual_rejects_bad_structure():
+    from golden_cases import gi_panel
+    p = gi_panel(seed=78, n=500, individuals=100)
+    gi = dict(p["gi"])
+    gi["group_individual"] = gi["group_individual"].copy()
+    gi["group_individual"][3] = gi["num_individuals"]  # out of range
+    with pytest.raises(xfe_amd.XfeError):
+        xfe_amd.absorb_group_individual(p["y"], p["X"], p["fes"], p["G"], gi)

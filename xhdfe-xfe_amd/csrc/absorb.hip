@@ -124,6 +124,9 @@ struct Workspace {
     DevBuf<int> seg_bk[2];
     DevBuf<float> f_bv[2];
     DevBuf<double> d_bv[2];
+    // group()/individual() structure: incidence lists, scales, colour classes
+    DevBuf<int> gi_iptr, gi_igroup, gi_members;
+    DevBuf<double> gi_scale, gi_denom;
     // FE recovery: the untransformed column and the level coefficients
     DevBuf<double> rec_p;
     DevBuf<double> rec_alpha[3];
@@ -166,6 +169,7 @@ struct Workspace {
         for (int k = 0; k < 2; ++k) { seg_bk[k].release(); f_bv[k].release(); d_bv[k].release(); }
         rec_p.release();
         for (auto& b : rec_alpha) b.release();
+        gi_iptr.release(); gi_igroup.release(); gi_members.release(); gi_scale.release(); gi_denom.release();
         fast_rec.used = false;
         if (host_scalars) (void)hipHostFree(host_scalars);
         if (stream) (void)hipStreamDestroy(stream);
@@ -2027,6 +2031,111 @@ static double max_delta_staged(Workspace& ws, int n, const xfe_fe_input* fes, in
     return mx;
 }
 
+// Certificate of a group()/individual() candidate on the host
+// (certify_group_individual_candidate, src/fe_absorption.cpp:15192-15300,
+// through certify_absorption_result :14942-15190 with the individual
+// dimension): abs_c^2 = sum_d ||D_d' W v~_c||^2 + ||A' W v~_c||^2 with A the
+// individual design (row g, individual j: s_g), ||D'W||_F^2 = sum_d sum_i w_i^2
+// + sum over incidences of (w_g s_g)^2, rel_c = abs_c / (||D'W||_F ||v_c||);
+// certified iff converged and max_c rel_c <= max(8 tol_eff, 64 eps).
+static void gi_certificate(const double* V0, const double* Vt, int n, int C, const xfe_fe_input* fes, int D,
+                           const xfe_group_individual& gi, const double* w, double tol_eff,
+                           xfe_absorb_result& res) {
+    long double op = 0.0L;
+    std::vector<long double> abs2(C, 0.0L), orig2(C, 0.0L);
+    for (int c = 0; c < C; ++c)
+        for (int i = 0; i < n; ++i) {
+            const long double v = V0[static_cast<size_t>(c) * n + i];
+            orig2[c] += v * v;
+        }
+    for (int d = 0; d < D; ++d) {
+        const int G = fes[d].num_groups;
+        std::vector<long double> sums(static_cast<size_t>(G) * C, 0.0L);
+        for (int i = 0; i < n; ++i) {
+            const long double wi = w ? w[i] : 1.0;
+            op += wi * wi;
+            long double* sg = &sums[static_cast<size_t>(fes[d].group_ids[i]) * C];
+            for (int c = 0; c < C; ++c) sg[c] += wi * Vt[static_cast<size_t>(c) * n + i];
+        }
+        for (size_t q = 0; q < sums.size(); ++q) abs2[q % C] += sums[q] * sums[q];
+    }
+    const int m = gi.num_individuals;
+    std::vector<long double> isum(static_cast<size_t>(m) * C, 0.0L);
+    for (int g = 0; g < n; ++g) {
+        const long double mult = static_cast<long double>(w ? w[g] : 1.0) * gi.group_scale[g];
+        for (int q = gi.group_ptr[g]; q < gi.group_ptr[g + 1]; ++q) {
+            op += mult * mult;
+            long double* sj = &isum[static_cast<size_t>(gi.group_individual[q]) * C];
+            for (int c = 0; c < C; ++c) sj[c] += mult * Vt[static_cast<size_t>(c) * n + g];
+        }
+    }
+    for (size_t q = 0; q < isum.size(); ++q) abs2[q % C] += isum[q] * isum[q];
+    const double opn = std::sqrt(static_cast<double>(op));
+    double max_abs = 0.0, max_rel = 0.0;
+    for (int c = 0; c < C; ++c) {
+        const double a = std::sqrt(static_cast<double>(abs2[c]));
+        const double scale = opn * std::sqrt(static_cast<double>(orig2[c]));
+        const double rel = scale > 0.0 ? a / scale : (a > 0.0 ? DBL_MAX : 0.0);
+        max_abs = std::max(max_abs, a);
+        max_rel = std::max(max_rel, rel);
+    }
+    res.abs_residual = max_abs;
+    res.abs_residual_rel = max_rel;
+    res.precision_certified = max_rel <= std::max(8.0 * std::max(0.0, tol_eff), 64.0 * DBL_EPSILON) ? 1 : 0;
+    res.certificate_computed = 1;
+}
+
+// Greedy colouring of the individuals in index order (the reference's host
+// step, src/fe_absorption_cuda.cu:4913-4947): each individual takes the
+// smallest colour not used by an earlier individual sharing a row with it.
+// Returns the members of each colour, colour by colour, and the offsets.
+static void gi_colour(const xfe_group_individual& gi, std::vector<int>& members, std::vector<int>& offsets) {
+    const int m = gi.num_individuals;
+    std::vector<int> colour(m, 0), used;
+    int ncol = 1;
+    for (int j = 0; j < m; ++j) {
+        for (int q = gi.individual_ptr[j]; q < gi.individual_ptr[j + 1]; ++q) {
+            const int g = gi.individual_group[q];
+            for (int r = gi.group_ptr[g]; r < gi.group_ptr[g + 1]; ++r) {
+                const int o = gi.group_individual[r];
+                if (o >= j) continue;
+                const int co = colour[o];
+                if (co >= static_cast<int>(used.size())) used.resize(co + 1, -1);
+                used[co] = j;
+            }
+        }
+        int c = 0;
+        while (c < static_cast<int>(used.size()) && used[c] == j) ++c;
+        colour[j] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    offsets.assign(ncol + 1, 0);
+    for (int j = 0; j < m; ++j) ++offsets[colour[j] + 1];
+    for (int c = 0; c < ncol; ++c) offsets[c + 1] += offsets[c];
+    members.assign(m, 0);
+    std::vector<int> cur(offsets.begin(), offsets.end() - 1);
+    for (int j = 0; j < m; ++j) members[cur[colour[j]]++] = j;
+}
+
+static bool gi_valid(const xfe_group_individual* gi, int n) {
+    if (!gi || gi->num_groups != n || gi->num_individuals <= 0 || !gi->group_ptr || !gi->group_individual ||
+        !gi->individual_ptr || !gi->individual_group || !gi->group_scale)
+        return false;
+    const int m = gi->num_individuals;
+    if (gi->group_ptr[0] != 0 || gi->individual_ptr[0] != 0) return false;
+    for (int g = 0; g < n; ++g)
+        if (gi->group_ptr[g + 1] < gi->group_ptr[g]) return false;
+    for (int j = 0; j < m; ++j)
+        if (gi->individual_ptr[j + 1] < gi->individual_ptr[j]) return false;
+    const int E = gi->group_ptr[n];
+    if (gi->individual_ptr[m] != E) return false;
+    for (int q = 0; q < E; ++q)
+        if (gi->group_individual[q] < 0 || gi->group_individual[q] >= m || gi->individual_group[q] < 0 ||
+            gi->individual_group[q] >= n)
+            return false;
+    return true;
+}
+
 extern "C" {
 
 int xfe_backend_available(void) {
@@ -2448,6 +2557,189 @@ int xfe_recover_fixed_effects(const double* partial, int n, const xfe_fe_input* 
         out->absorb_converged = ares.converged;
         out->max_delta = mx;
         out->converged = (ares.converged && mx <= fe_tol) ? 1 : 0;
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+int xfe_absorb_group_individual(const double* y, const double* X, int n, int k, const xfe_fe_input* fes,
+                                int num_fes, const xfe_group_individual* gi, const double* weights,
+                                const int* sweep_order, const xfe_options* opt, xfe_absorb_result* res,
+                                double* y_out, double* X_out) {
+    clear_error();
+    if (!res || !opt || !y || (k > 0 && !X) || n <= 0 || k < 0 || k + 1 > XFE_MAX_COLS || num_fes < 0 ||
+        num_fes > XFE_MAX_DIMS || (num_fes > 0 && !fes) || !y_out || (k > 0 && !X_out)) {
+        set_error("xfe_absorb_group_individual: invalid arguments");
+        return 0;
+    }
+    std::memset(res, 0, sizeof(*res));
+    if (!gi_valid(gi, n)) {
+        set_error("xfe_absorb_group_individual: invalid group/individual structure");
+        return 0;
+    }
+    for (int d = 0; d < num_fes; ++d) {
+        if (!fes[d].group_ids || fes[d].num_groups <= 0 || fes[d].is_slope) {
+            set_error("xfe_absorb_group_individual: invalid FE input");
+            return 0;
+        }
+        for (int i = 0; i < n; ++i)
+            if (fes[d].group_ids[i] < 0 || fes[d].group_ids[i] >= fes[d].num_groups) {
+                set_error("xfe_absorb_group_individual: FE id out of range");
+                return 0;
+            }
+    }
+    if (opt->tolerance_mode == 2) {
+        set_error("xfe_absorb_group_individual: strict-residual tolerance mode is not supported");
+        return 0;
+    }
+    try {
+        (void)hipGetLastError();
+        Workspace& ws = tls_ws;
+        ws.init();
+        hipStream_t s = ws.stream;
+        const int C = k + 1;
+        const int m = gi->num_individuals;
+        const int E = gi->group_ptr[n];
+        const size_t NC = static_cast<size_t>(n) * C;
+        ws.stage.ensure(NC);
+        XFE_CHECK(hipMemcpyAsync(ws.stage.p, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        if (k > 0) XFE_CHECK(hipMemcpyAsync(ws.stage.p + n, X, sizeof(double) * n * k, hipMemcpyHostToDevice, s));
+        double* V = ws.stage.p;
+        const double* w = nullptr;
+        if (weights) {
+            ws.stage_w.ensure(n);
+            XFE_CHECK(hipMemcpyAsync(ws.stage_w.p, weights, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            w = ws.stage_w.p;
+        }
+        const int grid = grid_for(n);
+        ws.stage_ids.ensure(static_cast<size_t>(n) * std::max(num_fes, 1));
+        std::vector<const int*> gid(num_fes);
+        for (int d = 0; d < num_fes; ++d) {
+            int* dst = ws.stage_ids.p + static_cast<size_t>(d) * n;
+            XFE_CHECK(hipMemcpyAsync(dst, fes[d].group_ids, sizeof(int) * n, hipMemcpyHostToDevice, s));
+            gid[d] = dst;
+            const int G = fes[d].num_groups;
+            ws.wsum[d].ensure(G);
+            ws.sums[d].ensure(static_cast<size_t>(G) * C);
+            if (fes[d].weight_sums) {
+                XFE_CHECK(hipMemcpyAsync(ws.wsum[d].p, fes[d].weight_sums, sizeof(double) * G, hipMemcpyHostToDevice, s));
+            } else {
+                XFE_CHECK(hipMemsetAsync(ws.wsum[d].p, 0, sizeof(double) * G, s));
+                XFE_CHECK(launch_gi_weight_sums(n, dst, w, ws.wsum[d].p, grid, s));
+            }
+            res->fe_levels[d] = G;
+        }
+        ws.gi_iptr.ensure(m + 1);
+        ws.gi_igroup.ensure(std::max(E, 1));
+        ws.gi_scale.ensure(n);
+        ws.gi_denom.ensure(m);
+        ws.gi_members.ensure(m);
+        XFE_CHECK(hipMemcpyAsync(ws.gi_iptr.p, gi->individual_ptr, sizeof(int) * (m + 1), hipMemcpyHostToDevice, s));
+        if (E > 0)
+            XFE_CHECK(hipMemcpyAsync(ws.gi_igroup.p, gi->individual_group, sizeof(int) * E, hipMemcpyHostToDevice, s));
+        XFE_CHECK(hipMemcpyAsync(ws.gi_scale.p, gi->group_scale, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        std::vector<int> members, offsets;
+        gi_colour(*gi, members, offsets);
+        XFE_CHECK(hipMemcpyAsync(ws.gi_members.p, members.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
+        XFE_CHECK(launch_gi_denom(ws.gi_iptr.p, ws.gi_igroup.p, ws.gi_scale.p, w, m, ws.gi_denom.p, grid_for(m), s));
+        // sweep order: the caller's permutation, else 0..D-1 (src/fe_absorption_cuda.cu:4771-4800)
+        std::vector<int> order(num_fes);
+        for (int d = 0; d < num_fes; ++d) order[d] = d;
+        if (sweep_order) {
+            std::vector<char> seen(num_fes, 0);
+            bool ok = true;
+            for (int d = 0; d < num_fes && ok; ++d) {
+                const int v = sweep_order[d];
+                if (v < 0 || v >= num_fes || seen[v]) ok = false;
+                else seen[v] = 1;
+            }
+            if (ok) for (int d = 0; d < num_fes; ++d) order[d] = sweep_order[d];
+        }
+        for (int d = 0; d < num_fes; ++d) res->sweep_order_used[d] = order[d];
+        const bool symmetric = opt->method == 2;
+        const double relax = 1.0;  // HdfeOptions::jacobi_relaxation's default
+        auto std_demean = [&](int d) {
+            const int G = fes[d].num_groups;
+            XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * G * C, s));
+            XFE_CHECK(launch_gi_std_accumulate(V, n, C, gid[d], w, ws.sums[d].p, grid, s));
+            XFE_CHECK(launch_gi_std_apply(V, n, C, gid[d], ws.sums[d].p, ws.wsum[d].p, grid, s));
+        };
+        auto individual_sweep = [&]() {
+            for (size_t c = 0; c + 1 < offsets.size(); ++c) {
+                const int cnt = offsets[c + 1] - offsets[c];
+                if (cnt <= 0) continue;
+                XFE_CHECK(launch_gi_color_sweep(V, n, C, ws.gi_iptr.p, ws.gi_igroup.p, ws.gi_scale.p, w,
+                                                ws.gi_denom.p, ws.gi_members.p + offsets[c], cnt, relax,
+                                                grid_for(cnt), s));
+            }
+        };
+        auto full_sweep = [&]() {
+            for (int d : order) std_demean(d);
+            individual_sweep();
+            if (symmetric) {
+                individual_sweep();
+                for (int j = num_fes - 1; j >= 0; --j) std_demean(order[j]);
+            }
+        };
+        const int ngrid = grid_for(static_cast<long long>(NC));
+        ws.partials.ensure(ngrid);
+        std::vector<double> hpart(ngrid);
+        auto norm = [&]() {
+            XFE_CHECK(launch_gi_sumsq(V, static_cast<long long>(NC), ws.partials.p, ngrid, s));
+            XFE_CHECK(hipMemcpyAsync(hpart.data(), ws.partials.p, sizeof(double) * ngrid, hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+            double t = 0.0;
+            for (double v : hpart) t += v;
+            return std::sqrt(t);
+        };
+        // change-of-norm rule (src/fe_absorption_cuda.cu:5119-5150)
+        const double tol_eff = (opt->tol > 0.0 && opt->tolerance_mode == 1) ? std::min(opt->tol, 1e-9) : opt->tol;
+        const double gi_tol = (tol_eff > 0.0 && opt->tolerance_mode == 1) ? std::min(tol_eff, 1e-12) : tol_eff;
+        const int max_iter = opt->max_iter;
+        double prev = norm();
+        int iterations = 0;
+        bool converged = false;
+        for (int it = 0; it < max_iter; ++it) {
+            full_sweep();
+            const double cur = norm();
+            const double rel = std::abs(cur - prev) / std::max(1.0, prev);
+            prev = cur;
+            if (rel < gi_tol) {
+                iterations = it + 1;
+                converged = true;
+                break;
+            }
+        }
+        iterations = converged ? std::max(1, iterations) : max_iter;
+        // 16 polish sweeps, then certify; 64 more sweeps per failed certificate
+        if (converged && iterations < max_iter) {
+            const int extra = std::min(16, max_iter - iterations);
+            for (int j = 0; j < extra; ++j) full_sweep();
+            iterations += extra;
+        }
+        std::vector<double> Vt(NC), V0(NC);
+        std::memcpy(V0.data(), y, sizeof(double) * n);
+        if (k > 0) std::memcpy(V0.data() + n, X, sizeof(double) * n * k);
+        auto fetch = [&]() {
+            XFE_CHECK(hipMemcpyAsync(Vt.data(), V, sizeof(double) * NC, hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+        };
+        fetch();
+        gi_certificate(V0.data(), Vt.data(), n, C, fes, num_fes, *gi, weights, tol_eff, *res);
+        while (!res->precision_certified && iterations < max_iter) {
+            const int batch = std::min(64, max_iter - iterations);
+            for (int j = 0; j < batch; ++j) full_sweep();
+            iterations += batch;
+            fetch();
+            gi_certificate(V0.data(), Vt.data(), n, C, fes, num_fes, *gi, weights, tol_eff, *res);
+        }
+        std::memcpy(y_out, Vt.data(), sizeof(double) * n);
+        if (k > 0) std::memcpy(X_out, Vt.data() + n, sizeof(double) * n * k);
+        res->iterations = iterations;
+        res->sweeps = iterations;
+        res->converged = res->precision_certified;
         return 1;
     } catch (const std::exception& e) {
         set_error(e.what());

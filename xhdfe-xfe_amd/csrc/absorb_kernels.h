@@ -289,7 +289,21 @@ hipError_t launch_recover_primary(const RecoverArgs& a, double* sums, int n, int
 hipError_t launch_recover_divide(double* alpha, const double* wsum, int G, int grid, hipStream_t s);
 hipError_t launch_recover_residual(const RecoverArgs& a, double* res, int n, int grid, hipStream_t s);
 
+// group()/individual() absorber (gi_kernels.hip); V is n x C, ld n.
+hipError_t launch_gi_std_accumulate(const double* V, int n, int C, const int* gid, const double* w, double* sums,
+                                    int grid, hipStream_t s);
+hipError_t launch_gi_std_apply(double* V, int n, int C, const int* gid, const double* sums, const double* wsum,
+                               int grid, hipStream_t s);
+hipError_t launch_gi_weight_sums(int n, const int* gid, const double* w, double* wsum, int grid, hipStream_t s);
+hipError_t launch_gi_denom(const int* ind_ptr, const int* ind_group, const double* scale, const double* w, int m,
+                           double* denom, int grid, hipStream_t s);
+hipError_t launch_gi_color_sweep(double* V, int n, int C, const int* ind_ptr, const int* ind_group,
+                                 const double* scale, const double* w, const double* denom, const int* members,
+                                 int count, double relax, int grid, hipStream_t s);
+hipError_t launch_gi_sumsq(const double* V, long long total, double* partials, int grid, hipStream_t s);
+
 }  // namespace xfe
+
 
 
 #endif

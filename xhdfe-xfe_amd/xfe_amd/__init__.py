@@ -42,6 +42,11 @@ class AbsorbResult(ctypes.Structure):
                 ("certificate_computed", _i)]
 
 
+class GroupIndividual(ctypes.Structure):
+    _fields_ = [("num_groups", _i), ("num_individuals", _i), ("group_ptr", _p), ("group_individual", _p),
+                ("individual_ptr", _p), ("individual_group", _p), ("group_scale", _p)]
+
+
 class RecoveryResult(ctypes.Structure):
     _fields_ = [("iterations", _i), ("converged", _i), ("max_delta", _d), ("absorb_converged", _i)]
 
@@ -57,7 +62,8 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_profile_get", "xfe_release_workspace", "xfe_certify", "xfe_akm_available",
            "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
            "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
-           "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects")
+           "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
+           "xfe_absorb_group_individual")
 
 _lib = None
 
@@ -154,6 +160,41 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
     Xo = np.empty((n, k), order="F")
     ok = lib().xfe_absorb(_ptr(y), _ptr(X), _i(n), _i(k), fes, _i(D), _ptr(w), _ptr(order),
                           ctypes.byref(o), ctypes.byref(r), _ptr(yo), _ptr(Xo))
+    if not ok:
+        raise XfeError(last_error())
+    return yo, Xo, _info(r, D)
+
+
+def absorb_group_individual(y, X, gids, G, gi, weights=None, sweep_order=None, tol=1e-8, tolerance_mode=1,
+                            max_iter=100000, method=1):
+    """absorb_fixed_effects_group_individual_cuda on host arrays: returns
+    (y_tilde, X_tilde, info).  gi: dict with group_ptr, group_individual,
+    individual_ptr, individual_group (int32), group_scale (float64),
+    num_individuals."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
+    n, k = X.shape
+    D = len(gids)
+    ids = [np.ascontiguousarray(g, dtype=np.int32) for g in gids]
+    fes = (FeInput * max(D, 1))()
+    for d in range(D):
+        fes[d].group_ids = ids[d].ctypes.data
+        fes[d].num_groups = int(G[d])
+        fes[d].num_levels_present = int(G[d])
+    arrs = {key: np.ascontiguousarray(gi[key], dtype=np.int32)
+            for key in ("group_ptr", "group_individual", "individual_ptr", "individual_group")}
+    scale = np.ascontiguousarray(gi["group_scale"], dtype=np.float64)
+    s = GroupIndividual(n, int(gi["num_individuals"]), arrs["group_ptr"].ctypes.data,
+                        arrs["group_individual"].ctypes.data, arrs["individual_ptr"].ctypes.data,
+                        arrs["individual_group"].ctypes.data, scale.ctypes.data)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
+    o = _opts(tol, tolerance_mode, max_iter, method, False, 0, True, 0.0)
+    r = AbsorbResult()
+    yo = np.empty(n)
+    Xo = np.empty((n, k), order="F")
+    ok = lib().xfe_absorb_group_individual(_ptr(y), _ptr(X), _i(n), _i(k), fes, _i(D), ctypes.byref(s), _ptr(w),
+                                           _ptr(order), ctypes.byref(o), ctypes.byref(r), _ptr(yo), _ptr(Xo))
     if not ok:
         raise XfeError(last_error())
     return yo, Xo, _info(r, D)
