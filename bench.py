@@ -257,6 +257,120 @@ def bench_slope(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_recover(args):
+    """The savefe FE recovery (xfe_recover_fixed_effects) of y on the AKM
+    panel at --scale (1.0 = 56M obs): the host entry point (upload, Schur
+    solve, coefficients, expansion, max-delta check, download) timed."""
+    import torch  # noqa: F401  (shares torch's HIP runtime)
+
+    import xfe_amd
+    from xfe_amd import panels
+
+    if not xfe_amd.backend_available():
+        raise SystemExit("no HIP device")
+    t0 = time.perf_counter()
+    p = panels.config1() if args.scale >= 1.0 else panels.akm_small(scale=args.scale)
+    keep, dropped, _ = xfe_amd.drop_singletons(p["fes"])
+    comp = [compact(f[keep]) for f in p["fes"]]
+    gids, G = [c[0] for c in comp], [c[1] for c in comp]
+    y = np.ascontiguousarray(p["y"][keep])
+    del p
+    n = y.size
+    log(f"recovery panel: n={n} levels={G} in {time.perf_counter() - t0:.1f} s")
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        _, _, info = xfe_amd.recover_fixed_effects(y, gids, G)
+        log(f"warmup {w}: {time.perf_counter() - t0:.3f} s {info}")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        alphas, contrib, info = xfe_amd.recover_fixed_effects(y, gids, G)
+    sec = (time.perf_counter() - t0) / args.steps
+    log(f"{sec:.3f} s per recovery, {info}")
+    out = {
+        "metric": "s to recover the FE values of y (savefe) on the AKM panel", "value": round(sec, 4),
+        "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "method": "Schur solve of the non-primary levels (FP32 PCG under FP64 refinement), primary-level means, "
+                  "contributions expanded on the host, one Gauss-Seidel check pass (host entry: PCIe included)",
+        "config": {"workload": f"akm_{n}obs_recover", "obs": int(n), "levels": G, "fe_tol": 1e-6,
+                   "singletons_dropped": int(dropped)},
+        "max_delta": info["max_delta"], "converged": info["converged"], "iterations": info["iterations"],
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def gi_big_panel(n=2_000_000, individuals=500_000, seed=20261203 + 600):
+    """A group()/individual() panel of n rows (patents) with 1-3 distinct
+    individuals each (log-normal activity), mean aggregation, standard FEs
+    year (30) and class (600), k = 3 regressors."""
+    rng = np.random.default_rng(seed)
+    act = rng.lognormal(0.0, 1.0, individuals)
+    cdf = np.cumsum(act) / act.sum()
+    size = rng.integers(1, 4, n)
+    draws = np.searchsorted(cdf, rng.random((n, 3)), side="right").astype(np.int64)
+    draws = np.minimum(draws, individuals - 1)
+    rows = np.repeat(np.arange(n, dtype=np.int64), 3)
+    cand = draws.ravel()
+    mask = (np.tile(np.arange(3), n) < np.repeat(size, 3))
+    pair = np.unique(rows[mask] * individuals + cand[mask])
+    grow, gind = (pair // individuals).astype(np.int64), (pair % individuals).astype(np.int32)
+    cnt = np.bincount(grow, minlength=n)
+    gp = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(cnt, out=gp[1:])
+    order = np.lexsort((grow, gind))
+    ip = np.zeros(individuals + 1, dtype=np.int64)
+    np.cumsum(np.bincount(gind, minlength=individuals), out=ip[1:])
+    scale = 1.0 / cnt.astype(np.float64)
+    year = rng.integers(0, 30, n).astype(np.int32)
+    cls = rng.integers(0, 600, n).astype(np.int32)
+    alpha = rng.normal(0.0, 0.5, individuals)
+    team = np.bincount(grow, weights=alpha[gind] * scale[grow], minlength=n)
+    X = np.asfortranarray(rng.standard_normal((n, 3)))
+    y = X @ np.array([0.4, -0.3, 0.2]) + team + rng.normal(0, 0.25, n)
+    gi = dict(num_individuals=individuals, group_ptr=gp.astype(np.int32), group_individual=gind,
+              individual_ptr=ip.astype(np.int32), individual_group=grow[order].astype(np.int32), group_scale=scale)
+    return y, X, [year, cls], [30, 600], gi
+
+
+def bench_gi(args):
+    """The group()/individual() absorber (xfe_absorb_group_individual) on a
+    2M-row panel: the host entry point timed (upload, sweeps, certificate,
+    download)."""
+    import torch  # noqa: F401  (shares torch's HIP runtime)
+
+    import xfe_amd
+
+    if not xfe_amd.backend_available():
+        raise SystemExit("no HIP device")
+    t0 = time.perf_counter()
+    y, X, fes, G, gi = gi_big_panel()
+    n = y.size
+    log(f"gi panel: n={n} individuals={gi['num_individuals']} incidences={gi['group_ptr'][-1]} "
+        f"in {time.perf_counter() - t0:.1f} s")
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        _, _, info = xfe_amd.absorb_group_individual(y, X, fes, G, gi)
+        log(f"warmup {w}: {time.perf_counter() - t0:.3f} s, sweeps {info['iterations']}")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        _, _, info = xfe_amd.absorb_group_individual(y, X, fes, G, gi)
+    sec = (time.perf_counter() - t0) / args.steps
+    log(f"{sec:.3f} s per absorption, sweeps {info['iterations']}, certificate {info['abs_residual_rel']:.2e}")
+    out = {
+        "metric": "s to absorb a 2M-row group()/individual() panel at 1e-8", "value": round(sec, 4),
+        "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "method": "colour-class Gauss-Seidel sweeps, change-of-norm stop, host certificate (host entry: PCIe included)",
+        "config": {"workload": "gi_2m_patents", "obs": int(n), "individuals": gi["num_individuals"],
+                   "incidences": int(gi["group_ptr"][-1]), "standard_levels": G, "columns": 4, "tol": 1e-8},
+        "iterations": info["iterations"], "converged": info["converged"],
+        "certificate_rel": info["abs_residual_rel"], "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,9 +380,10 @@ def main():
     ap.add_argument("--precision", type=int, default=1, help="1 FP32 under FP64 refinement, 0 FP64")
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain FP64 absorber")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--path", choices=("absorb", "akm", "slope"), default="absorb",
+    ap.add_argument("--path", choices=("absorb", "akm", "slope", "recover", "gi"), default="absorb",
                     help="absorb: the 56M-obs absorber (default); akm: the KSS leverage solver; "
-                         "slope: the heterogeneous-slope path")
+                         "slope: the heterogeneous-slope path; recover: savefe FE recovery; "
+                         "gi: the group()/individual() absorber")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -279,6 +394,12 @@ def main():
         return
     if args.path == "slope":
         bench_slope(args)
+        return
+    if args.path == "recover":
+        bench_recover(args)
+        return
+    if args.path == "gi":
+        bench_gi(args)
         return
 
     import torch

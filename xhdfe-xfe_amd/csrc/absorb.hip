@@ -2663,7 +2663,7 @@ int xfe_absorb_group_individual(const double* y, const double* X, int n, int k, 
         auto std_demean = [&](int d) {
             const int G = fes[d].num_groups;
             XFE_CHECK(hipMemsetAsync(ws.sums[d].p, 0, sizeof(double) * G * C, s));
-            XFE_CHECK(launch_gi_std_accumulate(V, n, C, gid[d], w, ws.sums[d].p, grid, s));
+            XFE_CHECK(launch_gi_std_accumulate(V, n, C, gid[d], w, ws.sums[d].p, G, grid, s));
             XFE_CHECK(launch_gi_std_apply(V, n, C, gid[d], ws.sums[d].p, ws.wsum[d].p, grid, s));
         };
         auto individual_sweep = [&]() {

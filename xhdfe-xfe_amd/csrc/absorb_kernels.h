@@ -291,7 +291,7 @@ hipError_t launch_recover_residual(const RecoverArgs& a, double* res, int n, int
 
 // group()/individual() absorber (gi_kernels.hip); V is n x C, ld n.
 hipError_t launch_gi_std_accumulate(const double* V, int n, int C, const int* gid, const double* w, double* sums,
-                                    int grid, hipStream_t s);
+                                    int G, int grid, hipStream_t s);
 hipError_t launch_gi_std_apply(double* V, int n, int C, const int* gid, const double* sums, const double* wsum,
                                int grid, hipStream_t s);
 hipError_t launch_gi_weight_sums(int n, const int* gid, const double* w, double* wsum, int grid, hipStream_t s);

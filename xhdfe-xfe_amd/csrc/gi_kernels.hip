@@ -11,14 +11,27 @@
 
 namespace xfe {
 
-// sums[g * C + c] += w_i v[i, c] for every row i of level g.
+// sums[g * C + c] += w_i v[i, c] for every row i of level g.  With lds (the
+// G x C table fits in the dynamic LDS), each block accumulates a private copy
+// in LDS and adds its non-zero entries to sums once.
 __global__ void gi_std_accumulate_kernel(const double* __restrict__ V, int n, int C, const int* __restrict__ gid,
-                                         const double* __restrict__ w, double* __restrict__ sums) {
+                                         const double* __restrict__ w, double* __restrict__ sums, int GC, int lds) {
+    extern __shared__ __align__(16) double lsum[];
     const int stride = gridDim.x * blockDim.x;
+    if (lds) {
+        for (int q = threadIdx.x; q < GC; q += blockDim.x) lsum[q] = 0.0;
+        __syncthreads();
+    }
+    double* tab = lds ? lsum : sums;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const double wi = w ? w[i] : 1.0;
-        double* s = sums + static_cast<size_t>(gid[i]) * C;
+        double* s = tab + static_cast<size_t>(gid[i]) * C;
         for (int c = 0; c < C; ++c) atomicAdd(&s[c], wi * V[static_cast<size_t>(c) * n + i]);
+    }
+    if (lds) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < GC; q += blockDim.x)
+            if (lsum[q] != 0.0) atomicAdd(&sums[q], lsum[q]);
     }
 }
 
@@ -104,8 +117,12 @@ __global__ void gi_sumsq_kernel(const double* __restrict__ V, long long total, d
 }
 
 hipError_t launch_gi_std_accumulate(const double* V, int n, int C, const int* gid, const double* w, double* sums,
-                                    int grid, hipStream_t s) {
-    hipLaunchKernelGGL(gi_std_accumulate_kernel, dim3(grid), dim3(kBlock), 0, s, V, n, C, gid, w, sums);
+                                    int G, int grid, hipStream_t s) {
+    const int GC = G * C;
+    const size_t bytes = static_cast<size_t>(GC) * sizeof(double);
+    const int lds = bytes <= 48 * 1024 ? 1 : 0;
+    hipLaunchKernelGGL(gi_std_accumulate_kernel, dim3(grid), dim3(kBlock), lds ? bytes : 0, s, V, n, C, gid, w,
+                       sums, GC, lds);
     return hipGetLastError();
 }
 
