@@ -96,6 +96,45 @@ int xfe_certify(const double* y, const double* X, const double* y_t, const doubl
                 const xfe_fe_input* fes, int num_fes, const double* weights,
                 const xfe_options* opt, int converged, xfe_certificate* out);
 
+/* Shadow-mode device certificate (cuda_cert_shadow_begin / _finish / _abort and
+ * CudaCertShadowResult, src/fe_absorption_cuda.hpp:55-103, impl
+ * src/fe_absorption_cuda_certificate.cu): brackets Res_sq(c) = sum_d sum_g
+ * (sum_{i in g} w_i r_ic)^2, Op_sq = sum_d sum_i w_i^2 and Orig_sq(c) =
+ * sum_i v_ic^2 with rigorous a-priori rounding bounds and decides per column,
+ * without sqrt or division: PASS iff R_upper <= L^2 A_lower B_lower, FAIL iff
+ * R_lower > L^2 A_upper B_upper, else INDETERMINATE; ERROR for non-finite
+ * input or any failure (fail-closed).  begin takes the original columns
+ * (device, column-major, ld), finish the residual columns after the solve;
+ * only scalars come back.  Standard dimensions only (a non-NULL slope value
+ * pointer is rejected).  finish and abort free the handle. */
+typedef struct {
+    int verdict;              /* 0 PASS, 1 FAIL, 2 INDETERMINATE, 3 ERROR */
+    int host_like_pass;       /* to-nearest rel_max <= L */
+    double rel_max;           /* to-nearest sqrt(Res_sq / (Op_sq Orig_sq)), max over columns */
+    double rel_max_perdim;    /* the same per dimension, max over dimensions and columns */
+    double r_lower;           /* Res_sq bracket of the deciding column */
+    double r_upper;
+    double wsum_maxdiff;      /* max_g |recomputed - given weight sum| */
+    double wsum_tol;          /* the gamma-scaled tolerance at the worst level */
+    int wsum_ok;              /* 1 iff every level's difference is within its tolerance */
+    long long empty_groups;   /* levels without rows, counted */
+    double shadow_begin_ms;
+    double shadow_finish_ms;
+    int dims;
+    int rhs;
+} xfe_cert_shadow_result;
+
+typedef struct xfe_cert_shadow xfe_cert_shadow;
+
+xfe_cert_shadow* xfe_cert_shadow_begin(const double* d_y, const double* d_x, int n, int cols, int ld,
+                                       const double* d_w, const int* const* d_group_ids,
+                                       const double* const* d_weight_sums, const int* num_groups, int num_dims,
+                                       const double* const* d_slope_values,
+                                       const unsigned char* slope_has_intercept);
+int xfe_cert_shadow_finish(xfe_cert_shadow* handle, const double* d_y_res, const double* d_x_res,
+                           double certificate_limit, xfe_cert_shadow_result* out);
+void xfe_cert_shadow_abort(xfe_cert_shadow* handle);
+
 /* cuda_backend_available (src/fe_absorption_cuda.hpp:125,
  * impl src/fe_absorption_cuda.cu:2281).  1 if a gfx950 device is usable. */
 int xfe_backend_available(void);

@@ -35,6 +35,11 @@ under ``xhdfe-xfe_amd/``) never imports it and never falls back to it.
   src/fe_absorption.cpp:15192-15300 with :15092-15190); pinned against the
   reference's CPU absorber (``ref_absorb_gi``) and the exact weighted least
   squares projection ``gi_exact``.
+* ``cert_shadow``: numpy restatement of the shadow certificate's brackets and
+  decision (src/fe_absorption_cuda_certificate.cu:27-101 decision contract);
+  pinned by exact rational arithmetic (``cert_shadow_exact``: the brackets
+  contain the exact Res_sq, Op_sq, Orig_sq) and by the reference's own
+  certificate values of its outputs (its to-nearest rel).
 """
 from __future__ import annotations
 
@@ -726,6 +731,99 @@ def absorb_group_individual(V, gids, G, gi, weights=None, tol=1e-8, tolerance_mo
         it += b
         cert = certify_group_individual(V0, Vt, gids, G, gi, weights, tol, tolerance_mode)[2]
     return Vt, it, cert
+
+
+_U = 2.0 ** -53
+_ETA = 2.0 ** -1021
+
+
+def _gamma(k):
+    ku = k * _U
+    return ku / (1.0 - ku) * (1.0 + 4.0 * _U)
+
+
+def _up(x):
+    return np.nextafter(x, np.inf)
+
+
+def _down(x):
+    return max(0.0, float(np.nextafter(x, -np.inf)))
+
+
+def _bracket_sum(lo_sum, hi_sum, k):
+    g = _gamma(k + 2.0)
+    return (_down(_down(lo_sum * _down(1.0 - g)) - (k + 2.0) * _ETA),
+            float(_up(_up(hi_sum * _up(1.0 + g)) + (k + 2.0) * _ETA)))
+
+
+def cert_shadow(V0, Vr, gids, G, limit, weights=None):
+    """The shadow certificate's brackets and decision per column (the
+    decision contract of src/fe_absorption_cuda_certificate.cu:15-25):
+    returns dict(verdict per column, R/A/B brackets, rel to nearest).  Level
+    sums S_g are bracketed by |S_g| -+ gamma_{n_g+1} T_g (T_g = sum |w r|), the
+    squares and totals by gamma_{K+2}; every bound is pushed one ulp outward;
+    2^-1021 per operation covers underflow."""
+    V0 = np.asarray(V0, dtype=np.float64).reshape(len(gids[0]), -1)
+    Vr = np.asarray(Vr, dtype=np.float64).reshape(V0.shape)
+    n, C = V0.shape
+    w = np.ones(n) if weights is None else np.asarray(weights, dtype=np.float64)
+    Q = float((w * w).sum())
+    D = len(gids)
+    if weights is None:
+        A_lo = A_hi = D * Q
+    else:
+        qlo, qhi = _bracket_sum(Q, Q, n)
+        A_lo, A_hi = _down(D * qlo), float(_up(D * qhi))
+    out = dict(verdict=[], R=[], A=(A_lo, A_hi), B=[], rel=[])
+    for c in range(C):
+        b = float((V0[:, c] ** 2).sum())
+        B = _bracket_sum(b, b, n)
+        lo = hi = mid = 0.0
+        K = 0.0
+        for d in range(D):
+            cnt = np.bincount(gids[d], minlength=int(G[d]))
+            S = np.abs(np.bincount(gids[d], weights=w * Vr[:, c], minlength=int(G[d])))
+            T = np.bincount(gids[d], weights=np.abs(w * Vr[:, c]), minlength=int(G[d]))
+            m = cnt > 0
+            e = np.array([_gamma(float(k) + 1.0) for k in cnt[m]]) * T[m] * (1.0 + 2.0 * _U) + (cnt[m] + 2.0) * _ETA
+            l = np.maximum(0.0, (S[m] - e) * (1.0 - 2.0 * _U))
+            h = (S[m] + e) * (1.0 + 2.0 * _U)
+            lo += float((l * l).sum())
+            hi += float((h * h).sum())
+            mid += float((S[m] ** 2).sum())
+            K += float(G[d])
+        R = _bracket_sum(lo, hi, K)
+        llo, lhi = _down(limit * limit), float(_up(limit * limit))
+        t_lo = _down(_down(llo * A_lo) * B[0])
+        t_hi = float(_up(_up(lhi * A_hi) * B[1]))
+        v = "PASS" if R[1] <= t_lo else ("FAIL" if R[0] > t_hi else "INDETERMINATE")
+        out["verdict"].append(v)
+        out["R"].append(R)
+        out["B"].append(B)
+        out["rel"].append(np.sqrt(mid) / (np.sqrt(D * Q) * np.sqrt(b)))
+    return out
+
+
+def cert_shadow_exact(V0, Vr, gids, G, weights=None):
+    """Exact Res_sq(c), Op_sq and Orig_sq(c) of the FP64 inputs in rational
+    arithmetic (small cases): the values the shadow brackets must contain."""
+    from fractions import Fraction
+    V0 = np.asarray(V0, dtype=np.float64).reshape(len(gids[0]), -1)
+    Vr = np.asarray(Vr, dtype=np.float64).reshape(V0.shape)
+    n, C = V0.shape
+    w = [Fraction(1)] * n if weights is None else [Fraction(float(x)) for x in weights]
+    A = len(gids) * sum(x * x for x in w)
+    R, B = [], []
+    for c in range(C):
+        B.append(sum(Fraction(float(x)) ** 2 for x in V0[:, c]))
+        tot = Fraction(0)
+        for d in range(len(gids)):
+            sums = [Fraction(0)] * int(G[d])
+            for i, g in enumerate(gids[d]):
+                sums[g] += w[i] * Fraction(float(Vr[i, c]))
+            tot += sum(s * s for s in sums)
+        R.append(tot)
+    return R, A, B
 
 
 def ref_absorb_gi(y, X, fes, gi, weights=None, tol=1e-8, method=1):

@@ -746,3 +746,83 @@ def test_group_individual_rejects_bad_structure():
     gi["group_individual"][3] = gi["num_individuals"]  # out of range
     with pytest.raises(xfe_amd.XfeError):
         xfe_amd.absorb_group_individual(p["y"], p["X"], p["fes"], p["G"], gi)
+
+
+# ---- shadow device certificate (xfe_cert_shadow_*)
+
+def _dev(a, dtype=None):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to("cuda")
+
+
+def _shadow(V0, Vr, gids, G, L, w=None):
+    import torch
+    y0, X0 = _dev(V0[:, 0]), _dev(V0[:, 1:].T)
+    yr, Xr = _dev(Vr[:, 0]), _dev(Vr[:, 1:].T)
+    g = [_dev(x.astype(np.int32)) for x in gids]
+    wd = None if w is None else _dev(w)
+    out = xfe_amd.cert_shadow(y0, X0, yr, Xr, g, G, L, weights=wd)
+    torch.cuda.synchronize()
+    return out
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_cert_shadow_matches_oracle(golden, cases, name):
+    """The device shadow certificate against the oracle's decision on the
+    reference's converged output (PASS at the tight limit) and on the raw
+    columns (FAIL; to-nearest rel within 1e-11 of the reference's own
+    certificate value); counts and weight sums consistent."""
+    _log(f"cert shadow {name}")
+    _, fes, gids, G, V0 = _prep(cases[name])
+    Vt = np.column_stack([golden[f"{name}/tight/y_tilde"], golden[f"{name}/tight/X_tilde"]])
+    conv = _shadow(V0, Vt, gids, G, 8e-12)
+    assert conv["verdict"] == "PASS" and conv["host_like_pass"] == 1, conv
+    # the reference's dense indexer keeps id gaps: empty levels are counted, not dropped
+    empty = sum(int((np.bincount(g, minlength=Gd) == 0).sum()) for g, Gd in zip(gids, G))
+    assert conv["wsum_ok"] == 1 and conv["empty_groups"] == empty and conv["rhs"] == V0.shape[1]
+    raw = _shadow(V0, V0, gids, G, 8e-9)
+    assert raw["verdict"] == "FAIL" and raw["host_like_pass"] == 0
+    ref_rel = golden[f"{name}/raw/cert"][1]
+    assert abs(raw["rel_max"] - ref_rel) <= 1e-11 * ref_rel
+    o = oracle.cert_shadow(V0, V0, gids, G, 8e-9)
+    assert abs(raw["rel_max"] - max(o["rel"])) <= 1e-12 * raw["rel_max"]
+
+
+def test_cert_shadow_bracket_and_straddle(golden, cases):
+    """One column: the device bracket contains the exact rational Res_sq;
+    at L = rel the verdict is INDETERMINATE, 1e-6 either side decides."""
+    _log("cert shadow straddle")
+    _, fes, gids, G, V0 = _prep(cases["sparse_two_way"])
+    y = V0[:, :1]
+    R, A, B = oracle.cert_shadow_exact(y, y, gids, G)
+    r = _shadow(y, y, gids, G, 1.0)
+    assert r["r_lower"] <= R[0] <= r["r_upper"]
+    rel = r["rel_max"]
+    assert _shadow(y, y, gids, G, rel)["verdict"] == "INDETERMINATE"
+    assert _shadow(y, y, gids, G, rel * (1 + 1e-6))["verdict"] == "PASS"
+    assert _shadow(y, y, gids, G, rel * (1 - 1e-6))["verdict"] == "FAIL"
+
+
+def test_cert_shadow_weighted_matches_oracle(golden, cases):
+    from golden_cases import certificate_weights
+    _log("cert shadow weighted")
+    _, fes, gids, G, V0 = _prep(cases["two_way"])
+    w = certificate_weights(V0.shape[0])
+    Vr = V0 * 1e-3
+    d = _shadow(V0, Vr, gids, G, 8e-9, w=w)
+    o = oracle.cert_shadow(V0, Vr, gids, G, 8e-9, weights=w)
+    assert d["verdict"] == ("FAIL" if "FAIL" in o["verdict"] else o["verdict"][0])
+    ref = max(o["rel"])
+    # the oracle's rel uses sum w^2 per dimension as the device's
+    assert abs(d["rel_max"] - ref) <= 1e-11 * ref
+
+
+def test_cert_shadow_rejects_bad_ids():
+    _log("cert shadow bad ids")
+    n = 1000
+    rng = np.random.default_rng(3)
+    V = rng.standard_normal((n, 2))
+    g = rng.integers(0, 10, n).astype(np.int32)
+    g[5] = 10  # outside [0, 10): found on the device, the handle is not created
+    with pytest.raises(xfe_amd.XfeError):
+        _shadow(V, V, [g], [10], 1e-8)

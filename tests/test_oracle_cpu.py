@@ -743,3 +743,65 @@ def test_gi_certificate_rejects_untransformed():
     p, V = _gi_case()
     a, rel, cert = oracle.certify_group_individual(V, V, p["fes"], p["G"], p["gi"])
     assert not cert and rel > 1e-3
+
+
+# ---- shadow device certificate (cuda_cert_shadow_*): the oracle's brackets
+
+def _cert_case(golden, cases, name):
+    p = cases[name]
+    keep = _kept(p)
+    fk = [f[keep] for f in p["fes"]]
+    ck = [oracle.build_indexer(f) for f in fk]
+    V0 = np.column_stack([p["y"][keep], p["X"][keep]])
+    Vt = np.column_stack([golden[f"{name}/tight/y_tilde"], golden[f"{name}/tight/X_tilde"]])
+    return V0, Vt, [c[0] for c in ck], [c[1] for c in ck]
+
+
+def test_cert_shadow_brackets_contain_exact(golden, cases):
+    """Exact rational Res_sq, Op_sq and Orig_sq of the reference's converged
+    output and of the raw columns lie inside the oracle's brackets; converged
+    passes at the tight limit, raw fails."""
+    V0, Vt, gids, G = _cert_case(golden, cases, "sparse_two_way")
+    for Vr, want in ((Vt, "PASS"), (V0, "FAIL")):
+        sh = oracle.cert_shadow(V0, Vr, gids, G, 8e-12)
+        R, A, B = oracle.cert_shadow_exact(V0, Vr, gids, G)
+        assert sh["A"][0] <= A <= sh["A"][1]
+        for c in range(V0.shape[1]):
+            assert sh["R"][c][0] <= R[c] <= sh["R"][c][1]
+            assert sh["B"][c][0] <= B[c] <= sh["B"][c][1]
+        assert set(sh["verdict"]) == {want}
+
+
+def test_cert_shadow_weighted_brackets_contain_exact(golden, cases):
+    from golden_cases import certificate_weights
+    V0, Vt, gids, G = _cert_case(golden, cases, "sparse_two_way")
+    w = certificate_weights(V0.shape[0])
+    sh = oracle.cert_shadow(V0[:, :1], V0[:, :1] * 0.5 + Vt[:, :1], gids, G, 8e-9, weights=w)
+    R, A, B = oracle.cert_shadow_exact(V0[:, :1], V0[:, :1] * 0.5 + Vt[:, :1], gids, G, weights=w)
+    assert sh["A"][0] <= A <= sh["A"][1]
+    assert sh["R"][0][0] <= R[0] <= sh["R"][0][1]
+
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+def test_cert_shadow_value_matches_reference(golden, cases, name):
+    """The to-nearest value behind the decision is the reference's own
+    certificate of the raw columns (fixture, 1e-11); its converged outputs
+    pass, as the reference certified them."""
+    V0, Vt, gids, G = _cert_case(golden, cases, name)
+    raw = oracle.cert_shadow(V0, V0, gids, G, 8e-9)
+    ref_rel = golden[f"{name}/raw/cert"][1]
+    assert abs(max(raw["rel"]) - ref_rel) <= 1e-11 * ref_rel
+    assert set(raw["verdict"]) == {"FAIL"}
+    assert golden[f"{name}/tight/cert"][2] == 1
+    assert set(oracle.cert_shadow(V0, Vt, gids, G, 8e-12)["verdict"]) == {"PASS"}
+
+
+def test_cert_shadow_limit_straddle(golden, cases):
+    """At L = the to-nearest rel the brackets straddle (INDETERMINATE); a
+    relative 1e-6 either side decides."""
+    V0, _, gids, G = _cert_case(golden, cases, "two_way")
+    y = V0[:, :1]
+    rel = oracle.cert_shadow(y, y, gids, G, 1.0)["rel"][0]
+    assert oracle.cert_shadow(y, y, gids, G, rel)["verdict"] == ["INDETERMINATE"]
+    assert oracle.cert_shadow(y, y, gids, G, rel * (1 + 1e-6))["verdict"] == ["PASS"]
+    assert oracle.cert_shadow(y, y, gids, G, rel * (1 - 1e-6))["verdict"] == ["FAIL"]

@@ -47,6 +47,13 @@ class GroupIndividual(ctypes.Structure):
                 ("individual_ptr", _p), ("individual_group", _p), ("group_scale", _p)]
 
 
+class CertShadowResult(ctypes.Structure):
+    _fields_ = [("verdict", _i), ("host_like_pass", _i), ("rel_max", _d), ("rel_max_perdim", _d),
+                ("r_lower", _d), ("r_upper", _d), ("wsum_maxdiff", _d), ("wsum_tol", _d), ("wsum_ok", _i),
+                ("empty_groups", ctypes.c_longlong), ("shadow_begin_ms", _d), ("shadow_finish_ms", _d),
+                ("dims", _i), ("rhs", _i)]
+
+
 class RecoveryResult(ctypes.Structure):
     _fields_ = [("iterations", _i), ("converged", _i), ("max_delta", _d), ("absorb_converged", _i)]
 
@@ -63,7 +70,8 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
            "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
            "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
-           "xfe_absorb_group_individual")
+           "xfe_absorb_group_individual",
+           "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort")
 
 _lib = None
 
@@ -198,6 +206,40 @@ def absorb_group_individual(y, X, gids, G, gi, weights=None, sweep_order=None, t
     if not ok:
         raise XfeError(last_error())
     return yo, Xo, _info(r, D)
+
+
+VERDICTS = ("PASS", "FAIL", "INDETERMINATE", "ERROR")
+
+
+def cert_shadow(y0, X0, y_res, X_res, gids, G, limit, weights=None, weight_sums=None):
+    """The shadow device certificate on torch device tensors: y (n), X (k x n,
+    each row one column of the column-major block, so ld = n), gids (int32
+    device tensors), optional weights (n) and weight_sums (per dimension).
+    Returns the result fields as a dict ("verdict" as a name)."""
+    L = lib()
+    n = int(y0.numel())
+    k = int(X0.shape[0]) if X0 is not None and X0.numel() else 0
+    D = len(gids)
+    gp = (_p * D)(*[g.data_ptr() for g in gids])
+    wsp = None
+    if weight_sums is not None:
+        wsp = (_p * D)(*[t.data_ptr() for t in weight_sums])
+    Garr = (_i * D)(*[int(x) for x in G])
+    L.xfe_cert_shadow_begin.restype = _p
+    L.xfe_cert_shadow_begin.argtypes = [_p, _p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p]
+    h = L.xfe_cert_shadow_begin(y0.data_ptr(), X0.data_ptr() if k else None, n, k, n,
+                                weights.data_ptr() if weights is not None else None, gp, wsp, Garr, D, None, None)
+    if not h:
+        raise XfeError(last_error())
+    r = CertShadowResult()
+    L.xfe_cert_shadow_finish.argtypes = [_p, _p, _p, _d, ctypes.c_void_p]
+    ok = L.xfe_cert_shadow_finish(h, y_res.data_ptr(), X_res.data_ptr() if k else None, float(limit),
+                                  ctypes.byref(r))
+    if not ok:
+        raise XfeError(last_error())
+    out = {name: getattr(r, name) for name, _ in CertShadowResult._fields_}
+    out["verdict"] = VERDICTS[r.verdict]
+    return out
 
 
 def recover_fixed_effects(partial, gids, G, weights=None, sweep_order=None, tol=1e-8, fe_tol=1e-6,
