@@ -509,6 +509,15 @@ def main():
         log(f"accuracy: {accuracy}")
         del V_plain, V_ref
 
+    # the shadow device certificate of the last step's result (outside the timed region)
+    torch.cuda.synchronize()
+    shadow = xfe_amd.cert_shadow_raw(V0.data_ptr(), V0.data_ptr() + 8 * ld, V.data_ptr(), V.data_ptr() + 8 * ld,
+                                     n, C - 1, ld, [ids.data_ptr() + 4 * n * d for d in range(len(G))], G,
+                                     limit=8e-9)
+    shadow = {"verdict": shadow["verdict"], "rel_max": shadow["rel_max"], "r_upper": shadow["r_upper"],
+              "ms": round(shadow["shadow_begin_ms"] + shadow["shadow_finish_ms"], 2), "limit": 8e-9}
+    log(f"shadow certificate: {shadow}")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -548,6 +557,7 @@ def main():
                      "bytes_per_launch": round(pass_bytes / max(launches, 1), 1)},
         "cpu_baseline": cpu,
         "singleton_s": round(t_single, 3),
+        "shadow_certificate": shadow,
     }
     print(json.dumps(out), flush=True)
 

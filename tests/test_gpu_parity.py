@@ -826,3 +826,20 @@ def test_cert_shadow_rejects_bad_ids():
     g[5] = 10  # outside [0, 10): found on the device, the handle is not created
     with pytest.raises(xfe_amd.XfeError):
         _shadow(V, V, [g], [10], 1e-8)
+
+
+def test_cert_shadow_akm_scaled():
+    """A 1 % AKM panel (worker and firm tables beyond the LDS copies, the year
+    table inside them): raw columns FAIL with the oracle's to-nearest value
+    (1e-12), the device absorber's converged output PASSES."""
+    _log("cert shadow akm 1%")
+    p = panels.akm_small(scale=0.01)
+    _, fes, gids, G, V0 = _prep(p)
+    raw = _shadow(V0, V0, gids, G, 8e-9)
+    o = oracle.cert_shadow(V0, V0, gids, G, 8e-9)
+    assert raw["verdict"] == "FAIL"
+    assert abs(raw["rel_max"] - max(o["rel"])) <= 1e-12 * raw["rel_max"]
+    yg, Xg, info = xfe_amd.absorb(V0[:, 0], V0[:, 1:], gids, G, tol=1e-8)
+    conv = _shadow(V0, np.column_stack([yg, Xg]), gids, G, 8e-9)
+    assert conv["verdict"] == "PASS" and conv["host_like_pass"] == 1, conv
+    assert o["verdict"] == ["FAIL"] * V0.shape[1]

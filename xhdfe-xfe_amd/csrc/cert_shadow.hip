@@ -106,11 +106,22 @@ __global__ void cs_sumsq_kernel(const double* __restrict__ y, const double* __re
 }
 
 // S[g * C1 + c] += w_i r_ic, T[g * C1 + c] += |w_i r_ic| (C1 = cols + 1).
+// With lds (both G x C1 tables fit), each block sums into LDS copies and adds
+// their non-zero entries to S and T once; any order of the additions is
+// covered by the bound.
 __global__ void cs_level_kernel(const double* __restrict__ y, const double* __restrict__ X, int ld, int n,
                                 int cols, const int* __restrict__ gid, int G, const double* __restrict__ w,
-                                double* __restrict__ S, double* __restrict__ T) {
+                                double* __restrict__ S, double* __restrict__ T, int lds) {
+    extern __shared__ __align__(16) double ltab[];
     const int stride = gridDim.x * blockDim.x;
     const int C1 = cols + 1;
+    const int GC = G * C1;
+    if (lds) {
+        for (int q = threadIdx.x; q < 2 * GC; q += blockDim.x) ltab[q] = 0.0;
+        __syncthreads();
+    }
+    double* s_tab = lds ? ltab : S;
+    double* t_tab = lds ? ltab + GC : T;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         if (gid[i] < 0 || gid[i] >= G) continue;  // rejected at begin
         const double wi = w ? w[i] : 1.0;
@@ -118,8 +129,15 @@ __global__ void cs_level_kernel(const double* __restrict__ y, const double* __re
         for (int c = 0; c < C1; ++c) {
             const double v = c == 0 ? y[i] : X[static_cast<size_t>(c - 1) * ld + i];
             const double x = wi * v;
-            atomicAdd(&S[base + c], x);
-            atomicAdd(&T[base + c], fabs(x));
+            atomicAdd(&s_tab[base + c], x);
+            atomicAdd(&t_tab[base + c], fabs(x));
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < GC; q += blockDim.x) {
+            if (ltab[q] != 0.0) atomicAdd(&S[q], ltab[q]);
+            if (ltab[GC + q] != 0.0) atomicAdd(&T[q], ltab[GC + q]);
         }
     }
 }
@@ -364,8 +382,10 @@ int xfe_cert_shadow_finish(xfe_cert_shadow* h, const double* d_y_res, const doub
             h->T.ensure(GC);
             XFE_CHECK(hipMemsetAsync(h->S.p, 0, sizeof(double) * GC, h->s));
             XFE_CHECK(hipMemsetAsync(h->T.p, 0, sizeof(double) * GC, h->s));
-            hipLaunchKernelGGL(cs_level_kernel, dim3(cs_grid(n)), dim3(kCsBlock), 0, h->s, d_y_res, d_x_res, h->ld, n,
-                               h->cols, h->gid[d], G, h->w, h->S.p, h->T.p);
+            const size_t lds_bytes = 2 * GC * sizeof(double);
+            const int lds = lds_bytes <= 48 * 1024 ? 1 : 0;
+            hipLaunchKernelGGL(cs_level_kernel, dim3(cs_grid(n)), dim3(kCsBlock), lds ? lds_bytes : 0, h->s, d_y_res,
+                               d_x_res, h->ld, n, h->cols, h->gid[d], G, h->w, h->S.p, h->T.p, lds);
             XFE_CHECK(hipGetLastError());
             const int gg = cs_grid(G);
             h->part.ensure(static_cast<size_t>(gg) * C1 * 3);

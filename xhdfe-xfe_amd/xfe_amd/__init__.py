@@ -211,35 +211,40 @@ def absorb_group_individual(y, X, gids, G, gi, weights=None, sweep_order=None, t
 VERDICTS = ("PASS", "FAIL", "INDETERMINATE", "ERROR")
 
 
-def cert_shadow(y0, X0, y_res, X_res, gids, G, limit, weights=None, weight_sums=None):
-    """The shadow device certificate on torch device tensors: y (n), X (k x n,
-    each row one column of the column-major block, so ld = n), gids (int32
-    device tensors), optional weights (n) and weight_sums (per dimension).
-    Returns the result fields as a dict ("verdict" as a name)."""
+def cert_shadow_raw(y0, x0, y_res, x_res, n, k, ld, gid_ptrs, G, limit, w=None, wsum_ptrs=None):
+    """The shadow device certificate on device addresses: original and
+    residual columns (y, then X column-major with leading dimension ld),
+    per-dimension id and weight-sum addresses.  Returns the result fields as
+    a dict ("verdict" as a name)."""
     L = lib()
-    n = int(y0.numel())
-    k = int(X0.shape[0]) if X0 is not None and X0.numel() else 0
-    D = len(gids)
-    gp = (_p * D)(*[g.data_ptr() for g in gids])
-    wsp = None
-    if weight_sums is not None:
-        wsp = (_p * D)(*[t.data_ptr() for t in weight_sums])
+    D = len(gid_ptrs)
+    gp = (_p * D)(*gid_ptrs)
+    wsp = None if wsum_ptrs is None else (_p * D)(*wsum_ptrs)
     Garr = (_i * D)(*[int(x) for x in G])
     L.xfe_cert_shadow_begin.restype = _p
     L.xfe_cert_shadow_begin.argtypes = [_p, _p, _i, _i, _i, _p, _p, _p, _p, _i, _p, _p]
-    h = L.xfe_cert_shadow_begin(y0.data_ptr(), X0.data_ptr() if k else None, n, k, n,
-                                weights.data_ptr() if weights is not None else None, gp, wsp, Garr, D, None, None)
+    h = L.xfe_cert_shadow_begin(y0, x0 if k else None, n, k, ld, w, gp, wsp, Garr, D, None, None)
     if not h:
         raise XfeError(last_error())
     r = CertShadowResult()
     L.xfe_cert_shadow_finish.argtypes = [_p, _p, _p, _d, ctypes.c_void_p]
-    ok = L.xfe_cert_shadow_finish(h, y_res.data_ptr(), X_res.data_ptr() if k else None, float(limit),
-                                  ctypes.byref(r))
-    if not ok:
+    if not L.xfe_cert_shadow_finish(h, y_res, x_res if k else None, float(limit), ctypes.byref(r)):
         raise XfeError(last_error())
     out = {name: getattr(r, name) for name, _ in CertShadowResult._fields_}
     out["verdict"] = VERDICTS[r.verdict]
     return out
+
+
+def cert_shadow(y0, X0, y_res, X_res, gids, G, limit, weights=None, weight_sums=None):
+    """cert_shadow_raw on torch device tensors: y (n), X (k x n, each row one
+    column, so ld = n), gids (int32 device tensors), optional weights (n) and
+    weight_sums (per dimension)."""
+    n = int(y0.numel())
+    k = int(X0.shape[0]) if X0 is not None and X0.numel() else 0
+    return cert_shadow_raw(y0.data_ptr(), X0.data_ptr() if k else None, y_res.data_ptr(),
+                           X_res.data_ptr() if k else None, n, k, n, [g.data_ptr() for g in gids], G, limit,
+                           w=weights.data_ptr() if weights is not None else None,
+                           wsum_ptrs=None if weight_sums is None else [t.data_ptr() for t in weight_sums])
 
 
 def recover_fixed_effects(partial, gids, G, weights=None, sweep_order=None, tol=1e-8, fe_tol=1e-6,
