@@ -2672,7 +2672,7 @@ int xfe_absorb_group_individual(const double* y, const double* X, int n, int k, 
                 if (cnt <= 0) continue;
                 XFE_CHECK(launch_gi_color_sweep(V, n, C, ws.gi_iptr.p, ws.gi_igroup.p, ws.gi_scale.p, w,
                                                 ws.gi_denom.p, ws.gi_members.p + offsets[c], cnt, relax,
-                                                grid_for(cnt), s));
+                                                grid_for(static_cast<long long>(cnt) * kWave), s));
             }
         };
         auto full_sweep = [&]() {

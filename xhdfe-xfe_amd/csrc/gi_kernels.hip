@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "absorb_kernels.h"
+#include "wave_ops.h"
 
 namespace xfe {
 
@@ -71,28 +72,32 @@ __global__ void gi_denom_kernel(const int* __restrict__ ind_ptr, const int* __re
     }
 }
 
-// One colour class: for each individual j of the class and column c,
-// a = relax * sum_g w_g s_g v[g, c] / denom[j]; v[g, c] -= s_g a.
+// One colour class, one wavefront per individual: for each column c the
+// lanes stride over the individual's groups, a = relax * sum_g w_g s_g v[g, c]
+// / denom[j] by a wave sum, then v[g, c] -= s_g a (atomic, so an incidence
+// listed twice is still subtracted twice).  Individuals with thousands of
+// groups no longer hold a launch back on one thread.
 __global__ void gi_color_sweep_kernel(double* __restrict__ V, int n, int C, const int* __restrict__ ind_ptr,
                                       const int* __restrict__ ind_group, const double* __restrict__ scale,
                                       const double* __restrict__ w, const double* __restrict__ denom,
                                       const int* __restrict__ members, int count, double relax) {
-    const int stride = gridDim.x * blockDim.x;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += stride) {
+    const int lane = lane_id();
+    const int waves = gridDim.x * (blockDim.x / kWave);
+    for (int t = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; t < count; t += waves) {
         const int j = members[t];
         const int b = ind_ptr[j], e = ind_ptr[j + 1];
         const double inv = relax / denom[j];
         for (int c = 0; c < C; ++c) {
             double* v = V + static_cast<size_t>(c) * n;
             double num = 0.0;
-            for (int q = b; q < e; ++q) {
+            for (int q = b + lane; q < e; q += kWave) {
                 const int g = ind_group[q];
                 num += (w ? w[g] : 1.0) * scale[g] * v[g];
             }
-            const double a = num * inv;
-            for (int q = b; q < e; ++q) {
+            const double a = __shfl(wave_sum(num), 0, kWave) * inv;  // the total, from lane 0
+            for (int q = b + lane; q < e; q += kWave) {
                 const int g = ind_group[q];
-                v[g] -= scale[g] * a;
+                atomicAdd(&v[g], -(scale[g] * a));
             }
         }
     }
