@@ -46,6 +46,17 @@ def test_fail_closed_without_device():
     with pytest.raises(xfe_amd.XfeError):
         xfe_amd.drop_singletons([g])
     assert xfe_amd.last_error() != ""
+    # the savefe recovery and the group()/individual() absorber fail closed too
+    with pytest.raises(xfe_amd.XfeError):
+        xfe_amd.recover_fixed_effects(y, [g, g[::-1].copy()], [4, 4])
+    gi = dict(num_individuals=2, group_ptr=np.arange(n + 1, dtype=np.int32),
+              group_individual=(np.arange(n) % 2).astype(np.int32),
+              individual_ptr=np.array([0, n // 2, n], dtype=np.int32),
+              individual_group=np.concatenate([np.arange(0, n, 2), np.arange(1, n, 2)]).astype(np.int32),
+              group_scale=np.ones(n))
+    with pytest.raises(xfe_amd.XfeError) as e:
+        xfe_amd.absorb_group_individual(y, X, [g], [4], gi)
+    assert "structure" not in str(e.value)  # a valid structure: the device is what is missing
 
 
 def test_invalid_arguments_rejected_before_any_device_call():
@@ -53,3 +64,24 @@ def test_invalid_arguments_rejected_before_any_device_call():
     X = np.zeros((4, 1))
     with pytest.raises(xfe_amd.XfeError, match="invalid"):
         xfe_amd.absorb(y, X, [np.zeros(4, np.int32)], [0])
+
+
+def test_new_entry_points_validate_before_the_device():
+    """Argument checks of the recovery, group()/individual() and shadow
+    entry points run on the host: they fail with a message and no device
+    call (also without a device)."""
+    y = np.arange(8, dtype=np.float64)
+    g = (np.arange(8) % 4).astype(np.int32)
+    with pytest.raises(xfe_amd.XfeError, match="weighted recovery"):
+        xfe_amd.recover_fixed_effects(y, [g, g[::-1].copy()], [4, 4], weights=np.ones(8))
+    bad = dict(num_individuals=2, group_ptr=np.arange(9, dtype=np.int32),
+               group_individual=np.full(8, 5, dtype=np.int32),  # individual 5 of 2
+               individual_ptr=np.array([0, 4, 8], dtype=np.int32),
+               individual_group=np.arange(8, dtype=np.int32), group_scale=np.ones(8))
+    with pytest.raises(xfe_amd.XfeError, match="group/individual structure"):
+        xfe_amd.absorb_group_individual(y, y.reshape(8, 1), [g], [4], bad)
+    import ctypes
+    L = xfe_amd.lib()
+    L.xfe_cert_shadow_begin.restype = ctypes.c_void_p
+    assert not L.xfe_cert_shadow_begin(None, None, 8, 0, 8, None, None, None, None, 1, None, None)
+    assert "invalid" in xfe_amd.last_error()
