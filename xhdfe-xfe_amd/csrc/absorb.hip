@@ -128,7 +128,7 @@ struct Workspace {
     DevBuf<int> gi_iptr, gi_igroup, gi_members;
     DevBuf<double> gi_scale, gi_denom;
     // FE recovery: the untransformed column and the level coefficients
-    DevBuf<double> rec_p;
+    DevBuf<double> rec_p, rec_contrib;
     DevBuf<double> rec_alpha[3];
     // the last mixed-precision solve of this workspace: primary dimension and
     // where each other dimension's coefficients sit in f_beta (level-major, C wide)
@@ -167,7 +167,7 @@ struct Workspace {
         p_rk.release(); p_r1.release(); p_len.release(); p_a.release(); p_wseg.release();
         f_invw.release(); d_invw.release();
         for (int k = 0; k < 2; ++k) { seg_bk[k].release(); f_bv[k].release(); d_bv[k].release(); }
-        rec_p.release();
+        rec_p.release(); rec_contrib.release();
         for (auto& b : rec_alpha) b.release();
         gi_iptr.release(); gi_igroup.release(); gi_members.release(); gi_scale.release(); gi_denom.release();
         fast_rec.used = false;
@@ -2546,12 +2546,12 @@ int xfe_recover_fixed_effects(const double* partial, int n, const xfe_fe_input* 
         XFE_CHECK(hipStreamSynchronize(s));
         const double mx = max_delta_staged(ws, n, fes, num_fes, w, sweep_order);
         if (contributions_out) {
-            for (int d = 0; d < num_fes; ++d) {
-                const int* g = fes[d].group_ids;
-                const double* al = alpha_out[d];
-                double* c = contributions_out + static_cast<size_t>(d) * n;
-                for (int i = 0; i < n; ++i) c[i] = al[g[i]];
-            }
+            // expanded on the device, downloaded once
+            ws.rec_contrib.ensure(static_cast<size_t>(n) * num_fes);
+            XFE_CHECK(launch_recover_expand(a, ws.rec_contrib.p, n, grid, s));
+            XFE_CHECK(hipMemcpyAsync(contributions_out, ws.rec_contrib.p, sizeof(double) * n * num_fes,
+                                     hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
         }
         out->iterations = ares.iterations;
         out->absorb_converged = ares.converged;

@@ -288,6 +288,7 @@ struct RecoverArgs {
 hipError_t launch_recover_primary(const RecoverArgs& a, double* sums, int n, int grid, hipStream_t s);
 hipError_t launch_recover_divide(double* alpha, const double* wsum, int G, int grid, hipStream_t s);
 hipError_t launch_recover_residual(const RecoverArgs& a, double* res, int n, int grid, hipStream_t s);
+hipError_t launch_recover_expand(const RecoverArgs& a, double* contrib, int n, int grid, hipStream_t s);
 
 // group()/individual() absorber (gi_kernels.hip); V is n x C, ld n.
 hipError_t launch_gi_std_accumulate(const double* V, int n, int C, const int* gid, const double* w, double* sums,

@@ -705,6 +705,18 @@ __global__ void recover_residual_kernel(RecoverArgs a, double* __restrict__ res,
     }
 }
 
+// contrib[d * n + i] = alpha_d[gid_d[i]] (the per-row contributions, dimension-major).
+__global__ void recover_expand_kernel(RecoverArgs a, double* __restrict__ contrib, int n) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        for (int k = 0; k < a.D; ++k) contrib[static_cast<size_t>(k) * n + i] = a.alpha[k][a.gid[k][i]];
+}
+
+hipError_t launch_recover_expand(const RecoverArgs& a, double* contrib, int n, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(recover_expand_kernel, dim3(grid), dim3(kBlock), 0, s, a, contrib, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_recover_primary(const RecoverArgs& a, double* sums, int n, int grid, hipStream_t s) {
     hipLaunchKernelGGL(recover_primary_kernel, dim3(grid), dim3(kBlock), 0, s, a, sums, n);
     return hipGetLastError();
