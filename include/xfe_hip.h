@@ -279,6 +279,16 @@ int xfe_recover_fixed_effects(const double* partial, int n, const xfe_fe_input* 
                               double fe_tol, double* const* alpha_out, double* contributions_out,
                               xfe_recovery_result* out);
 
+/* articulation_points (src/akm_kss.cpp:444-493; called by build_leave_out_sample's
+ * pruning loop, :736-870): artic[v] = 1 iff removing vertex v disconnects its
+ * connected component, for the undirected graph with neighbours
+ * adj[adj_ptr[v] .. adj_ptr[v+1]) of vertex v (each edge listed from both ends,
+ * no self loops or repeated edges).  Computed on the device by the
+ * Tarjan-Vishkin method on a breadth-first spanning forest; the graph is
+ * validated on the host.  Returns 1 on success, 0 on invalid input or a device
+ * failure. */
+int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, unsigned char* artic);
+
 /* ---- AKM/KSS leverage solver (include/hdfe/akm_kss_cuda.hpp:19-74, impl
  * src/akm_kss_cuda.cu:164-722): the grounded worker-partialled firm Laplacian
  * S = Df - B' Dw^{-1} B over n_firms levels (ground: the last firm), B the

@@ -40,6 +40,9 @@ under ``xhdfe-xfe_amd/``) never imports it and never falls back to it.
   pinned by exact rational arithmetic (``cert_shadow_exact``: the brackets
   contain the exact Res_sq, Op_sq, Orig_sq) and by the reference's own
   certificate values of its outputs (its to-nearest rel).
+* ``articulation_points``: restatement of the reference's iterative Tarjan
+  depth-first search (src/akm_kss.cpp:444-493); pinned by brute force (remove
+  each vertex, count components) on small graphs.
 """
 from __future__ import annotations
 
@@ -731,6 +734,65 @@ def absorb_group_individual(V, gids, G, gi, weights=None, tol=1e-8, tolerance_mo
         it += b
         cert = certify_group_individual(V0, Vt, gids, G, gi, weights, tol, tolerance_mode)[2]
     return Vt, it, cert
+
+
+def articulation_points(adj_ptr, adj):
+    """articulation_points (src/akm_kss.cpp:444-493): iterative depth-first
+    search from every undiscovered vertex in index order, disc/low times,
+    a non-root u with a child c of low(c) >= disc(u) is a cut vertex, a root
+    with two or more children is one."""
+    adj_ptr = np.asarray(adj_ptr)
+    adj = np.asarray(adj)
+    n = adj_ptr.size - 1
+    artic = np.zeros(n, dtype=np.uint8)
+    disc = np.full(n, -1, dtype=np.int64)
+    low = np.zeros(n, dtype=np.int64)
+    parent = np.full(n, -1, dtype=np.int64)
+    child = np.zeros(n, dtype=np.int64)
+    it = np.zeros(n, dtype=np.int64)
+    timer = 0
+    for root in range(n):
+        if disc[root] != -1:
+            continue
+        stack = [root]
+        disc[root] = low[root] = timer
+        timer += 1
+        it[root] = adj_ptr[root]
+        while stack:
+            u = stack[-1]
+            if it[u] < adj_ptr[u + 1]:
+                v = int(adj[it[u]])
+                it[u] += 1
+                if disc[v] == -1:
+                    parent[v] = u
+                    child[u] += 1
+                    disc[v] = low[v] = timer
+                    timer += 1
+                    it[v] = adj_ptr[v]
+                    stack.append(v)
+                elif v != parent[u]:
+                    low[u] = min(low[u], disc[v])
+            else:
+                stack.pop()
+                p = parent[u]
+                if p != -1:
+                    low[p] = min(low[p], low[u])
+                    if p != root and low[u] >= disc[p]:
+                        artic[p] = 1
+        if child[root] >= 2:
+            artic[root] = 1
+    return artic
+
+
+def graph_csr(n, edges):
+    """CSR (adj_ptr, adj) of an undirected edge list, neighbours sorted."""
+    e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    u = np.concatenate([e[:, 0], e[:, 1]])
+    v = np.concatenate([e[:, 1], e[:, 0]])
+    o = np.lexsort((v, u))
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(ptr, u[o] + 1, 1)
+    return np.cumsum(ptr).astype(np.int32), v[o].astype(np.int32)
 
 
 _U = 2.0 ** -53

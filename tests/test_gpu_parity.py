@@ -843,3 +843,81 @@ def test_cert_shadow_akm_scaled():
     conv = _shadow(V0, np.column_stack([yg, Xg]), gids, G, 8e-9)
     assert conv["verdict"] == "PASS" and conv["host_like_pass"] == 1, conv
     assert o["verdict"] == ["FAIL"] * V0.shape[1]
+
+
+# ---- articulation points (xfe_articulation_points)
+
+def _random_graph_list(seed, count, nmax):
+    rng = np.random.default_rng(seed)
+    for t in range(count):
+        n = int(rng.integers(1, nmax))
+        kind = t % 4
+        if kind == 0:
+            raw = rng.integers(0, n, (int(rng.integers(0, 2 * n + 1)), 2))
+            edges = {tuple(sorted(map(int, e))) for e in raw if e[0] != e[1]}
+        elif kind == 1:
+            edges = {(i, i + 1) for i in range(n - 1)}
+            edges |= {tuple(sorted(map(int, e))) for e in rng.integers(0, n, (int(rng.integers(0, 3)), 2)) if e[0] != e[1]}
+        elif kind == 2:
+            edges = {(int(rng.integers(0, i)), i) for i in range(1, n)}
+        else:
+            h = max(n // 2, 1)
+            edges = {tuple(sorted(map(int, e))) for e in rng.integers(0, h, (n, 2)) if e[0] != e[1]}
+            edges |= {tuple(sorted(map(int, e))) for e in rng.integers(h, n, (n, 2)) if e[0] != e[1]} if n > h else set()
+            if n > 2:
+                edges.add((0, n - 1))
+        yield n, sorted(edges)
+
+
+def test_articulation_points_random_graphs():
+    """The device cut-vertex search equals the oracle's depth-first search
+    (itself pinned by brute force) on 400 small graphs, bit for bit."""
+    _log("articulation random graphs")
+    for n, edges in _random_graph_list(11, 400, 40):
+        ptr, adj = oracle.graph_csr(n, edges)
+        got = xfe_amd.articulation_points(ptr, adj)
+        want = oracle.articulation_points(ptr, adj)
+        assert (got == want).all(), (n, edges)
+
+
+def test_articulation_points_path_and_isolated():
+    """A 3,000-vertex path (every inner vertex cut, 2,999 BFS levels from the
+    root), isolated vertices and a cycle (no cut vertex)."""
+    _log("articulation path")
+    n = 3000
+    ptr, adj = oracle.graph_csr(n + 5, [(i, i + 1) for i in range(n - 1)] + [(n + 2, n + 3), (n + 3, n + 4),
+                                                                             (n + 4, n + 2)])
+    got = xfe_amd.articulation_points(ptr, adj)
+    want = np.zeros(n + 5, dtype=np.uint8)
+    want[1:n - 1] = 1
+    np.testing.assert_array_equal(got, want)
+
+
+def test_articulation_points_mover_graph():
+    """The mover-firm graph of an AKM panel (workers with two or more firms
+    and their firms, the graph build_leave_out_sample prunes): the device
+    equals the oracle."""
+    _log("articulation mover graph")
+    p = panels.akm_panel(workers=40000, firms=6000, years=20, p_move=0.04)
+    w, f = p["fes"][0], p["fes"][1]
+    key = np.unique(w.astype(np.int64) * 6000 + f)
+    mw, mf = key // 6000, key % 6000
+    deg = np.bincount(mw, minlength=40000)
+    keep = deg[mw] >= 2
+    mw, mf = mw[keep], mf[keep]
+    _, wi = np.unique(mw, return_inverse=True)
+    fu, fi = np.unique(mf, return_inverse=True)
+    nm = wi.max() + 1
+    ptr, adj = oracle.graph_csr(nm + fu.size, np.column_stack([wi, nm + fi]))
+    got = xfe_amd.articulation_points(ptr, adj)
+    want = oracle.articulation_points(ptr, adj)
+    _log(f"  vertices {ptr.size - 1}, edges {adj.size // 2}, cut vertices {int(want.sum())}")
+    assert want.sum() > 0
+    np.testing.assert_array_equal(got, want)
+
+
+def test_articulation_points_rejects_bad_graph():
+    ptr = np.array([0, 1, 2], dtype=np.int32)
+    adj = np.array([1, 2], dtype=np.int32)  # vertex 2 does not exist
+    with pytest.raises(xfe_amd.XfeError):
+        xfe_amd.articulation_points(ptr, adj)

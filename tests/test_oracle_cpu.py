@@ -805,3 +805,49 @@ def test_cert_shadow_limit_straddle(golden, cases):
     assert oracle.cert_shadow(y, y, gids, G, rel)["verdict"] == ["INDETERMINATE"]
     assert oracle.cert_shadow(y, y, gids, G, rel * (1 + 1e-6))["verdict"] == ["PASS"]
     assert oracle.cert_shadow(y, y, gids, G, rel * (1 - 1e-6))["verdict"] == ["FAIL"]
+
+
+# ---- articulation points (the leave-out set's cut-vertex search)
+
+def _random_graphs(seed, count, nmax):
+    rng = np.random.default_rng(seed)
+    for t in range(count):
+        n = int(rng.integers(1, nmax))
+        kind = t % 4
+        if kind == 0:
+            raw = rng.integers(0, n, (int(rng.integers(0, 2 * n + 1)), 2))
+            edges = {tuple(sorted(map(int, e))) for e in raw if e[0] != e[1]}
+        elif kind == 1:  # chains with a few chords
+            edges = {(i, i + 1) for i in range(n - 1)}
+            edges |= {tuple(sorted(map(int, e))) for e in rng.integers(0, n, (int(rng.integers(0, 3)), 2)) if e[0] != e[1]}
+        elif kind == 2:  # trees
+            edges = {(int(rng.integers(0, i)), i) for i in range(1, n)}
+        else:  # two dense halves joined by one edge
+            h = max(n // 2, 1)
+            edges = {tuple(sorted(map(int, e))) for e in rng.integers(0, h, (n, 2)) if e[0] != e[1]}
+            edges |= {tuple(sorted(map(int, e))) for e in rng.integers(h, n, (n, 2)) if e[0] != e[1]} if n > h else set()
+            if n > 2:
+                edges.add((0, n - 1))
+        yield n, sorted(edges)
+
+
+def _brute_cut_vertices(n, ptr, adj):
+    A = sp.csr_matrix((np.ones(len(adj)), adj, ptr), shape=(n, n))
+    base = sp.csgraph.connected_components(A, directed=False)[0]
+    out = np.zeros(n, dtype=np.uint8)
+    for x in range(n):
+        keep = np.ones(n, bool)
+        keep[x] = False
+        c = sp.csgraph.connected_components(A[keep][:, keep], directed=False)[0]
+        out[x] = c > base - (1 if ptr[x + 1] == ptr[x] else 0)
+    return out
+
+
+def test_articulation_oracle_is_brute_force():
+    """The restated depth-first search flags exactly the vertices whose
+    removal adds a component (400 small graphs: random, chains with chords,
+    trees, two halves joined by a bridge)."""
+    import scipy.sparse.csgraph  # noqa: F401
+    for n, edges in _random_graphs(7, 400, 22):
+        ptr, adj = oracle.graph_csr(n, edges)
+        np.testing.assert_array_equal(oracle.articulation_points(ptr, adj), _brute_cut_vertices(n, ptr, adj))

@@ -71,7 +71,8 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
            "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
            "xfe_absorb_group_individual",
-           "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort")
+           "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort",
+           "xfe_articulation_points")
 
 _lib = None
 
@@ -206,6 +207,17 @@ def absorb_group_individual(y, X, gids, G, gi, weights=None, sweep_order=None, t
     if not ok:
         raise XfeError(last_error())
     return yo, Xo, _info(r, D)
+
+
+def articulation_points(adj_ptr, adj):
+    """Cut vertices of an undirected CSR graph (uint8 flag per vertex)."""
+    ptr = np.ascontiguousarray(adj_ptr, dtype=np.int32)
+    a = np.ascontiguousarray(adj, dtype=np.int32)
+    n = ptr.size - 1
+    out = np.zeros(max(n, 1), dtype=np.uint8)
+    if not lib().xfe_articulation_points(_i(n), _ptr(ptr), _ptr(a) if a.size else None, _ptr(out)):
+        raise XfeError(last_error())
+    return out[:n]
 
 
 VERDICTS = ("PASS", "FAIL", "INDETERMINATE", "ERROR")
