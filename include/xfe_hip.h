@@ -289,6 +289,33 @@ int xfe_recover_fixed_effects(const double* partial, int n, const xfe_fe_input* 
  * failure. */
 int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, unsigned char* artic);
 
+/* leave_out_connected_set (include/hdfe/akm_kss.hpp:55-90, impl
+ * src/akm_kss.cpp:2233-2290 over build_leave_out_sample :494-900): the rows
+ * of the largest worker-firm connected set (most firms, then most person-year
+ * rows, then the smallest original firm id) that stays connected when any
+ * one worker is left out: movers that are cut vertices of the mover-firm
+ * graph are dropped, with the largest set re-taken, until none is; then
+ * workers observed once leave.  keep (n) marks the kept input rows; fweights
+ * (optional, n) are frequency weights.  The match-key sort
+ * (xfe_akm_stable_sort_u64) and the cut-vertex search
+ * (xfe_articulation_points) run on the device, the bookkeeping on the host.
+ * Returns 1 on success, 0 on invalid input or a device failure. */
+typedef struct {
+    long long n_obs_input;
+    long long n_obs_connected;  /* rows of the largest connected set before pruning */
+    long long n_obs;            /* rows of the leave-out set */
+    int n_workers;
+    int n_firms;
+    long long n_matches;
+    int n_movers;
+    int n_stayers;
+    int prune_iterations;
+    int gpu_used;               /* the match-key sort ran on the device */
+} xfe_leave_out_result;
+
+int xfe_akm_leave_out_set(const int* worker_ids, const int* firm_ids, int n, const long long* fweights,
+                          unsigned char* keep, xfe_leave_out_result* out);
+
 /* ---- AKM/KSS leverage solver (include/hdfe/akm_kss_cuda.hpp:19-74, impl
  * src/akm_kss_cuda.cu:164-722): the grounded worker-partialled firm Laplacian
  * S = Df - B' Dw^{-1} B over n_firms levels (ground: the last firm), B the

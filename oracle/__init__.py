@@ -50,6 +50,8 @@ import ctypes
 import os
 
 import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.csgraph  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _lib = None
@@ -782,6 +784,104 @@ def articulation_points(adj_ptr, adj):
         if child[root] >= 2:
             artic[root] = 1
     return artic
+
+
+def ref_leave_out_set(worker, firm):
+    """The reference's leave_out_connected_set through oracle/_ref (checker):
+    (keep, info) with info = n_obs_input, n_obs_connected, n_obs, n_workers,
+    n_firms, n_matches, n_movers, n_stayers, prune_iterations."""
+    r = ref()
+    if r is None:
+        return None
+    n = len(worker)
+    w = np.ascontiguousarray(worker, dtype=np.int32)
+    f = np.ascontiguousarray(firm, dtype=np.int32)
+    keep = np.zeros(max(n, 1), dtype=np.uint8)
+    info = np.zeros(9, dtype=np.int64)
+    r.ref_leave_out_set.argtypes = [_i, _p, _p, _p, _p]
+    if r.ref_leave_out_set(n, _ptr(w), _ptr(f), _ptr(keep), _ptr(info)) != 0:
+        raise RuntimeError("ref_leave_out_set failed")
+    return keep[:n].astype(bool), info
+
+
+def leave_out_set(worker, firm, fweights=None):
+    """build_leave_out_sample with pruning (src/akm_kss.cpp:494-900): dense
+    labels in first-appearance order, the (worker, firm) match table, the
+    largest connected set (most firms, then rows, then smallest original
+    firm id), rounds that drop movers which are cut vertices of the
+    mover-firm graph and re-take the largest set, then workers observed once
+    leave.  Returns (keep per row, info dict)."""
+    worker = np.asarray(worker)
+    firm = np.asarray(firm)
+    n = worker.size
+    w0 = np.unique(worker, return_index=True)
+    _, first_w = np.unique(worker, return_index=True)
+    wl = np.empty(n, dtype=np.int64)
+    ordw = np.argsort(first_w)
+    uw = np.unique(worker)
+    rank_w = np.empty(uw.size, dtype=np.int64)
+    rank_w[ordw] = np.arange(uw.size)
+    wl = rank_w[np.searchsorted(uw, worker)]
+    uf_, first_f = np.unique(firm, return_index=True)
+    rank_f = np.empty(uf_.size, dtype=np.int64)
+    rank_f[np.argsort(first_f)] = np.arange(uf_.size)
+    fl = rank_f[np.searchsorted(uf_, firm)]
+    W0, F0 = uw.size, uf_.size
+    cnt = np.ones(n, dtype=np.int64) if fweights is None else np.asarray(fweights, dtype=np.int64)
+    key = wl * F0 + fl
+    mkey, row_match = np.unique(key, return_inverse=True)
+    mw, mf = mkey // F0, mkey % F0
+    mc = np.bincount(row_match, weights=cnt, minlength=mkey.size).astype(np.int64)
+    M = mkey.size
+    active = np.ones(M, dtype=bool)
+    firm_orig = np.empty(F0, dtype=np.int64)
+    firm_orig[fl] = firm
+
+    def largest_cc():
+        a = np.flatnonzero(active)
+        g = sp.coo_matrix((np.ones(a.size), (mw[a], W0 + mf[a])), shape=(W0 + F0, W0 + F0))
+        _, lab = sp.csgraph.connected_components(g, directed=False)
+        obs = np.bincount(lab[mw[a]], weights=mc[a], minlength=W0 + F0)
+        fs = np.unique(mf[a])
+        firms = np.bincount(lab[W0 + fs], minlength=W0 + F0)
+        minf = np.full(W0 + F0, np.iinfo(np.int64).max)
+        np.minimum.at(minf, lab[W0 + fs], firm_orig[fs])
+        cand = np.flatnonzero((firms > 0) | (obs > 0))
+        best = min(cand, key=lambda r: (-firms[r], -obs[r], minf[r]))
+        active[active] = lab[mw[active]] == best
+
+    largest_cc()
+    n_conn = int(mc[active].sum())
+    rounds = 0
+    while True:
+        deg = np.bincount(mw[active], minlength=W0)
+        sel = active & (deg[mw] >= 2)
+        if not sel.any():
+            break
+        movers = np.unique(mw[sel])
+        firms_ = np.unique(mf[sel])
+        mi = np.full(W0, -1)
+        mi[movers] = np.arange(movers.size)
+        fi = np.full(F0, -1)
+        fi[firms_] = np.arange(firms_.size)
+        ptr, adj = graph_csr(movers.size + firms_.size,
+                             np.column_stack([mi[mw[sel]], movers.size + fi[mf[sel]]]))
+        art = articulation_points(ptr, adj)
+        bad = np.zeros(W0, dtype=bool)
+        bad[movers[art[:movers.size] == 1]] = True
+        if not bad.any():
+            break
+        active &= ~bad[mw]
+        largest_cc()
+        rounds += 1
+    wobs = np.bincount(mw[active], weights=mc[active], minlength=W0)
+    active &= ~(wobs[mw] == 1)
+    keep = active[row_match]
+    wdeg = np.bincount(mw[active], minlength=W0)
+    info = dict(n_obs_input=n, n_obs_connected=n_conn, n_obs=int(cnt[keep].sum()), n_workers=int((wdeg > 0).sum()),
+                n_firms=int(np.unique(mf[active]).size), n_matches=int(active.sum()), n_movers=int((wdeg >= 2).sum()),
+                n_stayers=int((wdeg == 1).sum()), prune_iterations=rounds)
+    return keep, info
 
 
 def graph_csr(n, edges):

@@ -353,4 +353,25 @@ int ref_absorb_gi(int n, int k, int d, const double* y, const double* X, const i
     }
 }
 
+// hdfe::akm::leave_out_connected_set (src/akm_kss.cpp:2233): keep (n) and
+// info = n_obs_input, n_obs_connected, n_obs, n_workers, n_firms, n_matches,
+// n_movers, n_stayers, prune_iterations.  Returns 0 on success.
+int ref_leave_out_set(int n, const int* worker, const int* firm, unsigned char* keep, long long* info) {
+    try {
+        Eigen::VectorXi wv = Eigen::Map<const Eigen::VectorXi>(worker, n);
+        Eigen::VectorXi fv = Eigen::Map<const Eigen::VectorXi>(firm, n);
+        hdfe::akm::LeaveOutSetOptions o;
+        o.num_threads = 1;
+        auto r = hdfe::akm::leave_out_connected_set(wv, fv, nullptr, o);
+        for (int i = 0; i < n; ++i) keep[i] = r.keep[i];
+        const long long v[9] = {r.n_obs_input, r.n_obs_connected, r.n_obs, r.n_workers, r.n_firms,
+                                r.n_matches, r.n_movers, r.n_stayers, r.prune_iterations};
+        for (int k = 0; k < 9; ++k) info[k] = v[k];
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_leave_out_set: %s\n", e.what());
+        return -1;
+    }
+}
+
 }  // extern "C"

@@ -145,3 +145,18 @@ def gi_panel(seed=20261203 + 500, n=3000, individuals=800, years=20, classes=40)
 def gi_weights(n):
     """Deterministic row weights for the weighted group()/individual() case."""
     return 0.5 + (np.arange(n) % 5) / 3.0
+
+
+def leave_out_panels():
+    """Panels for the leave-out connected set: the KSS shape at small scale
+    with low mobility (many cut-vertex movers, several pruning rounds), the
+    same with moderate mobility, and permuted firm ids with gaps."""
+    out = {}
+    for name, seed, workers, firms, p_move in (("low_mobility", 311, 3000, 400, 0.02),
+                                               ("moderate", 312, 4000, 300, 0.08)):
+        p = panels.kss_panel(seed=panels.SEED0 + seed, workers=workers, firms=firms, p_move=p_move)
+        out[name] = (p["worker"].astype(np.int32), p["firm"].astype(np.int32))
+    w, f = out["moderate"]
+    perm = np.random.default_rng(5).permutation(300) * 7 + 11
+    out["gapped_ids"] = (w * 3 + 2, perm[f].astype(np.int32))
+    return out

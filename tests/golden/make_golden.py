@@ -20,7 +20,7 @@ sys.path.insert(0, os.path.join(ROOT, "xhdfe-xfe_amd"))
 
 import oracle  # noqa: E402
 from golden_cases import (CASES, JLA_DRAWS, JLA_SEED, SLOPE_CASES, certificate_weights,  # noqa: E402
-                          gi_panel, gi_weights, input_checksum, jla_panel, schwarz_edge_cases, slope_values)
+                          gi_panel, gi_weights, input_checksum, jla_panel, leave_out_panels, schwarz_edge_cases, slope_values)
 
 
 def main():
@@ -108,6 +108,11 @@ def main():
                                                 method=method)
         out[f"gi/{tag}/y_tilde"], out[f"gi/{tag}/X_tilde"] = yo, Xo
         out[f"gi/{tag}/info"] = np.array([it, conv], dtype=np.int64)
+    # the leave-out connected set (leave_out_connected_set)
+    for name, (w, f) in leave_out_panels().items():
+        keep, info = oracle.ref_leave_out_set(w, f)
+        out[f"los/{name}/keep"] = keep
+        out[f"los/{name}/info"] = info
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_cases.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -921,3 +921,40 @@ def test_articulation_points_rejects_bad_graph():
     adj = np.array([1, 2], dtype=np.int32)  # vertex 2 does not exist
     with pytest.raises(xfe_amd.XfeError):
         xfe_amd.articulation_points(ptr, adj)
+
+
+_LOS_KEYS = ("n_obs_input", "n_obs_connected", "n_obs", "n_workers", "n_firms", "n_matches", "n_movers",
+             "n_stayers", "prune_iterations")
+
+
+@pytest.mark.parametrize("name", ["low_mobility", "moderate", "gapped_ids"])
+def test_leave_out_set_matches_reference(golden, name):
+    """xfe_akm_leave_out_set equals the reference's leave_out_connected_set
+    (fixtures): the same rows, counts and pruning rounds."""
+    from golden_cases import leave_out_panels
+    _log(f"leave-out set {name}")
+    w, f = leave_out_panels()[name]
+    keep, info = xfe_amd.leave_out_set(w, f)
+    np.testing.assert_array_equal(keep, golden[f"los/{name}/keep"].astype(bool))
+    assert [info[k] for k in _LOS_KEYS] == golden[f"los/{name}/info"].tolist()
+    assert info["gpu_used"] == 1
+
+
+def test_leave_out_set_kss_scaled():
+    """A 5 % KSS-shape panel (60k workers, 7.5k firms, 0.5M rows, three
+    pruning rounds): the device result equals the oracle's, also with
+    frequency weights."""
+    _log("leave-out set kss 5%")
+    p = panels.kss_panel(workers=60000, firms=7500)
+    keep, info = xfe_amd.leave_out_set(p["worker"], p["firm"])
+    k2, i2 = oracle.leave_out_set(p["worker"], p["firm"])
+    np.testing.assert_array_equal(keep, k2)
+    assert [info[k] for k in _LOS_KEYS] == [i2[k] for k in _LOS_KEYS]
+    _log(f"  rows {info['n_obs']} of {info['n_obs_input']}, rounds {info['prune_iterations']}")
+    # frequency weights: a worker's weighted count decides the once-observed
+    # filter, as in the reference
+    fw = 1 + (np.arange(p["worker"].size) % 3).astype(np.int64)
+    k3, i3 = xfe_amd.leave_out_set(p["worker"], p["firm"], fweights=fw)
+    k4, i4 = oracle.leave_out_set(p["worker"], p["firm"], fweights=fw)
+    np.testing.assert_array_equal(k3, k4)
+    assert [i3[k] for k in _LOS_KEYS] == [i4[k] for k in _LOS_KEYS]

@@ -851,3 +851,19 @@ def test_articulation_oracle_is_brute_force():
     for n, edges in _random_graphs(7, 400, 22):
         ptr, adj = oracle.graph_csr(n, edges)
         np.testing.assert_array_equal(oracle.articulation_points(ptr, adj), _brute_cut_vertices(n, ptr, adj))
+
+
+_LOS_KEYS = ("n_obs_input", "n_obs_connected", "n_obs", "n_workers", "n_firms", "n_matches", "n_movers",
+             "n_stayers", "prune_iterations")
+
+
+@pytest.mark.parametrize("name", ["low_mobility", "moderate", "gapped_ids"])
+def test_leave_out_set_matches_reference(golden, name):
+    """The oracle's leave-out connected set equals the reference's
+    leave_out_connected_set row for row, with the same counts and pruning
+    rounds (fixtures)."""
+    from golden_cases import leave_out_panels
+    w, f = leave_out_panels()[name]
+    keep, info = oracle.leave_out_set(w, f)
+    np.testing.assert_array_equal(keep, golden[f"los/{name}/keep"].astype(bool))
+    assert [info[k] for k in _LOS_KEYS] == golden[f"los/{name}/info"].tolist()

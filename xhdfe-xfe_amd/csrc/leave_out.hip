@@ -342,3 +342,197 @@ int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, 
 }
 
 }  // extern "C"
+
+namespace {
+
+struct UnionFindHost {
+    std::vector<int> p;
+    explicit UnionFindHost(int n) : p(n) {
+        for (int i = 0; i < n; ++i) p[i] = i;
+    }
+    int find(int x) {
+        while (p[x] != x) {
+            p[x] = p[p[x]];
+            x = p[x];
+        }
+        return x;
+    }
+    void unite(int a, int b) {
+        a = find(a);
+        b = find(b);
+        if (a != b) p[std::max(a, b)] = std::min(a, b);
+    }
+};
+
+// dense labels in first-appearance row order (relabel_dense, src/akm_kss.cpp:383)
+int dense_first_appearance(const int* ids, int n, std::vector<int>& out) {
+    out.resize(n);
+    if (n == 0) return 0;
+    const int mx = *std::max_element(ids, ids + n);
+    std::vector<int> lut(static_cast<size_t>(mx) + 1, -1);
+    int next = 0;
+    for (int i = 0; i < n; ++i) {
+        int& slot = lut[ids[i]];
+        if (slot < 0) slot = next++;
+        out[i] = slot;
+    }
+    return next;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xfe_akm_leave_out_set(const int* worker_ids, const int* firm_ids, int n, const long long* fweights,
+                          unsigned char* keep, xfe_leave_out_result* out) {
+    clear_error();
+    if (n < 0 || !out || (n > 0 && (!worker_ids || !firm_ids || !keep))) {
+        set_error("xfe_akm_leave_out_set: invalid arguments");
+        return 0;
+    }
+    std::memset(out, 0, sizeof(*out));
+    out->n_obs_input = n;
+    if (n == 0) return 1;
+    for (int i = 0; i < n; ++i)
+        if (worker_ids[i] < 0 || firm_ids[i] < 0 || (fweights && fweights[i] < 0)) {
+            set_error("xfe_akm_leave_out_set: ids and frequency weights must be non-negative");
+            return 0;
+        }
+    try {
+        std::vector<int> w0, f0;
+        const int W0 = dense_first_appearance(worker_ids, n, w0);
+        const int F0 = dense_first_appearance(firm_ids, n, f0);
+        // match table: rows by (worker, firm), stable (the device sort of the match keys)
+        std::vector<uint64_t> key(n);
+        for (int i = 0; i < n; ++i)
+            key[i] = static_cast<uint64_t>(w0[i]) * static_cast<uint64_t>(F0) + static_cast<uint64_t>(f0[i]);
+        std::vector<int64_t> order(n);
+        if (!xfe_akm_stable_sort_u64(key.data(), order.data(), static_cast<size_t>(n)))
+            throw Error(std::string("match-key sort: ") + xfe_last_error());
+        out->gpu_used = 1;
+        std::vector<int> mw, mf, row_match(n);
+        std::vector<long long> mc;
+        for (int k = 0; k < n; ++k) {
+            const int row = static_cast<int>(order[k]);
+            if (mw.empty() || mw.back() != w0[row] || mf.back() != f0[row]) {
+                mw.push_back(w0[row]);
+                mf.push_back(f0[row]);
+                mc.push_back(0);
+            }
+            mc.back() += fweights ? fweights[row] : 1;
+            row_match[row] = static_cast<int>(mw.size()) - 1;
+        }
+        const int M = static_cast<int>(mw.size());
+        std::vector<unsigned char> active(M, 1);
+        std::vector<int> firm_orig(F0);
+        for (int i = 0; i < n; ++i) firm_orig[f0[i]] = firm_ids[i];
+        // largest component: most firms, then most person-year rows, then the
+        // smallest original firm id (src/akm_kss.cpp:636-720)
+        auto largest_cc = [&]() {
+            UnionFindHost uf(W0 + F0);
+            for (int m = 0; m < M; ++m)
+                if (active[m]) uf.unite(mw[m], W0 + mf[m]);
+            std::vector<long long> obs(W0 + F0, 0);
+            std::vector<int> firms(W0 + F0, 0), minf(W0 + F0, INT_MAX);
+            std::vector<unsigned char> seen(F0, 0);
+            for (int m = 0; m < M; ++m) {
+                if (!active[m]) continue;
+                obs[uf.find(mw[m])] += mc[m];
+                if (!seen[mf[m]]) {
+                    seen[mf[m]] = 1;
+                    const int r = uf.find(W0 + mf[m]);
+                    ++firms[r];
+                    minf[r] = std::min(minf[r], firm_orig[mf[m]]);
+                }
+            }
+            int best = -1;
+            for (int r = 0; r < W0 + F0; ++r) {
+                if (firms[r] == 0 && obs[r] == 0) continue;
+                if (best < 0 || firms[r] > firms[best] ||
+                    (firms[r] == firms[best] && (obs[r] > obs[best] || (obs[r] == obs[best] && minf[r] < minf[best]))))
+                    best = r;
+            }
+            if (best < 0) return;
+            for (int m = 0; m < M; ++m)
+                if (active[m] && uf.find(mw[m]) != best) active[m] = 0;
+        };
+        largest_cc();
+        for (int m = 0; m < M; ++m)
+            if (active[m]) out->n_obs_connected += mc[m];
+        // pruning: drop movers that are cut vertices of the mover-firm graph
+        // until none is (src/akm_kss.cpp:736-870), the cut-vertex search on the device
+        std::vector<int> deg(W0), mover_idx(W0), firm_idx(F0);
+        while (true) {
+            std::fill(deg.begin(), deg.end(), 0);
+            for (int m = 0; m < M; ++m)
+                if (active[m]) ++deg[mw[m]];
+            std::fill(mover_idx.begin(), mover_idx.end(), -1);
+            std::fill(firm_idx.begin(), firm_idx.end(), -1);
+            int nmov = 0, nfrm = 0;
+            long long nedges = 0;
+            for (int m = 0; m < M; ++m) {
+                if (!active[m] || deg[mw[m]] < 2) continue;
+                ++nedges;
+                if (mover_idx[mw[m]] < 0) mover_idx[mw[m]] = nmov++;
+                if (firm_idx[mf[m]] < 0) firm_idx[mf[m]] = nfrm++;
+            }
+            if (nmov == 0) break;
+            const int nv = nmov + nfrm;
+            std::vector<int> ptr(nv + 1, 0), adj(2 * nedges);
+            for (int m = 0; m < M; ++m) {
+                if (!active[m] || deg[mw[m]] < 2) continue;
+                ++ptr[mover_idx[mw[m]] + 1];
+                ++ptr[nmov + firm_idx[mf[m]] + 1];
+            }
+            for (int v = 0; v < nv; ++v) ptr[v + 1] += ptr[v];
+            std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+            for (int m = 0; m < M; ++m) {
+                if (!active[m] || deg[mw[m]] < 2) continue;
+                const int a = mover_idx[mw[m]], b = nmov + firm_idx[mf[m]];
+                adj[fill[a]++] = b;
+                adj[fill[b]++] = a;
+            }
+            std::vector<unsigned char> art(nv);
+            if (!xfe_articulation_points(nv, ptr.data(), adj.data(), art.data()))
+                throw Error(std::string("cut-vertex search: ") + xfe_last_error());
+            bool any = false;
+            std::vector<unsigned char> bad(W0, 0);
+            for (int w = 0; w < W0; ++w)
+                if (mover_idx[w] >= 0 && art[mover_idx[w]]) bad[w] = 1, any = true;
+            if (!any) break;
+            for (int m = 0; m < M; ++m)
+                if (active[m] && bad[mw[m]]) active[m] = 0;
+            largest_cc();
+            ++out->prune_iterations;
+        }
+        // workers observed once leave (src/akm_kss.cpp:858-869)
+        std::vector<long long> wobs(W0, 0);
+        for (int m = 0; m < M; ++m)
+            if (active[m]) wobs[mw[m]] += mc[m];
+        for (int m = 0; m < M; ++m)
+            if (active[m] && wobs[mw[m]] == 1) active[m] = 0;
+        std::vector<int> wdeg(W0, 0);
+        std::vector<unsigned char> fseen(F0, 0);
+        for (int m = 0; m < M; ++m) {
+            if (!active[m]) continue;
+            ++out->n_matches;
+            ++wdeg[mw[m]];
+            if (!fseen[mf[m]]) fseen[mf[m]] = 1, ++out->n_firms;
+        }
+        for (int w = 0; w < W0; ++w) {
+            if (wdeg[w] > 0) ++out->n_workers;
+            if (wdeg[w] >= 2) ++out->n_movers;
+            if (wdeg[w] == 1) ++out->n_stayers;
+        }
+        for (int i = 0; i < n; ++i) {
+            keep[i] = active[row_match[i]] ? 1 : 0;
+            if (keep[i]) out->n_obs += fweights ? fweights[i] : 1;
+        }
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+}  // extern "C"

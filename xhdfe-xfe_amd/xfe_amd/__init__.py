@@ -72,7 +72,7 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
            "xfe_absorb_group_individual",
            "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort",
-           "xfe_articulation_points")
+           "xfe_articulation_points", "xfe_akm_leave_out_set")
 
 _lib = None
 
@@ -218,6 +218,24 @@ def articulation_points(adj_ptr, adj):
     if not lib().xfe_articulation_points(_i(n), _ptr(ptr), _ptr(a) if a.size else None, _ptr(out)):
         raise XfeError(last_error())
     return out[:n]
+
+
+class LeaveOutResult(ctypes.Structure):
+    _fields_ = [("n_obs_input", ctypes.c_longlong), ("n_obs_connected", ctypes.c_longlong),
+                ("n_obs", ctypes.c_longlong), ("n_workers", _i), ("n_firms", _i), ("n_matches", ctypes.c_longlong),
+                ("n_movers", _i), ("n_stayers", _i), ("prune_iterations", _i), ("gpu_used", _i)]
+
+
+def leave_out_set(worker_ids, firm_ids, fweights=None):
+    """leave_out_connected_set: (keep mask per input row, info dict)."""
+    w = np.ascontiguousarray(worker_ids, dtype=np.int32)
+    f = np.ascontiguousarray(firm_ids, dtype=np.int32)
+    fw = None if fweights is None else np.ascontiguousarray(fweights, dtype=np.int64)
+    keep = np.zeros(max(w.size, 1), dtype=np.uint8)
+    r = LeaveOutResult()
+    if not lib().xfe_akm_leave_out_set(_ptr(w), _ptr(f), _i(w.size), _ptr(fw), _ptr(keep), ctypes.byref(r)):
+        raise XfeError(last_error())
+    return keep[:w.size].astype(bool), {name: getattr(r, name) for name, _ in LeaveOutResult._fields_}
 
 
 VERDICTS = ("PASS", "FAIL", "INDETERMINATE", "ERROR")
