@@ -371,6 +371,54 @@ def bench_gi(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_leaveout(args):
+    """The leave-out connected set (xfe_akm_leave_out_set) at the KSS shape
+    (1.2M workers, 150k firms, about 10M rows; --scale shrinks it): the host
+    entry point timed; the reference's own leave_out_connected_set (compiled
+    from its sources into the checker library, one thread) beside it when
+    that library is present."""
+    import torch  # noqa: F401  (shares torch's HIP runtime)
+
+    import xfe_amd
+    from xfe_amd import panels
+
+    if not xfe_amd.backend_available():
+        raise SystemExit("no HIP device")
+    sc = min(args.scale, 1.0)
+    p = panels.kss_panel(workers=int(1_200_000 * sc), firms=int(150_000 * sc))
+    w, f = p["worker"].astype(np.int32), p["firm"].astype(np.int32)
+    log(f"kss panel: n={w.size}")
+    for k in range(args.warmup):
+        xfe_amd.leave_out_set(w, f)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        keep, info = xfe_amd.leave_out_set(w, f)
+    sec = (time.perf_counter() - t0) / args.steps
+    log(f"{sec:.3f} s per leave-out set, {info}")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        import oracle
+        if oracle.ref() is not None:
+            t0 = time.perf_counter()
+            rk, ri = oracle.ref_leave_out_set(w, f)
+            ref_s = time.perf_counter() - t0
+            log(f"reference leave_out_connected_set: {ref_s:.3f} s, same rows: {bool((rk == keep).all())}")
+            cpu = {"value": round(ref_s, 3), "unit": "s", "cores": 1, "kind": "reference",
+                   "sample": "the whole panel", "same_rows": bool((rk == keep).all())}
+    out = {
+        "metric": "s to build the KSS leave-out connected set (10M rows)", "value": round(sec, 4), "unit": "s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "method": "match-key sort and Tarjan-Vishkin cut-vertex search on the device, host bookkeeping",
+        "config": {"workload": f"kss_{w.size}rows_leave_out", "rows": int(w.size)},
+        "result": {k: info[k] for k in ("n_obs_connected", "n_obs", "n_workers", "n_firms", "n_movers",
+                                        "prune_iterations")},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,10 +428,10 @@ def main():
     ap.add_argument("--precision", type=int, default=1, help="1 FP32 under FP64 refinement, 0 FP64")
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain FP64 absorber")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--path", choices=("absorb", "akm", "slope", "recover", "gi"), default="absorb",
+    ap.add_argument("--path", choices=("absorb", "akm", "slope", "recover", "gi", "leaveout"), default="absorb",
                     help="absorb: the 56M-obs absorber (default); akm: the KSS leverage solver; "
                          "slope: the heterogeneous-slope path; recover: savefe FE recovery; "
-                         "gi: the group()/individual() absorber")
+                         "gi: the group()/individual() absorber; leaveout: the KSS leave-out connected set")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -400,6 +448,9 @@ def main():
         return
     if args.path == "gi":
         bench_gi(args)
+        return
+    if args.path == "leaveout":
+        bench_leaveout(args)
         return
 
     import torch
