@@ -43,18 +43,37 @@ int lo_grid(long long work) {
 }
 
 #define LO_FOR(v) for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+// one wavefront per vertex (the loop variable is uniform across the wave), lane = lane index
+#define LO_FOR_WAVE(v)                                                                     \
+    for (int v = blockIdx.x * (blockDim.x / kWave) + static_cast<int>(threadIdx.x) / kWave; v < n; \
+         v += gridDim.x * (blockDim.x / kWave))
+
+__device__ __forceinline__ int wave_min_int(int m) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off, kWave));
+    return m;
+}
+
+__device__ __forceinline__ int wave_or_int(int m) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) m |= __shfl_xor(m, off, kWave);
+    return m;
+}
 
 __global__ void lo_iota_kernel(int* label, int n) {
     LO_FOR(v) label[v] = v;
 }
 
 // label[v] = min(label[v], label of every neighbour); *changed when lowered
+// (a wavefront per vertex: high-degree vertices do not serialize a launch)
 __global__ void lo_prop_kernel(const int* __restrict__ ptr, const int* __restrict__ adj, int* label, int n,
                                int* changed) {
-    LO_FOR(v) {
+    const int lane = threadIdx.x & (kWave - 1);
+    LO_FOR_WAVE(v) {
         int m = label[v];
-        for (int q = ptr[v]; q < ptr[v + 1]; ++q) m = min(m, label[adj[q]]);
-        if (m < label[v]) {
+        for (int q = ptr[v] + lane; q < ptr[v + 1]; q += kWave) m = min(m, label[adj[q]]);
+        m = wave_min_int(m);
+        if (lane == 0 && m < label[v]) {
             atomicMin(&label[v], m);
             *changed = 1;
         }
@@ -75,14 +94,19 @@ __global__ void lo_roots_kernel(const int* __restrict__ label, int* depth, int n
 }
 
 // one BFS level: neighbours of depth-d vertices without a depth get d + 1
+// (a wavefront per vertex)
 __global__ void lo_bfs_kernel(const int* __restrict__ ptr, const int* __restrict__ adj, int* depth, int n, int d,
                               int* changed) {
-    LO_FOR(v) {
+    const int lane = threadIdx.x & (kWave - 1);
+    LO_FOR_WAVE(v) {
         if (depth[v] != d) continue;
-        for (int q = ptr[v]; q < ptr[v + 1]; ++q) {
+        int found = 0;
+        for (int q = ptr[v] + lane; q < ptr[v + 1]; q += kWave) {
             const int x = adj[q];
-            if (depth[x] == -1 && atomicCAS(&depth[x], -1, d + 1) == -1) *changed = 1;
+            if (depth[x] == -1 && atomicCAS(&depth[x], -1, d + 1) == -1) found = 1;
         }
+        found = wave_or_int(found);
+        if (lane == 0 && found) *changed = 1;
     }
 }
 
@@ -171,14 +195,16 @@ __device__ __forceinline__ bool lo_rule2(const Tree& t, int w) {
     return v >= 0 && t.depth[v] > 0 && (t.low[w] < t.pre[v] || t.high[w] >= t.pre[v] + t.size[v]);
 }
 
-// one propagation sweep over the tree-edge classes (labels named by vertex)
+// one propagation sweep over the tree-edge classes (labels named by vertex;
+// a wavefront per vertex, the lanes over its neighbours)
 __global__ void lo_class_kernel(Tree t, int* cls, int n, int* changed) {
-    LO_FOR(u) {
+    const int lane = threadIdx.x & (kWave - 1);
+    LO_FOR_WAVE(u) {
         if (t.depth[u] <= 0) continue;
         int m = cls[u];
         const int pu = t.parent[u];
-        if (lo_rule2(t, u)) m = min(m, cls[pu]);
-        for (int q = t.ptr[u]; q < t.ptr[u + 1]; ++q) {
+        if (lane == 0 && lo_rule2(t, u)) m = min(m, cls[pu]);
+        for (int q = t.ptr[u] + lane; q < t.ptr[u + 1]; q += kWave) {
             const int w = t.adj[q];
             if (t.parent[w] == u) {  // child: rule 2 seen from the parent's edge
                 if (lo_rule2(t, w)) m = min(m, cls[w]);
@@ -189,7 +215,8 @@ __global__ void lo_class_kernel(Tree t, int* cls, int n, int* changed) {
                                  (t.pre[w] <= t.pre[u] && t.pre[u] < t.pre[w] + t.size[w]);
             if (!related) m = min(m, cls[w]);  // rule 1
         }
-        if (m < cls[u]) {
+        m = wave_min_int(m);
+        if (lane == 0 && m < cls[u]) {
             atomicMin(&cls[u], m);
             *changed = 1;
         }
@@ -215,6 +242,7 @@ __global__ void lo_test_kernel(const int* __restrict__ depth, const int* __restr
 }
 
 #undef LO_FOR
+#undef LO_FOR_WAVE
 
 }  // namespace
 }  // namespace xfe
@@ -268,6 +296,7 @@ int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, 
         XFE_CHECK(hipMemcpyAsync(d_ptr.p, adj_ptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
         if (E > 0) XFE_CHECK(hipMemcpyAsync(d_adj.p, adj, sizeof(int) * E, hipMemcpyHostToDevice, s));
         const int g = lo_grid(n);
+        const int gw = lo_grid(static_cast<long long>(n) * kWave);  // one wavefront per vertex
         int changed = 0;
         // every sweep lowers some label or is the last one: at most n sweeps
         auto until_stable = [&](auto&& sweep) {
@@ -284,7 +313,7 @@ int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, 
         // 1. components
         hipLaunchKernelGGL(lo_iota_kernel, dim3(g), dim3(kLoBlock), 0, s, label.p, n);
         until_stable([&] {
-            hipLaunchKernelGGL(lo_prop_kernel, dim3(g), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, label.p, n, flag.p);
+            hipLaunchKernelGGL(lo_prop_kernel, dim3(gw), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, label.p, n, flag.p);
             hipLaunchKernelGGL(lo_jump_kernel, dim3(g), dim3(kLoBlock), 0, s, label.p, n);
         });
         // 2. BFS forest from the component roots; maxd ends as the deepest level
@@ -292,7 +321,7 @@ int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, 
         int maxd = 0;
         for (; maxd < n; ++maxd) {
             XFE_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), s));
-            hipLaunchKernelGGL(lo_bfs_kernel, dim3(g), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, depth.p, n, maxd,
+            hipLaunchKernelGGL(lo_bfs_kernel, dim3(gw), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, depth.p, n, maxd,
                                flag.p);
             XFE_CHECK(hipGetLastError());
             XFE_CHECK(hipMemcpyAsync(&changed, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -318,7 +347,7 @@ int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, 
         Tree t{d_ptr.p, d_adj.p, depth.p, parent.p, pre.p, size.p, low.p, high.p};
         hipLaunchKernelGGL(lo_iota_kernel, dim3(g), dim3(kLoBlock), 0, s, cls.p, n);
         until_stable([&] {
-            hipLaunchKernelGGL(lo_class_kernel, dim3(g), dim3(kLoBlock), 0, s, t, cls.p, n, flag.p);
+            hipLaunchKernelGGL(lo_class_kernel, dim3(gw), dim3(kLoBlock), 0, s, t, cls.p, n, flag.p);
             hipLaunchKernelGGL(lo_jump_kernel, dim3(g), dim3(kLoBlock), 0, s, cls.p, n);
         });
         // 6. two or more classes among a vertex's tree edges
