@@ -133,20 +133,31 @@ __global__ void lo_size_kernel(const int* __restrict__ depth, const int* __restr
 }
 
 // top-down: the children of each depth-d vertex p get consecutive intervals
-// after pre[p], in adjacency order
+// after pre[p], in adjacency order (a wavefront per vertex: each chunk of 64
+// neighbours takes an exclusive prefix sum of its children's sizes)
 __global__ void lo_pre_kernel(const int* __restrict__ ptr, const int* __restrict__ adj,
                               const int* __restrict__ depth, const int* __restrict__ parent,
                               const int* __restrict__ size, int* pre, int n, int d) {
-    LO_FOR(p) {
+    const int lane = threadIdx.x & (kWave - 1);
+    LO_FOR_WAVE(p) {
         if (depth[p] != d) continue;
-        if (d == 0) pre[p] = 0;
-        int next = pre[p] + 1;
-        for (int q = ptr[p]; q < ptr[p + 1]; ++q) {
-            const int x = adj[q];
-            if (parent[x] == p) {
-                pre[x] = next;
-                next += size[x];
+        if (d == 0 && lane == 0) pre[p] = 0;
+        int next = (d == 0 ? 0 : pre[p]) + 1;
+        for (int q0 = ptr[p]; q0 < ptr[p + 1]; q0 += kWave) {
+            const int q = q0 + lane;
+            int x = -1, sz = 0;
+            if (q < ptr[p + 1]) {
+                x = adj[q];
+                if (parent[x] == p) sz = size[x];
             }
+            int incl = sz;  // inclusive scan over the lanes
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const int t = __shfl_up(incl, off, kWave);
+                if (lane >= off) incl += t;
+            }
+            if (sz > 0) pre[x] = next + incl - sz;
+            next += __shfl(incl, kWave - 1, kWave);
         }
     }
 }
@@ -334,7 +345,7 @@ int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, 
         for (int d = maxd; d >= 1; --d)
             hipLaunchKernelGGL(lo_size_kernel, dim3(g), dim3(kLoBlock), 0, s, depth.p, parent.p, size.p, n, d);
         for (int d = 0; d <= maxd; ++d)
-            hipLaunchKernelGGL(lo_pre_kernel, dim3(g), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, depth.p, parent.p,
+            hipLaunchKernelGGL(lo_pre_kernel, dim3(gw), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, depth.p, parent.p,
                                size.p, pre.p, n, d);
         // 4. low / high
         hipLaunchKernelGGL(lo_local_kernel, dim3(g), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, parent.p, pre.p, low.p,
