@@ -46,13 +46,19 @@ typedef struct {
     double tol;                 /* HdfeOptions::tol (default 1e-8) */
     int tolerance_mode;         /* ToleranceMode: 0 xhdfe-fast, 1 reghdfe-comparable, 2 strict-residual */
     int max_iter;               /* HdfeOptions::max_iter */
-    int method;                 /* AbsorptionMethod: 0 Auto, 1 GaussSeidel, 2 SymmetricGaussSeidel */
+    int method;                 /* AbsorptionMethod (include/hdfe/hdfe_regressor.hpp:18): 0 Auto, 1 GaussSeidel,
+                                   2 SymmetricGaussSeidel; 4 Schwarz and 7 AutoMlsmr sweep as GaussSeidel, as the
+                                   reference's device absorber does for every method but Jacobi
+                                   (src/fe_absorption_cuda.cu:3383); 3 Jacobi (never sent to the device by the
+                                   reference's dispatcher, src/fe_absorption.cpp:9141), 5 Lsmr and 6 Mlsmr
+                                   (CPU-only there, :9127) and any other value are rejected: the call returns 0 */
     int force_cg;               /* explicit reghdfe convergence criterion -> CG accelerator */
     int precision;              /* 0: FP64 throughout (the plain method); 1: FP32 sweeps under FP64 refinement */
     int certify;                /* 1: compute the normal-equation certificate of the result on the device
                                    (certify_absorption_result, src/fe_absorption.cpp:14942-15190) */
     double refine_tol;          /* precision 1: FP64 refinement target, ||b - S beta|| / ||b|| of the
-                                   Schur system over the non-primary levels; <= 0 selects 1e-11 */
+                                   Schur system over the non-primary levels; <= 0 selects
+                                   min(1e-11, tol / 1000) */
 } xfe_options;
 
 /* Outputs of one absorption (the fields of hdfe::detail::AbsorptionResult,
@@ -75,6 +81,15 @@ typedef struct {
     double abs_residual_rel;
     int precision_certified;
     int certificate_computed;
+    /* which method ran: 0 the plain FP64 method, 1 the mixed-precision Schur PCG.  A request
+       for precision 1 that the mixed path cannot take (weights, slopes, 1 or more than 3
+       dimensions, more than 32 columns, no level-aligned tiles) runs the plain FP64 method and
+       reports 0 here with precision_fallback = 1. */
+    int precision_used;
+    int precision_fallback;
+    double refine_target;       /* precision 1: the FP64 refinement target that was applied */
+    int preconditioner_used;    /* precision 1: 0 Jacobi, 1 the learned two-level preconditioner */
+    int precond_fallback;       /* the learned preconditioner stalled and Jacobi-PCG finished the solve */
 } xfe_absorb_result;
 
 /* The certificate on its own (certify_cuda_absorption_result,
@@ -212,6 +227,30 @@ int xfe_absorb_device(double* d_V, int n, int C, int ld,
 int xfe_schwarz_demean(double* Y, double* X, int64_t n, int k, const int32_t* const* fe,
                        const int* nlev, int ndim, int threads, double tol, int max_iter,
                        int* iters_out, int collapse);
+
+/* build_indexer (src/fe_absorption.cpp:845-940) with finalize_contiguous_groups
+ * (:547-590), on the device: raw int32 ids (n) become compact 0-based ids in
+ * ascending raw-id order.  When max - min + 1 <= min(2n, 50M) the id is
+ * raw - min (unused slots kept, num_levels_present < num_groups) unless fewer
+ * than 90 % of the slots are used, then the used slots are ranked; otherwise
+ * the distinct ids are ranked.  contiguous = 1 iff the compact ids are
+ * non-decreasing over the rows (the reference's groups_contiguous).  Host
+ * buffers; returns 1 on success, 0 on failure. */
+int xfe_build_indexer(const int* raw, int n, int* group_ids, int* num_groups, int* num_levels_present,
+                      int* contiguous);
+
+/* The same on device buffers (d_raw, d_group_ids in HBM; the counts come back
+ * to host pointers).  No counterpart in the reference, whose indexer runs on
+ * the host; it serves callers that keep their panel on the device.  Waits for
+ * all work already queued on the device before reading d_raw. */
+int xfe_build_indexer_device(const int* d_raw, int n, int* d_group_ids, int* num_groups,
+                             int* num_levels_present, int* contiguous);
+
+/* compute_weight_sums (src/fe_absorption.cpp:942-1037): out[g] = sum of
+ * weights (or the count, weights NULL) over the rows with group_ids[i] = g,
+ * g < num_groups.  Ids outside [0, num_groups) are rejected.  Host buffers;
+ * returns 1 on success, 0 on failure. */
+int xfe_weight_sums(const int* group_ids, int n, int num_groups, const double* weights, double* out);
 
 /* compute_keep_mask_drop_singletons (src/hdfe_regressor_v11.cpp:4056-4302):
  * iterative singleton drop on the device, bit-exact.  fes: num_fes x n raw

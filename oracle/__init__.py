@@ -107,6 +107,16 @@ def build_indexer(raw):
     return gid, G.value, lp.value, bool(cont.value)
 
 
+def weight_sums(gid, G, weights=None):
+    """compute_weight_sums (src/fe_absorption.cpp:942-1037): level sums of the
+    weights, or counts."""
+    g = np.ascontiguousarray(gid, dtype=np.int32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    out = np.empty(int(G))
+    lib().or_weight_sums(_i(g.size), _ptr(g), _i(int(G)), _ptr(w), _ptr(out))
+    return out
+
+
 def absorb(V, gids, G, weights=None, tol=1e-8, tolerance_mode=1, max_iter=100000,
            symmetric=None, force_cg=False, threads=None, sweep_order=None):
     """Within transform of the n x C block V (column-major [y | X]).
@@ -1163,7 +1173,27 @@ def ref_absorb_slope(y, X, fes, slope_dim, z, intercept=True, tol=1e-8, toleranc
     return yo, Xo, it.value, bool(cv.value)
 
 
-def ref_absorb(y, X, fes, tol=1e-8, tolerance_mode=1, threads=1, symmetric=False):
+def ref_indexer(fes, weights=None):
+    """The reference's build_indexer and compute_weight_sums through oracle/_ref
+    (checker): a list of (gids, num_groups, levels_present, weight_sums) per
+    dimension."""
+    r = ref()
+    if r is None:
+        return None
+    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    D, n = ids.shape
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    gids = np.empty((D, n), dtype=np.int32)
+    G = np.zeros(D, dtype=np.int32)
+    lp = np.zeros(D, dtype=np.int32)
+    ws = np.zeros((D, max(n, 1)))
+    rc = r.ref_indexer(_i(n), _i(D), _ptr(ids), _ptr(w), _ptr(gids), _ptr(G), _ptr(lp), _ptr(ws))
+    if rc != 0:
+        raise RuntimeError("ref_indexer failed")
+    return [(gids[j].copy(), int(G[j]), int(lp[j]), ws[j, :G[j]].copy()) for j in range(D)]
+
+
+def ref_absorb(y, X, fes, tol=1e-8, tolerance_mode=1, threads=1, symmetric=False, weights=None):
     r = ref()
     if r is None:
         return None
@@ -1171,10 +1201,11 @@ def ref_absorb(y, X, fes, tol=1e-8, tolerance_mode=1, threads=1, symmetric=False
     X = np.asfortranarray(X, dtype=np.float64)
     n, k = X.shape
     ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
     yo = np.empty(n)
     Xo = np.empty((n, k), order="F")
     it, conv = _i(), _i()
-    rc = r.ref_absorb(_i(n), _i(k), _i(len(fes)), _ptr(y), _ptr(X), _ptr(ids), _d(tol),
+    rc = r.ref_absorb(_i(n), _i(k), _i(len(fes)), _ptr(y), _ptr(X), _ptr(ids), _ptr(w), _d(tol),
                       _i(tolerance_mode), _i(threads), _i(int(symmetric)), _ptr(yo), _ptr(Xo),
                       ctypes.byref(it), ctypes.byref(conv))
     if rc != 0:

@@ -48,7 +48,8 @@ extern "C" {
 // Raw CPU absorber on already-filtered rows (no singleton drop).
 // y: n, X: n*k column-major, fes: d*n (dimension-major). Outputs overwrite
 // y_out/X_out.  Returns 0 on success, -1 on exception.
-int ref_absorb(int n, int k, int d, const double* y, const double* X, const int* fes,
+// w: optional analytic weights (n), nullptr for unit weights.
+int ref_absorb(int n, int k, int d, const double* y, const double* X, const int* fes, const double* w,
                double tol, int tolerance_mode, int threads, int symmetric,
                double* y_out, double* X_out, int* iterations, int* converged) {
     try {
@@ -57,7 +58,9 @@ int ref_absorb(int n, int k, int d, const double* y, const double* X, const int*
         auto fv = make_fes(fes, n, d);
         hdfe::HdfeOptions o = make_options(tol, tolerance_mode, threads);
         o.symmetric_sweep = symmetric != 0;
-        auto r = hdfe::detail::absorb_fixed_effects(yv, Xm, fv, nullptr, o);
+        Eigen::VectorXd wv;
+        if (w) wv = Eigen::Map<const Eigen::VectorXd>(w, n);
+        auto r = hdfe::detail::absorb_fixed_effects(yv, Xm, fv, w ? &wv : nullptr, o);
         std::memcpy(y_out, r.y_tilde.data(), sizeof(double) * n);
         if (k > 0) std::memcpy(X_out, r.X_tilde.data(), sizeof(double) * n * k);
         *iterations = r.iterations;
@@ -65,6 +68,43 @@ int ref_absorb(int n, int k, int d, const double* y, const double* X, const int*
         return 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "ref_absorb: %s\n", e.what());
+        return -1;
+    }
+}
+
+// The reference's FE indexer and weight sums (build_indexer, src/fe_absorption.cpp:845-940;
+// compute_weight_sums, :942-1037), read back through absorb_fixed_effects_v6 with
+// retain_fixed_effects, which returns indexers[d].group_ids, the weight sums and
+// num_levels_present (src/fe_absorption.cpp:9444-9452; fe_levels :9043).  fes: d x n raw ids;
+// w optional.  Outputs: gids (d x n), num_groups (d), levels_present (d), and the weight sums
+// of dimension j at wsums + j * n (capacity n per dimension).  Returns 0, -1 on failure.
+int ref_indexer(int n, int d, const int* fes, const double* w, int* gids, int* num_groups,
+                int* levels_present, double* wsums) {
+    try {
+        Eigen::VectorXd y = Eigen::VectorXd::LinSpaced(n, 1.0, 2.0);
+        Eigen::MatrixXd X(n, 0);
+        auto fv = make_fes(fes, n, d);
+        hdfe::HdfeOptions o = make_options(1e-8, 1, 1);
+        o.retain_fixed_effects = true;
+        Eigen::VectorXd wv;
+        if (w) wv = Eigen::Map<const Eigen::VectorXd>(w, n);
+        auto r = hdfe::detail::absorb_fixed_effects_v6(y, X, fv, w ? &wv : nullptr, o,
+                                                       hdfe::AbsorptionMethod::Auto);
+        if (static_cast<int>(r.fe_group_ids.size()) != d || static_cast<int>(r.fe_weight_sums.size()) != d ||
+            static_cast<int>(r.fe_levels.size()) != d)
+            return -1;
+        for (int j = 0; j < d; ++j) {
+            if (static_cast<int>(r.fe_group_ids[j].size()) != n) return -1;
+            std::memcpy(gids + static_cast<std::size_t>(j) * n, r.fe_group_ids[j].data(), sizeof(int) * n);
+            const int G = static_cast<int>(r.fe_weight_sums[j].size());
+            num_groups[j] = G;
+            levels_present[j] = r.fe_levels[j];
+            std::memcpy(wsums + static_cast<std::size_t>(j) * n, r.fe_weight_sums[j].data(),
+                        sizeof(double) * G);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_indexer: %s\n", e.what());
         return -1;
     }
 }

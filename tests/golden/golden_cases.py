@@ -39,6 +39,30 @@ def input_checksum(p):
     return np.frombuffer(h.digest()[:8], dtype=np.int64)[0]
 
 
+def indexer_cases():
+    """Raw id vectors that take every branch of build_indexer
+    (src/fe_absorption.cpp:845-940), each with seeded row weights:
+    dense with every slot used, dense with gaps above the 90 % rule, dense
+    below it (re-ranked), negative ids, a range wider than 2n (sorted rank),
+    ids near the int32 limits, one row, one repeated id, already sorted ids."""
+    rng = np.random.default_rng(20261204)
+    out = {}
+    n = 5000
+    out["dense_full"] = rng.permutation(np.arange(n) % 1200)
+    v = rng.integers(0, 1000, n)
+    out["dense_gaps_kept"] = v[(v % 20) != 3]                    # 95 % of the slots used
+    out["dense_reranked"] = rng.integers(0, 2 * n, n) * 1         # about 39 % used -> ranked
+    out["negative"] = rng.integers(-3000, 500, n)
+    out["sparse_range"] = rng.integers(-10 ** 6, 10 ** 6, n)
+    out["int32_limits"] = np.concatenate([rng.integers(2 ** 31 - 40, 2 ** 31, 300),
+                                          rng.integers(-2 ** 31, -2 ** 31 + 40, 300)])
+    out["one_row"] = np.array([17])
+    out["one_id"] = np.full(700, -5)
+    out["sorted"] = np.sort(rng.integers(100, 4000, n))
+    return {k: (np.asarray(v, dtype=np.int32), 0.25 + rng.uniform(0.0, 2.0, np.asarray(v).size))
+            for k, v in out.items()}
+
+
 def certificate_weights(n):
     """Deterministic row weights for the weighted certificate fixtures."""
     return 0.5 + (np.arange(n) % 7) / 4.0

@@ -19,7 +19,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "xhdfe-xfe_amd"))
 
 import oracle  # noqa: E402
-from golden_cases import (CASES, JLA_DRAWS, JLA_SEED, SLOPE_CASES, certificate_weights,  # noqa: E402
+from golden_cases import (CASES, JLA_DRAWS, JLA_SEED, SLOPE_CASES, certificate_weights, indexer_cases,  # noqa: E402
                           gi_panel, gi_weights, input_checksum, jla_panel, leave_out_panels, schwarz_edge_cases, slope_values)
 
 
@@ -56,6 +56,14 @@ def main():
             c = oracle.ref_certify(yk, Xk, yo, Xo, fk, weights=w, tol=tol, converged=conv)
             out[f"{name}/{tag}/cert_w"] = np.array([c["abs_residual"], c["abs_residual_rel"],
                                                     float(c["certified"])])
+        # the weighted within transform (absorb_fixed_effects with analytic weights)
+        w = certificate_weights(yk.size)
+        for tag, tol in (("tight", 1e-12), ("default", 1e-8)):
+            yo, Xo, it, conv = oracle.ref_absorb(yk, Xk, fk, tol=tol, symmetric=len(fes) == 2, weights=w)
+            out[f"{name}/w_{tag}/y_tilde"] = yo
+            out[f"{name}/w_{tag}/X_tilde"] = Xo
+            out[f"{name}/w_{tag}/iterations"] = np.int64(it)
+            out[f"{name}/w_{tag}/converged"] = np.int64(conv)
         # savefe recovery of y's FE values (recover_fixed_effects), at the
         # default fe_tolerance and tightly
         for tag, ftol in (("default", 1e-6), ("tight", 1e-12)):
@@ -89,6 +97,15 @@ def main():
             out[f"{name}/fit/{key}"] = np.asarray(f[key])
         for key in ("df_a", "df_resid", "num_singletons", "nobs", "rss"):
             out[f"{name}/fit/{key}"] = np.asarray(f[key])
+    # the reference's indexer and weight sums (build_indexer, compute_weight_sums)
+    for name, (raw, w) in indexer_cases().items():
+        (g, G, lp, ws), = oracle.ref_indexer([raw])
+        (gw, Gw, lpw, wsw), = oracle.ref_indexer([raw], weights=w)
+        assert np.array_equal(g, gw) and G == Gw and lp == lpw
+        out[f"indexer/{name}/gids"] = g
+        out[f"indexer/{name}/info"] = np.array([G, lp], dtype=np.int64)
+        out[f"indexer/{name}/counts"] = ws
+        out[f"indexer/{name}/wsums"] = wsw
     # schwarz_demean_raw's gates and edge cases on the two-way panel: the
     # returned flag and iteration count per case (order of SCHWARZ_EDGE_CASES)
     out["schwarz_edges"] = np.array([list(oracle.ref_schwarz(*args, tol=1e-10, max_iter=mi)[0::3])

@@ -85,3 +85,47 @@ def test_new_entry_points_validate_before_the_device():
     L.xfe_cert_shadow_begin.restype = ctypes.c_void_p
     assert not L.xfe_cert_shadow_begin(None, None, 8, 0, 8, None, None, None, None, 1, None, None)
     assert "invalid" in xfe_amd.last_error()
+
+
+def test_refused_methods_fail_closed():
+    """Jacobi, LSMR and MLSMR are not device methods (the reference's
+    dispatcher never sends them to its device absorber,
+    src/fe_absorption.cpp:9127-9141): the call fails with a message naming the
+    method, before any device work."""
+    n = 16
+    y = np.arange(n, dtype=float)
+    X = np.ones((n, 1))
+    g = (np.arange(n) % 4).astype(np.int32)
+    for method, word in ((3, "Jacobi"), (5, "LSMR"), (6, "LSMR"), (9, "unknown")):
+        with pytest.raises(xfe_amd.XfeError, match=word):
+            xfe_amd.absorb(y, X, [g, g[::-1].copy()], [4, 4], method=method)
+
+
+def test_ids_outside_int32_rejected():
+    """int64 ids that do not fit int32 raise instead of wrapping onto other levels."""
+    w = np.array([0, 1, 2 ** 32 + 5, 3], dtype=np.int64)
+    with pytest.raises(xfe_amd.XfeError, match="int32"):
+        xfe_amd.leave_out_set(w, np.zeros(4, dtype=np.int64))
+    with pytest.raises(xfe_amd.XfeError, match="int32"):
+        xfe_amd.build_indexer(w)
+
+
+def test_articulation_rejects_non_simple_graphs():
+    """A repeated edge or a self loop is refused on the host (the device method
+    assumes a simple graph)."""
+    ptr = np.array([0, 2, 3, 4])
+    with pytest.raises(xfe_amd.XfeError, match="repeated edge"):
+        xfe_amd.articulation_points(ptr, np.array([1, 1, 0, 0]))
+    with pytest.raises(xfe_amd.XfeError, match="self loop"):
+        xfe_amd.articulation_points(np.array([0, 1, 2]), np.array([0, 0]))
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/include"), reason="reference headers not present")
+def test_adapters_compile_against_reference_headers():
+    """The drop-in adapters (xhdfe-xfe_amd/adapter) compile against the
+    reference's own headers with -DHDFE_USE_CUDA, so the signatures they define
+    are exactly the ones the reference's callers declare."""
+    import subprocess
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "xhdfe-xfe_amd", "adapter"), "check"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]

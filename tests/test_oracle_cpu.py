@@ -867,3 +867,104 @@ def test_leave_out_set_matches_reference(golden, name):
     keep, info = oracle.leave_out_set(w, f)
     np.testing.assert_array_equal(keep, golden[f"los/{name}/keep"].astype(bool))
     assert [info[k] for k in _LOS_KEYS] == golden[f"los/{name}/info"].tolist()
+
+
+# ---- the indexer and weight sums (build_indexer, compute_weight_sums) ----
+
+def test_indexer_matches_reference(golden):
+    """Every branch of the reference's build_indexer (fixtures written from the
+    compiled reference through retain_fixed_effects): ids, level counts,
+    levels present; compute_weight_sums' counts and weighted sums."""
+    from golden_cases import indexer_cases
+    for name, (raw, w) in indexer_cases().items():
+        gid, G, lp, cont = oracle.build_indexer(raw)
+        np.testing.assert_array_equal(gid, golden[f"indexer/{name}/gids"], err_msg=name)
+        assert [G, lp] == list(golden[f"indexer/{name}/info"]), name
+        assert cont == bool(np.all(np.diff(gid) >= 0)), name
+        np.testing.assert_array_equal(oracle.weight_sums(gid, G), golden[f"indexer/{name}/counts"])
+        ref = golden[f"indexer/{name}/wsums"]
+        np.testing.assert_allclose(oracle.weight_sums(gid, G, w), ref, rtol=1e-13, atol=1e-13)
+
+
+def test_indexer_rules_by_brute_force():
+    """The documented rules restated independently on random vectors: ids
+    follow ascending raw ids; dense ranges keep unused slots unless fewer than
+    90 % are used; the sparse path ranks the distinct ids."""
+    rng = np.random.default_rng(7)
+    for t in range(400):
+        n = int(rng.integers(1, 300))
+        span = int(rng.choice([n // 2 + 1, n, 2 * n, 3 * n, 10 ** 7]))
+        raw = (rng.integers(0, span, n) - rng.integers(0, 3) * span // 2).astype(np.int32)
+        gid, G, lp, _ = oracle.build_indexer(raw)
+        uniq = np.unique(raw)
+        rng_ = int(raw.max()) - int(raw.min()) + 1
+        if rng_ <= 2 * n and uniq.size * 10 >= rng_ * 9:
+            assert G == rng_ and lp == uniq.size
+            np.testing.assert_array_equal(gid, raw.astype(np.int64) - raw.min())
+        else:
+            assert G == lp == uniq.size
+            np.testing.assert_array_equal(gid, np.searchsorted(uniq, raw))
+
+
+def test_indexer_against_compiled_reference_random():
+    if oracle.ref() is None:
+        pytest.skip("compiled reference not built here")
+    rng = np.random.default_rng(11)
+    for t in range(200):
+        n = int(rng.integers(1, 500))
+        raw = rng.integers(-(10 ** int(rng.integers(1, 9))), 10 ** int(rng.integers(1, 9)), n).astype(np.int32)
+        (g, G, lp, _), = oracle.ref_indexer([raw])
+        og, oG, olp, _ = oracle.build_indexer(raw)
+        np.testing.assert_array_equal(og, g)
+        assert (oG, olp) == (G, lp)
+
+
+# ---- the weighted within transform ----
+
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way", "sparse_two_way"])
+@pytest.mark.parametrize("tag", ["tight", "default"])
+def test_absorb_weighted_matches_reference(golden, cases, name, tag):
+    """The oracle's weighted absorber against the reference's
+    absorb_fixed_effects with the same analytic weights: same iterations
+    (one more on the low-mobility panel at tol 1e-12) and the same transform."""
+    from golden_cases import certificate_weights
+    p = cases[name]
+    keep = _kept(p)
+    fes = [f[keep] for f in p["fes"]]
+    gids, G = _compact(fes)
+    V = np.column_stack([p["y"][keep], p["X"][keep]])
+    w = certificate_weights(V.shape[0])
+    tol = 1e-12 if tag == "tight" else 1e-8
+    Vo, info = oracle.absorb(V, gids, G, weights=w, tol=tol)
+    ref = np.column_stack([golden[f"{name}/w_{tag}/y_tilde"], golden[f"{name}/w_{tag}/X_tilde"]])
+    err = np.abs(Vo - ref).max() / np.abs(V).max()
+    assert err <= (1e-10 if tag == "tight" else 1e-9), err
+    assert info["converged"] and bool(golden[f"{name}/w_{tag}/converged"])
+    assert abs(info["iterations"] - int(golden[f"{name}/w_{tag}/iterations"])) <= 1
+
+
+def test_absorb_weighted_closed_form():
+    """Weighted least squares by a dense solve on a tiny two-way case: the
+    weighted within transform is the W-orthogonal residual, D'W v~ = 0, and
+    it is idempotent."""
+    rng = np.random.default_rng(5)
+    n = 240
+    a = rng.integers(0, 30, n).astype(np.int32)
+    b = rng.integers(0, 12, n).astype(np.int32)
+    keep, _, _ = oracle.drop_singletons([a, b])
+    a, b = a[keep], b[keep]
+    n = a.size
+    gids, G = _compact([a, b])
+    V = rng.normal(size=(n, 3))
+    w = rng.uniform(0.2, 3.0, n)
+    Vo, info = oracle.absorb(V, gids, G, weights=w, tol=1e-14, tolerance_mode=0)
+    D = np.zeros((n, G[0] + G[1]))
+    D[np.arange(n), gids[0]] = 1
+    D[np.arange(n), G[0] + gids[1]] = 1
+    sw = np.sqrt(w)[:, None]
+    coef, *_ = np.linalg.lstsq(D * sw, V * sw, rcond=None)
+    exact = V - D @ coef
+    assert np.abs(Vo - exact).max() / np.abs(V).max() < 1e-10
+    assert np.abs(D.T @ (w[:, None] * Vo)).max() < 1e-9
+    Vo2, _ = oracle.absorb(Vo, gids, G, weights=w, tol=1e-14, tolerance_mode=0)
+    assert np.abs(Vo2 - Vo).max() < 1e-10

@@ -1193,7 +1193,9 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
     FastSolver fs(sv);
     if (!fs.setup()) return false;
     const int C = sv.C;
-    const double target = opt.refine_tol > 0.0 ? opt.refine_tol : 1e-11;
+    const double target = opt.refine_tol > 0.0 ? opt.refine_tol
+                                               : std::min(1e-11, (opt.tol > 0.0 ? opt.tol : 1e-8) * 1e-3);
+    res.refine_target = target;
     const double tol_in = std::getenv("XFE_FAST_INNER") ? std::atof(std::getenv("XFE_FAST_INNER")) : 1e-5;
     const int max_inner = std::max(10, std::min(opt.max_iter, 5000));
     double b0[kMaxCgCols], bb[kMaxCgCols];
@@ -1252,7 +1254,14 @@ static void solve_core(Solver& sv, double* V, const std::vector<int>& order_in,
     const double* w = sv.w;
     std::vector<int> G(sv.G, sv.G + D);
     for (int d = 0; d < D; ++d) res.sweep_order_used[d] = order_in[d];
-    if (opt.precision == 1 && solve_fast(sv, V, opt, res)) return;
+    if (opt.precision == 1) {
+        if (solve_fast(sv, V, opt, res)) {
+            res.precision_used = 1;
+            return;
+        }
+        res.precision_fallback = 1;
+    }
+    res.precision_used = 0;
     const bool honest = opt.tolerance_mode == 1;
     const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
     std::vector<int> order = order_in;
@@ -2117,6 +2126,24 @@ static void gi_colour(const xfe_group_individual& gi, std::vector<int>& members,
     for (int j = 0; j < m; ++j) members[cur[colour[j]]++] = j;
 }
 
+// AbsorptionMethod values (include/hdfe/hdfe_regressor.hpp:18) the device
+// absorber accepts: Auto, GaussSeidel, SymmetricGaussSeidel, and Schwarz /
+// AutoMlsmr, which the reference's device absorber sweeps as Gauss-Seidel
+// (every method but Jacobi, src/fe_absorption_cuda.cu:3383).  Jacobi is never
+// sent to the device by the reference's dispatcher (src/fe_absorption.cpp:9141)
+// and is refused with slopes and by the group()/individual() absorber
+// (src/fe_absorption_cuda.cu:2324, :4771); Lsmr and Mlsmr are CPU-only there
+// (src/fe_absorption.cpp:9127).  A refused method fails closed (return 0), so
+// the caller keeps its CPU path.
+static bool method_ok(int method, const char* fn) {
+    if (method == 0 || method == 1 || method == 2 || method == 4 || method == 7) return true;
+    const char* what = method == 3 ? "Jacobi is not a device method"
+                     : (method == 5 || method == 6) ? "LSMR/MLSMR are CPU-only methods"
+                                                    : "unknown absorption method";
+    set_error(std::string(fn) + ": " + what + " (method " + std::to_string(method) + ")");
+    return false;
+}
+
 static bool gi_valid(const xfe_group_individual* gi, int n) {
     if (!gi || gi->num_groups != n || gi->num_individuals <= 0 || !gi->group_ptr || !gi->group_individual ||
         !gi->individual_ptr || !gi->individual_group || !gi->group_scale)
@@ -2156,6 +2183,7 @@ int xfe_absorb_device(double* d_V, int n, int C, int ld, const int* d_group_ids,
         set_error("xfe_absorb_device: invalid arguments");
         return 0;
     }
+    if (!method_ok(opt->method, "xfe_absorb_device")) return 0;
     std::memset(res, 0, sizeof(*res));
     if (num_fes == 0 || n == 0) {
         res->converged = 1;
@@ -2192,6 +2220,7 @@ int xfe_absorb(const double* y, const double* X, int n, int k, const xfe_fe_inpu
         set_error("xfe_absorb: invalid arguments");
         return 0;
     }
+    if (!method_ok(opt->method, "xfe_absorb")) return 0;
     std::memset(res, 0, sizeof(*res));
     bool slopes = false;
     for (int d = 0; d < num_fes; ++d) {
@@ -2574,6 +2603,7 @@ int xfe_absorb_group_individual(const double* y, const double* X, int n, int k, 
         set_error("xfe_absorb_group_individual: invalid arguments");
         return 0;
     }
+    if (!method_ok(opt->method, "xfe_absorb_group_individual")) return 0;
     std::memset(res, 0, sizeof(*res));
     if (!gi_valid(gi, n)) {
         set_error("xfe_absorb_group_individual: invalid group/individual structure");

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "absorb_kernels.h"
@@ -110,19 +111,25 @@ __global__ void lo_bfs_kernel(const int* __restrict__ ptr, const int* __restrict
     }
 }
 
-// parent = smallest neighbour one level up; size = 1
+// parent = smallest neighbour one level up; size = 1 (a wavefront per vertex,
+// the lanes over its neighbours and a wave minimum)
 __global__ void lo_parent_kernel(const int* __restrict__ ptr, const int* __restrict__ adj,
                                  const int* __restrict__ depth, int* parent, int* size, int n) {
-    LO_FOR(v) {
-        int p = -1;
-        if (depth[v] > 0) {
-            for (int q = ptr[v]; q < ptr[v + 1]; ++q) {
+    const int lane = threadIdx.x & (kWave - 1);
+    LO_FOR_WAVE(v) {
+        const int dv = depth[v];
+        int p = INT_MAX;
+        if (dv > 0) {
+            for (int q = ptr[v] + lane; q < ptr[v + 1]; q += kWave) {
                 const int x = adj[q];
-                if (depth[x] == depth[v] - 1 && (p < 0 || x < p)) p = x;
+                if (depth[x] == dv - 1) p = min(p, x);
             }
+            p = wave_min_int(p);
         }
-        parent[v] = p;
-        size[v] = 1;
+        if (lane == 0) {
+            parent[v] = (dv > 0 && p != INT_MAX) ? p : -1;
+            size[v] = 1;
+        }
     }
 }
 
@@ -162,20 +169,28 @@ __global__ void lo_pre_kernel(const int* __restrict__ ptr, const int* __restrict
     }
 }
 
-// own pre and the pre of every neighbour except the parent
+// own pre and the pre of every neighbour except the parent (a wavefront per
+// vertex: wave minimum and maximum over its neighbours)
 __global__ void lo_local_kernel(const int* __restrict__ ptr, const int* __restrict__ adj,
                                 const int* __restrict__ parent, const int* __restrict__ pre, int* low, int* high,
                                 int n) {
-    LO_FOR(v) {
-        int lo = pre[v], hi = pre[v];
-        for (int q = ptr[v]; q < ptr[v + 1]; ++q) {
+    const int lane = threadIdx.x & (kWave - 1);
+    LO_FOR_WAVE(v) {
+        const int pv = parent[v];
+        int lo = pre[v], hi = lo;
+        for (int q = ptr[v] + lane; q < ptr[v + 1]; q += kWave) {
             const int x = adj[q];
-            if (x == parent[v]) continue;
-            lo = min(lo, pre[x]);
-            hi = max(hi, pre[x]);
+            if (x == pv) continue;
+            const int px = pre[x];
+            lo = min(lo, px);
+            hi = max(hi, px);
         }
-        low[v] = lo;
-        high[v] = hi;
+        lo = wave_min_int(lo);
+        hi = -wave_min_int(-hi);
+        if (lane == 0) {
+            low[v] = lo;
+            high[v] = hi;
+        }
     }
 }
 
@@ -289,6 +304,25 @@ int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, 
             set_error("xfe_articulation_points: neighbour outside [0, n_vertices)");
             return 0;
         }
+    {
+        // a repeated neighbour would give a child two preorder numbers (two lanes write
+        // it), so the simple-graph precondition is checked: one stamp per vertex
+        std::vector<int> stamp(n, -1);
+        for (int v = 0; v < n; ++v)
+            for (int q = adj_ptr[v]; q < adj_ptr[v + 1]; ++q) {
+                const int u = adj[q];
+                if (u == v) {
+                    set_error("xfe_articulation_points: self loop at vertex " + std::to_string(v));
+                    return 0;
+                }
+                if (stamp[u] == v) {
+                    set_error("xfe_articulation_points: repeated edge " + std::to_string(v) + "-" +
+                              std::to_string(u));
+                    return 0;
+                }
+                stamp[u] = v;
+            }
+    }
     hipStream_t s = nullptr;
     DevBuf<int> d_ptr, d_adj, label, depth, parent, size, pre, low, high, cls, rootmin, art, flag;
     auto release_all = [&]() {
@@ -339,7 +373,7 @@ int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, 
             XFE_CHECK(hipStreamSynchronize(s));
             if (!changed) break;
         }
-        hipLaunchKernelGGL(lo_parent_kernel, dim3(g), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, depth.p, parent.p,
+        hipLaunchKernelGGL(lo_parent_kernel, dim3(gw), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, depth.p, parent.p,
                            size.p, n);
         // 3. sizes bottom-up, preorder top-down
         for (int d = maxd; d >= 1; --d)
@@ -348,8 +382,8 @@ int xfe_articulation_points(int n_vertices, const int* adj_ptr, const int* adj, 
             hipLaunchKernelGGL(lo_pre_kernel, dim3(gw), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, depth.p, parent.p,
                                size.p, pre.p, n, d);
         // 4. low / high
-        hipLaunchKernelGGL(lo_local_kernel, dim3(g), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, parent.p, pre.p, low.p,
-                           high.p, n);
+        hipLaunchKernelGGL(lo_local_kernel, dim3(gw), dim3(kLoBlock), 0, s, d_ptr.p, d_adj.p, parent.p, pre.p,
+                           low.p, high.p, n);
         for (int d = maxd; d >= 1; --d)
             hipLaunchKernelGGL(lo_lowhigh_kernel, dim3(g), dim3(kLoBlock), 0, s, depth.p, parent.p, low.p, high.p, n,
                                d);
@@ -404,19 +438,44 @@ struct UnionFindHost {
     }
 };
 
-// dense labels in first-appearance row order (relabel_dense, src/akm_kss.cpp:383)
+// dense labels in first-appearance row order (relabel_dense, src/akm_kss.cpp:383):
+// a lookup table over [min, max] when the range is at most 4n + 1024, else the
+// rows sorted by id (stable), each id's first row found, and the ids ranked by
+// that row -- the same labels without a table the size of the id range.
 int dense_first_appearance(const int* ids, int n, std::vector<int>& out) {
     out.resize(n);
     if (n == 0) return 0;
-    const int mx = *std::max_element(ids, ids + n);
-    std::vector<int> lut(static_cast<size_t>(mx) + 1, -1);
-    int next = 0;
-    for (int i = 0; i < n; ++i) {
-        int& slot = lut[ids[i]];
-        if (slot < 0) slot = next++;
-        out[i] = slot;
+    const auto mm = std::minmax_element(ids, ids + n);
+    const long long lo = *mm.first, range = static_cast<long long>(*mm.second) - lo + 1;
+    if (range <= 4LL * n + 1024) {
+        std::vector<int> lut(static_cast<size_t>(range), -1);
+        int next = 0;
+        for (int i = 0; i < n; ++i) {
+            int& slot = lut[static_cast<size_t>(ids[i] - lo)];
+            if (slot < 0) slot = next++;
+            out[i] = slot;
+        }
+        return next;
     }
-    return next;
+    std::vector<int> rows(n);
+    for (int i = 0; i < n; ++i) rows[i] = i;
+    std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return ids[a] < ids[b]; });
+    // first row of each distinct id, in id order; the rows of one id share it
+    std::vector<int> first_of_run;   // first row per distinct id
+    std::vector<int> run_of_row(n);  // distinct-id index of each row
+    for (int k = 0; k < n; ++k) {
+        const int r = rows[k];
+        if (k == 0 || ids[r] != ids[rows[k - 1]]) first_of_run.push_back(r);
+        run_of_row[r] = static_cast<int>(first_of_run.size()) - 1;
+    }
+    const int m = static_cast<int>(first_of_run.size());
+    std::vector<int> runs(m);
+    for (int j = 0; j < m; ++j) runs[j] = j;
+    std::sort(runs.begin(), runs.end(), [&](int a, int b) { return first_of_run[a] < first_of_run[b]; });
+    std::vector<int> label(m);
+    for (int j = 0; j < m; ++j) label[runs[j]] = j;
+    for (int i = 0; i < n; ++i) out[i] = label[run_of_row[i]];
+    return m;
 }
 
 }  // namespace

@@ -39,7 +39,8 @@ class AbsorbResult(ctypes.Structure):
                 ("polish_sweeps", _i), ("polish_final_max", _d), ("sweep_order_used", _i * MAX_DIMS),
                 ("fe_levels", _i * MAX_DIMS), ("sweeps", _i), ("refinements", _i), ("solve_ms", _d),
                 ("abs_residual", _d), ("abs_residual_rel", _d), ("precision_certified", _i),
-                ("certificate_computed", _i)]
+                ("certificate_computed", _i), ("precision_used", _i), ("precision_fallback", _i),
+                ("refine_target", _d), ("preconditioner_used", _i), ("precond_fallback", _i)]
 
 
 class GroupIndividual(ctypes.Structure):
@@ -72,7 +73,8 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
            "xfe_absorb_group_individual",
            "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort",
-           "xfe_articulation_points", "xfe_akm_leave_out_set")
+           "xfe_articulation_points", "xfe_akm_leave_out_set", "xfe_build_indexer",
+           "xfe_build_indexer_device", "xfe_weight_sums")
 
 _lib = None
 
@@ -113,6 +115,23 @@ def backend_available():
     return bool(lib().xfe_backend_available())
 
 
+
+def _i32(a):
+    """int32 copy of an integer array for the C ABI; values outside the int32
+    range raise XfeError instead of wrapping (a wrapped id would merge levels)."""
+    arr = np.asarray(a)
+    if arr.dtype != np.int32 and arr.size:
+        if arr.dtype.kind not in "iub":
+            if arr.dtype.kind == "f" and np.all(np.isfinite(arr)) and np.all(arr == np.floor(arr)):
+                pass
+            else:
+                raise XfeError(f"integer ids expected, got dtype {arr.dtype}")
+        lo, hi = arr.min(), arr.max()
+        info = np.iinfo(np.int32)
+        if lo < info.min or hi > info.max:
+            raise XfeError(f"ids outside the int32 range ([{lo}, {hi}]); compact them first")
+    return np.ascontiguousarray(arr, dtype=np.int32)
+
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(_p)
 
@@ -129,7 +148,9 @@ def _info(r, D):
                 fe_levels=list(r.fe_levels)[:D], sweeps=r.sweeps, refinements=r.refinements,
                 solve_ms=r.solve_ms, abs_residual=r.abs_residual, abs_residual_rel=r.abs_residual_rel,
                 precision_certified=bool(r.precision_certified),
-                certificate_computed=bool(r.certificate_computed))
+                certificate_computed=bool(r.certificate_computed), precision_used=r.precision_used,
+                precision_fallback=bool(r.precision_fallback), refine_target=r.refine_target,
+                preconditioner_used=r.preconditioner_used, precond_fallback=bool(r.precond_fallback))
 
 
 def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=1e-8,
@@ -144,7 +165,7 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
     X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
     n, k = X.shape
     D = len(gids)
-    ids = [np.ascontiguousarray(g, dtype=np.int32) for g in gids]
+    ids = [_i32(g) for g in gids]
     ws = [None if weight_sums is None else np.ascontiguousarray(weight_sums[d], dtype=np.float64)
           for d in range(D)]
     fes = (FeInput * max(D, 1))()
@@ -160,7 +181,7 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
         fes[d].slope_has_intercept = int(bool(icpt))
         fes[d].slope_values = zs[d].ctypes.data
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
-    order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
+    order = None if sweep_order is None else _i32(sweep_order)
     if method is None:
         method = 2 if D == 2 else 1
     o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify, refine_tol)
@@ -184,20 +205,20 @@ def absorb_group_individual(y, X, gids, G, gi, weights=None, sweep_order=None, t
     X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
     n, k = X.shape
     D = len(gids)
-    ids = [np.ascontiguousarray(g, dtype=np.int32) for g in gids]
+    ids = [_i32(g) for g in gids]
     fes = (FeInput * max(D, 1))()
     for d in range(D):
         fes[d].group_ids = ids[d].ctypes.data
         fes[d].num_groups = int(G[d])
         fes[d].num_levels_present = int(G[d])
-    arrs = {key: np.ascontiguousarray(gi[key], dtype=np.int32)
+    arrs = {key: _i32(gi[key])
             for key in ("group_ptr", "group_individual", "individual_ptr", "individual_group")}
     scale = np.ascontiguousarray(gi["group_scale"], dtype=np.float64)
     s = GroupIndividual(n, int(gi["num_individuals"]), arrs["group_ptr"].ctypes.data,
                         arrs["group_individual"].ctypes.data, arrs["individual_ptr"].ctypes.data,
                         arrs["individual_group"].ctypes.data, scale.ctypes.data)
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
-    order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
+    order = None if sweep_order is None else _i32(sweep_order)
     o = _opts(tol, tolerance_mode, max_iter, method, False, 0, True, 0.0)
     r = AbsorbResult()
     yo = np.empty(n)
@@ -209,10 +230,42 @@ def absorb_group_individual(y, X, gids, G, gi, weights=None, sweep_order=None, t
     return yo, Xo, _info(r, D)
 
 
+def build_indexer(raw):
+    """build_indexer on the device: (compact ids, num_groups, levels_present, contiguous)."""
+    r = _i32(raw)
+    gid = np.empty(max(r.size, 1), dtype=np.int32)
+    G, lp, cont = _i(), _i(), _i()
+    if not lib().xfe_build_indexer(_ptr(r) if r.size else None, _i(r.size), _ptr(gid), ctypes.byref(G),
+                                   ctypes.byref(lp), ctypes.byref(cont)):
+        raise XfeError(last_error())
+    return gid[:r.size], G.value, lp.value, bool(cont.value)
+
+
+def build_indexer_device(raw_t):
+    """build_indexer on a device tensor of int32 ids: (ids tensor, num_groups, levels_present, contiguous)."""
+    import torch
+    out = torch.empty_like(raw_t)
+    G, lp, cont = _i(), _i(), _i()
+    if not lib().xfe_build_indexer_device(_p(raw_t.data_ptr()), _i(raw_t.numel()), _p(out.data_ptr()),
+                                          ctypes.byref(G), ctypes.byref(lp), ctypes.byref(cont)):
+        raise XfeError(last_error())
+    return out, G.value, lp.value, bool(cont.value)
+
+
+def weight_sums(gid, G, weights=None):
+    """compute_weight_sums on the device: level sums of the weights (counts when None)."""
+    g = _i32(gid)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    out = np.empty(int(G))
+    if not lib().xfe_weight_sums(_ptr(g) if g.size else None, _i(g.size), _i(int(G)), _ptr(w), _ptr(out)):
+        raise XfeError(last_error())
+    return out
+
+
 def articulation_points(adj_ptr, adj):
     """Cut vertices of an undirected CSR graph (uint8 flag per vertex)."""
-    ptr = np.ascontiguousarray(adj_ptr, dtype=np.int32)
-    a = np.ascontiguousarray(adj, dtype=np.int32)
+    ptr = _i32(adj_ptr)
+    a = _i32(adj)
     n = ptr.size - 1
     out = np.zeros(max(n, 1), dtype=np.uint8)
     if not lib().xfe_articulation_points(_i(n), _ptr(ptr), _ptr(a) if a.size else None, _ptr(out)):
@@ -228,8 +281,8 @@ class LeaveOutResult(ctypes.Structure):
 
 def leave_out_set(worker_ids, firm_ids, fweights=None):
     """leave_out_connected_set: (keep mask per input row, info dict)."""
-    w = np.ascontiguousarray(worker_ids, dtype=np.int32)
-    f = np.ascontiguousarray(firm_ids, dtype=np.int32)
+    w = _i32(worker_ids)
+    f = _i32(firm_ids)
     fw = None if fweights is None else np.ascontiguousarray(fweights, dtype=np.int64)
     keep = np.zeros(max(w.size, 1), dtype=np.uint8)
     r = LeaveOutResult()
@@ -284,7 +337,7 @@ def recover_fixed_effects(partial, gids, G, weights=None, sweep_order=None, tol=
     coefficients of dimension d, contributions is D x n."""
     partial = np.ascontiguousarray(partial, dtype=np.float64)
     n, D = partial.size, len(gids)
-    ids = [np.ascontiguousarray(g, dtype=np.int32) for g in gids]
+    ids = [_i32(g) for g in gids]
     fes = (FeInput * max(D, 1))()
     for d in range(D):
         fes[d].group_ids = ids[d].ctypes.data
@@ -294,7 +347,7 @@ def recover_fixed_effects(partial, gids, G, weights=None, sweep_order=None, tol=
     aptr = (_p * max(D, 1))(*[a.ctypes.data for a in alphas])
     contrib = np.empty((D, n)) if contributions else None
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
-    order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
+    order = None if sweep_order is None else _i32(sweep_order)
     o = _opts(tol, tolerance_mode, max_iter, 2 if D == 2 else 1, False, 1, False, refine_tol)
     r = RecoveryResult()
     ok = lib().xfe_recover_fixed_effects(_ptr(partial), _i(n), fes, _i(D), _ptr(w), _ptr(order),
@@ -310,8 +363,8 @@ def absorb_device(V_ptr, n, C, ld, gid_ptr, G, weights_ptr=None, sweep_order=Non
                   certify=True):
     """xfe_absorb_device on device pointers (e.g. torch tensors' data_ptr())."""
     D = len(G)
-    Garr = np.ascontiguousarray(G, dtype=np.int32)
-    order = None if sweep_order is None else np.ascontiguousarray(sweep_order, dtype=np.int32)
+    Garr = _i32(G)
+    order = None if sweep_order is None else _i32(sweep_order)
     if method is None:
         method = 2 if D == 2 else 1
     o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify)
@@ -334,7 +387,7 @@ def certify(y, X, y_t, X_t, gids, G, weights=None, tol=1e-8, tolerance_mode=1, c
     Xt = np.asfortranarray(np.asarray(X_t, dtype=np.float64).reshape(y.size, -1))
     n, k = X.shape
     D = len(gids)
-    ids = [np.ascontiguousarray(g, dtype=np.int32) for g in gids]
+    ids = [_i32(g) for g in gids]
     fes = (FeInput * max(D, 1))()
     for d in range(D):
         fes[d].group_ids = ids[d].ctypes.data
@@ -369,9 +422,9 @@ def schwarz_demean(Y, X, fe, nlev, tol=1e-8, max_iter=10000, threads=1, collapse
             raise XfeError("X must be a column-major float64 (n, k) array")
         k = X.shape[1] if X.ndim == 2 else 1
         Xp = _ptr(X)
-    ids = [np.ascontiguousarray(f, dtype=np.int32) for f in fe]
+    ids = [_i32(f) for f in fe]
     ptrs = (_p * max(len(ids), 1))(*[a.ctypes.data for a in ids])
-    lev = np.ascontiguousarray(nlev, dtype=np.int32)
+    lev = _i32(nlev)
     it = _i(0)
     L = lib()
     L.xfe_schwarz_demean.argtypes = [_p, _p, ctypes.c_int64, _i, _p, _p, _i, _i, _d, _i, _p, _i]
@@ -477,7 +530,7 @@ def akm_stable_sort_u64(keys):
 
 def drop_singletons(fes, max_rounds=100):
     """compute_keep_mask_drop_singletons on the device: (keep mask, dropped, rounds)."""
-    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    ids = np.ascontiguousarray(np.stack([_i32(f) for f in fes]))
     D, n = ids.shape
     keep = np.empty(n, dtype=np.uint8)
     rounds = _i()
@@ -490,8 +543,8 @@ def drop_singletons(fes, max_rounds=100):
 
 def components(a, b, expected_a=-1, expected_b=-1, labels=True):
     """count_bipartite_components on the device: (count, 1-based labels or None)."""
-    a = np.ascontiguousarray(a, dtype=np.int32)
-    b = np.ascontiguousarray(b, dtype=np.int32)
+    a = _i32(a)
+    b = _i32(b)
     lab = np.empty(a.size, dtype=np.int32) if labels else None
     c = lib().xfe_bipartite_components(_ptr(a), _ptr(b), _i(a.size), _i(expected_a),
                                        _i(expected_b), _ptr(lab))
@@ -502,9 +555,9 @@ def components(a, b, expected_a=-1, expected_b=-1, labels=True):
 
 def fe_dof(fes, levels):
     """compute_fe_dof_reghdfe (pairwise) on the device: (df_a, redundant, num_coefs)."""
-    ids = np.ascontiguousarray(np.stack([np.asarray(f, dtype=np.int32) for f in fes]))
+    ids = np.ascontiguousarray(np.stack([_i32(f) for f in fes]))
     D, n = ids.shape
-    lv = np.ascontiguousarray(levels, dtype=np.int32)
+    lv = _i32(levels)
     red = np.empty(D, dtype=np.int32)
     nc = np.empty(D, dtype=np.int32)
     df = lib().xfe_fe_dof(_ptr(ids), _i(n), _i(D), _ptr(lv), _ptr(red), _ptr(nc))
@@ -526,7 +579,7 @@ def cluster_meat(V, beta, cluster_ids, num_clusters):
     V = np.asfortranarray(V, dtype=np.float64)
     n, C = V.shape
     b = np.ascontiguousarray(beta, dtype=np.float64)
-    cl = np.ascontiguousarray(cluster_ids, dtype=np.int32)
+    cl = _i32(cluster_ids)
     out = np.empty((C - 1, C - 1), order="F")
     if lib().xfe_cluster_meat(_ptr(V), _i(n), _i(C), _ptr(b), _ptr(cl), _i(num_clusters),
                               _ptr(out)) != 0:
