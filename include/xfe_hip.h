@@ -246,6 +246,20 @@ int xfe_build_indexer(const int* raw, int n, int* group_ids, int* num_groups, in
 int xfe_build_indexer_device(const int* d_raw, int n, int* d_group_ids, int* num_groups,
                              int* num_levels_present, int* contiguous);
 
+/* Allocation-free form of xfe_build_indexer_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same kernels take their scratch (min/max, flags, rank tables and the
+ * radix-sort and scan storage) from a 40 MiB file-scope device array that each
+ * device holds until the process ends; the call makes no hipMalloc/hipFree and
+ * creates no stream (it runs on the null stream).  n may be at most 1,048,576
+ * rows (2^20); larger inputs are rejected with an error.  Every call on one
+ * device shares that device's array: calls that overlap on one device race
+ * (serialize them); each call clears what it needs.  Same outputs, ordering
+ * and error convention as xfe_build_indexer_device. */
+int xfe_build_indexer_device_noalloc(const int* d_raw, int n, int* d_group_ids, int* num_groups,
+                                     int* num_levels_present, int* contiguous);
+
 /* compute_weight_sums (src/fe_absorption.cpp:942-1037): out[g] = sum of
  * weights (or the count, weights NULL) over the rows with group_ids[i] = g,
  * g < num_groups.  Ids outside [0, num_groups) are rejected.  Host buffers;

@@ -74,7 +74,7 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_absorb_group_individual",
            "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort",
            "xfe_articulation_points", "xfe_akm_leave_out_set", "xfe_build_indexer",
-           "xfe_build_indexer_device", "xfe_weight_sums")
+           "xfe_build_indexer_device", "xfe_weight_sums", "xfe_build_indexer_device_noalloc")
 
 _lib = None
 
