@@ -58,7 +58,7 @@ typedef struct {
                                    (certify_absorption_result, src/fe_absorption.cpp:14942-15190) */
     double refine_tol;          /* precision 1: FP64 refinement target, ||b - S beta|| / ||b|| of the
                                    Schur system over the non-primary levels; <= 0 selects
-                                   min(1e-11, tol / 1000) */
+                                   min(3e-13, 3e-5 tol) */
 } xfe_options;
 
 /* Outputs of one absorption (the fields of hdfe::detail::AbsorptionResult,

@@ -131,9 +131,9 @@ def test_precision_report(cases):
     V = np.column_stack([p["y"][keep], p["X"][keep]])
     _, _, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, precision=1, tol=1e-8)
     assert info["precision_used"] == 1 and not info["precision_fallback"]
-    assert info["refine_target"] == pytest.approx(1e-11)
+    assert info["refine_target"] == pytest.approx(3e-13)
     _, _, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, precision=1, tol=1e-10)
-    assert info["refine_target"] == pytest.approx(1e-13)
+    assert info["refine_target"] == pytest.approx(3e-15)
     _, _, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, precision=1, tol=1e-8,
                                 weights=certificate_weights(V.shape[0]))
     assert info["precision_used"] == 0 and info["precision_fallback"]

@@ -57,7 +57,7 @@ xfe_options to_options(const HdfeOptions& options, AbsorptionMethod method, int 
                      : 0;
     o.precision = precision;  // 0: the plain FP64 method; 1: mixed-precision Schur PCG
     o.certify = 1;            // the normal-equation certificate, computed on the device
-    o.refine_tol = 0.0;       // min(1e-11, tol / 1000)
+    o.refine_tol = 0.0;       // min(3e-13, 3e-5 tol)
     return o;
 }
 

@@ -1026,7 +1026,7 @@ struct FastSolver {
         lds_extra64 = static_cast<size_t>(lo64) * sizeof(double);
         const size_t NC = static_cast<size_t>(N) * C;
         ws.f_beta.ensure(NC); ws.f_b.ensure(NC);
-        ws.f_diag.ensure(N); ws.f_counts.ensure(N); ws.f_scale.ensure(2 * kMaxCgCols);
+        ws.f_diag.ensure(N); ws.f_counts.ensure(N); ws.f_scale.ensure(6 * kMaxCgCols);
         ws.f_x.ensure(NC); ws.f_r.ensure(NC); ws.f_z.ensure(NC); ws.f_p.ensure(NC); ws.f_q.ensure(NC);
         ws.f_minv.ensure(N); ws.f_invp.ensure(sv.G[p]);
         for (int k = 0; k < NR; ++k)
@@ -1187,6 +1187,112 @@ struct FastSolver {
         inner_total += std::min(it, max_inner);
         return it;
     }
+
+    // The FP64 residual b0 - S beta into f_b (through the segments, or from the
+    // rows of V).
+    void true_residual(double* V) {
+        if (seg) residual_seg();
+        else residual64(V, nullptr);
+    }
+
+    // Mixed-precision CG with reliable updates (residual replacement; Sleijpen &
+    // van der Vorst, Computing 56 (1996); the mixed-precision form of Clark et
+    // al., Comput. Phys. Commun. 181 (2010)).  One FP32 Jacobi-PCG recurrence on
+    // the Schur system per column, in units of the current residual norm t_c.
+    // When the iterated FP32 residual has fallen to delta (or below the target),
+    // the FP32 iterate is added to the FP64 beta, the residual b0 - S beta is
+    // recomputed in FP64 and replaces the FP32 one, and the recurrence continues
+    // with its old search direction (rescaled), so the Krylov space is kept
+    // across the FP64 updates instead of restarting the solve (the direction
+    // keeps its beta p_prev part and takes the new residual's z).  A column stops
+    // when its FP64 residual is at most target |b0_c|, or when four updates in a
+    // row each fail to halve it.  Returns the FP64 relative residual (max over
+    // columns); iterations and updates through the out arguments.
+    double reliable_solve(double* V, const double* b0n, double target, double delta, int max_inner, int& iters,
+                          int& updates) {
+        constexpr int kPoll = 8;
+        constexpr int K = kMaxCgCols;
+        const long long total = static_cast<long long>(N) * C;
+        const int g = cg_grid(total, C, kMaxGrid);
+        double* cg = ws.cg.p;
+        double* part = ws.cg_part.p;
+        double* d_t = ws.f_scale.p + K;        // the units of the FP32 iterate, per column
+        double* d_rel = ws.f_scale.p + 2 * K;  // inv_t | f | tol2 | act
+        double hb[2 * K], t[K], prev[K];
+        int stall[K];
+        bool act[K];
+        XFE_CHECK(hipMemsetAsync(cg, 0, sizeof(double) * kCgSlots, s));
+        coldot<double>(ws.f_b.p, ws.f_b.p, hb);
+        double worst = 0.0;
+        auto plan = [&](bool first) {
+            double rel[4 * K + K];
+            std::fill(rel, rel + 5 * K, 0.0);
+            int nact = 0;
+            for (int c = 0; c < C; ++c) {
+                if (act[c]) {
+                    rel[c] = 1.0 / t[c];
+                    rel[K + c] = first ? 0.0 : prev[c] / t[c];  // old units -> new units (0: fresh start)
+                    const double lim = 0.5 * target * b0n[c] / t[c];  // half the target: FP32 drift margin
+                    rel[2 * K + c] = std::max(delta * delta, lim * lim);
+                    rel[3 * K + c] = 1.0;
+                    ++nact;
+                }
+                rel[4 * K + c] = t[c];
+            }
+            // d_t and the reset parameters are contiguous: one upload
+            upload(rel + 4 * K, d_t, K);
+            upload(rel, d_rel, 4 * K);
+            return nact;
+        };
+        for (int c = 0; c < C; ++c) {
+            t[c] = std::sqrt(hb[c]);
+            prev[c] = t[c];
+            stall[c] = 0;
+            act[c] = b0n[c] > 0.0 && t[c] > target * b0n[c];
+            if (b0n[c] > 0.0) worst = std::max(worst, t[c] / b0n[c]);
+        }
+        iters = 0;
+        updates = 0;
+        if (plan(true) == 0) return worst;
+        XFE_CHECK(launch_cg_reliable_reset(ws.f_b.p, d_rel, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p,
+                                           total, C, cg, part, g, s));
+        for (int it = 1; it <= max_inner; ++it) {
+            apply32(ws.f_p.p, ws.f_q.p);
+            XFE_CHECK(launch_cg_dot<float>(ws.f_p.p, ws.f_q.p, total, C, cg, part, g, s));
+            XFE_CHECK(launch_cg_step_rel(ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p, ws.f_q.p, ws.f_minv.p, total, C, cg,
+                                         part, max_inner + 1, g, s));
+            iters = it;
+            if (it % kPoll != 0 && it != max_inner) continue;
+            XFE_CHECK(hipMemcpyAsync(ws.host_scalars, cg + kCgCount, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+            const bool want = ws.host_scalars[1] != 0.0 || ws.host_scalars[0] == 0.0 || it == max_inner;
+            if (!want) continue;
+            // reliable update: beta += t_c x; FP64 residual; replace
+            XFE_CHECK(launch_refine_update(ws.f_beta.p, ws.f_x.p, d_t, N, C, grid_for(total), s));
+            true_residual(V);
+            ++updates;
+            coldot<double>(ws.f_b.p, ws.f_b.p, hb);
+            worst = 0.0;
+            for (int c = 0; c < C; ++c) {
+                const double tn = std::sqrt(hb[c]);
+                if (b0n[c] > 0.0) worst = std::max(worst, tn / b0n[c]);
+                if (!act[c]) continue;
+                if (tn <= target * b0n[c] || !(tn > 0.0)) {
+                    act[c] = false;
+                } else {
+                    stall[c] = tn > 0.5 * t[c] ? stall[c] + 1 : 0;
+                    if (stall[c] >= 4) act[c] = false;
+                }
+                prev[c] = t[c];
+                t[c] = tn > 0.0 ? tn : t[c];
+            }
+            if (it == max_inner || plan(false) == 0) break;
+            XFE_CHECK(launch_cg_reliable_reset(ws.f_b.p, d_rel, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p,
+                                               total, C, cg, part, g, s));
+        }
+        inner_total += iters;
+        return worst;
+    }
 };
 
 static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb_result& res) {
@@ -1194,40 +1300,27 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
     if (!fs.setup()) return false;
     const int C = sv.C;
     const double target = opt.refine_tol > 0.0 ? opt.refine_tol
-                                               : std::min(1e-11, (opt.tol > 0.0 ? opt.tol : 1e-8) * 1e-3);
+                                               : std::min(3e-13, (opt.tol > 0.0 ? opt.tol : 1e-8) * 3e-5);
     res.refine_target = target;
     const double tol_in = std::getenv("XFE_FAST_INNER") ? std::atof(std::getenv("XFE_FAST_INNER")) : 1e-5;
     const int max_inner = std::max(10, std::min(opt.max_iter, 5000));
-    double b0[kMaxCgCols], bb[kMaxCgCols];
+    double b0[kMaxCgCols];
     fs.residual64(V, nullptr);
     if (fs.seg) {
         XFE_CHECK(hipMemcpyAsync(sv.ws.f_b0.p, sv.ws.f_b.p, sizeof(double) * static_cast<size_t>(fs.N) * C,
                                  hipMemcpyDeviceToDevice, sv.s));
     }
     fs.coldot<double>(sv.ws.f_b.p, sv.ws.f_b.p, b0);
-    int refinements = 0;
-    bool converged = false;
-    double prev_rel = INFINITY;
-    for (int ref = 0; ref < 8; ++ref) {
-        fs.coldot<double>(sv.ws.f_b.p, sv.ws.f_b.p, bb);
-        double rel = 0.0;
-        for (int c = 0; c < C; ++c)
-            if (b0[c] > 0.0) rel = std::max(rel, std::sqrt(bb[c] / b0[c]));
-        if (rel <= target) { converged = true; break; }
-        if (rel > 0.5 * prev_rel) break;  // refinement no longer gains: accept what we have
-        prev_rel = rel;
-        fs.inner_solve(tol_in, max_inner);
-        ++refinements;
-        if (fs.seg) fs.residual_seg();
-        else fs.residual64(V, nullptr);
-    }
-    if (!converged) {
-        fs.coldot<double>(sv.ws.f_b.p, sv.ws.f_b.p, bb);
-        double rel = 0.0;
-        for (int c = 0; c < C; ++c)
-            if (b0[c] > 0.0) rel = std::max(rel, std::sqrt(bb[c] / b0[c]));
-        converged = rel <= std::max(target, 1e-9);
-    }
+    double b0n[kMaxCgCols];
+    for (int c = 0; c < C; ++c) b0n[c] = std::sqrt(b0[c]);
+    const double delta = std::getenv("XFE_RELIABLE_DELTA") ? std::atof(std::getenv("XFE_RELIABLE_DELTA")) : 0.01;
+    int iters = 0, updates = 0;
+    const double rel = fs.reliable_solve(V, b0n, target, delta, std::max(10, std::min(opt.max_iter, 20000)), iters,
+                                         updates);
+    const int refinements = updates;
+    const bool converged = rel <= std::max(target, 1e-9);
+    (void)tol_in;
+    (void)max_inner;
     fs.residual64(V, V);  // final within transform, in place
     auto& rec = sv.ws.fast_rec;
     rec.used = true;

@@ -184,7 +184,8 @@ enum {
     kCgAlpha = 0 * kMaxCgCols, kCgBeta = 1 * kMaxCgCols, kCgRz = 2 * kMaxCgCols,
     kCgActive = 3 * kMaxCgCols, kCgPq = 4 * kMaxCgCols, kCgRr = 5 * kMaxCgCols,
     kCgRzNew = 6 * kMaxCgCols, kCgIters = 7 * kMaxCgCols, kCgOk = 8 * kMaxCgCols,
-    kCgCount = 9 * kMaxCgCols, kCgSlots = 9 * kMaxCgCols + 8
+    kCgTol2 = 9 * kMaxCgCols,  // reliable-update CG: per-column r.r that calls for an FP64 residual
+    kCgCount = 10 * kMaxCgCols, kCgUpd = 10 * kMaxCgCols + 1, kCgSlots = 10 * kMaxCgCols + 8
 };
 // grid for the CG vector kernels: a multiple of C (each thread keeps one column)
 inline int cg_grid(long long total, int C, int max_grid) {
@@ -209,6 +210,19 @@ hipError_t launch_cg_init(const double* b, const double* scale, const T* minv, T
                           long long total, int C, double* cg, double* part, int grid, hipStream_t s);
 hipError_t launch_refine_update(double* beta, const float* x, const double* inv_scale, int N, int C,
                                 int grid, hipStream_t s);
+// Reliable-update CG (mixed-precision inner solve): an iteration's step with the
+// update test (r.r <= cg[kCgTol2 + c] sets cg[kCgUpd]) instead of the stopping
+// test, then p = z + beta p.
+hipError_t launch_cg_step_rel(float* x, float* r, float* z, float* p, const float* q, const float* minv,
+                              long long total, int C, double* cg, double* part, int max_iter, int grid,
+                              hipStream_t s);
+// Residual replacement: r = r64 / t_c, z = minv r, p = f_c (p - z_old) + z
+// (p = z for a restarted column, f_c = 0), x = 0; then per column rz, the
+// activity act_c and the update threshold tol2_c.
+// rel: [inv_t | f | tol2 | act], kMaxCgCols each.
+hipError_t launch_cg_reliable_reset(const double* r64, const double* rel, const float* minv, float* x, float* r,
+                                    float* z, float* p, long long total, int C, double* cg, double* part, int grid,
+                                    hipStream_t s);
 hipError_t launch_schur_diag(const int* tile_start, int num_tiles, const int* gid_p,
                              const int* gid_r, const double* inv_p, double* diag, int grid,
                              hipStream_t s);

@@ -507,6 +507,97 @@ refine_update_kernel(double* beta, const float* x, const double* inv_scale, int 
     }
 }
 
+// Reliable-update CG: one more iteration per active column; the iterated r.r
+// at or below cg[kCgTol2 + c] asks the host for an FP64 residual (cg[kCgUpd]);
+// reaching max_iter stops the column; beta = rz_new / rz.  The stop on the
+// tolerance is decided on the FP64 residual only.
+__global__ void __launch_bounds__(kRed)
+cg_beta_rel_kernel(double* cg, const double* part1, const double* part2, int nblk, int C, int max_iter) {
+    __shared__ double rrs[kMaxCgCols], rzs[kMaxCgCols];
+    __shared__ int cnt, upd;
+    cg_reduce_parts(part1, part2, nblk, C, rrs, rzs);
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        upd = 0;
+    }
+    __syncthreads();
+    const int c = threadIdx.x;
+    if (c < C) {
+        double be = 0.0;
+        if (cg[kCgActive + c] != 0.0) {
+            const double its = cg[kCgIters + c] + 1.0;
+            cg[kCgIters + c] = its;
+            if (its >= max_iter) {
+                cg[kCgActive + c] = 0.0;
+            } else {
+                if (rrs[c] <= cg[kCgTol2 + c]) atomicOr(&upd, 1);
+                be = rzs[c] > 0.0 && cg[kCgRz + c] > 0.0 ? rzs[c] / cg[kCgRz + c] : 0.0;
+                cg[kCgRz + c] = rzs[c];
+                atomicAdd(&cnt, 1);
+            }
+        }
+        cg[kCgBeta + c] = be;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cg[kCgCount] = cnt;
+        if (upd) cg[kCgUpd] = 1.0;
+    }
+}
+
+// Residual replacement, vector part: r = r64 * inv_t_c (rounded to FP32),
+// z = minv r.  The last step already formed p = z_iter + beta p_prev from the
+// iterated residual; the direction keeps its part beta p_prev (rescaled into the
+// new residual's units by f_c) and takes the new z: p = f (p - z_iter) + z.  A
+// column restarted after a stop (f_c = 0) takes p = z.  x = 0; part1 = r.z per
+// block and column.
+__global__ void __launch_bounds__(kBlock)
+cg_reset_kernel(const double* r64, const double* rel, const float* minv, float* x, float* r, float* z, float* p,
+                long long total, int C, double* part1) {
+    const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const int c = static_cast<int>(g0 % C);
+    const double it = rel[c];
+    const float f = static_cast<float>(rel[kMaxCgCols + c]);
+    double rz = 0.0;
+    for (long long j = g0; j < total; j += stride) {
+        const float rv = static_cast<float>(r64[j] * it);
+        r[j] = rv;
+        const float zv = minv[j / C] * rv;
+        p[j] = f == 0.0f ? zv : f * (p[j] - z[j]) + zv;
+        z[j] = zv;
+        x[j] = 0.0f;
+        rz += static_cast<double>(rv) * static_cast<double>(zv);
+    }
+    cg_block_store(rz, c, C, part1);
+}
+
+// Residual replacement, scalar part: rz of the new residual, the activity, the
+// update thresholds; the update flag is cleared.
+__global__ void __launch_bounds__(kRed)
+cg_reset_scalars_kernel(double* cg, const double* part1, int nblk, const double* rel, int C) {
+    __shared__ double rzs[kMaxCgCols];
+    __shared__ int cnt;
+    cg_reduce_parts(part1, nullptr, nblk, C, rzs, nullptr);
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int c = threadIdx.x;
+    if (c < C) {
+        const bool act = rel[3 * kMaxCgCols + c] != 0.0;
+        cg[kCgRz + c] = rzs[c];
+        cg[kCgActive + c] = (act && rzs[c] > 0.0) ? 1.0 : 0.0;
+        cg[kCgAlpha + c] = 0.0;
+        cg[kCgBeta + c] = 0.0;
+        cg[kCgTol2 + c] = rel[2 * kMaxCgCols + c];
+        if (cg[kCgActive + c] != 0.0) atomicAdd(&cnt, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cg[kCgCount] = cnt;
+        cg[kCgUpd] = 0.0;
+    }
+}
+
 // Diagonal of S for the Jacobi preconditioner, from runs of equal level of r
 // inside a level of p: sum over runs of n - n^2 / N_p (exact when every
 // (p-level, r-level) pair forms one run).  Result is inverted in place later.
@@ -685,6 +776,25 @@ XFE_CG_INST(double)
 hipError_t launch_refine_update(double* beta, const float* x, const double* inv_scale, int N, int C,
                                 int grid, hipStream_t s) {
     hipLaunchKernelGGL(refine_update_kernel, dim3(grid), dim3(kBlock), 0, s, beta, x, inv_scale, N, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_step_rel(float* x, float* r, float* z, float* p, const float* q, const float* minv,
+                              long long total, int C, double* cg, double* part, int max_iter, int grid,
+                              hipStream_t s) {
+    double* part2 = part + static_cast<size_t>(grid) * C;
+    hipLaunchKernelGGL((cg_step_kernel<float>), dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, total, C, cg,
+                       part, part2);
+    hipLaunchKernelGGL(cg_beta_rel_kernel, dim3(1), dim3(kRed), 0, s, cg, part, part2, grid, C, max_iter);
+    hipLaunchKernelGGL((cg_dir_kernel<float>), dim3(grid), dim3(kBlock), 0, s, p, z, cg, total, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_reliable_reset(const double* r64, const double* rel, const float* minv, float* x, float* r,
+                                    float* z, float* p, long long total, int C, double* cg, double* part, int grid,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL(cg_reset_kernel, dim3(grid), dim3(kBlock), 0, s, r64, rel, minv, x, r, z, p, total, C, part);
+    hipLaunchKernelGGL(cg_reset_scalars_kernel, dim3(1), dim3(kRed), 0, s, cg, part, grid, rel, C);
     return hipGetLastError();
 }
 
