@@ -442,6 +442,29 @@ int xfe_akm_max_lanes(void);
  * by the context, or NULL (then the caller uses its own buffer). */
 double* xfe_akm_pack_buffer(xfe_akm_context* ctx, int nb);
 
+/* ---- Learned two-level preconditioner (no counterpart in the reference, whose
+ * preconditioner for the FE normal equations is the randomized approximate
+ * Cholesky of src/schwarz_demean.cpp:100-133).  A graph network over the
+ * worker-firm mobility graph (firms; movers = workers with segments at two or
+ * more firms; one edge per mover segment), width 64, three message-passing
+ * layers, BF16 MFMA with FP32 accumulation, emits per firm a log per-level
+ * scaling u_f (the prolongation weight of the firm in its coarse aggregate) and
+ * per firm pair of one mover a score t_fg (the matching strength of the coarse
+ * aggregation).  The parameter vector is float32, xfe_lp_param_count() long. */
+int xfe_lp_param_count(void);
+
+/* One evaluation of the network, its loss and gradient (host buffers): firm
+ * features xf (F x 8), mover features xw (Wm x 8), firm -> mover edges
+ * (f_ptr: F + 1, f_idx: mover ids), mover -> firm edges (w_ptr: Wm + 1, w_idx:
+ * firm ids; the same edge set), firm pairs (pf, pg), targets u_tgt / u_mask (F)
+ * and y_tgt (npairs), parameters theta.  loss = mean over masked firms of
+ * (u - u_tgt)^2 + mean binary cross-entropy of sigmoid(t) against y_tgt; grad
+ * (kNumParams), u_out (F), t_out (npairs) optional.  Returns 1 on success. */
+int xfe_lp_gnn_eval(int F, int Wm, const float* xf, const float* xw, const int* f_ptr, const int* f_idx,
+                    const int* w_ptr, const int* w_idx, int npairs, const int* pf, const int* pg,
+                    const float* u_tgt, const float* u_mask, const float* y_tgt, const float* theta,
+                    double* loss, float* grad, float* u_out, float* t_out);
+
 /* Frees the calling thread's device workspace (it otherwise lives until the
  * process ends).  No counterpart in the reference. */
 void xfe_release_workspace(void);

@@ -968,3 +968,36 @@ def test_absorb_weighted_closed_form():
     assert np.abs(D.T @ (w[:, None] * Vo)).max() < 1e-9
     Vo2, _ = oracle.absorb(Vo, gids, G, weights=w, tol=1e-14, tolerance_mode=0)
     assert np.abs(Vo2 - Vo).max() < 1e-10
+
+
+# ---- the learned preconditioner's network (oracle/learned.py) ----
+
+def test_lp_gnn_gradient_finite_differences():
+    """The oracle's backward pass against central finite differences of its
+    forward pass (exact products, float64), on a random mobility graph: every
+    parameter block is probed."""
+    from oracle import learned as L
+    g = L.random_graph(3, F=30, Wm=40)
+    th = L.init_params(1).astype(np.float64)
+    _, grad, _ = L.forward_backward(th, g, bf16_products=False)
+    rng = np.random.default_rng(0)
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in L.LAYOUT])
+    probes = [int(rng.integers(offs[k], offs[k + 1])) for k in range(len(L.LAYOUT)) for _ in range(3)]
+    worst = 0.0
+    for i in probes:
+        h = 1e-6
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (L.forward_backward(tp, g, False, False)[0] - L.forward_backward(tm, g, False, False)[0]) / (2 * h)
+        worst = max(worst, abs(fd - grad[i]) / (abs(fd) + abs(grad[i]) + 1e-7))
+    assert worst < 1e-5, worst
+
+
+def test_lp_gnn_bf16_rounding():
+    """bf16() rounds to nearest even on the 16 dropped bits (known answers)."""
+    from oracle import learned as L
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 3.0e-3], dtype=np.float32)
+    got = L.bf16(x)
+    want = np.array([1.0, 1.0, 1.0 + 2 ** -6, 1.0, -2.5, 197 * 2.0 ** -16], dtype=np.float64)  # 3e-3 -> 197 x 2^-16
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9)

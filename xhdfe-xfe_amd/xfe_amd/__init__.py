@@ -74,7 +74,8 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_absorb_group_individual",
            "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort",
            "xfe_articulation_points", "xfe_akm_leave_out_set", "xfe_build_indexer",
-           "xfe_build_indexer_device", "xfe_weight_sums", "xfe_build_indexer_device_noalloc")
+           "xfe_build_indexer_device", "xfe_weight_sums", "xfe_build_indexer_device_noalloc",
+           "xfe_lp_param_count", "xfe_lp_gnn_eval")
 
 _lib = None
 
@@ -599,3 +600,33 @@ def profile_get():
     ms, n, b = _d(), ctypes.c_longlong(), _d()
     lib().xfe_profile_get(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b))
     return ms.value, n.value, b.value
+
+
+def lp_param_count():
+    return int(lib().xfe_lp_param_count())
+
+
+def lp_gnn_eval(g, theta):
+    """The learned preconditioner's network on the device: (loss, grad, u, t)
+    for a graph dict (xf, xw, f_ptr, f_idx, w_ptr, w_idx, pf, pg, u_tgt,
+    u_mask, y_tgt) and float32 parameters."""
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    xf, xw = f32(g["xf"]), f32(g["xw"])
+    F, Wm = xf.shape[0], xw.shape[0]
+    fp, fi, wp, wi = (_i32(g[k]) for k in ("f_ptr", "f_idx", "w_ptr", "w_idx"))
+    pf, pg = _i32(g["pf"]), _i32(g["pg"])
+    ut, um, yt, th = f32(g["u_tgt"]), f32(g["u_mask"]), f32(g["y_tgt"]), f32(theta)
+    npair = pf.size
+    loss = _d()
+    grad = np.empty(lp_param_count(), dtype=np.float32)
+    u = np.empty(max(F, 1), dtype=np.float32)
+    t = np.empty(max(npair, 1), dtype=np.float32)
+    L = lib()
+    L.xfe_lp_gnn_eval.argtypes = [_i, _i] + [_p] * 6 + [_i] + [_p] * 2 + [_p] * 4 + [ctypes.POINTER(_d)] + [_p] * 3
+    ok = L.xfe_lp_gnn_eval(F, Wm, _ptr(xf), _ptr(xw), _ptr(fp), _ptr(fi), _ptr(wp), _ptr(wi), npair, _ptr(pf),
+                           _ptr(pg), _ptr(ut), _ptr(um), _ptr(yt), _ptr(th), ctypes.byref(loss), _ptr(grad),
+                           _ptr(u), _ptr(t))
+    if not ok:
+        raise XfeError(last_error())
+    return loss.value, grad, u[:F], t[:npair]
+
