@@ -8,7 +8,10 @@ not timed); one step is the full within transform of [y | X] (C = 5 columns,
 3-way FE, tolerance 1e-8) starting from the raw columns already resident in
 HBM, through the level sort, the solve and the scatter back to the caller's
 layout.  The timed method is the mixed-precision absorber (precision 1: FP32
-Schur-complement PCG under FP64 refinement, final transform in FP64); the
+Schur-complement PCG with FP64 residual replacement, final transform in FP64)
+with the learned preconditioner (the trained graph network's aggregation of
+firms into blocks solved exactly); the same method with Jacobi and with the
+coupling-strength aggregation is timed beside it, and the
 plain FP64 absorber (precision 0: Irons-Tuck accelerated Gauss-Seidel with the
 reference's gates and stopping rule) is timed in the same run, and the two
 results are compared.
@@ -427,6 +430,10 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="panel scale (1.0 = 56M obs)")
     ap.add_argument("--precision", type=int, default=1, help="1 FP32 under FP64 refinement, 0 FP64")
     ap.add_argument("--plain-steps", type=int, default=1, help="timed steps of the plain FP64 absorber")
+    ap.add_argument("--preconditioner", type=int, default=2,
+                    help="precision 1: 0 Jacobi, 1 strength aggregates, 2 learned aggregates (default)")
+    ap.add_argument("--ablation-steps", type=int, default=2,
+                    help="timed steps of the mixed method with the other preconditioners")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--path", choices=("absorb", "akm", "slope", "recover", "gi", "leaveout"), default="absorb",
                     help="absorb: the 56M-obs absorber (default); akm: the KSS leverage solver; "
@@ -492,17 +499,20 @@ def main():
     del X, y
     torch.cuda.synchronize()
 
-    def step(precision):
+    lp_params = xfe_amd.default_lp_params()
+
+    def step(precision, pc=0):
         V.copy_(V0)
         return xfe_amd.absorb_device(V.data_ptr(), n, C, ld, ids.data_ptr(), G, tol=1e-8,
-                                     tolerance_mode=1, precision=precision)
+                                     tolerance_mode=1, precision=precision, preconditioner=pc,
+                                     lp_params=lp_params if pc == 2 else None)
 
-    def timed(precision, steps, warmup, profile):
+    def timed(precision, steps, warmup, profile, pc=0):
         for w in range(warmup):
             t0 = time.perf_counter()
-            info = step(precision)
+            info = step(precision, pc)
             torch.cuda.synchronize()
-            log(f"precision {precision} warmup {w}: {time.perf_counter() - t0:.3f} s, "
+            log(f"precision {precision} preconditioner {pc} warmup {w}: {time.perf_counter() - t0:.3f} s, "
                 f"iterations {info['iterations']}, converged {info['converged']}")
         if profile:
             xfe_amd.profile_reset()
@@ -511,9 +521,10 @@ def main():
         t0 = time.perf_counter()
         infos = []
         for k in range(steps):
-            infos.append(step(precision))
-            log(f"precision {precision} step {k}: iterations {infos[-1]['iterations']} "
-                f"refinements {infos[-1]['refinements']} converged {infos[-1]['converged']}")
+            infos.append(step(precision, pc))
+            log(f"precision {precision} preconditioner {pc} step {k}: iterations {infos[-1]['iterations']} "
+                f"refinements {infos[-1]['refinements']} converged {infos[-1]['converged']} "
+                f"precond setup {infos[-1]['precond_ms']:.1f} ms")
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         if profile:
@@ -537,7 +548,20 @@ def main():
         log(f"FP64 reference (tol 1e-14): {ref_info['iterations']} iterations, "
             f"certificate {ref_info['abs_residual_rel']:.3e}")
 
-    sec, info = timed(args.precision, args.steps, args.warmup, True)
+    # the mixed method with the other preconditioners (the method without the
+    # learned part: Jacobi; and the aggregation by coupling strength alone)
+    ablation = None
+    if args.precision == 1 and args.ablation_steps > 0:
+        ablation = {}
+        for pc, name in ((0, "mixed_jacobi"), (1, "mixed_strength_aggregates")):
+            if pc == args.preconditioner:
+                continue
+            a_s, a_info = timed(1, args.ablation_steps, 1, False, pc)
+            ablation[name] = {"s": round(a_s, 4), "iterations": a_info["iterations"],
+                              "refinements": a_info["refinements"], "precond_setup_ms": round(a_info["precond_ms"], 2)}
+            log(f"ablation {name}: {a_s:.4f} s/step, {a_info['iterations']} inner iterations")
+
+    sec, info = timed(args.precision, args.steps, args.warmup, True, args.preconditioner)
     pass_ms, launches, pass_bytes = xfe_amd.profile_get()
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     kernel, traffic_file = KERNELS[args.precision]
@@ -589,9 +613,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if args.precision == 1 else "f64",
         "data": "synthetic",
-        "method": ("mixed precision: FP32 Jacobi-PCG on the Schur complement of the worker FE under FP64 "
-                   "iterative refinement (no learned preconditioner)") if args.precision == 1
-                  else "plain FP64: Irons-Tuck accelerated Gauss-Seidel",
+        "method": (("mixed precision: FP32 PCG on the Schur complement of the worker FE with FP64 residual "
+                    "replacement; preconditioner: ")
+                   + {0: "Jacobi", 1: "aggregate blocks from the coupling strengths",
+                      2: "learned (graph-network aggregation into exactly solved firm blocks)"}[args.preconditioner])
+                  if args.precision == 1 else "plain FP64: Irons-Tuck accelerated Gauss-Seidel",
+        "preconditioner_used": info.get("preconditioner_used"),
+        "precond_fallback": info.get("precond_fallback"),
+        "precond_setup_ms": round(info.get("precond_ms", 0.0), 2),
+        "precond_blocked_firms": info.get("precond_blocks"),
+        "ablation": ablation,
         "config": {"workload": "akm_56m_3way", "obs": n, "workers": G[0], "firms": G[1],
                    "years": G[2], "columns": C, "tol": 1e-8,
                    "tolerance_mode": "reghdfe-comparable", "singletons_dropped": dropped},

@@ -59,6 +59,14 @@ typedef struct {
     double refine_tol;          /* precision 1: FP64 refinement target, ||b - S beta|| / ||b|| of the
                                    Schur system over the non-primary levels; <= 0 selects
                                    min(3e-13, 3e-5 tol) */
+    int preconditioner;         /* precision 1: 0 Jacobi; 1 aggregate blocks from the firm coupling
+                                   strengths; 2 the learned preconditioner (aggregates from the graph
+                                   network's scores, lp_params) -- the exact inverse of each
+                                   aggregate's block of the Schur system on the firm levels, Jacobi on
+                                   the others.  Convergence is always decided on the FP64 residual;
+                                   a stall hands the solve to Jacobi-PCG (precond_fallback). */
+    const float* lp_params;     /* preconditioner 2: the network's parameters (host, float32,
+                                   xfe_lp_param_count() values) */
 } xfe_options;
 
 /* Outputs of one absorption (the fields of hdfe::detail::AbsorptionResult,
@@ -88,8 +96,11 @@ typedef struct {
     int precision_used;
     int precision_fallback;
     double refine_target;       /* precision 1: the FP64 refinement target that was applied */
-    int preconditioner_used;    /* precision 1: 0 Jacobi, 1 the learned two-level preconditioner */
-    int precond_fallback;       /* the learned preconditioner stalled and Jacobi-PCG finished the solve */
+    int preconditioner_used;    /* precision 1: the options.preconditioner that ran (0 when no firm pair
+                                   exists or the path has no segment form) */
+    int precond_fallback;       /* the block preconditioner stalled and Jacobi-PCG finished the solve */
+    int precond_blocks;         /* firms inside aggregate blocks */
+    double precond_ms;          /* set-up time of the preconditioner (pairs, network, aggregation, blocks) */
 } xfe_absorb_result;
 
 /* The certificate on its own (certify_cuda_absorption_result,
@@ -442,28 +453,89 @@ int xfe_akm_max_lanes(void);
  * by the context, or NULL (then the caller uses its own buffer). */
 double* xfe_akm_pack_buffer(xfe_akm_context* ctx, int nb);
 
-/* ---- Learned two-level preconditioner (no counterpart in the reference, whose
+/* ---- Learned preconditioner (no counterpart in the reference, whose
  * preconditioner for the FE normal equations is the randomized approximate
- * Cholesky of src/schwarz_demean.cpp:100-133).  A graph network over the
+ * Cholesky of src/schwarz_demean.cpp:100-133).  On the firm levels the Schur
+ * system of the mixed-precision absorber is a weighted graph Laplacian whose
+ * slowest modes (under Jacobi scaling) live on small groups of firms coupled
+ * to each other far more than to the rest.  A graph network over the
  * worker-firm mobility graph (firms; movers = workers with segments at two or
- * more firms; one edge per mover segment), width 64, three message-passing
- * layers, BF16 MFMA with FP32 accumulation, emits per firm a log per-level
- * scaling u_f (the prolongation weight of the firm in its coarse aggregate) and
- * per firm pair of one mover a score t_fg (the matching strength of the coarse
- * aggregation).  The parameter vector is float32, xfe_lp_param_count() long. */
+ * more firms; one edge per (mover, firm)), width 64, three message-passing
+ * layers, BF16 MFMA with FP32 accumulation, emits a per-level score u_f and a
+ * score t_fg per coupled firm pair; the aggregates are formed by a parallel
+ * handshake over t_fg + (u_f + u_g) / 2 (at most 16 firms each), and the
+ * preconditioner solves each aggregate's block exactly (xfe_options
+ * .preconditioner = 2).  The parameter vector is float32, xfe_lp_param_count()
+ * long (the package ships trained values). */
 int xfe_lp_param_count(void);
 
 /* One evaluation of the network, its loss and gradient (host buffers): firm
  * features xf (F x 8), mover features xw (Wm x 8), firm -> mover edges
  * (f_ptr: F + 1, f_idx: mover ids), mover -> firm edges (w_ptr: Wm + 1, w_idx:
- * firm ids; the same edge set), firm pairs (pf, pg), targets u_tgt / u_mask (F)
- * and y_tgt (npairs), parameters theta.  loss = mean over masked firms of
- * (u - u_tgt)^2 + mean binary cross-entropy of sigmoid(t) against y_tgt; grad
- * (kNumParams), u_out (F), t_out (npairs) optional.  Returns 1 on success. */
+ * firm ids; the same edge set), firm pairs (pf, pg) with one feature each (pq,
+ * may be NULL: 0), targets u_tgt / u_mask (F) and y_tgt (npairs), parameters
+ * theta.  loss = mean over masked firms of (u - u_tgt)^2 + mean binary
+ * cross-entropy of sigmoid(t) against y_tgt; grad (kNumParams), u_out (F),
+ * t_out (npairs) optional.  Returns 1 on success. */
 int xfe_lp_gnn_eval(int F, int Wm, const float* xf, const float* xw, const int* f_ptr, const int* f_idx,
-                    const int* w_ptr, const int* w_idx, int npairs, const int* pf, const int* pg,
+                    const int* w_ptr, const int* w_idx, int npairs, const int* pf, const int* pg, const float* pq,
                     const float* u_tgt, const float* u_mask, const float* y_tgt, const float* theta,
                     double* loss, float* grad, float* u_out, float* t_out);
+
+/* Training data from the absorber itself: xfe_lp_probe runs the
+ * mixed-precision absorber (Jacobi-PCG, tol 1e-8) on [y | X] (host, n rows, X
+ * column-major) with 2 or 3 unweighted dimensions (the first is the
+ * eliminated one), records the FP64 iterate at inner iteration probe_iter, and
+ * keeps on the host: per firm and column the error of that iterate against the
+ * final solution, the firm pairs (pf < pg, coupling W, strength pq), the exact
+ * firm diagonal, and the network's graph and features as the preconditioner
+ * builds them.  xfe_lp_probe_fetch copies them into caller buffers of the
+ * sizes reported (xf F x 8, xw Wm x 8, f_ptr F + 1, f_idx E, w_ptr Wm + 1,
+ * w_idx E, pf / pg / pq / W U, dex F, err F x C firm-major); NULL skips one. */
+typedef struct {
+    int F, Wm, E, U, C;
+    int inner_iterations;
+    int converged;
+} xfe_lp_panel_sizes;
+
+int xfe_lp_probe(const double* y, const double* X, int n, int k, const xfe_fe_input* fes, int num_fes,
+                 int probe_iter, xfe_lp_panel_sizes* sizes);
+int xfe_lp_probe_fetch(float* xf, float* xw, int* f_ptr, int* f_idx, int* w_ptr, int* w_idx, int* pf, int* pg,
+                       float* pq, double* W, double* dex, double* err);
+
+/* The aggregation and block solves of the preconditioner on given pairs (a
+ * unit of the preconditioner, for tests and for callers with their own
+ * scores): firm pairs pf < pg (sorted, unique) with couplings W, the firm
+ * diagonal dex, pair scores; the handshake aggregation (cap firms per
+ * aggregate, `rounds` rounds) into agg_out (the root firm of each firm), and
+ * z_out = inverse of each aggregate's block times r on the firms of
+ * aggregates of two or more firms, 0 elsewhere (r, z: F x C level-major,
+ * float32).  Returns 1 on success. */
+int xfe_lp_blocks_apply(int F, int U, const int* pf, const int* pg, const double* W, const double* dex,
+                        const float* score, int cap, int rounds, int C, const float* r, int* agg_out,
+                        float* z_out);
+
+/* Training on the device: Adam (lr, beta 0.9 / 0.999) over `steps` steps, one
+ * panel per step in turn, each step one forward and backward pass of the
+ * network (the loss of xfe_lp_gnn_eval).  theta: parameters in/out (host);
+ * loss_hist: the loss of every step (may be NULL).  Returns 1 on success. */
+typedef struct {
+    int F, Wm, U;
+    const float* xf;
+    const float* xw;
+    const int* f_ptr;
+    const int* f_idx;
+    const int* w_ptr;
+    const int* w_idx;
+    const int* pf;
+    const int* pg;
+    const float* pq;
+    const float* u_tgt;
+    const float* u_mask;
+    const float* y_tgt;
+} xfe_lp_graph;
+
+int xfe_lp_train(int npanels, const xfe_lp_graph* panels, int steps, float lr, float* theta, double* loss_hist);
 
 /* Frees the calling thread's device workspace (it otherwise lives until the
  * process ends).  No counterpart in the reference. */

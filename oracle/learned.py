@@ -1,20 +1,23 @@
 """Test infrastructure only: a numpy restatement of the learned preconditioner's
 graph network (forward, loss and backward), the checker of the HIP kernels in
-xhdfe-xfe_amd/csrc/learned.hip.  Nothing in the product imports it.
+xhdfe-xfe_amd/csrc/lp_gnn.hip.  Nothing in the product imports it.
 
 The reference has no learned component: its preconditioner for the FE normal
 equations is the randomized approximate Cholesky of src/schwarz_demean.cpp:100-133
 inside the block PCG of :58-415.  The network here is this build's own design:
 
 Graph (the worker-firm mobility graph): firms, movers (workers with two or more
-firm segments), one edge per mover segment.  Message passing, width H = 64:
+firm segments), one edge per (mover, firm).  Message passing, width H = 64:
 
     hf = relu(xf Wf0 + bf0),  hw = relu(xw Ww0 + bw0)              (inputs: 8 features each)
     for l = 1..3:
         hw = relu([hw | mean_{f in N(w)} hf] Wwl + bwl)             (Wwl: 128 x 64)
-        hf = relu([hf | mean_{w in N(f)} hw] Wfl + bfl)             (Wfl: 128 x 64)
-    u_f  = hf . wa + ba                                             (log per-level scaling)
-    t_fg = sum_k hf_k hg_k we_k + be                                (pair score, firms f, g of one mover)
+        hf = relu([hf | mean_{w in N256(f)} hw] Wfl + bfl)          (Wfl: 128 x 64)
+
+N256(f): the first 256 movers of f's list (ascending ids), a fixed-size
+neighbour sample that keeps one firm's mean from serializing a launch.
+    u_f  = hf . wa + ba                                             (per-level score)
+    t_fg = sum_k hf_k hg_k we_k + be + ws q_fg                      (pair score; q_fg the pair's feature)
 
 Loss: mean over firms with a target of (u_f - u*_f)^2, plus the mean binary
 cross-entropy of sigmoid(t_fg) against y*_fg over the firm pairs.
@@ -39,7 +42,7 @@ LAYERS = 3
 LAYOUT = ([("Wf0", (NF_IN, H)), ("bf0", (H,)), ("Ww0", (NW_IN, H)), ("bw0", (H,))]
           + [item for l in range(1, LAYERS + 1)
              for item in ((f"Ww{l}", (2 * H, H)), (f"bw{l}", (H,)), (f"Wf{l}", (2 * H, H)), (f"bf{l}", (H,)))]
-          + [("wa", (H,)), ("ba", (1,)), ("we", (H,)), ("be", (1,))])
+          + [("wa", (H,)), ("ba", (1,)), ("we", (H,)), ("be", (1,)), ("ws", (1,))])
 NPARAM = int(sum(np.prod(s) for _, s in LAYOUT))
 
 
@@ -83,22 +86,35 @@ def _mm(a, b, rb):
     return (bf16(a) @ bf16(b)) if rb else (a @ b)
 
 
-def _mean_gather(ptr, idx, src):
-    """out[v] = mean of src[idx[ptr[v]:ptr[v+1]]] (0 for no neighbours)."""
+FIRM_CAP = 256  # a firm's mean over movers takes the first FIRM_CAP of its list
+
+
+def _capped(ptr, cap):
+    """Rows, positions kept and capped degrees of a CSR list."""
+    deg = np.diff(ptr)
     n = ptr.size - 1
-    deg = np.diff(ptr)
     rows = np.repeat(np.arange(n), deg)
+    pos = np.arange(rows.size) - np.repeat(ptr[:-1], deg)
+    keep = pos < cap
+    return rows, keep, np.minimum(deg, cap)
+
+
+def _mean_gather(ptr, idx, src, cap=None):
+    """out[v] = mean of src over the first cap entries of idx[ptr[v]:ptr[v+1]]
+    (all without cap; 0 for no neighbours)."""
+    n = ptr.size - 1
+    rows, keep, d = _capped(np.asarray(ptr, np.int64), cap if cap is not None else np.iinfo(np.int64).max)
     out = np.zeros((n, src.shape[1]))
-    np.add.at(out, rows, src[idx])
-    return out / np.maximum(deg, 1)[:, None]
+    np.add.at(out, rows[keep], src[np.asarray(idx)[keep]])
+    return out / np.maximum(d, 1)[:, None]
 
 
-def _mean_scatter(ptr, idx, g, nsrc):
+def _mean_scatter(ptr, idx, g, nsrc, cap=None):
     """Transpose of _mean_gather: d src from d out."""
-    deg = np.diff(ptr)
-    rows = np.repeat(np.arange(ptr.size - 1), deg)
+    rows, keep, d = _capped(np.asarray(ptr, np.int64), cap if cap is not None else np.iinfo(np.int64).max)
     out = np.zeros((nsrc, g.shape[1]))
-    np.add.at(out, idx, g[rows] / np.maximum(deg, 1)[rows, None])
+    r = rows[keep]
+    np.add.at(out, np.asarray(idx)[keep], g[r] / np.maximum(d, 1)[r, None])
     return out
 
 
@@ -123,7 +139,7 @@ def forward_backward(theta, g, bf16_products=True, want_grad=True):
         inw = np.hstack([hw, aw])
         zw = _mm(inw, P[f"Ww{l}"], rb) + P[f"bw{l}"]
         hw_new = np.maximum(zw, 0.0)
-        af = _mean_gather(g["f_ptr"], g["f_idx"], hw_new)
+        af = _mean_gather(g["f_ptr"], g["f_idx"], hw_new, FIRM_CAP)
         inf = np.hstack([hf, af])
         zf = _mm(inf, P[f"Wf{l}"], rb) + P[f"bf{l}"]
         hf_new = np.maximum(zf, 0.0)
@@ -132,7 +148,8 @@ def forward_backward(theta, g, bf16_products=True, want_grad=True):
     u = hf @ P["wa"] + P["ba"][0]
     pf, pg = np.asarray(g["pf"]), np.asarray(g["pg"])
     prod = hf[pf] * hf[pg]
-    t = prod @ P["we"] + P["be"][0]
+    pq = np.asarray(g.get("pq", np.zeros(pf.size)), np.float64)
+    t = prod @ P["we"] + P["be"][0] + P["ws"][0] * pq
     m = np.asarray(g["u_mask"], np.float64)
     nm = max(m.sum(), 1.0)
     du = 2.0 * m * (u - g["u_tgt"]) / nm
@@ -151,6 +168,7 @@ def forward_backward(theta, g, bf16_products=True, want_grad=True):
     G["ba"] = np.array([du.sum()])
     G["we"] = prod.T @ dt
     G["be"] = np.array([dt.sum()])
+    G["ws"] = np.array([(dt * pq).sum()])
     dhf = np.outer(du, P["wa"])
     dprod = np.outer(dt, P["we"])
     np.add.at(dhf, pf, dprod * hf[pg])
@@ -163,7 +181,7 @@ def forward_backward(theta, g, bf16_products=True, want_grad=True):
         G[f"bf{l}"] = dzf.sum(axis=0)
         dinf = _mm(dzf, P[f"Wf{l}"].T, rb)
         dhf_prev = dinf[:, :H]
-        dhw_new = dhw + _mean_scatter(g["f_ptr"], g["f_idx"], dinf[:, H:], Wm)
+        dhw_new = dhw + _mean_scatter(g["f_ptr"], g["f_idx"], dinf[:, H:], Wm, FIRM_CAP)
         dzw = dhw_new * (zw > 0)
         G[f"Ww{l}"] = _mm(inw.T, dzw, rb)
         G[f"bw{l}"] = dzw.sum(axis=0)
@@ -195,6 +213,7 @@ def random_graph(seed, F=40, Wm=60, deg=(2, 4)):
     g["u_tgt"] = rng.standard_normal(F)
     g["u_mask"] = (rng.random(F) < 0.8).astype(np.float64)
     g["y_tgt"] = rng.random(g["pf"].size)
+    g["pq"] = rng.random(g["pf"].size).astype(np.float32)
     return g
 
 

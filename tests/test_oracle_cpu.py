@@ -972,12 +972,16 @@ def test_absorb_weighted_closed_form():
 
 # ---- the learned preconditioner's network (oracle/learned.py) ----
 
-def test_lp_gnn_gradient_finite_differences():
+@pytest.mark.parametrize("F,Wm", [(30, 40), (3, 400)])
+def test_lp_gnn_gradient_finite_differences(F, Wm):
     """The oracle's backward pass against central finite differences of its
     forward pass (exact products, float64), on a random mobility graph: every
-    parameter block is probed."""
+    parameter block is probed.  With 3 firms and 400 movers every firm has
+    more movers than the firm mean's cap of 256, so the capped mean and its
+    transpose are checked too."""
     from oracle import learned as L
-    g = L.random_graph(3, F=30, Wm=40)
+    g = L.random_graph(3, F=F, Wm=Wm, deg=(2, 3) if F < 4 else (2, 4))
+    assert (F >= 4) or np.diff(g["f_ptr"]).min() > L.FIRM_CAP
     th = L.init_params(1).astype(np.float64)
     _, grad, _ = L.forward_backward(th, g, bf16_products=False)
     rng = np.random.default_rng(0)
@@ -1001,3 +1005,101 @@ def test_lp_gnn_bf16_rounding():
     got = L.bf16(x)
     want = np.array([1.0, 1.0, 1.0 + 2 ** -6, 1.0, -2.5, 197 * 2.0 ** -16], dtype=np.float64)  # 3e-3 -> 197 x 2^-16
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-9)
+
+
+# ---- the aggregate-block preconditioner (oracle/precond.py) ----
+
+def _tiny_akm(seed, workers=60, firms=12, years=6, p_move=0.3):
+    rng = np.random.default_rng(seed)
+    T = rng.integers(2, years + 1, workers)
+    w = np.repeat(np.arange(workers), T)
+    y = np.concatenate([np.arange(t) for t in T])
+    f = np.empty(w.size, np.int64)
+    cur = rng.integers(0, firms, workers)
+    for i in range(w.size):
+        if i > 0 and w[i] == w[i - 1] and rng.random() < p_move:
+            cur[w[i]] = rng.integers(0, firms)
+        f[i] = cur[w[i]]
+    return w, f, y
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_precond_pairs_equal_dense_schur(seed):
+    """The firm block of the Schur complement of the worker dimension from the
+    sparse formula (pairs_dense) equals the dense D_R' M_p D_R: the diagonal,
+    every coupled pair, and zeros elsewhere."""
+    from oracle import precond as P
+    w, f, y = _tiny_akm(seed)
+    F = f.max() + 1
+    pf, pg, W, dex = P.pairs_dense(w, f, F=F)
+    S = P.schur_dense(w, f, y)[:F, :F]
+    assert np.abs(np.diag(S) - dex).max() < 1e-10
+    off = np.zeros((F, F))
+    off[pf, pg] = -W
+    off = off + off.T
+    mask = ~np.eye(F, dtype=bool)
+    assert np.abs(S[mask] - off[mask]).max() < 1e-10
+    assert (pf < pg).all() and (np.diff(pf.astype(np.int64) * F + pg) > 0).all()
+
+
+def test_precond_handshake_rules():
+    """One round merges exactly the mutual best pairs (brute force over the
+    firms); every aggregate respects the cap and is connected by pairs."""
+    from oracle import precond as P
+    rng = np.random.default_rng(5)
+    F = 40
+    cand = {(int(a), int(b)) for a, b in rng.integers(0, F, (120, 2)) if a < b}
+    pf, pg = map(np.array, zip(*sorted(cand)))
+    score = rng.random(pf.size).astype(np.float32)
+    agg1 = P.handshake(pf, pg, score, F, cap=16, rounds=1)
+    best = {}
+    for p in range(pf.size):
+        for a in (pf[p], pg[p]):
+            k = (score[p], -p)
+            if a not in best or k > best[a][0]:
+                best[a] = (k, p)
+    expect = np.arange(F)
+    for a, (_, p) in best.items():
+        b = pg[p] if pf[p] == a else pf[p]
+        if best.get(b, (None, -1))[1] == p and a < b:
+            expect[b] = a
+    assert (agg1 == expect).all()
+    for cap in (2, 4, 16):
+        agg = P.handshake(pf, pg, score, F, cap=cap, rounds=8)
+        for r in np.unique(agg):
+            mem = set(np.flatnonzero(agg == r))
+            assert len(mem) <= cap
+            seen, todo = {min(mem)}, [min(mem)]
+            while todo:
+                x = todo.pop()
+                for a, b in zip(pf, pg):
+                    for u, v in ((a, b), (b, a)):
+                        if u == x and v in mem and v not in seen:
+                            seen.add(v)
+                            todo.append(v)
+            assert seen == mem
+
+
+def test_precond_block_inverse():
+    """A non-singular block: the plain inverse.  The Laplacian of an isolated
+    group (constant null vector): a symmetric generalized inverse, B X B = B."""
+    from oracle import precond as P
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((6, 6))
+    B = A @ A.T + 0.5 * np.eye(6)
+    assert np.abs(P.block_inverse(B) - np.linalg.inv(B)).max() < 1e-10 * np.abs(np.linalg.inv(B)).max()
+    Wm = rng.random((5, 5))
+    Wm = np.triu(Wm, 1)
+    Wm = Wm + Wm.T
+    Lap = np.diag(Wm.sum(1)) - Wm
+    X = P.block_inverse(Lap)
+    assert np.abs(X - X.T).max() < 1e-10
+    assert np.abs(Lap @ X @ Lap - Lap).max() < 1e-9 * np.abs(Lap).max()
+    assert np.linalg.eigvalsh(X).min() > -1e-10
+
+
+def test_lp_trained_parameters_shipped():
+    """The package's trained network parameters load and match the network's size."""
+    import xfe_amd
+    th = xfe_amd.default_lp_params()
+    assert th.size == xfe_amd.lp_param_count() and np.isfinite(th).all()

@@ -30,6 +30,7 @@
 
 #include "absorb_kernels.h"
 #include "common.h"
+#include "lp_precond.h"
 #include "xfe_hip.h"
 
 namespace xfe {
@@ -130,6 +131,8 @@ struct Workspace {
     // FE recovery: the untransformed column and the level coefficients
     DevBuf<double> rec_p, rec_contrib;
     DevBuf<double> rec_alpha[3];
+    // aggregate-block preconditioner of the mixed-precision solver
+    xfe::lp::AggPrecond agg;
     // the last mixed-precision solve of this workspace: primary dimension and
     // where each other dimension's coefficients sit in f_beta (level-major, C wide)
     struct {
@@ -169,6 +172,7 @@ struct Workspace {
         for (int k = 0; k < 2; ++k) { seg_bk[k].release(); f_bv[k].release(); d_bv[k].release(); }
         rec_p.release(); rec_contrib.release();
         for (auto& b : rec_alpha) b.release();
+        agg.release();
         gi_iptr.release(); gi_igroup.release(); gi_members.release(); gi_scale.release(); gi_denom.release();
         fast_rec.used = false;
         if (host_scalars) (void)hipHostFree(host_scalars);
@@ -796,6 +800,16 @@ struct FastSolver {
     bool seg = false;
     int k1 = 0, k2 = -1, M = 0, Nw = 0, G2 = 0, seg_grid_w = 1, seg_grid_f = 1;
     bool empty_firms = true;  // some level of r1 has no segment
+    // aggregate-block preconditioner on the firm levels (segment form only):
+    // precond 0 Jacobi, 1 aggregates from the coupling strengths, 2 from the
+    // learned network's scores (params: its parameter vector, device)
+    int precond = 0;
+    const float* lp_params = nullptr;
+    bool blocks = false;
+    bool block_fallback = false;
+    lp::BlockView bview;
+    double precond_ms = 0.0;
+    lp::ProbeState* probe = nullptr;  // training-data probe (xfe_lp_probe)
 
     explicit FastSolver(Solver& sv_) : sv(sv_), ws(sv_.ws), s(sv_.s) {}
 
@@ -1053,9 +1067,96 @@ struct FastSolver {
             XFE_CHECK(hipMemcpyAsync(ws.f_counts.p + off[k], ws.wsum[rdim[k]].p,
                                      sizeof(double) * sv.G[rdim[k]], hipMemcpyDeviceToDevice, s));
         }
+        if (seg && precond > 0) setup_blocks();
         XFE_CHECK(launch_diag_invert(ws.f_diag.p, ws.f_counts.p, ws.f_minv.p, N, grid_for(N), s));
         XFE_CHECK(hipMemsetAsync(ws.f_beta.p, 0, sizeof(double) * NC, s));
         return true;
+    }
+
+    // The aggregate blocks: firm pairs and their strengths from the worker-major
+    // segments, the scores (the strengths, or the learned network's), the
+    // handshake aggregation and the block inverses; the firm part of the
+    // Jacobi diagonal becomes the exact diagonal (the per-segment formula
+    // misses the cross terms of a worker's returns to a firm).
+    void setup_blocks() {
+        auto t0 = std::chrono::steady_clock::now();
+        lp::AggPrecond& ag = ws.agg;
+        lp::SegView v;
+        v.Nw = Nw;
+        v.G1 = sv.G[rdim[k1]];
+        v.wseg = ws.p_wseg.p;
+        v.r1 = ws.p_r1.p;
+        v.len = ws.p_len.p;
+        v.inv_w = ws.d_invw.p;
+        v.diag = ws.f_diag.p + off[k1];
+        v.rows = ws.wsum[rdim[k1]].p;
+        if (!ag.build_pairs(v, s)) return;
+        const float* score = ag.st.p;
+        if (precond == 2) score = lp::learned_pair_scores(ag, v, lp_params, s);
+        const int cap = std::getenv("XFE_LP_CAP") ? std::atoi(std::getenv("XFE_LP_CAP")) : 16;
+        const int rounds = std::getenv("XFE_LP_ROUNDS") ? std::atoi(std::getenv("XFE_LP_ROUNDS")) : 6;
+        ag.aggregate(score, cap, rounds, s);
+        ag.build_blocks(s);
+        XFE_CHECK(hipMemcpyAsync(ws.f_diag.p + off[k1], ag.dex.p, sizeof(double) * v.G1, hipMemcpyDeviceToDevice, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        blocks = ag.nb > 0;
+        bview = ag.view();
+        precond_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    // Training data of the learned preconditioner: the error of the iterate
+    // recorded at probe_iter against the final solution on the firm levels,
+    // and the pairs, graph and features of this panel (host copies).
+    void record_probe() {
+        lp::ProbeState& P = *probe;
+        const int G1 = sv.G[rdim[k1]];
+        std::vector<double> fin(static_cast<size_t>(N) * C), itv(static_cast<size_t>(N) * C);
+        XFE_CHECK(hipMemcpyAsync(fin.data(), ws.f_beta.p, sizeof(double) * fin.size(), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipMemcpyAsync(itv.data(), P.iter.p, sizeof(double) * itv.size(), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        P.C = C;
+        P.F = G1;
+        P.inner = inner_total;
+        P.err.assign(static_cast<size_t>(G1) * C, 0.0);
+        for (int f = 0; f < G1; ++f)
+            for (int c = 0; c < C; ++c) {
+                const size_t j = static_cast<size_t>(off[k1] + f) * C + c;
+                P.err[static_cast<size_t>(f) * C + c] = fin[j] - itv[j];
+            }
+        lp::AggPrecond& ag = ws.agg;
+        lp::SegView v;
+        v.Nw = Nw;
+        v.G1 = G1;
+        v.wseg = ws.p_wseg.p;
+        v.r1 = ws.p_r1.p;
+        v.len = ws.p_len.p;
+        v.inv_w = ws.d_invw.p;
+        v.diag = ws.f_diag.p + off[k1];
+        v.rows = ws.wsum[rdim[k1]].p;
+        if (!ag.build_pairs(v, s)) throw Error("xfe_lp_probe: the panel has no firm pair");
+        lp::LearnedGraph gr;
+        gr.build(ag, v, s);
+        auto down = [&](auto& vec, const auto* src, size_t m) {
+            vec.resize(m);
+            if (m) XFE_CHECK(hipMemcpyAsync(vec.data(), src, sizeof(vec[0]) * m, hipMemcpyDeviceToHost, s));
+        };
+        P.Wm = gr.Wm;
+        P.E = gr.E;
+        P.U = ag.U;
+        down(P.xf, gr.xf.p, static_cast<size_t>(G1) * 8);
+        down(P.xw, gr.xw.p, static_cast<size_t>(gr.Wm) * 8);
+        down(P.f_ptr, gr.f_ptr.p, static_cast<size_t>(G1) + 1);
+        down(P.f_idx, gr.f_idx.p, static_cast<size_t>(gr.E));
+        down(P.w_ptr, gr.w_ptr.p, static_cast<size_t>(gr.Wm) + 1);
+        down(P.w_idx, gr.w_idx.p, static_cast<size_t>(gr.E));
+        down(P.pf, ag.pf.p, static_cast<size_t>(ag.U));
+        down(P.pg, ag.pg.p, static_cast<size_t>(ag.U));
+        down(P.pq, ag.st.p, static_cast<size_t>(ag.U));
+        down(P.W, ag.W.p, static_cast<size_t>(ag.U));
+        down(P.dex, ag.dex.p, static_cast<size_t>(G1));
+        XFE_CHECK(hipStreamSynchronize(s));
+        gr.release();
+        P.done = true;
     }
 
     template <typename T>
@@ -1254,13 +1355,16 @@ struct FastSolver {
         iters = 0;
         updates = 0;
         if (plan(true) == 0) return worst;
+        const int foff = seg ? off[k1] : 0;
         XFE_CHECK(launch_cg_reliable_reset(ws.f_b.p, d_rel, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p,
-                                           total, C, cg, part, g, s));
+                                           total, C, cg, part, g, blocks ? &bview : nullptr, foff, s));
         for (int it = 1; it <= max_inner; ++it) {
             apply32(ws.f_p.p, ws.f_q.p);
             XFE_CHECK(launch_cg_dot<float>(ws.f_p.p, ws.f_q.p, total, C, cg, part, g, s));
             XFE_CHECK(launch_cg_step_rel(ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p, ws.f_q.p, ws.f_minv.p, total, C, cg,
-                                         part, max_inner + 1, g, s));
+                                         part, max_inner + 1, g, blocks ? &bview : nullptr, foff, s));
+            if (probe && inner_total + it == probe->probe_iter)
+                XFE_CHECK(lp::launch_probe_iterate(ws.f_beta.p, ws.f_x.p, d_t, N, C, probe->iter.p, s));
             iters = it;
             if (it % kPoll != 0 && it != max_inner) continue;
             XFE_CHECK(hipMemcpyAsync(ws.host_scalars, cg + kCgCount, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1273,6 +1377,7 @@ struct FastSolver {
             ++updates;
             coldot<double>(ws.f_b.p, ws.f_b.p, hb);
             worst = 0.0;
+            bool stalled = false;
             for (int c = 0; c < C; ++c) {
                 const double tn = std::sqrt(hb[c]);
                 if (b0n[c] > 0.0) worst = std::max(worst, tn / b0n[c]);
@@ -1281,23 +1386,51 @@ struct FastSolver {
                     act[c] = false;
                 } else {
                     stall[c] = tn > 0.5 * t[c] ? stall[c] + 1 : 0;
-                    if (stall[c] >= 4) act[c] = false;
+                    if (stall[c] >= 4) {
+                        act[c] = false;
+                        stalled = true;
+                    }
                 }
                 prev[c] = t[c];
                 t[c] = tn > 0.0 ? tn : t[c];
             }
-            if (it == max_inner || plan(false) == 0) break;
+            bool fresh = false;
+            // XFE_LP_STALL_TEST: the host declares the block preconditioner stalled at
+            // its first residual replacement (tests of the fallback; never set otherwise)
+            if (blocks && std::getenv("XFE_LP_STALL_TEST")) stalled = true;
+            if (stalled && blocks) {
+                // the block preconditioner stalled: Jacobi-PCG continues from beta (fresh directions)
+                blocks = false;
+                block_fallback = true;
+                fresh = true;
+                for (int c = 0; c < C; ++c) {
+                    stall[c] = 0;
+                    act[c] = b0n[c] > 0.0 && t[c] > target * b0n[c];
+                }
+            }
+            if (it == max_inner || plan(fresh) == 0) break;
             XFE_CHECK(launch_cg_reliable_reset(ws.f_b.p, d_rel, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p,
-                                               total, C, cg, part, g, s));
+                                               total, C, cg, part, g, blocks ? &bview : nullptr, foff, s));
         }
         inner_total += iters;
         return worst;
     }
 };
 
+static int C_of(const Solver& sv) { return sv.C; }
+
 static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb_result& res) {
     FastSolver fs(sv);
+    fs.precond = opt.preconditioner;
+    fs.lp_params = opt.lp_params;
+    lp::ProbeState& probe = lp::probe_state();
+    if (probe.active) fs.precond = 0;
     if (!fs.setup()) return false;
+    if (probe.active && fs.seg) {
+        fs.probe = &probe;
+        probe.iter.ensure(static_cast<size_t>(fs.N) * C_of(sv));
+        XFE_CHECK(hipMemsetAsync(probe.iter.p, 0, sizeof(double) * static_cast<size_t>(fs.N) * C_of(sv), sv.s));
+    }
     const int C = sv.C;
     const double target = opt.refine_tol > 0.0 ? opt.refine_tol
                                                : std::min(3e-13, (opt.tol > 0.0 ? opt.tol : 1e-8) * 3e-5);
@@ -1321,6 +1454,7 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
     const bool converged = rel <= std::max(target, 1e-9);
     (void)tol_in;
     (void)max_inner;
+    if (fs.probe) fs.record_probe();
     fs.residual64(V, V);  // final within transform, in place
     auto& rec = sv.ws.fast_rec;
     rec.used = true;
@@ -1330,6 +1464,10 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
     for (int k = 0; k < 2; ++k) { rec.rdim[k] = fs.rdim[k]; rec.off[k] = fs.off[k]; }
     res.iterations = fs.inner_total;
     res.refinements = refinements;
+    res.preconditioner_used = fs.blocks || fs.block_fallback ? fs.precond : 0;
+    res.precond_fallback = fs.block_fallback ? 1 : 0;
+    res.precond_blocks = fs.blocks || fs.block_fallback ? sv.ws.agg.nb : 0;
+    res.precond_ms = fs.precond_ms;
     res.converged = converged ? 1 : 0;
     res.used_accel = 0;
     res.used_cg = 1;
@@ -2392,6 +2530,49 @@ int xfe_absorb(const double* y, const double* X, int n, int k, const xfe_fe_inpu
         set_error(e.what());
         return 0;
     }
+}
+
+int xfe_lp_probe(const double* y, const double* X, int n, int k, const xfe_fe_input* fes, int num_fes,
+                 int probe_iter, xfe_lp_panel_sizes* sizes) {
+    clear_error();
+    if (!sizes || probe_iter < 1 || num_fes < 2 || num_fes > 3 || k + 1 > 32) {
+        set_error("xfe_lp_probe: invalid arguments (2 or 3 dimensions, at most 32 columns, probe_iter >= 1)");
+        return 0;
+    }
+    for (int d = 0; d < num_fes; ++d)
+        if (fes[d].is_slope || fes[d].weight_sums) {
+            set_error("xfe_lp_probe: standard unweighted dimensions only");
+            return 0;
+        }
+    lp::ProbeState& P = lp::probe_state();
+    P.active = true;
+    P.done = false;
+    P.probe_iter = probe_iter;
+    xfe_options o;
+    std::memset(&o, 0, sizeof(o));
+    o.tol = 1e-8;
+    o.tolerance_mode = 1;
+    o.max_iter = 100000;
+    o.method = 1;
+    o.precision = 1;
+    xfe_absorb_result r;
+    std::vector<double> yo(static_cast<size_t>(n)), Xo(static_cast<size_t>(n) * k + 1);
+    const int ok = xfe_absorb(y, X, n, k, fes, num_fes, nullptr, nullptr, &o, &r, yo.data(), Xo.data());
+    P.active = false;
+    P.iter.release();
+    if (!ok) return 0;
+    if (!P.done) {
+        set_error("xfe_lp_probe: the panel did not take the mixed-precision segment path");
+        return 0;
+    }
+    sizes->F = P.F;
+    sizes->Wm = P.Wm;
+    sizes->E = P.E;
+    sizes->U = P.U;
+    sizes->C = P.C;
+    sizes->inner_iterations = P.inner;
+    sizes->converged = r.converged;
+    return 1;
 }
 
 int xfe_schwarz_demean(double* Y, double* X, int64_t n64, int k, const int32_t* const* fe,

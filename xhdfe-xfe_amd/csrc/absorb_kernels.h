@@ -6,6 +6,10 @@
 
 namespace xfe {
 
+namespace lp {
+struct BlockView;
+}
+
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 
@@ -213,16 +217,18 @@ hipError_t launch_refine_update(double* beta, const float* x, const double* inv_
 // Reliable-update CG (mixed-precision inner solve): an iteration's step with the
 // update test (r.r <= cg[kCgTol2 + c] sets cg[kCgUpd]) instead of the stopping
 // test, then p = z + beta p.
+// blk (may be null): the aggregate blocks of the firm levels, which start at
+// level firm_off: z = inv r on the blocked firms, minv r elsewhere.
 hipError_t launch_cg_step_rel(float* x, float* r, float* z, float* p, const float* q, const float* minv,
                               long long total, int C, double* cg, double* part, int max_iter, int grid,
-                              hipStream_t s);
+                              const lp::BlockView* blk, int firm_off, hipStream_t s);
 // Residual replacement: r = r64 / t_c, z = minv r, p = f_c (p - z_old) + z
 // (p = z for a restarted column, f_c = 0), x = 0; then per column rz, the
 // activity act_c and the update threshold tol2_c.
 // rel: [inv_t | f | tol2 | act], kMaxCgCols each.
 hipError_t launch_cg_reliable_reset(const double* r64, const double* rel, const float* minv, float* x, float* r,
                                     float* z, float* p, long long total, int C, double* cg, double* part, int grid,
-                                    hipStream_t s);
+                                    const lp::BlockView* blk, int firm_off, hipStream_t s);
 hipError_t launch_schur_diag(const int* tile_start, int num_tiles, const int* gid_p,
                              const int* gid_r, const double* inv_p, double* diag, int grid,
                              hipStream_t s);

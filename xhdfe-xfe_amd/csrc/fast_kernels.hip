@@ -17,6 +17,7 @@
 #include <cstdint>
 
 #include "absorb_kernels.h"
+#include "lp_precond.h"
 #include "wave_ops.h"
 
 namespace xfe {
@@ -727,6 +728,41 @@ hipError_t launch_pull(const T* S, int C, int c0, int cc, const int* order, cons
 template hipError_t launch_pull<float>(const float*, int, int, int, const int*, const int*, int, float*, int, hipStream_t);
 template hipError_t launch_pull<double>(const double*, int, int, int, const int*, const int*, int, double*, int, hipStream_t);
 
+// Residual replacement with the block preconditioner, first part: r = r64 *
+// inv_t_c, the kept direction p = f (p - z_iter) (0 for a restart), z = minv r
+// (the blocked firms are overwritten next), x = 0.
+__global__ void __launch_bounds__(kBlock)
+cg_reset_pre_kernel(const double* r64, const double* rel, const float* minv, float* x, float* r, float* z, float* p,
+                    long long total, int C) {
+    const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const int c = static_cast<int>(g0 % C);
+    const double it = rel[c];
+    const float f = static_cast<float>(rel[kMaxCgCols + c]);
+    for (long long j = g0; j < total; j += stride) {
+        const float rv = static_cast<float>(r64[j] * it);
+        r[j] = rv;
+        p[j] = f == 0.0f ? 0.0f : f * (p[j] - z[j]);
+        z[j] = minv[j / C] * rv;
+        x[j] = 0.0f;
+    }
+}
+
+// p += z; part1 = r.z per block and column
+__global__ void __launch_bounds__(kBlock)
+cg_padd_dot_kernel(const float* r, const float* z, float* p, long long total, int C, double* part1) {
+    const long long g0 = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    const int c = static_cast<int>(g0 % C);
+    double rz = 0.0;
+    for (long long j = g0; j < total; j += stride) {
+        const float zv = z[j];
+        p[j] += zv;
+        rz += static_cast<double>(r[j]) * static_cast<double>(zv);
+    }
+    cg_block_store(rz, c, C, part1);
+}
+
 template <typename T>
 hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, int grid, hipStream_t s) {
     hipLaunchKernelGGL((vdot_kernel<T>), dim3(grid), dim3(kBlock), 0, s, a, b, N, C, partials);
@@ -781,10 +817,17 @@ hipError_t launch_refine_update(double* beta, const float* x, const double* inv_
 
 hipError_t launch_cg_step_rel(float* x, float* r, float* z, float* p, const float* q, const float* minv,
                               long long total, int C, double* cg, double* part, int max_iter, int grid,
-                              hipStream_t s) {
+                              const lp::BlockView* blk, int firm_off, hipStream_t s) {
     double* part2 = part + static_cast<size_t>(grid) * C;
     hipLaunchKernelGGL((cg_step_kernel<float>), dim3(grid), dim3(kBlock), 0, s, x, r, z, p, q, minv, total, C, cg,
                        part, part2);
+    if (blk && blk->nb > 0) {
+        // the step's r.z used minv on the blocked firms too: recompute it after the block solves
+        const size_t o = static_cast<size_t>(firm_off) * C;
+        hipError_t e = lp::launch_block_apply(*blk, r + o, z + o, C, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((cg_dot_kernel<float>), dim3(grid), dim3(kBlock), 0, s, r, z, total, C, part2);
+    }
     hipLaunchKernelGGL(cg_beta_rel_kernel, dim3(1), dim3(kRed), 0, s, cg, part, part2, grid, C, max_iter);
     hipLaunchKernelGGL((cg_dir_kernel<float>), dim3(grid), dim3(kBlock), 0, s, p, z, cg, total, C);
     return hipGetLastError();
@@ -792,8 +835,17 @@ hipError_t launch_cg_step_rel(float* x, float* r, float* z, float* p, const floa
 
 hipError_t launch_cg_reliable_reset(const double* r64, const double* rel, const float* minv, float* x, float* r,
                                     float* z, float* p, long long total, int C, double* cg, double* part, int grid,
-                                    hipStream_t s) {
-    hipLaunchKernelGGL(cg_reset_kernel, dim3(grid), dim3(kBlock), 0, s, r64, rel, minv, x, r, z, p, total, C, part);
+                                    const lp::BlockView* blk, int firm_off, hipStream_t s) {
+    if (blk && blk->nb > 0) {
+        hipLaunchKernelGGL(cg_reset_pre_kernel, dim3(grid), dim3(kBlock), 0, s, r64, rel, minv, x, r, z, p, total, C);
+        const size_t o = static_cast<size_t>(firm_off) * C;
+        hipError_t e = lp::launch_block_apply(*blk, r + o, z + o, C, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(cg_padd_dot_kernel, dim3(grid), dim3(kBlock), 0, s, r, z, p, total, C, part);
+    } else {
+        hipLaunchKernelGGL(cg_reset_kernel, dim3(grid), dim3(kBlock), 0, s, r64, rel, minv, x, r, z, p, total, C,
+                           part);
+    }
     hipLaunchKernelGGL(cg_reset_scalars_kernel, dim3(1), dim3(kRed), 0, s, cg, part, grid, rel, C);
     return hipGetLastError();
 }

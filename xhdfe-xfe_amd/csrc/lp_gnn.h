@@ -16,10 +16,12 @@ constexpr int kH = 64;      // width
 constexpr int kInF = 8;     // firm input features
 constexpr int kInW = 8;     // mover input features
 constexpr int kLayers = 3;  // message-passing layers
+constexpr int kFirmCap = 256;  // movers a firm's mean looks at: its first kFirmCap (ascending ids)
 
 // Parameter vector (float32), in this order: Wf0 (8 x 64), bf0, Ww0 (8 x 64), bw0,
 // then per layer l = 1..3: Wwl (128 x 64), bwl, Wfl (128 x 64), bfl; then wa
-// (64), ba, we (64), be.  Matrices are row-major (input x output).
+// (64), ba, we (64), be, ws (the weight of the pair feature in the pair head).
+// Matrices are row-major (input x output).
 constexpr int kOffWf0 = 0;
 constexpr int kOffBf0 = kOffWf0 + kInF * kH;
 constexpr int kOffWw0 = kOffBf0 + kH;
@@ -30,7 +32,8 @@ constexpr int kOffWa = kOffL1 + kLayers * kLayerSize;
 constexpr int kOffBa = kOffWa + kH;
 constexpr int kOffWe = kOffBa + 1;
 constexpr int kOffBe = kOffWe + kH;
-constexpr int kNumParams = kOffBe + 1;
+constexpr int kOffWs = kOffBe + 1;
+constexpr int kNumParams = kOffWs + 1;
 
 constexpr int off_Ww(int l) { return kOffL1 + (l - 1) * kLayerSize; }
 constexpr int off_bw(int l) { return off_Ww(l) + 2 * kH * kH; }
@@ -39,7 +42,10 @@ constexpr int off_bf(int l) { return off_Wf(l) + 2 * kH * kH; }
 
 // Device views of one mobility graph: firm features xf (F x 8), mover features
 // xw (Wm x 8), firm -> mover edge lists (f_ptr, f_idx), mover -> firm edge lists
-// (w_ptr, w_idx), and the firm pairs (pf, pg) scored by the pair head.
+// (w_ptr, w_idx) with w_rank = the position of each mover -> firm edge in
+// the firm's list (a firm's mean over movers takes the first kFirmCap of its
+// list, like a fixed-size neighbour sample), and the firm pairs (pf, pg)
+// scored by the pair head, with one feature pq per pair (null: 0).
 struct Graph {
     const float* xf;
     const float* xw;
@@ -49,6 +55,8 @@ struct Graph {
     const int* w_idx;
     const int* pf;
     const int* pg;
+    const float* pq;
+    const int* w_rank;
 };
 
 // Training targets: u_tgt / u_mask per firm, y_tgt per pair; nmask = sum of u_mask.
@@ -84,6 +92,9 @@ struct Gnn {
         dpart.release();
     }
 };
+
+// w_rank of host edge lists (the position of each mover -> firm edge in the firm's list).
+std::vector<int> host_w_rank(int F, int Wm, const int* f_ptr, const int* f_idx, const int* w_ptr, const int* w_idx);
 
 // One Adam step (step counts from 1).
 void adam_step(float* th, float* m, float* v, const float* g, int step, float lr, hipStream_t s);

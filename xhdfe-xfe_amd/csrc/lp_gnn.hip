@@ -10,12 +10,14 @@
 //   hf = relu(xf Wf0 + bf0), hw = relu(xw Ww0 + bw0)                (8 input features each)
 //   for l = 1..3:  hw = relu([hw | mean_{f in N(w)} hf] Wwl + bwl)
 //                  hf = relu([hf | mean_{w in N(f)} hw] Wfl + bfl)
-//   u_f = hf . wa + ba  (log per-level scaling),  t_fg = sum_k hf_k hg_k we_k + be
+//   u_f = hf . wa + ba  (per-level score),  t_fg = sum_k hf_k hg_k we_k + be + ws q_fg
+//   (q_fg: the pair's own feature, its coupling strength)
 // Loss: mean over firms with a target of (u_f - u*_f)^2 plus the mean binary
 // cross-entropy of sigmoid(t_fg) against y*_fg over the firm pairs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -32,6 +34,7 @@ namespace {
 
 constexpr int kB = 256;   // threads per block (4 waves)
 constexpr int kW = 64;    // wave size
+constexpr int kPart = 67; // per-block partials of a head: 64 weight gradients, loss, bias gradient, ws gradient
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
@@ -234,29 +237,42 @@ __global__ void __launch_bounds__(kB) colsum_kernel(const float* __restrict__ dZ
 }
 
 // ---------------------------------------------------------------- message passing
-// out[v] = mean over src[idx[ptr[v] .. ptr[v+1])] (0 when empty); one wave per v
+// out[v] = mean over src[idx[ptr[v] .. ptr[v] + min(deg, cap))] (0 when
+// empty); one wave per v, one lane per feature, four rows in flight
 __global__ void __launch_bounds__(kB) mean_gather_kernel(const int* __restrict__ ptr, const int* __restrict__ idx,
-                                                         const float* __restrict__ src, int n, float* __restrict__ out) {
+                                                         const float* __restrict__ src, int n, int cap,
+                                                         float* __restrict__ out) {
     const int lane = threadIdx.x & (kW - 1);
     for (int v = blockIdx.x * 4 + threadIdx.x / kW; v < n; v += gridDim.x * 4) {
-        const int b = ptr[v], e = ptr[v + 1];
-        float s = 0.f;
-        for (int q = b; q < e; ++q) s += src[static_cast<size_t>(idx[q]) * 64 + lane];
+        const int b = ptr[v], e = ptr[v + 1] - b > cap ? b + cap : ptr[v + 1];  // no b + cap overflow
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+        int q = b;
+        for (; q + 4 <= e; q += 4) {
+            s0 += src[static_cast<size_t>(idx[q]) * 64 + lane];
+            s1 += src[static_cast<size_t>(idx[q + 1]) * 64 + lane];
+            s2 += src[static_cast<size_t>(idx[q + 2]) * 64 + lane];
+            s3 += src[static_cast<size_t>(idx[q + 3]) * 64 + lane];
+        }
+        for (; q < e; ++q) s0 += src[static_cast<size_t>(idx[q]) * 64 + lane];
+        const float s = (s0 + s1) + (s2 + s3);
         out[static_cast<size_t>(v) * 64 + lane] = e > b ? s / static_cast<float>(e - b) : 0.f;
     }
 }
 
 // transpose of mean_gather seen from the sources: acc[u] += sum over the
-// destinations v listed for u (rptr/ridx) of g[v] / deg(v)
+// destinations v listed for u (rptr/ridx) of g[v] / min(deg(v), cap), over the
+// edges whose position in v's list (rank; null: all) is below cap
 __global__ void __launch_bounds__(kB) mean_scatter_kernel(const int* __restrict__ rptr, const int* __restrict__ ridx,
-                                                          const int* __restrict__ dptr, const float* __restrict__ g,
-                                                          int n, float* __restrict__ acc) {
+                                                          const int* __restrict__ rank, const int* __restrict__ dptr,
+                                                          const float* __restrict__ g, int n, int cap,
+                                                          float* __restrict__ acc) {
     const int lane = threadIdx.x & (kW - 1);
     for (int u = blockIdx.x * 4 + threadIdx.x / kW; u < n; u += gridDim.x * 4) {
         float s = 0.f;
         for (int q = rptr[u]; q < rptr[u + 1]; ++q) {
+            if (rank && rank[q] >= cap) continue;
             const int v = ridx[q];
-            const int d = dptr[v + 1] - dptr[v];
+            const int d = min(dptr[v + 1] - dptr[v], cap);
             s += g[static_cast<size_t>(v) * 64 + lane] / static_cast<float>(d);
         }
         acc[static_cast<size_t>(u) * 64 + lane] += s;
@@ -289,15 +305,15 @@ __global__ void __launch_bounds__(kB) head_u_kernel(const float* __restrict__ hf
     }
 }
 
-// t_p = sum_k hf[pf]_k hf[pg]_k we_k + be
+// t_p = sum_k hf[pf]_k hf[pg]_k we_k + be + ws q_p
 __global__ void __launch_bounds__(kB) head_t_kernel(const float* __restrict__ hf, const int* __restrict__ pf,
-                                                    const int* __restrict__ pg, int np, const float* __restrict__ th,
-                                                    float* __restrict__ t) {
+                                                    const int* __restrict__ pg, const float* __restrict__ pq, int np,
+                                                    const float* __restrict__ th, float* __restrict__ t) {
     const int lane = threadIdx.x & (kW - 1);
-    const float w = th[kOffWe + lane], b = th[kOffBe];
+    const float w = th[kOffWe + lane], b = th[kOffBe], ws = th[kOffWs];
     for (int p = blockIdx.x * 4 + threadIdx.x / kW; p < np; p += gridDim.x * 4) {
         const float s = wave_sum_f(hf[static_cast<size_t>(pf[p]) * 64 + lane] * hf[static_cast<size_t>(pg[p]) * 64 + lane] * w);
-        if (lane == 0) t[p] = s + b;
+        if (lane == 0) t[p] = s + b + (pq ? ws * pq[p] : 0.0f);
     }
 }
 
@@ -326,32 +342,33 @@ __global__ void __launch_bounds__(kB) loss_u_kernel(const float* __restrict__ hf
     if (lane == 0) ls[wave] = l;
     __syncthreads();
     if (threadIdx.x < 64) {
-        double* o = part + static_cast<size_t>(blockIdx.x) * 66;
+        double* o = part + static_cast<size_t>(blockIdx.x) * kPart;
         o[threadIdx.x] = static_cast<double>(gw[0][threadIdx.x]) + gw[1][threadIdx.x] + gw[2][threadIdx.x] + gw[3][threadIdx.x];
     }
     if (threadIdx.x == 0) {
-        part[static_cast<size_t>(blockIdx.x) * 66 + 64] = ls[0] + ls[1] + ls[2] + ls[3];
+        part[static_cast<size_t>(blockIdx.x) * kPart + 64] = ls[0] + ls[1] + ls[2] + ls[3];
+        part[static_cast<size_t>(blockIdx.x) * kPart + 66] = 0.0;
     }
     // ba gradient: the lane-0 sums of du across waves (every lane holds the same gb)
     __shared__ float gbs[4];
     if (lane == 0) gbs[wave] = gb;
     __syncthreads();
-    if (threadIdx.x == 0) part[static_cast<size_t>(blockIdx.x) * 66 + 65] = static_cast<double>(gbs[0]) + gbs[1] + gbs[2] + gbs[3];
+    if (threadIdx.x == 0) part[static_cast<size_t>(blockIdx.x) * kPart + 65] = static_cast<double>(gbs[0]) + gbs[1] + gbs[2] + gbs[3];
 }
 
 // pair part: dt = (sigmoid(t) - y) / np; BCE sum; we / be gradients (per
 // block); dhf[pf] += dt we hf[pg] and symmetric (FP32 atomics)
 __global__ void __launch_bounds__(kB) loss_t_kernel(const float* __restrict__ hf, const float* __restrict__ t,
                                                     const float* __restrict__ y, const int* __restrict__ pf,
-                                                    const int* __restrict__ pg, int np, float inv_np,
-                                                    const float* __restrict__ th, float* __restrict__ dhf,
-                                                    double* __restrict__ part) {
+                                                    const int* __restrict__ pg, const float* __restrict__ pq, int np,
+                                                    float inv_np, const float* __restrict__ th,
+                                                    float* __restrict__ dhf, double* __restrict__ part) {
     __shared__ float gw[4][64];
     __shared__ double ls[4];
-    __shared__ float gbs[4];
+    __shared__ float gbs[4], gss[4];
     const int lane = threadIdx.x & (kW - 1), wave = threadIdx.x / kW;
     const float w = th[kOffWe + lane];
-    float g = 0.f, gb = 0.f;
+    float g = 0.f, gb = 0.f, gs = 0.f;
     double l = 0.0;
     for (int p = blockIdx.x * 4 + wave; p < np; p += gridDim.x * 4) {
         const float tv = t[p], yv = y[p];
@@ -361,6 +378,7 @@ __global__ void __launch_bounds__(kB) loss_t_kernel(const float* __restrict__ hf
         const float ha = hf[a], hb = hf[b];
         g += dt * ha * hb;
         gb += dt;
+        gs += pq ? dt * pq[p] : 0.f;
         l += static_cast<double>(fmaxf(tv, 0.f) - tv * yv + log1pf(expf(-fabsf(tv))));
         atomicAdd(&dhf[a], dt * w * hb);
         atomicAdd(&dhf[b], dt * w * ha);
@@ -369,14 +387,16 @@ __global__ void __launch_bounds__(kB) loss_t_kernel(const float* __restrict__ hf
     if (lane == 0) {
         ls[wave] = l;
         gbs[wave] = gb;
+        gss[wave] = gs;
     }
     __syncthreads();
-    double* o = part + static_cast<size_t>(blockIdx.x) * 66;
+    double* o = part + static_cast<size_t>(blockIdx.x) * kPart;
     if (threadIdx.x < 64)
         o[threadIdx.x] = static_cast<double>(gw[0][threadIdx.x]) + gw[1][threadIdx.x] + gw[2][threadIdx.x] + gw[3][threadIdx.x];
     if (threadIdx.x == 0) {
         o[64] = ls[0] + ls[1] + ls[2] + ls[3];
         o[65] = static_cast<double>(gbs[0]) + gbs[1] + gbs[2] + gbs[3];
+        o[66] = static_cast<double>(gss[0]) + gss[1] + gss[2] + gss[3];
     }
 }
 
@@ -428,21 +448,21 @@ void Gnn::forward(const Graph& g, const float* th, hipStream_t s) {
     for (int l = 1; l <= kLayers; ++l) {
         if (Wm > 0) {
             hipLaunchKernelGGL(mean_gather_kernel, dim3(grid_of(Wm, 4)), dim3(kB), 0, s, g.w_ptr, g.w_idx, hf_at(l - 1),
-                               Wm, aw_at(l));
+                               Wm, INT_MAX, aw_at(l));
             hipLaunchKernelGGL((gemm_fwd_kernel<64, true>), dim3(gw), dim3(kB), 0, s, hw_at(l - 1), aw_at(l), Wm,
                                th + off_Ww(l), th + off_bw(l), hw_at(l));
         }
         if (F > 0) {
             hipLaunchKernelGGL(mean_gather_kernel, dim3(grid_of(F, 4)), dim3(kB), 0, s, g.f_ptr, g.f_idx, hw_at(l), F,
-                               af_at(l));
+                               kFirmCap, af_at(l));
             hipLaunchKernelGGL((gemm_fwd_kernel<64, true>), dim3(gf), dim3(kB), 0, s, hf_at(l - 1), af_at(l), F,
                                th + off_Wf(l), th + off_bf(l), hf_at(l));
         }
     }
     if (F > 0) hipLaunchKernelGGL(head_u_kernel, dim3(grid_of(F, 4)), dim3(kB), 0, s, hf_at(kLayers), F, th, u.p);
     if (np > 0)
-        hipLaunchKernelGGL(head_t_kernel, dim3(grid_of(np, 4)), dim3(kB), 0, s, hf_at(kLayers), g.pf, g.pg, np, th,
-                           t.p);
+        hipLaunchKernelGGL(head_t_kernel, dim3(grid_of(np, 4)), dim3(kB), 0, s, hf_at(kLayers), g.pf, g.pg, g.pq, np,
+                           th, t.p);
     XFE_CHECK(hipGetLastError());
 }
 
@@ -473,21 +493,21 @@ double Gnn::loss_backward(const Graph& g, const float* th, const Targets& tg, fl
     float* tw = dtw.p;
     part.ensure(static_cast<size_t>(512) * 128 * 64);
     const int nbl = 512;
-    dpart.ensure(static_cast<size_t>(2) * nbl * 66);
-    hdpart.resize(static_cast<size_t>(2) * nbl * 66);
+    dpart.ensure(static_cast<size_t>(2) * nbl * kPart);
+    hdpart.resize(static_cast<size_t>(2) * nbl * kPart);
     XFE_CHECK(hipMemsetAsync(grad, 0, sizeof(float) * kNumParams, s));
     XFE_CHECK(hipMemsetAsync(gw, 0, sizeof(float) * static_cast<size_t>(std::max(Wm, 1)) * 64, s));
     double nm = tg.nmask > 0 ? tg.nmask : 1.0;
     const int gu = std::min(grid_of(F, 4), nbl), gt = std::min(grid_of(np, 4), nbl);
-    XFE_CHECK(hipMemsetAsync(dpart.p, 0, sizeof(double) * 2 * nbl * 66, s));
+    XFE_CHECK(hipMemsetAsync(dpart.p, 0, sizeof(double) * 2 * nbl * kPart, s));
     if (F > 0)
         hipLaunchKernelGGL(loss_u_kernel, dim3(gu), dim3(kB), 0, s, hf_at(kLayers), u.p, tg.u_tgt, tg.u_mask, F,
                            static_cast<float>(1.0 / nm), th, gf, dpart.p);
     if (np > 0)
-        hipLaunchKernelGGL(loss_t_kernel, dim3(gt), dim3(kB), 0, s, hf_at(kLayers), t.p, tg.y_tgt, g.pf, g.pg, np,
-                           static_cast<float>(1.0 / np), th, gf, dpart.p + static_cast<size_t>(nbl) * 66);
+        hipLaunchKernelGGL(loss_t_kernel, dim3(gt), dim3(kB), 0, s, hf_at(kLayers), t.p, tg.y_tgt, g.pf, g.pg, g.pq,
+                           np, static_cast<float>(1.0 / np), th, gf, dpart.p + static_cast<size_t>(nbl) * kPart);
     XFE_CHECK(hipGetLastError());
-    XFE_CHECK(hipMemcpyAsync(hdpart.data(), dpart.p, sizeof(double) * 2 * nbl * 66, hipMemcpyDeviceToHost, s));
+    XFE_CHECK(hipMemcpyAsync(hdpart.data(), dpart.p, sizeof(double) * 2 * nbl * kPart, hipMemcpyDeviceToHost, s));
     // layers, top down: gf = d hf_l, gw = d hw_l (from layer l + 1's use of hw_l)
     for (int l = kLayers; l >= 1; --l) {
         if (F > 0) {
@@ -500,8 +520,8 @@ double Gnn::loss_backward(const Graph& g, const float* th, const Targets& tg, fl
                                dagg.p);
             // d hw_l += transpose of the firm mean (over the firms listed for each mover)
             if (Wm > 0)
-                hipLaunchKernelGGL(mean_scatter_kernel, dim3(grid_of(Wm, 4)), dim3(kB), 0, s, g.w_ptr, g.w_idx, g.f_ptr,
-                                   dagg.p, Wm, gw);
+                hipLaunchKernelGGL(mean_scatter_kernel, dim3(grid_of(Wm, 4)), dim3(kB), 0, s, g.w_ptr, g.w_idx,
+                                   g.w_rank, g.f_ptr, dagg.p, Wm, kFirmCap, gw);
             std::swap(gf, tf);  // gf = d hf_{l-1}
         }
         if (Wm > 0) {
@@ -512,8 +532,8 @@ double Gnn::loss_backward(const Graph& g, const float* th, const Targets& tg, fl
             hipLaunchKernelGGL(gemm_dx_kernel, dim3(grid_of(Wm, 128)), dim3(kB), 0, s, gw, Wm, th + off_Ww(l), tw,
                                dagg.p);
             if (F > 0)
-                hipLaunchKernelGGL(mean_scatter_kernel, dim3(grid_of(F, 4)), dim3(kB), 0, s, g.f_ptr, g.f_idx, g.w_ptr,
-                                   dagg.p, F, gf);
+                hipLaunchKernelGGL(mean_scatter_kernel, dim3(grid_of(F, 4)), dim3(kB), 0, s, g.f_ptr, g.f_idx,
+                                   static_cast<const int*>(nullptr), g.w_ptr, dagg.p, F, INT_MAX, gf);
             std::swap(gw, tw);  // gw = d hw_{l-1}
         }
     }
@@ -532,11 +552,11 @@ double Gnn::loss_backward(const Graph& g, const float* th, const Targets& tg, fl
     XFE_CHECK(hipGetLastError());
     XFE_CHECK(hipStreamSynchronize(s));
     // head gradients and the loss from the per-block partials
-    double lu = 0.0, le = 0.0, gba = 0.0, gbe = 0.0;
+    double lu = 0.0, le = 0.0, gba = 0.0, gbe = 0.0, gws = 0.0;
     std::vector<double> gwa(64, 0.0), gwe(64, 0.0);
     for (int b = 0; b < nbl; ++b) {
-        const double* pu = hdpart.data() + static_cast<size_t>(b) * 66;
-        const double* pt = hdpart.data() + static_cast<size_t>(nbl + b) * 66;
+        const double* pu = hdpart.data() + static_cast<size_t>(b) * kPart;
+        const double* pt = hdpart.data() + static_cast<size_t>(nbl + b) * kPart;
         for (int k = 0; k < 64; ++k) {
             gwa[k] += pu[k];
             gwe[k] += pt[k];
@@ -545,13 +565,15 @@ double Gnn::loss_backward(const Graph& g, const float* th, const Targets& tg, fl
         gba += pu[65];
         le += pt[64];
         gbe += pt[65];
+        gws += pt[66];
     }
-    float hg[2 * 64 + 2];
+    float hg[2 * 64 + 3];
     for (int k = 0; k < 64; ++k) hg[k] = static_cast<float>(gwa[k]);
     hg[64] = static_cast<float>(gba);
     for (int k = 0; k < 64; ++k) hg[65 + k] = static_cast<float>(gwe[k]);
     hg[129] = static_cast<float>(gbe);
-    // wa, ba, we, be are contiguous at the end of the vector
+    hg[130] = static_cast<float>(gws);
+    // wa, ba, we, be, ws are contiguous at the end of the vector
     XFE_CHECK(hipMemcpyAsync(grad + kOffWa, hg, sizeof(hg), hipMemcpyHostToDevice, s));
     XFE_CHECK(hipStreamSynchronize(s));
     last_loss_u = lu / nm;
@@ -573,12 +595,34 @@ void adam_step(float* th, float* m, float* v, const float* g, int step, float lr
 
 using namespace xfe;
 
+namespace xfe {
+namespace lp {
+// w_rank of host edge lists: the position of each mover -> firm edge in the
+// firm's list (-1 when the edge is missing there)
+std::vector<int> host_w_rank(int F, int Wm, const int* f_ptr, const int* f_idx, const int* w_ptr, const int* w_idx) {
+    const int E = Wm > 0 ? w_ptr[Wm] : 0;
+    std::vector<int> rank(static_cast<size_t>(std::max(E, 1)), -1);
+    for (int f = 0; f < F; ++f)
+        for (int q = f_ptr[f]; q < f_ptr[f + 1]; ++q) {
+            const int u = f_idx[q];
+            for (int e = w_ptr[u]; e < w_ptr[u + 1]; ++e)
+                if (w_idx[e] == f && rank[e] < 0) {
+                    rank[e] = q - f_ptr[f];
+                    break;
+                }
+        }
+    return rank;
+}
+}  // namespace lp
+}  // namespace xfe
+
 extern "C" {
 
 int xfe_lp_param_count(void) { return lp::kNumParams; }
 
 int xfe_lp_gnn_eval(int F, int Wm, const float* xf, const float* xw, const int* f_ptr, const int* f_idx,
-                    const int* w_ptr, const int* w_idx, int npairs, const int* pf, const int* pg, const float* u_tgt,
+                    const int* w_ptr, const int* w_idx, int npairs, const int* pf, const int* pg, const float* pq,
+                    const float* u_tgt,
                     const float* u_mask, const float* y_tgt, const float* theta, double* loss, float* grad,
                     float* u_out, float* t_out) {
     clear_error();
@@ -610,8 +654,8 @@ int xfe_lp_gnn_eval(int F, int Wm, const float* xf, const float* xw, const int* 
             hipStream_t s;
             ~StreamGuard() { (void)hipStreamDestroy(s); }
         } guard{s};
-        DevBuf<float> d_xf, d_xw, d_ut, d_um, d_y, d_th, d_g;
-        DevBuf<int> d_fp, d_fi, d_wp, d_wi, d_pf, d_pg;
+        DevBuf<float> d_xf, d_xw, d_ut, d_um, d_y, d_th, d_g, d_pq;
+        DevBuf<int> d_fp, d_fi, d_wp, d_wi, d_pf, d_pg, d_rk;
         auto up = [&](auto& buf, const auto* src, size_t n) {
             buf.ensure(n);
             if (n) XFE_CHECK(hipMemcpyAsync(buf.p, src, sizeof(*src) * n, hipMemcpyHostToDevice, s));
@@ -629,7 +673,10 @@ int xfe_lp_gnn_eval(int F, int Wm, const float* xf, const float* xw, const int* 
         up(d_wi, w_idx, static_cast<size_t>(Ew));
         up(d_pf, pf, static_cast<size_t>(npairs));
         up(d_pg, pg, static_cast<size_t>(npairs));
-        lp::Graph g{d_xf.p, d_xw.p, d_fp.p, d_fi.p, d_wp.p, d_wi.p, d_pf.p, d_pg.p};
+        if (pq) up(d_pq, pq, static_cast<size_t>(npairs));
+        const std::vector<int> rk = lp::host_w_rank(F, Wm, f_ptr, f_idx, w_ptr, w_idx);
+        up(d_rk, rk.data(), static_cast<size_t>(Ew));
+        lp::Graph g{d_xf.p, d_xw.p, d_fp.p, d_fi.p, d_wp.p, d_wi.p, d_pf.p, d_pg.p, pq ? d_pq.p : nullptr, d_rk.p};
         double nm = 0.0;
         for (int f = 0; f < F; ++f) nm += u_mask[f];
         lp::Targets tg{d_ut.p, d_um.p, d_y.p, nm};
@@ -642,8 +689,8 @@ int xfe_lp_gnn_eval(int F, int Wm, const float* xf, const float* xw, const int* 
         if (t_out && npairs) XFE_CHECK(hipMemcpyAsync(t_out, net.t.p, sizeof(float) * npairs, hipMemcpyDeviceToHost, s));
         XFE_CHECK(hipStreamSynchronize(s));
         net.release();
-        for (auto* b : {&d_xf, &d_xw, &d_ut, &d_um, &d_y, &d_th, &d_g}) b->release();
-        for (auto* b : {&d_fp, &d_fi, &d_wp, &d_wi, &d_pf, &d_pg}) b->release();
+        for (auto* b : {&d_xf, &d_xw, &d_ut, &d_um, &d_y, &d_th, &d_g, &d_pq}) b->release();
+        for (auto* b : {&d_fp, &d_fi, &d_wp, &d_wi, &d_pf, &d_pg, &d_rk}) b->release();
         return 1;
     } catch (const std::exception& e) {
         set_error(e.what());

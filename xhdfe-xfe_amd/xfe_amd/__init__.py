@@ -31,7 +31,7 @@ class FeInput(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("tol", _d), ("tolerance_mode", _i), ("max_iter", _i), ("method", _i),
                 ("force_cg", _i), ("precision", _i), ("certify", _i),
-                ("refine_tol", _d)]
+                ("refine_tol", _d), ("preconditioner", _i), ("lp_params", _p)]
 
 
 class AbsorbResult(ctypes.Structure):
@@ -40,7 +40,8 @@ class AbsorbResult(ctypes.Structure):
                 ("fe_levels", _i * MAX_DIMS), ("sweeps", _i), ("refinements", _i), ("solve_ms", _d),
                 ("abs_residual", _d), ("abs_residual_rel", _d), ("precision_certified", _i),
                 ("certificate_computed", _i), ("precision_used", _i), ("precision_fallback", _i),
-                ("refine_target", _d), ("preconditioner_used", _i), ("precond_fallback", _i)]
+                ("refine_target", _d), ("preconditioner_used", _i), ("precond_fallback", _i),
+                ("precond_blocks", _i), ("precond_ms", _d)]
 
 
 class GroupIndividual(ctypes.Structure):
@@ -75,7 +76,7 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort",
            "xfe_articulation_points", "xfe_akm_leave_out_set", "xfe_build_indexer",
            "xfe_build_indexer_device", "xfe_weight_sums", "xfe_build_indexer_device_noalloc",
-           "xfe_lp_param_count", "xfe_lp_gnn_eval")
+           "xfe_lp_param_count", "xfe_lp_gnn_eval", "xfe_lp_probe", "xfe_lp_probe_fetch", "xfe_lp_train", "xfe_lp_blocks_apply")
 
 _lib = None
 
@@ -137,9 +138,25 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(_p)
 
 
-def _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify=True, refine_tol=0.0):
+def _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify=True, refine_tol=0.0,
+          preconditioner=0, lp_params=None):
+    """xfe_options; lp_params (float32 array) must stay alive for the call."""
     return Options(float(tol), int(tolerance_mode), int(max_iter), int(method), int(bool(force_cg)),
-                   int(precision), int(bool(certify)), float(refine_tol))
+                   int(precision), int(bool(certify)), float(refine_tol), int(preconditioner),
+                   None if lp_params is None else lp_params.ctypes.data_as(_p))
+
+
+def _lp_params(preconditioner, lp_params):
+    """The network parameters for preconditioner 2: the given ones or the
+    trained defaults shipped with the package."""
+    if int(preconditioner) != 2:
+        return None
+    if lp_params is None:
+        return default_lp_params()
+    a = np.ascontiguousarray(lp_params, dtype=np.float32)
+    if a.size != lp_param_count():
+        raise XfeError(f"lp_params: {a.size} values, the network has {lp_param_count()}")
+    return a
 
 
 def _info(r, D):
@@ -151,12 +168,13 @@ def _info(r, D):
                 precision_certified=bool(r.precision_certified),
                 certificate_computed=bool(r.certificate_computed), precision_used=r.precision_used,
                 precision_fallback=bool(r.precision_fallback), refine_target=r.refine_target,
-                preconditioner_used=r.preconditioner_used, precond_fallback=bool(r.precond_fallback))
+                preconditioner_used=r.preconditioner_used, precond_fallback=bool(r.precond_fallback),
+                precond_blocks=r.precond_blocks, precond_ms=r.precond_ms)
 
 
 def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=1e-8,
            tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0, certify=True,
-           refine_tol=0.0, slopes=None):
+           refine_tol=0.0, slopes=None, preconditioner=0, lp_params=None):
     """absorb_fixed_effects_cuda on host arrays: returns (y_tilde, X_tilde, info).
 
     gids: compact 0-based level ids per dimension; G: level counts.  ``method``
@@ -185,7 +203,8 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
     order = None if sweep_order is None else _i32(sweep_order)
     if method is None:
         method = 2 if D == 2 else 1
-    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify, refine_tol)
+    lpp = _lp_params(preconditioner, lp_params)
+    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify, refine_tol, preconditioner, lpp)
     r = AbsorbResult()
     yo = np.empty(n)
     Xo = np.empty((n, k), order="F")
@@ -361,14 +380,15 @@ def recover_fixed_effects(partial, gids, G, weights=None, sweep_order=None, tol=
 
 def absorb_device(V_ptr, n, C, ld, gid_ptr, G, weights_ptr=None, sweep_order=None, tol=1e-8,
                   tolerance_mode=1, max_iter=100000, method=None, force_cg=False, precision=0,
-                  certify=True):
+                  certify=True, preconditioner=0, lp_params=None):
     """xfe_absorb_device on device pointers (e.g. torch tensors' data_ptr())."""
     D = len(G)
     Garr = _i32(G)
     order = None if sweep_order is None else _i32(sweep_order)
     if method is None:
         method = 2 if D == 2 else 1
-    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify)
+    lpp = _lp_params(preconditioner, lp_params)
+    o = _opts(tol, tolerance_mode, max_iter, method, force_cg, precision, certify, 0.0, preconditioner, lpp)
     r = AbsorbResult()
     ok = lib().xfe_absorb_device(_p(V_ptr), _i(n), _i(C), _i(ld), _p(gid_ptr), _ptr(Garr), _i(D),
                                  _p(weights_ptr) if weights_ptr else None, _ptr(order),
@@ -602,19 +622,38 @@ def profile_get():
     return ms.value, n.value, b.value
 
 
+_LP_DEFAULT = None
+LP_PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lp_params.f32")
+
+
+def default_lp_params():
+    """The trained parameters of the preconditioner network (lp_params.f32,
+    written by train_lp.py)."""
+    global _LP_DEFAULT
+    if _LP_DEFAULT is None:
+        if not os.path.exists(LP_PARAMS_FILE):
+            raise XfeError(f"no trained preconditioner parameters at {LP_PARAMS_FILE}")
+        a = np.fromfile(LP_PARAMS_FILE, dtype="<f4")
+        if a.size != lp_param_count():
+            raise XfeError(f"{LP_PARAMS_FILE}: {a.size} values, the network has {lp_param_count()}")
+        _LP_DEFAULT = a
+    return _LP_DEFAULT
+
+
 def lp_param_count():
     return int(lib().xfe_lp_param_count())
 
 
 def lp_gnn_eval(g, theta):
     """The learned preconditioner's network on the device: (loss, grad, u, t)
-    for a graph dict (xf, xw, f_ptr, f_idx, w_ptr, w_idx, pf, pg, u_tgt,
-    u_mask, y_tgt) and float32 parameters."""
+    for a graph dict (xf, xw, f_ptr, f_idx, w_ptr, w_idx, pf, pg, pq optional,
+    u_tgt, u_mask, y_tgt) and float32 parameters."""
     f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
     xf, xw = f32(g["xf"]), f32(g["xw"])
     F, Wm = xf.shape[0], xw.shape[0]
     fp, fi, wp, wi = (_i32(g[k]) for k in ("f_ptr", "f_idx", "w_ptr", "w_idx"))
     pf, pg = _i32(g["pf"]), _i32(g["pg"])
+    pq = f32(g["pq"]) if g.get("pq") is not None else None
     ut, um, yt, th = f32(g["u_tgt"]), f32(g["u_mask"]), f32(g["y_tgt"]), f32(theta)
     npair = pf.size
     loss = _d()
@@ -622,11 +661,101 @@ def lp_gnn_eval(g, theta):
     u = np.empty(max(F, 1), dtype=np.float32)
     t = np.empty(max(npair, 1), dtype=np.float32)
     L = lib()
-    L.xfe_lp_gnn_eval.argtypes = [_i, _i] + [_p] * 6 + [_i] + [_p] * 2 + [_p] * 4 + [ctypes.POINTER(_d)] + [_p] * 3
+    L.xfe_lp_gnn_eval.argtypes = [_i, _i] + [_p] * 6 + [_i] + [_p] * 3 + [_p] * 4 + [ctypes.POINTER(_d)] + [_p] * 3
     ok = L.xfe_lp_gnn_eval(F, Wm, _ptr(xf), _ptr(xw), _ptr(fp), _ptr(fi), _ptr(wp), _ptr(wi), npair, _ptr(pf),
-                           _ptr(pg), _ptr(ut), _ptr(um), _ptr(yt), _ptr(th), ctypes.byref(loss), _ptr(grad),
-                           _ptr(u), _ptr(t))
+                           _ptr(pg), _ptr(pq), _ptr(ut), _ptr(um), _ptr(yt), _ptr(th), ctypes.byref(loss),
+                           _ptr(grad), _ptr(u), _ptr(t))
     if not ok:
         raise XfeError(last_error())
     return loss.value, grad, u[:F], t[:npair]
 
+
+class LpPanelSizes(ctypes.Structure):
+    _fields_ = [("F", _i), ("Wm", _i), ("E", _i), ("U", _i), ("C", _i), ("inner_iterations", _i),
+                ("converged", _i)]
+
+
+class LpGraph(ctypes.Structure):
+    _fields_ = [("F", _i), ("Wm", _i), ("U", _i)] + [(k, _p) for k in (
+        "xf", "xw", "f_ptr", "f_idx", "w_ptr", "w_idx", "pf", "pg", "pq", "u_tgt", "u_mask", "y_tgt")]
+
+
+def lp_probe(y, X, gids, G, probe_iter=8):
+    """xfe_lp_probe + xfe_lp_probe_fetch: the absorber's iterate errors at inner
+    iteration probe_iter on the firm levels (the second dimension), and the
+    preconditioner's pairs, graph and features for this panel (a dict)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    X = np.asfortranarray(np.asarray(X, dtype=np.float64).reshape(y.size, -1))
+    n, k = X.shape
+    D = len(gids)
+    ids = [_i32(g) for g in gids]
+    fes = (FeInput * D)()
+    for d in range(D):
+        fes[d].group_ids = ids[d].ctypes.data
+        fes[d].num_groups = int(G[d])
+        fes[d].num_levels_present = int(G[d])
+    sz = LpPanelSizes()
+    L = lib()
+    ok = L.xfe_lp_probe(_ptr(y), _ptr(X), _i(n), _i(k), fes, _i(D), _i(probe_iter), ctypes.byref(sz))
+    if not ok:
+        raise XfeError(last_error())
+    F, Wm, E, U, C = sz.F, sz.Wm, sz.E, sz.U, sz.C
+    out = dict(xf=np.empty((F, 8), np.float32), xw=np.empty((max(Wm, 1), 8), np.float32),
+               f_ptr=np.empty(F + 1, np.int32), f_idx=np.empty(max(E, 1), np.int32),
+               w_ptr=np.empty(Wm + 1, np.int32), w_idx=np.empty(max(E, 1), np.int32),
+               pf=np.empty(max(U, 1), np.int32), pg=np.empty(max(U, 1), np.int32),
+               pq=np.empty(max(U, 1), np.float32), W=np.empty(max(U, 1)), dex=np.empty(F),
+               err=np.empty((F, C)))
+    ok = L.xfe_lp_probe_fetch(*[_ptr(out[key]) for key in ("xf", "xw", "f_ptr", "f_idx", "w_ptr", "w_idx", "pf",
+                                                           "pg", "pq", "W", "dex", "err")])
+    if not ok:
+        raise XfeError(last_error())
+    out["xw"], out["f_idx"], out["w_idx"] = out["xw"][:Wm], out["f_idx"][:E], out["w_idx"][:E]
+    for key in ("pf", "pg", "pq", "W"):
+        out[key] = out[key][:U]
+    out.update(F=F, Wm=Wm, E=E, U=U, inner_iterations=sz.inner_iterations, converged=bool(sz.converged))
+    return out
+
+
+def lp_train(graphs, theta, steps, lr=1e-3):
+    """xfe_lp_train: Adam on the device over the graphs (dicts with the arrays
+    of lp_gnn_eval), one graph per step in turn.  Returns (theta, losses)."""
+    keep = []
+    arr = (LpGraph * len(graphs))()
+    for i, g in enumerate(graphs):
+        cv = {}
+        for key in ("xf", "xw", "pq", "u_tgt", "u_mask", "y_tgt"):
+            cv[key] = np.ascontiguousarray(g[key], dtype=np.float32)
+        for key in ("f_ptr", "f_idx", "w_ptr", "w_idx", "pf", "pg"):
+            cv[key] = _i32(g[key])
+        keep.append(cv)
+        arr[i].F = int(cv["xf"].shape[0])
+        arr[i].Wm = int(cv["w_ptr"].size - 1)
+        arr[i].U = int(cv["pf"].size)
+        for key, v in cv.items():
+            setattr(arr[i], key, v.ctypes.data)
+    th = np.ascontiguousarray(theta, dtype=np.float32).copy()
+    hist = np.zeros(max(int(steps), 1))
+    ok = lib().xfe_lp_train(_i(len(graphs)), arr, _i(steps), ctypes.c_float(lr), _ptr(th), _ptr(hist))
+    if not ok:
+        raise XfeError(last_error())
+    return th, hist[:steps]
+
+
+def lp_blocks_apply(pf, pg, W, dex, score, r, cap=16, rounds=6):
+    """xfe_lp_blocks_apply: (agg, z) for host pairs, couplings, diagonal and
+    scores; r: F x C."""
+    pf, pg = _i32(pf), _i32(pg)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    dex = np.ascontiguousarray(dex, dtype=np.float64)
+    sc = np.ascontiguousarray(score, dtype=np.float32)
+    F = dex.size
+    r = np.ascontiguousarray(np.asarray(r, dtype=np.float32).reshape(F, -1))
+    C = r.shape[1]
+    agg = np.empty(F, np.int32)
+    z = np.empty((F, C), np.float32)
+    ok = lib().xfe_lp_blocks_apply(_i(F), _i(pf.size), _ptr(pf), _ptr(pg), _ptr(W), _ptr(dex), _ptr(sc), _i(cap),
+                                   _i(rounds), _i(C), _ptr(r), _ptr(agg), _ptr(z))
+    if not ok:
+        raise XfeError(last_error())
+    return agg, z
