@@ -102,6 +102,7 @@ struct Workspace {
     DevBuf<unsigned char> sort_tmp;
     DevBuf<double> w;
     DevBuf<double> partials;
+    DevBuf<double> prof_out;  // level-size profiles, 4 per dimension
     DevBuf<double> scalars;
     DevBuf<double> lanes;  // per-column scalars (CG alpha/beta)
     DevBuf<double> stage;  // host-API staging of [y | X]
@@ -151,7 +152,7 @@ struct Workspace {
             wsum[d].release(); inv[d].release();
         }
         perm.release(); keys_tmp.release(); vals_tmp.release(); flag.release();
-        sort_tmp.release(); w.release(); partials.release(); scalars.release();
+        sort_tmp.release(); w.release(); partials.release(); scalars.release(); prof_out.release();
         lanes.release(); stage.release(); stage_ids.release(); stage_w.release();
         tile_start.release(); level_count.release(); tmp_flags.release(); tmp_inc.release();
         tmp_iota.release(); tmp_key.release();
@@ -192,6 +193,31 @@ struct Workspace {
 };
 
 thread_local Workspace tls_ws;
+
+// Phase timings of one absorption on stderr when XFE_PHASES is set (each mark
+// waits for the stream, so the marks themselves cost time; off otherwise).
+struct PhaseClock {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::chrono::steady_clock::time_point t;
+    void start(hipStream_t st) {
+        on = std::getenv("XFE_PHASES") != nullptr;
+        s = st;
+        if (on) {
+            XFE_CHECK(hipStreamSynchronize(s));
+            t = std::chrono::steady_clock::now();
+        }
+    }
+    void mark(const char* name) {
+        if (!on) return;
+        XFE_CHECK(hipStreamSynchronize(s));
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[xfe phase] %-28s %8.3f ms\n", name,
+                     std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
+thread_local PhaseClock tls_phase;
 
 // Source of a sweep: a stored block, or the Irons-Tuck extrapolation
 // P - coef (P - Q) computed on the fly and written to xout.
@@ -550,22 +576,15 @@ struct ProfileStats {
 };
 
 // profile_fe, src/fe_absorption.cpp:4945-4982.
-ProfileStats profile_of(const std::vector<double>& ws, bool unit) {
+ProfileStats profile_from_sums(double sum, double sq, double mx, double single, int G, bool unit) {
     ProfileStats p;
-    const int G = static_cast<int>(ws.size());
     p.num_groups = G;
     if (G <= 0) return p;
-    double sum = 0.0, sq = 0.0, mx = -1e308;
-    int single = 0;
-    for (double v : ws) {
-        sum += v; sq += v * v; mx = std::max(mx, v);
-        if (v <= 1.0 + 1e-12) ++single;
-    }
     p.mean = sum / G;
     const double var = std::max(0.0, sq / G - p.mean * p.mean);
     p.cv = p.mean > 0.0 ? std::sqrt(var) / p.mean : 0.0;
     p.max_val = mx;
-    p.singleton_ratio = unit ? static_cast<double>(single) / G : 0.0;
+    p.singleton_ratio = unit ? single / G : 0.0;
     return p;
 }
 
@@ -1053,6 +1072,7 @@ struct FastSolver {
         XFE_CHECK(hipMemsetAsync(ws.cg.p, 0, sizeof(double) * kCgSlots, s));
         XFE_CHECK(launch_to_float(ws.inv[p].p, ws.f_invp.p, sv.G[p], grid_for(sv.G[p]), s));
         seg = setup_segments();
+        tls_phase.mark("segments");
         // Jacobi preconditioner: diag(S) from the segments, or from runs of each
         // R level inside levels of p
         if (seg) {
@@ -1067,7 +1087,9 @@ struct FastSolver {
             XFE_CHECK(hipMemcpyAsync(ws.f_counts.p + off[k], ws.wsum[rdim[k]].p,
                                      sizeof(double) * sv.G[rdim[k]], hipMemcpyDeviceToDevice, s));
         }
+        tls_phase.mark("jacobi diagonal");
         if (seg && precond > 0) setup_blocks();
+        tls_phase.mark("block preconditioner");
         XFE_CHECK(launch_diag_invert(ws.f_diag.p, ws.f_counts.p, ws.f_minv.p, N, grid_for(N), s));
         XFE_CHECK(hipMemsetAsync(ws.f_beta.p, 0, sizeof(double) * NC, s));
         return true;
@@ -1091,11 +1113,14 @@ struct FastSolver {
         v.diag = ws.f_diag.p + off[k1];
         v.rows = ws.wsum[rdim[k1]].p;
         if (!ag.build_pairs(v, s)) return;
+        tls_phase.mark("  pairs");
         const float* score = ag.st.p;
         if (precond == 2) score = lp::learned_pair_scores(ag, v, lp_params, s);
+        tls_phase.mark("  network scores");
         const int cap = std::getenv("XFE_LP_CAP") ? std::atoi(std::getenv("XFE_LP_CAP")) : 16;
         const int rounds = std::getenv("XFE_LP_ROUNDS") ? std::atoi(std::getenv("XFE_LP_ROUNDS")) : 6;
         ag.aggregate(score, cap, rounds, s);
+        tls_phase.mark("  aggregation");
         ag.build_blocks(s);
         XFE_CHECK(hipMemcpyAsync(ws.f_diag.p + off[k1], ag.dex.p, sizeof(double) * v.G1, hipMemcpyDeviceToDevice, s));
         XFE_CHECK(hipStreamSynchronize(s));
@@ -1444,6 +1469,7 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
                                  hipMemcpyDeviceToDevice, sv.s));
     }
     fs.coldot<double>(sv.ws.f_b.p, sv.ws.f_b.p, b0);
+    tls_phase.mark("b0 = D_R' M_p V");
     double b0n[kMaxCgCols];
     for (int c = 0; c < C; ++c) b0n[c] = std::sqrt(b0[c]);
     const double delta = std::getenv("XFE_RELIABLE_DELTA") ? std::atof(std::getenv("XFE_RELIABLE_DELTA")) : 0.01;
@@ -1454,8 +1480,10 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
     const bool converged = rel <= std::max(target, 1e-9);
     (void)tol_in;
     (void)max_inner;
+    tls_phase.mark("PCG");
     if (fs.probe) fs.record_probe();
     fs.residual64(V, V);  // final within transform, in place
+    tls_phase.mark("final transform");
     auto& rec = sv.ws.fast_rec;
     rec.used = true;
     rec.p = fs.p;
@@ -1946,6 +1974,7 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
     hipStream_t s = ws.stream;
     const int D = static_cast<int>(G.size());
     const auto t0 = std::chrono::steady_clock::now();
+    tls_phase.start(s);
     for (int d = 0; d < D; ++d) res.fe_levels[d] = G[d];
     const int ld = ((n + 63) / 64) * 64;
     const int grid = grid_for(n);
@@ -1991,6 +2020,7 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
         XFE_CHECK(launch_gather_vec(d_w_in, perm, ws.w.p, n, grid, s));
         w = ws.w.p;
     }
+    tls_phase.mark("level sort + gather");
     Solver sv(ws, n, C, ld);
     sv.D = D;
     sv.w = w;
@@ -2002,21 +2032,29 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
         sv.G[d] = G[d];
         sv.gid[d] = ws.gid[d].p;
         sv.prepare_dim(d);
-        std::vector<double> hw(G[d]);
         if (h_wsum[d]) {
-            std::memcpy(hw.data(), h_wsum[d], sizeof(double) * G[d]);
             XFE_CHECK(hipMemcpyAsync(ws.wsum[d].p, h_wsum[d], sizeof(double) * G[d],
                                      hipMemcpyHostToDevice, s));
             XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G[d], grid_for(G[d]), s));
         } else {
             sv.weight_sums(d);
-            XFE_CHECK(hipMemcpyAsync(hw.data(), ws.wsum[d].p, sizeof(double) * G[d],
-                                     hipMemcpyDeviceToHost, s));
         }
-        XFE_CHECK(hipStreamSynchronize(s));
-        profs[d] = profile_of(hw, w == nullptr);
+        // the level-size profile (order_by_profile) from a device reduction: no
+        // download of the level table
+        const int pgrid = std::max(1, std::min(1024, grid_for(G[d])));
+        ws.partials.ensure(static_cast<size_t>(4) * pgrid);
+        ws.prof_out.ensure(4 * XFE_MAX_DIMS);
+        XFE_CHECK(launch_level_profile(ws.wsum[d].p, G[d], ws.partials.p, pgrid, ws.prof_out.p + 4 * d, s));
     }
+    XFE_CHECK(hipMemcpyAsync(ws.host_scalars, ws.prof_out.p, sizeof(double) * 4 * D, hipMemcpyDeviceToHost, s));
+    XFE_CHECK(hipStreamSynchronize(s));
+    for (int d = 0; d < D; ++d) {
+        const double* h = ws.host_scalars + 4 * d;
+        profs[d] = profile_from_sums(h[0], h[1], h[2], h[3], G[d], w == nullptr);
+    }
+    tls_phase.mark("ids + weight sums");
     setup_tiles(sv, ws, n, D, G, w == nullptr && h_wsum[p] == nullptr);
+    tls_phase.mark("tiles");
     // partial buffer: pass partials (3 per block and chunk), per-column CG dots,
     // then the polish maxima of one sweep (<= 2D projections)
     const size_t pg = static_cast<size_t>(std::max(std::max(sv.grid, grid), sv.tile_grid));
@@ -2046,14 +2084,17 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
     }
     std::vector<double> orig_sq;
     if (opt.certify) orig_sq = column_sumsq(sv, V, ld);
+    tls_phase.mark("input norms");
     solve_core(sv, V, order, profs, opt, res);
     if (opt.certify) {
         certificate(sv, V, orig_sq, opt, res.converged, res.abs_residual, res.abs_residual_rel,
                     res.precision_certified);
         res.certificate_computed = 1;
+        tls_phase.mark("certificate");
     }
     XFE_CHECK(launch_scatter_cols(V, ld, perm, d_Vin, ld_in, n, C, grid, s));
     XFE_CHECK(hipStreamSynchronize(s));
+    tls_phase.mark("scatter");
     const auto t1 = std::chrono::steady_clock::now();
     res.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 }

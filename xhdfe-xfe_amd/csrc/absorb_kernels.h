@@ -42,6 +42,7 @@ struct PassArgs {
     int one = 0;           // values are 1 (weight sums)
     int xcd_copies = 1;    // kRedGlobal: private accumulator copies
     size_t copy_stride = 0;
+    int lds_copies = 1;    // kRedLds: LDS tables per block (set by the launcher)
 };
 
 // Arguments of the tiled pass (tile_kernels.hip).
@@ -243,6 +244,9 @@ hipError_t launch_reduce_partials(const double* partials, int count, int nq, int
                                   hipStream_t s);
 hipError_t launch_it_stats(const double* x, const double* gx, const double* ggx, int n, int C,
                            int ld, double* partials, int grid, hipStream_t s);
+// sum, sum of squares, max and count of entries <= 1 + 1e-12 of a level table
+// (G values) into out[0..3]; part: 4 x grid scratch
+hipError_t launch_level_profile(const double* ws, int G, double* part, int grid, double* out, hipStream_t s);
 hipError_t launch_col_dot(const double* a, const double* b, int n, int C, int ld, double* partials,
                           int grid, hipStream_t s);
 // per-column sums of squares of a level-major table (certificate), grid from cg_grid

@@ -53,11 +53,15 @@ namespace xfe {
 // ---------------------------------------------------------------------------
 template <int CC, int RM, int NQ, int SRC>
 __global__ void __launch_bounds__(kBlock) pass_kernel(PassArgs a) {
-    extern __shared__ __align__(16) double lsum[];
+    extern __shared__ __align__(16) double lsum_all[];
     const int n = a.n;
     const size_t ld = static_cast<size_t>(a.ld);
+    // kRedLds: one table per wave when lds_copies = 4 (the few levels of a small
+    // dimension otherwise serialize the LDS atomics of the whole block)
+    double* lsum = lsum_all;
     if constexpr (RM == kRedLds) {
-        for (int j = threadIdx.x; j < a.G_r * CC; j += kBlock) lsum[j] = 0.0;
+        for (int j = threadIdx.x; j < a.lds_copies * a.G_r * CC; j += kBlock) lsum_all[j] = 0.0;
+        lsum = lsum_all + static_cast<size_t>((threadIdx.x / kWave) % a.lds_copies) * a.G_r * CC;
         __syncthreads();
     }
     constexpr int NQA = NQ > 0 ? NQ : 1;
@@ -162,7 +166,8 @@ __global__ void __launch_bounds__(kBlock) pass_kernel(PassArgs a) {
     if constexpr (RM == kRedLds) {
         __syncthreads();
         for (int j = threadIdx.x; j < a.G_r * CC; j += kBlock) {
-            const double s = lsum[j];
+            double s = 0.0;
+            for (int k = 0; k < a.lds_copies; ++k) s += lsum_all[static_cast<size_t>(k) * a.G_r * CC + j];
             if (s != 0.0) {
                 atomicAdd(&a.sums_r[static_cast<size_t>(j / CC) * a.C + a.c0 + (j % CC)], s);
             }
@@ -368,8 +373,11 @@ __global__ void __launch_bounds__(kBlock) sorted_check_kernel(const int* gid, in
 // ---------------------------------------------------------------------------
 
 template <int CC, int RM, int NQ, int SRC>
-static hipError_t launch_pass_t(const PassArgs& a, int grid, hipStream_t s) {
-    const size_t shmem = RM == kRedLds ? static_cast<size_t>(a.G_r) * CC * sizeof(double) : 0;
+static hipError_t launch_pass_t(const PassArgs& a_in, int grid, hipStream_t s) {
+    PassArgs a = a_in;
+    const size_t table = static_cast<size_t>(a.G_r) * CC * sizeof(double);
+    a.lds_copies = (RM == kRedLds && (kBlock / kWave) * table <= 64 * 1024) ? kBlock / kWave : 1;
+    const size_t shmem = RM == kRedLds ? a.lds_copies * table : 0;
     hipLaunchKernelGGL((pass_kernel<CC, RM, NQ, SRC>), dim3(grid), dim3(kBlock), shmem, s, a);
     return hipGetLastError();
 }
@@ -453,6 +461,58 @@ hipError_t launch_it_stats(const double* x, const double* gx, const double* ggx,
                            int ld, double* partials, int grid, hipStream_t s) {
     hipLaunchKernelGGL(it_stats_kernel, dim3(grid), dim3(kBlock), 0, s, x, gx, ggx, n, C, ld,
                        partials);
+    return hipGetLastError();
+}
+
+namespace {
+// Profile of a level table (sum, sum of squares, max, entries <= 1 + 1e-12):
+// per block partials in thread order, then one block sums them in block order
+__global__ void __launch_bounds__(kBlock) level_profile_kernel(const double* ws, int G, double* part) {
+    __shared__ double r[4][kBlock];
+    double s = 0.0, q = 0.0, m = -1e308, one = 0.0;
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < G; g += gridDim.x * kBlock) {
+        const double v = ws[g];
+        s += v;
+        q += v * v;
+        m = fmax(m, v);
+        one += v <= 1.0 + 1e-12 ? 1.0 : 0.0;
+    }
+    r[0][threadIdx.x] = s;
+    r[1][threadIdx.x] = q;
+    r[2][threadIdx.x] = m;
+    r[3][threadIdx.x] = one;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            r[0][threadIdx.x] += r[0][threadIdx.x + w];
+            r[1][threadIdx.x] += r[1][threadIdx.x + w];
+            r[2][threadIdx.x] = fmax(r[2][threadIdx.x], r[2][threadIdx.x + w]);
+            r[3][threadIdx.x] += r[3][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[static_cast<size_t>(blockIdx.x) * 4 + threadIdx.x] = r[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(64) level_profile_final_kernel(const double* part, int nblk, double* out) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0, q = 0.0, m = -1e308, one = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        s += part[4 * b];
+        q += part[4 * b + 1];
+        m = fmax(m, part[4 * b + 2]);
+        one += part[4 * b + 3];
+    }
+    out[0] = s;
+    out[1] = q;
+    out[2] = m;
+    out[3] = one;
+}
+}  // namespace
+
+hipError_t launch_level_profile(const double* ws, int G, double* part, int grid, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(level_profile_kernel, dim3(grid), dim3(kBlock), 0, s, ws, G, part);
+    hipLaunchKernelGGL(level_profile_final_kernel, dim3(1), dim3(64), 0, s, part, grid, out);
     return hipGetLastError();
 }
 

@@ -10,9 +10,9 @@
 // (lp_precond.hip); the solve's stopping rule is the FP64 residual, so the
 // answer never depends on the network.
 //
-// Features are computed from integer counts and FP64 values in a fixed order
-// (fixed-point sums for the per-firm strength totals), so the same panel gives
-// the same scores and the same aggregation on every run.
+// Features are computed from integer counts and values summed in a fixed
+// order (stable sorts, one wavefront per firm), so the same panel gives the
+// same scores and the same aggregation on every run.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "absorb_kernels.h"
 #include "common.h"
 #include "lp_gnn.h"
 #include "lp_precond.h"
@@ -137,44 +138,60 @@ __global__ void __launch_bounds__(kB) transpose_kernel(const int* skey, const in
     }
 }
 
-__global__ void __launch_bounds__(kB) count_kernel(const int* key, int m, int* cnt) {
-    for (int i = blockIdx.x * kB + threadIdx.x; i < m; i += gridDim.x * kB) atomicAdd(&cnt[key[i]], 1);
-}
-
-// per firm: pairs, the largest strength (as ordered bits; strengths are >= 0)
-// and the sum of strengths in fixed point (2^-24 units: integer adds, any order)
-__global__ void __launch_bounds__(kB) pair_stats_kernel(const int* pf, const int* pg, const float* st, int U,
-                                                        int* pcnt, unsigned int* smax, unsigned long long* ssum) {
+// the two half-edges of every pair: (f, strength) and (g, strength)
+__global__ void __launch_bounds__(kB) half_edge_kernel(const int* pf, const int* pg, const float* st, int U, int* key,
+                                                       float* val) {
     for (int p = blockIdx.x * kB + threadIdx.x; p < U; p += gridDim.x * kB) {
         const float v = fmaxf(st[p], 0.0f);
-        const unsigned int bits = __float_as_uint(v);
-        const unsigned long long fx = static_cast<unsigned long long>(static_cast<double>(v) * 16777216.0);
-        for (int k = 0; k < 2; ++k) {
-            const int f = k ? pg[p] : pf[p];
-            atomicAdd(&pcnt[f], 1);
-            atomicMax(&smax[f], bits);
-            atomicAdd(&ssum[f], fx);
-        }
+        key[2 * p] = pf[p];
+        key[2 * p + 1] = pg[p];
+        val[2 * p] = v;
+        val[2 * p + 1] = v;
     }
 }
 
-// firm features: log(1 + rows), log(1 + movers), log(1 + S_ff), S_ff / rows,
-// log(1 + pairs), max strength, sum of strengths, 1
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// firm features, one wavefront per firm (its pair half-edges strided over the
+// lanes, then wave reductions in a fixed order): log(1 + rows), log(1 +
+// movers), log(1 + S_ff), S_ff / rows, log(1 + pairs), max strength, sum of
+// strengths, 1
 __global__ void __launch_bounds__(kB) firm_feat_kernel(const double* rows, const int* f_ptr, const double* dex,
-                                                       const int* pcnt, const unsigned int* smax,
-                                                       const unsigned long long* ssum, int F, float* xf) {
-    for (int f = blockIdx.x * kB + threadIdx.x; f < F; f += gridDim.x * kB) {
-        const double r = rows[f];
-        const double d = fmax(dex[f], 0.0);
-        float* x = xf + static_cast<size_t>(f) * kInF;
-        x[0] = static_cast<float>(log1p(r));
-        x[1] = static_cast<float>(log1p(static_cast<double>(f_ptr[f + 1] - f_ptr[f])));
-        x[2] = static_cast<float>(log1p(d));
-        x[3] = static_cast<float>(d / fmax(r, 1.0));
-        x[4] = static_cast<float>(log1p(static_cast<double>(pcnt[f])));
-        x[5] = __uint_as_float(smax[f]);
-        x[6] = static_cast<float>(static_cast<double>(ssum[f]) / 16777216.0);
-        x[7] = 1.0f;
+                                                       const int* hoff, const float* hval, int F, float* xf) {
+    const int lane = threadIdx.x & 63;
+    for (int f = blockIdx.x * 4 + threadIdx.x / 64; f < F; f += gridDim.x * 4) {
+        const int b = hoff[f], e = hoff[f + 1];
+        float mx = 0.0f, sm = 0.0f;
+        for (int q = b + lane; q < e; q += 64) {
+            const float v = hval[q];
+            mx = fmaxf(mx, v);
+            sm += v;
+        }
+        mx = wave_max(mx);
+        sm = wave_sum(sm);
+        if (lane == 0) {
+            const double r = rows[f];
+            const double d = fmax(dex[f], 0.0);
+            float* x = xf + static_cast<size_t>(f) * kInF;
+            x[0] = static_cast<float>(log1p(r));
+            x[1] = static_cast<float>(log1p(static_cast<double>(f_ptr[f + 1] - f_ptr[f])));
+            x[2] = static_cast<float>(log1p(d));
+            x[3] = static_cast<float>(d / fmax(r, 1.0));
+            x[4] = static_cast<float>(log1p(static_cast<double>(e - b)));
+            x[5] = mx;
+            x[6] = sm;
+            x[7] = 1.0f;
+        }
     }
 }
 
@@ -232,45 +249,54 @@ void LearnedGraph::build(const AggPrecond& ag, const SegView& sv, hipStream_t s)
     w_rank.ensure(E);
     xw.ensure(static_cast<size_t>(std::max(Wm, 1)) * kInW);
     xf.ensure(static_cast<size_t>(F) * kInF);
-    fcnt.ensure(static_cast<size_t>(F) + 1);
     f_ptr.ensure(static_cast<size_t>(F) + 1);
-    pcnt.ensure(F); smax.ensure(F); ssum.ensure(F);
     hipLaunchKernelGGL(mover_fill_kernel, dim3(gw), dim3(kB), 0, s, sv.wseg, sv.r1, sv.len, sv.inv_w, Nw, mflag.p,
                        moff.p, midx.p, w_ptr.p, w_idx.p, e_mov.p, xw.p);
     XFE_CHECK(hipMemcpyAsync(w_ptr.p + Wm, &E, sizeof(int), hipMemcpyHostToDevice, s));
     // firm -> mover lists: a stable sort of the edges by firm (movers ascending
-    // inside a firm), and each edge's rank in its firm's list
-    XFE_CHECK(hipMemsetAsync(fcnt.p, 0, sizeof(int) * (static_cast<size_t>(F) + 1), s));
-    if (E > 0) hipLaunchKernelGGL(count_kernel, dim3(grid_of(E)), dim3(kB), 0, s, w_idx.p, E, fcnt.p);
-    exclusive_scan<int>(tmp, fcnt.p, f_ptr.p, static_cast<size_t>(F) + 1, s);
+    // inside a firm), the firm offsets from the sorted keys, and each edge's
+    // rank in its firm's list
+    const int fb = bits_for(static_cast<unsigned long long>(F));
     if (E > 0) {
         hipLaunchKernelGGL(iota_kernel, dim3(grid_of(E)), dim3(kB), 0, s, eidx.p, E);
         size_t bytes = 0;
-        const int eb = bits_for(static_cast<unsigned long long>(F));
         XFE_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, w_idx.p, skey.p, eidx.p, sidx.p,
-                                            static_cast<size_t>(E), 0, eb, s));
+                                            static_cast<size_t>(E), 0, fb, s));
         tmp.ensure(bytes);
         XFE_CHECK(rocprim::radix_sort_pairs(tmp.p, bytes, w_idx.p, skey.p, eidx.p, sidx.p,
-                                            static_cast<size_t>(E), 0, eb, s));
+                                            static_cast<size_t>(E), 0, fb, s));
+        XFE_CHECK(launch_seg_offsets(skey.p, E, F, f_ptr.p, grid_of(E), s));
         hipLaunchKernelGGL(transpose_kernel, dim3(grid_of(E)), dim3(kB), 0, s, skey.p, sidx.p, e_mov.p, f_ptr.p, E,
                            f_idx.p, w_rank.p);
+    } else {
+        XFE_CHECK(hipMemsetAsync(f_ptr.p, 0, sizeof(int) * (static_cast<size_t>(F) + 1), s));
     }
-    XFE_CHECK(hipMemsetAsync(pcnt.p, 0, sizeof(int) * F, s));
-    XFE_CHECK(hipMemsetAsync(smax.p, 0, sizeof(unsigned int) * F, s));
-    XFE_CHECK(hipMemsetAsync(ssum.p, 0, sizeof(unsigned long long) * F, s));
-    if (ag.U > 0)
-        hipLaunchKernelGGL(pair_stats_kernel, dim3(grid_of(ag.U)), dim3(kB), 0, s, ag.pf.p, ag.pg.p, ag.st.p, ag.U,
-                           pcnt.p, smax.p, ssum.p);
-    hipLaunchKernelGGL(firm_feat_kernel, dim3(grid_of(F)), dim3(kB), 0, s, sv.rows, f_ptr.p, ag.dex.p, pcnt.p,
-                       smax.p, ssum.p, F, xf.p);
+    // the pairs' half-edges by firm (stable), for the per-firm pair features
+    const int H = 2 * ag.U;
+    hkey.ensure(H); hkey2.ensure(H); hval.ensure(H); hval2.ensure(H); hoff.ensure(static_cast<size_t>(F) + 1);
+    if (H > 0) {
+        hipLaunchKernelGGL(half_edge_kernel, dim3(grid_of(ag.U)), dim3(kB), 0, s, ag.pf.p, ag.pg.p, ag.st.p, ag.U,
+                           hkey.p, hval.p);
+        size_t bytes = 0;
+        XFE_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, hkey.p, hkey2.p, hval.p, hval2.p, static_cast<size_t>(H),
+                                            0, fb, s));
+        tmp.ensure(bytes);
+        XFE_CHECK(rocprim::radix_sort_pairs(tmp.p, bytes, hkey.p, hkey2.p, hval.p, hval2.p, static_cast<size_t>(H),
+                                            0, fb, s));
+        XFE_CHECK(launch_seg_offsets(hkey2.p, H, F, hoff.p, grid_of(H), s));
+    } else {
+        XFE_CHECK(hipMemsetAsync(hoff.p, 0, sizeof(int) * (static_cast<size_t>(F) + 1), s));
+    }
+    hipLaunchKernelGGL(firm_feat_kernel, dim3(grid_of(F, 4)), dim3(kB), 0, s, sv.rows, f_ptr.p, ag.dex.p, hoff.p,
+                       hval2.p, F, xf.p);
     XFE_CHECK(hipGetLastError());
 }
 
 void LearnedGraph::release() {
-    for (auto* b : {&mcnt, &mflag, &moff, &midx, &w_ptr, &w_idx, &e_mov, &f_ptr, &f_idx, &fcnt, &skey, &pcnt, &eidx,
-                    &sidx, &w_rank})
+    for (auto* b : {&mcnt, &mflag, &moff, &midx, &w_ptr, &w_idx, &e_mov, &f_ptr, &f_idx, &skey, &eidx, &sidx, &w_rank,
+                    &hkey, &hkey2, &hoff})
         b->release();
-    smax.release(); ssum.release(); xf.release(); xw.release(); tmp.release();
+    hval.release(); hval2.release(); xf.release(); xw.release(); tmp.release();
     if (host_int) (void)hipHostFree(host_int);
     host_int = nullptr;
 }

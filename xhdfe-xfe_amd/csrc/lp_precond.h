@@ -36,6 +36,8 @@ struct SegView {
 // Device view of the blocks for the apply kernel.
 struct BlockView {
     int nb = 0;                    // firms inside blocks
+    int nblocks = 0;               // blocks
+    const int* heads = nullptr;    // first blocked-firm position of each block
     const int* firm = nullptr;     // blocked firms, grouped by block, ascending inside a block
     const int* start = nullptr;    // per blocked firm: first position of its block
     const int* size = nullptr;     // per blocked firm: its block's size
@@ -62,7 +64,7 @@ struct AggPrecond {
     DevBuf<float> st;             // strengths W / sqrt(d_f d_g)
     DevBuf<int> agg, asize, newroot, cnt, bsz, bsq, bstart, boff2, skey, sval, iota, loc;
     DevBuf<unsigned long long> best;
-    DevBuf<int> b_firm, b_start, b_size, b_row;
+    DevBuf<int> b_firm, b_start, b_size, b_row, b_head, b_hflag, b_hidx;
     DevBuf<double> bmat;
     DevBuf<float> binv;
     DevBuf<unsigned char> tmp;
@@ -78,7 +80,7 @@ struct AggPrecond {
     // Blocks of the aggregates with two or more firms and their inverses.
     void build_blocks(hipStream_t s);
     BlockView view() const {
-        return BlockView{nb, b_firm.p, b_start.p, b_size.p, b_row.p, binv.p};
+        return BlockView{nb, nblocks, b_head.p, b_firm.p, b_start.p, b_size.p, b_row.p, binv.p};
     }
     void release();
 };
@@ -89,9 +91,9 @@ struct AggPrecond {
 // and movers; the pairs of an AggPrecond are the scored firm pairs.
 struct LearnedGraph {
     int F = 0, Wm = 0, E = 0;
-    DevBuf<int> mcnt, mflag, moff, midx, w_ptr, w_idx, e_mov, f_ptr, f_idx, fcnt, skey, pcnt, eidx, sidx, w_rank;
-    DevBuf<unsigned int> smax;
-    DevBuf<unsigned long long> ssum;
+    DevBuf<int> mcnt, mflag, moff, midx, w_ptr, w_idx, e_mov, f_ptr, f_idx, skey, eidx, sidx, w_rank;
+    DevBuf<int> hkey, hkey2, hoff;  // half-edges of the pairs (both ends), sorted by firm
+    DevBuf<float> hval, hval2;
     DevBuf<float> xf, xw;
     DevBuf<unsigned char> tmp;
     int* host_int = nullptr;

@@ -239,61 +239,91 @@ __global__ void __launch_bounds__(kB) block_off_kernel(const int* pf, const int*
 // Generalized inverse of each block (symmetric positive semi-definite):
 // L D L' with a pivot dropped when it falls to 1e-10 of its diagonal entry
 // (an isolated group of firms: its constant vector is in the null space of
-// S), then inv = L^-T D^+ L^-1.  One thread per block, FP64; stored in FP32.
-__global__ void __launch_bounds__(64) block_invert_kernel(const int* start, const int* size, const int* row, int nb,
-                                                          const double* bmat, float* inv) {
-    for (int i = blockIdx.x * 64 + threadIdx.x; i < nb; i += gridDim.x * 64) {
-        if (start[i] != i) continue;  // block heads only
+// S), then inv = L^-T D^+ L^-1.  One wavefront (the whole 64-thread
+// workgroup) per block, the block in LDS, FP64; stored in FP32.
+__global__ void __launch_bounds__(64) block_invert_kernel(const int* heads, int nblocks, const int* size,
+                                                          const int* row, const double* bmat, float* inv) {
+    __shared__ double A[kCapMax][kCapMax + 1];
+    __shared__ double Y[kCapMax][kCapMax + 1];
+    __shared__ double D[kCapMax], A0[kCapMax];
+    const int t = threadIdx.x;
+    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const int i = heads[b];
         const int s = size[i];
         const int o = row[i];
-        double A[kCapMax][kCapMax], D[kCapMax];
-        for (int a = 0; a < s; ++a)
-            for (int b = 0; b < s; ++b) A[a][b] = bmat[o + a * s + b];
-        // in place: strict lower part of A becomes L
+        for (int e = t; e < s * s; e += 64) A[e / s][e % s] = bmat[o + e];
+        __syncthreads();
+        if (t < s) A0[t] = A[t][t];
+        __syncthreads();
         for (int k = 0; k < s; ++k) {
-            const double a0 = A[k][k];
-            double d = a0;
-            for (int j = 0; j < k; ++j) d -= A[k][j] * A[k][j] * D[j];
-            const bool keep = d > 1e-10 * a0 && a0 > 0.0;
-            D[k] = keep ? d : 0.0;
-            for (int m = k + 1; m < s; ++m) {
-                double v = A[m][k];
-                for (int j = 0; j < k; ++j) v -= A[m][j] * A[k][j] * D[j];
-                A[m][k] = keep ? v / d : 0.0;
+            // A[k][k] holds the pivot after the updates of columns < k
+            const double d = A[k][k];
+            const bool keep = d > 1e-10 * A0[k] && A0[k] > 0.0;
+            __syncthreads();
+            if (t == 0) D[k] = keep ? d : 0.0;
+            if (t > k && t < s) A[t][k] = keep ? A[t][k] / d : 0.0;  // L[t][k]
+            __syncthreads();
+            // trailing update: A[m][n] -= L[m][k] D[k] L[n][k], m, n > k
+            const int r = s - k - 1;
+            for (int e = t; e < r * r; e += 64) {
+                const int m = k + 1 + e / r, n = k + 1 + e % r;
+                if (n <= m) A[m][n] -= A[m][k] * D[k] * A[n][k];
+            }
+            __syncthreads();
+        }
+        // Y = L^-1, one lane per column
+        if (t < s) {
+            for (int a = 0; a < s; ++a) Y[a][t] = a == t ? 1.0 : 0.0;
+            for (int a = t + 1; a < s; ++a) {
+                double v = 0.0;
+                for (int k = t; k < a; ++k) v -= A[a][k] * Y[k][t];
+                Y[a][t] = v;
             }
         }
-        // Y = L^-1 (unit lower), stored in the upper part's place through a second array
-        double Y[kCapMax][kCapMax];
-        for (int a = 0; a < s; ++a) {
-            for (int b = 0; b < s; ++b) Y[a][b] = a == b ? 1.0 : 0.0;
-            for (int b = 0; b < a; ++b) {
-                double v = 0.0;
-                for (int k = b; k < a; ++k) v -= A[a][k] * Y[k][b];
-                Y[a][b] = v;
-            }
+        __syncthreads();
+        for (int e = t; e < s * s; e += 64) {
+            const int a = e / s, c = e % s;
+            double v = 0.0;
+            for (int k = max(a, c); k < s; ++k)
+                if (D[k] > 0.0) v += Y[k][a] * Y[k][c] / D[k];
+            inv[o + e] = static_cast<float>(v);
         }
-        for (int a = 0; a < s; ++a)
-            for (int b = a; b < s; ++b) {
-                double v = 0.0;
-                for (int k = b; k < s; ++k)
-                    if (D[k] > 0.0) v += Y[k][a] * Y[k][b] / D[k];
-                inv[o + a * s + b] = static_cast<float>(v);
-                inv[o + b * s + a] = static_cast<float>(v);
-            }
+        __syncthreads();
     }
 }
 
+__global__ void __launch_bounds__(kB) head_flag_kernel(const int* start, int nb, int* flag) {
+    for (int i = blockIdx.x * kB + threadIdx.x; i < nb; i += gridDim.x * kB) flag[i] = start[i] == i ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kB) head_compact_kernel(const int* flag, const int* idx, int nb, int* heads) {
+    for (int i = blockIdx.x * kB + threadIdx.x; i < nb; i += gridDim.x * kB)
+        if (flag[i]) heads[idx[i]] = i;
+}
+
+// One thread per (block, column): the block's r values once, then every row
+// of its inverse.
 __global__ void __launch_bounds__(kB) block_apply_kernel(BlockView b, const float* __restrict__ r,
                                                          float* __restrict__ z, int C) {
-    const long long total = static_cast<long long>(b.nb) * C;
+    const long long total = static_cast<long long>(b.nblocks) * C;
     for (long long j = blockIdx.x * static_cast<long long>(kB) + threadIdx.x; j < total;
          j += static_cast<long long>(gridDim.x) * kB) {
-        const int i = static_cast<int>(j / C), c = static_cast<int>(j % C);
-        const int st = b.start[i], s = b.size[i];
-        const float* row = b.inv + b.row[i];
-        float acc = 0.0f;
-        for (int k = 0; k < s; ++k) acc += row[k] * r[static_cast<size_t>(b.firm[st + k]) * C + c];
-        z[static_cast<size_t>(b.firm[i]) * C + c] = acc;
+        const int h = b.heads[static_cast<int>(j / C)], c = static_cast<int>(j % C);
+        const int s = b.size[h];
+        const float* inv = b.inv + b.row[h];
+        float rv[kCapMax];
+        int fv[kCapMax];
+#pragma unroll
+        for (int k = 0; k < kCapMax; ++k) {
+            fv[k] = k < s ? b.firm[h + k] : 0;
+            rv[k] = k < s ? r[static_cast<size_t>(fv[k]) * C + c] : 0.0f;
+        }
+        for (int a = 0; a < s; ++a) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kCapMax; ++k) acc += (k < s ? inv[a * s + k] : 0.0f) * rv[k];
+            z[static_cast<size_t>(b.firm[h + a]) * C + c] = acc;
+        }
     }
 }
 
@@ -325,9 +355,9 @@ void inclusive_scan(DevBuf<unsigned char>& tmp, const T* in, T* out, size_t m, h
 }  // namespace
 
 hipError_t launch_block_apply(const BlockView& b, const float* r, float* z, int C, hipStream_t s) {
-    if (b.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(block_apply_kernel, dim3(grid_of(static_cast<long long>(b.nb) * C)), dim3(kB), 0, s, b, r, z,
-                       C);
+    if (b.nb <= 0 || b.nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(block_apply_kernel, dim3(grid_of(static_cast<long long>(b.nblocks) * C)), dim3(kB), 0, s, b,
+                       r, z, C);
     return hipGetLastError();
 }
 
@@ -438,8 +468,18 @@ void AggPrecond::build_blocks(hipStream_t s) {
                        bmat.p);
     hipLaunchKernelGGL(block_off_kernel, dim3(grid_of(U)), dim3(kB), 0, s, pf.p, pg.p, W.p, U, agg.p, bsz.p, boff2.p,
                        loc.p, bmat.p);
-    hipLaunchKernelGGL(block_invert_kernel, dim3(grid_of(nb, 64)), dim3(64), 0, s, b_start.p, b_size.p, b_row.p, nb,
-                       bmat.p, binv.p);
+    // block heads: the first blocked firm of each block
+    b_hflag.ensure(nb); b_hidx.ensure(nb);
+    hipLaunchKernelGGL(head_flag_kernel, dim3(grid_of(nb)), dim3(kB), 0, s, b_start.p, nb, b_hflag.p);
+    exclusive_scan<int>(tmp, b_hflag.p, b_hidx.p, static_cast<size_t>(nb), s);
+    XFE_CHECK(hipMemcpyAsync(host_int, b_hidx.p + (nb - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    XFE_CHECK(hipMemcpyAsync(host_int + 1, b_hflag.p + (nb - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    XFE_CHECK(hipStreamSynchronize(s));
+    nblocks = host_int[0] + host_int[1];
+    b_head.ensure(nblocks);
+    hipLaunchKernelGGL(head_compact_kernel, dim3(grid_of(nb)), dim3(kB), 0, s, b_hflag.p, b_hidx.p, nb, b_head.p);
+    hipLaunchKernelGGL(block_invert_kernel, dim3(std::max(1, std::min(nblocks, 16384))), dim3(64), 0, s, b_head.p,
+                       nblocks, b_size.p, b_row.p, bmat.p, binv.p);
     XFE_CHECK(hipGetLastError());
 }
 
@@ -447,7 +487,7 @@ void AggPrecond::release() {
     for (auto* b : {&wcnt, &woff}) b->release();
     key_a.release(); key_b.release(); val_a.release(); val_b.release();
     for (auto* b : {&head, &run, &pos, &pf, &pg, &agg, &asize, &newroot, &cnt, &bsz, &bsq, &bstart, &boff2, &skey,
-                    &sval, &iota, &loc, &b_firm, &b_start, &b_size, &b_row})
+                    &sval, &iota, &loc, &b_firm, &b_start, &b_size, &b_row, &b_head, &b_hflag, &b_hidx})
         b->release();
     W.release(); dex.release(); st.release(); best.release(); bmat.release(); binv.release(); tmp.release();
     if (host_int) (void)hipHostFree(host_int);
