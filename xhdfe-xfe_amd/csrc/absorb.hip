@@ -121,8 +121,11 @@ struct Workspace {
     // segment form of the Schur operator (worker-major and firm-major records)
     DevBuf<int> seg_start, seg_w, seg_r1, seg_len, seg_a, wseg, f_key, f_order, f_w, f_len, f_a;
     DevBuf<int> w_pf, w_pi, w_rank, seg_rk, p_order, p_rk, p_r1, p_len, p_a, p_wseg;
-    DevBuf<float> f_wmean, f_hpart, f_invw;
-    DevBuf<double> d_wmean, d_hpart, seg_tot, f_b0, f_tmp, d_invw;
+    DevBuf<float> f_wmean, f_hpart, f_invw, f_invwm;
+    DevBuf<double> d_wmean, d_hpart, seg_tot, f_b0, f_tmp, d_invw, d_invwm;
+    DevBuf<int> m_flag, m_idx, m_list;  // movers among the worker ranks
+    DevBuf<int> fm_key, fm_w, fm_len, fm_a, stay_h;  // movers' firm-major records; stayer histogram
+    DevBuf<double> stay_K;              // the stayers' year block (G2 x G2)
     DevBuf<int> seg_bk[2];
     DevBuf<float> f_bv[2];
     DevBuf<double> d_bv[2];
@@ -169,7 +172,9 @@ struct Workspace {
         f_b0.release(); f_tmp.release();
         w_pf.release(); w_pi.release(); w_rank.release(); seg_rk.release(); p_order.release();
         p_rk.release(); p_r1.release(); p_len.release(); p_a.release(); p_wseg.release();
-        f_invw.release(); d_invw.release();
+        f_invw.release(); d_invw.release(); f_invwm.release(); d_invwm.release();
+        m_flag.release(); m_idx.release(); m_list.release();
+        fm_key.release(); fm_w.release(); fm_len.release(); fm_a.release(); stay_h.release(); stay_K.release();
         for (int k = 0; k < 2; ++k) { seg_bk[k].release(); f_bv[k].release(); d_bv[k].release(); }
         rec_p.release(); rec_contrib.release();
         for (auto& b : rec_alpha) b.release();
@@ -818,6 +823,9 @@ struct FastSolver {
     // dimension r2 (-1 without one)
     bool seg = false;
     int k1 = 0, k2 = -1, M = 0, Nw = 0, G2 = 0, seg_grid_w = 1, seg_grid_f = 1;
+    int Nm = 0;  // movers (workers with two or more segments)
+    int Mf = 0;  // their segments: the firm-major records of the firm pass
+    bool stay_years = false;  // stayers add the year block stay_K
     bool empty_firms = true;  // some level of r1 has no segment
     // aggregate-block preconditioner on the firm levels (segment form only):
     // precond 0 Jacobi, 1 aggregates from the coupling strengths, 2 from the
@@ -914,9 +922,77 @@ struct FastSolver {
         XFE_CHECK(launch_seg_gather<int>(ws.p_rk.p, ws.f_order.p, ws.f_w.p, M, mgrid, s));
         XFE_CHECK(launch_seg_gather<int>(ws.p_len.p, ws.f_order.p, ws.f_len.p, M, mgrid, s));
         if (k2 >= 0) XFE_CHECK(launch_seg_gather<int>(ws.p_a.p, ws.f_order.p, ws.f_a.p, M, mgrid, s));
-        // firms without segments (their sums must be cleared before each apply)
+        // movers: only their worker means are formed and read (a stayer's one
+        // segment is handled by the firm pass alone)
+        ws.m_flag.ensure(Nw); ws.m_idx.ensure(Nw); ws.m_list.ensure(Nw);
+        XFE_CHECK(launch_seg_movers(ws.p_wseg.p, Nw, ws.m_flag.p, wgrid, s));
+        {
+            size_t sb = 0;
+            XFE_CHECK(rocprim::exclusive_scan(nullptr, sb, ws.m_flag.p, ws.m_idx.p, 0, static_cast<size_t>(Nw),
+                                              rocprim::plus<int>(), s));
+            ws.sort_tmp.ensure(sb);
+            XFE_CHECK(rocprim::exclusive_scan(ws.sort_tmp.p, sb, ws.m_flag.p, ws.m_idx.p, 0, static_cast<size_t>(Nw),
+                                              rocprim::plus<int>(), s));
+        }
+        int mcount[2] = {0, 0};
+        XFE_CHECK(hipMemcpyAsync(&mcount[0], ws.m_idx.p + (Nw - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipMemcpyAsync(&mcount[1], ws.m_flag.p + (Nw - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        Nm = mcount[0] + mcount[1];
+        XFE_CHECK(launch_seg_mover_relabel(ws.m_flag.p, ws.m_idx.p, Nw, M, ws.m_list.p, ws.f_w.p, ws.f_len.p,
+                                           std::max(wgrid, mgrid), s));
+        ws.f_invwm.ensure(std::max(Nm, 1)); ws.d_invwm.ensure(std::max(Nm, 1));
+        if (Nm > 0) {
+            XFE_CHECK(launch_seg_gather<float>(ws.f_invw.p, ws.m_list.p, ws.f_invwm.p, Nm, grid_for(Nm), s));
+            XFE_CHECK(launch_seg_gather<double>(ws.d_invw.p, ws.m_list.p, ws.d_invwm.p, Nm, grid_for(Nm), s));
+        }
+        // the firm pass takes the movers' segments only (a stayer's segment adds
+        // nothing to its firm); the stayers' year block is the fixed matrix stay_K
+        XFE_CHECK(launch_seg_mover_segments(ws.f_len.p, M, ws.tmp_flags.p, mgrid, s));
+        {
+            size_t sb = 0;
+            XFE_CHECK(rocprim::exclusive_scan(nullptr, sb, ws.tmp_flags.p, ws.tmp_inc.p, 0, static_cast<size_t>(M),
+                                              rocprim::plus<int>(), s));
+            ws.sort_tmp.ensure(sb);
+            XFE_CHECK(rocprim::exclusive_scan(ws.sort_tmp.p, sb, ws.tmp_flags.p, ws.tmp_inc.p, 0,
+                                              static_cast<size_t>(M), rocprim::plus<int>(), s));
+        }
+        XFE_CHECK(hipMemcpyAsync(&mcount[0], ws.tmp_inc.p + (M - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipMemcpyAsync(&mcount[1], ws.tmp_flags.p + (M - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        Mf = mcount[0] + mcount[1];
+        if (Mf <= 0) return false;
+        ws.fm_key.ensure(Mf); ws.fm_w.ensure(Mf); ws.fm_len.ensure(Mf); ws.fm_a.ensure(Mf);
+        XFE_CHECK(launch_seg_compact(ws.tmp_flags.p, ws.tmp_inc.p, M, ws.f_key.p, ws.f_w.p, ws.f_len.p,
+                                     k2 >= 0 ? ws.f_a.p : nullptr, ws.fm_key.p, ws.fm_w.p, ws.fm_len.p, ws.fm_a.p,
+                                     mgrid, s));
+        stay_years = false;
+        if (k2 >= 0 && Nw > Nm) {
+            const int E = G2 * (G2 + 1);
+            ws.stay_h.ensure(E);
+            XFE_CHECK(hipMemsetAsync(ws.stay_h.p, 0, sizeof(int) * E, s));
+            XFE_CHECK(launch_seg_stayer_hist(ws.f_len.p, ws.f_a.p, M, G2, ws.stay_h.p, mgrid, s));
+            std::vector<int> H(E);
+            XFE_CHECK(hipMemcpyAsync(H.data(), ws.stay_h.p, sizeof(int) * E, hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+            // K[y][y'] = -sum over stayer segments covering y and y' of 1 / L
+            std::vector<double> K(static_cast<size_t>(G2) * G2, 0.0);
+            for (int a0 = 0; a0 < G2; ++a0)
+                for (int L = 1; a0 + L <= G2; ++L) {
+                    const int cnt = H[a0 * (G2 + 1) + L];
+                    if (!cnt) continue;
+                    const double v = static_cast<double>(cnt) / L;
+                    for (int y = a0; y < a0 + L; ++y)
+                        for (int t = a0; t < a0 + L; ++t) K[static_cast<size_t>(y) * G2 + t] -= v;
+                }
+            ws.stay_K.ensure(K.size());
+            XFE_CHECK(hipMemcpyAsync(ws.stay_K.p, K.data(), sizeof(double) * K.size(), hipMemcpyHostToDevice, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+            stay_years = true;
+        }
+        // firms without a mover segment (their sums must be cleared before each apply)
         ws.w_pf.ensure(std::max(Nw, G1 + 1));
-        XFE_CHECK(launch_seg_offsets(ws.f_key.p, M, G1, ws.w_pf.p, mgrid, s));
+        XFE_CHECK(launch_seg_offsets(ws.fm_key.p, Mf, G1, ws.w_pf.p, grid_for(Mf), s));
         ws.flag.ensure(1);
         XFE_CHECK(hipMemsetAsync(ws.flag.p, 0, sizeof(int), s));
         XFE_CHECK(launch_seg_empty(ws.w_pf.p, G1, ws.flag.p, grid_for(G1), s));
@@ -924,11 +1000,11 @@ struct FastSolver {
         XFE_CHECK(hipMemcpyAsync(&nempty, ws.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
         XFE_CHECK(hipStreamSynchronize(s));
         empty_firms = nempty > 0;
-        seg_grid_w = grid_for(Nw);
-        seg_grid_f = mgrid;
+        seg_grid_w = grid_for(std::max(Nm, 1));
+        seg_grid_f = grid_for(Mf);
         const size_t tabs = static_cast<size_t>(seg_grid_f) * (G2 + 1) * 8;
-        ws.f_wmean.ensure(static_cast<size_t>(Nw) * 8);
-        ws.d_wmean.ensure(static_cast<size_t>(Nw) * 8);
+        ws.f_wmean.ensure(static_cast<size_t>(std::max(Nm, 1)) * 8);
+        ws.d_wmean.ensure(static_cast<size_t>(std::max(Nm, 1)) * 8);
         ws.f_hpart.ensure(tabs);
         ws.d_hpart.ensure(tabs);
         ws.seg_tot.ensure(static_cast<size_t>(G2 + 1) * 8);
@@ -955,7 +1031,7 @@ struct FastSolver {
     // rest in one block.
     template <typename T>
     void seg_boundaries(T* out1, int stride, int c0, int cc) {
-        int m = static_cast<int>(2 * ((static_cast<long long>(M) + kWave - 1) / kWave));
+        int m = static_cast<int>(2 * ((static_cast<long long>(Mf) + kWave - 1) / kWave));
         int cur = 0;
         while (m > kBndTail) {
             XFE_CHECK(launch_seg_bnd<T>(ws.seg_bk[cur].p, bv_buf<T>(cur), m, out1, stride, c0, cc,
@@ -970,6 +1046,12 @@ struct FastSolver {
     // out = S x through the segment records (T = float for the inner CG,
     // double for the refinement residual).
     template <typename T>
+    const T* inv_movers() const {
+        if constexpr (std::is_same<T, float>::value) return ws.f_invwm.p;
+        else return ws.d_invwm.p;
+    }
+
+    template <typename T>
     void seg_apply(const T* x, T* out, const T* inv_w, T* wmean, double* hpart) {
         // every firm with a segment and every year is written; clear only when some firm has none
         if (empty_firms) XFE_CHECK(hipMemsetAsync(out, 0, sizeof(T) * static_cast<size_t>(N) * C, s));
@@ -977,43 +1059,48 @@ struct FastSolver {
             const int c0 = ch * 8;
             const int cc = std::min(8, C - c0);
             SegArgs<T> a;
-            a.C = C; a.c0 = c0; a.Nw = Nw; a.M = M; a.G2 = G2;
+            a.C = C; a.c0 = c0; a.Nw = Nw; a.M = Mf; a.G2 = G2;
+            a.Nm = Nm; a.mlist = ws.m_list.p; a.inv_wm = inv_movers<T>();
             a.x1 = x + static_cast<size_t>(off[k1]) * C;
             a.x2 = k2 >= 0 ? x + static_cast<size_t>(off[k2]) * C : nullptr;
             a.inv_w = inv_w;
             a.wseg = ws.p_wseg.p; a.seg_r1 = ws.p_r1.p; a.seg_len = ws.p_len.p; a.seg_a = ws.p_a.p;
             a.wmean = wmean;
-            a.f_key = ws.f_key.p; a.f_w = ws.f_w.p; a.f_len = ws.f_len.p; a.f_a = ws.f_a.p;
+            a.f_key = ws.fm_key.p; a.f_w = ws.fm_w.p; a.f_len = ws.fm_len.p; a.f_a = ws.fm_a.p;
             a.out1 = out + static_cast<size_t>(off[k1]) * C;
             a.hpart = hpart;
             a.bkey = ws.seg_bk[0].p;
             a.bval = bv_buf<T>(0);
-            XFE_CHECK(launch_seg_worker<T>(a, cc, k2 >= 0, seg_grid_w, s));
+            if (Nm > 0) XFE_CHECK(launch_seg_worker<T>(a, cc, k2 >= 0, seg_grid_w, s));
             const int pslot = profile_begin(s);
             XFE_CHECK(launch_seg_firm<T>(a, cc, k2 >= 0, false, seg_grid_f, s));
             if (pslot >= 0) {
-                // the firm pass alone: segment records (key, worker, length, start), one worker
-                // mean per segment, the firm coefficients read and written once
+                // the firm pass alone: the movers' segment records (key, mover, length,
+                // start), one worker mean per segment, the firm coefficients read and
+                // written once
                 const double e = sizeof(T);
                 const double G1 = sv.G[rdim[k1]];
                 profile_end(pslot, s,
-                            static_cast<double>(M) * (12.0 + (k2 >= 0 ? 4.0 : 0.0) + e * cc) + 2.0 * e * cc * G1);
+                            static_cast<double>(Mf) * (12.0 + (k2 >= 0 ? 4.0 : 0.0) + e * cc) + 2.0 * e * cc * G1);
             }
             seg_boundaries<T>(a.out1, C, c0, cc);
             if (k2 >= 0)
                 XFE_CHECK(launch_seg_year<T>(hpart, seg_grid_f, G2, cc, ws.seg_tot.p, ws.wsum[rdim[k2]].p,
-                                             a.x2, C, c0, out + static_cast<size_t>(off[k2]) * C, s));
+                                             a.x2, C, c0, stay_years ? ws.stay_K.p : nullptr,
+                                             out + static_cast<size_t>(off[k2]) * C, s));
         }
     }
 
-    // Jacobi diagonal from the segments: firms L - L^2 / n_w per segment,
-    // years n_y - sum of 1 / n_w over the segments covering the year.
+    // Jacobi diagonal from the segments: firms L - L^2 / n_w per mover segment
+    // (0 for a stayer's), years n_y - sum of 1 / n_w over the segments covering
+    // the year (the stayers' part: the diagonal of stay_K).
     void seg_diag() {
         XFE_CHECK(hipMemsetAsync(ws.f_diag.p, 0, sizeof(double) * N, s));
         SegArgs<double> a;
-        a.C = 1; a.c0 = 0; a.Nw = Nw; a.M = M; a.G2 = G2;
+        a.C = 1; a.c0 = 0; a.Nw = Nw; a.M = Mf; a.G2 = G2;
         a.inv_w = ws.d_invw.p;
-        a.f_key = ws.f_key.p; a.f_w = ws.f_w.p; a.f_len = ws.f_len.p; a.f_a = ws.f_a.p;
+        a.Nm = Nm; a.mlist = ws.m_list.p; a.inv_wm = ws.d_invwm.p;
+        a.f_key = ws.fm_key.p; a.f_w = ws.fm_w.p; a.f_len = ws.fm_len.p; a.f_a = ws.fm_a.p;
         a.out1 = ws.f_diag.p + off[k1];
         a.hpart = ws.d_hpart.p;
         a.bkey = ws.seg_bk[0].p;
@@ -1022,7 +1109,8 @@ struct FastSolver {
         seg_boundaries<double>(a.out1, 1, 0, 1);
         if (k2 >= 0)
             XFE_CHECK(launch_seg_year<double>(ws.d_hpart.p, seg_grid_f, G2, 1, ws.seg_tot.p,
-                                              ws.wsum[rdim[k2]].p, nullptr, 1, 0, ws.f_diag.p + off[k2], s));
+                                              ws.wsum[rdim[k2]].p, nullptr, 1, 0, stay_years ? ws.stay_K.p : nullptr,
+                                              ws.f_diag.p + off[k2], s));
     }
 
     // Refinement residual b = b0 - S beta in FP64 through the segments.

@@ -130,15 +130,18 @@ struct SegArgs {
     int G2 = 0;                   // levels of r2 (0: no r2)
     const T* x1 = nullptr;        // coefficients of r1 (G1 x C)
     const T* x2 = nullptr;        // coefficients of r2 (G2 x C)
-    const T* inv_w = nullptr;     // 1 / n_w
+    const T* inv_w = nullptr;     // 1 / n_w (per worker rank)
+    int Nm = 0;                   // movers: workers with two or more segments
+    const int* mlist = nullptr;   // worker rank of each mover
+    const T* inv_wm = nullptr;    // 1 / n_w per mover
     const int* wseg = nullptr;    // Nw + 1 segment offsets per level of p
     const int* seg_r1 = nullptr;  // worker-major segment records
     const int* seg_len = nullptr;
     const int* seg_a = nullptr;
-    T* wmean = nullptr;           // Nw x cc means of u over p
+    T* wmean = nullptr;           // Nm x cc means of u over the movers
     const int* f_key = nullptr;   // firm-major segment records (sorted by r1)
-    const int* f_w = nullptr;
-    const int* f_len = nullptr;
+    const int* f_w = nullptr;     // mover index (segments of movers only)
+    const int* f_len = nullptr;   // length; negative for the one segment of a stayer
     const int* f_a = nullptr;
     T* out1 = nullptr;            // (S x)_1, zeroed before the pass
     double* hpart = nullptr;      // per-block (G2 + 1) x cc year differences
@@ -153,6 +156,12 @@ hipError_t launch_seg_fill(const int* flags, const int* inc, const int* gp, cons
                            hipStream_t s);
 hipError_t launch_seg_len(const int* seg_start, int M, int n, int* seg_len, int grid, hipStream_t s);
 hipError_t launch_seg_offsets(const int* key, int M, int Nw, int* off, int grid, hipStream_t s);
+// flag[w] = 1 for workers with two or more segments
+hipError_t launch_seg_movers(const int* wseg, int Nw, int* flag, int grid, hipStream_t s);
+// mlist[idx[w]] = w for movers; firm-major records: f_w = mover index for a
+// mover's segments, f_len negated for a stayer's
+hipError_t launch_seg_mover_relabel(const int* flag, const int* idx, int Nw, int M, int* mlist, int* f_w, int* f_len,
+                                    int grid, hipStream_t s);
 hipError_t launch_seg_primary(const int* wseg, const int* seg_r1, int Nw, int G1, int* pf, int grid,
                               hipStream_t s);
 hipError_t launch_seg_rank(const int* perm, int m, int* rank, int grid, hipStream_t s);
@@ -174,7 +183,15 @@ hipError_t launch_seg_bnd_tail(int* k0, T* v0, int* k1, T* v1, int m, T* out, in
                                hipStream_t s);
 template <typename T>
 hipError_t launch_seg_year(const double* hpart, int nblk, int G2, int cc, double* tot, const double* counts,
-                           const T* x2, int C, int c0, T* out2, hipStream_t s);
+                           const T* x2, int C, int c0, const double* K, T* out2, hipStream_t s);
+// stayer segments (negative length) counted by (first year, length) into H (G2 x (G2 + 1), zeroed)
+hipError_t launch_seg_stayer_hist(const int* f_len, const int* f_a, int M, int G2, int* H, int grid, hipStream_t s);
+// flag[j] = 1 for the records with a positive length (movers' segments)
+hipError_t launch_seg_mover_segments(const int* f_len, int M, int* flag, int grid, hipStream_t s);
+// the flagged records, in order, at positions idx (exclusive scan of flag); a_in may be null
+hipError_t launch_seg_compact(const int* flag, const int* idx, int M, const int* k_in, const int* w_in,
+                              const int* l_in, const int* a_in, int* k_out, int* w_out, int* l_out, int* a_out,
+                              int grid, hipStream_t s);
 
 template <typename T>
 hipError_t launch_schur(const SchurArgs<T>& a, int cc, int nr, bool has_v, bool store,

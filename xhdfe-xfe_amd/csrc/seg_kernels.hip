@@ -171,6 +171,28 @@ __global__ void __launch_bounds__(kBlock) seg_empty_kernel(const int* off, int G
         if (off[f] == off[f + 1]) atomicAdd(flag, 1);
 }
 
+// movers (two or more segments) among the worker ranks
+__global__ void __launch_bounds__(kBlock) seg_mover_flag_kernel(const int* wseg, int Nw, int* flag) {
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < Nw; w += gridDim.x * kBlock)
+        flag[w] = wseg[w + 1] - wseg[w] >= 2 ? 1 : 0;
+}
+
+// mover list, and the firm-major records relabelled: a mover's segments carry
+// its mover index, a stayer's segment a negative length
+__global__ void __launch_bounds__(kBlock) seg_mover_list_kernel(const int* flag, const int* idx, int Nw, int* mlist) {
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < Nw; w += gridDim.x * kBlock)
+        if (flag[w]) mlist[idx[w]] = w;
+}
+
+__global__ void __launch_bounds__(kBlock) seg_mover_relabel_kernel(const int* flag, const int* idx, int M, int* f_w,
+                                                                   int* f_len) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < M; j += gridDim.x * kBlock) {
+        const int w = f_w[j];
+        if (flag[w]) f_w[j] = idx[w];
+        else f_len[j] = -f_len[j];
+    }
+}
+
 // rank[perm[t]] = t
 __global__ void __launch_bounds__(kBlock) seg_rank_kernel(const int* perm, int m, int* rank) {
     for (int t = blockIdx.x * kBlock + threadIdx.x; t < m; t += gridDim.x * kBlock) rank[perm[t]] = t;
@@ -183,13 +205,19 @@ __global__ void __launch_bounds__(kBlock) seg_gather_kernel(const T* src, const 
 
 // ------------------------------------------------------------------ apply
 
-// Pass 1 (one thread per worker): m_w = (1 / n_w) sum_{s in w} U_s.
+// Pass 1 (one thread per mover): m_w = (1 / n_w) sum_{s in w} U_s.  A stayer
+// (one segment s, n_w = L) needs no mean: x1[f] - m_w = -(P[a + L] - P[a]) / L,
+// so its segment adds L (x1[f] - m_w) + (P[a + L] - P[a]) = 0 to the firm and
+// -(1 / L) sum_{y' in s} x2[y'] to each year y in s: over all stayers the
+// fixed matrix K[y][y'] = -sum_{stayer segments s containing y and y'} 1 / L_s
+// (seg_year_kernel).
 template <typename T, int CC, bool HAS_Y>
 __global__ void __launch_bounds__(kBlock) seg_worker_kernel(SegArgs<T> a) {
     extern __shared__ __align__(16) unsigned char seg_dyn[];
     T* P = reinterpret_cast<T*>(seg_dyn);
     if constexpr (HAS_Y) build_prefix<T, CC>(a.x2, a.G2, a.C, a.c0, P);
-    for (int w = blockIdx.x * kBlock + threadIdx.x; w < a.Nw; w += gridDim.x * kBlock) {
+    for (int mi = blockIdx.x * kBlock + threadIdx.x; mi < a.Nm; mi += gridDim.x * kBlock) {
+        const int w = a.mlist[mi];
         const int s0 = a.wseg[w], s1 = a.wseg[w + 1];
         T acc[CC];
 #pragma unroll
@@ -208,18 +236,19 @@ __global__ void __launch_bounds__(kBlock) seg_worker_kernel(SegArgs<T> a) {
             }
         }
         const T iw = a.inv_w[w];
-        T* m = a.wmean + static_cast<size_t>(w) * CC;
+        T* m = a.wmean + static_cast<size_t>(mi) * CC;
 #pragma unroll
         for (int c = 0; c < CC; ++c) m[c] = acc[c] * iw;
     }
 }
 
-// Pass 2 (one thread per segment, segments in firm order): (S x)_1 by a
+// Pass 2 (one thread per mover segment, segments in firm order): (S x)_1 by a
 // segmented sum over equal firms, and the year differences into an FP64 LDS
 // histogram with one copy per wave (FP32 LDS atomics on the few year slots
 // stall the LDS pipe), summed into hpart[block].  DIAG: the Jacobi diagonal
 // instead (T = double, CC = 1): L - L^2 / n_w per segment, and -1 / n_w over
-// its years.
+// its years.  Stayers' segments are not in the list: they add nothing to the
+// firms, and to the years a fixed linear map of x2 (seg_year_kernel's K).
 template <typename T, int CC, bool HAS_Y, bool DIAG>
 __global__ void __launch_bounds__(kBlock) seg_firm_kernel(SegArgs<T> a) {
     extern __shared__ __align__(16) unsigned char seg_dyn[];
@@ -245,7 +274,7 @@ __global__ void __launch_bounds__(kBlock) seg_firm_kernel(SegArgs<T> a) {
             const int L = a.f_len[j];
             const T tl = static_cast<T>(L);
             if constexpr (DIAG) {
-                const T iw = a.inv_w[w];
+                const T iw = a.inv_wm[w];
                 v[0] = tl - tl * tl * iw;
                 if constexpr (HAS_Y) {
                     const int y0 = a.f_a[j];
@@ -364,10 +393,12 @@ __global__ void __launch_bounds__(kBlock) seg_hsum_kernel(const double* hpart, i
 // (S x)_2[y] = n_y x2[y] + prefix_y(tot); with x2 == nullptr the result is
 // n_y + prefix_y(tot) (the Jacobi diagonal of the year block).  The block
 // stages tot, n_y and x2 in LDS, one thread per column scans.
+// K (G2 x G2, may be null): the stayers' year block, added as K x2 (or its
+// diagonal when x2 == nullptr).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 seg_year_kernel(const double* tot, int G2, int cc, const double* counts, const T* x2, int C, int c0,
-                T* out2) {
+                const double* K, T* out2) {
     __shared__ double st[(kSegMaxRange + 1) * 8];
     __shared__ double sx[kSegMaxRange * 8];
     __shared__ double sn[kSegMaxRange];
@@ -384,7 +415,58 @@ seg_year_kernel(const double* tot, int G2, int cc, const double* counts, const T
         const int c = k - y * cc;
         double run = 0.0;
         for (int t = 0; t <= y; ++t) run += st[t * cc + c];
-        out2[static_cast<size_t>(y) * C + c0 + c] = static_cast<T>(sn[y] * sx[k] + run);
+        double ks = 0.0;
+        if (K) {
+            if (x2) {
+                for (int t = 0; t < G2; ++t) ks += K[static_cast<size_t>(y) * G2 + t] * sx[t * cc + c];
+            } else {
+                ks = K[static_cast<size_t>(y) * G2 + y];
+            }
+        }
+        out2[static_cast<size_t>(y) * C + c0 + c] = static_cast<T>(sn[y] * sx[k] + run + ks);
+    }
+}
+
+// Stayer segments (negative length in the firm-major records) counted by
+// (first year, length): H[a * (G2 + 1) + L]; LDS copy per block when it fits.
+__global__ void __launch_bounds__(kBlock) seg_stayer_hist_kernel(const int* f_len, const int* f_a, int M, int G2,
+                                                                 int* H) {
+    extern __shared__ __align__(16) int hl[];
+    const int E = G2 * (G2 + 1);
+    const bool lds = E <= 16384;
+    if (lds) {
+        for (int k = threadIdx.x; k < E; k += kBlock) hl[k] = 0;
+        __syncthreads();
+    }
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < M; j += gridDim.x * kBlock) {
+        const int Ls = f_len[j];
+        if (Ls >= 0) continue;
+        const int idx = f_a[j] * (G2 + 1) - Ls;
+        if (lds) atomicAdd(&hl[idx], 1);
+        else atomicAdd(&H[idx], 1);
+    }
+    if (lds) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < E; k += kBlock)
+            if (hl[k]) atomicAdd(&H[k], hl[k]);
+    }
+}
+
+// mover segments of the firm-major records, compacted in order
+__global__ void __launch_bounds__(kBlock) seg_mover_flag_seg_kernel(const int* f_len, int M, int* flag) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < M; j += gridDim.x * kBlock) flag[j] = f_len[j] > 0 ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) seg_compact_kernel(const int* flag, const int* idx, int M, const int* k_in,
+                                                             const int* w_in, const int* l_in, const int* a_in,
+                                                             int* k_out, int* w_out, int* l_out, int* a_out) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < M; j += gridDim.x * kBlock) {
+        if (!flag[j]) continue;
+        const int o = idx[j];
+        k_out[o] = k_in[j];
+        w_out[o] = w_in[j];
+        l_out[o] = l_in[j];
+        if (a_in) a_out[o] = a_in[j];
     }
 }
 
@@ -417,6 +499,18 @@ hipError_t launch_seg_offsets(const int* key, int M, int Nw, int* off, int grid,
 hipError_t launch_seg_primary(const int* wseg, const int* seg_r1, int Nw, int G1, int* pf, int grid,
                               hipStream_t s) {
     hipLaunchKernelGGL(seg_primary_kernel, dim3(grid), dim3(kBlock), 0, s, wseg, seg_r1, Nw, G1, pf);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_movers(const int* wseg, int Nw, int* flag, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(seg_mover_flag_kernel, dim3(grid), dim3(kBlock), 0, s, wseg, Nw, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_mover_relabel(const int* flag, const int* idx, int Nw, int M, int* mlist, int* f_w, int* f_len,
+                                    int grid, hipStream_t s) {
+    hipLaunchKernelGGL(seg_mover_list_kernel, dim3(grid), dim3(kBlock), 0, s, flag, idx, Nw, mlist);
+    hipLaunchKernelGGL(seg_mover_relabel_kernel, dim3(grid), dim3(kBlock), 0, s, flag, idx, M, f_w, f_len);
     return hipGetLastError();
 }
 
@@ -541,16 +635,36 @@ template hipError_t launch_seg_bnd_tail<double>(int*, double*, int*, double*, in
 
 template <typename T>
 hipError_t launch_seg_year(const double* hpart, int nblk, int G2, int cc, double* tot, const double* counts,
-                           const T* x2, int C, int c0, T* out2, hipStream_t s) {
+                           const T* x2, int C, int c0, const double* K, T* out2, hipStream_t s) {
     const int E = (G2 + 1) * cc;
     hipLaunchKernelGGL(seg_hsum_kernel, dim3(E), dim3(kBlock), 0, s, hpart, nblk, E, tot);
-    hipLaunchKernelGGL((seg_year_kernel<T>), dim3(1), dim3(kBlock), 0, s, tot, G2, cc, counts, x2, C, c0,
+    hipLaunchKernelGGL((seg_year_kernel<T>), dim3(1), dim3(kBlock), 0, s, tot, G2, cc, counts, x2, C, c0, K,
                        out2);
     return hipGetLastError();
 }
 template hipError_t launch_seg_year<float>(const double*, int, int, int, double*, const double*, const float*,
-                                           int, int, float*, hipStream_t);
+                                           int, int, const double*, float*, hipStream_t);
 template hipError_t launch_seg_year<double>(const double*, int, int, int, double*, const double*,
-                                            const double*, int, int, double*, hipStream_t);
+                                            const double*, int, int, const double*, double*, hipStream_t);
+
+hipError_t launch_seg_stayer_hist(const int* f_len, const int* f_a, int M, int G2, int* H, int grid, hipStream_t s) {
+    const int E = G2 * (G2 + 1);
+    const size_t shm = E <= 16384 ? static_cast<size_t>(E) * sizeof(int) : 0;
+    hipLaunchKernelGGL(seg_stayer_hist_kernel, dim3(grid), dim3(kBlock), shm, s, f_len, f_a, M, G2, H);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_mover_segments(const int* f_len, int M, int* flag, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(seg_mover_flag_seg_kernel, dim3(grid), dim3(kBlock), 0, s, f_len, M, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_compact(const int* flag, const int* idx, int M, const int* k_in, const int* w_in,
+                              const int* l_in, const int* a_in, int* k_out, int* w_out, int* l_out, int* a_out,
+                              int grid, hipStream_t s) {
+    hipLaunchKernelGGL(seg_compact_kernel, dim3(grid), dim3(kBlock), 0, s, flag, idx, M, k_in, w_in, l_in, a_in,
+                       k_out, w_out, l_out, a_out);
+    return hipGetLastError();
+}
 
 }  // namespace xfe
