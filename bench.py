@@ -143,24 +143,64 @@ def bench_akm(args):
     rng = np.random.default_rng(20261503)
     rhs = rng.choice([-1.0, 1.0], size=(lanes, J))
     max_iter = 20000
+    # the wide solver: all 256 lanes in one FP32 pass per iteration, FP64
+    # residual replacement, right-hand sides and solutions resident in HBM
+    # (timed; its pair pass is the roofline kernel)
+    import torch
+    rhs_d = torch.from_numpy(rhs).to("cuda:0")
+    z_d = torch.empty_like(rhs_d)
+    torch.cuda.synchronize()
     for w in range(args.warmup):
         t0 = time.perf_counter()
-        _, its = solver.solve_multi(rhs[:batch], tol=1e-10, max_iter=max_iter)
-        log(f"warmup {w}: {time.perf_counter() - t0:.3f} s, iterations {its.min()}..{its.max()}")
+        its, _ = solver.solve_wide_device(rhs_d.data_ptr(), z_d.data_ptr(), lanes, tol=1e-10, max_iter=max_iter)
+        log(f"wide warmup {w}: {time.perf_counter() - t0:.3f} s, iterations {its.min()}..{its.max()}")
     xfe_amd.profile_reset()
     xfe_amd.profile_enable(True)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    all_its = []
     for k in range(args.steps):
-        for b0 in range(0, lanes, batch):
-            _, its = solver.solve_multi(rhs[b0:b0 + batch], tol=1e-10, max_iter=max_iter)
-            if (its < 0).any():
-                raise SystemExit(f"lanes did not converge: {its}")
-            all_its.append(its)
-        log(f"step {k}: {time.perf_counter() - t0:.3f} s so far")
+        its, (passes, updates) = solver.solve_wide_device(rhs_d.data_ptr(), z_d.data_ptr(), lanes, tol=1e-10,
+                                                          max_iter=max_iter)
+        if (its < 0).any():
+            raise SystemExit(f"lanes did not converge: {its}")
+        log(f"wide step {k}: {time.perf_counter() - t0:.3f} s so far, {passes} FP32 passes, "
+            f"{updates} FP64 replacements")
+    torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / args.steps
     xfe_amd.profile_enable(False)
     pass_ms, launches, pass_bytes = xfe_amd.profile_get()
+    z_wide = z_d.cpu().numpy()
+    # the host-pack entry point (PCIe transfers of the 256 x J packs included)
+    t0 = time.perf_counter()
+    z_host, _, _ = solver.solve_wide(rhs, tol=1e-10, max_iter=max_iter)
+    wide_host_s = time.perf_counter() - t0
+    if not np.array_equal(z_host, z_wide):
+        raise SystemExit("host-pack and device entry points differ")
+    log(f"wide host-pack entry point (PCIe included): {wide_host_s:.3f} s")
+    # the same wide solver with Jacobi alone (no aggregate blocks), timed once
+    os.environ["XFE_AKM_JACOBI"] = "1"
+    jac = xfe_amd.AkmSolver(st)
+    del os.environ["XFE_AKM_JACOBI"]
+    jac.solve_wide(rhs[:8], tol=1e-10, max_iter=max_iter)
+    t0 = time.perf_counter()
+    _, its_j, (passes_j, _) = jac.solve_wide(rhs, tol=1e-10, max_iter=max_iter)
+    wide_jacobi_s = time.perf_counter() - t0
+    jac.close()
+    log(f"wide Jacobi: {wide_jacobi_s:.3f} s, {passes_j} FP32 passes")
+    # the reference's pattern: FP64, 32 lanes per launch, 8 batches (timed once)
+    t0 = time.perf_counter()
+    all_its = []
+    err = 0.0
+    for b0 in range(0, lanes, batch):
+        zb, itb = solver.solve_multi(rhs[b0:b0 + batch], tol=1e-10, max_iter=max_iter)
+        if (itb < 0).any():
+            raise SystemExit(f"lanes did not converge: {itb}")
+        all_its.append(itb)
+        err = max(err, float(np.max(np.abs(z_wide[b0:b0 + batch] - zb) / np.abs(zb).max(axis=1, keepdims=True))))
+    fp64_32_s = time.perf_counter() - t0
+    its64 = np.concatenate(all_its)
+    log(f"FP64 32-lane batches: {fp64_32_s:.3f} s, iterations {its64.min()}..{its64.max()}; "
+        f"wide vs FP64 max rel diff {err:.2e}")
     # the JLA draws with the reference's default 200 (streams, right-hand
     # sides, solves and per-row sums on the device): P_ii, then the B_ii
     # weights of the three components (two solves per draw)
@@ -175,26 +215,33 @@ def bench_akm(args):
         f"{bcov.sum():.4g}")
     solver.close()
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
-    its = np.concatenate(all_its[:lanes // batch])
-    log(f"{sec:.3f} s per {lanes} lanes; akm_firm_kernel: {launches} launches, "
+    log(f"{sec:.3f} s per {lanes} lanes; wide_pair_kernel: {launches} launches, "
         f"{pass_ms / max(launches, 1):.4f} ms avg, {achieved:.0f} GB/s algorithmic")
     cpu = None
     if not args.no_cpu:
-        cpu = akm_cpu_baseline(st, its, lanes)
+        cpu = akm_cpu_baseline(st, its64, lanes)
         log(f"cpu baseline {cpu['value']} s (extrapolated)")
     out = {
         "metric": AKM_METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "method": "FP64 Jacobi-PCG over the match segments, 32 lanes per launch, per-lane retire",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "method": ("FP32 PCG on the firm-pair form of S, 256 lanes per operator pass, aggregate blocks of "
+                   "strongly coupled firms solved exactly (Jacobi elsewhere), FP64 residual replacement "
+                   "(x accumulated in FP64), per-lane retire on the FP64 residual"),
         "config": {"workload": "kss_leverage_256rhs", "obs": int(n), "workers": N, "firms": J, "matches": int(M),
-                   "lanes": lanes, "lanes_per_launch": batch, "cg_tol": 1e-10},
-        "iterations": {"min": int(its.min()), "max": int(its.max()), "mean": round(float(its.mean()), 1)},
+                   "lanes": lanes, "lanes_per_launch": lanes, "cg_tol": 1e-10},
+        "iterations": {"min": int(its.min()), "max": int(its.max()), "mean": round(float(its.mean()), 1),
+                       "fp32_passes": passes, "fp64_replacements": updates},
+        "wide_host_pack_s": round(wide_host_s, 4),
+        "wide_jacobi_s": round(wide_jacobi_s, 4), "wide_jacobi_fp32_passes": passes_j,
+        "fp64_32lane_batches_s": round(fp64_32_s, 4),
+        "fp64_32lane_iterations": {"min": int(its64.min()), "max": int(its64.max())},
+        "max_rel_diff_vs_fp64_32lane": err,
         "jla_pii_200_draws_s": round(jla_s, 3), "jla_bii_200_draws_s": round(bii_s, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(os.path.join(ROOT, "profiles", "akm_firm_kss10m.md")),
-                     "kernel": "akm_firm_kernel (firm-major match pass of S, 32 lanes)",
+                     "traffic": measured_traffic(os.path.join(ROOT, "profiles", "wide_firm_kss10m.md")),
+                     "kernel": "wide_pair_kernel<float, 4> (firm-major pass over the firm pairs, 256 lanes)",
                      "avg_launch_ms": round(pass_ms / max(launches, 1), 4),
                      "bytes_per_launch": round(pass_bytes / max(launches, 1), 1)},
         "cpu_baseline": cpu,

@@ -449,6 +449,33 @@ int xfe_akm_jla_bii(xfe_akm_context* ctx, int draws, uint64_t seed, double tol, 
 /* akm_cuda_max_lanes (:65). */
 int xfe_akm_max_lanes(void);
 
+/* Wide batched solve: no counterpart in the reference, whose caller
+ * (src/akm_kss.cpp:1659-1707) loops over akm_cuda_solve_S_multi in batches
+ * of akm_cuda_max_lanes() = 32.  Up to xfe_akm_wide_max_lanes() (256) lanes
+ * per operator pass, FP32 Jacobi-PCG whose residual is replaced by the FP64
+ * residual b - S x (x accumulated in FP64) whenever r.r has dropped 1e4-fold;
+ * a lane stops when its FP64 residual meets |r|^2 <= tol^2 |rhs|^2.  Same
+ * pack layout, ground pin and per-lane results as xfe_akm_solve_S_multi
+ * (iters_out[l]: FP32 iterations, or -1 with that lane's z zero).  Only the
+ * movers' matches enter the passes (a single-match worker with c = Dw adds
+ * nothing to S beyond its diagonal).  Returns 0, or -1 on a device error. */
+int xfe_akm_solve_S_wide(xfe_akm_context* ctx, const double* rhs_pack, double* z_pack, int nb,
+                         double tol, int max_iter, int* iters_out);
+
+/* xfe_akm_solve_S_wide on device arrays (nb x n_firms lane-major, FP64;
+ * may alias): waits for the work already queued on the device, returns after
+ * its own work has ended.  No counterpart in the reference (its solves take
+ * host packs). */
+int xfe_akm_solve_S_wide_device(xfe_akm_context* ctx, const double* d_rhs_pack, double* d_z_pack, int nb,
+                                double tol, int max_iter, int* iters_out);
+
+/* 256: the lane limit of xfe_akm_solve_S_wide. */
+int xfe_akm_wide_max_lanes(void);
+
+/* FP32 operator passes and FP64 residual replacements of the context's last
+ * xfe_akm_solve_S_wide call.  Returns 0, or -1 on invalid arguments. */
+int xfe_akm_wide_last(xfe_akm_context* ctx, int* fp32_passes, int* fp64_replacements);
+
 /* akm_cuda_pack_buffer (:74): pinned host buffer of nb x n_firms doubles owned
  * by the context, or NULL (then the caller uses its own buffer). */
 double* xfe_akm_pack_buffer(xfe_akm_context* ctx, int nb);

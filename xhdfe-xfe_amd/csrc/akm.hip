@@ -22,9 +22,11 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <cstdlib>
 #include <vector>
 
 #include "absorb_kernels.h"
+#include "akm_wide.h"
 #include "common.h"
 #include "xfe_hip.h"
 
@@ -613,8 +615,10 @@ struct xfe_akm_context {
     double* h_pack = nullptr;   // pinned J x lanes staging
     int h_pack_lanes = 0;
     double* h_small = nullptr;  // pinned scalars
+    AkmWide wide;               // the 256-lane FP32 / FP64-replacement solver
 
     ~xfe_akm_context() {
+        wide.release();
         for (DevBuf<int>* d : {&wofs, &rorig, &forig, &wpiece, &rw, &r1, &fkey, &fw, &has, &bk[0], &bk[1]})
             d->release();
         for (DevBuf<double>* d : {&rsq, &rcv, &idw, &jtw, &pacc, &macc, &pii, &jsu, &bfe, &bpe, &bcov, &rwt, &fc,
@@ -964,6 +968,13 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
         c->scale.ensure(3 * kMaxCgCols);
         XFE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_small), 4 * kMaxCgCols * sizeof(double),
                                 hipHostMallocDefault));
+        std::vector<double> dw_new(N);
+        for (int k = 0; k < N; ++k) dw_new[k] = Dw[order[k]];
+        AkmWideInput wi;
+        wi.N = N; wi.J = J; wi.M = M; wi.ground = J - 1;
+        wi.wofs = wofs.data(); wi.rf = rf.data(); wi.rc = rcv.data(); wi.dw = dw_new.data();
+        wi.corr = corr.data(); wi.inv_diag = inv_diagS;
+        c->wide.build(wi, c->s, std::getenv("XFE_AKM_JACOBI") == nullptr);
         XFE_CHECK(hipStreamSynchronize(c->s));  // the host vectors above go out of scope
         return c;
     } catch (const std::exception& e) {
@@ -1116,6 +1127,56 @@ int xfe_akm_jla_bii(xfe_akm_context* c, int draws, uint64_t seed, double tol, in
         set_error(e.what());
         return -1;
     }
+}
+
+int xfe_akm_wide_max_lanes(void) { return kWideMaxLanes; }
+
+int xfe_akm_solve_S_wide(xfe_akm_context* c, const double* rhs_pack, double* z_pack, int nb, double tol,
+                         int max_iter, int* iters_out) {
+    clear_error();
+    if (!c || !rhs_pack || !z_pack || !iters_out || nb < 1 || nb > kWideMaxLanes || max_iter < 1 || !(tol >= 0.0)) {
+        set_error("xfe_akm_solve_S_wide: invalid arguments");
+        return -1;
+    }
+    try {
+        (void)hipGetLastError();
+        c->wide.solve(rhs_pack, z_pack, nb, tol, max_iter, iters_out, false);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int xfe_akm_solve_S_wide_device(xfe_akm_context* c, const double* d_rhs_pack, double* d_z_pack, int nb, double tol,
+                                int max_iter, int* iters_out) {
+    clear_error();
+    if (!c || !d_rhs_pack || !d_z_pack || !iters_out || nb < 1 || nb > kWideMaxLanes || max_iter < 1 ||
+        !(tol >= 0.0)) {
+        set_error("xfe_akm_solve_S_wide_device: invalid arguments");
+        return -1;
+    }
+    try {
+        (void)hipGetLastError();
+        // the caller may have filled d_rhs_pack on another stream
+        XFE_CHECK(hipDeviceSynchronize());
+        c->wide.solve(d_rhs_pack, d_z_pack, nb, tol, max_iter, iters_out, true);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int xfe_akm_wide_last(xfe_akm_context* c, int* fp32_passes, int* fp64_replacements) {
+    clear_error();
+    if (!c || !fp32_passes || !fp64_replacements) {
+        set_error("xfe_akm_wide_last: invalid arguments");
+        return -1;
+    }
+    *fp32_passes = c->wide.last_passes;
+    *fp64_replacements = c->wide.last_updates;
+    return 0;
 }
 
 int xfe_akm_solve_S(xfe_akm_context* c, const double* rhs, double* z, double tol, int max_iter) {

@@ -1,0 +1,88 @@
+// Wide batched solver of the AKM/KSS grounded firm Laplacian
+//     S = Df - B' Dw^{-1} B
+// for up to kWideMaxLanes right-hand sides per operator pass: FP32 Jacobi-PCG
+// whose residual is replaced by the FP64 residual b - S x (x accumulated in
+// FP64) each time the FP32 r.r has dropped 1e4-fold, so the
+// answer is certified in FP64 while the bulk of the passes move FP32 lanes.
+//
+// S is applied in its firm-pair form: per worker, diag(c) - c c' / Dw is the
+// Laplacian of the complete graph on the worker's firms with edge weights
+// c_a c_b / Dw (plus c_a (1 - C_w / Dw) on the diagonal when Dw differs from
+// the worker's count total C_w), so
+//     (S p)_f = sum_g W_fg (p_f - p_g) + corr_f p_f,
+// W_fg summed over the workers employed by both firms and corr = Df - deg plus
+// those diagonal terms.  Differences of neighbouring firms' values are formed
+// before they are weighted, so FP32 keeps the accuracy of the slow (smooth)
+// modes that a worker mean t_w = sum (c / Dw) p_f, rounded to FP32, loses.
+#ifndef XFE_AKM_WIDE_H
+#define XFE_AKM_WIDE_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+#include "lp_precond.h"
+
+namespace xfe {
+
+constexpr int kWideMaxLanes = 256;
+
+// Host-side records handed over by the context (workers renumbered by their
+// primary firm): worker-major matches (rf firm, rc count) with worker offsets
+// wofs (N + 1) and Dw per worker; per firm corr = Df - deg and inv_diag.
+struct AkmWideInput {
+    int N = 0, J = 0, M = 0, ground = 0;
+    const int* wofs = nullptr;
+    const int* rf = nullptr;
+    const double* rc = nullptr;
+    const double* dw = nullptr;
+    const double* corr = nullptr;
+    const double* inv_diag = nullptr;
+};
+
+struct AkmWide {
+    int J = 0, ground = 0;
+    int E = 0;  // directed firm-pair entries (firm-major)
+    int L = 0;  // lane stride of the allocated vectors (64, 128 or 256)
+    hipStream_t s = nullptr;
+    DevBuf<int> fkey, fnb, has;
+    DevBuf<float> w32, corr32, minv32;
+    DevBuf<double> w64, corr64;
+    DevBuf<int> bk[2];
+    DevBuf<float> bv32[2];
+    DevBuf<double> bv64[2];
+    DevBuf<float> x, r, z, p, q;
+    DevBuf<double> x64, b64, r64, q64, part, sc;
+    // aggregate blocks of strongly coupled firms solved exactly inside the
+    // preconditioner (Jacobi on the other firms)
+    lp::AggPrecond agg;
+    bool blocks = false;
+    double* h_sc = nullptr;  // pinned per-lane scalars
+    int last_updates = 0;    // FP64 residual replacements of the last solve
+    int last_passes = 0;     // FP32 operator passes of the last solve
+
+    // use_blocks: build the aggregate blocks (cap firms per block, rounds of
+    // handshake aggregation over the coupling strengths)
+    void build(const AkmWideInput& in, hipStream_t stream, bool use_blocks);
+    void release();
+    // rhs_pack / z_pack: lane-major arrays (nb x J), on the host or, with
+    // device_ptrs, on the device; iters_out[l]: FP32 iterations of lane l, -1
+    // when it did not converge (its solution zero)
+    void solve(const double* rhs_pack, double* z_pack, int nb, double tol, int max_iter, int* iters_out,
+               bool device_ptrs);
+    // the same on right-hand sides already on the device, level-major J x L
+    // FP64 (ground rows ignored); leaves the unscaled solutions level-major in
+    // x64 and the per-lane results in h_sc
+    void solve_device(int nb, double tol, int max_iter);
+
+  private:
+    void ensure(int nb);
+    template <typename T>
+    void apply(const T* pv, T* out, const T* w_, const T* corr_, DevBuf<T>* bv);
+};
+
+}  // namespace xfe
+
+#endif

@@ -73,6 +73,10 @@ struct AggPrecond {
     // Pairs, weights, the exact diagonal and the strengths.  Returns false
     // when there is no pair or too many to index with int.
     bool build_pairs(const SegView& sv, hipStream_t s);
+    // The same from host arrays: U pairs f < g with weights w, the exact
+    // diagonal d (G1); the strengths on the device.  Returns false when empty.
+    bool set_pairs(int G1, const std::vector<int>& f, const std::vector<int>& g, const std::vector<double>& w,
+                   const std::vector<double>& d, hipStream_t s);
     // Handshake aggregation over the U pairs: per round every aggregate picks
     // its best-scoring pair (ties: lower pair index) to an aggregate it can
     // join without exceeding cap firms, and mutual picks merge.

@@ -417,6 +417,25 @@ bool AggPrecond::build_pairs(const SegView& sv, hipStream_t s) {
     return true;
 }
 
+bool AggPrecond::set_pairs(int G1_, const std::vector<int>& f, const std::vector<int>& g,
+                           const std::vector<double>& w, const std::vector<double>& d, hipStream_t s) {
+    G1 = G1_;
+    U = static_cast<int>(f.size());
+    nb = 0;
+    nblocks = 0;
+    if (U <= 0 || G1 <= 0) return false;
+    if (!host_int) XFE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_int), 64 * sizeof(long long)));
+    pf.ensure(U); pg.ensure(U); W.ensure(U); st.ensure(U); dex.ensure(G1);
+    XFE_CHECK(hipMemcpyAsync(pf.p, f.data(), sizeof(int) * U, hipMemcpyHostToDevice, s));
+    XFE_CHECK(hipMemcpyAsync(pg.p, g.data(), sizeof(int) * U, hipMemcpyHostToDevice, s));
+    XFE_CHECK(hipMemcpyAsync(W.p, w.data(), sizeof(double) * U, hipMemcpyHostToDevice, s));
+    XFE_CHECK(hipMemcpyAsync(dex.p, d.data(), sizeof(double) * G1, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(strength_kernel, dim3(grid_of(U)), dim3(kB), 0, s, pf.p, pg.p, W.p, dex.p, U, st.p);
+    XFE_CHECK(hipGetLastError());
+    XFE_CHECK(hipStreamSynchronize(s));  // the host vectors belong to the caller
+    return true;
+}
+
 void AggPrecond::aggregate(const float* score, int cap_, int rounds_, hipStream_t s) {
     cap = std::max(2, std::min(cap_, kCapMax));
     rounds = std::max(0, rounds_);

@@ -70,7 +70,9 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_recovery_max_delta", "xfe_last_error", "xfe_profile_enable", "xfe_profile_reset",
            "xfe_profile_get", "xfe_release_workspace", "xfe_certify", "xfe_akm_available",
            "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
-           "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
+           "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_solve_S_wide", "xfe_akm_solve_S_wide_device",
+           "xfe_akm_wide_max_lanes",
+           "xfe_akm_wide_last", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
            "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
            "xfe_absorb_group_individual",
            "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort",
@@ -493,6 +495,36 @@ class AkmSolver:
         if rc != 0:
             raise XfeError(last_error())
         return z, its
+
+    def solve_wide(self, rhs_lanes, tol=1e-10, max_iter=1000):
+        """xfe_akm_solve_S_wide: up to 256 lanes per operator pass, FP32 PCG
+        under FP64 residual replacement.  rhs_lanes: (nb, J); returns
+        (z (nb, J), iterations, (fp32 passes, fp64 replacements))."""
+        rhs = np.ascontiguousarray(rhs_lanes, dtype=np.float64)
+        nb = rhs.shape[0]
+        z = np.empty_like(rhs)
+        its = np.empty(nb, dtype=np.int32)
+        L = lib()
+        rc = L.xfe_akm_solve_S_wide(_p(self._ctx), _ptr(rhs), _ptr(z), _i(nb), _d(tol), _i(max_iter), _ptr(its))
+        if rc != 0:
+            raise XfeError(last_error())
+        passes, upd = ctypes.c_int(0), ctypes.c_int(0)
+        L.xfe_akm_wide_last(_p(self._ctx), ctypes.byref(passes), ctypes.byref(upd))
+        return z, its, (passes.value, upd.value)
+
+    def solve_wide_device(self, d_rhs, d_z, nb, tol=1e-10, max_iter=1000):
+        """xfe_akm_solve_S_wide_device on device pointers (nb x J lane-major
+        FP64); returns (iterations, (fp32 passes, fp64 replacements))."""
+        its = np.empty(nb, dtype=np.int32)
+        L = lib()
+        L.xfe_akm_solve_S_wide_device.argtypes = [_p, _p, _p, _i, _d, _i, _p]
+        rc = L.xfe_akm_solve_S_wide_device(_p(self._ctx), _p(d_rhs), _p(d_z), int(nb), float(tol), int(max_iter),
+                                           _ptr(its))
+        if rc != 0:
+            raise XfeError(last_error())
+        passes, upd = ctypes.c_int(0), ctypes.c_int(0)
+        L.xfe_akm_wide_last(_p(self._ctx), ctypes.byref(passes), ctypes.byref(upd))
+        return its, (passes.value, upd.value)
 
     def jla_pii(self, draws=200, seed=20260705, tol=1e-10, max_iter=10000):
         """xfe_akm_jla_pii: JLA leverages of the matches (in the given order)
