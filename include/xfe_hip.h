@@ -476,6 +476,37 @@ int xfe_akm_wide_max_lanes(void);
  * xfe_akm_solve_S_wide call.  Returns 0, or -1 on invalid arguments. */
 int xfe_akm_wide_last(xfe_akm_context* ctx, int* fp32_passes, int* fp64_replacements);
 
+/* ---- Cluster bootstrap refits of the two-way FE regression on the wide
+ * solver's launch path.  No counterpart in the reference (its bootstrap is the
+ * Gelbach decomposition's, src/akm_kss.cpp:4965-7305, outside this path); the
+ * refits reuse the absorber's weighted semantics (rows weighted by their
+ * worker's multiplicity, src/fe_absorption.cpp:942-1037). */
+typedef struct xfe_boot_context xfe_boot_context;
+
+/* Panel rows (worker, firm: compact 0-based ids; the firm graph should be
+ * connected, the last firm grounds the firm effects) and the columns
+ * V = [y | X] (n x C column-major with leading dimension ldv, C <= 4).
+ * Returns NULL on failure. */
+xfe_boot_context* xfe_boot_create(int n, int n_workers, int n_firms, const int* worker, const int* firm,
+                                  const double* V, int ldv, int C);
+
+void xfe_boot_destroy(xfe_boot_context* ctx);
+
+/* R refits: mult (R x n_workers, resample-major) holds each worker's
+ * multiplicity in each resample (its rows enter that many times; 0 leaves it
+ * out; resample sizes may differ).  gram_out (R x C x C) receives the Gram
+ * X~' W X~ of the within-transformed columns of each refit (W: the
+ * multiplicities), from which beta = G[1:,1:]^-1 G[1:,0]; iters_out[r]: the
+ * FP32 iterations of refit r's firm-effect solves, -1 when one did not
+ * converge to tol.  64 refits share each operator pass.  Returns 0, 1 when
+ * some solve did not converge, -1 on failure. */
+int xfe_boot_refits(xfe_boot_context* ctx, int R, const int* mult, double tol, int max_iter, double* gram_out,
+                    int* iters_out);
+
+/* FP32 operator passes and FP64 residual replacements summed over the
+ * batches of the context's last xfe_boot_refits call. */
+int xfe_boot_last(xfe_boot_context* ctx, int* fp32_passes, int* fp64_replacements);
+
 /* akm_cuda_pack_buffer (:74): pinned host buffer of nb x n_firms doubles owned
  * by the context, or NULL (then the caller uses its own buffer). */
 double* xfe_akm_pack_buffer(xfe_akm_context* ctx, int nb);

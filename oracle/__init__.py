@@ -9,6 +9,12 @@ under ``xhdfe-xfe_amd/``) never imports it and never falls back to it.
 * ``_ref/libxhdfe_ref.so``: the reference's own CPU core compiled from
   /root/reference by ``oracle/Makefile`` (present only where the reference
   is; used to pin the restatement).
+* ``boot_gram_exact``: the Gram of the weighted two-way within transform of a
+  cluster-bootstrap refit (rows weighted by their worker's multiplicity, the
+  reference's weighted absorber semantics, src/fe_absorption.cpp:942-1037)
+  by a dense weighted projection; pinned against the explicitly replicated
+  panel (a worker drawn k times is k copies of its rows) and against the C
+  restatement's weighted absorber.
 * ``ols_cluster``: numpy restatement of the OLS + one-way cluster VCV
   (src/ols.cpp:1237-1556, scale G/(G-1) (N-1)/df_resid at :682-687).
 * ``akm_build`` / ``akm_solve_S``: numpy restatement of the AKM/KSS grounded
@@ -1260,3 +1266,26 @@ def ref_fit(y, X, fes, cluster=None, tol=1e-8, tolerance_mode=1, threads=1, grou
     return dict(coef=coef, se=se, sample_index=idx[:kept].copy(), groupvar=gv[:kept].copy(),
                 df_a=df_a.value, df_resid=df_r.value, fe_redundant=red, fe_levels=lev,
                 num_singletons=ns.value, iterations=it.value, rss=rss.value, nobs=kept)
+
+
+def boot_gram_exact(worker, firm, V, mult):
+    """Gram R' W R of the within transform R of V (n x C) on worker and firm
+    dummies under the row weights W = mult[worker] (rows of workers with
+    multiplicity 0 drop out), by a dense weighted least-squares projection."""
+    worker = np.asarray(worker)
+    firm = np.asarray(firm)
+    V = np.asarray(V, dtype=np.float64)
+    w = np.asarray(mult, dtype=np.float64)[worker]
+    keep = w > 0
+    wk, fk, Vk, sw = worker[keep], firm[keep], V[keep], np.sqrt(w[keep])
+    _, wi = np.unique(wk, return_inverse=True)
+    _, fi = np.unique(fk, return_inverse=True)
+    n = wk.size
+    D = np.zeros((n, wi.max() + 1 + fi.max() + 1))
+    D[np.arange(n), wi] = 1.0
+    D[np.arange(n), wi.max() + 1 + fi] = 1.0
+    A = D * sw[:, None]
+    B = Vk * sw[:, None]
+    coef, *_ = np.linalg.lstsq(A, B, rcond=None)
+    R = B - A @ coef
+    return R.T @ R

@@ -152,3 +152,47 @@ def test_wide_blocks_match_jacobi_with_fewer_iterations(monkeypatch):
     assert (np.abs(zb - zj).max(axis=1) <= 1e-8 * scale).all()
     _log(f"iterations: blocks {itb.max()}, Jacobi {itj.max()}")
     assert itb.max() < itj.max()
+
+
+def _boot_panel(seed, workers, years, firms):
+    from xfe_amd import panels
+    p = panels.config0(seed=seed, workers=workers, years=years, firms=firms)
+    worker = p["fes"][0]
+    firm = np.unique(p["fes"][1], return_inverse=True)[1].astype(np.int32)
+    V = np.column_stack([p["y"], p["X"]])
+    return worker, firm, V
+
+
+def test_boot_refits_match_exact_weighted_projection():
+    """Cluster-bootstrap refits (64 per operator pass, two batches, uneven
+    resample sizes) against the dense weighted two-way projection: the Gram of
+    each refit and its beta."""
+    _log("bootstrap refits")
+    worker, firm, V = _boot_panel(20261203 + 900, 1500, 6, 60)
+    Nw = int(worker.max()) + 1
+    rng = np.random.default_rng(4)
+    R = 70
+    mult = np.stack([rng.multinomial(int(rng.uniform(0.7, 1.3) * Nw), np.ones(Nw) / Nw) for _ in range(R)])
+    mult[5] = 1                                        # the original sample
+    boot = xfe_amd.BootRefits(worker, firm, V)
+    gram, its, (passes, upd), ok = boot.refits(mult, tol=1e-10, max_iter=5000)
+    assert ok and (its > 0).all() and upd >= 1
+    betas = xfe_amd.BootRefits.betas(gram)
+    for r in (0, 5, 31, 63, 64, 69):
+        Go = oracle.boot_gram_exact(worker, firm, V, mult[r])
+        assert np.abs(gram[r] - Go).max() <= 1e-8 * np.abs(Go).max(), (r, np.abs(gram[r] - Go).max())
+        bo = np.linalg.solve(Go[1:, 1:], Go[1:, 0])
+        assert np.abs(betas[r] - bo).max() <= 1e-7 * np.abs(bo).max(), (r, betas[r], bo)
+    # the original sample: the plain FP64 absorber and OLS give the same beta
+    yt, Xt, info = xfe_amd.absorb(V[:, 0], V[:, 1:], [worker.astype(np.int32), firm],
+                                  [Nw, int(firm.max()) + 1], tol=1e-12)
+    b_abs = np.linalg.lstsq(Xt, yt, rcond=None)[0]
+    assert np.abs(betas[5] - b_abs).max() <= 1e-7 * np.abs(b_abs).max()
+    # deterministic, and a multiplicity below zero is refused
+    gram2, its2, _, _ = boot.refits(mult, tol=1e-10, max_iter=5000)
+    assert np.array_equal(gram2, gram) and np.array_equal(its2, its)
+    bad = mult[:2].copy()
+    bad[0, 3] = -1
+    with pytest.raises(xfe_amd.XfeError):
+        boot.refits(bad)
+    boot.close()

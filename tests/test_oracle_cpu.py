@@ -1103,3 +1103,28 @@ def test_lp_trained_parameters_shipped():
     import xfe_amd
     th = xfe_amd.default_lp_params()
     assert th.size == xfe_amd.lp_param_count() and np.isfinite(th).all()
+
+
+def test_boot_gram_exact_pins():
+    """The bootstrap refit's Gram: a worker drawn k times equals k copies of
+    its rows (the replicated panel, unweighted), and the weighted projection
+    equals the C restatement's weighted absorber (pinned to the reference)."""
+    rng = np.random.default_rng(5)
+    Nw, T, J = 120, 5, 15
+    worker = np.repeat(np.arange(Nw), T)
+    firm = rng.integers(0, J, size=Nw * T)
+    V = rng.normal(size=(Nw * T, 3))
+    mult = rng.multinomial(int(0.9 * Nw), np.ones(Nw) / Nw)
+    G = oracle.boot_gram_exact(worker, firm, V, mult)
+    rep = np.repeat(np.arange(Nw * T), mult[worker])
+    Gr = oracle.boot_gram_exact(worker[rep], firm[rep], V[rep], np.ones(Nw, dtype=int))
+    assert np.abs(G - Gr).max() <= 1e-10 * np.abs(G).max()
+    # weighted absorber (C restatement) on the rows that stay, tight tolerance
+    keep = mult[worker] > 0
+    wk, fk = worker[keep], firm[keep]
+    gw = np.unique(wk, return_inverse=True)[1].astype(np.int32)
+    gf = np.unique(fk, return_inverse=True)[1].astype(np.int32)
+    wts = mult[wk].astype(np.float64)
+    Vt, info = oracle.absorb(V[keep], [gw, gf], [int(gw.max()) + 1, int(gf.max()) + 1], weights=wts, tol=1e-14)
+    Ga = (Vt * wts[:, None]).T @ Vt
+    assert np.abs(G - Ga).max() <= 1e-9 * np.abs(G).max()

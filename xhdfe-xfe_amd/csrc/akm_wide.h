@@ -47,7 +47,7 @@ struct AkmWide {
     int E = 0;  // directed firm-pair entries (firm-major)
     int L = 0;  // lane stride of the allocated vectors (64, 128 or 256)
     hipStream_t s = nullptr;
-    DevBuf<int> fkey, fnb, has;
+    DevBuf<int> fkey, fnb, fwk, has;
     DevBuf<float> w32, corr32, minv32;
     DevBuf<double> w64, corr64;
     DevBuf<int> bk[2];
@@ -59,6 +59,12 @@ struct AkmWide {
     // preconditioner (Jacobi on the other firms)
     lp::AggPrecond agg;
     bool blocks = false;
+    // per-lane entry weights (resampling): entry e of worker fwk[e] weighs
+    // base_e mult[fwk[e] * mstride + l / mcols] in lane l; per-lane Jacobi
+    // inverse diagonal minv_lane (J x L, FP32) in place of minv32
+    const int* mult = nullptr;
+    int mstride = 0, mcols = 1;
+    const float* minv_lane = nullptr;
     double* h_sc = nullptr;  // pinned per-lane scalars
     int last_updates = 0;    // FP64 residual replacements of the last solve
     int last_passes = 0;     // FP32 operator passes of the last solve
@@ -77,8 +83,19 @@ struct AkmWide {
     // x64 and the per-lane results in h_sc
     void solve_device(int nb, double tol, int max_iter);
 
-  private:
+    // entries given per worker, not merged (f, g, worker, base weight), no
+    // diagonal correction, no blocks: the operator sum_w mult_w L_w of a
+    // resample once mult is set
+    void build_worker_pairs(int J, int ground, const std::vector<int>& f, const std::vector<int>& g,
+                            const std::vector<int>& w, const std::vector<double>& base, hipStream_t stream);
+    // vectors for nb lanes (sets L)
     void ensure(int nb);
+    // x64 = bn x64 (the unscaled solutions; lanes that did not converge zero)
+    void unscale_x64();
+    // iterations of lane l in the last solve, -1 when it did not converge
+    int lane_iterations(int l) const;
+
+  private:
     template <typename T>
     void apply(const T* pv, T* out, const T* w_, const T* corr_, DevBuf<T>* bv);
 };

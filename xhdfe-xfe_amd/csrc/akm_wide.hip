@@ -105,11 +105,15 @@ __device__ __forceinline__ void close_piece(bool first, bool final, int key, int
 
 // Pair pass over fixed pieces of kWPiece firm-major entries (firm key,
 // neighbour, weight): sum of W (p[f] - p[g]) per firm.
-template <typename T, int V>
+// With MULT, entry e belongs to worker fwk[e] and lane l weighs it by
+// mult[fwk[e] * mstride + l / mcols] (the worker's multiplicity in the
+// resample that lane l belongs to).
+template <typename T, int V, bool MULT>
 __global__ void __launch_bounds__(kBlock)
 wide_pair_kernel(const int* __restrict__ fkey, const int* __restrict__ fw, const T* __restrict__ fc,
                  const T* __restrict__ p, int m, int L, T* __restrict__ out, int* __restrict__ kout,
-                 T* __restrict__ vout, int final) {
+                 T* __restrict__ vout, int final, const int* __restrict__ fwk, const int* __restrict__ mult,
+                 int mstride, int mcols) {
     const int lane = threadIdx.x & (kWave - 1);
     const size_t lo = static_cast<size_t>(lane) * V;
     const int pieces = (m + kWPiece - 1) / kWPiece;
@@ -119,6 +123,7 @@ wide_pair_kernel(const int* __restrict__ fkey, const int* __restrict__ fw, const
         const int kA = lane < n ? fkey[j0 + lane] : 0;
         const int wA = lane < n ? fw[j0 + lane] : 0;
         const T xA = lane < n ? fc[j0 + lane] : T(0);
+        const int mA = MULT && lane < n ? fwk[j0 + lane] : 0;
         int key = __shfl(kA, 0, kWave);
         LaneVec<T, V> acc;
         zero(acc);
@@ -144,8 +149,16 @@ wide_pair_kernel(const int* __restrict__ fkey, const int* __restrict__ fw, const
                         key = k;
                         zero(acc);
                     }
+                    if constexpr (MULT) {
+                        const int wk = __shfl(mA, (e0 + u) & (kWave - 1), kWave);
+                        const int* mrow = mult + static_cast<size_t>(wk) * mstride;
 #pragma unroll
-                    for (int k2 = 0; k2 < V; ++k2) acc.v[k2] += x * v[u].v[k2];
+                        for (int k2 = 0; k2 < V; ++k2)
+                            acc.v[k2] += x * static_cast<T>(mrow[(static_cast<int>(lo) + k2) / mcols]) * v[u].v[k2];
+                    } else {
+#pragma unroll
+                        for (int k2 = 0; k2 < V; ++k2) acc.v[k2] += x * v[u].v[k2];
+                    }
                 }
             }
         }
@@ -233,7 +246,7 @@ wide_dot_kernel(const T* __restrict__ a, const T* __restrict__ b, int J, int L, 
 __global__ void __launch_bounds__(kBlock)
 wide_step_kernel(float* __restrict__ x, float* __restrict__ r, float* __restrict__ z, const float* __restrict__ p,
                  const float* __restrict__ q, const float* __restrict__ minv, const double* __restrict__ sc, int J,
-                 int L, double* __restrict__ part) {
+                 int L, double* __restrict__ part, const float* __restrict__ minv_lane) {
     __shared__ double red[2][kBlock];
     const int l = threadIdx.x % L, r0 = threadIdx.x / L, R = kBlock / L;
     const float a = static_cast<float>(sc[kSAlpha * kWideMaxLanes + l]);
@@ -241,7 +254,7 @@ wide_step_kernel(float* __restrict__ x, float* __restrict__ r, float* __restrict
     for (long long j = static_cast<long long>(blockIdx.x) * R + r0; j < J; j += static_cast<long long>(gridDim.x) * R) {
         const size_t i = static_cast<size_t>(j) * L + l;
         const float ri = r[i] - a * q[i];
-        const float zi = minv[j] * ri;
+        const float zi = (minv_lane ? minv_lane[i] : minv[j]) * ri;
         x[i] += a * p[i];
         r[i] = ri;
         z[i] = zi;
@@ -448,7 +461,7 @@ wide_res_kernel(double* r64, const double* b, const double* q64, int J, int L, i
 __global__ void __launch_bounds__(kBlock)
 wide_reset_kernel(float* __restrict__ r, float* __restrict__ z, float* __restrict__ p, const double* __restrict__ r64,
                   const float* __restrict__ minv, const double* __restrict__ sc, int J, int L,
-                  double* __restrict__ part) {
+                  double* __restrict__ part, const float* __restrict__ minv_lane) {
     __shared__ double red[kBlock];
     const int l = threadIdx.x % L, r0 = threadIdx.x / L, R = kBlock / L;
     const bool act = sc[kSActive * kWideMaxLanes + l] != 0.0;
@@ -457,7 +470,7 @@ wide_reset_kernel(float* __restrict__ r, float* __restrict__ z, float* __restric
         const size_t i = static_cast<size_t>(j) * L + l;
         if (act) {
             const float ri = static_cast<float>(r64[i]);
-            const float zi = minv[j] * ri;
+            const float zi = (minv_lane ? minv_lane[i] : minv[j]) * ri;
             r[i] = ri;
             z[i] = zi;
             rz += static_cast<double>(ri) * zi;
@@ -498,6 +511,18 @@ wide_to_lanes_kernel(const double* x, const double* sc, int J, int nb, int L, in
         const int f = static_cast<int>(i % J);
         const bool ok = sc[kSOk * kWideMaxLanes + l] != 0.0;
         z[i] = (f == ground || !ok) ? 0.0 : x[static_cast<size_t>(f) * L + l] * sc[kSBn * kWideMaxLanes + l];
+    }
+}
+
+// x = bn x in place (level-major; lanes that did not converge and the ground zero)
+__global__ void __launch_bounds__(kBlock) wide_unscale_kernel(double* x, const double* sc, int J, int L, int ground) {
+    const long long total = static_cast<long long>(J) * L;
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * kBlock) {
+        const int f = static_cast<int>(i / L);
+        const int l = static_cast<int>(i % L);
+        const bool ok = sc[kSOk * kWideMaxLanes + l] != 0.0;
+        x[i] = (f == ground || !ok) ? 0.0 : x[i] * sc[kSBn * kWideMaxLanes + l];
     }
 }
 
@@ -598,9 +623,54 @@ void AkmWide::build(const AkmWideInput& in, hipStream_t stream, bool use_blocks)
     XFE_CHECK(hipStreamSynchronize(s));  // the host vectors go out of scope
 }
 
+void AkmWide::build_worker_pairs(int J_, int ground_, const std::vector<int>& f, const std::vector<int>& g,
+                                 const std::vector<int>& w, const std::vector<double>& base, hipStream_t stream) {
+    s = stream;
+    J = J_;
+    ground = ground_;
+    blocks = false;
+    // firm-major order (stable in the given order within a firm)
+    std::vector<int> order(f.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = static_cast<int>(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return f[a] < f[b]; });
+    std::vector<int> h_key(order.size()), h_nb(order.size()), h_wk(order.size()), h_has(J, 0);
+    std::vector<double> h_w64(order.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+        const int e = order[i];
+        h_key[i] = f[e];
+        h_nb[i] = g[e];
+        h_wk[i] = w[e];
+        h_w64[i] = base[e];
+        h_has[f[e]] = 1;
+    }
+    E = static_cast<int>(h_key.size());
+    std::vector<float> h_w32(h_w64.begin(), h_w64.end()), zero32(J, 0.0f);
+    std::vector<double> zero64(J, 0.0);
+    auto up = [&](auto& d, const auto& h) {
+        d.ensure(h.size());
+        if (!h.empty())
+            XFE_CHECK(hipMemcpyAsync(d.p, h.data(), sizeof(h[0]) * h.size(), hipMemcpyHostToDevice, s));
+    };
+    up(fkey, h_key); up(fnb, h_nb); up(fwk, h_wk); up(has, h_has); up(w32, h_w32); up(w64, h_w64);
+    up(corr32, zero32); up(corr64, zero64); up(minv32, zero32);
+    sc.ensure(kScTotal);
+    if (!h_sc) XFE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_sc), sizeof(double) * kScTotal, hipHostMallocDefault));
+    XFE_CHECK(hipStreamSynchronize(s));
+}
+
+void AkmWide::unscale_x64() {
+    hipLaunchKernelGGL(wide_unscale_kernel, dim3(ew_grid(static_cast<long long>(J) * L)), dim3(kBlock), 0, s, x64.p,
+                       sc.p, J, L, ground);
+    XFE_CHECK(hipGetLastError());
+}
+
+int AkmWide::lane_iterations(int l) const {
+    return h_sc[kSOk * kWideMaxLanes + l] != 0.0 ? static_cast<int>(h_sc[kSIters * kWideMaxLanes + l]) : -1;
+}
+
 void AkmWide::release() {
     agg.release();
-    for (DevBuf<int>* d : {&fkey, &fnb, &has, &bk[0], &bk[1]}) d->release();
+    for (DevBuf<int>* d : {&fkey, &fnb, &fwk, &has, &bk[0], &bk[1]}) d->release();
     for (DevBuf<float>* d : {&w32, &corr32, &minv32, &bv32[0], &bv32[1], &x, &r, &z, &p, &q}) d->release();
     for (DevBuf<double>* d : {&w64, &corr64, &bv64[0], &bv64[1], &x64, &b64, &r64, &q64, &part, &sc}) d->release();
     if (h_sc) (void)hipHostFree(h_sc);
@@ -630,8 +700,14 @@ static void apply_v(AkmWide& w, const T* pv, T* out, const T* w_, const T* corr_
         int m = w.E;
         int pieces = (m + kWPiece - 1) / kWPiece;
         const int slot = sizeof(T) == 4 ? profile_begin(s) : -1;
-        hipLaunchKernelGGL((wide_pair_kernel<T, V>), dim3(wave_grid(pieces)), dim3(kBlock), 0, s, w.fkey.p, w.fnb.p,
-                           w_, pv, m, L, out, w.bk[0].p, bv[0].p, pieces == 1 ? 1 : 0);
+        if (w.mult)
+            hipLaunchKernelGGL((wide_pair_kernel<T, V, true>), dim3(wave_grid(pieces)), dim3(kBlock), 0, s, w.fkey.p,
+                               w.fnb.p, w_, pv, m, L, out, w.bk[0].p, bv[0].p, pieces == 1 ? 1 : 0, w.fwk.p, w.mult,
+                               w.mstride, w.mcols);
+        else
+            hipLaunchKernelGGL((wide_pair_kernel<T, V, false>), dim3(wave_grid(pieces)), dim3(kBlock), 0, s, w.fkey.p,
+                               w.fnb.p, w_, pv, m, L, out, w.bk[0].p, bv[0].p, pieces == 1 ? 1 : 0, nullptr, nullptr,
+                               0, 1);
         XFE_CHECK(hipGetLastError());
         // the FP32 pair pass: entries (key, neighbour, weight), the neighbour's L
         // lanes per entry, the firm's lanes read and written once
@@ -694,7 +770,7 @@ void AkmWide::solve_device(int nb, double tol, int max_iter) {
         hipLaunchKernelGGL(wide_check_kernel, dim3(1), dim3(kScalarThreads), 0, s, part.p, dg, L, sc.p, tol2,
                            max_iter);
         hipLaunchKernelGGL(wide_reset_kernel, dim3(dg), dim3(kBlock), 0, s, r.p, z.p, p.p, r64.p, minv32.p, sc.p, J,
-                           L, part.p);
+                           L, part.p, minv_lane);
         block_precondition();
         hipLaunchKernelGGL(wide_rz_kernel, dim3(1), dim3(kScalarThreads), 0, s, part.p, dg, L, sc.p);
         XFE_CHECK(hipGetLastError());
@@ -711,7 +787,7 @@ void AkmWide::solve_device(int nb, double tol, int max_iter) {
         hipLaunchKernelGGL((wide_dot_kernel<float>), dim3(dg), dim3(kBlock), 0, s, p.p, q.p, J, L, part.p);
         hipLaunchKernelGGL(wide_alpha_kernel, dim3(1), dim3(kScalarThreads), 0, s, part.p, dg, L, sc.p);
         hipLaunchKernelGGL(wide_step_kernel, dim3(dg), dim3(kBlock), 0, s, x.p, r.p, z.p, p.p, q.p, minv32.p, sc.p,
-                           J, L, part.p);
+                           J, L, part.p, minv_lane);
         block_precondition();
         hipLaunchKernelGGL(wide_beta_kernel, dim3(1), dim3(kScalarThreads), 0, s, part.p, dg, L, sc.p, tol2,
                            max_iter);

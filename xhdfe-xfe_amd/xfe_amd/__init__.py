@@ -71,7 +71,7 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop
            "xfe_profile_get", "xfe_release_workspace", "xfe_certify", "xfe_akm_available",
            "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
            "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_solve_S_wide", "xfe_akm_solve_S_wide_device",
-           "xfe_akm_wide_max_lanes",
+           "xfe_akm_wide_max_lanes", "xfe_boot_create", "xfe_boot_destroy", "xfe_boot_refits", "xfe_boot_last",
            "xfe_akm_wide_last", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
            "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
            "xfe_absorb_group_individual",
@@ -454,6 +454,64 @@ def schwarz_demean(Y, X, fe, nlev, tol=1e-8, max_iter=10000, threads=1, collapse
     ok = L.xfe_schwarz_demean(_ptr(Y), Xp, n, k, ptrs, _ptr(lev), len(ids), int(threads), float(tol),
                               int(max_iter), ctypes.byref(it), int(collapse))
     return bool(ok), it.value
+
+
+class BootRefits:
+    """xfe_boot_create / _refits / _destroy: cluster (worker) bootstrap refits
+    of the two-way FE regression of V[:, 0] on V[:, 1:], 64 refits per
+    operator pass on the wide solver."""
+
+    def __init__(self, worker, firm, V, n_workers=None, n_firms=None):
+        w = _i32(worker)
+        f = _i32(firm)
+        V = np.asarray(V, dtype=np.float64)
+        if V.ndim != 2 or V.shape[0] != w.size or f.size != w.size or not 1 <= V.shape[1] <= 4:
+            raise XfeError("BootRefits: V must be n x C with C <= 4 and one row per id")
+        self.N = int(n_workers if n_workers is not None else w.max() + 1)
+        self.J = int(n_firms if n_firms is not None else f.max() + 1)
+        self.C = V.shape[1]
+        self._V = np.asfortranarray(V)
+        self._w, self._f = w, f
+        L = lib()
+        L.xfe_boot_create.restype = _p
+        L.xfe_boot_create.argtypes = [_i, _i, _i, _p, _p, _p, _i, _i]
+        self._ctx = L.xfe_boot_create(w.size, self.N, self.J, _ptr(w), _ptr(f), _ptr(self._V), w.size, self.C)
+        if not self._ctx:
+            raise XfeError(last_error())
+
+    def refits(self, mult, tol=1e-10, max_iter=5000):
+        """mult: (R, n_workers) multiplicities; returns (gram (R, C, C),
+        iterations (R,), (fp32 passes, fp64 replacements), converged)."""
+        m = np.ascontiguousarray(mult, dtype=np.int32)
+        if m.ndim != 2 or m.shape[1] != self.N:
+            raise XfeError("refits: mult must be R x n_workers")
+        R = m.shape[0]
+        gram = np.empty((R, self.C, self.C))
+        its = np.empty(R, dtype=np.int32)
+        L = lib()
+        L.xfe_boot_refits.argtypes = [_p, _i, _p, _d, _i, _p, _p]
+        rc = L.xfe_boot_refits(_p(self._ctx), R, _ptr(m), float(tol), int(max_iter), _ptr(gram), _ptr(its))
+        if rc < 0:
+            raise XfeError(last_error())
+        passes, upd = ctypes.c_int(0), ctypes.c_int(0)
+        L.xfe_boot_last(_p(self._ctx), ctypes.byref(passes), ctypes.byref(upd))
+        return gram, its, (passes.value, upd.value), rc == 0
+
+    @staticmethod
+    def betas(gram):
+        """beta of each refit from its Gram (y first): G[1:,1:]^-1 G[1:,0]."""
+        return np.stack([np.linalg.solve(g[1:, 1:], g[1:, 0]) for g in gram])
+
+    def close(self):
+        if self._ctx:
+            lib().xfe_boot_destroy(_p(self._ctx))
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class AkmSolver:
