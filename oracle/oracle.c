@@ -106,6 +106,8 @@ void or_weight_sums(int n, const int* gid, int G, const double* w, double* out) 
 
 /* ------------------------------------------------------------- projections */
 
+#define OR_OBSPART_MAX_GROUPS 2048 /* kObsPartMaxGroups, src/fe_absorption.cpp:6377 */
+
 typedef struct {
     int n, S, D;
     const int* gid[OR_MAX_DIMS];
@@ -115,6 +117,12 @@ typedef struct {
     double* means[OR_MAX_DIMS]; /* G x S, level-major */
     int* ptr[OR_MAX_DIMS];      /* CSR: rows of each level in ascending row order */
     int* idx[OR_MAX_DIMS];
+    /* per-dimension strategy of the packed path (src/fe_absorption.cpp:6373-6400):
+     * 0 contiguous (the rows of each level are consecutive), 1 CSR gather,
+     * 2 observation-partitioned (few levels, not contiguous): per-chunk sums */
+    int mode[OR_MAX_DIMS];
+    double* tls[OR_MAX_DIMS];   /* chunks x G x S partial sums (mode 2) */
+    int chunks;
     const double* w;
     int threads;
 } or_ws;
@@ -124,6 +132,7 @@ static void ws_init(or_ws* ws, int n, int S, int D, const int* gids, const int* 
     memset(ws, 0, sizeof(*ws));
     ws->n = n; ws->S = S; ws->D = D; ws->w = w;
     ws->threads = threads > 0 ? threads : 1;
+    ws->chunks = 4 * ws->threads;
     for (int d = 0; d < D; ++d) {
         const int* g = gids + (size_t)d * n;
         const int Gd = G[d];
@@ -145,44 +154,88 @@ static void ws_init(or_ws* ws, int n, int S, int D, const int* gids, const int* 
         free(fill);
         ws->ptr[d] = ptr;
         ws->idx[d] = idx;
+        int contiguous = 1;
+        for (int p = 0; p < n && contiguous; ++p) contiguous = idx[p] == p;
+        ws->mode[d] = contiguous ? 0 : (Gd <= OR_OBSPART_MAX_GROUPS ? 2 : 1);
+        if (ws->mode[d] == 2)
+            ws->tls[d] = (double*)malloc(sizeof(double) * (size_t)ws->chunks * (size_t)(Gd > 0 ? Gd : 1) * (size_t)S);
     }
 }
 
 static void ws_free(or_ws* ws) {
     for (int d = 0; d < ws->D; ++d) {
         free(ws->wsum[d]); free(ws->wsinv[d]); free(ws->means[d]);
-        free(ws->ptr[d]); free(ws->idx[d]);
+        free(ws->ptr[d]); free(ws->idx[d]); free(ws->tls[d]);
     }
 }
 
-/* One projection on the packed row-interleaved buffer cur[i*S + s]:
- * packed_demean_grouped (src/fe_absorption.cpp:6470-6500 dispatch; the per-level
- * reduce-then-subtract of demean_inplace_contiguous :2380-2470).  Optionally
- * returns the post-projection sum of squares over all lanes. */
-static void demean_dim(or_ws* ws, double* cur, int d, double* out_sumsq) {
-    const int S = ws->S;
+/* One projection on the packed row-interleaved buffer cur[i*S + s]: per level
+ * the (weighted) sum of its rows in row order, the mean, then the subtraction
+ * (packed_demean_grouped, src/fe_absorption.cpp:5719-5800; the
+ * observation-partitioned variant for few non-contiguous levels merges
+ * per-chunk sums in chunk order).  S is a compile-time constant in the
+ * specializations below.  Optionally returns the post-projection sum of
+ * squares over all lanes. */
+static inline __attribute__((always_inline)) void demean_dim_s(or_ws* ws, double* cur, int d,
+                                                              double* out_sumsq, const int S) {
     const int Gd = ws->G[d];
+    const int n = ws->n;
     const int* ptr = ws->ptr[d];
     const int* idx = ws->idx[d];
+    const int* gid = ws->gid[d];
     const double* w = ws->w;
     const double* inv = ws->wsinv[d];
     double* means = ws->means[d];
     double sumsq = 0.0;
+    if (ws->mode[d] == 2) {
+        const int chunks = ws->chunks;
+        double* tls = ws->tls[d];
+#pragma omp parallel for schedule(static) num_threads(ws->threads)
+        for (int c = 0; c < chunks; ++c) {
+            double* loc = tls + (size_t)c * Gd * S;
+            memset(loc, 0, sizeof(double) * (size_t)Gd * S);
+            const int i0 = (int)(((long long)n * c) / chunks), i1 = (int)(((long long)n * (c + 1)) / chunks);
+            for (int i = i0; i < i1; ++i) {
+                double* a = loc + (size_t)gid[i] * S;
+                const double* row = cur + (size_t)i * S;
+                if (w) { const double wi = w[i]; for (int s = 0; s < S; ++s) a[s] += wi * row[s]; }
+                else for (int s = 0; s < S; ++s) a[s] += row[s];
+            }
+        }
+        for (int g = 0; g < Gd; ++g)
+            for (int s = 0; s < S; ++s) {
+                double t = 0.0;
+                for (int c = 0; c < chunks; ++c) t += tls[((size_t)c * Gd + g) * S + s];
+                means[(size_t)g * S + s] = t * inv[g];
+            }
+#pragma omp parallel for schedule(static) num_threads(ws->threads) reduction(+ : sumsq)
+        for (int i = 0; i < n; ++i) {
+            double* row = cur + (size_t)i * S;
+            const double* m = means + (size_t)gid[i] * S;
+            for (int s = 0; s < S; ++s) {
+                row[s] -= m[s];
+                sumsq += row[s] * row[s];
+            }
+        }
+        if (out_sumsq) *out_sumsq = sumsq;
+        return;
+    }
+    const int contiguous = ws->mode[d] == 0;
 #pragma omp parallel for schedule(static) num_threads(ws->threads) reduction(+ : sumsq)
     for (int g = 0; g < Gd; ++g) {
         double acc[64];
         for (int s = 0; s < S; ++s) acc[s] = 0.0;
         for (int p = ptr[g]; p < ptr[g + 1]; ++p) {
-            const int i = idx[p];
-            const double wi = w ? w[i] : 1.0;
+            const int i = contiguous ? p : idx[p];
+            if (!contiguous && p + 16 < n) __builtin_prefetch(cur + (size_t)idx[p + 16] * S, 0, 1);
             const double* row = cur + (size_t)i * S;
-            if (w) for (int s = 0; s < S; ++s) acc[s] += wi * row[s];
+            if (w) { const double wi = w[i]; for (int s = 0; s < S; ++s) acc[s] += wi * row[s]; }
             else for (int s = 0; s < S; ++s) acc[s] += row[s];
         }
         double* m = means + (size_t)g * S;
         for (int s = 0; s < S; ++s) m[s] = acc[s] * inv[g];
         for (int p = ptr[g]; p < ptr[g + 1]; ++p) {
-            double* row = cur + (size_t)idx[p] * S;
+            double* row = cur + (size_t)(contiguous ? p : idx[p]) * S;
             for (int s = 0; s < S; ++s) {
                 row[s] -= m[s];
                 sumsq += row[s] * row[s];
@@ -190,6 +243,20 @@ static void demean_dim(or_ws* ws, double* cur, int d, double* out_sumsq) {
         }
     }
     if (out_sumsq) *out_sumsq = sumsq;
+}
+
+static void demean_dim(or_ws* ws, double* cur, int d, double* out_sumsq) {
+    switch (ws->S) {
+        case 1: demean_dim_s(ws, cur, d, out_sumsq, 1); break;
+        case 2: demean_dim_s(ws, cur, d, out_sumsq, 2); break;
+        case 3: demean_dim_s(ws, cur, d, out_sumsq, 3); break;
+        case 4: demean_dim_s(ws, cur, d, out_sumsq, 4); break;
+        case 5: demean_dim_s(ws, cur, d, out_sumsq, 5); break;
+        case 6: demean_dim_s(ws, cur, d, out_sumsq, 6); break;
+        case 7: demean_dim_s(ws, cur, d, out_sumsq, 7); break;
+        case 8: demean_dim_s(ws, cur, d, out_sumsq, 8); break;
+        default: demean_dim_s(ws, cur, d, out_sumsq, ws->S); break;
+    }
 }
 
 /* packed_sweep, src/fe_absorption.cpp:6423-6443: forward over `order`, then
@@ -220,17 +287,50 @@ static double sumsq_all(const double* v, size_t m) {
 
 typedef struct { double vprod, ssq, d1sq; } it_stats;
 
-/* irons_tuck_accumulate, src/fe_absorption.cpp:5242-5275. */
-static it_stats it_accumulate(const double* x, const double* gx, const double* ggx, size_t m) {
-    it_stats st = {0.0, 0.0, 0.0};
-    for (size_t i = 0; i < m; ++i) {
-        const double dg = ggx[i] - gx[i];
-        const double d2 = dg - gx[i] + x[i];
-        st.vprod += dg * d2;
-        st.ssq += d2 * d2;
-        st.d1sq += dg * dg;
+/* irons_tuck_accumulate, src/fe_absorption.cpp:5242-5275, over fixed chunks
+ * whose partial sums are added in chunk order (the reference's deterministic
+ * chunking), so the result does not depend on the thread count. */
+#define OR_VEC_CHUNKS 64
+static it_stats it_accumulate_t(const double* x, const double* gx, const double* ggx, size_t m, int threads) {
+    double pv[OR_VEC_CHUNKS], ps[OR_VEC_CHUNKS], pd[OR_VEC_CHUNKS];
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int c = 0; c < OR_VEC_CHUNKS; ++c) {
+        const size_t i0 = m * (size_t)c / OR_VEC_CHUNKS, i1 = m * (size_t)(c + 1) / OR_VEC_CHUNKS;
+        double v = 0.0, q = 0.0, d = 0.0;
+        for (size_t i = i0; i < i1; ++i) {
+            const double dg = ggx[i] - gx[i];
+            const double d2 = dg - gx[i] + x[i];
+            v += dg * d2;
+            q += d2 * d2;
+            d += dg * dg;
+        }
+        pv[c] = v; ps[c] = q; pd[c] = d;
     }
+    it_stats st = {0.0, 0.0, 0.0};
+    for (int c = 0; c < OR_VEC_CHUNKS; ++c) { st.vprod += pv[c]; st.ssq += ps[c]; st.d1sq += pd[c]; }
     return st;
+}
+
+static void par_copy(double* dst, const double* src, size_t m, int threads) {
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int c = 0; c < OR_VEC_CHUNKS; ++c) {
+        const size_t i0 = m * (size_t)c / OR_VEC_CHUNKS, i1 = m * (size_t)(c + 1) / OR_VEC_CHUNKS;
+        memcpy(dst + i0, src + i0, sizeof(double) * (i1 - i0));
+    }
+}
+
+static double sumsq_t(const double* v, size_t m, int threads) {
+    double part[OR_VEC_CHUNKS];
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int c = 0; c < OR_VEC_CHUNKS; ++c) {
+        const size_t i0 = m * (size_t)c / OR_VEC_CHUNKS, i1 = m * (size_t)(c + 1) / OR_VEC_CHUNKS;
+        double t = 0.0;
+        for (size_t i = i0; i < i1; ++i) t += v[i] * v[i];
+        part[c] = t;
+    }
+    double s = 0.0;
+    for (int c = 0; c < OR_VEC_CHUNKS; ++c) s += part[c];
+    return s;
 }
 
 typedef struct {
@@ -253,29 +353,31 @@ static int accel_phase(accel_ctx* c, const int* order, int len, int max_iter, in
     const double kDiverge = 4.0, kBail = 1.5;
     const int cg_iter_cap = 1000, grand_every = 4;
     int last_check = -1;
-    double prev_norm = sqrt(sumsq_all(c->cur, m));
+    const int th = c->ws->threads;
+    double prev_norm = sqrt(sumsq_t(c->cur, m, th));
     int grand = 0;
     double best = prev_norm;
     int suspended = 0;
-    memcpy(c->gx, c->cur, sizeof(double) * m);
+    par_copy(c->gx, c->cur, m, th);
     sweep(c->ws, c->gx, order, len, c->symmetric, NULL);
     for (int iter = 0; iter < max_iter; ++iter) {
-        memcpy(c->ggx, c->gx, sizeof(double) * m);
+        par_copy(c->ggx, c->gx, m, th);
         sweep(c->ws, c->ggx, order, len, c->symmetric, NULL);
-        const it_stats st = it_accumulate(c->cur, c->gx, c->ggx, m);
+        const it_stats st = it_accumulate_t(c->cur, c->gx, c->ggx, m, th);
         if (c->honest && sqrt(st.d1sq) <= c->honest_tol * fmax(1.0, prev_norm)) {
-            memcpy(c->cur, c->ggx, sizeof(double) * m);
+            par_copy(c->cur, c->ggx, m, th);
             *iter_used = iter + 1;
             return 1;
         }
         if (st.ssq == 0.0) {
-            memcpy(c->cur, c->gx, sizeof(double) * m);
+            par_copy(c->cur, c->gx, m, th);
             *iter_used = iter + 1;
             return 1;
         }
         double coef = st.vprod / st.ssq;
         if (suspended) coef = 0.0;
         /* irons_tuck_update_broadcast (:5366-5394) */
+#pragma omp parallel for schedule(static) num_threads(th)
         for (size_t i = 0; i < m; ++i) {
             const double nx = c->ggx[i] - coef * (c->ggx[i] - c->gx[i]);
             c->cur[i] = nx;
@@ -296,7 +398,7 @@ static int accel_phase(accel_ctx* c, const int* order, int len, int max_iter, in
             } else {
                 if (curr > kBail * best) {
                     c->diverged = 1;
-                    memcpy(c->cur, c->gx, sizeof(double) * m);
+                    par_copy(c->cur, c->gx, m, th);
                     *iter_used = iter + 1;
                     return 0;
                 }
@@ -306,40 +408,41 @@ static int accel_phase(accel_ctx* c, const int* order, int len, int max_iter, in
                 }
             }
             if (!c->honest && step > 0 && rel_change / (double)step < c->conv_tol) {
-                memcpy(c->cur, c->gx, sizeof(double) * m);
+                par_copy(c->cur, c->gx, m, th);
                 *iter_used = iter + 1;
                 return 1;
             }
         }
         if (c->honest && cg_iter_cap > 0 && !c->diverged && iter + 1 >= cg_iter_cap) {
             c->diverged = 1;
-            memcpy(c->cur, c->gx, sizeof(double) * m);
+            par_copy(c->cur, c->gx, m, th);
             *iter_used = iter + 1;
             return 0;
         }
         if (!suspended && grand_every > 0 && (iter + 1) % grand_every == 0) {
             ++grand;
             if (grand == 1) {
-                memcpy(c->acc_a, c->gx, sizeof(double) * m);
+                par_copy(c->acc_a, c->gx, m, th);
             } else if (grand == 2) {
-                memcpy(c->acc_b, c->gx, sizeof(double) * m);
+                par_copy(c->acc_b, c->gx, m, th);
             } else {
-                memcpy(c->acc_c, c->gx, sizeof(double) * m);
-                const it_stats as = it_accumulate(c->acc_a, c->acc_b, c->acc_c, m);
+                par_copy(c->acc_c, c->gx, m, th);
+                const it_stats as = it_accumulate_t(c->acc_a, c->acc_b, c->acc_c, m, th);
                 if (as.ssq != 0.0) {
                     const double ac = as.vprod / as.ssq;
                     /* irons_tuck_update (:5277-5298) */
+#pragma omp parallel for schedule(static) num_threads(th)
                     for (size_t i = 0; i < m; ++i)
                         c->acc_a[i] = c->acc_c[i] - ac * (c->acc_c[i] - c->acc_b[i]);
                 }
-                memcpy(c->cur, c->acc_a, sizeof(double) * m);
-                memcpy(c->gx, c->cur, sizeof(double) * m);
+                par_copy(c->cur, c->acc_a, m, th);
+                par_copy(c->gx, c->cur, m, th);
                 sweep(c->ws, c->gx, order, len, c->symmetric, NULL);
                 grand = 0;
             }
         }
     }
-    memcpy(c->cur, c->gx, sizeof(double) * m);
+    par_copy(c->cur, c->gx, m, th);
     *iter_used = max_iter;
     return 0;
 }
