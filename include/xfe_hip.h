@@ -223,6 +223,21 @@ int xfe_absorb_device(double* d_V, int n, int C, int ld,
                       const double* d_weights, const int* sweep_order,
                       const xfe_options* opt, xfe_absorb_result* res);
 
+/* xfe_absorb_device with heterogeneous slope dimensions (the slope terms of
+ * absorb_fixed_effects_v6_mixed, src/fe_absorption.cpp:8162, whose device call
+ * at :8336 stages host memory; no device-resident counterpart in the
+ * reference).  slope_mode[d]: 0 standard, 1 slope with intercept, 2 slope
+ * only; d_slope_values[d] (device, n) for slope dimensions.  With
+ * opt->precision 1 an unweighted problem with at most 32 columns and a
+ * standard dimension runs the mixed-precision Schur PCG (res->precision_used
+ * 1), otherwise the FP64 slope path (precision_fallback 1 when precision 1
+ * was asked).  The certificate (opt->certify) includes the slope moments.
+ * Ordering as xfe_absorb_device.  Returns 1 on success, 0 on failure. */
+int xfe_absorb_device_slopes(double* d_V, int n, int C, int ld, const int* d_group_ids, const int* num_groups,
+                             int num_fes, const double* const* d_slope_values, const int* slope_mode,
+                             const double* d_weights, const int* sweep_order, const xfe_options* opt,
+                             xfe_absorb_result* res);
+
 /* xhdfe::schwarz_demean_raw (include/schwarz_demean.hpp:30, impl
  * src/schwarz_demean.cpp:419-438; called by absorb_fixed_effects,
  * src/fe_absorption.cpp:5185-5215): unweighted within transform of Y (n) and

@@ -31,6 +31,7 @@
 #include "absorb_kernels.h"
 #include "common.h"
 #include "lp_precond.h"
+#include "slope_mixed.h"
 #include "xfe_hip.h"
 
 namespace xfe {
@@ -110,6 +111,7 @@ struct Workspace {
     DevBuf<double> stage_w;
     DevBuf<double> stage_z[XFE_MAX_DIMS];  // host-API staging of slope variables
     DevBuf<double> slope_z[XFE_MAX_DIMS];  // slope variables in the slope path's row order
+    SlopeMixed* slope_mixed = nullptr;     // the mixed-precision slope path's buffers
     // level-aligned tiles of the sorted dimension and spell structures
     DevBuf<int> tile_start, level_count, tmp_flags, tmp_inc, tmp_iota, tmp_key;
     DevBuf<int> sp_key[XFE_MAX_DIMS], sp_order[XFE_MAX_DIMS], sp_base[XFE_MAX_DIMS];
@@ -149,6 +151,8 @@ struct Workspace {
 
     // Frees every device resource of this thread's workspace.
     void release() {
+        slope_mixed_delete(slope_mixed);
+        slope_mixed = nullptr;
         for (auto& b : vec) b.release();
         for (int d = 0; d < XFE_MAX_DIMS; ++d) {
             gid[d].release(); sums[d].release(); means[d].release();
@@ -2495,6 +2499,94 @@ static void gi_colour(const xfe_group_individual& gi, std::vector<int>& members,
 // (src/fe_absorption_cuda.cu:2324, :4771); Lsmr and Mlsmr are CPU-only there
 // (src/fe_absorption.cpp:9127).  A refused method fails closed (return 0), so
 // the caller keeps its CPU path.
+// Mixed-precision absorption with heterogeneous slopes (precision 1): the
+// Schur PCG of slope_mixed.hip, then the normal-equation certificate with the
+// slope moments (certify_absorption_result with slopes,
+// src/fe_absorption.cpp:14942-15100: per dimension the level sums of w v~ and,
+// for a slope, of w z v~; ||D'W||_F^2 = sum over dimensions and moments of
+// sum_i (w_i m_i)^2).  V: n x C (ld), device, transformed in place.
+static void absorb_slopes_mixed(double* V, int n, int C, int ld, const std::vector<const int*>& gid,
+                                const std::vector<int>& G, const std::vector<const double*>& z,
+                                const std::vector<int>& mode, const xfe_options& opt, xfe_absorb_result& res) {
+    Workspace& ws = tls_ws;
+    hipStream_t s = ws.stream;
+    const int D = static_cast<int>(G.size());
+    const int grid = grid_for(n);
+    // both buffers of the column sums below (this entry allocates them: no
+    // solver set-up has run on this path)
+    ws.partials.ensure(static_cast<size_t>(grid) * std::max(C, 1) + 64);
+    ws.scalars.ensure(std::max(C, 64));
+    if (!ws.host_scalars) throw Error("absorb_slopes_mixed: workspace not initialised");
+    auto col_sumsq = [&](const double* a, int cols, int lda, double* out) {
+        XFE_CHECK(launch_col_dot(a, a, n, cols, lda, ws.partials.p, grid, s));
+        XFE_CHECK(launch_reduce_partials(ws.partials.p, grid, cols, 0, ws.scalars.p, s));
+        XFE_CHECK(hipMemcpyAsync(ws.host_scalars, ws.scalars.p, sizeof(double) * cols, hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        for (int c = 0; c < cols; ++c) out[c] = ws.host_scalars[c];
+    };
+    std::vector<double> orig(C, 0.0);
+    if (opt.certify) col_sumsq(V, C, ld, orig.data());
+    if (!ws.slope_mixed) ws.slope_mixed = slope_mixed_new();
+    const double target = opt.refine_tol > 0.0 ? opt.refine_tol
+                                               : std::min(3e-13, (opt.tol > 0.0 ? opt.tol : 1e-8) * 3e-5);
+    SlopeMixedResult out;
+    const auto t0 = std::chrono::steady_clock::now();
+    slope_mixed_run(*ws.slope_mixed, V, n, C, ld, gid, G, z, mode, target,
+                    std::max(10, std::min(opt.max_iter, 20000)), s, out);
+    res.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    res.iterations = out.iterations;
+    res.refinements = out.replacements;
+    res.sweeps = out.passes;
+    res.converged = out.converged ? 1 : 0;
+    res.used_cg = 1;
+    res.used_accel = 0;
+    res.precision_used = 1;
+    res.refine_target = target;
+    for (int d = 0; d < D; ++d) {
+        res.fe_levels[d] = G[d];
+        res.sweep_order_used[d] = d;
+    }
+    if (!opt.certify) return;
+    // the dropped intercepts' level sums, and ||D||_F^2
+    std::vector<double> resid = out.resid_sq;
+    double op = static_cast<double>(n);  // the eliminated dimension
+    for (int d = 0; d < D; ++d) {
+        if (d == out.primary) continue;
+        if (mode[d] == 0 || mode[d] == 1) op += static_cast<double>(n);
+        if (mode[d] == 1 || mode[d] == 2) {
+            double zz = 0.0;
+            col_sumsq(z[d], 1, n, &zz);
+            op += zz;
+        }
+    }
+    {
+        // every slope intercept that slope_mixed_run left out belongs to a
+        // dimension with mode 1 whose intercept a finer dimension spans; the
+        // run reports their count, and their level sums are added here
+        int left = out.intercepts_dropped;
+        for (int d = 0; d < D && left > 0; ++d) {
+            if (mode[d] != 1 || d == out.primary) continue;
+            if (!slope_mixed_intercept_dropped(*ws.slope_mixed, d)) continue;
+            slope_mixed_level_sumsq(*ws.slope_mixed, V, n, C, ld, gid[d], G[d], s, resid);
+            --left;
+        }
+    }
+    double amax = 0.0, rmax = 0.0;
+    for (int c = 0; c < C; ++c) {
+        const double a = std::sqrt(std::max(0.0, resid[c]));
+        const double scale = std::sqrt(std::max(0.0, op)) * std::sqrt(std::max(0.0, orig[c]));
+        const double rel = scale > 0.0 ? a / scale : (a > 0.0 ? DBL_MAX : 0.0);
+        amax = std::max(amax, a);
+        rmax = std::max(rmax, rel);
+    }
+    const double tol_eff = (opt.tol > 0.0 && opt.tolerance_mode == 1) ? std::min(opt.tol, 1e-9) : opt.tol;
+    const double limit = std::max(8.0 * std::max(0.0, tol_eff), 64.0 * DBL_EPSILON);
+    res.abs_residual = amax;
+    res.abs_residual_rel = rmax;
+    res.precision_certified = (res.converged && rmax <= limit) ? 1 : 0;
+    res.certificate_computed = 1;
+}
+
 static bool method_ok(int method, const char* fn) {
     if (method == 0 || method == 1 || method == 2 || method == 4 || method == 7) return true;
     const char* what = method == 3 ? "Jacobi is not a device method"
@@ -2564,6 +2656,64 @@ int xfe_absorb_device(double* d_V, int n, int C, int ld, const int* d_group_ids,
             if (G[d] <= 0) throw Error("num_groups must be positive");
         }
         absorb_device_impl(d_V, n, C, ld, gids, G, wsum, d_weights, sweep_order, *opt, *res);
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 0;
+    }
+}
+
+int xfe_absorb_device_slopes(double* d_V, int n, int C, int ld, const int* d_group_ids, const int* num_groups,
+                             int num_fes, const double* const* d_slope_values, const int* slope_mode,
+                             const double* d_weights, const int* sweep_order, const xfe_options* opt,
+                             xfe_absorb_result* res) {
+    clear_error();
+    if (!res || !opt || !d_V || !d_group_ids || !num_groups || !slope_mode || n < 0 || C <= 0 ||
+        C > XFE_MAX_COLS || ld < n || num_fes < 1 || num_fes > XFE_MAX_DIMS) {
+        set_error("xfe_absorb_device_slopes: invalid arguments");
+        return 0;
+    }
+    if (!method_ok(opt->method, "xfe_absorb_device_slopes")) return 0;
+    for (int d = 0; d < num_fes; ++d) {
+        if (slope_mode[d] < 0 || slope_mode[d] > 2 || num_groups[d] <= 0 ||
+            (slope_mode[d] != 0 && (!d_slope_values || !d_slope_values[d]))) {
+            set_error("xfe_absorb_device_slopes: each dimension needs levels, a mode in 0..2 and, for a slope, "
+                      "its values");
+            return 0;
+        }
+        if (sweep_order && (sweep_order[d] < 0 || sweep_order[d] >= num_fes)) {
+            set_error("xfe_absorb_device_slopes: invalid sweep_order");
+            return 0;
+        }
+    }
+    std::memset(res, 0, sizeof(*res));
+    if (n == 0) {
+        res->converged = 1;
+        return 1;
+    }
+    try {
+        (void)hipGetLastError();
+        XFE_CHECK(hipDeviceSynchronize());
+        Workspace& ws = tls_ws;
+        ws.init();
+        std::vector<const int*> gids(num_fes);
+        std::vector<int> G(num_fes), mode(num_fes), order(num_fes);
+        std::vector<const double*> z(num_fes, nullptr);
+        for (int d = 0; d < num_fes; ++d) {
+            gids[d] = d_group_ids + static_cast<size_t>(d) * n;
+            G[d] = num_groups[d];
+            mode[d] = slope_mode[d];
+            z[d] = mode[d] != 0 ? d_slope_values[d] : nullptr;
+            order[d] = sweep_order ? sweep_order[d] : d;
+            res->fe_levels[d] = G[d];
+        }
+        if (opt->precision == 1 && slope_mixed_eligible(C, num_fes, mode.data(), d_weights)) {
+            absorb_slopes_mixed(d_V, n, C, ld, gids, G, z, mode, *opt, *res);
+        } else {
+            absorb_slopes_impl(d_V, n, C, ld, gids, G, z, mode, d_weights, order, *opt, *res);
+            res->precision_fallback = opt->precision == 1 ? 1 : 0;
+        }
+        XFE_CHECK(hipStreamSynchronize(ws.stream));
         return 1;
     } catch (const std::exception& e) {
         set_error(e.what());
@@ -2643,8 +2793,13 @@ int xfe_absorb(const double* y, const double* X, int n, int k, const xfe_fe_inpu
                 z[d] = ws.stage_z[d].p;
                 mode[d] = fes[d].slope_has_intercept ? 1 : 2;
             }
-            absorb_slopes_impl(ws.stage.p, n, C, n, gids, G, z, mode, weights ? ws.stage_w.p : nullptr, order,
-                               *opt, *res);
+            if (opt->precision == 1 && slope_mixed_eligible(C, num_fes, mode.data(), weights)) {
+                absorb_slopes_mixed(ws.stage.p, n, C, n, gids, G, z, mode, *opt, *res);
+            } else {
+                absorb_slopes_impl(ws.stage.p, n, C, n, gids, G, z, mode, weights ? ws.stage_w.p : nullptr,
+                                   order, *opt, *res);
+                res->precision_fallback = opt->precision == 1 ? 1 : 0;
+            }
         } else {
             absorb_device_impl(ws.stage.p, n, C, n, gids, G, wsum, weights ? ws.stage_w.p : nullptr,
                                sweep_order, *opt, *res);

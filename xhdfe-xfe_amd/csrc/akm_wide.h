@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -40,6 +41,15 @@ struct AkmWideInput {
     const double* dw = nullptr;
     const double* corr = nullptr;
     const double* inv_diag = nullptr;
+};
+
+// An operator other than the firm-pair form (the absorber's slope path): q =
+// S p on level-major n_rows x L vectors, FP32 for the PCG passes and FP64 for
+// the residual replacements.
+struct WideOp {
+    virtual void apply(const float* p, float* q) = 0;
+    virtual void apply(const double* p, double* q) = 0;
+    virtual ~WideOp() = default;
 };
 
 struct AkmWide {
@@ -65,6 +75,9 @@ struct AkmWide {
     const int* mult = nullptr;
     int mstride = 0, mcols = 1;
     const float* minv_lane = nullptr;
+    // a custom operator (then J is its row count, ground -1 for none, minv32
+    // its Jacobi inverse diagonal, and 32 lanes are allowed)
+    WideOp* op = nullptr;
     double* h_sc = nullptr;  // pinned per-lane scalars
     int last_updates = 0;    // FP64 residual replacements of the last solve
     int last_passes = 0;     // FP32 operator passes of the last solve
@@ -90,10 +103,15 @@ struct AkmWide {
                             const std::vector<int>& w, const std::vector<double>& base, hipStream_t stream);
     // vectors for nb lanes (sets L)
     void ensure(int nb);
+    // a custom operator over n_rows levels with the Jacobi inverse diagonal
+    // minv (device, FP32, n_rows); no pair entries
+    void set_operator(WideOp* o, int n_rows, hipStream_t stream, const float* minv);
     // x64 = bn x64 (the unscaled solutions; lanes that did not converge zero)
     void unscale_x64();
     // iterations of lane l in the last solve, -1 when it did not converge
     int lane_iterations(int l) const;
+    // lane l's final scalars as text (diagnostics)
+    std::string lane_report(int l) const;
 
   private:
     template <typename T>

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -33,13 +34,17 @@ constexpr int kWBatch = 8;      // gathers issued together
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kPoll = 8;        // iterations between host polls
 constexpr double kDelta2 = 1e-4;  // FP32 residual drop (in r.r) that triggers an FP64 replacement
+// a recurrence that has dropped further than this (in r.r) since the last
+// replacement has left FP32's accuracy (its residual no longer follows the
+// true one), so the replacement restarts the direction as well: p = z
+constexpr double kRestart2 = 1e-10;
 constexpr int kMaxDotGrid = 1024;
 constexpr int kWideBlockCap = 16;    // firms per aggregate block
 constexpr int kWideBlockRounds = 6;  // handshake rounds
 
 // per-lane scalar slots (each kWideMaxLanes doubles), then two flags
 enum {
-    kSRz = 0, kSPq, kSAlpha, kSBeta, kSRr, kSRref, kSActive, kSIters, kSOk, kSBn, kSR64, kSSlots
+    kSRz = 0, kSPq, kSAlpha, kSBeta, kSRr, kSRref, kSActive, kSIters, kSOk, kSBn, kSR64, kSRestart, kSSlots
 };
 constexpr int kSFlags = kSSlots * kWideMaxLanes;  // [0] replacement wanted, [1] active lanes
 constexpr int kScTotal = kSFlags + 8;
@@ -411,7 +416,9 @@ wide_check_kernel(const double* part, int nblk, int L, double* sc, double tol2, 
     bool act = false;
     if (l < L) {
         S(sc, kSR64, l) = r2;
+        S(sc, kSRestart, l) = 0.0;
         if (S(sc, kSActive, l) != 0.0) {
+            if (S(sc, kSRr, l) < kRestart2 * S(sc, kSRref, l)) S(sc, kSRestart, l) = 1.0;
             if (r2 <= tol2) {
                 S(sc, kSActive, l) = 0.0;
                 S(sc, kSOk, l) = 1.0;
@@ -434,6 +441,14 @@ __global__ void __launch_bounds__(kScalarThreads) wide_rz_kernel(const double* p
     if (l >= L) return;
     S(sc, kSRz, l) = S(sc, kSActive, l) != 0.0 ? rz : 0.0;
     S(sc, kSBeta, l) = 0.0;
+}
+
+// p = z on the lanes that restart their direction
+__global__ void __launch_bounds__(kBlock) wide_restart_kernel(float* p, const float* z, const double* sc, int J, int L) {
+    const long long total = static_cast<long long>(J) * L;
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * kBlock)
+        if (sc[kSRestart * kWideMaxLanes + static_cast<int>(i % L)] != 0.0) p[i] = z[i];
 }
 
 // x64 += x32, x32 = 0
@@ -664,6 +679,16 @@ void AkmWide::unscale_x64() {
     XFE_CHECK(hipGetLastError());
 }
 
+std::string AkmWide::lane_report(int l) const {
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "ok %g active %g iters %g bn %.3e r64 %.3e rref %.3e rr %.3e rz %.3e pq %.3e",
+                  h_sc[kSOk * kWideMaxLanes + l], h_sc[kSActive * kWideMaxLanes + l],
+                  h_sc[kSIters * kWideMaxLanes + l], h_sc[kSBn * kWideMaxLanes + l], h_sc[kSR64 * kWideMaxLanes + l],
+                  h_sc[kSRref * kWideMaxLanes + l], h_sc[kSRr * kWideMaxLanes + l], h_sc[kSRz * kWideMaxLanes + l],
+                  h_sc[kSPq * kWideMaxLanes + l]);
+    return buf;
+}
+
 int AkmWide::lane_iterations(int l) const {
     return h_sc[kSOk * kWideMaxLanes + l] != 0.0 ? static_cast<int>(h_sc[kSIters * kWideMaxLanes + l]) : -1;
 }
@@ -677,9 +702,24 @@ void AkmWide::release() {
     h_sc = nullptr;
 }
 
+void AkmWide::set_operator(WideOp* o, int n_rows, hipStream_t stream, const float* minv) {
+    op = o;
+    s = stream;
+    J = n_rows;
+    ground = -1;
+    E = 0;
+    blocks = false;
+    mult = nullptr;
+    minv_lane = nullptr;
+    minv32.ensure(n_rows);
+    XFE_CHECK(hipMemcpyAsync(minv32.p, minv, sizeof(float) * n_rows, hipMemcpyDeviceToDevice, s));
+    sc.ensure(kScTotal);
+    if (!h_sc) XFE_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_sc), sizeof(double) * kScTotal, hipHostMallocDefault));
+}
+
 void AkmWide::ensure(int nb) {
     const int V = nb <= 64 ? 1 : nb <= 128 ? 2 : 4;
-    L = 64 * V;
+    L = (op && nb <= 32) ? 32 : 64 * V;
     const size_t JL = static_cast<size_t>(J) * L;
     const size_t BL = 2 * ((static_cast<size_t>(std::max(E, 1)) + kWPiece - 1) / kWPiece);
     for (DevBuf<float>* d : {&x, &r, &z, &p, &q}) d->ensure(JL);
@@ -730,6 +770,10 @@ static void apply_v(AkmWide& w, const T* pv, T* out, const T* w_, const T* corr_
 
 template <typename T>
 void AkmWide::apply(const T* pv, T* out, const T* w_, const T* corr_, DevBuf<T>* bv) {
+    if (op) {
+        op->apply(pv, out);
+        return;
+    }
     if (L == 64) apply_v<T, 1>(*this, pv, out, w_, corr_, bv);
     else if (L == 128) apply_v<T, 2>(*this, pv, out, w_, corr_, bv);
     else apply_v<T, 4>(*this, pv, out, w_, corr_, bv);
@@ -772,6 +816,7 @@ void AkmWide::solve_device(int nb, double tol, int max_iter) {
         hipLaunchKernelGGL(wide_reset_kernel, dim3(dg), dim3(kBlock), 0, s, r.p, z.p, p.p, r64.p, minv32.p, sc.p, J,
                            L, part.p, minv_lane);
         block_precondition();
+        hipLaunchKernelGGL(wide_restart_kernel, dim3(eg), dim3(kBlock), 0, s, p.p, z.p, sc.p, J, L);
         hipLaunchKernelGGL(wide_rz_kernel, dim3(1), dim3(kScalarThreads), 0, s, part.p, dg, L, sc.p);
         XFE_CHECK(hipGetLastError());
     };

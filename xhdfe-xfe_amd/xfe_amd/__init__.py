@@ -65,7 +65,8 @@ class Certificate(ctypes.Structure):
 
 
 # Every entry point of include/xfe_hip.h (the CPU test checks that each is exported).
-EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_drop_singletons",
+EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_absorb_device_slopes",
+           "xfe_drop_singletons",
            "xfe_bipartite_components", "xfe_fe_dof", "xfe_gram", "xfe_cluster_meat",
            "xfe_recovery_max_delta", "xfe_last_error", "xfe_profile_enable", "xfe_profile_reset",
            "xfe_profile_get", "xfe_release_workspace", "xfe_certify", "xfe_akm_available",
@@ -215,6 +216,26 @@ def absorb(y, X, gids, G, weight_sums=None, weights=None, sweep_order=None, tol=
     if not ok:
         raise XfeError(last_error())
     return yo, Xo, _info(r, D)
+
+
+def absorb_device_slopes(d_V, n, C, ld, d_ids, G, slope_ptrs, slope_mode, tol=1e-8, tolerance_mode=1,
+                         max_iter=100000, method=1, precision=1, certify=True, refine_tol=0.0):
+    """xfe_absorb_device_slopes on device pointers: d_V (n x C, ld), d_ids
+    (num_fes x n int32), slope_ptrs[d] (device pointer or 0), slope_mode[d]
+    (0 standard, 1 slope with intercept, 2 slope only).  Returns info."""
+    D = len(G)
+    Garr = np.ascontiguousarray(G, dtype=np.int32)
+    modes = np.ascontiguousarray(slope_mode, dtype=np.int32)
+    zp = (ctypes.c_void_p * D)(*[int(p) if p else None for p in slope_ptrs])
+    o = _opts(tol, tolerance_mode, max_iter, method, False, precision, certify, refine_tol)
+    r = AbsorbResult()
+    L = lib()
+    L.xfe_absorb_device_slopes.argtypes = [_p, _i, _i, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p]
+    ok = L.xfe_absorb_device_slopes(_p(d_V), n, C, ld, _p(d_ids), _ptr(Garr), D, zp, _ptr(modes), None, None,
+                                    ctypes.byref(o), ctypes.byref(r))
+    if not ok:
+        raise XfeError(last_error())
+    return _info(r, D)
 
 
 def absorb_group_individual(y, X, gids, G, gi, weights=None, sweep_order=None, tol=1e-8, tolerance_mode=1,
