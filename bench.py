@@ -368,57 +368,114 @@ def bench_boot(args):
     print(json.dumps(out), flush=True)
 
 
-SLOPE_METRIC = "s to absorb a heterogeneous-slope panel (firm intercept + slope on x1, worker, firm x year) at 1e-8"
+SLOPE_METRIC = "s to absorb the 30M-obs heterogeneous-slope panel (worker, firm intercept + slope on x1, firm x year; 32 columns) at 1e-8, mixed precision"
 
 
-def bench_slope(args):
-    """The heterogeneous-slope path (xfe_absorb with a slope dimension) on the
-    slope shape at --scale of 30M obs (default 0.1): singletons dropped on the
-    device, then the host entry point (upload, absorb, download) timed."""
-    import torch  # noqa: F401  (shares torch's HIP runtime)
-
+def slope_case(scale):
+    """The slope shape at `scale` of 30M obs, singletons dropped on the device,
+    ids compacted: (V n x C, [worker, firm, cell] ids, levels, z, dropped)."""
     import xfe_amd
     from xfe_amd import panels
-
-    if not xfe_amd.backend_available():
-        raise SystemExit("no HIP device")
-    scale = args.scale if args.scale < 1.0 else 0.1
-    t0 = time.perf_counter()
     p = panels.slope_panel(scale=scale)
     keep, dropped, _ = xfe_amd.drop_singletons([f.astype(np.int32) for f in p["fes"]])
     comp = [compact(f[keep]) for f in p["fes"]]
-    gids = [c[0] for c in comp]
-    G = [c[1] for c in comp]
-    y, X, z = p["y"][keep], np.asfortranarray(p["X"][keep]), p["z"][keep]
-    n, C = y.size, X.shape[1] + 1
-    del p
+    V = np.column_stack([p["y"][keep], p["X"][keep]])
+    z = p["z"][keep]
+    return V, [c[0] for c in comp], [c[1] for c in comp], z, dropped
+
+
+def bench_slope(args):
+    """Config 2: the heterogeneous-slope shape (worker, firm with intercept and
+    slope on x1, firm x year cells; y and 31 regressors, C = 32) at --scale of
+    30M obs (default 1.0), singletons dropped on the device; the timed step is
+    the mixed-precision absorption (xfe_absorb_device_slopes, precision 1) of
+    the columns resident in HBM.  Beside it, on the same shape at 3 %, the
+    mixed path and the FP64 slope path (CG on the symmetric projection sweep)
+    are timed and compared."""
+    import torch
+
+    import xfe_amd
+
+    if not xfe_amd.backend_available():
+        raise SystemExit("no HIP device")
+    scale = args.scale
+    t0 = time.perf_counter()
+    V, gids, G, z, dropped = slope_case(scale)
+    n, C = V.shape
     log(f"slope panel: n={n} C={C} levels={G} ({dropped} singletons) in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    V0 = torch.from_numpy(np.ascontiguousarray(V.T)).to(dev)  # C x n: column-major, ld n
+    Vd = torch.empty_like(V0)
+    ids = torch.from_numpy(np.ascontiguousarray(np.stack(gids))).to(dev)
+    zd = torch.from_numpy(np.ascontiguousarray(z)).to(dev)
+    vmax = float(np.abs(V).max())
+    del V
+    torch.cuda.synchronize()
+
+    def step():
+        Vd.copy_(V0)
+        return xfe_amd.absorb_device_slopes(Vd.data_ptr(), n, C, n, ids.data_ptr(), G, [0, zd.data_ptr(), 0],
+                                            [0, 1, 0], tol=1e-8, precision=1)
+
     for w in range(args.warmup):
         t0 = time.perf_counter()
-        _, _, info = xfe_amd.absorb(y, X, gids, G, tol=1e-8, slopes={1: (z, True)})
-        log(f"warmup {w}: {time.perf_counter() - t0:.3f} s, steps {info['iterations']}, sweeps {info['sweeps']}")
+        info = step()
+        torch.cuda.synchronize()
+        log(f"warmup {w}: {time.perf_counter() - t0:.3f} s, iterations {info['iterations']}, "
+            f"replacements {info['refinements']}, converged {info['converged']}")
     xfe_amd.profile_reset()
     xfe_amd.profile_enable(True)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        _, _, info = xfe_amd.absorb(y, X, gids, G, tol=1e-8, slopes={1: (z, True)})
+        info = step()
+        log(f"step {k}: iterations {info['iterations']} replacements {info['refinements']} "
+            f"converged {info['converged']} certificate {info['abs_residual_rel']:.2e}")
+    torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / args.steps
     xfe_amd.profile_enable(False)
     pass_ms, launches, pass_bytes = xfe_amd.profile_get()
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
-    log(f"{sec:.3f} s per absorption; slope_reduce_kernel: {launches} launches, "
+    log(f"{sec:.3f} s per absorption; sm_apply_kernel<float>: {launches} launches, "
         f"{pass_ms / max(launches, 1):.3f} ms avg, {achieved:.0f} GB/s algorithmic")
+    del V0, Vd, ids, zd
+    torch.cuda.empty_cache()
+    # equal-size comparison with the FP64 slope path (host entry points)
+    Vs, gs, Gs, zs, _ = slope_case(0.03)
+    t0 = time.perf_counter()
+    ym, Xm, im = xfe_amd.absorb(Vs[:, 0], Vs[:, 1:], gs, Gs, tol=1e-8, precision=1, slopes={1: (zs, True)})
+    mixed_small = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    y6, X6, i6 = xfe_amd.absorb(Vs[:, 0], Vs[:, 1:], gs, Gs, tol=1e-8, precision=0, slopes={1: (zs, True)})
+    fp64_small = time.perf_counter() - t0
+    y7, X7, i7 = xfe_amd.absorb(Vs[:, 0], Vs[:, 1:], gs, Gs, tol=1e-13, precision=0, slopes={1: (zs, True)})
+    ref = np.column_stack([y7, X7])
+    colscale = np.abs(Vs).max(axis=0)
+    err_m = float((np.abs(np.column_stack([ym, Xm]) - ref).max(axis=0) / colscale).max())
+    err_6 = float((np.abs(np.column_stack([y6, X6]) - ref).max(axis=0) / colscale).max())
+    log(f"3% shape ({Vs.shape[0]} obs): mixed {mixed_small:.2f} s ({im['iterations']} it), FP64 path "
+        f"{fp64_small:.2f} s ({i6['iterations']} it); max column error vs FP64 tol 1e-13: mixed {err_m:.2e}, "
+        f"FP64 tol 1e-8 {err_6:.2e}")
     out = {
         "metric": SLOPE_METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "method": "FP64 CG on the symmetric projection sweep, rows in slope-level order (host entry: PCIe included)",
-        "config": {"workload": f"slope_{scale:g}x30m", "obs": int(n), "columns": C, "levels": G,
-                   "slope": "firm intercept + slope", "tol": 1e-8, "singletons_dropped": int(dropped)},
-        "iterations": info["iterations"], "sweeps": info["sweeps"], "converged": info["converged"],
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "method": ("worker dimension eliminated; FP32 Jacobi-PCG on the Schur system of the firm x year cells and "
+                   "firm slopes (the firm intercept lies in the cells' span), 32 columns as lanes, FP64 residual "
+                   "replacement, FP64 final transform; columns resident in HBM"),
+        "config": {"workload": f"slope_{scale:g}x30m_32cols", "obs": int(n), "columns": C, "levels": G,
+                   "slope": "firm intercept + slope on x1", "tol": 1e-8, "singletons_dropped": int(dropped)},
+        "iterations": info["iterations"], "replacements": info["refinements"], "converged": info["converged"],
+        "certificate_rel": info["abs_residual_rel"], "certified": info["precision_certified"],
+        "obs_cols_per_s": round(n * C / sec, 1),
+        "equal_size_3pct": {"obs": int(Vs.shape[0]), "mixed_s": round(mixed_small, 3),
+                            "fp64_slope_path_s": round(fp64_small, 3), "mixed_iterations": im["iterations"],
+                            "fp64_iterations": i6["iterations"], "mixed_max_col_err_vs_fp64_tight": err_m,
+                            "fp64_tol_1e8_max_col_err_vs_fp64_tight": err_6,
+                            "note": "host entry points (PCIe transfers included) for both"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "slope_reduce_kernel (level sums of w v and w z v, FP64 atomics)",
+                     "kernel": "sm_apply_kernel<float> (per worker: coefficient gathers, worker mean, R rows)",
                      "avg_launch_ms": round(pass_ms / max(launches, 1), 4),
                      "bytes_per_launch": round(pass_bytes / max(launches, 1), 1)},
         "cpu_baseline": None,

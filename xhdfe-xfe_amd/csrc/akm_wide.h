@@ -49,6 +49,10 @@ struct AkmWideInput {
 struct WideOp {
     virtual void apply(const float* p, float* q) = 0;
     virtual void apply(const double* p, double* q) = 0;
+    // a block preconditioner of its own: z = B^-1 r on its blocks (z keeps
+    // the Jacobi values elsewhere)
+    virtual bool has_blocks() const { return false; }
+    virtual void precondition(const float* r, float* z) { (void)r; (void)z; }
     virtual ~WideOp() = default;
 };
 
@@ -81,6 +85,12 @@ struct AkmWide {
     double* h_sc = nullptr;  // pinned per-lane scalars
     int last_updates = 0;    // FP64 residual replacements of the last solve
     int last_passes = 0;     // FP32 operator passes of the last solve
+    const char* trace = nullptr;  // when set: one line per FP64 replacement on stderr, with this tag
+    // restart the direction (p = z) at every FP64 replacement: restarted
+    // iterative refinement, each FP32 run a fresh PCG on the current residual
+    // (robust where the FP32 operator loses the slow modes' accuracy); by
+    // default the direction is kept unless the recurrence outran FP32
+    bool restart_each_replacement = false;
 
     // use_blocks: build the aggregate blocks (cap firms per block, rounds of
     // handshake aggregation over the coupling strengths)
@@ -94,7 +104,10 @@ struct AkmWide {
     // the same on right-hand sides already on the device, level-major J x L
     // FP64 (ground rows ignored); leaves the unscaled solutions level-major in
     // x64 and the per-lane results in h_sc
-    void solve_device(int nb, double tol, int max_iter);
+    // accept (> tol): a lane whose FP64 residual stalls (fails to halve over
+    // four replacements in a row) counts as converged at |r| <= accept; the
+    // default keeps the strict rule |r| <= tol
+    void solve_device(int nb, double tol, int max_iter, double accept = 0.0);
 
     // entries given per worker, not merged (f, g, worker, base weight), no
     // diagonal correction, no blocks: the operator sum_w mult_w L_w of a

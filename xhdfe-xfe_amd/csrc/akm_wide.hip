@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -34,17 +35,29 @@ constexpr int kWBatch = 8;      // gathers issued together
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kPoll = 8;        // iterations between host polls
 constexpr double kDelta2 = 1e-4;  // FP32 residual drop (in r.r) that triggers an FP64 replacement
+// the restarted mode lets each FP32 run reduce |r| 1000-fold: far enough that
+// the restart costs little, short of where FP32's rounding of the operator and
+// of the block inverses makes the recurrence drift from the true residual
+constexpr double kDelta2Restart = 1e-6;
 // a recurrence that has dropped further than this (in r.r) since the last
 // replacement has left FP32's accuracy (its residual no longer follows the
 // true one), so the replacement restarts the direction as well: p = z
 constexpr double kRestart2 = 1e-10;
+// a lane is also due for a replacement after this many iterations without one
+// (a recurrence that stagnates above the kDelta2 drop would otherwise drift)
+constexpr int kMaxRun = 128;
+// in the restarted mode a run that has not reached its kDelta2Restart drop
+// after this many iterations is stagnating in FP32: it is replaced and
+// restarted (and stops under the stall rule when that does not help)
+constexpr int kMaxRunRestart = 512;
 constexpr int kMaxDotGrid = 1024;
 constexpr int kWideBlockCap = 16;    // firms per aggregate block
 constexpr int kWideBlockRounds = 6;  // handshake rounds
 
 // per-lane scalar slots (each kWideMaxLanes doubles), then two flags
 enum {
-    kSRz = 0, kSPq, kSAlpha, kSBeta, kSRr, kSRref, kSActive, kSIters, kSOk, kSBn, kSR64, kSRestart, kSSlots
+    kSRz = 0, kSPq, kSAlpha, kSBeta, kSRr, kSRref, kSActive, kSIters, kSOk, kSBn, kSR64, kSRestart, kSStall, kSWant,
+    kSSince, kSForced, kSSlots
 };
 constexpr int kSFlags = kSSlots * kWideMaxLanes;  // [0] replacement wanted, [1] active lanes
 constexpr int kScTotal = kSFlags + 8;
@@ -342,6 +355,7 @@ __global__ void __launch_bounds__(kBlock) wide_init_kernel(double* sc, int nb) {
     const int l = threadIdx.x;
     for (int s = 0; s < kSSlots; ++s) S(sc, s, l) = 0.0;
     S(sc, kSActive, l) = l < nb ? 1.0 : 0.0;
+    S(sc, kSWant, l) = 1.0;  // the first FP64 check covers every lane
     if (l < 8) sc[kSFlags + l] = 0.0;
 }
 
@@ -384,7 +398,8 @@ __global__ void __launch_bounds__(kScalarThreads) wide_alpha_kernel(const double
 // beta = rz_new / rz; an FP64 replacement is wanted once r.r has dropped by
 // kDelta2 since the last one, fallen below tol^2, or at max_iter
 __global__ void __launch_bounds__(kScalarThreads)
-wide_beta_kernel(const double* part, int nblk, int L, double* sc, double tol2, int max_iter) {
+wide_beta_kernel(const double* part, int nblk, int L, double* sc, double tol2, int max_iter, int max_run,
+                 double delta2) {
     __shared__ double red[kScalarThreads];
     const double rzn = lane_total(part, nblk, L, 0, red);
     const double rr = lane_total(part, nblk, L, 1, red);
@@ -399,29 +414,56 @@ wide_beta_kernel(const double* part, int nblk, int L, double* sc, double tol2, i
             S(sc, kSRz, l) = rzn;
             S(sc, kSRr, l) = rr;
             S(sc, kSIters, l) += 1.0;
-            want = rr <= kDelta2 * S(sc, kSRref, l) || rr <= tol2 || S(sc, kSIters, l) >= max_iter;
+            S(sc, kSSince, l) += 1.0;
+            want = rr <= delta2 * S(sc, kSRref, l) || rr <= tol2 || S(sc, kSIters, l) >= max_iter ||
+                   S(sc, kSSince, l) >= max_run;
         }
         S(sc, kSBeta, l) = beta;
+        S(sc, kSWant, l) = want ? 1.0 : 0.0;
+        S(sc, kSForced, l) = 0.0;
     }
     store_flags(sc, want, act);
 }
 
 // FP64 residual check after a replacement: converged lanes stop (ok), lanes
 // at max_iter stop unconverged, the others restart their reference norm
+// A lane whose FP64 residual fails to halve over kStallUpdates replacements in
+// a row has reached what the FP64 residual can resolve: it stops, converged
+// when r.r <= accept2 (the caller's acceptance; accept2 = tol2 keeps the
+// strict rule).
+constexpr int kStallUpdates = 4;
+
 __global__ void __launch_bounds__(kScalarThreads)
-wide_check_kernel(const double* part, int nblk, int L, double* sc, double tol2, int max_iter) {
+wide_check_kernel(const double* part, int nblk, int L, double* sc, double tol2, int max_iter, double accept2,
+                  int restart_all) {
     __shared__ double red[kScalarThreads];
     const double r2 = lane_total(part, nblk, L, 0, red);
     const int l = threadIdx.x;
     bool act = false;
     if (l < L) {
-        S(sc, kSR64, l) = r2;
         S(sc, kSRestart, l) = 0.0;
-        if (S(sc, kSActive, l) != 0.0) {
-            if (S(sc, kSRr, l) < kRestart2 * S(sc, kSRref, l)) S(sc, kSRestart, l) = 1.0;
+        if (S(sc, kSActive, l) != 0.0 && S(sc, kSWant, l) == 0.0) {
+            act = true;  // a lane not due for a replacement keeps its recurrence
+        } else if (S(sc, kSActive, l) != 0.0) {
+            S(sc, kSR64, l) = r2;
+            S(sc, kSSince, l) = 0.0;
+            // the direction is kept only while the recurrence follows the true
+            // residual: restart when it outran FP32 (kRestart2) or when the FP64
+            // residual is more than 10x (in norm) what the recurrence claimed
+            // (a lane replaced with the others before its own drop neither
+            // restarts on the gap nor counts toward a stall)
+            const double rr = S(sc, kSRr, l);
+            const bool own = S(sc, kSForced, l) == 0.0;
+            if (restart_all || rr < kRestart2 * S(sc, kSRref, l) || (own && r2 > 100.0 * rr))
+                S(sc, kSRestart, l) = 1.0;
+            const bool first = S(sc, kSRref, l) == 0.0;
+            if (own) S(sc, kSStall, l) = (!first && r2 > 0.25 * S(sc, kSRref, l)) ? S(sc, kSStall, l) + 1.0 : 0.0;
             if (r2 <= tol2) {
                 S(sc, kSActive, l) = 0.0;
                 S(sc, kSOk, l) = 1.0;
+            } else if (S(sc, kSStall, l) >= kStallUpdates) {
+                S(sc, kSActive, l) = 0.0;
+                S(sc, kSOk, l) = r2 <= accept2 ? 1.0 : 0.0;
             } else if (S(sc, kSIters, l) >= max_iter || !(r2 < 1e300)) {
                 S(sc, kSActive, l) = 0.0;
                 S(sc, kSOk, l) = 0.0;
@@ -451,12 +493,23 @@ __global__ void __launch_bounds__(kBlock) wide_restart_kernel(float* p, const fl
         if (sc[kSRestart * kWideMaxLanes + static_cast<int>(i % L)] != 0.0) p[i] = z[i];
 }
 
-// x64 += x32, x32 = 0
-__global__ void __launch_bounds__(kBlock) wide_fold_kernel(double* x64, float* x32, long long total) {
+// x64 += x32, x32 = 0 on the lanes due for a replacement
+__global__ void __launch_bounds__(kBlock)
+wide_fold_kernel(double* x64, float* x32, long long total, int L, const double* sc) {
     for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
          i += static_cast<long long>(gridDim.x) * kBlock) {
+        if (sc[kSWant * kWideMaxLanes + static_cast<int>(i % L)] == 0.0) continue;
         x64[i] += static_cast<double>(x32[i]);
         x32[i] = 0.0f;
+    }
+}
+
+// every active lane is due (the last iteration)
+__global__ void __launch_bounds__(kBlock) wide_want_all_kernel(double* sc) {
+    const int l = threadIdx.x;
+    if (S(sc, kSActive, l) != 0.0 && S(sc, kSWant, l) == 0.0) {
+        S(sc, kSWant, l) = 1.0;
+        S(sc, kSForced, l) = 1.0;  // replaced with the others, not on its own progress
     }
 }
 
@@ -480,10 +533,13 @@ wide_reset_kernel(float* __restrict__ r, float* __restrict__ z, float* __restric
     __shared__ double red[kBlock];
     const int l = threadIdx.x % L, r0 = threadIdx.x / L, R = kBlock / L;
     const bool act = sc[kSActive * kWideMaxLanes + l] != 0.0;
+    const bool due = sc[kSWant * kWideMaxLanes + l] != 0.0;
     double rz = 0.0;
     for (long long j = static_cast<long long>(blockIdx.x) * R + r0; j < J; j += static_cast<long long>(gridDim.x) * R) {
         const size_t i = static_cast<size_t>(j) * L + l;
-        if (act) {
+        if (act && !due) {
+            rz += static_cast<double>(r[i]) * z[i];
+        } else if (act) {
             const float ri = static_cast<float>(r64[i]);
             const float zi = (minv_lane ? minv_lane[i] : minv[j]) * ri;
             r[i] = ri;
@@ -782,14 +838,21 @@ void AkmWide::apply(const T* pv, T* out, const T* w_, const T* corr_, DevBuf<T>*
 template void AkmWide::apply<float>(const float*, float*, const float*, const float*, DevBuf<float>*);
 template void AkmWide::apply<double>(const double*, double*, const double*, const double*, DevBuf<double>*);
 
-void AkmWide::solve_device(int nb, double tol, int max_iter) {
+void AkmWide::solve_device(int nb, double tol, int max_iter, double accept) {
     const long long JL = static_cast<long long>(J) * L;
     const int eg = ew_grid(JL), dg = dot_grid(J, L);
     const double tol2 = tol * tol;
+    const double accept2 = accept > tol ? accept * accept : tol2;
     // z = inv(block) r on the blocked firms (Jacobi elsewhere, from the step or
     // reset kernel) and the partials of r.z in place of the Jacobi ones; an
     // inactive lane has r = 0, so its z stays 0
     auto block_precondition = [&]() {
+        if (op && op->has_blocks()) {
+            op->precondition(r.p, z.p);
+            hipLaunchKernelGGL((wide_dot_kernel<float>), dim3(dg), dim3(kBlock), 0, s, r.p, z.p, J, L, part.p);
+            XFE_CHECK(hipGetLastError());
+            return;
+        }
         if (!blocks) return;
         XFE_CHECK(lp::launch_block_apply(agg.view(), r.p, z.p, L, s));
         hipLaunchKernelGGL((wide_dot_kernel<float>), dim3(dg), dim3(kBlock), 0, s, r.p, z.p, J, L, part.p);
@@ -812,7 +875,7 @@ void AkmWide::solve_device(int nb, double tol, int max_iter) {
     auto replace = [&]() {
         hipLaunchKernelGGL((wide_dot_kernel<double>), dim3(dg), dim3(kBlock), 0, s, r64.p, r64.p, J, L, part.p);
         hipLaunchKernelGGL(wide_check_kernel, dim3(1), dim3(kScalarThreads), 0, s, part.p, dg, L, sc.p, tol2,
-                           max_iter);
+                           max_iter, accept2, restart_each_replacement ? 1 : 0);
         hipLaunchKernelGGL(wide_reset_kernel, dim3(dg), dim3(kBlock), 0, s, r.p, z.p, p.p, r64.p, minv32.p, sc.p, J,
                            L, part.p, minv_lane);
         block_precondition();
@@ -835,24 +898,49 @@ void AkmWide::solve_device(int nb, double tol, int max_iter) {
                            J, L, part.p, minv_lane);
         block_precondition();
         hipLaunchKernelGGL(wide_beta_kernel, dim3(1), dim3(kScalarThreads), 0, s, part.p, dg, L, sc.p, tol2,
-                           max_iter);
+                           max_iter, restart_each_replacement ? kMaxRunRestart : kMaxRun,
+                           restart_each_replacement ? kDelta2Restart : kDelta2);
         hipLaunchKernelGGL(wide_dir_kernel, dim3(eg), dim3(kBlock), 0, s, p.p, z.p, sc.p, J, L);
         XFE_CHECK(hipGetLastError());
         if (it % kPoll != 0 && it != max_iter) continue;
         XFE_CHECK(hipMemcpyAsync(h_sc + kSFlags, sc.p + kSFlags, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         XFE_CHECK(hipStreamSynchronize(s));
         if (h_sc[kSFlags + 1] <= 0.0) break;
+        if (trace && it % 256 == 0) {
+            XFE_CHECK(hipMemcpyAsync(h_sc, sc.p, sizeof(double) * kScTotal, hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+            double least = 1.0;
+            for (int l = 0; l < nb; ++l)
+                if (h_sc[kSActive * kWideMaxLanes + l] != 0.0 && h_sc[kSRref * kWideMaxLanes + l] > 0.0)
+                    least = std::min(least, h_sc[kSRr * kWideMaxLanes + l] / h_sc[kSRref * kWideMaxLanes + l]);
+            std::fprintf(stderr, "[%s] iteration %d: %g lanes active, least r.r drop of the run %.3e\n", trace, it,
+                         h_sc[kSFlags + 1], least);
+        }
         if (h_sc[kSFlags] == 0.0 && it != max_iter) continue;
         // FP64 residual replacement: x64 += x32, r64 = b - S x64
-        hipLaunchKernelGGL(wide_fold_kernel, dim3(eg), dim3(kBlock), 0, s, x64.p, x.p, JL);
+        // the restarted mode replaces the lanes that are due; the kept-direction
+        // mode replaces every active lane together (a lane whose recurrence has
+        // not yet dropped keeps its direction, and a stale one is restarted)
+        if (it == max_iter || !restart_each_replacement)
+            hipLaunchKernelGGL(wide_want_all_kernel, dim3(1), dim3(kWideMaxLanes), 0, s, sc.p);
+        hipLaunchKernelGGL(wide_fold_kernel, dim3(eg), dim3(kBlock), 0, s, x64.p, x.p, JL, L, sc.p);
         apply<double>(x64.p, q64.p, w64.p, corr64.p, bv64);
         hipLaunchKernelGGL(wide_res_kernel, dim3(eg), dim3(kBlock), 0, s, r64.p, b64.p, q64.p, J, L, ground);
         XFE_CHECK(hipGetLastError());
         replace();
         ++last_updates;
         XFE_CHECK(hipMemcpyAsync(h_sc + kSFlags, sc.p + kSFlags, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (trace)
+            XFE_CHECK(hipMemcpyAsync(h_sc + kSR64 * kWideMaxLanes, sc.p + kSR64 * kWideMaxLanes,
+                                     sizeof(double) * kWideMaxLanes, hipMemcpyDeviceToHost, s));
         XFE_CHECK(hipStreamSynchronize(s));
         live = h_sc[kSFlags + 1] > 0.0;
+        if (trace) {
+            double worst = 0.0;
+            for (int l = 0; l < nb; ++l) worst = std::max(worst, h_sc[kSR64 * kWideMaxLanes + l]);
+            std::fprintf(stderr, "[%s] iteration %d replacement %d: %g lanes active, worst |r|/|b| %.3e\n", trace, it,
+                         last_updates, h_sc[kSFlags + 1], std::sqrt(worst));
+        }
     }
     // lanes still running at the end stop unconverged (their x32 is not folded)
     XFE_CHECK(hipMemcpyAsync(h_sc, sc.p, sizeof(double) * kScTotal, hipMemcpyDeviceToHost, s));

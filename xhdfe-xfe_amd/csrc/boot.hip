@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -321,6 +323,7 @@ int xfe_boot_refits(xfe_boot_context* c, int R, const int* mult, double tol, int
                                dim3(kBlock), 0, s, c->f_ptr.p, c->f_m.p, c->m_w.p, c->m_c.p, c->dw.p, c->ms.p,
                                c->multT.p, J, J - 1, c->wide.b64.p, c->minv.p);
             XFE_CHECK(hipGetLastError());
+            c->wide.trace = std::getenv("XFE_BOOT_TRACE") ? "boot" : nullptr;
             c->wide.solve_device(kWideMaxLanes, tol, max_iter);
             c->last_passes += c->wide.last_passes;
             c->last_updates += c->wide.last_updates;
@@ -344,6 +347,10 @@ int xfe_boot_refits(xfe_boot_context* c, int R, const int* mult, double tol, int
                     it = std::max(it, li);
                 }
                 if (it < 0) failed = true;
+                if (it < 0 && c->wide.trace)
+                    for (int col = 0; col < C; ++col)
+                        std::fprintf(stderr, "[boot] refit %d column %d: %s\n", r0 + b, col,
+                                     c->wide.lane_report(b * kBootCols + col).c_str());
                 iters_out[r0 + b] = it;
                 double* g = gram_out + static_cast<size_t>(r0 + b) * C * C;
                 int k = 0;

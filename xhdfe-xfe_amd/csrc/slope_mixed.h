@@ -22,6 +22,7 @@ struct SlopeMixedResult {
     int unknowns = 0;            // levels and slopes of the Schur system
     int primary = -1;            // the eliminated dimension
     int intercepts_dropped = 0;  // slope intercepts spanned by a finer dimension
+    int pinned = 0;              // levels pinned to zero (one per connected component and standard block)
     bool converged = false;
     // per column: the sum over every dimension's levels (intercept and slope
     // moments) of the squared level sums of the transform, except the dropped
@@ -38,7 +39,9 @@ void slope_mixed_delete(SlopeMixed* sm);
 bool slope_mixed_eligible(int C, int D, const int* mode, const double* w);
 
 // V: n x C column-major (ld), device, the caller's row order on entry and on
-// exit (transformed in place); gid / z: device arrays of n.
+// exit (transformed in place); gid / z: device arrays of n.  tol: the target
+// of the Schur residual |b - S x| / |b| per column; a column whose FP64
+// residual stalls above it is accepted at 1e-9 (the absorber's mixed rule).
 void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std::vector<const int*>& gid,
                      const std::vector<int>& G, const std::vector<const double*>& z, const std::vector<int>& mode,
                      double tol, int max_iter, hipStream_t s, SlopeMixedResult& out);

@@ -74,26 +74,44 @@ __global__ void __launch_bounds__(kBlock) sm_apply_kernel(SmArgs<T> a) {
         if (i1 <= i0 || !live) continue;
         T cache[kRowCache];
         T sum = T(0);
-        auto row_value = [&](int i) {
-            T u = T(0);
-            if constexpr (HAS_V) u = static_cast<T>(a.V[static_cast<size_t>(col) * a.ld + i]);
-            if (a.x) {
+        // u_i - u_i0, formed coefficient by coefficient (differences of the
+        // level values first; a slope term of one level as (z_i - z_i0) x), so
+        // the FP32 result keeps the accuracy of small differences on the
+        // slow, smooth modes instead of rounding each u to |u|
+        // the level's first data value, read once: the STORE pass writes the
+        // transform over V in place (row i0 first), and rows past the register
+        // cache are formed again after that
+        double v0 = 0.0;
+        if constexpr (HAS_V) v0 = a.V[static_cast<size_t>(col) * a.ld + i0];
+        auto row_delta = [&](int i) {
+            T d = T(0);
+            if constexpr (HAS_V) d = static_cast<T>(a.V[static_cast<size_t>(col) * a.ld + i] - v0);
+            if (a.x && i != i0) {
                 for (int r = 0; r < a.NR; ++r) {
-                    const T m = a.mult[r] ? a.mult[r][i] : T(1);
-                    u += a.sign * m * a.x[static_cast<size_t>(a.off[r] + a.gid[r][i]) * a.L + col];
+                    const int gi = a.gid[r][i], g0 = a.gid[r][i0];
+                    const T xi = a.x[static_cast<size_t>(a.off[r] + gi) * a.L + col];
+                    const T x0 = a.x[static_cast<size_t>(a.off[r] + g0) * a.L + col];
+                    T t;
+                    if (a.mult[r]) {
+                        const T mi = a.mult[r][i], m0 = a.mult[r][i0];
+                        t = gi == g0 ? (mi - m0) * xi : mi * xi - m0 * x0;
+                    } else {
+                        t = xi - x0;
+                    }
+                    d += a.sign * t;
                 }
             }
-            return u;
+            return d;
         };
         for (int i = i0; i < i1; ++i) {
-            const T u = row_value(i);
+            const T u = row_delta(i);
             if (i - i0 < kRowCache) cache[i - i0] = u;
             sum += u;
         }
         const T mean = sum / static_cast<T>(i1 - i0);
         double tot = 0.0;
         for (int i = i0; i < i1; ++i) {
-            const T u = i - i0 < kRowCache ? cache[i - i0] : row_value(i);
+            const T u = i - i0 < kRowCache ? cache[i - i0] : row_delta(i);
             const T r = u - mean;
             a.R[static_cast<size_t>(i) * kCols + col] = r;
             if constexpr (STORE) {
@@ -278,6 +296,274 @@ __global__ void __launch_bounds__(kBlock) sm_fill_int_kernel(int* v, int n, int 
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[i] = x;
 }
 
+// connected components of the bipartite graph (p level, block level) of the
+// rows: parent over Gp + Gb nodes, hooking the larger root under the smaller
+// until no edge changes a root, then full compression
+__global__ void __launch_bounds__(kBlock)
+sm_hook_kernel(const int* a, const int* b, int n, int Gp, int* parent, int* changed) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        int u = a[i], v = Gp + b[i];
+        while (parent[u] != u) u = parent[u];
+        while (parent[v] != v) v = parent[v];
+        if (u == v) continue;
+        // another hook of the same root may win the race: the round is
+        // repeated while any edge still joins two roots
+        atomicMin(&parent[max(u, v)], min(u, v));
+        *changed = 1;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) sm_compress_kernel(int* parent, int m) {
+    for (int x = blockIdx.x * kBlock + threadIdx.x; x < m; x += gridDim.x * kBlock) {
+        int r = x;
+        while (parent[r] != r) r = parent[r];
+        parent[x] = r;
+    }
+}
+
+// the pinned level of each component: the smallest block level in it
+__global__ void __launch_bounds__(kBlock) sm_pin_min_kernel(const int* parent, int Gp, int Gb, int* pin_of_root) {
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < Gb; g += gridDim.x * kBlock)
+        atomicMin(&pin_of_root[parent[Gp + g]], g);
+}
+
+__global__ void __launch_bounds__(kBlock)
+sm_pin_flag_kernel(const int* parent, const int* pin_of_root, const int* cnt, int Gp, int Gb, int* flag) {
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < Gb; g += gridDim.x * kBlock)
+        flag[g] = cnt[g] > 0 && pin_of_root[parent[Gp + g]] == g ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock)
+sm_pin_list_kernel(const int* flag, const int* pos, int Gb, int off, int base, int* list) {
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < Gb; g += gridDim.x * kBlock)
+        if (flag[g]) list[base + pos[g]] = off + g;
+}
+
+// q (and b, minv) at the pinned unknowns: zero in every lane
+template <typename T>
+__global__ void __launch_bounds__(kBlock) sm_pin_zero_kernel(const int* list, int np, int L, T* q) {
+    const long long total = static_cast<long long>(np) * L;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock)
+        q[static_cast<size_t>(list[j / L]) * L + j % L] = T(0);
+}
+
+__global__ void __launch_bounds__(kBlock) sm_pin_minv_kernel(const int* list, int np, float* minv) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < np; j += gridDim.x * kBlock) minv[list[j]] = 0.0f;
+}
+
+// ---- per-group block preconditioner: the unknowns whose levels nest in one
+// level of the slope's grouping (a firm's year cells and its slope) form a
+// dense block of S, assembled exactly in FP64 and inverted
+
+constexpr int kBlkMax = 64;  // largest block inverted; larger groups stay on Jacobi
+
+// block of each unknown: the grouping level of one of its rows (every row of
+// a nested level has the same); unknowns without rows keep -1
+__global__ void __launch_bounds__(kBlock)
+sm_blk_of_kernel(const int* gid_r, const int* gid_b, int n, int off, int* blk) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) blk[off + gid_r[i]] = gid_b[i];
+}
+
+// sort keys: the block, or nb (past the last block) for unknowns outside the
+// blocks (no rows, no Jacobi weight: pinned or without coupling)
+__global__ void __launch_bounds__(kBlock)
+sm_blk_key_kernel(const int* blk, const float* minv, int N, int nb, int* key, int* val) {
+    for (int u = blockIdx.x * kBlock + threadIdx.x; u < N; u += gridDim.x * kBlock) {
+        key[u] = (blk[u] >= 0 && minv[u] != 0.0f) ? blk[u] : nb;
+        val[u] = u;
+    }
+}
+
+// per sorted position: the unknown's local index inside its block
+__global__ void __launch_bounds__(kBlock)
+sm_blk_local_kernel(const int* skey, const int* sval, const int* bptr, int N, int nb, int* loc) {
+    for (int q = blockIdx.x * kBlock + threadIdx.x; q < N; q += gridDim.x * kBlock) {
+        const int b = skey[q];
+        loc[sval[q]] = b < nb ? q - bptr[b] : -1;
+    }
+}
+
+// matrix offsets: k^2 for blocks of 2..kBlkMax unknowns, 0 otherwise
+__global__ void __launch_bounds__(kBlock) sm_blk_sq_kernel(const int* bcnt, int nb, int* sq) {
+    for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock) {
+        const int k = bcnt[b];
+        sq[b] = (k >= 2 && k <= kBlkMax) ? k * k : 0;
+    }
+}
+
+struct BlkArgs {
+    const int* pofs = nullptr;
+    int Np = 0, NR = 0;
+    const int* gid[kSmMaxR] = {};
+    const double* mult[kSmMaxR] = {};
+    int off[kSmMaxR] = {};
+    const int* blk = nullptr;    // block of each unknown
+    const int* loc = nullptr;    // local index (-1: outside the blocks)
+    const int* bcnt = nullptr;   // unknowns per block
+    const int* moff = nullptr;   // matrix offset per block
+    double* mat = nullptr;
+};
+
+// S restricted to the blocks: per p level (thread), the rows' own terms
+// m_u m_v and the elimination term -a_u a_v / n (a = sums over the p level's
+// rows of m per unknown), for pairs of unknowns of one block
+__global__ void __launch_bounds__(kBlock) sm_blk_assemble_kernel(BlkArgs a) {
+    constexpr int kMaxE = 96;
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < a.Np; w += gridDim.x * kBlock) {
+        const int i0 = a.pofs[w], i1 = a.pofs[w + 1];
+        if (i1 <= i0) continue;
+        const double inv_n = 1.0 / static_cast<double>(i1 - i0);
+        int eu[kMaxE];
+        double ea[kMaxE];
+        int ne = 0;
+        bool overflow = false;
+        for (int i = i0; i < i1; ++i) {
+            // the row's own terms (its unknowns: one per remaining block)
+            int ru[kSmMaxR];
+            double rm[kSmMaxR];
+            int rn = 0;
+            for (int r = 0; r < a.NR; ++r) {
+                const int u = a.off[r] + a.gid[r][i];
+                if (a.loc[u] < 0) continue;
+                ru[rn] = u;
+                rm[rn] = a.mult[r] ? a.mult[r][i] : 1.0;
+                ++rn;
+                // accumulate a_u
+                int k = 0;
+                while (k < ne && eu[k] != u) ++k;
+                if (k == ne) {
+                    if (ne == kMaxE) {
+                        overflow = true;
+                        continue;
+                    }
+                    eu[ne] = u;
+                    ea[ne] = 0.0;
+                    ++ne;
+                }
+                ea[k] += rm[rn - 1];
+            }
+            for (int x = 0; x < rn; ++x)
+                for (int y = 0; y < rn; ++y) {
+                    if (a.blk[ru[x]] != a.blk[ru[y]]) continue;
+                    const int b = a.blk[ru[x]];
+                    const int k = a.bcnt[b];
+                    if (k < 2 || k > kBlkMax) continue;
+                    atomicAdd(&a.mat[a.moff[b] + a.loc[ru[x]] * k + a.loc[ru[y]]], rm[x] * rm[y]);
+                }
+        }
+        if (overflow) continue;  // a p level touching more unknowns than kMaxE: its correction is skipped
+        for (int x = 0; x < ne; ++x)
+            for (int y = 0; y < ne; ++y) {
+                if (a.blk[eu[x]] != a.blk[eu[y]]) continue;
+                const int b = a.blk[eu[x]];
+                const int k = a.bcnt[b];
+                if (k < 2 || k > kBlkMax) continue;
+                atomicAdd(&a.mat[a.moff[b] + a.loc[eu[x]] * k + a.loc[eu[y]]], -ea[x] * ea[y] * inv_n);
+            }
+    }
+}
+
+// Gauss-Jordan inverse of each block (symmetric positive semi-definite: no
+// pivoting), one wave per block in LDS.  A pivot at or below kBlkDrop of the
+// largest diagonal is a zero: the leading unknowns up to it carry a null
+// vector of S (a slope collinear with its firm's cells), so the unknown is
+// left out of the block (its row and column of the inverse zero: the solve
+// keeps it at zero, which leaves the attainable residual unchanged).  A pivot
+// between that and kBlkFail marks the block too ill-conditioned for an FP32
+// inverse: Jacobi keeps its unknowns.
+constexpr double kBlkDrop = 1e-10;
+constexpr double kBlkFail = 1e-6;
+__global__ void __launch_bounds__(64)
+sm_blk_invert_kernel(const int* bcnt, const int* moff, int nb, const double* mat, float* inv, int* ok, double fail) {
+    __shared__ double A[kBlkMax][kBlkMax + 1];
+    __shared__ double Y[kBlkMax][kBlkMax + 1];
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int k = bcnt[b];
+        if (k < 2 || k > kBlkMax) {
+            if (t == 0) ok[b] = 0;
+            continue;
+        }
+        const double* M = mat + moff[b];
+        for (int e = t; e < k * k; e += 64) {
+            A[e / k][e % k] = M[e];
+            Y[e / k][e % k] = (e / k == e % k) ? 1.0 : 0.0;
+        }
+        if (t == 0) bad = 0;
+        __syncthreads();
+        double dmax = 0.0;
+        for (int j = 0; j < k; ++j) dmax = fmax(dmax, fabs(A[j][j]));
+        for (int c = 0; c < k; ++c) {
+            const double piv = A[c][c];
+            if (!(piv > kBlkDrop * dmax)) {
+                __syncthreads();
+                if (t < k) {
+                    A[c][t] = 0.0;
+                    A[t][c] = 0.0;
+                    Y[c][t] = 0.0;
+                }
+                __syncthreads();
+                continue;
+            }
+            if (!(piv > fail * dmax)) {
+                if (t == 0) bad = 1;
+                break;
+            }
+            __syncthreads();
+            // scale the pivot row
+            if (t < k) {
+                A[c][t] /= piv;
+                Y[c][t] /= piv;
+            }
+            __syncthreads();
+            // eliminate the other rows (thread t owns row t)
+            if (t < k && t != c) {
+                const double f = A[t][c];
+                for (int j = 0; j < k; ++j) {
+                    A[t][j] -= f * A[c][j];
+                    Y[t][j] -= f * Y[c][j];
+                }
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        const int good = bad == 0;
+        if (good)
+            for (int e = t; e < k * k; e += 64) inv[moff[b] + e] = static_cast<float>(Y[e / k][e % k]);
+        if (t == 0) ok[b] = good;
+        __syncthreads();
+    }
+}
+
+// z = B^-1 r on the blocks (one thread per block and lane); the block's
+// unknowns are sval[bptr[b] .. bptr[b] + k)
+__global__ void __launch_bounds__(kBlock)
+sm_blk_apply_kernel(const int* bptr, const int* bcnt, const int* moff, const int* ok, const int* sval, int nb, int L,
+                    int C, const float* inv, const float* r, float* z) {
+    const long long total = static_cast<long long>(nb) * L;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int b = static_cast<int>(j / L), l = static_cast<int>(j % L);
+        if (l >= C || !ok[b]) continue;
+        const int k = bcnt[b];
+        const int* u = sval + bptr[b];
+        const float* Bi = inv + moff[b];
+        float rv[kBlkMax];
+#pragma unroll
+        for (int e = 0; e < kBlkMax; ++e) rv[e] = e < k ? r[static_cast<size_t>(u[e]) * L + l] : 0.0f;
+        for (int x = 0; x < k; ++x) {
+            const float* row = Bi + x * k;
+            float acc = 0.0f;
+#pragma unroll
+            for (int e = 0; e < kBlkMax; ++e)
+                if (e < k) acc += row[e] * rv[e];
+            z[static_cast<size_t>(u[x]) * L + l] = acc;
+        }
+    }
+}
+
 int sm_grid(long long work) {
     const long long g = (work + kBlock - 1) / kBlock;
     return static_cast<int>(std::max(1LL, std::min(g, 16384LL)));
@@ -309,7 +595,14 @@ struct SlopeMixed final : WideOp {
     int n = 0, C = 0, Np = 0, NR = 0, N = 0, ld = 0;
     std::vector<RDim> rd;
     std::vector<Grouping> gr;
-    DevBuf<int> pperm, pkey, pcnt, pofs, tmp_int, flag;
+    DevBuf<int> pperm, pkey, pcnt, pofs, tmp_int, flag, uf, pin_root, pin_flag, pin_pos, pins;
+    int npins = 0;
+    // block preconditioner
+    bool blk_on = false;
+    int nblk = 0;
+    DevBuf<int> blk, bkey, bval, skey, sval, bcnt, bptr, bsq, moff, loc, bok;
+    DevBuf<double> bmat;
+    DevBuf<float> binv;
     DevBuf<unsigned char> tmp;
     DevBuf<double> Vs, z64[kSmMaxR], R64, part64, d1, d2, dl1, dl2, ssq;
     DevBuf<float> z32[kSmMaxR], R32, part32, minv;
@@ -350,12 +643,35 @@ struct SlopeMixed final : WideOp {
         }
     }
 
+    bool has_blocks() const override { return blk_on; }
+
+    void precondition(const float* r, float* z) override {
+        hipLaunchKernelGGL(sm_blk_apply_kernel, dim3(sm_grid(static_cast<long long>(nblk) * wide.L)), dim3(kBlock), 0,
+                           s, bptr.p, bcnt.p, moff.p, bok.p, sval.p, nblk, wide.L, C, binv.p, r, z);
+        XFE_CHECK(hipGetLastError());
+    }
+
     void apply(const float* p, float* q) override {
         SmArgs<float> a = args<float>(p, R32.p);
         a.sign = 1.0f;
+        const int slot = profile_begin(s);
         hipLaunchKernelGGL((sm_apply_kernel<float, false, false>), dim3(gridw), dim3(kBlock), 0, s, a);
         XFE_CHECK(hipGetLastError());
+        // per row: the ids and slope values of the remaining blocks, one C-float
+        // coefficient row per block, the C-float row of R; the offsets of p
+        int nslope = 0;
+        for (int r = 0; r < NR; ++r) nslope += rd[r].slope ? 1 : 0;
+        profile_end(slot, s, static_cast<double>(n) * (4.0 * NR + 4.0 * nslope + 4.0 * C * (NR + 1)) + 4.0 * Np);
         reduce<float>(R32.p, part32.p, q, kCols, C);
+        pin_zero(q);
+    }
+
+    template <typename T>
+    void pin_zero(T* q) {
+        if (npins <= 0) return;
+        hipLaunchKernelGGL((sm_pin_zero_kernel<T>), dim3(sm_grid(static_cast<long long>(npins) * wide.L)),
+                           dim3(kBlock), 0, s, pins.p, npins, wide.L, q);
+        XFE_CHECK(hipGetLastError());
     }
 
     void apply(const double* p, double* q) override {
@@ -364,13 +680,18 @@ struct SlopeMixed final : WideOp {
         hipLaunchKernelGGL((sm_apply_kernel<double, false, false>), dim3(gridw), dim3(kBlock), 0, s, a);
         XFE_CHECK(hipGetLastError());
         reduce<double>(R64.p, part64.p, q, kCols, C);
+        pin_zero(q);
     }
 
     void release() {
         wide.release();
         for (auto& g : gr)
             for (DevBuf<int>* b : {&g.gid, &g.perm, &g.cnt, &g.ptr, &g.nch, &g.cptr, &g.cstart, &g.cend}) b->release();
-        for (DevBuf<int>* b : {&pperm, &pkey, &pcnt, &pofs, &tmp_int, &flag}) b->release();
+        for (DevBuf<int>* b : {&pperm, &pkey, &pcnt, &pofs, &tmp_int, &flag, &uf, &pin_root, &pin_flag, &pin_pos, &pins,
+                               &blk, &bkey, &bval, &skey, &sval, &bcnt, &bptr, &bsq, &moff, &loc, &bok})
+            b->release();
+        bmat.release();
+        binv.release();
         tmp.release();
         for (DevBuf<double>* b : {&Vs, &R64, &part64, &d1, &d2, &dl1, &dl2, &ssq}) b->release();
         for (int r = 0; r < kSmMaxR; ++r) {
@@ -475,12 +796,40 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
     sm.n = n;
     sm.C = C;
     const int D = static_cast<int>(G.size());
-    // p: the standard dimension with the most levels
-    int p = -1;
-    for (int d = 0; d < D; ++d)
-        if (mode[d] == 0 && (p < 0 || G[d] > G[p])) p = d;
-    sm.Np = G[p];
     const int grid = sm_grid(n);
+    // nested: every level of `fine` inside one level of `coarse`
+    auto nested_in = [&](int fine, int coarse) {
+        sm.tmp_int.ensure(std::max(n, 2 * G[fine]));
+        int* lo = sm.tmp_int.p;
+        int* hi = sm.tmp_int.p + G[fine];
+        hipLaunchKernelGGL(sm_fill_int_kernel, dim3(sm_grid(G[fine])), dim3(kBlock), 0, s, lo, G[fine], INT_MAX);
+        hipLaunchKernelGGL(sm_fill_int_kernel, dim3(sm_grid(G[fine])), dim3(kBlock), 0, s, hi, G[fine], -1);
+        hipLaunchKernelGGL(sm_nest_kernel, dim3(grid), dim3(kBlock), 0, s, gid[fine], gid[coarse], n, lo, hi);
+        sm.flag.ensure(1);
+        XFE_CHECK(hipMemsetAsync(sm.flag.p, 0, sizeof(int), s));
+        hipLaunchKernelGGL(sm_nest_flag_kernel, dim3(sm_grid(G[fine])), dim3(kBlock), 0, s, lo, hi, G[fine],
+                           sm.flag.p);
+        int bad = 1;
+        XFE_CHECK(hipMemcpyAsync(&bad, sm.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        return bad == 0;
+    };
+    // p: the standard dimension with the most levels among those that do not
+    // nest inside a slope's grouping (firm x year cells inside firms): with
+    // such a dimension eliminated the slope varies inside p's levels and the
+    // FP32 recurrence stalls; if every standard dimension nests, the one with
+    // the most levels
+    int p = -1, p_any = -1;
+    for (int d = 0; d < D; ++d) {
+        if (mode[d] != 0) continue;
+        if (p_any < 0 || G[d] > G[p_any]) p_any = d;
+        bool in_slope = false;
+        for (int e = 0; e < D && !in_slope; ++e)
+            if (mode[e] != 0 && G[e] <= G[d] && nested_in(d, e)) in_slope = true;
+        if (!in_slope && (p < 0 || G[d] > G[p])) p = d;
+    }
+    if (p < 0) p = p_any;
+    sm.Np = G[p];
     // rows sorted by p (stable)
     sm.pkey.ensure(n); sm.pperm.ensure(n); sm.tmp_int.ensure(n);
     hipLaunchKernelGGL(sm_iota_kernel, dim3(grid), dim3(kBlock), 0, s, sm.tmp_int.p, n);
@@ -524,24 +873,8 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
         XFE_CHECK(hipGetLastError());
         gidx[d] = ng++;
     }
-    // a slope's intercept is left out when a finer standard dimension (or p)
-    // nests in the slope's grouping
-    auto nested_in = [&](int fine, int coarse) {
-        sm.tmp_int.ensure(std::max(n, 2 * G[fine]));
-        int* lo = sm.tmp_int.p;
-        int* hi = sm.tmp_int.p + G[fine];
-        hipLaunchKernelGGL(sm_fill_int_kernel, dim3(sm_grid(G[fine])), dim3(kBlock), 0, s, lo, G[fine], INT_MAX);
-        hipLaunchKernelGGL(sm_fill_int_kernel, dim3(sm_grid(G[fine])), dim3(kBlock), 0, s, hi, G[fine], -1);
-        hipLaunchKernelGGL(sm_nest_kernel, dim3(grid), dim3(kBlock), 0, s, gid[fine], gid[coarse], n, lo, hi);
-        sm.flag.ensure(1);
-        XFE_CHECK(hipMemsetAsync(sm.flag.p, 0, sizeof(int), s));
-        hipLaunchKernelGGL(sm_nest_flag_kernel, dim3(sm_grid(G[fine])), dim3(kBlock), 0, s, lo, hi, G[fine],
-                           sm.flag.p);
-        int bad = 1;
-        XFE_CHECK(hipMemcpyAsync(&bad, sm.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        XFE_CHECK(hipStreamSynchronize(s));
-        return bad == 0;
-    };
+    // a slope's intercept is left out when a finer standard dimension nests in
+    // the slope's grouping
     sm.rd.clear();
     sm.dropped.clear();
     int N = 0;
@@ -609,6 +942,116 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
     }
     hipLaunchKernelGGL(sm_minv_kernel, dim3(sm_grid(N)), dim3(kBlock), 0, s, sm.dl1.p, sm.dl2.p, N, sm.minv.p);
     XFE_CHECK(hipGetLastError());
+    // ground: a constant over a connected component of (p, a standard block)
+    // lies in p's span, so S is singular along it; one level per component of
+    // each standard block is pinned to zero (its row and column of S dropped)
+    sm.npins = 0;
+    {
+        std::vector<int> counts;
+        int total_pins = 0;
+        sm.pins.ensure(static_cast<size_t>(N) + 1);
+        for (int r = 0; r < sm.NR; ++r) {
+            if (sm.rd[r].slope) continue;
+            Grouping& g = sm.gr[sm.rd[r].grouping];
+            const int Gb = g.G, Gp = sm.Np, m = Gp + Gb;
+            sm.uf.ensure(m);
+            hipLaunchKernelGGL(sm_iota_kernel, dim3(sm_grid(m)), dim3(kBlock), 0, s, sm.uf.p, m);
+            sm.flag.ensure(1);
+            for (int round = 0; round < 1024; ++round) {
+                XFE_CHECK(hipMemsetAsync(sm.flag.p, 0, sizeof(int), s));
+                hipLaunchKernelGGL(sm_hook_kernel, dim3(grid), dim3(kBlock), 0, s, gid[p], gid[sm.rd[r].d], n, Gp,
+                                   sm.uf.p, sm.flag.p);
+                hipLaunchKernelGGL(sm_compress_kernel, dim3(sm_grid(m)), dim3(kBlock), 0, s, sm.uf.p, m);
+                int changed = 0;
+                XFE_CHECK(hipMemcpyAsync(&changed, sm.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                XFE_CHECK(hipStreamSynchronize(s));
+                if (!changed) break;
+                if (round == 1023) throw Error("slope_mixed: components did not settle");
+            }
+            sm.pin_root.ensure(m);
+            sm.pin_flag.ensure(static_cast<size_t>(Gb) + 1);
+            sm.pin_pos.ensure(static_cast<size_t>(Gb) + 1);
+            hipLaunchKernelGGL(sm_fill_int_kernel, dim3(sm_grid(m)), dim3(kBlock), 0, s, sm.pin_root.p, m, INT_MAX);
+            hipLaunchKernelGGL(sm_pin_min_kernel, dim3(sm_grid(Gb)), dim3(kBlock), 0, s, sm.uf.p, Gp, Gb,
+                               sm.pin_root.p);
+            hipLaunchKernelGGL(sm_pin_flag_kernel, dim3(sm_grid(Gb)), dim3(kBlock), 0, s, sm.uf.p, sm.pin_root.p,
+                               g.cnt.p, Gp, Gb, sm.pin_flag.p);
+            XFE_CHECK(hipMemsetAsync(sm.pin_flag.p + Gb, 0, sizeof(int), s));
+            excl_scan(sm.tmp, sm.pin_flag.p, sm.pin_pos.p, Gb + 1, s);
+            int np = 0;
+            XFE_CHECK(hipMemcpyAsync(&np, sm.pin_pos.p + Gb, sizeof(int), hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+            hipLaunchKernelGGL(sm_pin_list_kernel, dim3(sm_grid(Gb)), dim3(kBlock), 0, s, sm.pin_flag.p,
+                               sm.pin_pos.p, Gb, sm.rd[r].off, total_pins, sm.pins.p);
+            XFE_CHECK(hipGetLastError());
+            total_pins += np;
+        }
+        sm.npins = total_pins;
+        if (total_pins > 0)
+            hipLaunchKernelGGL(sm_pin_minv_kernel, dim3(sm_grid(total_pins)), dim3(kBlock), 0, s, sm.pins.p,
+                               total_pins, sm.minv.p);
+        XFE_CHECK(hipGetLastError());
+        out.pinned = total_pins;
+    }
+    // blocks: the unknowns nesting in one level of a slope's grouping (its
+    // slope, the cells of a firm) form one exactly inverted block of S
+    sm.blk_on = false;
+    sm.nblk = 0;
+    int ds = -1;
+    for (int r = 0; r < sm.NR && ds < 0; ++r)
+        if (sm.rd[r].slope) ds = sm.rd[r].d;
+    if (ds >= 0 && !std::getenv("XFE_SLOPE_JACOBI")) {
+        std::vector<int> members;
+        for (int r = 0; r < sm.NR; ++r) {
+            const int d = sm.rd[r].d;
+            if (d == ds || (G[d] >= G[ds] && nested_in(d, ds))) members.push_back(r);
+        }
+        if (members.size() >= 2) {
+            const int nb = G[ds];
+            sm.blk.ensure(N);
+            hipLaunchKernelGGL(sm_fill_int_kernel, dim3(sm_grid(N)), dim3(kBlock), 0, s, sm.blk.p, N, -1);
+            for (int r : members)
+                hipLaunchKernelGGL(sm_blk_of_kernel, dim3(grid), dim3(kBlock), 0, s, gid[sm.rd[r].d], gid[ds], n,
+                                   sm.rd[r].off, sm.blk.p);
+            sm.bkey.ensure(N); sm.bval.ensure(N); sm.skey.ensure(N); sm.sval.ensure(N); sm.loc.ensure(N);
+            hipLaunchKernelGGL(sm_blk_key_kernel, dim3(sm_grid(N)), dim3(kBlock), 0, s, sm.blk.p, sm.minv.p, N, nb,
+                               sm.bkey.p, sm.bval.p);
+            XFE_CHECK(hipGetLastError());
+            radix_pairs<int, int>(sm.tmp, sm.bkey.p, sm.skey.p, sm.bval.p, sm.sval.p, N, bits_of(nb + 1), s);
+            offsets(sm.tmp, sm.skey.p, N, nb + 1, sm.bcnt, sm.bptr, s);
+            hipLaunchKernelGGL(sm_blk_local_kernel, dim3(sm_grid(N)), dim3(kBlock), 0, s, sm.skey.p, sm.sval.p,
+                               sm.bptr.p, N, nb, sm.loc.p);
+            sm.bsq.ensure(static_cast<size_t>(nb) + 1);
+            sm.moff.ensure(static_cast<size_t>(nb) + 1);
+            hipLaunchKernelGGL(sm_blk_sq_kernel, dim3(sm_grid(nb)), dim3(kBlock), 0, s, sm.bcnt.p, nb, sm.bsq.p);
+            XFE_CHECK(hipMemsetAsync(sm.bsq.p + nb, 0, sizeof(int), s));
+            XFE_CHECK(hipGetLastError());
+            excl_scan(sm.tmp, sm.bsq.p, sm.moff.p, nb + 1, s);
+            int total = 0;
+            XFE_CHECK(hipMemcpyAsync(&total, sm.moff.p + nb, sizeof(int), hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+            if (total > 0) {
+                sm.bmat.ensure(total);
+                sm.binv.ensure(total);
+                sm.bok.ensure(nb);
+                XFE_CHECK(hipMemsetAsync(sm.bmat.p, 0, sizeof(double) * total, s));
+                BlkArgs a;
+                a.pofs = sm.pofs.p; a.Np = sm.Np; a.NR = sm.NR;
+                for (int r = 0; r < sm.NR; ++r) {
+                    a.gid[r] = sm.gr[sm.rd[r].grouping].gid.p;
+                    a.mult[r] = sm.rd[r].slope ? sm.z64[r].p : nullptr;
+                    a.off[r] = sm.rd[r].off;
+                }
+                a.blk = sm.blk.p; a.loc = sm.loc.p; a.bcnt = sm.bcnt.p; a.moff = sm.moff.p; a.mat = sm.bmat.p;
+                hipLaunchKernelGGL(sm_blk_assemble_kernel, dim3(sm_grid(sm.Np)), dim3(kBlock), 0, s, a);
+                hipLaunchKernelGGL(sm_blk_invert_kernel, dim3(std::min(nb, 8192)), dim3(64), 0, s, sm.bcnt.p,
+                                   sm.moff.p, nb, sm.bmat.p, sm.binv.p, sm.bok.p, kBlkFail);
+                XFE_CHECK(hipGetLastError());
+                sm.blk_on = true;
+                sm.nblk = nb;
+            }
+        }
+    }
     // the wide solver on this operator; b = D_R' M_p v in FP64
     sm.wide.set_operator(&sm, N, s, sm.minv.p);
     sm.wide.ensure(C);
@@ -620,8 +1063,11 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
         hipLaunchKernelGGL(sm_zero_b_kernel, dim3(sm_grid(static_cast<long long>(N) * sm.wide.L)), dim3(kBlock), 0, s,
                            sm.wide.b64.p, sm.minv.p, N, sm.wide.L);
         XFE_CHECK(hipGetLastError());
+        sm.pin_zero(sm.wide.b64.p);
     }
-    sm.wide.solve_device(C, tol, max_iter);
+    sm.wide.trace = std::getenv("XFE_SLOPE_TRACE") ? "slope mixed" : nullptr;
+    sm.wide.restart_each_replacement = true;
+    sm.wide.solve_device(C, tol, max_iter, std::max(tol, 1e-9));
     sm.wide.unscale_x64();
     out.iterations = 0;
     out.converged = true;
@@ -633,8 +1079,10 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
     out.passes = sm.wide.last_passes;
     out.replacements = sm.wide.last_updates;
     if (std::getenv("XFE_SLOPE_TRACE")) {
-        std::fprintf(stderr, "[slope mixed] N %d NR %d p %d dropped %d passes %d replacements %d\n", N, sm.NR, p,
-                     out.intercepts_dropped, out.passes, out.replacements);
+        std::fprintf(stderr,
+                     "[slope mixed] N %d NR %d p %d dropped %d pinned %d blocks %d passes %d replacements %d\n", N,
+                     sm.NR, p, out.intercepts_dropped, out.pinned, sm.blk_on ? sm.nblk : 0, out.passes,
+                     out.replacements);
         for (int r = 0; r < sm.NR; ++r)
             std::fprintf(stderr, "[slope mixed]   block %d: dim %d slope %d levels %d off %d chunks %d\n", r,
                          sm.rd[r].d, sm.rd[r].slope ? 1 : 0, sm.rd[r].G, sm.rd[r].off,
