@@ -441,7 +441,8 @@ def bench_slope(args):
     del V0, Vd, ids, zd
     torch.cuda.empty_cache()
     # equal-size comparison with the FP64 slope path (host entry points)
-    Vs, gs, Gs, zs, _ = slope_case(0.03)
+    cmp_scale = args.cmp_scale
+    Vs, gs, Gs, zs, _ = slope_case(cmp_scale)
     t0 = time.perf_counter()
     ym, Xm, im = xfe_amd.absorb(Vs[:, 0], Vs[:, 1:], gs, Gs, tol=1e-8, precision=1, slopes={1: (zs, True)})
     mixed_small = time.perf_counter() - t0
@@ -453,22 +454,29 @@ def bench_slope(args):
     colscale = np.abs(Vs).max(axis=0)
     err_m = float((np.abs(np.column_stack([ym, Xm]) - ref).max(axis=0) / colscale).max())
     err_6 = float((np.abs(np.column_stack([y6, X6]) - ref).max(axis=0) / colscale).max())
-    log(f"3% shape ({Vs.shape[0]} obs): mixed {mixed_small:.2f} s ({im['iterations']} it), FP64 path "
+    log(f"{cmp_scale:g} shape ({Vs.shape[0]} obs): mixed {mixed_small:.2f} s ({im['iterations']} it, "
+        f"precision_used {im['precision_used']}), FP64 path "
         f"{fp64_small:.2f} s ({i6['iterations']} it); max column error vs FP64 tol 1e-13: mixed {err_m:.2e}, "
         f"FP64 tol 1e-8 {err_6:.2e}")
     out = {
         "metric": SLOPE_METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "method": ("worker dimension eliminated; FP32 Jacobi-PCG on the Schur system of the firm x year cells and "
-                   "firm slopes (the firm intercept lies in the cells' span), 32 columns as lanes, FP64 residual "
-                   "replacement, FP64 final transform; columns resident in HBM"),
+        "method": ("worker dimension eliminated; FP32 PCG on the Schur system of the firm x year cells and "
+                   "firm slopes (the firm intercept lies in the cells' span) with one exactly inverted block per "
+                   "firm (its cells and slope), 32 columns as lanes, restarted FP64 residual replacement, FP64 "
+                   "final transform; a run not converged in 5000 FP32 iterations falls back to the FP64 slope "
+                   "path on the untouched columns; columns resident in HBM"),
+        "path": ("mixed" if info["precision_used"] == 1 else
+                 "FP64 slope path (fallback: the mixed run did not converge)" if info["precision_fallback"]
+                 else "FP64 slope path"),
         "config": {"workload": f"slope_{scale:g}x30m_32cols", "obs": int(n), "columns": C, "levels": G,
                    "slope": "firm intercept + slope on x1", "tol": 1e-8, "singletons_dropped": int(dropped)},
         "iterations": info["iterations"], "replacements": info["refinements"], "converged": info["converged"],
         "certificate_rel": info["abs_residual_rel"], "certified": info["precision_certified"],
         "obs_cols_per_s": round(n * C / sec, 1),
-        "equal_size_3pct": {"obs": int(Vs.shape[0]), "mixed_s": round(mixed_small, 3),
+        "equal_size_small": {"scale": cmp_scale, "obs": int(Vs.shape[0]), "mixed_s": round(mixed_small, 3),
+                             "mixed_precision_used": im["precision_used"],
                             "fp64_slope_path_s": round(fp64_small, 3), "mixed_iterations": im["iterations"],
                             "fp64_iterations": i6["iterations"], "mixed_max_col_err_vs_fp64_tight": err_m,
                             "fp64_tol_1e8_max_col_err_vs_fp64_tight": err_6,
@@ -657,6 +665,8 @@ def main():
                     help="precision 1: 0 Jacobi, 1 strength aggregates, 2 learned aggregates (default)")
     ap.add_argument("--ablation-steps", type=int, default=2,
                     help="timed steps of the mixed method with the other preconditioners")
+    ap.add_argument("--cmp-scale", type=float, default=0.03,
+                    help="--path slope: scale of the mixed vs FP64 slope path comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--path", choices=("absorb", "akm", "boot", "slope", "recover", "gi", "leaveout"),
                     default="absorb",

@@ -231,7 +231,10 @@ int xfe_absorb_device(double* d_V, int n, int C, int ld,
  * opt->precision 1 an unweighted problem with at most 32 columns and a
  * standard dimension runs the mixed-precision Schur PCG (res->precision_used
  * 1), otherwise the FP64 slope path (precision_fallback 1 when precision 1
- * was asked).  The certificate (opt->certify) includes the slope moments.
+ * was asked); a mixed run that does not converge within 5000 FP32 iterations
+ * leaves the columns untouched and the FP64 slope path absorbs them
+ * (precision_fallback 1).  The certificate (opt->certify) includes the slope
+ * moments.
  * Ordering as xfe_absorb_device.  Returns 1 on success, 0 on failure. */
 int xfe_absorb_device_slopes(double* d_V, int n, int C, int ld, const int* d_group_ids, const int* num_groups,
                              int num_fes, const double* const* d_slope_values, const int* slope_mode,

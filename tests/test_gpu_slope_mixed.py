@@ -103,3 +103,29 @@ def test_slope_mixed_device_entry_matches_host_entry():
     yw, Xw, iw = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, weights=np.ones(n), tol=1e-8, precision=1,
                                 slopes={1: (p["z"], True)})
     assert iw["precision_used"] == 0 and iw["precision_fallback"]
+
+
+def test_slope_mixed_falls_back_to_fp64_path(monkeypatch):
+    """A mixed run that stops unconverged leaves the columns to the FP64 slope
+    path, which absorbs them and says so (precision_used 0, precision_fallback).
+    The run is made to stop by the Jacobi preconditioner in place of the firm
+    blocks (XFE_SLOPE_JACOBI), which needs several times the 1,500 FP32
+    iterations allowed here on this shape, where the FP64 path needs about 360."""
+    _log("mixed slopes, fallback to the FP64 path")
+    p, gids, G = _c2_panel(0.003, 7)
+    V = np.column_stack([p["y"], p["X"]])
+    z = p["z"]
+    monkeypatch.setenv("XFE_SLOPE_JACOBI", "1")
+    ym, Xm, im = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=1, slopes={1: (z, True)},
+                                max_iter=1500)
+    monkeypatch.delenv("XFE_SLOPE_JACOBI")
+    y6, X6, i6 = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=0, slopes={1: (z, True)},
+                                max_iter=1500)
+    _log(f"  fallback: converged {im['converged']} precision_used {im['precision_used']} "
+         f"fallback {im['precision_fallback']} iterations {im['iterations']} (FP64 path {i6['iterations']})")
+    assert i6["converged"]
+    assert im["converged"] and im["precision_used"] == 0 and im["precision_fallback"], im
+    # the same FP64 path both times; its level sums use atomics, so two runs
+    # agree to the tolerance, not bit for bit
+    Vm, V6 = np.column_stack([ym, Xm]), np.column_stack([y6, X6])
+    assert np.abs(Vm - V6).max() <= 1e-8 * np.abs(V).max()

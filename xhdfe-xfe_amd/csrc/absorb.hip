@@ -2320,6 +2320,7 @@ static void absorb_slopes_impl(double* V_in, int n, int C, int ld, const std::ve
     wdot(r, r, rr.data());
     bool converged = false;
     int it = 0;
+    const bool trace = std::getenv("XFE_SLOPE_TRACE") != nullptr;  // progress lines on stderr
     while (true) {
         bool done = true;
         for (int c = 0; c < C; ++c)
@@ -2341,6 +2342,12 @@ static void absorb_slopes_impl(double* V_in, int n, int C, int ld, const std::ve
         upload(beta);
         XFE_CHECK(launch_col_axpy(pv, r, pv, ws.lanes.p, n, C, ld, 1.0, grid, s));
         rr = rr_new;
+        if (trace && it % 100 == 0) {
+            double worst = 0.0;
+            for (int c = 0; c < C; ++c)
+                if (limit[c] > 0.0) worst = std::max(worst, std::sqrt(std::max(0.0, rr[c])) / limit[c]);
+            std::fprintf(stderr, "[slope fp64] iteration %d: worst |r| / limit %.3e\n", it, worst);
+        }
     }
     XFE_CHECK(launch_sub(V, V, y, n, C, ld, grid, s));
     if (perm) XFE_CHECK(launch_scatter_cols(V, ld, perm, V_in, ld, n, C, grid, s));
@@ -2505,7 +2512,12 @@ static void gi_colour(const xfe_group_individual& gi, std::vector<int>& members,
 // src/fe_absorption.cpp:14942-15100: per dimension the level sums of w v~ and,
 // for a slope, of w z v~; ||D'W||_F^2 = sum over dimensions and moments of
 // sum_i (w_i m_i)^2).  V: n x C (ld), device, transformed in place.
-static void absorb_slopes_mixed(double* V, int n, int C, int ld, const std::vector<const int*>& gid,
+// FP32 iterations the mixed slope path gets before the FP64 path takes over
+constexpr int kSlopeMixedMaxIter = 5000;
+
+// false when a column did not converge: V is then untouched and the caller
+// runs the FP64 slope path on it
+static bool absorb_slopes_mixed(double* V, int n, int C, int ld, const std::vector<const int*>& gid,
                                 const std::vector<int>& G, const std::vector<const double*>& z,
                                 const std::vector<int>& mode, const xfe_options& opt, xfe_absorb_result& res) {
     Workspace& ws = tls_ws;
@@ -2532,7 +2544,8 @@ static void absorb_slopes_mixed(double* V, int n, int C, int ld, const std::vect
     SlopeMixedResult out;
     const auto t0 = std::chrono::steady_clock::now();
     slope_mixed_run(*ws.slope_mixed, V, n, C, ld, gid, G, z, mode, target,
-                    std::max(10, std::min(opt.max_iter, 20000)), s, out);
+                    std::max(10, std::min(opt.max_iter, kSlopeMixedMaxIter)), s, out);
+    if (!out.converged) return false;
     res.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     res.iterations = out.iterations;
     res.refinements = out.replacements;
@@ -2546,7 +2559,7 @@ static void absorb_slopes_mixed(double* V, int n, int C, int ld, const std::vect
         res.fe_levels[d] = G[d];
         res.sweep_order_used[d] = d;
     }
-    if (!opt.certify) return;
+    if (!opt.certify) return true;
     // the dropped intercepts' level sums, and ||D||_F^2
     std::vector<double> resid = out.resid_sq;
     double op = static_cast<double>(n);  // the eliminated dimension
@@ -2585,6 +2598,7 @@ static void absorb_slopes_mixed(double* V, int n, int C, int ld, const std::vect
     res.abs_residual_rel = rmax;
     res.precision_certified = (res.converged && rmax <= limit) ? 1 : 0;
     res.certificate_computed = 1;
+    return true;
 }
 
 static bool method_ok(int method, const char* fn) {
@@ -2707,8 +2721,8 @@ int xfe_absorb_device_slopes(double* d_V, int n, int C, int ld, const int* d_gro
             order[d] = sweep_order ? sweep_order[d] : d;
             res->fe_levels[d] = G[d];
         }
-        if (opt->precision == 1 && slope_mixed_eligible(C, num_fes, mode.data(), d_weights)) {
-            absorb_slopes_mixed(d_V, n, C, ld, gids, G, z, mode, *opt, *res);
+        if (opt->precision == 1 && slope_mixed_eligible(C, num_fes, mode.data(), d_weights) &&
+            absorb_slopes_mixed(d_V, n, C, ld, gids, G, z, mode, *opt, *res)) {
         } else {
             absorb_slopes_impl(d_V, n, C, ld, gids, G, z, mode, d_weights, order, *opt, *res);
             res->precision_fallback = opt->precision == 1 ? 1 : 0;
@@ -2793,8 +2807,8 @@ int xfe_absorb(const double* y, const double* X, int n, int k, const xfe_fe_inpu
                 z[d] = ws.stage_z[d].p;
                 mode[d] = fes[d].slope_has_intercept ? 1 : 2;
             }
-            if (opt->precision == 1 && slope_mixed_eligible(C, num_fes, mode.data(), weights)) {
-                absorb_slopes_mixed(ws.stage.p, n, C, n, gids, G, z, mode, *opt, *res);
+            if (opt->precision == 1 && slope_mixed_eligible(C, num_fes, mode.data(), weights) &&
+                absorb_slopes_mixed(ws.stage.p, n, C, n, gids, G, z, mode, *opt, *res)) {
             } else {
                 absorb_slopes_impl(ws.stage.p, n, C, n, gids, G, z, mode, weights ? ws.stage_w.p : nullptr,
                                    order, *opt, *res);

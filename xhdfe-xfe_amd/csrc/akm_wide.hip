@@ -684,7 +684,7 @@ void AkmWide::build(const AkmWideInput& in, hipStream_t stream, bool use_blocks)
                 pw.push_back(h_w64[e]);
             }
         }
-        dex[ground] = 0.0;
+        if (ground >= 0) dex[ground] = 0.0;
         if (agg.set_pairs(J, pf, pg, pw, dex, s)) {
             agg.aggregate(agg.st.p, kWideBlockCap, kWideBlockRounds, s);
             agg.build_blocks(s);

@@ -39,9 +39,10 @@ void slope_mixed_delete(SlopeMixed* sm);
 bool slope_mixed_eligible(int C, int D, const int* mode, const double* w);
 
 // V: n x C column-major (ld), device, the caller's row order on entry and on
-// exit (transformed in place); gid / z: device arrays of n.  tol: the target
-// of the Schur residual |b - S x| / |b| per column; a column whose FP64
-// residual stalls above it is accepted at 1e-9 (the absorber's mixed rule).
+// exit (transformed in place when every column converged, untouched
+// otherwise); gid / z: device arrays of n.  tol: the target of the Schur
+// residual |b - S x| / |b| per column; a column whose FP64 residual stalls
+// above it is accepted at 1e-9 (the absorber's mixed rule).
 void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std::vector<const int*>& gid,
                      const std::vector<int>& G, const std::vector<const double*>& z, const std::vector<int>& mode,
                      double tol, int max_iter, hipStream_t s, SlopeMixedResult& out);

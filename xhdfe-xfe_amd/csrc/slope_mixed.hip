@@ -134,6 +134,9 @@ __global__ void __launch_bounds__(kBlock) sm_apply_kernel(SmArgs<T> a) {
 }
 
 // part[k][col] = sum over chunk k's rows (level order through perm) of m R[row][col]
+// (accumulated in FP64 for either storage type: a slope's level sums run over
+// every row of a firm, and an FP32 sum of that many terms of both signs loses
+// the digits the solve needs)
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 sm_chunk_kernel(const int* __restrict__ perm, const int* __restrict__ cstart, int nchunks, const int* __restrict__ cend,
@@ -141,13 +144,13 @@ sm_chunk_kernel(const int* __restrict__ perm, const int* __restrict__ cstart, in
     const int col = threadIdx.x % kCols, slot = threadIdx.x / kCols;
     if (col >= C) return;
     for (int k = blockIdx.x * kSlotsPerBlock + slot; k < nchunks; k += gridDim.x * kSlotsPerBlock) {
-        T acc = T(0);
+        double acc = 0.0;
         for (int q = cstart[k]; q < cend[k]; ++q) {
             const int i = perm[q];
-            const T m = mult ? mult[i] : T(1);
-            acc += m * R[static_cast<size_t>(i) * stride + col];
+            const double m = mult ? static_cast<double>(mult[i]) : 1.0;
+            acc += m * static_cast<double>(R[static_cast<size_t>(i) * stride + col]);
         }
-        part[static_cast<size_t>(k) * kCols + col] = acc;
+        part[static_cast<size_t>(k) * kCols + col] = static_cast<T>(acc);
     }
 }
 
@@ -160,10 +163,10 @@ sm_level_kernel(const int* __restrict__ cptr, const T* __restrict__ part, int G,
     for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
          j += static_cast<long long>(gridDim.x) * kBlock) {
         const int g = static_cast<int>(j / L), l = static_cast<int>(j % L);
-        T acc = T(0);
+        double acc = 0.0;
         if (l < C)
-            for (int k = cptr[g]; k < cptr[g + 1]; ++k) acc += part[static_cast<size_t>(k) * kCols + l];
-        out[static_cast<size_t>(off + g) * L + l] = acc;
+            for (int k = cptr[g]; k < cptr[g + 1]; ++k) acc += static_cast<double>(part[static_cast<size_t>(k) * kCols + l]);
+        out[static_cast<size_t>(off + g) * L + l] = static_cast<T>(acc);
     }
 }
 
@@ -1000,6 +1003,8 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
     int ds = -1;
     for (int r = 0; r < sm.NR && ds < 0; ++r)
         if (sm.rd[r].slope) ds = sm.rd[r].d;
+    // (XFE_SLOPE_JACOBI set: Jacobi only, a diagnostic and the fallback test's
+    // way to make a run stop unconverged)
     if (ds >= 0 && !std::getenv("XFE_SLOPE_JACOBI")) {
         std::vector<int> members;
         for (int r = 0; r < sm.NR; ++r) {
@@ -1089,6 +1094,15 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
                          sm.gr[sm.rd[r].grouping].nchunks);
         for (int c = 0; c < C; ++c) std::fprintf(stderr, "[slope mixed]   lane %d: %s\n", c, sm.wide.lane_report(c).c_str());
     }
+    out.unknowns = N;
+    out.primary = p;
+    // a lane that did not converge leaves the caller's columns untouched (the
+    // caller falls back to the FP64 path on them)
+    if (!out.converged) {
+        out.resid_sq.assign(C, 0.0);
+        XFE_CHECK(hipStreamSynchronize(s));
+        return;
+    }
     // the transform v - D_R beta - (its p means), in FP64, into Vs and R64;
     // the certificate's level sums: p from the pass, the others from R64
     const int gw = sm.gridw;
@@ -1122,8 +1136,6 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
                        sm.Vs.p, sm.ld, sm.pperm.p, n, C, V, ld);
     XFE_CHECK(hipGetLastError());
     XFE_CHECK(hipStreamSynchronize(s));
-    out.unknowns = N;
-    out.primary = p;
 }
 
 }  // namespace xfe
