@@ -264,7 +264,7 @@ def bench_akm(args):
         "jla_pii_200_draws_s": round(jla_s, 3), "jla_bii_200_draws_s": round(bii_s, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(os.path.join(ROOT, "profiles", "wide_firm_kss10m.md")),
+                     "traffic": measured_traffic(os.path.join(ROOT, "profiles", "wide_pair_kss10m.md")),
                      "kernel": "wide_pair_kernel<float, 4> (firm-major pass over the firm pairs, 256 lanes)",
                      "avg_launch_ms": round(pass_ms / max(launches, 1), 4),
                      "bytes_per_launch": round(pass_bytes / max(launches, 1), 1)},
