@@ -470,7 +470,8 @@ int xfe_akm_max_lanes(void);
 /* Wide batched solve: no counterpart in the reference, whose caller
  * (src/akm_kss.cpp:1659-1707) loops over akm_cuda_solve_S_multi in batches
  * of akm_cuda_max_lanes() = 32.  Up to xfe_akm_wide_max_lanes() (256) lanes
- * per operator pass, FP32 Jacobi-PCG whose residual is replaced by the FP64
+ * per operator pass, FP32 PCG (aggregate blocks of strongly coupled firms
+ * solved exactly, Jacobi elsewhere) whose residual is replaced by the FP64
  * residual b - S x (x accumulated in FP64) whenever r.r has dropped 1e4-fold;
  * a lane stops when its FP64 residual meets |r|^2 <= tol^2 |rhs|^2.  Same
  * pack layout, ground pin and per-lane results as xfe_akm_solve_S_multi
