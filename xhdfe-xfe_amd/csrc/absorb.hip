@@ -146,7 +146,13 @@ struct Workspace {
         int p = -1, NR = 0, C = 0;
         int rdim[2] = {-1, -1};
         int off[2] = {0, 0};
+        // the certificate's per-column sums of squared level sums of the final
+        // transform, taken from the transform pass itself (valid for cert_V)
+        bool cert_ready = false;
+        const double* cert_V = nullptr;
+        std::vector<double> cert_sq;
     } fast_rec;
+    DevBuf<double> cert_psum;  // per level of p: sums of the final transform's rows
     double* host_scalars = nullptr;  // pinned
 
     // Frees every device resource of this thread's workspace.
@@ -185,6 +191,8 @@ struct Workspace {
         agg.release();
         gi_iptr.release(); gi_igroup.release(); gi_members.release(); gi_scale.release(); gi_denom.release();
         fast_rec.used = false;
+        fast_rec.cert_ready = false;
+        cert_psum.release();
         if (host_scalars) (void)hipHostFree(host_scalars);
         if (stream) (void)hipStreamDestroy(stream);
         host_scalars = nullptr; stream = nullptr; device = -1;
@@ -1278,7 +1286,7 @@ struct FastSolver {
 
     template <typename T>
     void schur_pass(const T* coef, T* out, const T* V, T* dst, T sign, const T* inv_p,
-                    T* const* spell_sums, size_t lds_extra, double* partials) {
+                    T* const* spell_sums, size_t lds_extra, double* partials, T* p_sums = nullptr) {
         XFE_CHECK(hipMemsetAsync(out, 0, sizeof(T) * static_cast<size_t>(N) * C, s));
         for (int ch = 0; ch < sv.nchunks; ++ch) {
             const int c0 = ch * 8;
@@ -1300,6 +1308,7 @@ struct FastSolver {
                 a.lds_off[k] = lds_off[k] / 8 * cc;  // element offset for this chunk width
             }
             a.partials = partials ? partials + static_cast<size_t>(ch) * sv.tile_grid : nullptr;
+            a.p_sums = p_sums;
             // LDS offsets were laid out for full 8-wide chunks; recompute for cc
             int lo = 0;
             for (int k = 0; k < NR; ++k) {
@@ -1344,12 +1353,13 @@ struct FastSolver {
         schur_pass<float>(x, q, nullptr, nullptr, 1.0f, ws.f_invp.p, sp, lds_extra32, nullptr);
     }
 
-    // b = D_R' M_p (V - D_R beta) in FP64; with dst, also writes M_p (V - D_R beta).
-    double residual64(double* V, double* dst) {
+    // b = D_R' M_p (V - D_R beta) in FP64; with dst, also writes M_p (V - D_R beta)
+    // and, with p_sums, that block's sums over the levels of p.
+    double residual64(double* V, double* dst, double* p_sums = nullptr) {
         double* sp[2] = {nullptr, nullptr};
         for (int k = 0; k < NR; ++k) if (mode[k] == kRedSpell) sp[k] = ws.sp_sums[rdim[k]].p;
         schur_pass<double>(ws.f_beta.p, ws.f_b.p, V, dst, -1.0, ws.inv[p].p, sp, lds_extra64,
-                           dst ? ws.partials.p : nullptr);
+                           dst ? ws.partials.p : nullptr, p_sums);
         double sq = 0.0;
         if (dst) sv.fetch(ws.partials.p, sv.tile_grid * sv.nchunks, 1, 0, &sq);
         return sq;
@@ -1574,9 +1584,35 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
     (void)max_inner;
     tls_phase.mark("PCG");
     if (fs.probe) fs.record_probe();
-    fs.residual64(V, V);  // final within transform, in place
+    // final within transform, in place; its pass also leaves D_R' v~ in f_b and,
+    // for the certificate, the sums of v~ over the levels of p
+    double* psum = nullptr;
+    if (opt.certify) {
+        sv.ws.cert_psum.ensure(static_cast<size_t>(sv.G[fs.p]) * C);
+        XFE_CHECK(hipMemsetAsync(sv.ws.cert_psum.p, 0, sizeof(double) * static_cast<size_t>(sv.G[fs.p]) * C, sv.s));
+        psum = sv.ws.cert_psum.p;
+    }
+    fs.residual64(V, V, psum);
     tls_phase.mark("final transform");
     auto& rec = sv.ws.fast_rec;
+    rec.cert_ready = false;
+    if (psum) {
+        // the certificate's squared level sums (unweighted: the mixed path takes
+        // no weights): every level of the R dimensions from f_b, of p from psum
+        rec.cert_sq.assign(C, 0.0);
+        std::vector<double> tmp(C, 0.0);
+        auto add = [&](const double* t, long long total) {
+            const int g = cg_grid(total, C, kMaxGrid);
+            sv.ws.partials.ensure(static_cast<size_t>(g) * C);
+            XFE_CHECK(launch_table_sumsq(t, total, C, sv.ws.partials.p, g, sv.s));
+            sv.fetch(sv.ws.partials.p, g, C, 0, tmp.data());
+            for (int c = 0; c < C; ++c) rec.cert_sq[c] += tmp[c];
+        };
+        add(sv.ws.f_b.p, static_cast<long long>(fs.N) * C);
+        add(psum, static_cast<long long>(sv.G[fs.p]) * C);
+        rec.cert_V = V;
+        rec.cert_ready = true;
+    }
     rec.used = true;
     rec.p = fs.p;
     rec.NR = fs.NR;
@@ -1932,7 +1968,11 @@ static void certificate(Solver& sv, const double* Vt, const std::vector<double>&
     const int C = sv.C;
     const bool prof = g_profile.enabled;
     g_profile.enabled = false;  // the profile covers the solver's passes only
-    const std::vector<double> resid = normal_sq(sv, Vt);
+    // the mixed path's transform pass already summed its output over every
+    // level of every dimension; otherwise one reduce pass per dimension
+    const auto& rec = ws.fast_rec;
+    const std::vector<double> resid =
+        (rec.used && rec.cert_ready && rec.cert_V == Vt && !sv.w) ? rec.cert_sq : normal_sq(sv, Vt);
     g_profile.enabled = prof;
     double wsq = static_cast<double>(sv.n);
     if (sv.w) {
@@ -2063,6 +2103,7 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
     Workspace& ws = tls_ws;
     ws.init();
     ws.fast_rec.used = false;
+    ws.fast_rec.cert_ready = false;
     hipStream_t s = ws.stream;
     const int D = static_cast<int>(G.size());
     const auto t0 = std::chrono::steady_clock::now();

@@ -117,6 +117,7 @@ struct SchurArgs {
     T* lds_target[2] = {nullptr, nullptr};
     int lds_off[2] = {0, 0};
     double* partials = nullptr;               // STORE: per-block sum of squares
+    T* p_sums = nullptr;                      // STORE: per level of p, the sums of the stored rows (G_p x C)
 };
 
 // Arguments of the segment form of the Schur operator (seg_kernels.hip).

@@ -149,6 +149,35 @@ __global__ void __launch_bounds__(kBlock) schur_kernel(SchurArgs<T> a) {
                     q += static_cast<double>(u[j][c]) * static_cast<double>(u[j][c]);
                 }
             }
+            if (a.p_sums) {
+                // the sums of the stored rows over each level of p (every level
+                // lies inside one tile, so each is complete here and written
+                // with plain stores by its run heads)
+                for (int k = threadIdx.x; k < kTileRows * CC; k += kBlock) sm[k] = T(0);
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < kSlots; ++j) {
+                    const int rank = tile_rank2(cnt, j * (kBlock / kWave) + wave, mask[j]);
+                    const int end = run_end_lane(kp[j]);
+                    const bool seg = run_head(kp[j]) && valid[j];
+#pragma unroll
+                    for (int c = 0; c < CC; ++c) {
+                        const T s = run_sum_t(u[j][c], end);
+                        if (seg) atomicAdd(&sm[rank * CC + c], s);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < kSlots; ++j) {
+                    const bool seg = run_head(kp[j]) && valid[j];
+                    const int rank = tile_rank2(cnt, j * (kBlock / kWave) + wave, mask[j]);
+                    if (!seg) continue;
+                    T* ps = a.p_sums + static_cast<size_t>(kp[j]) * a.C + a.c0;
+#pragma unroll
+                    for (int c = 0; c < CC; ++c) ps[c] = sm[rank * CC + c];
+                }
+                __syncthreads();
+            }
         }
         // ---- reduce into the levels of each R dimension
 #pragma unroll
