@@ -231,9 +231,10 @@ int xfe_absorb_device(double* d_V, int n, int C, int ld,
  * opt->precision 1 an unweighted problem with at most 32 columns and a
  * standard dimension runs the mixed-precision Schur PCG (res->precision_used
  * 1), otherwise the FP64 slope path (precision_fallback 1 when precision 1
- * was asked); a mixed run that does not converge within 5000 FP32 iterations
- * leaves the columns untouched and the FP64 slope path absorbs them
- * (precision_fallback 1).  The certificate (opt->certify) includes the slope
+ * was asked); columns the mixed run leaves unconverged after 2000 FP32
+ * iterations continue with up to 10000 FP64 block-PCG iterations from their
+ * iterate, and only if that fails are the (untouched) columns absorbed by the
+ * FP64 slope path (precision_fallback 1).  The certificate (opt->certify) includes the slope
  * moments.
  * Ordering as xfe_absorb_device.  Returns 1 on success, 0 on failure. */
 int xfe_absorb_device_slopes(double* d_V, int n, int C, int ld, const int* d_group_ids, const int* num_groups,

@@ -105,13 +105,13 @@ def test_slope_mixed_device_entry_matches_host_entry():
     assert iw["precision_used"] == 0 and iw["precision_fallback"]
 
 
-def test_slope_mixed_falls_back_to_fp64_path(monkeypatch):
-    """A mixed run that stops unconverged leaves the columns to the FP64 slope
-    path, which absorbs them and says so (precision_used 0, precision_fallback).
-    The run is made to stop by the Jacobi preconditioner in place of the firm
-    blocks (XFE_SLOPE_JACOBI), which needs several times the 1,500 FP32
-    iterations allowed here on this shape, where the FP64 path needs about 360."""
-    _log("mixed slopes, fallback to the FP64 path")
+def test_slope_mixed_fp64_continuation(monkeypatch):
+    """Columns the FP32 runs leave unconverged continue with FP64 block-PCG
+    from their iterate: with the Jacobi preconditioner in place of the firm
+    blocks (XFE_SLOPE_JACOBI) the FP32 runs need several times the 1,500
+    iterations allowed here, the FP64 continuation finishes, and the result is
+    certified and equal to the FP64 slope path run tight."""
+    _log("mixed slopes, FP64 continuation")
     p, gids, G = _c2_panel(0.003, 7)
     V = np.column_stack([p["y"], p["X"]])
     z = p["z"]
@@ -119,6 +119,33 @@ def test_slope_mixed_falls_back_to_fp64_path(monkeypatch):
     ym, Xm, im = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=1, slopes={1: (z, True)},
                                 max_iter=1500)
     monkeypatch.delenv("XFE_SLOPE_JACOBI")
+    y6, X6, i6 = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-13, precision=0, slopes={1: (z, True)})
+    _log(f"  continuation: converged {im['converged']} precision_used {im['precision_used']} "
+         f"iterations {im['iterations']} sweeps {im['sweeps']}")
+    assert im["converged"] and im["precision_used"] == 1 and im["precision_certified"], im
+    assert im["iterations"] > 1500  # the FP32 runs stopped at 1,500; the rest is the FP64 continuation
+    Vm, V6 = np.column_stack([ym, Xm]), np.column_stack([y6, X6])
+    err = np.abs(Vm - V6).max(axis=0) / np.abs(V).max(axis=0)
+    assert err.max() <= 1e-8
+
+
+def test_slope_mixed_falls_back_to_fp64_path(monkeypatch):
+    """A mixed run that stops unconverged leaves the columns to the FP64 slope
+    path, which absorbs them and says so (precision_used 0, precision_fallback).
+    The run is made to stop by the Jacobi preconditioner in place of the firm
+    blocks (XFE_SLOPE_JACOBI), which needs several times the 1,500 FP32
+    iterations allowed here on this shape, and no FP64 continuation
+    (XFE_SLOPE_NO_CONTINUATION); the FP64 path needs about 360."""
+    _log("mixed slopes, fallback to the FP64 path")
+    p, gids, G = _c2_panel(0.003, 7)
+    V = np.column_stack([p["y"], p["X"]])
+    z = p["z"]
+    monkeypatch.setenv("XFE_SLOPE_JACOBI", "1")
+    monkeypatch.setenv("XFE_SLOPE_NO_CONTINUATION", "1")
+    ym, Xm, im = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=1, slopes={1: (z, True)},
+                                max_iter=1500)
+    monkeypatch.delenv("XFE_SLOPE_JACOBI")
+    monkeypatch.delenv("XFE_SLOPE_NO_CONTINUATION")
     y6, X6, i6 = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=0, slopes={1: (z, True)},
                                 max_iter=1500)
     _log(f"  fallback: converged {im['converged']} precision_used {im['precision_used']} "

@@ -2553,8 +2553,11 @@ static void gi_colour(const xfe_group_individual& gi, std::vector<int>& members,
 // src/fe_absorption.cpp:14942-15100: per dimension the level sums of w v~ and,
 // for a slope, of w z v~; ||D'W||_F^2 = sum over dimensions and moments of
 // sum_i (w_i m_i)^2).  V: n x C (ld), device, transformed in place.
-// FP32 iterations the mixed slope path gets before the FP64 path takes over
-constexpr int kSlopeMixedMaxIter = 5000;
+// FP32 iterations the mixed slope path gets; the columns they leave
+// unconverged continue with up to kSlopeFp64MaxIter FP64 block-PCG
+// iterations, and only what that leaves goes to the FP64 slope path
+constexpr int kSlopeMixedMaxIter = 2000;
+constexpr int kSlopeFp64MaxIter = 10000;
 
 // false when a column did not converge: V is then untouched and the caller
 // runs the FP64 slope path on it
@@ -2585,12 +2588,13 @@ static bool absorb_slopes_mixed(double* V, int n, int C, int ld, const std::vect
     SlopeMixedResult out;
     const auto t0 = std::chrono::steady_clock::now();
     slope_mixed_run(*ws.slope_mixed, V, n, C, ld, gid, G, z, mode, target,
-                    std::max(10, std::min(opt.max_iter, kSlopeMixedMaxIter)), s, out);
+                    std::max(10, std::min(opt.max_iter, kSlopeMixedMaxIter)),
+                    std::max(10, std::min(opt.max_iter, kSlopeFp64MaxIter)), s, out);
     if (!out.converged) return false;
     res.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    res.iterations = out.iterations;
+    res.iterations = out.iterations;  // FP32 and FP64 continuation iterations of the slowest column
     res.refinements = out.replacements;
-    res.sweeps = out.passes;
+    res.sweeps = out.passes + out.fp64_iterations;
     res.converged = out.converged ? 1 : 0;
     res.used_cg = 1;
     res.used_accel = 0;

@@ -123,6 +123,10 @@ struct AkmWide {
     void unscale_x64();
     // iterations of lane l in the last solve, -1 when it did not converge
     int lane_iterations(int l) const;
+    // after a caller's own continuation of the last solve on x64 (scaled
+    // units): lane l (< nb) converged there when ok[l], with extra[l] more
+    // iterations (unscale_x64 then keeps its solution)
+    void mark_converged(int nb, const std::vector<char>& ok, const std::vector<int>& extra);
     // lane l's final scalars as text (diagnostics)
     std::string lane_report(int l) const;
 

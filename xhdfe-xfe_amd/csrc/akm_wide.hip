@@ -749,6 +749,17 @@ int AkmWide::lane_iterations(int l) const {
     return h_sc[kSOk * kWideMaxLanes + l] != 0.0 ? static_cast<int>(h_sc[kSIters * kWideMaxLanes + l]) : -1;
 }
 
+void AkmWide::mark_converged(int nb, const std::vector<char>& ok, const std::vector<int>& extra) {
+    for (int l = 0; l < nb && l < kWideMaxLanes; ++l) {
+        if (!ok[l]) continue;
+        h_sc[kSOk * kWideMaxLanes + l] = 1.0;
+        h_sc[kSIters * kWideMaxLanes + l] += extra[l];
+    }
+    XFE_CHECK(hipMemcpyAsync(sc.p + kSOk * kWideMaxLanes, h_sc + kSOk * kWideMaxLanes, sizeof(double) * kWideMaxLanes,
+                             hipMemcpyHostToDevice, s));
+    XFE_CHECK(hipStreamSynchronize(s));
+}
+
 void AkmWide::release() {
     agg.release();
     for (DevBuf<int>* d : {&fkey, &fnb, &fwk, &has, &bk[0], &bk[1]}) d->release();

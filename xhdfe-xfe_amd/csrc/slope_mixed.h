@@ -19,6 +19,7 @@ struct SlopeMixedResult {
     int iterations = 0;          // FP32 PCG iterations (max over columns)
     int passes = 0;              // FP32 operator passes
     int replacements = 0;        // FP64 residual replacements
+    int fp64_iterations = 0;     // FP64 block-PCG iterations after the FP32 runs
     int unknowns = 0;            // levels and slopes of the Schur system
     int primary = -1;            // the eliminated dimension
     int intercepts_dropped = 0;  // slope intercepts spanned by a finer dimension
@@ -42,10 +43,12 @@ bool slope_mixed_eligible(int C, int D, const int* mode, const double* w);
 // exit (transformed in place when every column converged, untouched
 // otherwise); gid / z: device arrays of n.  tol: the target of the Schur
 // residual |b - S x| / |b| per column; a column whose FP64 residual stalls
-// above it is accepted at 1e-9 (the absorber's mixed rule).
+// above it is accepted at 1e-9 (the absorber's mixed rule).  max_iter bounds
+// the FP32 iterations; the columns they leave unconverged continue with up to
+// fp64_max_iter FP64 block-PCG iterations from their iterate.
 void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std::vector<const int*>& gid,
                      const std::vector<int>& G, const std::vector<const double*>& z, const std::vector<int>& mode,
-                     double tol, int max_iter, hipStream_t s, SlopeMixedResult& out);
+                     double tol, int max_iter, int fp64_max_iter, hipStream_t s, SlopeMixedResult& out);
 
 // whether the last run left the intercept of dimension d out of the solve
 bool slope_mixed_intercept_dropped(const SlopeMixed& sm, int d);

@@ -478,7 +478,8 @@ __global__ void __launch_bounds__(kBlock) sm_blk_assemble_kernel(BlkArgs a) {
 constexpr double kBlkDrop = 1e-10;
 constexpr double kBlkFail = 1e-6;
 __global__ void __launch_bounds__(64)
-sm_blk_invert_kernel(const int* bcnt, const int* moff, int nb, const double* mat, float* inv, int* ok, double fail) {
+sm_blk_invert_kernel(const int* bcnt, const int* moff, int nb, const double* mat, float* inv, double* inv64, int* ok,
+                     double fail) {
     __shared__ double A[kBlkMax][kBlkMax + 1];
     __shared__ double Y[kBlkMax][kBlkMax + 1];
     __shared__ int bad;
@@ -534,7 +535,10 @@ sm_blk_invert_kernel(const int* bcnt, const int* moff, int nb, const double* mat
         __syncthreads();
         const int good = bad == 0;
         if (good)
-            for (int e = t; e < k * k; e += 64) inv[moff[b] + e] = static_cast<float>(Y[e / k][e % k]);
+            for (int e = t; e < k * k; e += 64) {
+                inv[moff[b] + e] = static_cast<float>(Y[e / k][e % k]);
+                inv64[moff[b] + e] = Y[e / k][e % k];
+            }
         if (t == 0) ok[b] = good;
         __syncthreads();
     }
@@ -542,9 +546,10 @@ sm_blk_invert_kernel(const int* bcnt, const int* moff, int nb, const double* mat
 
 // z = B^-1 r on the blocks (one thread per block and lane); the block's
 // unknowns are sval[bptr[b] .. bptr[b] + k)
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
 sm_blk_apply_kernel(const int* bptr, const int* bcnt, const int* moff, const int* ok, const int* sval, int nb, int L,
-                    int C, const float* inv, const float* r, float* z) {
+                    int C, const T* inv, const T* r, T* z) {
     const long long total = static_cast<long long>(nb) * L;
     for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
          j += static_cast<long long>(gridDim.x) * kBlock) {
@@ -552,19 +557,78 @@ sm_blk_apply_kernel(const int* bptr, const int* bcnt, const int* moff, const int
         if (l >= C || !ok[b]) continue;
         const int k = bcnt[b];
         const int* u = sval + bptr[b];
-        const float* Bi = inv + moff[b];
-        float rv[kBlkMax];
+        const T* Bi = inv + moff[b];
+        T rv[kBlkMax];
 #pragma unroll
-        for (int e = 0; e < kBlkMax; ++e) rv[e] = e < k ? r[static_cast<size_t>(u[e]) * L + l] : 0.0f;
+        for (int e = 0; e < kBlkMax; ++e) rv[e] = e < k ? r[static_cast<size_t>(u[e]) * L + l] : T(0);
         for (int x = 0; x < k; ++x) {
-            const float* row = Bi + x * k;
-            float acc = 0.0f;
+            const T* row = Bi + x * k;
+            T acc = T(0);
 #pragma unroll
             for (int e = 0; e < kBlkMax; ++e)
                 if (e < k) acc += row[e] * rv[e];
             z[static_cast<size_t>(u[x]) * L + l] = acc;
         }
     }
+}
+
+// ---- FP64 block-PCG continuation (level-major N x L vectors, lanes < C)
+
+// part[b * L + l] = sum over block b's rows of a[u, l] b[u, l] (kBlock % L == 0)
+__global__ void __launch_bounds__(kBlock)
+sm_dot64_kernel(const double* a, const double* b, int N, int L, double* part) {
+    __shared__ double red[kBlock];
+    const int l = threadIdx.x % L, rows = kBlock / L;
+    double acc = 0.0;
+    for (long long u = static_cast<long long>(blockIdx.x) * rows + threadIdx.x / L; u < N;
+         u += static_cast<long long>(gridDim.x) * rows) {
+        const size_t i = static_cast<size_t>(u) * L + l;
+        acc += a[i] * b[i];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < L) {
+        double t = 0.0;
+        for (int k = 0; k < rows; ++k) t += red[k * L + threadIdx.x];
+        part[static_cast<size_t>(blockIdx.x) * L + threadIdx.x] = t;
+    }
+}
+
+// x += alpha p, r -= alpha q (per lane alpha)
+__global__ void __launch_bounds__(kBlock)
+sm_update64_kernel(double* x, double* r, const double* p, const double* q, const double* alpha, long long total,
+                   int L) {
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * kBlock) {
+        const double a = alpha[i % L];
+        if (a == 0.0) continue;
+        x[i] += a * p[i];
+        r[i] -= a * q[i];
+    }
+}
+
+// z = minv r (Jacobi; the block rows are overwritten by the block solve)
+__global__ void __launch_bounds__(kBlock)
+sm_jacobi64_kernel(const double* r, const float* minv, double* z, long long total, int L) {
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * kBlock)
+        z[i] = static_cast<double>(minv[i / L]) * r[i];
+}
+
+// p = z + beta p (per lane beta)
+__global__ void __launch_bounds__(kBlock)
+sm_dir64_kernel(double* p, const double* z, const double* beta, long long total, int L) {
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * kBlock)
+        p[i] = z[i] + beta[i % L] * p[i];
+}
+
+// r = b - q
+__global__ void __launch_bounds__(kBlock)
+sm_res64_kernel(double* r, const double* b, const double* q, long long total) {
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * kBlock)
+        r[i] = b[i] - q[i];
 }
 
 int sm_grid(long long work) {
@@ -604,8 +668,10 @@ struct SlopeMixed final : WideOp {
     bool blk_on = false;
     int nblk = 0;
     DevBuf<int> blk, bkey, bval, skey, sval, bcnt, bptr, bsq, moff, loc, bok;
-    DevBuf<double> bmat;
+    DevBuf<double> bmat, binv64;
     DevBuf<float> binv;
+    // FP64 continuation vectors (N x L) and per-lane scalars
+    DevBuf<double> c_r, c_z, c_p, c_q, c_part, c_scal;
     DevBuf<unsigned char> tmp;
     DevBuf<double> Vs, z64[kSmMaxR], R64, part64, d1, d2, dl1, dl2, ssq;
     DevBuf<float> z32[kSmMaxR], R32, part32, minv;
@@ -649,8 +715,8 @@ struct SlopeMixed final : WideOp {
     bool has_blocks() const override { return blk_on; }
 
     void precondition(const float* r, float* z) override {
-        hipLaunchKernelGGL(sm_blk_apply_kernel, dim3(sm_grid(static_cast<long long>(nblk) * wide.L)), dim3(kBlock), 0,
-                           s, bptr.p, bcnt.p, moff.p, bok.p, sval.p, nblk, wide.L, C, binv.p, r, z);
+        hipLaunchKernelGGL((sm_blk_apply_kernel<float>), dim3(sm_grid(static_cast<long long>(nblk) * wide.L)),
+                           dim3(kBlock), 0, s, bptr.p, bcnt.p, moff.p, bok.p, sval.p, nblk, wide.L, C, binv.p, r, z);
         XFE_CHECK(hipGetLastError());
     }
 
@@ -686,6 +752,119 @@ struct SlopeMixed final : WideOp {
         pin_zero(q);
     }
 
+    // Per-lane sums of a . b over the N levels (lanes < C).
+    void dots(const double* a, const double* b, std::vector<double>& out) {
+        const int L = wide.L;
+        const int g = std::min(1024, std::max(1, (N + kBlock / L - 1) / (kBlock / L)));
+        c_part.ensure(static_cast<size_t>(g) * L);
+        hipLaunchKernelGGL(sm_dot64_kernel, dim3(g), dim3(kBlock), 0, s, a, b, N, L, c_part.p);
+        XFE_CHECK(hipGetLastError());
+        std::vector<double> h(static_cast<size_t>(g) * L);
+        XFE_CHECK(hipMemcpyAsync(h.data(), c_part.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
+        out.assign(L, 0.0);
+        for (int b2 = 0; b2 < g; ++b2)
+            for (int l = 0; l < L; ++l) out[l] += h[static_cast<size_t>(b2) * L + l];
+    }
+
+    void precondition64(const double* r, double* z) {
+        const long long total = static_cast<long long>(N) * wide.L;
+        hipLaunchKernelGGL(sm_jacobi64_kernel, dim3(sm_grid(total)), dim3(kBlock), 0, s, r, minv.p, z, total, wide.L);
+        if (blk_on)
+            hipLaunchKernelGGL((sm_blk_apply_kernel<double>), dim3(sm_grid(static_cast<long long>(nblk) * wide.L)),
+                               dim3(kBlock), 0, s, bptr.p, bcnt.p, moff.p, bok.p, sval.p, nblk, wide.L, C,
+                               binv64.p, r, z);
+        XFE_CHECK(hipGetLastError());
+    }
+
+    // FP64 PCG (Jacobi + the firm blocks) on the wide solver's scaled system
+    // (b64 of unit norm per lane) from its iterate x64, for the lanes the FP32
+    // runs left unconverged; the residual is recomputed from x every 128
+    // iterations.  A lane converges at |b - S x| <= tol, or at the end within
+    // accept.  Marks the converged lanes in the wide solver; returns the
+    // iterations run.
+    int fp64_continue(double tol, double accept, int max_iter) {
+        const int L = wide.L;
+        const long long total = static_cast<long long>(N) * L;
+        for (DevBuf<double>* b : {&c_r, &c_z, &c_p, &c_q}) b->ensure(static_cast<size_t>(total));
+        c_scal.ensure(2 * static_cast<size_t>(L));
+        std::vector<char> act(L, 0), ok(L, 0);
+        std::vector<int> extra(L, 0);
+        int nact = 0;
+        for (int c = 0; c < C; ++c)
+            if (wide.lane_iterations(c) < 0) {
+                act[c] = 1;
+                ++nact;
+            }
+        if (nact == 0) return 0;
+        auto residual = [&]() {
+            apply(wide.x64.p, c_q.p);
+            hipLaunchKernelGGL(sm_res64_kernel, dim3(sm_grid(total)), dim3(kBlock), 0, s, c_r.p, wide.b64.p, c_q.p,
+                               total);
+            XFE_CHECK(hipGetLastError());
+        };
+        const bool trace = std::getenv("XFE_SLOPE_TRACE") != nullptr;
+        residual();
+        precondition64(c_r.p, c_z.p);
+        XFE_CHECK(hipMemcpyAsync(c_p.p, c_z.p, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+        std::vector<double> rz, rr, pq, coef(2 * static_cast<size_t>(L), 0.0);
+        dots(c_r.p, c_z.p, rz);
+        dots(c_r.p, c_r.p, rr);
+        int it = 0;
+        for (it = 1; it <= max_iter && nact > 0; ++it) {
+            apply(c_p.p, c_q.p);
+            dots(c_p.p, c_q.p, pq);
+            for (int l = 0; l < L; ++l) {
+                double a = 0.0;
+                if (act[l] && pq[l] > 0.0 && rz[l] > 0.0) a = rz[l] / pq[l];
+                coef[l] = a;
+            }
+            XFE_CHECK(hipMemcpyAsync(c_scal.p, coef.data(), sizeof(double) * L, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(sm_update64_kernel, dim3(sm_grid(total)), dim3(kBlock), 0, s, wide.x64.p, c_r.p, c_p.p,
+                               c_q.p, c_scal.p, total, L);
+            XFE_CHECK(hipGetLastError());
+            if (it % 128 == 0) residual();
+            precondition64(c_r.p, c_z.p);
+            std::vector<double> rz_new;
+            dots(c_r.p, c_z.p, rz_new);
+            dots(c_r.p, c_r.p, rr);
+            for (int l = 0; l < L; ++l) {
+                if (!act[l]) {
+                    coef[L + l] = 0.0;
+                    continue;
+                }
+                ++extra[l];
+                if (std::sqrt(std::max(0.0, rr[l])) <= tol || !(pq[l] > 0.0)) {
+                    act[l] = 0;
+                    --nact;
+                    ok[l] = std::sqrt(std::max(0.0, rr[l])) <= accept;
+                }
+                coef[L + l] = rz[l] > 0.0 ? rz_new[l] / rz[l] : 0.0;
+            }
+            rz = rz_new;
+            XFE_CHECK(hipMemcpyAsync(c_scal.p + L, coef.data() + L, sizeof(double) * L, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(sm_dir64_kernel, dim3(sm_grid(total)), dim3(kBlock), 0, s, c_p.p, c_z.p, c_scal.p + L,
+                               total, L);
+            XFE_CHECK(hipGetLastError());
+            if (trace && it % 100 == 0) {
+                double worst = 0.0;
+                for (int l = 0; l < L; ++l)
+                    if (act[l]) worst = std::max(worst, std::sqrt(std::max(0.0, rr[l])));
+                std::fprintf(stderr, "[slope fp64 continuation] iteration %d: %d lanes active, worst |r| %.3e\n", it,
+                             nact, worst);
+            }
+        }
+        // the lanes still running: their true residual decides (accept)
+        if (nact > 0) {
+            residual();
+            dots(c_r.p, c_r.p, rr);
+            for (int l = 0; l < L; ++l)
+                if (act[l]) ok[l] = std::sqrt(std::max(0.0, rr[l])) <= accept;
+        }
+        wide.mark_converged(L, ok, extra);
+        return it - 1;
+    }
+
     void release() {
         wide.release();
         for (auto& g : gr)
@@ -695,6 +874,8 @@ struct SlopeMixed final : WideOp {
             b->release();
         bmat.release();
         binv.release();
+        binv64.release();
+        for (DevBuf<double>* b : {&c_r, &c_z, &c_p, &c_q, &c_part, &c_scal}) b->release();
         tmp.release();
         for (DevBuf<double>* b : {&Vs, &R64, &part64, &d1, &d2, &dl1, &dl2, &ssq}) b->release();
         for (int r = 0; r < kSmMaxR; ++r) {
@@ -794,7 +975,7 @@ bool slope_mixed_eligible(int C, int D, const int* mode, const double* w) {
 
 void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std::vector<const int*>& gid,
                      const std::vector<int>& G, const std::vector<const double*>& z, const std::vector<int>& mode,
-                     double tol, int max_iter, hipStream_t s, SlopeMixedResult& out) {
+                     double tol, int max_iter, int fp64_max_iter, hipStream_t s, SlopeMixedResult& out) {
     sm.s = s;
     sm.n = n;
     sm.C = C;
@@ -1038,6 +1219,7 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
             if (total > 0) {
                 sm.bmat.ensure(total);
                 sm.binv.ensure(total);
+                sm.binv64.ensure(total);
                 sm.bok.ensure(nb);
                 XFE_CHECK(hipMemsetAsync(sm.bmat.p, 0, sizeof(double) * total, s));
                 BlkArgs a;
@@ -1050,7 +1232,7 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
                 a.blk = sm.blk.p; a.loc = sm.loc.p; a.bcnt = sm.bcnt.p; a.moff = sm.moff.p; a.mat = sm.bmat.p;
                 hipLaunchKernelGGL(sm_blk_assemble_kernel, dim3(sm_grid(sm.Np)), dim3(kBlock), 0, s, a);
                 hipLaunchKernelGGL(sm_blk_invert_kernel, dim3(std::min(nb, 8192)), dim3(64), 0, s, sm.bcnt.p,
-                                   sm.moff.p, nb, sm.bmat.p, sm.binv.p, sm.bok.p, kBlkFail);
+                                   sm.moff.p, nb, sm.bmat.p, sm.binv.p, sm.binv64.p, sm.bok.p, kBlkFail);
                 XFE_CHECK(hipGetLastError());
                 sm.blk_on = true;
                 sm.nblk = nb;
@@ -1073,6 +1255,11 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
     sm.wide.trace = std::getenv("XFE_SLOPE_TRACE") ? "slope mixed" : nullptr;
     sm.wide.restart_each_replacement = true;
     sm.wide.solve_device(C, tol, max_iter, std::max(tol, 1e-9));
+    // lanes the FP32 runs left unconverged continue in FP64 from their iterate
+    // (XFE_SLOPE_NO_CONTINUATION set: none -- the fallback test's way to reach
+    // the FP64 slope path)
+    const bool cont = fp64_max_iter > 0 && !std::getenv("XFE_SLOPE_NO_CONTINUATION");
+    out.fp64_iterations = cont ? sm.fp64_continue(tol, std::max(tol, 1e-9), fp64_max_iter) : 0;
     sm.wide.unscale_x64();
     out.iterations = 0;
     out.converged = true;
@@ -1085,9 +1272,10 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
     out.replacements = sm.wide.last_updates;
     if (std::getenv("XFE_SLOPE_TRACE")) {
         std::fprintf(stderr,
-                     "[slope mixed] N %d NR %d p %d dropped %d pinned %d blocks %d passes %d replacements %d\n", N,
-                     sm.NR, p, out.intercepts_dropped, out.pinned, sm.blk_on ? sm.nblk : 0, out.passes,
-                     out.replacements);
+                     "[slope mixed] N %d NR %d p %d dropped %d pinned %d blocks %d passes %d replacements %d "
+                     "fp64 iterations %d\n",
+                     N, sm.NR, p, out.intercepts_dropped, out.pinned, sm.blk_on ? sm.nblk : 0, out.passes,
+                     out.replacements, out.fp64_iterations);
         for (int r = 0; r < sm.NR; ++r)
             std::fprintf(stderr, "[slope mixed]   block %d: dim %d slope %d levels %d off %d chunks %d\n", r,
                          sm.rd[r].d, sm.rd[r].slope ? 1 : 0, sm.rd[r].G, sm.rd[r].off,
