@@ -464,9 +464,10 @@ def bench_slope(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "method": ("worker dimension eliminated; FP32 PCG on the Schur system of the firm x year cells and "
                    "firm slopes (the firm intercept lies in the cells' span) with one exactly inverted block per "
-                   "firm (its cells and slope), 32 columns as lanes, restarted FP64 residual replacement, FP64 "
-                   "final transform; a run not converged in 5000 FP32 iterations falls back to the FP64 slope "
-                   "path on the untouched columns; columns resident in HBM"),
+                   "firm (its cells and slope), 32 columns as lanes, restarted FP64 residual replacement; columns "
+                   "the 2000 FP32 iterations leave unconverged continue with FP64 block-PCG from their iterate; "
+                   "FP64 final transform and certificate; the FP64 slope path only if that fails; columns "
+                   "resident in HBM"),
         "path": ("mixed" if info["precision_used"] == 1 else
                  "FP64 slope path (fallback: the mixed run did not converge)" if info["precision_fallback"]
                  else "FP64 slope path"),
