@@ -72,7 +72,10 @@ __global__ void __launch_bounds__(kBlock) sm_apply_kernel(SmArgs<T> a) {
     for (int w = blockIdx.x * kSlotsPerBlock + slot; w < a.Np; w += gridDim.x * kSlotsPerBlock) {
         const int i0 = a.pofs[w], i1 = a.pofs[w + 1];
         if (i1 <= i0 || !live) continue;
-        T cache[kRowCache];
+        // FP32 rows are kept in registers between the two loops; FP64 rows are
+        // formed again (a 32-entry FP64 array goes to scratch memory)
+        constexpr int kCacheRows = sizeof(T) == 4 ? kRowCache : 0;
+        T cache[kCacheRows > 0 ? kCacheRows : 1];
         T sum = T(0);
         // u_i - u_i0, formed coefficient by coefficient (differences of the
         // level values first; a slope term of one level as (z_i - z_i0) x), so
@@ -105,13 +108,13 @@ __global__ void __launch_bounds__(kBlock) sm_apply_kernel(SmArgs<T> a) {
         };
         for (int i = i0; i < i1; ++i) {
             const T u = row_delta(i);
-            if (i - i0 < kRowCache) cache[i - i0] = u;
+            if (i - i0 < kCacheRows) cache[i - i0] = u;
             sum += u;
         }
         const T mean = sum / static_cast<T>(i1 - i0);
         double tot = 0.0;
         for (int i = i0; i < i1; ++i) {
-            const T u = i - i0 < kRowCache ? cache[i - i0] : row_delta(i);
+            const T u = i - i0 < kCacheRows ? cache[i - i0] : row_delta(i);
             const T r = u - mean;
             a.R[static_cast<size_t>(i) * kCols + col] = r;
             if constexpr (STORE) {
@@ -154,20 +157,59 @@ sm_chunk_kernel(const int* __restrict__ perm, const int* __restrict__ cstart, in
     }
 }
 
-// out[(off + g) L + l] = sum of the level's chunk totals (l < C), 0 for l >= C
+// out[(off + g) L + l] = sum of the level's chunk totals (l < C), 0 for l >= C;
+// levels of more than big chunks are left to sm_level_big_kernel
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 sm_level_kernel(const int* __restrict__ cptr, const T* __restrict__ part, int G, int off, int L, int C,
-                T* __restrict__ out) {
+                T* __restrict__ out, int big) {
     const long long total = static_cast<long long>(G) * L;
     for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
          j += static_cast<long long>(gridDim.x) * kBlock) {
         const int g = static_cast<int>(j / L), l = static_cast<int>(j % L);
+        const int k0 = cptr[g], k1 = cptr[g + 1];
+        if (k1 - k0 > big) continue;
         double acc = 0.0;
         if (l < C)
-            for (int k = cptr[g]; k < cptr[g + 1]; ++k) acc += static_cast<double>(part[static_cast<size_t>(k) * kCols + l]);
+            for (int k = k0; k < k1; ++k) acc += static_cast<double>(part[static_cast<size_t>(k) * kCols + l]);
         out[static_cast<size_t>(off + g) * L + l] = static_cast<T>(acc);
     }
+}
+
+// The levels with many chunks (large firms), one block per level: 8 groups of
+// kCols threads each sum every 8th chunk, then the 8 group sums are added in
+// order (the same order on every run).
+constexpr int kLevelGroups = kBlock / kCols;
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+sm_level_big_kernel(const int* __restrict__ list, int nbig, const int* __restrict__ cptr, const T* __restrict__ part,
+                    int off, int L, int C, T* __restrict__ out) {
+    __shared__ double red[kBlock];
+    const int l = threadIdx.x % kCols, grp = threadIdx.x / kCols;
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int g = list[b];
+        const int k0 = cptr[g], k1 = cptr[g + 1];
+        double acc = 0.0;
+        if (l < C)
+            for (int k = k0 + grp; k < k1; k += kLevelGroups)
+                acc += static_cast<double>(part[static_cast<size_t>(k) * kCols + l]);
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        if (grp == 0) {
+            double t = 0.0;
+            for (int q = 0; q < kLevelGroups; ++q) t += red[q * kCols + l];
+            if (l < L) out[static_cast<size_t>(off + g) * L + l] = static_cast<T>(l < C ? t : 0.0);
+        }
+        for (int l2 = kCols + threadIdx.x; l2 < L; l2 += kBlock) out[static_cast<size_t>(off + g) * L + l2] = T(0);
+        __syncthreads();
+    }
+}
+
+// the levels of more than big chunks, in any order (each is summed on its own)
+__global__ void __launch_bounds__(kBlock)
+sm_big_list_kernel(const int* cptr, int G, int big, int* list, int* count) {
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < G; g += gridDim.x * kBlock)
+        if (cptr[g + 1] - cptr[g] > big) list[atomicAdd(count, 1)] = g;
 }
 
 // Per row of a p level, the Jacobi diagonal contributions of dimension r:
@@ -647,11 +689,14 @@ struct RDim {
     int grouping = 0;   // index into the groupings
 };
 
+constexpr int kBigChunks = 16;  // levels of more chunks are summed by a block each
+
 struct Grouping {
     int d = 0;
     int G = 0;
     int nchunks = 0;
-    DevBuf<int> gid, perm, cnt, ptr, nch, cptr, cstart, cend;
+    int nbig = 0;
+    DevBuf<int> gid, perm, cnt, ptr, nch, cptr, cstart, cend, big, nbig_d;
 };
 
 }  // namespace
@@ -707,7 +752,10 @@ struct SlopeMixed final : WideOp {
                                dim3(kBlock), 0, s, g.perm.p, g.cstart.p, g.nchunks, g.cend.p, m, R, stride, cols,
                                part);
             hipLaunchKernelGGL((sm_level_kernel<T>), dim3(sm_grid(static_cast<long long>(g.G) * wide.L)),
-                               dim3(kBlock), 0, s, g.cptr.p, part, g.G, rd[r].off, wide.L, cols, out);
+                               dim3(kBlock), 0, s, g.cptr.p, part, g.G, rd[r].off, wide.L, cols, out, kBigChunks);
+            if (g.nbig > 0)
+                hipLaunchKernelGGL((sm_level_big_kernel<T>), dim3(std::min(g.nbig, 65535)), dim3(kBlock), 0, s,
+                                   g.big.p, g.nbig, g.cptr.p, part, rd[r].off, wide.L, cols, out);
             XFE_CHECK(hipGetLastError());
         }
     }
@@ -868,7 +916,9 @@ struct SlopeMixed final : WideOp {
     void release() {
         wide.release();
         for (auto& g : gr)
-            for (DevBuf<int>* b : {&g.gid, &g.perm, &g.cnt, &g.ptr, &g.nch, &g.cptr, &g.cstart, &g.cend}) b->release();
+            for (DevBuf<int>* b : {&g.gid, &g.perm, &g.cnt, &g.ptr, &g.nch, &g.cptr, &g.cstart, &g.cend, &g.big,
+                                   &g.nbig_d})
+                b->release();
         for (DevBuf<int>* b : {&pperm, &pkey, &pcnt, &pofs, &tmp_int, &flag, &uf, &pin_root, &pin_flag, &pin_pos, &pins,
                                &blk, &bkey, &bval, &skey, &sval, &bcnt, &bptr, &bsq, &moff, &loc, &bok})
             b->release();
@@ -1055,6 +1105,15 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
         hipLaunchKernelGGL(sm_chunk_fill_kernel, dim3(sm_grid(G[d])), dim3(kBlock), 0, s, g.ptr.p, g.cptr.p, G[d],
                            g.cstart.p, g.cend.p);
         XFE_CHECK(hipGetLastError());
+        // the levels with more than kBigChunks chunks
+        g.big.ensure(std::max(G[d], 1));
+        g.nbig_d.ensure(1);
+        XFE_CHECK(hipMemsetAsync(g.nbig_d.p, 0, sizeof(int), s));
+        hipLaunchKernelGGL(sm_big_list_kernel, dim3(sm_grid(G[d])), dim3(kBlock), 0, s, g.cptr.p, G[d], kBigChunks,
+                           g.big.p, g.nbig_d.p);
+        XFE_CHECK(hipGetLastError());
+        XFE_CHECK(hipMemcpyAsync(&g.nbig, g.nbig_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        XFE_CHECK(hipStreamSynchronize(s));
         gidx[d] = ng++;
     }
     // a slope's intercept is left out when a finer standard dimension nests in
@@ -1120,7 +1179,7 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
                                dim3(kBlock), 0, s, g.perm.p, g.cstart.p, g.nchunks, g.cend.p, nullptr,
                                k == 0 ? sm.d1.p : sm.d2.p, 1, 1, sm.part64.p);
             hipLaunchKernelGGL((sm_level_kernel<double>), dim3(sm_grid(g.G)), dim3(kBlock), 0, s, g.cptr.p,
-                               sm.part64.p, g.G, sm.rd[r].off, 1, 1, k == 0 ? sm.dl1.p : sm.dl2.p);
+                               sm.part64.p, g.G, sm.rd[r].off, 1, 1, k == 0 ? sm.dl1.p : sm.dl2.p, INT_MAX);
         }
         XFE_CHECK(hipGetLastError());
     }
