@@ -709,6 +709,8 @@ struct SlopeMixed final : WideOp {
     std::vector<Grouping> gr;
     DevBuf<int> pperm, pkey, pcnt, pofs, tmp_int, flag, uf, pin_root, pin_flag, pin_pos, pins;
     int npins = 0;
+    std::vector<const int*> src_gid;  // the caller's id arrays of the last run, per dimension
+    bool transform_in_r64 = false;    // R64 holds the last run's transform rows (rows sorted by p)
     // block preconditioner
     bool blk_on = false;
     int nblk = 0;
@@ -987,9 +989,29 @@ sm_level_atomic_kernel(const double* V, int n, int C, int ld, const int* gid, do
 void slope_mixed_level_sumsq(SlopeMixed& sm, const double* V, int n, int C, int ld, const int* gid, int G,
                              hipStream_t s, std::vector<double>& acc) {
     sm.dl1.ensure(static_cast<size_t>(G) * kCols);
-    XFE_CHECK(hipMemsetAsync(sm.dl1.p, 0, sizeof(double) * static_cast<size_t>(G) * kCols, s));
-    hipLaunchKernelGGL(sm_level_atomic_kernel, dim3(sm_grid(static_cast<long long>(n) * C)), dim3(kBlock), 0, s, V, n,
-                       C, ld, gid, sm.dl1.p);
+    // the dimension's grouping from the run (its slope's), when the run's R64
+    // still holds the transform rows: chunked sums in a fixed order; else
+    // atomics over V
+    int gi = -1;
+    for (int r = 0; r < sm.NR && gi < 0; ++r)
+        if (sm.gr[sm.rd[r].grouping].gid.p && sm.gr[sm.rd[r].grouping].G == G && gid == sm.src_gid[sm.rd[r].d])
+            gi = sm.rd[r].grouping;
+    if (gi >= 0 && sm.transform_in_r64) {
+        Grouping& g = sm.gr[gi];
+        hipLaunchKernelGGL((sm_chunk_kernel<double>), dim3(sm_grid(static_cast<long long>(g.nchunks) * kCols)),
+                           dim3(kBlock), 0, s, g.perm.p, g.cstart.p, g.nchunks, g.cend.p, nullptr, sm.R64.p, kCols, C,
+                           sm.part64.p);
+        hipLaunchKernelGGL((sm_level_kernel<double>), dim3(sm_grid(static_cast<long long>(G) * kCols)), dim3(kBlock), 0,
+                           s, g.cptr.p, sm.part64.p, G, 0, kCols, C, sm.dl1.p, kBigChunks);
+        if (g.nbig > 0)
+            hipLaunchKernelGGL((sm_level_big_kernel<double>), dim3(std::min(g.nbig, 65535)), dim3(kBlock), 0, s,
+                               g.big.p, g.nbig, g.cptr.p, sm.part64.p, 0, kCols, C, sm.dl1.p);
+        XFE_CHECK(hipGetLastError());
+    } else {
+        XFE_CHECK(hipMemsetAsync(sm.dl1.p, 0, sizeof(double) * static_cast<size_t>(G) * kCols, s));
+        hipLaunchKernelGGL(sm_level_atomic_kernel, dim3(sm_grid(static_cast<long long>(n) * C)), dim3(kBlock), 0, s,
+                           V, n, C, ld, gid, sm.dl1.p);
+    }
     const int gs = std::min(1024, sm_grid(static_cast<long long>(G) * kCols));
     sm.ssq.ensure(static_cast<size_t>(gs) * kCols);
     hipLaunchKernelGGL(sm_sumsq_kernel, dim3(gs), dim3(kBlock), 0, s, sm.dl1.p, G, kCols, C, sm.ssq.p);
@@ -1029,6 +1051,8 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
     sm.s = s;
     sm.n = n;
     sm.C = C;
+    sm.src_gid = gid;
+    sm.transform_in_r64 = false;
     const int D = static_cast<int>(G.size());
     const int grid = sm_grid(n);
     // nested: every level of `fine` inside one level of `coarse`
@@ -1361,6 +1385,7 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
         a.ssq = sm.ssq.p;
         hipLaunchKernelGGL((sm_apply_kernel<double, true, true>), dim3(gw), dim3(kBlock), 0, s, a);
         XFE_CHECK(hipGetLastError());
+        sm.transform_in_r64 = true;
     }
     std::vector<double> resid(C, 0.0), hp(static_cast<size_t>(gw) * kCols);
     XFE_CHECK(hipMemcpyAsync(hp.data(), sm.ssq.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
