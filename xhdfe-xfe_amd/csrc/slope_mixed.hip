@@ -586,34 +586,6 @@ sm_blk_invert_kernel(const int* bcnt, const int* moff, int nb, const double* mat
     }
 }
 
-// z = B^-1 r on the blocks (one thread per block and lane); the block's
-// unknowns are sval[bptr[b] .. bptr[b] + k)
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-sm_blk_apply_kernel(const int* bptr, const int* bcnt, const int* moff, const int* ok, const int* sval, int nb, int L,
-                    int C, const T* inv, const T* r, T* z) {
-    const long long total = static_cast<long long>(nb) * L;
-    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
-         j += static_cast<long long>(gridDim.x) * kBlock) {
-        const int b = static_cast<int>(j / L), l = static_cast<int>(j % L);
-        if (l >= C || !ok[b]) continue;
-        const int k = bcnt[b];
-        const int* u = sval + bptr[b];
-        const T* Bi = inv + moff[b];
-        T rv[kBlkMax];
-#pragma unroll
-        for (int e = 0; e < kBlkMax; ++e) rv[e] = e < k ? r[static_cast<size_t>(u[e]) * L + l] : T(0);
-        for (int x = 0; x < k; ++x) {
-            const T* row = Bi + x * k;
-            T acc = T(0);
-#pragma unroll
-            for (int e = 0; e < kBlkMax; ++e)
-                if (e < k) acc += row[e] * rv[e];
-            z[static_cast<size_t>(u[x]) * L + l] = acc;
-        }
-    }
-}
-
 // ---- FP64 block-PCG continuation (level-major N x L vectors, lanes < C)
 
 // part[b * L + l] = sum over block b's rows of a[u, l] b[u, l] (kBlock % L == 0)
@@ -673,6 +645,28 @@ sm_res64_kernel(double* r, const double* b, const double* q, long long total) {
         r[i] = b[i] - q[i];
 }
 
+// z = B^-1 r on the blocks, one thread per (blocked unknown, lane): the
+// unknown at sorted position q (block skey[q], row q - bptr[b] of its inverse)
+// gets the dot of that row with the block's r values
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+sm_blk_rows_kernel(const int* skey, const int* bptr, const int* bcnt, const int* moff, const int* ok, const int* sval,
+                   int nin, int L, int C, const T* inv, const T* r, T* z) {
+    const long long total = static_cast<long long>(nin) * L;
+    for (long long j = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; j < total;
+         j += static_cast<long long>(gridDim.x) * kBlock) {
+        const int q = static_cast<int>(j / L), l = static_cast<int>(j % L);
+        const int b = skey[q];
+        if (l >= C || !ok[b]) continue;
+        const int k = bcnt[b], q0 = bptr[b];
+        const T* row = inv + moff[b] + static_cast<size_t>(q - q0) * k;
+        const int* u = sval + q0;
+        T acc = T(0);
+        for (int e = 0; e < k; ++e) acc += row[e] * r[static_cast<size_t>(u[e]) * L + l];
+        z[static_cast<size_t>(sval[q]) * L + l] = acc;
+    }
+}
+
 int sm_grid(long long work) {
     const long long g = (work + kBlock - 1) / kBlock;
     return static_cast<int>(std::max(1LL, std::min(g, 16384LL)));
@@ -714,6 +708,7 @@ struct SlopeMixed final : WideOp {
     // block preconditioner
     bool blk_on = false;
     int nblk = 0;
+    int nin = 0;  // unknowns inside the blocks (sorted positions 0 .. nin - 1)
     DevBuf<int> blk, bkey, bval, skey, sval, bcnt, bptr, bsq, moff, loc, bok;
     DevBuf<double> bmat, binv64;
     DevBuf<float> binv;
@@ -765,8 +760,9 @@ struct SlopeMixed final : WideOp {
     bool has_blocks() const override { return blk_on; }
 
     void precondition(const float* r, float* z) override {
-        hipLaunchKernelGGL((sm_blk_apply_kernel<float>), dim3(sm_grid(static_cast<long long>(nblk) * wide.L)),
-                           dim3(kBlock), 0, s, bptr.p, bcnt.p, moff.p, bok.p, sval.p, nblk, wide.L, C, binv.p, r, z);
+        hipLaunchKernelGGL((sm_blk_rows_kernel<float>), dim3(sm_grid(static_cast<long long>(nin) * wide.L)),
+                           dim3(kBlock), 0, s, skey.p, bptr.p, bcnt.p, moff.p, bok.p, sval.p, nin, wide.L, C, binv.p, r,
+                           z);
         XFE_CHECK(hipGetLastError());
     }
 
@@ -821,8 +817,8 @@ struct SlopeMixed final : WideOp {
         const long long total = static_cast<long long>(N) * wide.L;
         hipLaunchKernelGGL(sm_jacobi64_kernel, dim3(sm_grid(total)), dim3(kBlock), 0, s, r, minv.p, z, total, wide.L);
         if (blk_on)
-            hipLaunchKernelGGL((sm_blk_apply_kernel<double>), dim3(sm_grid(static_cast<long long>(nblk) * wide.L)),
-                               dim3(kBlock), 0, s, bptr.p, bcnt.p, moff.p, bok.p, sval.p, nblk, wide.L, C,
+            hipLaunchKernelGGL((sm_blk_rows_kernel<double>), dim3(sm_grid(static_cast<long long>(nin) * wide.L)),
+                               dim3(kBlock), 0, s, skey.p, bptr.p, bcnt.p, moff.p, bok.p, sval.p, nin, wide.L, C,
                                binv64.p, r, z);
         XFE_CHECK(hipGetLastError());
     }
@@ -1298,6 +1294,7 @@ void slope_mixed_run(SlopeMixed& sm, double* V, int n, int C, int ld, const std:
             excl_scan(sm.tmp, sm.bsq.p, sm.moff.p, nb + 1, s);
             int total = 0;
             XFE_CHECK(hipMemcpyAsync(&total, sm.moff.p + nb, sizeof(int), hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipMemcpyAsync(&sm.nin, sm.bptr.p + nb, sizeof(int), hipMemcpyDeviceToHost, s));
             XFE_CHECK(hipStreamSynchronize(s));
             if (total > 0) {
                 sm.bmat.ensure(total);
