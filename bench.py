@@ -799,6 +799,12 @@ def main():
             ablation[name] = {"s": round(a_s, 4), "iterations": a_info["iterations"],
                               "refinements": a_info["refinements"], "precond_setup_ms": round(a_info["precond_ms"], 2)}
             log(f"ablation {name}: {a_s:.4f} s/step, {a_info['iterations']} inner iterations")
+        # the same Schur-complement Jacobi-PCG entirely in FP64 (precision 2): what
+        # the precision alone changes, against mixed_jacobi
+        a_s, a_info = timed(2, args.ablation_steps, 1, False, 0)
+        ablation["schur_fp64_jacobi"] = {"s": round(a_s, 4), "iterations": a_info["iterations"],
+                                         "note": "precision 2: mixed_jacobi's algorithm in FP64 throughout"}
+        log(f"ablation schur_fp64_jacobi: {a_s:.4f} s/step, {a_info['iterations']} iterations")
 
     sec, info = timed(args.precision, args.steps, args.warmup, True, args.preconditioner)
     pass_ms, launches, pass_bytes = xfe_amd.profile_get()

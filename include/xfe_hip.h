@@ -53,7 +53,9 @@ typedef struct {
                                    reference's dispatcher, src/fe_absorption.cpp:9141), 5 Lsmr and 6 Mlsmr
                                    (CPU-only there, :9127) and any other value are rejected: the call returns 0 */
     int force_cg;               /* explicit reghdfe convergence criterion -> CG accelerator */
-    int precision;              /* 0: FP64 throughout (the plain method); 1: FP32 sweeps under FP64 refinement */
+    int precision;              /* 0: FP64 throughout (the plain method); 1: FP32 sweeps under FP64 refinement;
+                                   2: the Schur-complement Jacobi-PCG of precision 1 entirely in FP64 (its
+                                   precision ablation; no counterpart in the reference) */
     int certify;                /* 1: compute the normal-equation certificate of the result on the device
                                    (certify_absorption_result, src/fe_absorption.cpp:14942-15190) */
     double refine_tol;          /* precision 1: FP64 refinement target, ||b - S beta|| / ||b|| of the
@@ -89,7 +91,8 @@ typedef struct {
     double abs_residual_rel;
     int precision_certified;
     int certificate_computed;
-    /* which method ran: 0 the plain FP64 method, 1 the mixed-precision Schur PCG.  A request
+    /* which method ran: 0 the plain FP64 method, 1 the mixed-precision Schur PCG, 2 the FP64
+       Schur PCG.  A request
        for precision 1 that the mixed path cannot take (weights, slopes, 1 or more than 3
        dimensions, more than 32 columns, no level-aligned tiles) runs the plain FP64 method and
        reports 0 here with precision_fallback = 1. */

@@ -307,6 +307,26 @@ def test_mixed_precision_matches_tight_oracle(cases, name):
     assert err <= 1e-9
 
 
+@pytest.mark.parametrize("name", ["two_way", "akm_three_way"])
+def test_fp64_schur_matches_tight_oracle_and_mixed(cases, name):
+    """precision 2 (the Schur-complement Jacobi-PCG entirely in FP64, the
+    mixed path's FP64 ablation): certified, within 1e-9 of the tight oracle,
+    and within 1e-9 of the mixed path run with the same (Jacobi)
+    preconditioner."""
+    _log(f"fp64 schur {name}")
+    p = cases[name]
+    _, fes, gids, G, V = _prep(p)
+    Vo, _ = oracle.absorb(V, gids, G, tol=1e-12)
+    y2, X2, i2 = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=2, preconditioner=0)
+    y1, X1, i1 = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=1, preconditioner=0)
+    scale = np.abs(V).max()
+    err = max(_rel(y2, Vo[:, 0], scale), _rel(X2, Vo[:, 1:], scale))
+    err12 = max(_rel(y2, y1, scale), _rel(X2, X1, scale))
+    _log(f"  FP64 iterations {i2['iterations']} (mixed {i1['iterations']}) err {err:.2e} vs mixed {err12:.2e}")
+    assert i2["converged"] and i2["precision_used"] == 2 and i2["precision_certified"], i2
+    assert err <= 1e-9 and err12 <= 1e-9
+
+
 @pytest.mark.parametrize("which", ["config0", "akm_1pct"])
 def test_mixed_precision_spell_path(which):
     _log(f"mixed precision {which}")

@@ -125,6 +125,7 @@ struct Workspace {
     DevBuf<int> w_pf, w_pi, w_rank, seg_rk, p_order, p_rk, p_r1, p_len, p_a, p_wseg;
     DevBuf<float> f_wmean, f_hpart, f_invw, f_invwm;
     DevBuf<double> d_wmean, d_hpart, seg_tot, f_b0, f_tmp, d_invw, d_invwm;
+    DevBuf<double> s64_z, s64_p, s64_q, s64_coef;  // the FP64 Schur PCG (precision 2)
     DevBuf<int> m_flag, m_idx, m_list;  // movers among the worker ranks
     DevBuf<int> fm_key, fm_w, fm_len, fm_a, stay_h;  // movers' firm-major records; stayer histogram
     DevBuf<double> stay_K;              // the stayers' year block (G2 x G2)
@@ -180,6 +181,7 @@ struct Workspace {
         wseg.release(); f_key.release(); f_order.release(); f_w.release(); f_len.release(); f_a.release();
         f_wmean.release(); f_hpart.release(); d_wmean.release(); d_hpart.release(); seg_tot.release();
         f_b0.release(); f_tmp.release();
+        s64_z.release(); s64_p.release(); s64_q.release(); s64_coef.release();
         w_pf.release(); w_pi.release(); w_rank.release(); seg_rk.release(); p_order.release();
         p_rk.release(); p_r1.release(); p_len.release(); p_a.release(); p_wseg.release();
         f_invw.release(); d_invw.release(); f_invwm.release(); d_invwm.release();
@@ -1436,6 +1438,71 @@ struct FastSolver {
     // when its FP64 residual is at most target |b0_c|, or when four updates in a
     // row each fail to halve it.  Returns the FP64 relative residual (max over
     // columns); iterations and updates through the out arguments.
+    // S x in FP64 for any level-major x (the segment operator, or the row pass)
+    void apply64(const double* x, double* q) {
+        if (seg) {
+            seg_apply<double>(x, q, ws.d_invw.p, ws.d_wmean.p, ws.d_hpart.p);
+            return;
+        }
+        double* sp[2] = {nullptr, nullptr};
+        for (int k = 0; k < NR; ++k) if (mode[k] == kRedSpell) sp[k] = ws.sp_sums[rdim[k]].p;
+        schur_pass<double>(x, q, nullptr, nullptr, 1.0, ws.inv[p].p, sp, lds_extra64, nullptr);
+    }
+
+    // precision 2: Jacobi-PCG on S beta = b0 entirely in FP64 (the same
+    // elimination and preconditioner as the mixed path's Jacobi variant, so
+    // the two differ only in precision); r in f_b, beta in f_beta; a column
+    // stops at |r| <= target |b0|.  Returns the worst |r| / |b0|.
+    double fp64_solve(const double* b0n, double target, int max_iter, int& iters) {
+        const long long total = static_cast<long long>(N) * C;
+        const int g = grid_for(total);
+        for (DevBuf<double>* b : {&ws.s64_z, &ws.s64_p, &ws.s64_q}) b->ensure(static_cast<size_t>(total));
+        ws.s64_coef.ensure(2 * static_cast<size_t>(kMaxCgCols));
+        double* r = ws.f_b.p;
+        double *z = ws.s64_z.p, *pv = ws.s64_p.p, *q = ws.s64_q.p;
+        double* d_alpha = ws.s64_coef.p;
+        double* d_beta = ws.s64_coef.p + kMaxCgCols;
+        double rz[kMaxCgCols], rz_new[kMaxCgCols], pq[kMaxCgCols], rr[kMaxCgCols], coef[kMaxCgCols];
+        bool act[kMaxCgCols];
+        XFE_CHECK(launch_s64_jacobi(r, ws.f_minv.p, z, total, C, g, s));
+        XFE_CHECK(hipMemcpyAsync(pv, z, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+        coldot<double>(r, z, rz);
+        coldot<double>(r, r, rr);
+        double worst = 0.0;
+        int nact = 0;
+        for (int c = 0; c < C; ++c) {
+            act[c] = b0n[c] > 0.0 && std::sqrt(std::max(0.0, rr[c])) > target * b0n[c];
+            nact += act[c] ? 1 : 0;
+        }
+        iters = 0;
+        for (int it = 1; it <= max_iter && nact > 0; ++it) {
+            apply64(pv, q);
+            coldot<double>(pv, q, pq);
+            for (int c = 0; c < C; ++c) coef[c] = (act[c] && pq[c] > 0.0 && rz[c] > 0.0) ? rz[c] / pq[c] : 0.0;
+            upload(coef, d_alpha, C);
+            XFE_CHECK(launch_s64_axpy(ws.f_beta.p, pv, d_alpha, total, C, 1.0, g, s));
+            XFE_CHECK(launch_s64_axpy(r, q, d_alpha, total, C, -1.0, g, s));
+            coldot<double>(r, r, rr);
+            XFE_CHECK(launch_s64_jacobi(r, ws.f_minv.p, z, total, C, g, s));
+            coldot<double>(r, z, rz_new);
+            for (int c = 0; c < C; ++c) {
+                if (act[c] && (std::sqrt(std::max(0.0, rr[c])) <= target * b0n[c] || !(pq[c] > 0.0))) {
+                    act[c] = false;
+                    --nact;
+                }
+                coef[c] = (act[c] && rz[c] > 0.0) ? rz_new[c] / rz[c] : 0.0;
+                rz[c] = rz_new[c];
+            }
+            upload(coef, d_beta, C);
+            XFE_CHECK(launch_s64_dir(pv, z, d_beta, total, C, g, s));
+            iters = it;
+        }
+        for (int c = 0; c < C; ++c)
+            if (b0n[c] > 0.0) worst = std::max(worst, std::sqrt(std::max(0.0, rr[c])) / b0n[c]);
+        inner_total += iters;
+        return worst;
+    }
+
     double reliable_solve(double* V, const double* b0n, double target, double delta, int max_inner, int& iters,
                           int& updates) {
         constexpr int kPoll = 8;
@@ -1576,8 +1643,10 @@ static bool solve_fast(Solver& sv, double* V, const xfe_options& opt, xfe_absorb
     for (int c = 0; c < C; ++c) b0n[c] = std::sqrt(b0[c]);
     const double delta = std::getenv("XFE_RELIABLE_DELTA") ? std::atof(std::getenv("XFE_RELIABLE_DELTA")) : 0.01;
     int iters = 0, updates = 0;
-    const double rel = fs.reliable_solve(V, b0n, target, delta, std::max(10, std::min(opt.max_iter, 20000)), iters,
-                                         updates);
+    const double rel =
+        opt.precision == 2
+            ? fs.fp64_solve(b0n, target, std::max(10, std::min(opt.max_iter, 20000)), iters)
+            : fs.reliable_solve(V, b0n, target, delta, std::max(10, std::min(opt.max_iter, 20000)), iters, updates);
     const int refinements = updates;
     const bool converged = rel <= std::max(target, 1e-9);
     (void)tol_in;
@@ -1641,9 +1710,9 @@ static void solve_core(Solver& sv, double* V, const std::vector<int>& order_in,
     const double* w = sv.w;
     std::vector<int> G(sv.G, sv.G + D);
     for (int d = 0; d < D; ++d) res.sweep_order_used[d] = order_in[d];
-    if (opt.precision == 1) {
+    if (opt.precision == 1 || opt.precision == 2) {
         if (solve_fast(sv, V, opt, res)) {
-            res.precision_used = 1;
+            res.precision_used = opt.precision;
             return;
         }
         res.precision_fallback = 1;

@@ -202,6 +202,14 @@ hipError_t launch_pull(const T* S, int C, int c0, int cc, const int* order, cons
                        T* sums, int grid, hipStream_t s);
 template <typename T>
 hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, int grid, hipStream_t s);
+// the FP64 Schur PCG (precision 2): y += sign coef[c] x; z = minv[level] r;
+// p = z + beta[c] p over level-major N x C vectors (total = N C)
+hipError_t launch_s64_axpy(double* y, const double* x, const double* coef, long long total, int C, double sign,
+                           int grid, hipStream_t s);
+hipError_t launch_s64_jacobi(const double* r, const float* minv, double* z, long long total, int C, int grid,
+                             hipStream_t s);
+hipError_t launch_s64_dir(double* p, const double* z, const double* beta, long long total, int C, int grid,
+                          hipStream_t s);
 // Scalar block of the device-side CG (doubles, kMaxCgCols per field).
 enum {
     kCgAlpha = 0 * kMaxCgCols, kCgBeta = 1 * kMaxCgCols, kCgRz = 2 * kMaxCgCols,

@@ -792,6 +792,53 @@ cg_padd_dot_kernel(const float* r, const float* z, float* p, long long total, in
     cg_block_store(rz, c, C, part1);
 }
 
+// ---- the Schur-complement PCG in FP64 (precision 2): level-major N x C vectors,
+// per-column scalars coef[c]
+
+// y += sign coef[c] x
+__global__ void __launch_bounds__(kBlock)
+s64_axpy_kernel(double* y, const double* x, const double* coef, long long total, int C, double sign) {
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * kBlock) {
+        const double a = coef[i % C];
+        if (a != 0.0) y[i] += sign * a * x[i];
+    }
+}
+
+// z = minv[level] r
+__global__ void __launch_bounds__(kBlock)
+s64_jacobi_kernel(const double* r, const float* minv, double* z, long long total, int C) {
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * kBlock)
+        z[i] = static_cast<double>(minv[i / C]) * r[i];
+}
+
+// p = z + beta[c] p
+__global__ void __launch_bounds__(kBlock)
+s64_dir_kernel(double* p, const double* z, const double* beta, long long total, int C) {
+    for (long long i = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; i < total;
+         i += static_cast<long long>(gridDim.x) * kBlock)
+        p[i] = z[i] + beta[i % C] * p[i];
+}
+
+hipError_t launch_s64_axpy(double* y, const double* x, const double* coef, long long total, int C, double sign,
+                           int grid, hipStream_t s) {
+    hipLaunchKernelGGL(s64_axpy_kernel, dim3(grid), dim3(kBlock), 0, s, y, x, coef, total, C, sign);
+    return hipGetLastError();
+}
+
+hipError_t launch_s64_jacobi(const double* r, const float* minv, double* z, long long total, int C, int grid,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(s64_jacobi_kernel, dim3(grid), dim3(kBlock), 0, s, r, minv, z, total, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_s64_dir(double* p, const double* z, const double* beta, long long total, int C, int grid,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(s64_dir_kernel, dim3(grid), dim3(kBlock), 0, s, p, z, beta, total, C);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_vdot(const T* a, const T* b, int N, int C, double* partials, int grid, hipStream_t s) {
     hipLaunchKernelGGL((vdot_kernel<T>), dim3(grid), dim3(kBlock), 0, s, a, b, N, C, partials);
