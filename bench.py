@@ -238,6 +238,32 @@ def bench_akm(args):
     log(f"JLA B_ii, 200 draws: {bii_s:.2f} s, converged {bii_ok}, sums {bfe.sum():.4g} {bpe.sum():.4g} "
         f"{bcov.sum():.4g}")
     solver.close()
+    # exact leverages (the reference's LeverageMethod::Exact: one solve per
+    # match, here 256 matches per wide solve) against the 200 JLA draws on a
+    # 200k-obs panel of the same shape (24k workers, 3k firms)
+    ps = panels.kss_panel(workers=24_000, firms=3_000)
+    mw2, mf2, mc2, N2, J2 = akm.panel_matches(ps["worker"], ps["firm"])
+    s2 = xfe_amd.AkmSolver(akm.build_structure(mw2, mf2, mc2, N2, J2))
+    t0 = time.perf_counter()
+    ex_p, ex_fe, ex_pe, ex_cov, ex_failed = s2.exact_leverages(tol=1e-10, max_iter=max_iter, accept=1e-8)
+    exact_s = time.perf_counter() - t0
+    jp2, _ = s2.jla_pii(draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
+    jfe, jpe, jcov, _ = s2.jla_bii(None, draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
+    exact_cmp = {
+        "obs": int(ps["worker"].size), "workers": int(N2), "firms": int(J2), "matches": int(mw2.size),
+        "exact_s": round(exact_s, 3), "unconverged_solves": ex_failed, "fp64_retries": s2.exact_retries,
+        "tol": 1e-10, "accept_on_stall": 1e-8,
+        "sum_pii_minus_rank": float(ex_p.sum() - (N2 + J2 - 1)),
+        "jla200_pii_mean_abs_err": float(np.abs(jp2 - ex_p).mean()),
+        "jla200_pii_max_abs_err": float(np.abs(jp2 - ex_p).max()),
+        "jla200_sum_b_rel_err": [float(abs(j.sum() - e.sum()) / abs(e.sum()))
+                                 for j, e in ((jfe, ex_fe), (jpe, ex_pe), (jcov, ex_cov))],
+    }
+    s2.close()
+    log(f"exact leverages, {mw2.size} matches ({ps['worker'].size} obs): {exact_s:.2f} s, unconverged {ex_failed}, "
+        f"FP64 retries {s2.exact_retries}; "
+        f"JLA-200 P_ii mean abs err {exact_cmp['jla200_pii_mean_abs_err']:.3e}, sum B rel err "
+        f"{exact_cmp['jla200_sum_b_rel_err']}")
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     log(f"{sec:.3f} s per {lanes} lanes; wide_pair_kernel: {launches} launches, "
         f"{pass_ms / max(launches, 1):.4f} ms avg, {achieved:.0f} GB/s algorithmic")
@@ -262,6 +288,7 @@ def bench_akm(args):
         "fp64_32lane_iterations": {"min": int(its64.min()), "max": int(its64.max())},
         "max_rel_diff_vs_fp64_32lane": err,
         "jla_pii_200_draws_s": round(jla_s, 3), "jla_bii_200_draws_s": round(bii_s, 3),
+        "exact_vs_jla_200k": exact_cmp,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(os.path.join(ROOT, "profiles", "wide_pair_kss10m.md")),

@@ -499,6 +499,30 @@ int xfe_akm_wide_max_lanes(void);
  * xfe_akm_solve_S_wide call.  Returns 0, or -1 on invalid arguments. */
 int xfe_akm_wide_last(xfe_akm_context* ctx, int* fp32_passes, int* fp64_replacements);
 
+/* Exact leverages and quadratic-form weights, the reference's exact path of
+ * akm_kss_decompose (LeverageMethod::Exact, src/akm_kss.cpp:2667-2741, match
+ * level): per match m one solve of K [zw; zf] = [e_w; e_f] (here through the
+ * firm Schur complement, 256 matches per wide solve), then
+ *   pii[m]  = c_m (zw[w] + zf[f]),
+ *   bfe[m]  = c_m (sum_f Df zf^2 - (sum_f Df zf)^2 / n_py),
+ *   bpe[m]  = c_m (sum_w Dw zw^2 - (sum_w Dw zw)^2 / n_py),
+ *   bcov[m] = c_m (sum_k c_k zw[w_k] zf[f_k] - (sum_w Dw zw)(sum_f Df zf) / n_py),
+ * n_py = sum_m c_m, in the context's match order (outputs of M doubles each).
+ * tol / max_iter: the wide solver's (FP32 passes under FP64 residual
+ * replacement, stop on the FP64 residual |r| <= tol |rhs|); accept (>= tol, or
+ * 0 for the strict rule): a solve whose FP64 residual stops improving counts
+ * as converged at |r| <= accept |rhs|.  Matches whose wide solve does not
+ * converge are solved again by the FP64 batched Jacobi-PCG.  Returns 0, the
+ * number of matches whose solve did not converge there either (their values
+ * are those of a zero solution), or -1 on failure. */
+int xfe_akm_exact_leverages(xfe_akm_context* ctx, double tol, double accept, int max_iter, double* pii, double* bfe,
+                            double* bpe, double* bcov);
+
+/* Matches of this thread's last xfe_akm_exact_leverages call whose wide solve
+ * did not converge and that were solved again by the FP64 batched Jacobi-PCG
+ * (32 lanes per solve).  No counterpart in the reference. */
+int xfe_akm_exact_leverages_last_retries(void);
+
 /* ---- Cluster bootstrap refits of the two-way FE regression on the wide
  * solver's launch path.  No counterpart in the reference (its bootstrap is the
  * Gelbach decomposition's, src/akm_kss.cpp:4965-7305, outside this path); the

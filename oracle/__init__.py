@@ -539,6 +539,63 @@ def akm_jla_bii(st, draws, seed, count):
     return bfe, bpe, bcov
 
 
+def ref_akm_exact(y, worker, firm):
+    """The reference's akm_kss_decompose with exact leverages at match level
+    (no pruning) through oracle/_ref (checker): (row_worker, row_firm,
+    row_weight, pii, comp, sigma_i), comp as in ref_akm_jla_components."""
+    r = ref()
+    if r is None:
+        return None
+    n = len(y)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    w = np.ascontiguousarray(worker, dtype=np.int32)
+    f = np.ascontiguousarray(firm, dtype=np.int32)
+    rw = np.empty(n, dtype=np.int32)
+    rf = np.empty(n, dtype=np.int32)
+    rc = np.empty(n)
+    pii = np.empty(n)
+    sig = np.empty(n)
+    comp = np.empty(11)
+    r.ref_akm_exact.argtypes = [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p]
+    R = r.ref_akm_exact(n, _ptr(y), _ptr(w), _ptr(f), _ptr(rw), _ptr(rf), _ptr(rc), _ptr(pii), _ptr(comp),
+                        _ptr(sig))
+    if R < 0:
+        raise RuntimeError("ref_akm_exact failed")
+    return rw[:R], rf[:R], rc[:R], pii[:R], comp, sig[:R]
+
+
+def akm_exact_leverages(st):
+    """The exact path of akm_kss_decompose (src/akm_kss.cpp:2667-2741, match
+    level): per match m, K [zw; zf] = [e_w; e_f] solved exactly (firms
+    grounded at J-1, dense Schur solve); P_ii = c (zw[w] + zf[f]); with
+    s1 = sum Df zf^2, s2 = sum Df zf, s3 = sum Dw zw^2, s4 = sum Dw zw and
+    cross = sum_k c_k zw[w_k] zf[f_k]: Bfe = c (s1 - s2^2 / n_py), Bpe =
+    c (s3 - s4^2 / n_py), Bcov = c (cross - s4 s2 / n_py), n_py = sum c."""
+    mw, mf, c = st["m_w"], st["m_f"], st["m_c"]
+    N, J = st["N"], st["J"]
+    S = np.zeros((J, J))
+    for f in range(J):
+        e = np.zeros(J)
+        e[f] = 1.0
+        S[:, f] = akm_apply_S(st, e)
+    Sg = S[:J - 1, :J - 1]
+    n_py = float(c.sum())
+    Dw, Df = st["Dw"], st["Df"]
+    out = np.zeros((4, mw.size))
+    for m in range(mw.size):
+        tw = np.zeros(N)
+        tf = np.zeros(J)
+        tw[mw[m]] = 1.0
+        tf[mf[m]] = 1.0
+        zw, zf = _two_way_solve(st, Sg, tw, tf)
+        s1, s2 = float(Df @ (zf * zf)), float(Df @ zf)
+        s3, s4 = float(Dw @ (zw * zw)), float(Dw @ zw)
+        cross = float(np.sum(c * zw[mw] * zf[mf]))
+        out[:, m] = c[m] * np.array([zw[mw[m]] + zf[mf[m]], s1 - s2 * s2 / n_py, s3 - s4 * s4 / n_py,
+                                     cross - s4 * s2 / n_py])
+    return out[0], out[1], out[2], out[3]
+
+
 def ref_akm_jla_components(y, worker, firm, draws, seed):
     """The reference's akm_kss_decompose (JLA, match level, no pruning)
     through oracle/_ref (checker): (comp, sigma_i) with comp = plug-in, AGSU

@@ -296,6 +296,44 @@ int ref_akm_jla_pii(int n, const double* y, const int* worker, const int* firm, 
     }
 }
 
+// akm_kss_decompose with exact leverages (LeverageMethod::Exact,
+// src/akm_kss.cpp:2667-2741), match level, no pruning: the working rows, their
+// P_ii, the components (as ref_akm_jla_components) and sigma_i.  Row outputs
+// have capacity n; returns the number of working rows, -1 on exception.
+int ref_akm_exact(int n, const double* y, const int* worker, const int* firm, int* row_worker, int* row_firm,
+                  double* row_weight, double* pii, double* comp, double* sigma) {
+    try {
+        Eigen::VectorXd yv = Eigen::Map<const Eigen::VectorXd>(y, n);
+        Eigen::VectorXi wv = Eigen::Map<const Eigen::VectorXi>(worker, n);
+        Eigen::VectorXi fv = Eigen::Map<const Eigen::VectorXi>(firm, n);
+        hdfe::akm::AkmOptions o;
+        o.leverage_method = hdfe::akm::LeverageMethod::Exact;
+        o.prune = false;
+        o.num_threads = 1;
+        auto r = hdfe::akm::akm_kss_decompose(yv, wv, fv, nullptr, o);
+        const int R = static_cast<int>(r.pii.size());
+        for (int k = 0; k < R && k < n; ++k) {
+            row_worker[k] = r.row_worker[k];
+            row_firm[k] = r.row_firm[k];
+            row_weight[k] = r.row_weight[k];
+            pii[k] = r.pii[k];
+            sigma[k] = r.sigma_i[k];
+        }
+        const hdfe::akm::AkmComponents* cs[3] = {&r.plugin, &r.agsu, &r.kss};
+        for (int i = 0; i < 3; ++i) {
+            comp[3 * i] = cs[i]->var_alpha;
+            comp[3 * i + 1] = cs[i]->var_psi;
+            comp[3 * i + 2] = cs[i]->cov_alpha_psi;
+        }
+        comp[9] = r.sigma2_ho;
+        comp[10] = r.row_weight.sum();
+        return R;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_akm_exact: %s\n", e.what());
+        return -1;
+    }
+}
+
 // The same decomposition's variance components and KSS sigma_i
 // (src/akm_kss.cpp:3354-3412): comp = plug-in, AGSU and KSS
 // (var_alpha, var_psi, cov_alpha_psi) each, then sigma2_ho and n_py; sigma

@@ -610,6 +610,38 @@ def test_akm_jla_pii_matches_reference(golden):
     solver.close()
 
 
+def test_akm_exact_leverages_match_reference():
+    """xfe_akm_exact_leverages (one solve per match, 256 per wide solve)
+    against the reference's exact path (LeverageMethod::Exact, fixture): P_ii,
+    and the B_ii weights through the components they enter (AGSU: sum B;
+    KSS: B . sigma_i); against the oracle's exact leverages per match; sum
+    P_ii = N + J - 1."""
+    import os
+    _log("akm exact leverages")
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "exact_leverages.npz"))
+    wu, wi = np.unique(fx["row_worker"], return_inverse=True)
+    fu, fi = np.unique(fx["row_firm"], return_inverse=True)
+    st = oracle.akm_build(wi, fi, fx["row_weight"], wu.size, fu.size)
+    solver = xfe_amd.AkmSolver(st)
+    P, Bfe, Bpe, Bcov, failed = solver.exact_leverages(tol=1e-11, max_iter=5000)
+    solver.close()
+    assert failed == 0, xfe_amd.last_error()
+    err = np.abs(P - fx["pii"]).max()
+    _log(f"  P_ii vs reference {err:.2e}; sum {P.sum():.6f} (N + J - 1 = {st['N'] + st['J'] - 1})")
+    assert err <= 1e-8
+    assert abs(P.sum() - (st["N"] + st["J"] - 1)) <= 1e-6
+    comp, sig = fx["components"], fx["sigma_i"]
+    dof = comp[10] - 1.0
+    for i, B in enumerate((Bpe, Bfe, Bcov)):
+        want_sum = (comp[i] - comp[3 + i]) * dof / comp[9]
+        want_sig = (comp[i] - comp[6 + i]) * dof
+        assert abs(B.sum() - want_sum) <= 1e-8 * abs(want_sum)
+        assert abs(B @ sig - want_sig) <= 1e-8 * abs(want_sig)
+    want = oracle.akm_exact_leverages(st)
+    for got, w in zip((P, Bfe, Bpe, Bcov), want):
+        assert np.abs(got - w).max() <= 1e-8 * max(1.0, np.abs(w).max())
+
+
 def test_akm_jla_bii_matches_reference(golden):
     """xfe_akm_jla_bii (50 draws, seed 42: batches of 16 + 16 + 16 + 2)
     against the oracle's weights per row, and against the reference: its AGSU

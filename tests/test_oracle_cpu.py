@@ -1128,3 +1128,68 @@ def test_boot_gram_exact_pins():
     Vt, info = oracle.absorb(V[keep], [gw, gf], [int(gw.max()) + 1, int(gf.max()) + 1], weights=wts, tol=1e-14)
     Ga = (Vt * wts[:, None]).T @ Vt
     assert np.abs(G - Ga).max() <= 1e-9 * np.abs(G).max()
+
+
+def _exact_fixture():
+    import os
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "exact_leverages.npz"))
+
+
+def test_exact_leverages_match_reference():
+    """The oracle's exact leverages (one dense two-way solve per match) equal
+    the reference's exact path (LeverageMethod::Exact, fixture): P_ii to
+    rounding; the B_ii weights through the components they enter, AGSU =
+    plug-in - sigma2_ho sum(B) / dof and KSS = plug-in - sum(B sigma_i) / dof;
+    and sum P_ii = N + J - 1, the rank of the match-level design."""
+    fx = _exact_fixture()
+    wu, wi = np.unique(fx["row_worker"], return_inverse=True)
+    fu, fi = np.unique(fx["row_firm"], return_inverse=True)
+    st = oracle.akm_build(wi, fi, fx["row_weight"], wu.size, fu.size)
+    P, Bfe, Bpe, Bcov = oracle.akm_exact_leverages(st)
+    assert np.abs(P - fx["pii"]).max() <= 1e-12
+    assert abs(P.sum() - (st["N"] + st["J"] - 1)) <= 1e-9
+    comp, sig = fx["components"], fx["sigma_i"]
+    dof = comp[10] - 1.0
+    for i, B in enumerate((Bpe, Bfe, Bcov)):
+        want_sum = (comp[i] - comp[3 + i]) * dof / comp[9]
+        want_sig = (comp[i] - comp[6 + i]) * dof
+        assert abs(B.sum() - want_sum) <= 1e-12 * abs(want_sum)
+        assert abs(B @ sig - want_sig) <= 1e-12 * abs(want_sig)
+
+
+def test_exact_leverages_brute_force():
+    """P_ii is the diagonal of the hat matrix A (A'A)^+ A' of the match-level
+    design (rows sqrt(c)(e_w + e_f)), and Bfe the c-scaled D_f-weighted
+    variance of the firm part of (A'A)^+ x_m, both from a pseudo-inverse of
+    the full normal matrix (no grounding: the weighted variance does not see
+    the shift between the worker and firm parts)."""
+    rng = np.random.default_rng(5)
+    N, J = 30, 8
+    # a connected panel: a chain through the firms, then random extra matches
+    mw = list(range(N))
+    mf = [w % J for w in range(N)]
+    for f in range(J - 1):
+        mw.append(f)
+        mf.append(f + 1)
+    for _ in range(40):
+        mw.append(int(rng.integers(0, N)))
+        mf.append(int(rng.integers(0, J)))
+    key = np.unique(np.array(mw, dtype=np.int64) * J + np.array(mf))
+    mw, mf = (key // J).astype(np.int32), (key % J).astype(np.int32)
+    c = rng.integers(1, 5, mw.size).astype(np.float64)
+    st = oracle.akm_build(mw, mf, c, N, J)
+    A = np.zeros((mw.size, N + J))
+    sq = np.sqrt(c)
+    A[np.arange(mw.size), mw] = sq
+    A[np.arange(mw.size), N + mf] = sq
+    Kp = np.linalg.pinv(A.T @ A)
+    P, Bfe, Bpe, Bcov = oracle.akm_exact_leverages(st)
+    assert np.allclose(P, np.diag(A @ Kp @ A.T), atol=1e-10)
+    n_py = c.sum()
+    for m in range(mw.size):
+        e = np.zeros(N + J)
+        e[mw[m]] = 1.0
+        e[N + mf[m]] = 1.0
+        zf = (Kp @ e)[N:]
+        want = c[m] * (st["Df"] @ zf ** 2 - (st["Df"] @ zf) ** 2 / n_py)
+        assert abs(Bfe[m] - want) <= 1e-9 * max(1.0, abs(want))

@@ -475,6 +475,89 @@ jla_accumulate_kernel(int M, const int* mw, const int* mf, const int* orig, cons
 }
 
 // pii = P / (P + M)
+// ---- exact leverages (src/akm_kss.cpp:2667-2741): per match m, one lane:
+// K [zw; zf] = [e_w; e_f] through the firm Schur complement
+
+// the firm right-hand side e_f - B' Dw^-1 e_w of lane l (match m0 + l), and
+// the lane's worker and firm
+__global__ void __launch_bounds__(kBlock)
+exact_rhs_kernel(int nb, int m0, const int* mlist, const int* rinv, const int* rw, const int* wofs, const int* r1,
+                 const double* rcv, const double* idw, int L, double* b, int* lane_w, int* lane_f) {
+    for (int l = blockIdx.x * kBlock + threadIdx.x; l < nb; l += gridDim.x * kBlock) {
+        const int a = rinv[mlist ? mlist[l] : m0 + l];
+        const int w = rw[a], f = r1[a];
+        for (int a2 = wofs[w]; a2 < wofs[w + 1]; ++a2) b[static_cast<size_t>(r1[a2]) * L + l] -= rcv[a2] * idw[w];
+        b[static_cast<size_t>(f) * L + l] += 1.0;
+        lane_w[l] = w;
+        lane_f[l] = f;
+    }
+}
+
+// per block and lane: sum Df zf^2, sum Df zf over the firms (kBlock % L == 0)
+__global__ void __launch_bounds__(kBlock)
+exact_firm_kernel(const double* zf, const double* df, int J, int L, double* part) {
+    __shared__ double r1s[kBlock], r2s[kBlock];
+    const int l = threadIdx.x % L, rows = kBlock / L;
+    double a1 = 0.0, a2 = 0.0;
+    for (long long f = static_cast<long long>(blockIdx.x) * rows + threadIdx.x / L; f < J;
+         f += static_cast<long long>(gridDim.x) * rows) {
+        const double z = zf[static_cast<size_t>(f) * L + l];
+        a1 += df[f] * z * z;
+        a2 += df[f] * z;
+    }
+    r1s[threadIdx.x] = a1;
+    r2s[threadIdx.x] = a2;
+    __syncthreads();
+    if (threadIdx.x < L) {
+        double t1 = 0.0, t2 = 0.0;
+        for (int k = 0; k < rows; ++k) {
+            t1 += r1s[k * L + threadIdx.x];
+            t2 += r2s[k * L + threadIdx.x];
+        }
+        part[(static_cast<size_t>(blockIdx.x) * 5 + 0) * L + threadIdx.x] = t1;
+        part[(static_cast<size_t>(blockIdx.x) * 5 + 1) * L + threadIdx.x] = t2;
+    }
+}
+
+// per block and lane over the workers: zw = (e_w - B zf) / Dw, then sum Dw zw^2,
+// sum Dw zw and sum_m c_m zw[w_m] zf[f_m]; the lane's own P base zw[w] + zf[f]
+__global__ void __launch_bounds__(kBlock)
+exact_worker_kernel(const double* zf, const int* wofs, const int* r1, const double* rcv, const double* idw,
+                    const double* dwv, const int* lane_w, const int* lane_f, int N, int L, int nb, double* part,
+                    double* base_p) {
+    __shared__ double s3[kBlock], s4[kBlock], sc[kBlock];
+    const int l = threadIdx.x % L, rows = kBlock / L;
+    double a3 = 0.0, a4 = 0.0, ac = 0.0;
+    if (l < nb) {
+        const int lw = lane_w[l];
+        for (long long w = static_cast<long long>(blockIdx.x) * rows + threadIdx.x / L; w < N;
+             w += static_cast<long long>(gridDim.x) * rows) {
+            double bz = 0.0;
+            for (int a = wofs[w]; a < wofs[w + 1]; ++a) bz += rcv[a] * zf[static_cast<size_t>(r1[a]) * L + l];
+            const double zw = ((w == lw ? 1.0 : 0.0) - bz) * idw[w];
+            a3 += dwv[w] * zw * zw;
+            a4 += dwv[w] * zw;
+            ac += zw * bz;
+            if (w == lw) base_p[l] = zw + zf[static_cast<size_t>(lane_f[l]) * L + l];
+        }
+    }
+    s3[threadIdx.x] = a3;
+    s4[threadIdx.x] = a4;
+    sc[threadIdx.x] = ac;
+    __syncthreads();
+    if (threadIdx.x < L) {
+        double t3 = 0.0, t4 = 0.0, tc = 0.0;
+        for (int k = 0; k < rows; ++k) {
+            t3 += s3[k * L + threadIdx.x];
+            t4 += s4[k * L + threadIdx.x];
+            tc += sc[k * L + threadIdx.x];
+        }
+        part[(static_cast<size_t>(blockIdx.x) * 5 + 2) * L + threadIdx.x] = t3;
+        part[(static_cast<size_t>(blockIdx.x) * 5 + 3) * L + threadIdx.x] = t4;
+        part[(static_cast<size_t>(blockIdx.x) * 5 + 4) * L + threadIdx.x] = tc;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) jla_ratio_kernel(int M, const double* pacc, const double* macc, double* pii) {
     for (int m = blockIdx.x * kBlock + threadIdx.x; m < M; m += gridDim.x * kBlock) {
         const double den = pacc[m] + macc[m];
@@ -616,11 +699,18 @@ struct xfe_akm_context {
     int h_pack_lanes = 0;
     double* h_small = nullptr;  // pinned scalars
     AkmWide wide;               // the 256-lane FP32 / FP64-replacement solver
+    // exact leverages: the record of each match, Dw (new worker order) and Df,
+    // per-lane worker / firm, partial sums
+    DevBuf<int> rinv, lane_w, lane_f, lane_m;
+    DevBuf<double> dwv, dfv, ex_part, ex_bp;
+    double n_py = 0.0;
 
     ~xfe_akm_context() {
         wide.release();
-        for (DevBuf<int>* d : {&wofs, &rorig, &forig, &wpiece, &rw, &r1, &fkey, &fw, &has, &bk[0], &bk[1]})
+        for (DevBuf<int>* d : {&wofs, &rorig, &forig, &wpiece, &rw, &r1, &fkey, &fw, &has, &bk[0], &bk[1], &rinv,
+                               &lane_w, &lane_f, &lane_m})
             d->release();
+        for (DevBuf<double>* d : {&dwv, &dfv, &ex_part, &ex_bp}) d->release();
         for (DevBuf<double>* d : {&rsq, &rcv, &idw, &jtw, &pacc, &macc, &pii, &jsu, &bfe, &bpe, &bcov, &rwt, &fc,
                                   &inv_diag, &corr, &wmean, &lanes, &b, &x, &r, &z, &p, &q, &cg, &part, &scale,
                                   &bv[0], &bv[1]})
@@ -956,6 +1046,19 @@ xfe_akm_context* xfe_akm_create(int n_workers, int n_firms, const int* m_w, cons
         up_i(c->has, has);
         up_d(c->inv_diag, inv_diagS, J);
         up_d(c->corr, corr.data(), J);
+        {
+            std::vector<int> rinv(M);
+            std::vector<double> dwv(N);
+            for (int a = 0; a < M; ++a) rinv[rorig[a]] = a;
+            for (int k = 0; k < N; ++k) dwv[k] = Dw[order[k]];
+            up_i(c->rinv, rinv);
+            up_d(c->dwv, dwv.data(), N);
+            up_d(c->dfv, Df, J);
+            double npy = 0.0;
+            for (int a = 0; a < M; ++a) npy += m_c[a];
+            c->n_py = npy;
+            XFE_CHECK(hipStreamSynchronize(c->s));
+        }
         c->wmean.ensure(static_cast<size_t>(N) * kMaxCgCols);
         const size_t nb2 = 2 * ((static_cast<size_t>(M) + kPiece - 1) / kPiece);
         for (int k = 0; k < 2; ++k) {
@@ -1162,6 +1265,123 @@ int xfe_akm_solve_S_wide_device(xfe_akm_context* c, const double* d_rhs_pack, do
         XFE_CHECK(hipDeviceSynchronize());
         c->wide.solve(d_rhs_pack, d_z_pack, nb, tol, max_iter, iters_out, true);
         return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+static thread_local int last_exact_retries = 0;
+
+int xfe_akm_exact_leverages_last_retries(void) { return last_exact_retries; }
+
+int xfe_akm_exact_leverages(xfe_akm_context* c, double tol, double accept, int max_iter, double* pii, double* bfe,
+                            double* bpe, double* bcov) {
+    clear_error();
+    if (!c || !pii || !bfe || !bpe || !bcov || max_iter < 1 || !(tol >= 0.0) || !(accept >= 0.0)) {
+        set_error("xfe_akm_exact_leverages: invalid arguments");
+        return -1;
+    }
+    try {
+        (void)hipGetLastError();
+        const int M = c->M, N = c->N, J = c->J;
+        hipStream_t s = c->s;
+        AkmWide& wd = c->wide;
+        std::vector<double> h, bp;
+        std::vector<double> mc(M);
+        // the match weights in the original order (for the match-level scaling)
+        {
+            std::vector<int> ri(M);
+            std::vector<double> rc(M);
+            XFE_CHECK(hipMemcpyAsync(ri.data(), c->rinv.p, sizeof(int) * M, hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipMemcpyAsync(rc.data(), c->rcv.p, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+            for (int m = 0; m < M; ++m) mc[m] = rc[ri[m]];
+        }
+        // the sums over firms and workers of the solutions zf (J x L) of nb
+        // lanes (lane l: match ms[l]) into the outputs
+        auto finish = [&](const double* zf, int nb, int L, const std::vector<int>& ms) {
+            const int rows = kBlock / L;
+            const int gf = std::max(1, std::min(1024, (J + rows - 1) / rows));
+            const int gw = std::max(1, std::min(1024, (N + rows - 1) / rows));
+            const int g = std::max(gf, gw);
+            c->ex_part.ensure(static_cast<size_t>(g) * 5 * L);
+            XFE_CHECK(hipMemsetAsync(c->ex_part.p, 0, sizeof(double) * static_cast<size_t>(g) * 5 * L, s));
+            hipLaunchKernelGGL(exact_firm_kernel, dim3(gf), dim3(kBlock), 0, s, zf, c->dfv.p, J, L, c->ex_part.p);
+            hipLaunchKernelGGL(exact_worker_kernel, dim3(gw), dim3(kBlock), 0, s, zf, c->wofs.p, c->r1.p, c->rcv.p,
+                               c->idw.p, c->dwv.p, c->lane_w.p, c->lane_f.p, N, L, nb, c->ex_part.p, c->ex_bp.p);
+            XFE_CHECK(hipGetLastError());
+            h.resize(static_cast<size_t>(g) * 5 * L);
+            bp.resize(L);
+            XFE_CHECK(hipMemcpyAsync(h.data(), c->ex_part.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipMemcpyAsync(bp.data(), c->ex_bp.p, sizeof(double) * L, hipMemcpyDeviceToHost, s));
+            XFE_CHECK(hipStreamSynchronize(s));
+            for (int l = 0; l < nb; ++l) {
+                double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                for (int b = 0; b < g; ++b)
+                    for (int k = 0; k < 5; ++k) t[k] += h[(static_cast<size_t>(b) * 5 + k) * L + l];
+                const double npy = c->n_py;
+                const int m = ms[l];
+                const double cm = mc[m];
+                pii[m] = cm * bp[l];
+                bfe[m] = cm * (t[0] - t[1] * t[1] / npy);
+                bpe[m] = cm * (t[2] - t[3] * t[3] / npy);
+                bcov[m] = cm * (t[4] - t[3] * t[1] / npy);
+            }
+        };
+        // 256 matches per wide solve (FP32 passes, FP64 residual replacement)
+        std::vector<int> fails, ms;
+        for (int m0 = 0; m0 < M; m0 += kWideMaxLanes) {
+            const int nb = std::min(kWideMaxLanes, M - m0);
+            wd.ensure(nb);
+            const int L = wd.L;
+            c->lane_w.ensure(kWideMaxLanes);
+            c->lane_f.ensure(kWideMaxLanes);
+            c->ex_bp.ensure(kWideMaxLanes);
+            XFE_CHECK(hipMemsetAsync(wd.b64.p, 0, sizeof(double) * static_cast<size_t>(J) * L, s));
+            hipLaunchKernelGGL(exact_rhs_kernel, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nb, m0,
+                               static_cast<const int*>(nullptr), c->rinv.p, c->rw.p, c->wofs.p, c->r1.p, c->rcv.p,
+                               c->idw.p, L, wd.b64.p, c->lane_w.p, c->lane_f.p);
+            XFE_CHECK(hipGetLastError());
+            XFE_CHECK(hipMemsetAsync(wd.b64.p + static_cast<size_t>(c->ground) * L, 0, sizeof(double) * L, s));
+            wd.solve_device(nb, tol, max_iter, accept);
+            wd.unscale_x64();
+            ms.resize(nb);
+            for (int l = 0; l < nb; ++l) {
+                ms[l] = m0 + l;
+                if (wd.lane_iterations(l) < 0) fails.push_back(m0 + l);
+            }
+            finish(wd.x64.p, nb, L, ms);
+        }
+        // the matches whose wide solve did not converge again, 32 at a time,
+        // with the FP64 batched Jacobi-PCG
+        int failed = 0;
+        constexpr int kL = kMaxCgCols;
+        c->lane_m.ensure(kL);
+        for (size_t k0 = 0; k0 < fails.size(); k0 += kL) {
+            const int nb = static_cast<int>(std::min<size_t>(kL, fails.size() - k0));
+            ms.assign(fails.begin() + static_cast<long>(k0), fails.begin() + static_cast<long>(k0) + nb);
+            XFE_CHECK(hipMemcpyAsync(c->lane_m.p, ms.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+            c->ensure_lanes(kL);
+            XFE_CHECK(hipMemsetAsync(c->b.p, 0, sizeof(double) * static_cast<size_t>(J) * kL, s));
+            hipLaunchKernelGGL(exact_rhs_kernel, dim3(1), dim3(kBlock), 0, s, nb, 0,
+                               static_cast<const int*>(c->lane_m.p), c->rinv.p, c->rw.p, c->wofs.p, c->r1.p,
+                               c->rcv.p, c->idw.p, kL, c->b.p, c->lane_w.p, c->lane_f.p);
+            XFE_CHECK(hipGetLastError());
+            XFE_CHECK(hipMemsetAsync(c->b.p + static_cast<size_t>(c->ground) * kL, 0, sizeof(double) * kL, s));
+            c->solve_levels(kL, tol, max_iter);
+            int its[kL];
+            c->lane_iterations(kL, its);
+            for (int l = 0; l < nb; ++l)
+                if (its[l] < 0) ++failed;
+            const long long JL = static_cast<long long>(J) * kL;
+            hipLaunchKernelGGL(jla_unscale_kernel, dim3(akm_grid(JL)), dim3(kBlock), 0, s, c->x.p, J, kL, c->ground,
+                               c->scale.p + kMaxCgCols, c->cg.p, c->lanes.p);
+            XFE_CHECK(hipGetLastError());
+            finish(c->lanes.p, nb, kL, ms);
+        }
+        last_exact_retries = static_cast<int>(fails.size());
+        return failed;
     } catch (const std::exception& e) {
         set_error(e.what());
         return -1;

@@ -73,7 +73,7 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_abso
            "xfe_akm_stable_sort_u64", "xfe_akm_create", "xfe_akm_destroy", "xfe_akm_solve_S",
            "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_solve_S_wide", "xfe_akm_solve_S_wide_device",
            "xfe_akm_wide_max_lanes", "xfe_boot_create", "xfe_boot_destroy", "xfe_boot_refits", "xfe_boot_last",
-           "xfe_akm_wide_last", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
+           "xfe_akm_wide_last", "xfe_akm_exact_leverages", "xfe_akm_exact_leverages_last_retries", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
            "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
            "xfe_absorb_group_individual",
            "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort",
@@ -604,6 +604,21 @@ class AkmSolver:
         passes, upd = ctypes.c_int(0), ctypes.c_int(0)
         L.xfe_akm_wide_last(_p(self._ctx), ctypes.byref(passes), ctypes.byref(upd))
         return its, (passes.value, upd.value)
+
+    def exact_leverages(self, tol=1e-10, max_iter=10000, accept=0.0):
+        """xfe_akm_exact_leverages: the exact P_ii and B_ii (fe, pe, cov) of
+        the matches (in the given order), one solve per match, 256 per wide
+        solve; returns (pii, bfe, bpe, bcov, number of unconverged solves)."""
+        M = self._keep["m_w"].size
+        out = [np.empty(M) for _ in range(4)]
+        L = lib()
+        L.xfe_akm_exact_leverages.argtypes = [_p, _d, _d, _i, _p, _p, _p, _p]
+        rc = L.xfe_akm_exact_leverages(_p(self._ctx), float(tol), float(accept), int(max_iter),
+                                       *[_ptr(o) for o in out])
+        if rc < 0:
+            raise XfeError(last_error())
+        self.exact_retries = int(L.xfe_akm_exact_leverages_last_retries())
+        return out[0], out[1], out[2], out[3], int(rc)
 
     def jla_pii(self, draws=200, seed=20260705, tol=1e-10, max_iter=10000):
         """xfe_akm_jla_pii: JLA leverages of the matches (in the given order)
