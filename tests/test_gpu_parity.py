@@ -327,6 +327,30 @@ def test_fp64_schur_matches_tight_oracle_and_mixed(cases, name):
     assert err <= 1e-9 and err12 <= 1e-9
 
 
+def test_mixed_precision_graph_replays_match_eager(monkeypatch):
+    """The mixed path's inner iterations replayed as HIP graphs (eight per
+    graph, captured per solve, the FP64 replacements eager between the
+    replays) give the iteration and replacement counts of the same launches
+    issued one by one (XFE_NO_GRAPH) and the same result to 1e-12 of the
+    column scale (two eager solves already differ in the last bits, about
+    4e-14: the per-tile FP64 LDS sums of the row passes add in arrival
+    order), on three solves in a row with a plain FP64 absorption between."""
+    _log("mixed precision graph replays")
+    p = panels.akm_small(scale=0.01)
+    _, fes, gids, G, V = _prep(p)
+    monkeypatch.setenv("XFE_NO_GRAPH", "1")
+    y0, X0, i0 = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=1)
+    monkeypatch.delenv("XFE_NO_GRAPH")
+    assert i0["converged"] and i0["precision_used"] == 1
+    for k in range(3):
+        y, X, info = xfe_amd.absorb(V[:, 0], V[:, 1:], gids, G, tol=1e-8, precision=1)
+        _log(f"  solve {k}: iterations {info['iterations']} (eager {i0['iterations']})")
+        assert info["iterations"] == i0["iterations"] and info["refinements"] == i0["refinements"]
+        scale = np.abs(V).max()
+        assert max(_rel(y, y0, scale), _rel(X, X0, scale)) <= 1e-12
+        xfe_amd.absorb(V[:, 0], V[:, 1:2], gids, G, tol=1e-6, precision=0)
+
+
 @pytest.mark.parametrize("which", ["config0", "akm_1pct"])
 def test_mixed_precision_spell_path(which):
     _log(f"mixed precision {which}")

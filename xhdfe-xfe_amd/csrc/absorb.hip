@@ -1552,13 +1552,46 @@ struct FastSolver {
         const int foff = seg ? off[k1] : 0;
         XFE_CHECK(launch_cg_reliable_reset(ws.f_b.p, d_rel, ws.f_minv.p, ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p,
                                            total, C, cg, part, g, blocks ? &bview : nullptr, foff, s));
-        for (int it = 1; it <= max_inner; ++it) {
+        auto one_iteration = [&]() {
             apply32(ws.f_p.p, ws.f_q.p);
             XFE_CHECK(launch_cg_dot<float>(ws.f_p.p, ws.f_q.p, total, C, cg, part, g, s));
             XFE_CHECK(launch_cg_step_rel(ws.f_x.p, ws.f_r.p, ws.f_z.p, ws.f_p.p, ws.f_q.p, ws.f_minv.p, total, C, cg,
                                          part, max_inner + 1, g, blocks ? &bview : nullptr, foff, s));
-            if (probe && inner_total + it == probe->probe_iter)
-                XFE_CHECK(lp::launch_probe_iterate(ws.f_beta.p, ws.f_x.p, d_t, N, C, probe->iter.p, s));
+        };
+        // the kPoll iterations between two polls as one HIP graph (about a
+        // dozen launches each, all on this stream, no host work and no memset
+        // nodes): captured at first use, replayed while the preconditioner is
+        // the same; only with the segment operator when every firm has a
+        // segment (the row pass and the other case clear with a memset), and
+        // not with the probe or the profile
+        struct GraphGuard {
+            hipGraphExec_t e = nullptr;
+            ~GraphGuard() { if (e) (void)hipGraphExecDestroy(e); }
+        } gexec;
+        bool gexec_blocks = false;
+        const bool graphs = seg && !empty_firms && !probe && !g_profile.enabled && !std::getenv("XFE_NO_GRAPH");
+        for (int it = 1; it <= max_inner; ++it) {
+            if (graphs && (it - 1) % kPoll == 0 && it + kPoll - 1 <= max_inner) {
+                if (gexec.e && gexec_blocks != blocks) {
+                    XFE_CHECK(hipGraphExecDestroy(gexec.e));
+                    gexec.e = nullptr;
+                }
+                if (!gexec.e) {
+                    hipGraph_t graph = nullptr;
+                    XFE_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    for (int k = 0; k < kPoll; ++k) one_iteration();
+                    XFE_CHECK(hipStreamEndCapture(s, &graph));
+                    XFE_CHECK(hipGraphInstantiate(&gexec.e, graph, nullptr, nullptr, 0));
+                    XFE_CHECK(hipGraphDestroy(graph));
+                    gexec_blocks = blocks;
+                }
+                XFE_CHECK(hipGraphLaunch(gexec.e, s));
+                it += kPoll - 1;
+            } else {
+                one_iteration();
+                if (probe && inner_total + it == probe->probe_iter)
+                    XFE_CHECK(lp::launch_probe_iterate(ws.f_beta.p, ws.f_x.p, d_t, N, C, probe->iter.p, s));
+            }
             iters = it;
             if (it % kPoll != 0 && it != max_inner) continue;
             XFE_CHECK(hipMemcpyAsync(ws.host_scalars, cg + kCgCount, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
