@@ -1314,7 +1314,12 @@ struct FastSolver {
             // LDS offsets were laid out for full 8-wide chunks; recompute for cc
             int lo = 0;
             for (int k = 0; k < NR; ++k) {
-                if (mode[k] == kRedLds) { a.lds_off[k] = lo; lo += sv.G[rdim[k]] * cc; }
+                if (mode[k] != kRedLds) continue;
+                // one copy per wave when the copies stay within 16 KB
+                const size_t one = static_cast<size_t>(sv.G[rdim[k]]) * cc * sizeof(T);
+                a.lds_copies[k] = one * (kBlock / kWave) <= 16 * 1024 ? kBlock / kWave : 1;
+                a.lds_off[k] = lo;
+                lo += sv.G[rdim[k]] * cc * a.lds_copies[k];
             }
             const size_t extra = static_cast<size_t>(lo) * sizeof(T);
             (void)lds_extra;

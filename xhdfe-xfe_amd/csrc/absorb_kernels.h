@@ -116,6 +116,7 @@ struct SchurArgs {
     const int* spell_base[2] = {nullptr, nullptr};
     T* lds_target[2] = {nullptr, nullptr};
     int lds_off[2] = {0, 0};
+    int lds_copies[2] = {1, 1};               // LDS-mode sums: copies (one per wave when they fit)
     double* partials = nullptr;               // STORE: per-block sum of squares
     T* p_sums = nullptr;                      // STORE: per level of p, the sums of the stored rows (G_p x C)
 };

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(kBlock) schur_kernel(SchurArgs<T> a) {
     const int wave = threadIdx.x / kWave;
     for (int k = 0; k < NR; ++k) {
         if (a.mode_r[k] == kRedLds) {
-            for (int j = threadIdx.x; j < a.G_r[k] * CC; j += kBlock) lr[a.lds_off[k] + j] = T(0);
+            for (int j = threadIdx.x; j < a.G_r[k] * CC * a.lds_copies[k]; j += kBlock) lr[a.lds_off[k] + j] = T(0);
         }
     }
     double q = 0.0;
@@ -219,16 +219,16 @@ __global__ void __launch_bounds__(kBlock) schur_kernel(SchurArgs<T> a) {
                 }
                 __syncthreads();
             } else {
-                T* l = lr + a.lds_off[k];
+                // a small dimension (years): each row adds to its wave's copy of
+                // the level table (rows of one worker are consecutive years, so
+                // runs of one level are a row long and a segmented sum gains
+                // nothing)
+                T* l = lr + a.lds_off[k] + static_cast<size_t>(wave % a.lds_copies[k]) * a.G_r[k] * CC;
 #pragma unroll
                 for (int j = 0; j < kSlots; ++j) {
-                    const int end = run_end_lane(kr[k][j]);
-                    const bool seg = run_head(kr[k][j]) && valid[j];
+                    if (!valid[j]) continue;
 #pragma unroll
-                    for (int c = 0; c < CC; ++c) {
-                        const T s = run_sum_t(u[j][c], end);
-                        if (seg) atomicAdd(&l[kr[k][j] * CC + c], s);
-                    }
+                    for (int c = 0; c < CC; ++c) atomicAdd(&l[kr[k][j] * CC + c], u[j][c]);
                 }
             }
         }
@@ -237,8 +237,10 @@ __global__ void __launch_bounds__(kBlock) schur_kernel(SchurArgs<T> a) {
     for (int k = 0; k < NR; ++k) {
         if (a.mode_r[k] != kRedLds) continue;
         const T* l = lr + a.lds_off[k];
-        for (int j = threadIdx.x; j < a.G_r[k] * CC; j += kBlock) {
-            const T s = l[j];
+        const int GC = a.G_r[k] * CC;
+        for (int j = threadIdx.x; j < GC; j += kBlock) {
+            T s = T(0);
+            for (int q = 0; q < a.lds_copies[k]; ++q) s += l[static_cast<size_t>(q) * GC + j];
             if (s != T(0)) atomicAdd(&a.lds_target[k][static_cast<size_t>(j / CC) * a.C + a.c0 + (j % CC)], s);
         }
     }
