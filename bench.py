@@ -239,19 +239,32 @@ def bench_akm(args):
         f"{bcov.sum():.4g}")
     solver.close()
     # exact leverages (the reference's LeverageMethod::Exact: one solve per
-    # match, here 256 matches per wide solve) against the 200 JLA draws on a
-    # 200k-obs panel of the same shape (24k workers, 3k firms)
+    # match, here 256 matches per wide solve) and the KSS decomposition built
+    # on them, against the 200 JLA draws, on the leave-out connected set (the
+    # device's xfe_akm_leave_out_set) of a 200k-obs panel of the same shape
+    # (24k workers, 3k firms); y = alpha + psi + N(0, 1), synthetic
     ps = panels.kss_panel(workers=24_000, firms=3_000)
-    mw2, mf2, mc2, N2, J2 = akm.panel_matches(ps["worker"], ps["firm"])
-    s2 = xfe_amd.AkmSolver(akm.build_structure(mw2, mf2, mc2, N2, J2))
+    keep, _ = xfe_amd.leave_out_set(ps["worker"], ps["firm"])
+    keep = np.asarray(keep, dtype=bool)
+    wk, fk = ps["worker"][keep], ps["firm"][keep]
+    rng_y = np.random.default_rng(7)
+    yk = (rng_y.normal(size=int(wk.max()) + 1)[wk] + rng_y.normal(0.0, 0.3, size=int(fk.max()) + 1)[fk]
+          + rng_y.normal(size=wk.size))
     t0 = time.perf_counter()
-    ex_p, ex_fe, ex_pe, ex_cov, ex_failed = s2.exact_leverages(tol=1e-10, max_iter=max_iter, accept=1e-8)
+    dec = akm.kss_components_exact(yk, wk, fk, tol=1e-10, max_iter=max_iter, accept=1e-8)
     exact_s = time.perf_counter() - t0
+    ex_p, ex_fe, ex_pe, ex_cov = dec["pii"], dec["bfe"], dec["bpe"], dec["bcov"]
+    _, _, _, _, _, mw2, mf2, mc2 = akm.match_panel(wk, fk)
+    N2, J2 = int(mw2.max()) + 1, int(mf2.max()) + 1
+    s2 = xfe_amd.AkmSolver(akm.build_structure(mw2, mf2, mc2, N2, J2))
     jp2, _ = s2.jla_pii(draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
     jfe, jpe, jcov, _ = s2.jla_bii(None, draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
     exact_cmp = {
         "obs": int(ps["worker"].size), "workers": int(N2), "firms": int(J2), "matches": int(mw2.size),
-        "exact_s": round(exact_s, 3), "unconverged_solves": ex_failed, "fp64_retries": s2.exact_retries,
+        "rows_leave_out_set": int(wk.size),
+        "exact_s": round(exact_s, 3), "seconds": {k: round(v, 3) for k, v in dec["seconds"].items()},
+        "kss": [float(v) for v in dec["kss"]], "plugin": [float(v) for v in dec["plugin"]],
+        "unit_leverage_rows": dec["unit_leverage"],
         "tol": 1e-10, "accept_on_stall": 1e-8,
         "sum_pii_minus_rank": float(ex_p.sum() - (N2 + J2 - 1)),
         "jla200_pii_mean_abs_err": float(np.abs(jp2 - ex_p).mean()),
@@ -260,8 +273,8 @@ def bench_akm(args):
                                  for j, e in ((jfe, ex_fe), (jpe, ex_pe), (jcov, ex_cov))],
     }
     s2.close()
-    log(f"exact leverages, {mw2.size} matches ({ps['worker'].size} obs): {exact_s:.2f} s, unconverged {ex_failed}, "
-        f"FP64 retries {s2.exact_retries}; "
+    log(f"KSS decomposition with exact leverages, {mw2.size} matches ({wk.size} obs of {ps['worker'].size} in "
+        f"the leave-out set): {exact_s:.2f} s {exact_cmp['seconds']}; KSS {exact_cmp['kss']}; "
         f"JLA-200 P_ii mean abs err {exact_cmp['jla200_pii_mean_abs_err']:.3e}, sum B rel err "
         f"{exact_cmp['jla200_sum_b_rel_err']}")
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0

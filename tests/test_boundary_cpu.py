@@ -129,3 +129,36 @@ def test_adapters_compile_against_reference_headers():
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "xhdfe-xfe_amd", "adapter"), "check"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+def test_kss_sums_match_reference_with_dense_leverages():
+    """The host's O(rows) variance-component sums (xfe_amd.akm.kss_sums), fed
+    with the oracle's dense two-way fit and exact leverages, against the
+    reference's akm_kss_decompose with LeverageMethod::Exact (fixture): the
+    plug-in, AGSU and KSS components, sigma2_ho, n_py and sigma_i per match.
+    With the device's fit and leverages the same sums are the GPU test's."""
+    import oracle
+    from golden_cases import jla_panel
+    from xfe_amd import akm
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "exact_leverages.npz"))
+    jp = jla_panel()
+    y = jp["y"]
+    wu, fu, wi, fi, ridx, mw, mf, c = akm.match_panel(jp["worker"], jp["firm"])
+    N, J = wu.size, fu.size
+    st = oracle.akm_build(mw, mf, c, N, J)
+    rhs, _, tw = akm.two_way_rhs(y, ridx, mw, mf, c, N, J, st["Dw"])
+    S = np.column_stack([oracle.akm_apply_S(st, np.eye(J)[f]) for f in range(J)])
+    psi = np.zeros(J)
+    psi[:J - 1] = np.linalg.solve(S[:J - 1, :J - 1], rhs[:J - 1])
+    alpha = (tw - np.bincount(mw, weights=c * psi[mf], minlength=N)) / st["Dw"]
+    r = akm.kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, *oracle.akm_exact_leverages(st))
+    got = list(r["plugin"]) + list(r["agsu"]) + list(r["kss"]) + [r["sigma2_ho"], r["n_py"]]
+    comp = fx["components"]
+    for i, (g, w) in enumerate(zip(got, comp)):
+        assert abs(g - w) <= 1e-10 * max(abs(w), abs(comp[i % 3]) if 3 <= i < 9 else 0.0), (i, g, w)
+    key = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(wu[mw], fu[mf]))}
+    order = np.array([key[(int(a), int(b))] for a, b in zip(fx["row_worker"], fx["row_firm"])])
+    assert order.size == mw.size
+    np.testing.assert_array_equal(c[order], fx["row_weight"])
+    assert np.abs(r["sigma_i"][order] - fx["sigma_i"]).max() <= 1e-12 * np.abs(fx["sigma_i"]).max()
+    assert r["unit_leverage"] == 0

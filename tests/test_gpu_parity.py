@@ -666,6 +666,39 @@ def test_akm_exact_leverages_match_reference():
         assert np.abs(got - w).max() <= 1e-8 * max(1.0, np.abs(w).max())
 
 
+def test_kss_components_exact_match_reference():
+    """The plug-in, AGSU and KSS variance components with exact leverages
+    (the device's two-way fit and exact leverages, the O(rows) sums on the
+    host) against the reference's akm_kss_decompose with LeverageMethod::Exact
+    on the same panel (fixture): every component and sigma2_ho to 1e-8
+    relative (the corrected ones relative to their plug-in), sigma_i per match
+    to 1e-8 of its scale."""
+    import os
+    from golden_cases import jla_panel
+    from xfe_amd import akm
+    _log("kss components (exact)")
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "exact_leverages.npz"))
+    jp = jla_panel()
+    r = akm.kss_components_exact(jp["y"], jp["worker"], jp["firm"])
+    comp = fx["components"]
+    got = list(r["plugin"]) + list(r["agsu"]) + list(r["kss"]) + [r["sigma2_ho"], r["n_py"]]
+    # a corrected component is the plug-in less a correction of nearly its
+    # size: its error is measured against the plug-in's magnitude
+    scale = [abs(w) for w in comp]
+    for i in range(3, 9):
+        scale[i] = max(scale[i], abs(comp[i % 3]))
+    err = max(abs(g - w) / s for g, w, s in zip(got, comp, scale))
+    _log(f"  components vs reference: max rel {err:.2e}")
+    assert err <= 1e-8, (got, comp)
+    assert r["unit_leverage"] == 0
+    # sigma_i in the reference's working-row order (its rows are matches)
+    key = {(int(w), int(f)): k for k, (w, f) in enumerate(zip(r["match_worker"], r["match_firm"]))}
+    order = np.array([key[(int(w), int(f))] for w, f in zip(fx["row_worker"], fx["row_firm"])])
+    sig = r["sigma_i"][order]
+    assert np.abs(sig - fx["sigma_i"]).max() <= 1e-8 * np.abs(fx["sigma_i"]).max()
+    assert np.abs(r["pii"][order] - fx["pii"]).max() <= 1e-8
+
+
 def test_akm_jla_bii_matches_reference(golden):
     """xfe_akm_jla_bii (50 draws, seed 42: batches of 16 + 16 + 16 + 2)
     against the oracle's weights per row, and against the reference: its AGSU

@@ -54,3 +54,122 @@ def panel_matches(worker, firm, connected=True):
     mw = (uk // J).astype(np.int32)
     mf = (uk % J).astype(np.int32)
     return mw, mf, counts.astype(np.float64), int(N), int(J)
+
+
+
+def match_panel(worker, firm):
+    """Panel rows -> matches on compact 0-based ids, without pruning: (wu, fu,
+    wi, fi, ridx, mw, mf, c) with wu / fu the original ids of the compact
+    workers / firms, wi / fi the rows' compact ids, ridx each row's match,
+    mw / mf / c the matches grouped by worker with firms ascending and their
+    person-year counts."""
+    wu, wi = np.unique(np.asarray(worker), return_inverse=True)
+    fu, fi = np.unique(np.asarray(firm), return_inverse=True)
+    J = fu.size
+    key = wi.astype(np.int64) * J + fi
+    uk, ridx, counts = np.unique(key, return_inverse=True, return_counts=True)
+    return (wu, fu, wi, fi, ridx, (uk // J).astype(np.int32), (uk % J).astype(np.int32),
+            counts.astype(np.float64))
+
+
+def two_way_rhs(y, ridx, mw, mf, c, N, J, Dw):
+    """Right-hand side of the firm Schur system of the two-way fit
+    K [alpha; psi] = [W'y; F'y]: F'y - B' Dw^-1 W'y, with the per-match sums
+    of y and W'y (rhs, m_ysum, tw)."""
+    m_ysum = np.bincount(ridx, weights=y, minlength=mw.size)
+    tw = np.bincount(mw, weights=m_ysum, minlength=N)
+    tf = np.bincount(mf, weights=m_ysum, minlength=J)
+    return tf - np.bincount(mf, weights=c * (tw / Dw)[mw], minlength=J), m_ysum, tw
+
+
+def kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, P, Bfe, Bpe, Bcov):
+    """The O(rows) part of the reference's variance decomposition at match
+    level with unweighted rows (src/akm_kss.cpp:2632-2662 for yt / eta / the
+    stayer flags, :3299-3349 for sigma_i, :3353-3410 for the components), from
+    the two-way fit (alpha, psi) and the per-match leverages P and weights
+    (Bfe, Bpe, Bcov):
+
+      yt = sqrt(c) ybar_m, eta = yt - sqrt(c) (alpha_w + psi_f);
+      sigma_i = (yt - mean yt) (eta / M_ii), M_ii = 1 - P_ii, on a mover's match
+      (0 where M_ii <= 1e-12: such a match is counted in unit_leverage); on a
+      stayer's match the mean over its person-years of
+      (y - mean y) e / (1 - 1 / T_w), e = y - alpha - psi, T_w = Dw;
+      plug-in: D-weighted variances and the covariance of alpha and psi over
+      n_py - 1; KSS = plug-in - sum(B sigma) / (n_py - 1);
+      AGSU = plug-in - sigma2_ho sum(B) / (n_py - 1), sigma2_ho =
+      RSS / (n_py - (N + J - 1)) (0 when that is not positive)."""
+    N, J = alpha.size, psi.size
+    M = mw.size
+    Dw = np.bincount(mw, weights=c, minlength=N)
+    Df = np.bincount(mf, weights=c, minlength=J)
+    sc = np.sqrt(c)
+    m_ysum = np.bincount(ridx, weights=y, minlength=M)
+    yt = sc * (m_ysum / c)
+    eta = yt - sc * (alpha[mw] + psi[mf])
+    stayer = np.bincount(mw, minlength=N)[mw] < 2
+    Mii = 1.0 - P
+    sigma = np.zeros(M)
+    mov = ~stayer & (Mii > 1e-12)
+    unit = int(np.count_nonzero(~stayer & ~(Mii > 1e-12)))
+    sigma[mov] = (yt[mov] - yt.mean()) * (eta[mov] / Mii[mov])
+    T = Dw[wi]
+    e = y - alpha[wi] - psi[fi]
+    rows = stayer[ridx]
+    acc = np.bincount(ridx[rows], weights=((y[rows] - y.mean()) * (e[rows] / (1.0 - 1.0 / T[rows]))),
+                      minlength=M)
+    sigma[stayer] = acc[stayer] / c[stayer]
+    n_py = float(c.sum())
+    dof = n_py - 1.0
+    pb = float(Df @ psi) / n_py
+    ab = float(Dw @ alpha) / n_py
+    plugin = (float(Dw @ ((alpha - ab) ** 2)) / dof, float(Df @ ((psi - pb) ** 2)) / dof,
+              float(np.sum(c * (alpha[mw] - ab) * (psi[mf] - pb))) / dof)
+    B = (Bpe, Bfe, Bcov)  # the weights of var_alpha, var_psi, cov_alpha_psi
+    kss = tuple(plugin[i] - float(B[i] @ sigma) / dof for i in range(3))
+    dof_ho = n_py - (N + J - 1)
+    s2 = float(e @ e) / dof_ho if dof_ho > 0 else 0.0
+    agsu = tuple(plugin[i] - s2 * float(B[i].sum()) / dof for i in range(3))
+    return dict(plugin=plugin, agsu=agsu, kss=kss, sigma2_ho=s2, n_py=n_py, pii=P, sigma_i=sigma,
+                unit_leverage=unit)
+
+
+def kss_components_exact(y, worker, firm, tol=1e-11, max_iter=10000, accept=0.0):
+    """The variance decomposition of the reference's akm_kss_decompose with
+    exact leverages (LeverageMethod::Exact, match level, no controls, no
+    weights, no pruning: the panel must be leave-out connected;
+    src/akm_kss.cpp:2551-2741, :3285-3410).  The two-way fit (the grounded
+    firm Schur solve) and the leverages with their quadratic-form weights
+    (xfe_akm_exact_leverages) run on the device; kss_sums forms the O(rows)
+    sums.  accept: as in AkmSolver.exact_leverages (a leverage solve whose
+    FP64 residual stalls at or below it counts as converged).  Returns
+    kss_sums' dict (plugin, agsu, kss as (var_alpha, var_psi, cov_alpha_psi);
+    sigma2_ho; n_py; per match P_ii and sigma_i) with the matches' original
+    ids (match_worker, match_firm), counts, the weights bfe / bpe / bcov and
+    the seconds of the fit, the leverages and the sums (seconds)."""
+    import time
+    from . import AkmSolver, XfeError
+    y = np.asarray(y, dtype=np.float64)
+    wu, fu, wi, fi, ridx, mw, mf, c = match_panel(worker, firm)
+    N, J = wu.size, fu.size
+    st = build_structure(mw, mf, c, N, J)
+    t0 = time.perf_counter()
+    rhs, _, tw = two_way_rhs(y, ridx, mw, mf, c, N, J, st["Dw"])
+    solver = AkmSolver(st)
+    try:
+        z, its = solver.solve_multi(rhs[None, :], tol=tol, max_iter=max_iter)
+        if its[0] < 0:
+            raise XfeError("kss_components_exact: the two-way fit did not converge")
+        t1 = time.perf_counter()
+        P, Bfe, Bpe, Bcov, failed = solver.exact_leverages(tol=tol, max_iter=max_iter, accept=accept)
+        if failed:
+            raise XfeError(f"kss_components_exact: {failed} leverage solves did not converge")
+        t2 = time.perf_counter()
+    finally:
+        solver.close()
+    psi = np.asarray(z[0], dtype=np.float64)
+    alpha = (tw - np.bincount(mw, weights=c * psi[mf], minlength=N)) / st["Dw"]
+    r = kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, P, Bfe, Bpe, Bcov)
+    t3 = time.perf_counter()
+    r.update(match_worker=wu[mw], match_firm=fu[mf], match_count=c, bfe=Bfe, bpe=Bpe, bcov=Bcov,
+             seconds=dict(fit=t1 - t0, leverages=t2 - t1, sums=t3 - t2))
+    return r
