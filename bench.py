@@ -254,17 +254,20 @@ def bench_akm(args):
     dec = akm.kss_components_exact(yk, wk, fk, tol=1e-10, max_iter=max_iter, accept=1e-8)
     exact_s = time.perf_counter() - t0
     ex_p, ex_fe, ex_pe, ex_cov = dec["pii"], dec["bfe"], dec["bpe"], dec["bcov"]
-    _, _, _, _, _, mw2, mf2, mc2 = akm.match_panel(wk, fk)
-    N2, J2 = int(mw2.max()) + 1, int(mf2.max()) + 1
-    s2 = xfe_amd.AkmSolver(akm.build_structure(mw2, mf2, mc2, N2, J2))
-    jp2, _ = s2.jla_pii(draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
-    jfe, jpe, jcov, _ = s2.jla_bii(None, draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
+    t0 = time.perf_counter()
+    dj = akm.kss_components(yk, wk, fk, method="jla", draws=200, seed=20260705, tol=1e-10, max_iter=max_iter)
+    jla_s = time.perf_counter() - t0
+    jp2, jfe, jpe, jcov = dj["pii"], dj["bfe"], dj["bpe"], dj["bcov"]
+    N2, J2 = int(np.unique(wk).size), int(np.unique(fk).size)
+    mw2 = dec["match_worker"]
     exact_cmp = {
         "obs": int(ps["worker"].size), "workers": int(N2), "firms": int(J2), "matches": int(mw2.size),
         "rows_leave_out_set": int(wk.size),
         "exact_s": round(exact_s, 3), "seconds": {k: round(v, 3) for k, v in dec["seconds"].items()},
         "kss": [float(v) for v in dec["kss"]], "plugin": [float(v) for v in dec["plugin"]],
         "unit_leverage_rows": dec["unit_leverage"],
+        "jla200_s": round(jla_s, 3), "jla200_seconds": {k: round(v, 3) for k, v in dj["seconds"].items()},
+        "jla200_kss": [float(v) for v in dj["kss"]],
         "tol": 1e-10, "accept_on_stall": 1e-8,
         "sum_pii_minus_rank": float(ex_p.sum() - (N2 + J2 - 1)),
         "jla200_pii_mean_abs_err": float(np.abs(jp2 - ex_p).mean()),
@@ -272,7 +275,8 @@ def bench_akm(args):
         "jla200_sum_b_rel_err": [float(abs(j.sum() - e.sum()) / abs(e.sum()))
                                  for j, e in ((jfe, ex_fe), (jpe, ex_pe), (jcov, ex_cov))],
     }
-    s2.close()
+    log(f"KSS decomposition with 200 JLA draws: {jla_s:.2f} s {exact_cmp['jla200_seconds']}; "
+        f"KSS {exact_cmp['jla200_kss']}")
     log(f"KSS decomposition with exact leverages, {mw2.size} matches ({wk.size} obs of {ps['worker'].size} in "
         f"the leave-out set): {exact_s:.2f} s {exact_cmp['seconds']}; KSS {exact_cmp['kss']}; "
         f"JLA-200 P_ii mean abs err {exact_cmp['jla200_pii_mean_abs_err']:.3e}, sum B rel err "

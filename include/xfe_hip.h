@@ -454,6 +454,17 @@ int xfe_akm_solve_S_multi(xfe_akm_context* ctx, const double* rhs_pack, double* 
 int xfe_akm_jla_pii(xfe_akm_context* ctx, int draws, uint64_t seed, double tol, int max_iter,
                     double* pii);
 
+/* xfe_akm_jla_pii with the reference's non-linearity correction of the JLA
+ * ratio estimator per working row (src/akm_kss.cpp:3194-3206 and :3273-3288,
+ * the corr factor of its sigma_i): with the draw means P, M of Z^2 and
+ * (r - Z)^2 and Psq, Msq, PM of Z^4, (r - Z)^4 and Z^2 (r - Z)^2, Ph = pii,
+ * Mh = 1 - Ph, V = (Mh^2 Psq + Ph^2 Msq - 2 Mh Ph PM) / p,
+ * B = (Mh Psq - Ph Msq + 2 (Mh - Ph) PM) / p, corr = 1 - V / Mh^2 + B / Mh
+ * (1 where Mh <= 0).  pii and corr: host arrays of M.  Returns as
+ * xfe_akm_jla_pii. */
+int xfe_akm_jla_leverages(xfe_akm_context* ctx, int draws, uint64_t seed, double tol, int max_iter,
+                          double* pii, double* corr);
+
 /* JLA estimates of the KSS quadratic-form weights B_ii of the variance
  * components (src/akm_kss.cpp:2985-3047 and :3128-3146, match level), on the
  * device, rows as in xfe_akm_jla_pii.  Per draw d, u_m = sum of py_count[m]

@@ -438,6 +438,18 @@ def akm_jla_pii(st, draws, seed):
     exactly (firms grounded at J-1, a dense solve of the Schur complement S),
     Z = A [zw; zf]; with P = mean Z^2 and M = mean (r - Z)^2 over the draws,
     P_ii = P / (P + M).  st: akm_build's structure (rows = matches)."""
+    return akm_jla_moments(st, draws, seed)[0]
+
+
+def akm_jla_moments(st, draws, seed):
+    """akm_jla_pii with the reference's non-linearity correction of the JLA
+    ratio estimator (src/akm_kss.cpp:3194-3206 for the moments, :3273-3288 for
+    the correction): besides P = mean Z^2 and M = mean (r - Z)^2, the means of
+    Z^4, (r - Z)^4 and Z^2 (r - Z)^2 (Psq, Msq, PM); with Ph = P / (P + M)
+    (0 when P + M = 0) and Mh = 1 - Ph,
+    V = (Mh^2 Psq + Ph^2 Msq - 2 Mh Ph PM) / p,
+    B = (Mh Psq - Ph Msq + 2 (Mh - Ph) PM) / p and
+    corr = 1 - V / Mh^2 + B / Mh (1 where Mh <= 0).  Returns (P_ii, corr)."""
     mw, mf, c = st["m_w"], st["m_f"], st["m_c"]
     N, J = st["N"], st["J"]
     R = mw.size
@@ -448,8 +460,7 @@ def akm_jla_pii(st, draws, seed):
         e[f] = 1.0
         S[:, f] = akm_apply_S(st, e)
     Sg = S[:J - 1, :J - 1]
-    Pacc = np.zeros(R)
-    Macc = np.zeros(R)
+    acc = np.zeros((5, R))
     for d in range(draws):
         r = jla_signs(seed, d, R)
         v = sq * r
@@ -460,10 +471,18 @@ def akm_jla_pii(st, draws, seed):
         zf[:J - 1] = np.linalg.solve(Sg, rhs[:J - 1])
         zw = (tw - np.bincount(mw, weights=c * zf[mf], minlength=N)) / st["Dw"]
         Z = sq * (zw[mw] + zf[mf])
-        Pacc += Z * Z / draws
-        Macc += (r - Z) ** 2 / draws
-    den = Pacc + Macc
-    return np.where(den > 0.0, Pacc / np.where(den > 0.0, den, 1.0), 0.0)
+        z2, m2 = Z * Z, (r - Z) ** 2
+        acc += np.array([z2, m2, z2 * z2, m2 * m2, z2 * m2]) / draws
+    P, M, Psq, Msq, PM = acc
+    den = P + M
+    Ph = np.where(den > 0.0, P / np.where(den > 0.0, den, 1.0), 0.0)
+    Mh = 1.0 - Ph
+    corr = np.ones(R)
+    ok = Mh > 0.0
+    V = (Mh * Mh * Psq + Ph * Ph * Msq - 2.0 * Mh * Ph * PM) / draws
+    B = (Mh * Psq - Ph * Msq + 2.0 * (Mh - Ph) * PM) / draws
+    corr[ok] = 1.0 - V[ok] / (Mh[ok] * Mh[ok]) + B[ok] / Mh[ok]
+    return Ph, corr
 
 
 def rademacher_sums(seed, tag, count):

@@ -162,3 +162,47 @@ def test_kss_sums_match_reference_with_dense_leverages():
     np.testing.assert_array_equal(c[order], fx["row_weight"])
     assert np.abs(r["sigma_i"][order] - fx["sigma_i"]).max() <= 1e-12 * np.abs(fx["sigma_i"]).max()
     assert r["unit_leverage"] == 0
+
+
+def _dense_two_way(y, panel):
+    """The oracle's dense two-way fit on match_panel's matches: (st, alpha, psi)."""
+    import oracle
+    from xfe_amd import akm
+    wu, fu, wi, fi, ridx, mw, mf, c = panel
+    N, J = wu.size, fu.size
+    st = oracle.akm_build(mw, mf, c, N, J)
+    rhs, _, tw = akm.two_way_rhs(y, ridx, mw, mf, c, N, J, st["Dw"])
+    S = np.column_stack([oracle.akm_apply_S(st, np.eye(J)[f]) for f in range(J)])
+    psi = np.zeros(J)
+    psi[:J - 1] = np.linalg.solve(S[:J - 1, :J - 1], rhs[:J - 1])
+    alpha = (tw - np.bincount(mw, weights=c * psi[mf], minlength=N)) / st["Dw"]
+    return st, alpha, psi
+
+
+def test_kss_sums_match_reference_with_jla_draws(golden):
+    """kss_sums with the JLA leverages (50 draws, seed 42) and their
+    non-linearity correction, fed with the oracle's dense fit, JLA moments
+    and B_ii draws on match_panel's rows (the reference's working-row order,
+    which the draws' streams are indexed by), against the reference's JLA
+    decomposition on the same panel (fixture): the 11 numbers and sigma_i."""
+    import oracle
+    from golden_cases import JLA_DRAWS, JLA_SEED, jla_panel
+    from xfe_amd import akm
+    jp = jla_panel()
+    panel = akm.match_panel(jp["worker"], jp["firm"])
+    wu, fu, wi, fi, ridx, mw, mf, c = panel
+    np.testing.assert_array_equal(wu[mw], golden["jla/row_worker"])
+    np.testing.assert_array_equal(fu[mf], golden["jla/row_firm"])
+    st, alpha, psi = _dense_two_way(jp["y"], panel)
+    P, corr = oracle.akm_jla_moments(st, JLA_DRAWS, JLA_SEED)
+    assert np.abs(P - golden["jla/pii"]).max() <= 1e-12
+    bfe, bpe, bcov = oracle.akm_jla_bii(st, JLA_DRAWS, JLA_SEED, c.astype(np.int64))
+    r = akm.kss_sums(jp["y"], wi, fi, ridx, mw, mf, c, alpha, psi, P, bfe, bpe, bcov, corr=corr)
+    got = list(r["plugin"]) + list(r["agsu"]) + list(r["kss"]) + [r["sigma2_ho"], r["n_py"]]
+    comp = golden["jla/components"]
+    for i, (g, w) in enumerate(zip(got, comp)):
+        assert abs(g - w) <= 1e-10 * max(abs(w), abs(comp[i % 3]) if 3 <= i < 9 else 0.0), (i, g, w)
+    sig = golden["jla/sigma_i"]
+    assert np.abs(r["sigma_i"] - sig).max() <= 1e-12 * np.abs(sig).max()
+    # the correction is what moves sigma_i off the exact-leverage form
+    assert np.abs(corr - 1.0).max() > 1e-3

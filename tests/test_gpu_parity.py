@@ -679,7 +679,7 @@ def test_kss_components_exact_match_reference():
     _log("kss components (exact)")
     fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "exact_leverages.npz"))
     jp = jla_panel()
-    r = akm.kss_components_exact(jp["y"], jp["worker"], jp["firm"])
+    r = akm.kss_components(jp["y"], jp["worker"], jp["firm"], method="exact")
     comp = fx["components"]
     got = list(r["plugin"]) + list(r["agsu"]) + list(r["kss"]) + [r["sigma2_ho"], r["n_py"]]
     # a corrected component is the plug-in less a correction of nearly its
@@ -697,6 +697,38 @@ def test_kss_components_exact_match_reference():
     sig = r["sigma_i"][order]
     assert np.abs(sig - fx["sigma_i"]).max() <= 1e-8 * np.abs(fx["sigma_i"]).max()
     assert np.abs(r["pii"][order] - fx["pii"]).max() <= 1e-8
+
+
+def test_kss_components_jla_match_reference(golden):
+    """The JLA decomposition (50 draws, seed 42: the reference's streams on
+    its working-row order) with the device's fit, leverages, non-linearity
+    correction and B_ii draws against the reference's akm_kss_decompose with
+    LeverageMethod::Jla on the same panel (fixture): the 11 numbers to 1e-8
+    (corrected ones relative to their plug-in), P_ii and sigma_i to 1e-8; the
+    correction against the oracle's."""
+    from golden_cases import JLA_DRAWS, JLA_SEED, jla_panel
+    from xfe_amd import akm
+    _log("kss components (jla)")
+    jp = jla_panel()
+    r = akm.kss_components(jp["y"], jp["worker"], jp["firm"], method="jla", draws=JLA_DRAWS, seed=JLA_SEED,
+                           tol=1e-12, max_iter=5000)
+    np.testing.assert_array_equal(r["match_worker"], golden["jla/row_worker"])
+    np.testing.assert_array_equal(r["match_firm"], golden["jla/row_firm"])
+    comp = golden["jla/components"]
+    got = list(r["plugin"]) + list(r["agsu"]) + list(r["kss"]) + [r["sigma2_ho"], r["n_py"]]
+    scale = [abs(w) for w in comp]
+    for i in range(3, 9):
+        scale[i] = max(scale[i], abs(comp[i % 3]))
+    err = max(abs(g - w) / s for g, w, s in zip(got, comp, scale))
+    _log(f"  components vs reference: max rel {err:.2e}")
+    assert err <= 1e-8, (got, comp)
+    assert np.abs(r["pii"] - golden["jla/pii"]).max() <= 1e-8
+    sig = golden["jla/sigma_i"]
+    assert np.abs(r["sigma_i"] - sig).max() <= 1e-8 * np.abs(sig).max()
+    wu, fu, wi, fi, ridx, mw, mf, c = akm.match_panel(jp["worker"], jp["firm"])
+    st = oracle.akm_build(mw, mf, c, wu.size, fu.size)
+    _, corr = oracle.akm_jla_moments(st, JLA_DRAWS, JLA_SEED)
+    assert np.abs(r["corr"] - corr).max() <= 1e-8
 
 
 def test_akm_jla_bii_matches_reference(golden):

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstring>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "absorb_kernels.h"
@@ -454,23 +455,35 @@ jla_zw_kernel(const int* wofs, int N, const int* mf, const double* mc, const dou
 }
 
 // Per working row (worker-major record a, row orig[a]): Z = sqrt(c)(zw + zf)
-// and r for the nb draws; P += Z^2 / p, M += (r - Z)^2 / p
+// and r for the nb draws; P += Z^2 / p, M += (r - Z)^2 / p; with mom (3 M:
+// the means of Z^4, (r - Z)^4 and Z^2 (r - Z)^2, for the ratio estimator's
+// non-linearity correction, src/akm_kss.cpp:3194-3206) those too
 __global__ void __launch_bounds__(kBlock)
 jla_accumulate_kernel(int M, const int* mw, const int* mf, const int* orig, const double* sq, const double* zw,
-                      const double* zf, uint64_t seed, int d0, int nb, double inv_p, double* pacc, double* macc) {
+                      const double* zf, uint64_t seed, int d0, int nb, double inv_p, double* pacc, double* macc,
+                      double* mom) {
     for (int a = blockIdx.x * kBlock + threadIdx.x; a < M; a += gridDim.x * kBlock) {
         const int o = orig[a];
         const double* zwa = zw + static_cast<size_t>(mw[a]) * nb;
         const double* zfa = zf + static_cast<size_t>(mf[a]) * nb;
-        double pp = 0.0, mm = 0.0;
+        double pp = 0.0, mm = 0.0, p4 = 0.0, m4 = 0.0, pm = 0.0;
         for (int l = 0; l < nb; ++l) {
             const double Z = sq[a] * (zwa[l] + zfa[l]);
             const double e = jla_sign(seed, d0 + l, o) - Z;
-            pp += Z * Z;
-            mm += e * e;
+            const double z2 = Z * Z, e2 = e * e;
+            pp += z2;
+            mm += e2;
+            p4 += z2 * z2;
+            m4 += e2 * e2;
+            pm += z2 * e2;
         }
         pacc[o] += pp * inv_p;
         macc[o] += mm * inv_p;
+        if (mom) {
+            mom[o] += p4 * inv_p;
+            mom[static_cast<size_t>(M) + o] += m4 * inv_p;
+            mom[2 * static_cast<size_t>(M) + o] += pm * inv_p;
+        }
     }
 }
 
@@ -562,6 +575,25 @@ __global__ void __launch_bounds__(kBlock) jla_ratio_kernel(int M, const double* 
     for (int m = blockIdx.x * kBlock + threadIdx.x; m < M; m += gridDim.x * kBlock) {
         const double den = pacc[m] + macc[m];
         pii[m] = den > 0.0 ? pacc[m] / den : 0.0;
+    }
+}
+
+// The non-linearity correction of the ratio estimator (src/akm_kss.cpp:
+// 3273-3288): Ph = pii, Mh = 1 - Ph, V = (Mh^2 Psq + Ph^2 Msq - 2 Mh Ph PM) / p,
+// B = (Mh Psq - Ph Msq + 2 (Mh - Ph) PM) / p, corr = 1 - V / Mh^2 + B / Mh
+// (1 where Mh <= 0)
+__global__ void __launch_bounds__(kBlock)
+jla_corr_kernel(int M, const double* pii, const double* mom, double inv_p, double* corr) {
+    for (int m = blockIdx.x * kBlock + threadIdx.x; m < M; m += gridDim.x * kBlock) {
+        const double Ph = pii[m], Mh = 1.0 - Ph;
+        double cr = 1.0;
+        if (Mh > 0.0) {
+            const double psq = mom[m], msq = mom[static_cast<size_t>(M) + m], pm = mom[2 * static_cast<size_t>(M) + m];
+            const double V = (Mh * Mh * psq + Ph * Ph * msq - 2.0 * Mh * Ph * pm) * inv_p;
+            const double B = (Mh * psq - Ph * msq + 2.0 * (Mh - Ph) * pm) * inv_p;
+            cr = 1.0 - V / (Mh * Mh) + B / Mh;
+        }
+        corr[m] = cr;
     }
 }
 
@@ -679,7 +711,7 @@ struct xfe_akm_context {
     // JLA draws: worker offsets and per-record row index, sqrt(c), c in the
     // worker-major record order; row index per firm-major record; 1 / Dw
     DevBuf<int> wofs, rorig, forig;
-    DevBuf<double> rsq, rcv, idw, jtw, pacc, macc, pii;
+    DevBuf<double> rsq, rcv, idw, jtw, pacc, macc, pii, jmom;
     // B_ii draws: person-year count per row, u per (row, draw), integer totals, sums
     DevBuf<long long> jcnt;
     DevBuf<double> jsu, bfe, bpe, bcov;
@@ -711,7 +743,7 @@ struct xfe_akm_context {
                                &lane_w, &lane_f, &lane_m})
             d->release();
         for (DevBuf<double>* d : {&dwv, &dfv, &ex_part, &ex_bp}) d->release();
-        for (DevBuf<double>* d : {&rsq, &rcv, &idw, &jtw, &pacc, &macc, &pii, &jsu, &bfe, &bpe, &bcov, &rwt, &fc,
+        for (DevBuf<double>* d : {&rsq, &rcv, &idw, &jtw, &pacc, &macc, &pii, &jmom, &jsu, &bfe, &bpe, &bcov, &rwt, &fc,
                                   &inv_diag, &corr, &wmean, &lanes, &b, &x, &r, &z, &p, &q, &cg, &part, &scale,
                                   &bv[0], &bv[1]})
             d->release();
@@ -1123,10 +1155,14 @@ int xfe_akm_solve_S_multi(xfe_akm_context* c, const double* rhs_pack, double* z_
     }
 }
 
-int xfe_akm_jla_pii(xfe_akm_context* c, int draws, uint64_t seed, double tol, int max_iter, double* pii) {
+namespace {
+
+// xfe_akm_jla_pii, and with corr the correction (xfe_akm_jla_leverages)
+int jla_pii_impl(xfe_akm_context* c, int draws, uint64_t seed, double tol, int max_iter, double* pii,
+                 double* corr, const char* name) {
     clear_error();
     if (!c || !pii || draws < 1 || max_iter < 1) {
-        set_error("xfe_akm_jla_pii: invalid arguments");
+        set_error(std::string(name) + ": invalid arguments");
         return -1;
     }
     try {
@@ -1140,6 +1176,10 @@ int xfe_akm_jla_pii(xfe_akm_context* c, int draws, uint64_t seed, double tol, in
         c->pii.ensure(M);
         XFE_CHECK(hipMemsetAsync(c->pacc.p, 0, sizeof(double) * M, s));
         XFE_CHECK(hipMemsetAsync(c->macc.p, 0, sizeof(double) * M, s));
+        if (corr) {
+            c->jmom.ensure(3 * static_cast<size_t>(M));
+            XFE_CHECK(hipMemsetAsync(c->jmom.p, 0, sizeof(double) * 3 * M, s));
+        }
         std::vector<int> its(kMaxCgCols);
         bool failed = false;
         for (int d0 = 0; d0 < draws; d0 += kMaxCgCols) {
@@ -1152,20 +1192,43 @@ int xfe_akm_jla_pii(xfe_akm_context* c, int draws, uint64_t seed, double tol, in
             if (c->jla_solve(seed, d0, nb, nullptr, 0, tol, max_iter, its.data())) failed = true;
             hipLaunchKernelGGL(jla_accumulate_kernel, dim3(akm_grid(M)), dim3(kBlock), 0, s, M, c->rw.p, c->r1.p,
                                c->rorig.p, c->rsq.p, c->wmean.p, c->lanes.p, seed, d0, nb, 1.0 / draws, c->pacc.p,
-                               c->macc.p);
+                               c->macc.p, corr ? c->jmom.p : nullptr);
             XFE_CHECK(hipGetLastError());
         }
         hipLaunchKernelGGL(jla_ratio_kernel, dim3(akm_grid(M)), dim3(kBlock), 0, s, M, c->pacc.p, c->macc.p,
                            c->pii.p);
         XFE_CHECK(hipGetLastError());
         XFE_CHECK(hipMemcpyAsync(pii, c->pii.p, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+        if (corr) {
+            // the correction over the moments, into pacc (no longer needed)
+            hipLaunchKernelGGL(jla_corr_kernel, dim3(akm_grid(M)), dim3(kBlock), 0, s, M, c->pii.p, c->jmom.p,
+                               1.0 / draws, c->pacc.p);
+            XFE_CHECK(hipGetLastError());
+            XFE_CHECK(hipMemcpyAsync(corr, c->pacc.p, sizeof(double) * M, hipMemcpyDeviceToHost, s));
+        }
         XFE_CHECK(hipStreamSynchronize(s));
-        if (failed) set_error("xfe_akm_jla_pii: some draw solves did not converge");
+        if (failed) set_error(std::string(name) + ": some draw solves did not converge");
         return failed ? 1 : 0;
     } catch (const std::exception& e) {
         set_error(e.what());
         return -1;
     }
+}
+
+}  // namespace
+
+int xfe_akm_jla_pii(xfe_akm_context* c, int draws, uint64_t seed, double tol, int max_iter, double* pii) {
+    return jla_pii_impl(c, draws, seed, tol, max_iter, pii, nullptr, "xfe_akm_jla_pii");
+}
+
+int xfe_akm_jla_leverages(xfe_akm_context* c, int draws, uint64_t seed, double tol, int max_iter, double* pii,
+                          double* corr) {
+    if (!corr) {
+        clear_error();
+        set_error("xfe_akm_jla_leverages: invalid arguments");
+        return -1;
+    }
+    return jla_pii_impl(c, draws, seed, tol, max_iter, pii, corr, "xfe_akm_jla_leverages");
 }
 
 int xfe_akm_jla_bii(xfe_akm_context* c, int draws, uint64_t seed, double tol, int max_iter, const int64_t* py_count,

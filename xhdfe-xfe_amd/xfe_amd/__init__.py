@@ -74,7 +74,7 @@ EXPORTS = ("xfe_backend_available", "xfe_absorb", "xfe_absorb_device", "xfe_abso
            "xfe_akm_solve_S_multi", "xfe_akm_max_lanes", "xfe_akm_solve_S_wide", "xfe_akm_solve_S_wide_device",
            "xfe_akm_wide_max_lanes", "xfe_boot_create", "xfe_boot_destroy", "xfe_boot_refits", "xfe_boot_last",
            "xfe_akm_wide_last", "xfe_akm_exact_leverages", "xfe_akm_exact_leverages_last_retries", "xfe_akm_pack_buffer", "xfe_schwarz_demean",
-           "xfe_akm_jla_pii", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
+           "xfe_akm_jla_pii", "xfe_akm_jla_leverages", "xfe_akm_jla_bii", "xfe_recover_fixed_effects",
            "xfe_absorb_group_individual",
            "xfe_cert_shadow_begin", "xfe_cert_shadow_finish", "xfe_cert_shadow_abort",
            "xfe_articulation_points", "xfe_akm_leave_out_set", "xfe_build_indexer",
@@ -630,6 +630,20 @@ class AkmSolver:
         if rc < 0:
             raise XfeError(last_error())
         return pii, rc == 0
+
+    def jla_leverages(self, draws=200, seed=20260705, tol=1e-10, max_iter=10000):
+        """xfe_akm_jla_leverages: JLA leverages of the matches and the ratio
+        estimator's non-linearity correction; returns (pii, corr, converged)."""
+        M = self._keep["m_w"].size
+        pii = np.empty(M)
+        corr = np.empty(M)
+        L = lib()
+        L.xfe_akm_jla_leverages.argtypes = [_p, _i, ctypes.c_uint64, _d, _i, _p, _p]
+        rc = L.xfe_akm_jla_leverages(_p(self._ctx), int(draws), int(seed), float(tol), int(max_iter), _ptr(pii),
+                                     _ptr(corr))
+        if rc < 0:
+            raise XfeError(last_error())
+        return pii, corr, rc == 0
 
     def jla_bii(self, py_count=None, draws=200, seed=20260705, tol=1e-10, max_iter=10000, n_py=0.0):
         """xfe_akm_jla_bii: JLA estimates of the KSS weights (Bfe, Bpe, Bcov)

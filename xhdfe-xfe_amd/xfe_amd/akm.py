@@ -57,18 +57,28 @@ def panel_matches(worker, firm, connected=True):
 
 
 
+def _first_appearance(ids):
+    """Dense labels in first-appearance order and the original id of each."""
+    u, first, inv = np.unique(np.asarray(ids), return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty(u.size, dtype=np.int64)
+    rank[order] = np.arange(u.size)
+    return u[order], rank[inv.reshape(-1)]
+
+
 def match_panel(worker, firm):
-    """Panel rows -> matches on compact 0-based ids, without pruning: (wu, fu,
-    wi, fi, ridx, mw, mf, c) with wu / fu the original ids of the compact
-    workers / firms, wi / fi the rows' compact ids, ridx each row's match,
-    mw / mf / c the matches grouped by worker with firms ascending and their
-    person-year counts."""
-    wu, wi = np.unique(np.asarray(worker), return_inverse=True)
-    fu, fi = np.unique(np.asarray(firm), return_inverse=True)
+    """Panel rows -> matches without pruning, as the reference's working rows
+    (src/akm_kss.cpp:873-960): workers and firms take dense labels in
+    first-appearance order over the rows, matches are sorted by (worker
+    label, firm label).  Returns (wu, fu, wi, fi, ridx, mw, mf, c) with wu /
+    fu the original ids of the labels, wi / fi the rows' labels, ridx each
+    row's match and mw / mf / c the matches and their person-year counts."""
+    wu, wi = _first_appearance(worker)
+    fu, fi = _first_appearance(firm)
     J = fu.size
-    key = wi.astype(np.int64) * J + fi
+    key = wi * J + fi
     uk, ridx, counts = np.unique(key, return_inverse=True, return_counts=True)
-    return (wu, fu, wi, fi, ridx, (uk // J).astype(np.int32), (uk % J).astype(np.int32),
+    return (wu, fu, wi, fi, ridx.reshape(-1), (uk // J).astype(np.int32), (uk % J).astype(np.int32),
             counts.astype(np.float64))
 
 
@@ -82,7 +92,7 @@ def two_way_rhs(y, ridx, mw, mf, c, N, J, Dw):
     return tf - np.bincount(mf, weights=c * (tw / Dw)[mw], minlength=J), m_ysum, tw
 
 
-def kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, P, Bfe, Bpe, Bcov):
+def kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, P, Bfe, Bpe, Bcov, corr=None):
     """The O(rows) part of the reference's variance decomposition at match
     level with unweighted rows (src/akm_kss.cpp:2632-2662 for yt / eta / the
     stayer flags, :3299-3349 for sigma_i, :3353-3410 for the components), from
@@ -90,7 +100,8 @@ def kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, P, Bfe, Bpe, Bcov):
     (Bfe, Bpe, Bcov):
 
       yt = sqrt(c) ybar_m, eta = yt - sqrt(c) (alpha_w + psi_f);
-      sigma_i = (yt - mean yt) (eta / M_ii), M_ii = 1 - P_ii, on a mover's match
+      sigma_i = (yt - mean yt) corr (eta / M_ii), M_ii = 1 - P_ii, on a mover's match
+      (corr: the JLA estimator's non-linearity correction, 1 for exact leverages)
       (0 where M_ii <= 1e-12: such a match is counted in unit_leverage); on a
       stayer's match the mean over its person-years of
       (y - mean y) e / (1 - 1 / T_w), e = y - alpha - psi, T_w = Dw;
@@ -111,7 +122,8 @@ def kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, P, Bfe, Bpe, Bcov):
     sigma = np.zeros(M)
     mov = ~stayer & (Mii > 1e-12)
     unit = int(np.count_nonzero(~stayer & ~(Mii > 1e-12)))
-    sigma[mov] = (yt[mov] - yt.mean()) * (eta[mov] / Mii[mov])
+    cf = np.ones(M) if corr is None else np.asarray(corr, dtype=np.float64)
+    sigma[mov] = (yt[mov] - yt.mean()) * cf[mov] * (eta[mov] / Mii[mov])
     T = Dw[wi]
     e = y - alpha[wi] - psi[fi]
     rows = stayer[ridx]
@@ -133,21 +145,28 @@ def kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, P, Bfe, Bpe, Bcov):
                 unit_leverage=unit)
 
 
-def kss_components_exact(y, worker, firm, tol=1e-11, max_iter=10000, accept=0.0):
-    """The variance decomposition of the reference's akm_kss_decompose with
-    exact leverages (LeverageMethod::Exact, match level, no controls, no
-    weights, no pruning: the panel must be leave-out connected;
-    src/akm_kss.cpp:2551-2741, :3285-3410).  The two-way fit (the grounded
-    firm Schur solve) and the leverages with their quadratic-form weights
-    (xfe_akm_exact_leverages) run on the device; kss_sums forms the O(rows)
-    sums.  accept: as in AkmSolver.exact_leverages (a leverage solve whose
-    FP64 residual stalls at or below it counts as converged).  Returns
-    kss_sums' dict (plugin, agsu, kss as (var_alpha, var_psi, cov_alpha_psi);
-    sigma2_ho; n_py; per match P_ii and sigma_i) with the matches' original
-    ids (match_worker, match_firm), counts, the weights bfe / bpe / bcov and
-    the seconds of the fit, the leverages and the sums (seconds)."""
+def kss_components(y, worker, firm, method="exact", draws=200, seed=20260705, tol=1e-11, max_iter=10000,
+                   accept=0.0):
+    """The variance decomposition of the reference's akm_kss_decompose at
+    match level with unweighted rows, no controls and no pruning (the panel
+    must be leave-out connected; src/akm_kss.cpp:2551-2741, :2753-3288,
+    :3299-3410), for method "exact" (LeverageMethod::Exact) or "jla"
+    (LeverageMethod::Jla: `draws` Rademacher draws of the reference's streams
+    from `seed`).  The two-way fit (the grounded firm Schur solve), the
+    leverages, the JLA correction and the quadratic-form weights run on the
+    device (xfe_akm_exact_leverages, or xfe_akm_jla_leverages and
+    xfe_akm_jla_bii); kss_sums forms the O(rows) sums.  The rows are
+    match_panel's, the reference's working-row order, so the JLA draws equal
+    the reference's.  accept: as in AkmSolver.exact_leverages.  Returns
+    kss_sums' dict (plugin, agsu, kss as (var_alpha, var_psi,
+    cov_alpha_psi); sigma2_ho; n_py; per match P_ii and sigma_i) with the
+    matches' original ids (match_worker, match_firm), counts, the weights
+    bfe / bpe / bcov, the correction corr and the seconds of the fit, the
+    leverages and the sums."""
     import time
     from . import AkmSolver, XfeError
+    if method not in ("exact", "jla"):
+        raise XfeError(f"kss_components: unknown method {method!r}")
     y = np.asarray(y, dtype=np.float64)
     wu, fu, wi, fi, ridx, mw, mf, c = match_panel(worker, firm)
     N, J = wu.size, fu.size
@@ -155,21 +174,33 @@ def kss_components_exact(y, worker, firm, tol=1e-11, max_iter=10000, accept=0.0)
     t0 = time.perf_counter()
     rhs, _, tw = two_way_rhs(y, ridx, mw, mf, c, N, J, st["Dw"])
     solver = AkmSolver(st)
+    corr = None
     try:
         z, its = solver.solve_multi(rhs[None, :], tol=tol, max_iter=max_iter)
         if its[0] < 0:
-            raise XfeError("kss_components_exact: the two-way fit did not converge")
+            raise XfeError("kss_components: the two-way fit did not converge")
         t1 = time.perf_counter()
-        P, Bfe, Bpe, Bcov, failed = solver.exact_leverages(tol=tol, max_iter=max_iter, accept=accept)
-        if failed:
-            raise XfeError(f"kss_components_exact: {failed} leverage solves did not converge")
+        if method == "exact":
+            P, Bfe, Bpe, Bcov, failed = solver.exact_leverages(tol=tol, max_iter=max_iter, accept=accept)
+            if failed:
+                raise XfeError(f"kss_components: {failed} leverage solves did not converge")
+        else:
+            P, corr, ok = solver.jla_leverages(draws=draws, seed=seed, tol=tol, max_iter=max_iter)
+            Bfe, Bpe, Bcov, ok2 = solver.jla_bii(None, draws=draws, seed=seed, tol=tol, max_iter=max_iter)
+            if not (ok and ok2):
+                raise XfeError("kss_components: some JLA draw solves did not converge")
         t2 = time.perf_counter()
     finally:
         solver.close()
     psi = np.asarray(z[0], dtype=np.float64)
     alpha = (tw - np.bincount(mw, weights=c * psi[mf], minlength=N)) / st["Dw"]
-    r = kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, P, Bfe, Bpe, Bcov)
+    r = kss_sums(y, wi, fi, ridx, mw, mf, c, alpha, psi, P, Bfe, Bpe, Bcov, corr=corr)
     t3 = time.perf_counter()
-    r.update(match_worker=wu[mw], match_firm=fu[mf], match_count=c, bfe=Bfe, bpe=Bpe, bcov=Bcov,
+    r.update(match_worker=wu[mw], match_firm=fu[mf], match_count=c, bfe=Bfe, bpe=Bpe, bcov=Bcov, corr=corr,
              seconds=dict(fit=t1 - t0, leverages=t2 - t1, sums=t3 - t2))
     return r
+
+
+def kss_components_exact(y, worker, firm, tol=1e-11, max_iter=10000, accept=0.0):
+    """kss_components with exact leverages."""
+    return kss_components(y, worker, firm, method="exact", tol=tol, max_iter=max_iter, accept=accept)
