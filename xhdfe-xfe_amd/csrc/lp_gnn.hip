@@ -238,24 +238,69 @@ __global__ void __launch_bounds__(kB) colsum_kernel(const float* __restrict__ dZ
 
 // ---------------------------------------------------------------- message passing
 // out[v] = mean over src[idx[ptr[v] .. ptr[v] + min(deg, cap))] (0 when
-// empty); one wave per v, one lane per feature, four rows in flight
+// empty); one wave per v: a row of 64 features is 16 lanes x float4, so the
+// wave's four 16-lane groups take rows q, q + 1, q + 2, q + 3 of the list and
+// each group keeps two rows in flight (eight per wave); the groups' sums meet
+// by two lane exchanges (xor 16, xor 32) in a fixed order
 __global__ void __launch_bounds__(kB) mean_gather_kernel(const int* __restrict__ ptr, const int* __restrict__ idx,
                                                          const float* __restrict__ src, int n, int cap,
                                                          float* __restrict__ out) {
     const int lane = threadIdx.x & (kW - 1);
+    const int grp = lane >> 4, c4 = lane & 15;
+    const float4* __restrict__ src4 = reinterpret_cast<const float4*>(src);
     for (int v = blockIdx.x * 4 + threadIdx.x / kW; v < n; v += gridDim.x * 4) {
         const int b = ptr[v], e = ptr[v + 1] - b > cap ? b + cap : ptr[v + 1];  // no b + cap overflow
-        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-        int q = b;
-        for (; q + 4 <= e; q += 4) {
-            s0 += src[static_cast<size_t>(idx[q]) * 64 + lane];
-            s1 += src[static_cast<size_t>(idx[q + 1]) * 64 + lane];
-            s2 += src[static_cast<size_t>(idx[q + 2]) * 64 + lane];
-            s3 += src[static_cast<size_t>(idx[q + 3]) * 64 + lane];
+        float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+        int q = b + grp;
+        for (; q + 4 < e; q += 8) {
+            const float4 x0 = src4[static_cast<size_t>(idx[q]) * 16 + c4];
+            const float4 x1 = src4[static_cast<size_t>(idx[q + 4]) * 16 + c4];
+            s0.x += x0.x; s0.y += x0.y; s0.z += x0.z; s0.w += x0.w;
+            s1.x += x1.x; s1.y += x1.y; s1.z += x1.z; s1.w += x1.w;
         }
-        for (; q < e; ++q) s0 += src[static_cast<size_t>(idx[q]) * 64 + lane];
-        const float s = (s0 + s1) + (s2 + s3);
-        out[static_cast<size_t>(v) * 64 + lane] = e > b ? s / static_cast<float>(e - b) : 0.f;
+        if (q < e) {
+            const float4 x0 = src4[static_cast<size_t>(idx[q]) * 16 + c4];
+            s0.x += x0.x; s0.y += x0.y; s0.z += x0.z; s0.w += x0.w;
+        }
+        float4 t = make_float4(s0.x + s1.x, s0.y + s1.y, s0.z + s1.z, s0.w + s1.w);
+        t.x += __shfl_xor(t.x, 16); t.y += __shfl_xor(t.y, 16); t.z += __shfl_xor(t.z, 16); t.w += __shfl_xor(t.w, 16);
+        t.x += __shfl_xor(t.x, 32); t.y += __shfl_xor(t.y, 32); t.z += __shfl_xor(t.z, 32); t.w += __shfl_xor(t.w, 32);
+        if (grp == 0) {
+            const float d = static_cast<float>(e - b);
+            reinterpret_cast<float4*>(out)[static_cast<size_t>(v) * 16 + c4] =
+                e > b ? make_float4(t.x / d, t.y / d, t.z / d, t.w / d) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+}
+
+// mean_gather for short lists (the movers' firms): each 16-lane group of a
+// wave takes a node of its own (four per wave) and walks its list two rows
+// at a time, so four independent index-then-row load chains run per wave
+__global__ void __launch_bounds__(kB) mean_gather_short_kernel(const int* __restrict__ ptr,
+                                                               const int* __restrict__ idx,
+                                                               const float* __restrict__ src, int n, int cap,
+                                                               float* __restrict__ out) {
+    const int lane = threadIdx.x & (kW - 1);
+    const int grp = lane >> 4, c4 = lane & 15;
+    const float4* __restrict__ src4 = reinterpret_cast<const float4*>(src);
+    for (int v = (blockIdx.x * 4 + threadIdx.x / kW) * 4 + grp; v < n; v += gridDim.x * 16) {
+        const int b = ptr[v], e = ptr[v + 1] - b > cap ? b + cap : ptr[v + 1];
+        float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+        int q = b;
+        for (; q + 1 < e; q += 2) {
+            const float4 x0 = src4[static_cast<size_t>(idx[q]) * 16 + c4];
+            const float4 x1 = src4[static_cast<size_t>(idx[q + 1]) * 16 + c4];
+            s0.x += x0.x; s0.y += x0.y; s0.z += x0.z; s0.w += x0.w;
+            s1.x += x1.x; s1.y += x1.y; s1.z += x1.z; s1.w += x1.w;
+        }
+        if (q < e) {
+            const float4 x0 = src4[static_cast<size_t>(idx[q]) * 16 + c4];
+            s0.x += x0.x; s0.y += x0.y; s0.z += x0.z; s0.w += x0.w;
+        }
+        const float d = static_cast<float>(e - b);
+        reinterpret_cast<float4*>(out)[static_cast<size_t>(v) * 16 + c4] =
+            e > b ? make_float4((s0.x + s1.x) / d, (s0.y + s1.y) / d, (s0.z + s1.z) / d, (s0.w + s1.w) / d)
+                  : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
@@ -447,8 +492,8 @@ void Gnn::forward(const Graph& g, const float* th, hipStream_t s) {
                            th + kOffBw0, hw_at(0));
     for (int l = 1; l <= kLayers; ++l) {
         if (Wm > 0) {
-            hipLaunchKernelGGL(mean_gather_kernel, dim3(grid_of(Wm, 4)), dim3(kB), 0, s, g.w_ptr, g.w_idx, hf_at(l - 1),
-                               Wm, INT_MAX, aw_at(l));
+            hipLaunchKernelGGL(mean_gather_short_kernel, dim3(grid_of(Wm, 16)), dim3(kB), 0, s, g.w_ptr, g.w_idx,
+                               hf_at(l - 1), Wm, INT_MAX, aw_at(l));
             hipLaunchKernelGGL((gemm_fwd_kernel<64, true>), dim3(gw), dim3(kB), 0, s, hw_at(l - 1), aw_at(l), Wm,
                                th + off_Ww(l), th + off_bw(l), hw_at(l));
         }

@@ -163,7 +163,8 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 // firm features, one wavefront per firm (its pair half-edges strided over the
-// lanes, then wave reductions in a fixed order): log(1 + rows), log(1 +
+// lanes, then wave reductions in a fixed order; lanes 0-7 then form the eight
+// features and store them together): log(1 + rows), log(1 +
 // movers), log(1 + S_ff), S_ff / rows, log(1 + pairs), max strength, sum of
 // strengths, 1
 __global__ void __launch_bounds__(kB) firm_feat_kernel(const double* rows, const int* f_ptr, const double* dex,
@@ -179,18 +180,22 @@ __global__ void __launch_bounds__(kB) firm_feat_kernel(const double* rows, const
         }
         mx = wave_max(mx);
         sm = wave_sum(sm);
-        if (lane == 0) {
+        mx = __shfl(mx, 0, 64);
+        sm = __shfl(sm, 0, 64);
+        // lane k < 8 forms feature k: the four logarithms run side by side
+        if (lane < kInF) {
             const double r = rows[f];
             const double d = fmax(dex[f], 0.0);
-            float* x = xf + static_cast<size_t>(f) * kInF;
-            x[0] = static_cast<float>(log1p(r));
-            x[1] = static_cast<float>(log1p(static_cast<double>(f_ptr[f + 1] - f_ptr[f])));
-            x[2] = static_cast<float>(log1p(d));
-            x[3] = static_cast<float>(d / fmax(r, 1.0));
-            x[4] = static_cast<float>(log1p(static_cast<double>(e - b)));
-            x[5] = mx;
-            x[6] = sm;
-            x[7] = 1.0f;
+            const double arg = lane == 0 ? r
+                               : lane == 1 ? static_cast<double>(f_ptr[f + 1] - f_ptr[f])
+                               : lane == 2 ? d
+                                           : static_cast<double>(e - b);
+            float v = static_cast<float>(log1p(arg));
+            if (lane == 3) v = static_cast<float>(d / fmax(r, 1.0));
+            else if (lane == 5) v = mx;
+            else if (lane == 6) v = sm;
+            else if (lane == 7) v = 1.0f;
+            xf[static_cast<size_t>(f) * kInF + lane] = v;
         }
     }
 }
