@@ -220,7 +220,11 @@ int xfe_absorb_group_individual(const double* y, const double* X, int n, int k, 
  * host memory); it serves callers that keep their panel on the device.
  * Ordering: the call first waits for all work already queued on the device
  * (any stream), so buffers filled asynchronously by the caller are complete,
- * and it returns after its own work has finished. */
+ * and it returns after its own work has finished.  When the ids of the
+ * dimension with the most levels are already sorted, ld is n rounded up to a
+ * multiple of 64 and d_V is 256-byte aligned, the solve works in d_V itself
+ * (no working copy): rows n..ld-1 may then be used as scratch, and after a
+ * device failure (return 0) d_V's content is unspecified. */
 int xfe_absorb_device(double* d_V, int n, int C, int ld,
                       const int* d_group_ids, const int* num_groups, int num_fes,
                       const double* d_weights, const int* sweep_order,

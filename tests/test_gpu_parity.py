@@ -287,6 +287,38 @@ def test_device_entry_point_waits_for_queued_work(precision):
     assert max(_rel(out[:, 0], yg, scale), _rel(out[:, 1:], Xg, scale)) <= (1e-12 if precision == 0 else 1e-9)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_entry_point_in_place_and_copied(precision):
+    """Rows already sorted by the dimension with the most levels and the
+    padded stride: the solve works in the caller's block itself; with a wider
+    stride it works in a copy, and with the rows shuffled in a sorted
+    permutation.  All three equal the host entry point on the same rows."""
+    _log(f"device entry point in place / copied, precision {precision}")
+    torch = pytest.importorskip("torch")
+    p = panels.akm_small(scale=0.02)
+    _, fes, gids, G, V = _prep(p)
+    pdim = int(np.argmax(G))
+    srt = np.argsort(gids[pdim], kind="stable")
+    rng = np.random.default_rng(3)
+    n, C = V.shape
+    for label, rows, extra in (("in place", srt, 0), ("copied", srt, 64), ("permuted", rng.permutation(n), 0)):
+        Vr = V[rows]
+        gr = [g[rows] for g in gids]
+        yg, Xg, _ = xfe_amd.absorb(Vr[:, 0], Vr[:, 1:], gr, G, tol=1e-8, precision=precision)
+        ld = ((n + 63) // 64) * 64 + extra
+        dV = torch.zeros((C, ld), dtype=torch.float64, device="cuda")
+        dV[:, :n] = torch.from_numpy(np.ascontiguousarray(Vr.T)).cuda()
+        ids = torch.from_numpy(np.ascontiguousarray(np.stack(gr))).cuda()
+        torch.cuda.synchronize()
+        info = xfe_amd.absorb_device(dV.data_ptr(), n, C, ld, ids.data_ptr(), G, tol=1e-8, precision=precision)
+        out = dV[:, :n].cpu().numpy().T
+        scale = np.abs(V).max()
+        err = max(_rel(out[:, 0], yg, scale), _rel(out[:, 1:], Xg, scale))
+        _log(f"  {label}: {err:.2e}")
+        assert info["converged"]
+        assert err <= (1e-12 if precision == 0 else 1e-9), label
+
+
 # ---------------------------------------------------------------- mixed precision
 # The FP32 Schur-complement CG under FP64 refinement converges to the
 # projection itself; it is compared with the oracle run to tol 1e-12 (error

@@ -21,6 +21,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -2250,10 +2251,15 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
                                             bits, s));
         perm = ws.perm.p;
     }
-    // permuted working copies
-    ws.vec[0].ensure(static_cast<size_t>(ld) * C);
-    double* V = ws.vec[0].p;
-    XFE_CHECK(launch_gather_cols(d_Vin, ld_in, perm, V, ld, n, C, grid, s));
+    // permuted working copies; rows already in level order with the padded
+    // stride and an aligned block are worked on in place (no copy in or out)
+    const bool in_place = !perm && !rule && ld_in == ld && (reinterpret_cast<uintptr_t>(d_Vin) & 255) == 0;
+    double* V = d_Vin;
+    if (!in_place) {
+        ws.vec[0].ensure(static_cast<size_t>(ld) * C);
+        V = ws.vec[0].p;
+        XFE_CHECK(launch_gather_cols(d_Vin, ld_in, perm, V, ld, n, C, grid, s));
+    }
     const double* w = nullptr;
     if (d_w_in) {
         ws.w.ensure(n);
@@ -2267,10 +2273,14 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
     sv.sorted_dim = p;  // level-sorted after the permutation
     std::vector<ProfileStats> profs(D);
     for (int d = 0; d < D; ++d) {
-        ws.gid[d].ensure(n);
-        XFE_CHECK(launch_gather_ids(d_gid_in[d], perm, ws.gid[d].p, n, grid, s));
+        if (perm) {
+            ws.gid[d].ensure(n);
+            XFE_CHECK(launch_gather_ids(d_gid_in[d], perm, ws.gid[d].p, n, grid, s));
+            sv.gid[d] = ws.gid[d].p;
+        } else {
+            sv.gid[d] = d_gid_in[d];  // already in level order: read in place
+        }
         sv.G[d] = G[d];
-        sv.gid[d] = ws.gid[d].p;
         sv.prepare_dim(d);
         if (h_wsum[d]) {
             XFE_CHECK(hipMemcpyAsync(ws.wsum[d].p, h_wsum[d], sizeof(double) * G[d],
@@ -2332,7 +2342,7 @@ static void absorb_device_impl(double* d_Vin, int n, int C, int ld_in,
         res.certificate_computed = 1;
         tls_phase.mark("certificate");
     }
-    XFE_CHECK(launch_scatter_cols(V, ld, perm, d_Vin, ld_in, n, C, grid, s));
+    if (!in_place) XFE_CHECK(launch_scatter_cols(V, ld, perm, d_Vin, ld_in, n, C, grid, s));
     XFE_CHECK(hipStreamSynchronize(s));
     tls_phase.mark("scatter");
     const auto t1 = std::chrono::steady_clock::now();
