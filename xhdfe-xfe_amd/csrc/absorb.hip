@@ -114,7 +114,7 @@ struct Workspace {
     DevBuf<double> slope_z[XFE_MAX_DIMS];  // slope variables in the slope path's row order
     SlopeMixed* slope_mixed = nullptr;     // the mixed-precision slope path's buffers
     // level-aligned tiles of the sorted dimension and spell structures
-    DevBuf<int> tile_start, level_count, tmp_flags, tmp_inc, tmp_iota, tmp_key;
+    DevBuf<int> tile_start, level_count, lcount, tmp_flags, tmp_inc, tmp_iota, tmp_key;
     DevBuf<int> sp_key[XFE_MAX_DIMS], sp_order[XFE_MAX_DIMS], sp_base[XFE_MAX_DIMS];
     DevBuf<double> sp_sums[XFE_MAX_DIMS];
     // mixed-precision solver (coefficient space of the non-sorted dimensions)
@@ -169,7 +169,7 @@ struct Workspace {
         perm.release(); keys_tmp.release(); vals_tmp.release(); flag.release();
         sort_tmp.release(); w.release(); partials.release(); scalars.release(); prof_out.release();
         lanes.release(); stage.release(); stage_ids.release(); stage_w.release();
-        tile_start.release(); level_count.release(); tmp_flags.release(); tmp_inc.release();
+        tile_start.release(); level_count.release(); lcount.release(); tmp_flags.release(); tmp_inc.release();
         tmp_iota.release(); tmp_key.release();
         for (int d = 0; d < XFE_MAX_DIMS; ++d) {
             sp_key[d].release(); sp_order[d].release(); sp_base[d].release(); sp_sums[d].release();
@@ -315,6 +315,14 @@ struct Solver {
     // sum of weights (counts when w == nullptr) per level of d into ws.wsum[d]
     // and its inverse into ws.inv[d]; runs the fused pass on a column of ones
     void weight_sums(int d) {
+        if (!w && (d == sorted_dim || G[d] <= kLevelCountLds)) {
+            // counts: integer histograms (exact, as the FP64 sums of ones)
+            ws.lcount.ensure(static_cast<size_t>(std::max(G[d], 1)));
+            XFE_CHECK(hipMemsetAsync(ws.lcount.p, 0, sizeof(int) * std::max(G[d], 1), s));
+            XFE_CHECK(launch_level_weights(gid[d], n, G[d], d == sorted_dim, ws.lcount.p, ws.wsum[d].p, grid, s));
+            XFE_CHECK(launch_inverse(ws.wsum[d].p, ws.inv[d].p, G[d], grid_for(G[d]), s));
+            return;
+        }
         PassArgs a;
         a.n = n; a.ld = ld; a.C = 1; a.c0 = 0; a.one = 1;
         a.gid_r = gid[d]; a.sums_r = ws.sums[d].p; a.G_r = G[d]; a.w = w;

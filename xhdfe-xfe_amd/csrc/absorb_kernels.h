@@ -80,6 +80,14 @@ hipError_t launch_tile_pass(const TileArgs& a, int cc, int rr, bool pp, int nq, 
 hipError_t launch_spell_pull(const double* S, int C, int c0, int cc, const int* order,
                              const int* skey, int M, double* sums, int grid, hipStream_t s);
 hipError_t launch_level_count(const int* gid, int n, int* cnt, int grid, hipStream_t s);
+// rows per level into cnt (zeroed by the caller, G ints) and their FP64 copy
+// wsum, for ids that are non-decreasing (sorted: each level's rows
+// contiguous) or that have at most kLevelCountLds levels (an LDS histogram);
+// other dimensions count through the FP64 pass, whose XCD-private copies of
+// the level table beat global integer atomics there
+constexpr int kLevelCountLds = 8192;
+hipError_t launch_level_weights(const int* gid, int n, int G, bool sorted, int* cnt, double* wsum, int grid,
+                                hipStream_t s);
 hipError_t launch_max_int(const int* v, int m, int* out, int grid, hipStream_t s);
 // out = max over m of (int) v (level sizes from unit-weight sums)
 hipError_t launch_max_count(const double* v, int m, int* out, int grid, hipStream_t s);

@@ -22,6 +22,7 @@
 // are combined in LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 
@@ -310,6 +311,42 @@ __global__ void __launch_bounds__(kBlock) level_count_kernel(const int* gid, int
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) atomicAdd(&cnt[gid[i]], 1);
 }
 
+// Rows per level (unweighted weight sums) for a dimension with few levels:
+// a block-private LDS histogram, flushed with one atomic per non-empty bin
+__global__ void __launch_bounds__(kBlock) level_count_lds_kernel(const int* gid, int n, int G, int* cnt) {
+    extern __shared__ int hist[];
+    for (int j = threadIdx.x; j < G; j += kBlock) hist[j] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) atomicAdd(&hist[gid[i]], 1);
+    __syncthreads();
+    for (int j = threadIdx.x; j < G; j += kBlock)
+        if (hist[j]) atomicAdd(&cnt[j], hist[j]);
+}
+
+// The same for the level-sorted dimension: a wave's run of equal ids that
+// neither starts at lane 0 nor ends at lane 63 is its level's every row, so
+// its head stores the count; the two edge runs of a wave add theirs with an
+// atomic (a level split over waves)
+__global__ void __launch_bounds__(kBlock) level_count_sorted_kernel(const int* gid, int n, int* cnt) {
+    const int lane = lane_id();
+    for (long long base = static_cast<long long>(blockIdx.x) * kBlock; base < n;
+         base += static_cast<long long>(gridDim.x) * kBlock) {
+        const long long i = base + threadIdx.x;
+        const bool valid = i < n;
+        const int key = valid ? gid[i] : -1 - lane;
+        const int end = run_end_lane(key);
+        if (valid && run_head(key)) {
+            if (lane > 0 && end < kWave - 1) cnt[key] = end - lane + 1;
+            else atomicAdd(&cnt[key], end - lane + 1);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) count_to_double_kernel(const int* cnt, int G, double* out) {
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < G; g += gridDim.x * kBlock)
+        out[g] = static_cast<double>(cnt[g]);
+}
+
 __global__ void __launch_bounds__(kBlock) max_int_kernel(const int* v, int m, int* out) {
     int mx = 0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) mx = max(mx, v[i]);
@@ -438,6 +475,19 @@ hipError_t launch_spell_pull(const double* S, int C, int c0, int cc, const int* 
 
 hipError_t launch_level_count(const int* gid, int n, int* cnt, int grid, hipStream_t s) {
     hipLaunchKernelGGL(level_count_kernel, dim3(grid), dim3(kBlock), 0, s, gid, n, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_weights(const int* gid, int n, int G, bool sorted, int* cnt, double* wsum, int grid,
+                                hipStream_t s) {
+    if (sorted)
+        hipLaunchKernelGGL(level_count_sorted_kernel, dim3(grid), dim3(kBlock), 0, s, gid, n, cnt);
+    else if (G <= kLevelCountLds)
+        hipLaunchKernelGGL(level_count_lds_kernel, dim3(grid), dim3(kBlock), sizeof(int) * G, s, gid, n, G, cnt);
+    else
+        return hipErrorInvalidValue;
+    const int gg = static_cast<int>(std::min<long long>((G + kBlock - 1) / kBlock, 4096));
+    hipLaunchKernelGGL(count_to_double_kernel, dim3(std::max(gg, 1)), dim3(kBlock), 0, s, cnt, G, wsum);
     return hipGetLastError();
 }
 
